@@ -1,0 +1,681 @@
+// Host runtime of the V1 path behind the C ABI (include/fgr.h): settings validation and refusals, the buffer layout,
+// and the stream-ordered sequence of kernels for each record. Mirrors framegen::Interpolator::Impl
+// (gpu/framegen.cpp): sizeLayout :658-685, validate :1679-1700, setSettings/reconfigure :1724-1755,
+// recordEstimate :2866-3153, recordSynth :3467-3532, record :3636-3643, profiling :2041-2071, debug :2073-2159.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+#include "../../include/fgr.h"
+#include "v1_launch.hpp"
+#include "v1_params.hpp"
+#include "runtime.hpp"
+
+using namespace fgr;
+
+namespace {
+
+thread_local const char* gInitError = nullptr;
+
+const char* kErrSettings = "invalid settings: a field is out of range or not finite (see fgr_validate)";
+const char* kErrVersion = "only MotionVersion::V1 is built in this library";
+const char* kErrFormat = "unsupported format (Rgba8, Rgb10, RgbaF16 and P010 are built; Nv12 is not)";
+const char* kErrMinSize = "the frame size is below FGR_MIN_SIZE (16 pixels) in width or height";
+const char* kErrPixels = "the frame size exceeds 2^30 pixels (32-bit indexing of the library's buffers)";
+const char* kErrDim = "the frame size exceeds 32768 pixels in width or height";
+const char* kErrDevice = "the HIP device could not be selected";
+const char* kErrAlloc = "a device allocation failed";
+const char* kErrNull = "null InitInfo";
+const char* kErrWhite = "linearWhiteNits must be in (0, 10000]";
+const char* kErrYuv = "invalid yuvMatrix";
+
+bool in(float v, float lo, float hi) { return std::isfinite(v) && v >= lo && v <= hi; }
+
+int rhe(double v) { return (int)std::nearbyint(v); }   // framegen.cpp:657 rintHalfEven (default rounding mode)
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------------ settings
+
+extern "C" void fgr_default_settings(FgrSettings* s) {
+    // framegen.h:167-409 == interp.py:43-61
+    s->trustLow = 12.0f;
+    s->trustHigh = 40.0f;
+    s->globalFallback = 0.15f;
+    s->sceneCutSad = 30.0f;
+    s->hudBonus = 1.0f;
+    s->smoothBonus = 0.5f;
+    s->searchPenalty = 0.5f;
+    s->searchRadius = 2;
+    s->searchRadiusTop = 6;
+    s->subpixel = 1;
+    s->motionVersion = FGR_MOTION_V1;
+    s->blurSamples = 0;
+    s->blurWeights = 1;
+    s->flowScale = 2;
+    s->pyramidMinSide = 48;
+    s->seedNet = 0;
+}
+
+extern "C" int fgr_validate(const FgrSettings* s) {
+    // framegen.cpp:1679-1700, the V1 fields (motionVersion <= V3 is a valid value; only V1 is built: settings_error)
+    if (!s) return 0;
+    return in(s->trustLow, 0.0f, 255.0f) && in(s->trustHigh, 0.0f, 255.0f) && s->trustHigh - s->trustLow >= 1.0f &&
+           in(s->globalFallback, 0.0f, 1.0f) && s->globalFallback > 0.0f && in(s->sceneCutSad, 0.0f, 255.0f) &&
+           s->sceneCutSad > 0.0f && in(s->hudBonus, 0.0f, 255.0f) && in(s->smoothBonus, 0.0f, 255.0f) &&
+           in(s->searchPenalty, 0.0f, 255.0f) && s->searchRadius >= 1 && s->searchRadius <= 12 &&
+           s->searchRadiusTop >= 1 && s->searchRadiusTop <= 12 && s->subpixel <= 1 &&
+           (s->flowScale == 1 || s->flowScale == 2 || s->flowScale == 4) && s->pyramidMinSide >= 16 &&
+           s->pyramidMinSide <= 256 && s->motionVersion <= 3 && s->blurSamples <= FGR_BLUR_MAX_SAMPLES &&
+           s->blurWeights <= 1 && s->seedNet <= 1;
+}
+
+static bool formatBuilt(uint32_t f) {
+    return f == FGR_FORMAT_RGBA8 || f == FGR_FORMAT_RGB10 || f == FGR_FORMAT_RGBAF16 || f == FGR_FORMAT_P010;
+}
+
+extern "C" const char* fgr_settings_error_for_format(const FgrSettings* s, uint32_t format) {
+    // framegen.cpp:1702-1707
+    if (!fgr_validate(s)) return kErrSettings;
+    if (s->motionVersion != FGR_MOTION_V1) return kErrVersion;
+    (void)format;   // no Format rule for V1
+    return nullptr;
+}
+
+extern "C" int fgr_size_layout(uint32_t W, uint32_t H, uint32_t flowScale, uint32_t minSide, FgrSizeLayout* out) {
+    if (!out || (flowScale != 1 && flowScale != 2 && flowScale != 4) || W < 1 || H < 1) return 0;
+    std::memset(out, 0, sizeof(*out));
+    out->fw = rhe(W * (1.0 / flowScale));
+    out->fh = rhe(H * (1.0 / flowScale));
+    int w = out->fw, h = out->fh, n = 0;
+    for (;;) {
+        out->w[n] = w;
+        out->h[n] = h;
+        out->bw[n] = (w + 7) / 8;
+        out->bh[n] = (h + 7) / 8;
+        n++;
+        if ((uint32_t)(std::min(w, h) / 2) < minSide || n == 16) break;
+        w = rhe(w * 0.5);
+        h = rhe(h * 0.5);
+    }
+    out->levels = n;
+    return 1;
+}
+
+extern "C" uint32_t fgr_version(void) { return 0x00010000u; }
+
+static const char* sizeLimitError(uint32_t W, uint32_t H) {
+    // framegen.cpp:760-790 (the device limits here: 32-bit indexing and the launch grid)
+    if (W < FGR_MIN_SIZE || H < FGR_MIN_SIZE) return kErrMinSize;
+    if ((uint64_t)W * H > (1ull << 30)) return kErrPixels;
+    if (W > 32768 || H > 32768) return kErrDim;
+    return nullptr;
+}
+
+// ------------------------------------------------------------------------------------------------------ handle
+
+void FgrInterpolator::freeBuffers() {
+    if (luma) hipFree(luma);
+    if (field) hipFree(field);
+    if (u) hipFree(u);
+    luma = nullptr;
+    field = nullptr;
+    u = nullptr;
+    W = H = 0;
+    haveEst = false;
+}
+
+bool FgrInterpolator::allocBuffers(uint32_t w, uint32_t h) {
+    FgrSizeLayout L;
+    fgr_size_layout(w, h, set.flowScale, set.pyramidMinSide, &L);
+    fw = L.fw;
+    fh = L.fh;
+    lv.clear();
+    int lumaOff = 0, fieldOff = 0;
+    for (int i = 0; i < L.levels; i++) {
+        Level l;
+        l.w = L.w[i];
+        l.h = L.h[i];
+        l.bw = L.bw[i];
+        l.bh = L.bh[i];
+        l.offA = lumaOff;
+        l.offC = lumaOff + l.w * l.h;
+        lumaOff += 2 * l.w * l.h;
+        l.fS = fieldOff;
+        l.fM = fieldOff + l.bw * l.bh;
+        fieldOff += 2 * l.bw * l.bh;
+        lv.push_back(l);
+    }
+    fieldSub = fieldOff;
+    fieldOff += lv[0].bw * lv[0].bh;
+    fieldSeed = fieldOff;   // learned seed proposals (one per block of the largest level that can take them)
+    fieldOff += lv[0].bw * lv[0].bh;
+    fieldPF = fieldOff;
+    fieldOff += fw * fh;
+    fieldT = fieldOff;
+    fieldOff += fw * fh;
+    lumaWords = lumaOff;
+    fieldEntries = fieldOff;
+    if (hipMalloc(&luma, (size_t)lumaWords * 4) != hipSuccess) return false;
+    if (hipMalloc(&field, (size_t)fieldEntries * 8) != hipSuccess) return false;
+    if (hipMalloc(&u, kUWords * 4) != hipSuccess) return false;
+    if (hipMemset(u, 0, kUWords * 4) != hipSuccess) return false;
+    if (hipMemset(field, 0, (size_t)fieldEntries * 8) != hipSuccess) return false;
+    W = w;
+    H = h;
+    return true;
+}
+
+bool FgrInterpolator::rebuild(uint32_t w, uint32_t h) {
+    if (hipSetDevice(device) != hipSuccess) return false;
+    freeBuffers();
+    if (!allocBuffers(w, h)) {
+        freeBuffers();
+        return false;
+    }
+    return true;
+}
+
+extern "C" FgrInterpolator* fgr_create(const FgrInitInfo* info) {
+    gInitError = nullptr;
+    if (!info) { gInitError = kErrNull; return nullptr; }
+    if (!formatBuilt(info->format)) { gInitError = kErrFormat; return nullptr; }
+    if (const char* e = fgr_settings_error_for_format(&info->settings, info->format)) { gInitError = e; return nullptr; }
+    if (const char* e = sizeLimitError(info->width, info->height)) { gInitError = e; return nullptr; }
+    if (info->format == FGR_FORMAT_RGBAF16 && info->transfer == FGR_TRANSFER_LINEAR &&
+        !(info->linearWhiteNits > 0.0f && info->linearWhiteNits <= 10000.0f)) { gInitError = kErrWhite; return nullptr; }
+    if (info->format == FGR_FORMAT_P010 && info->yuvMatrix > FGR_YUV_BT2020) { gInitError = kErrYuv; return nullptr; }
+    if (hipSetDevice(info->device) != hipSuccess) { gInitError = kErrDevice; return nullptr; }
+    FgrInterpolator* h = new FgrInterpolator();
+    h->info = *info;
+    h->set = info->settings;
+    h->device = info->device;
+    h->format = (int)info->format;
+    if (!h->formatInit() || !h->rebuild(info->width, info->height)) {
+        gInitError = kErrAlloc;
+        fgr_destroy(h);
+        return nullptr;
+    }
+    return h;
+}
+
+extern "C" const char* fgr_last_init_error(void) { return gInitError; }
+
+extern "C" void fgr_destroy(FgrInterpolator* h) {
+    if (!h) return;
+    hipSetDevice(h->device);
+    h->freeBuffers();
+    h->formatFree();
+    for (hipEvent_t e : h->events) hipEventDestroy(e);
+    delete h;
+}
+
+extern "C" uint32_t fgr_format(const FgrInterpolator* h) { return h ? (uint32_t)h->format : FGR_FORMAT_RGBA8; }
+
+extern "C" int fgr_resize(FgrInterpolator* h, uint32_t width, uint32_t height) {
+    if (!h) return 0;
+    if (sizeLimitError(width, height)) {
+        hipSetDevice(h->device);
+        h->freeBuffers();   // framegen.h:457-458: on false the object holds no size until a later resize succeeds
+        return 0;
+    }
+    return h->rebuild(width, height) ? 1 : 0;
+}
+
+extern "C" const char* fgr_settings_error(const FgrInterpolator* h, const FgrSettings* s) {
+    if (const char* e = fgr_settings_error_for_format(s, h ? (uint32_t)h->format : FGR_FORMAT_RGBA8)) return e;
+    return h && h->W ? sizeLimitError(h->W, h->H) : nullptr;
+}
+
+extern "C" const char* fgr_size_error(const FgrInterpolator* h, uint32_t w, uint32_t hgt) {
+    if (!h) return "not initialised";
+    return sizeLimitError(w, hgt);
+}
+
+extern "C" int fgr_set_settings(FgrInterpolator* h, const FgrSettings* s) {
+    // framegen.cpp:1724-1740: runtime fields only; structural ones need reconfigure
+    if (!h || fgr_settings_error(h, s) || s->flowScale != h->set.flowScale || s->pyramidMinSide != h->set.pyramidMinSide)
+        return 0;
+    if (s->motionVersion != h->set.motionVersion) h->haveEst = false;
+    h->set = *s;
+    return 1;
+}
+
+extern "C" int fgr_reconfigure(FgrInterpolator* h, const FgrSettings* s) {
+    // framegen.cpp:1742-1755
+    if (!h || fgr_settings_error(h, s)) return 0;
+    const bool structural = s->flowScale != h->set.flowScale || s->pyramidMinSide != h->set.pyramidMinSide;
+    const FgrSettings old = h->set;
+    h->set = *s;
+    if (structural && h->W) {
+        if (!h->rebuild(h->W, h->H)) return 0;   // (buffers gone: the object holds no size, as a failed resize)
+        return 1;
+    }
+    if (s->motionVersion != old.motionVersion) h->haveEst = false;
+    return 1;
+}
+
+extern "C" void fgr_get_settings(const FgrInterpolator* h, FgrSettings* out) {
+    if (!out) return;
+    if (h) *out = h->set;
+    else fgr_default_settings(out);
+}
+
+extern "C" void fgr_begin_frame(FgrInterpolator* h) {
+    if (h) h->ringEst = h->ringSynth = 0;
+}
+
+// ------------------------------------------------------------------------------------------------------ profiling
+
+extern "C" int fgr_set_profiling(FgrInterpolator* h, int on) {
+    if (!h) return 0;
+    h->profiling = false;
+    if (on && h->events.empty()) {
+        hipSetDevice(h->device);
+        for (int i = 0; i < FgrInterpolator::kMaxQ; i++) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return 0;
+            h->events.push_back(e);
+        }
+    }
+    h->profiling = on != 0;
+    return 1;
+}
+
+void FgrInterpolator::stamp(hipStream_t s, const char* name) {
+    if (!profiling || events.empty() || qcount >= kMaxQ) return;
+    hipEventRecord(events[qcount], s);
+    qnames.push_back(name);
+    qcount++;
+}
+
+void FgrInterpolator::beginProfile(hipStream_t s) {
+    qcount = 0;
+    qnames.clear();
+    if (profiling) stamp(s, "start");
+}
+
+extern "C" int fgr_read_timings(FgrInterpolator* h, const char** names, uint64_t* ticks, int cap) {
+    if (!h || h->qcount < 2) return -1;
+    if (hipEventSynchronize(h->events[h->qcount - 1]) != hipSuccess) return -1;
+    const int n = h->qcount - 1;
+    for (int i = 0; i < n && i < cap; i++) {
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, h->events[i], h->events[i + 1]);
+        if (names) names[i] = h->qnames[i + 1];
+        if (ticks) ticks[i] = (uint64_t)std::llround((double)ms * 1e6);
+    }
+    return n;
+}
+
+// ------------------------------------------------------------------------------------------------------ records
+
+bool FgrInterpolator::frameOk(const FgrFrame& f) const {
+    if (!f.ptr || f.pitch == 0) return false;
+    if (format == FGR_FORMAT_P010 && !f.chroma) return false;
+    return true;
+}
+
+FrameRef FgrInterpolator::ref(const FgrFrame& f) const {
+    FrameRef r;
+    r.p = (const uint8_t*)f.ptr;
+    r.c = (const uint8_t*)f.chroma;
+    r.pitch = (int64_t)f.pitch;
+    r.cpitch = (int64_t)f.chromaPitch;
+    return r;
+}
+
+bool FgrInterpolator::enqueueEstimate(hipStream_t s, const FgrFrame& prev, const FgrFrame& curr) {
+    const FgrSettings& st = set;
+    beginProfile(s);
+    // 1. luma + INTER_AREA 1/S (also clears the counters)
+    LumaParams lp{};
+    lp.a = ref(prev);
+    lp.c = ref(curr);
+    lp.W = (int)W;
+    lp.H = (int)H;
+    lp.S = (int)st.flowScale;
+    lp.fw = fw;
+    lp.fh = fh;
+    lp.la = luma + lv[0].offA;
+    lp.lc = luma + lv[0].offC;
+    lp.u = u;
+    lp.bgra = (int)info.bgra;
+    lp.lut = fmtLut;
+    lp.yBlack = yBlack;
+    lp.yGain = yGain;
+    if (launchLuma(lp, format, s) != hipSuccess) return false;
+    stamp(s, "luma");
+    // 2. pyramid
+    for (size_t i = 1; i < lv.size(); i++) {
+        DownParams dp{};
+        dp.sa = luma + lv[i - 1].offA;
+        dp.sc = luma + lv[i - 1].offC;
+        dp.da = luma + lv[i].offA;
+        dp.dc = luma + lv[i].offC;
+        dp.pw = lv[i - 1].w;
+        dp.ph = lv[i - 1].h;
+        dp.w = lv[i].w;
+        dp.h = lv[i].h;
+        if (launchDown(dp, s) != hipSuccess) return false;
+    }
+    stamp(s, "pyramid");
+    // 3-5. per level, coarse -> fine: median, seed + search, median3
+    static const char* lvNames[] = {"level 0 (finest)", "level 1", "level 2", "level 3", "level 4", "level 5", "level 6", "level 7"};
+    const int n = (int)lv.size();
+    float2* F = (float2*)field;
+    for (int i = n - 1; i >= 0; i--) {
+        const Level& l = lv[i];
+        SearchParams sp{};
+        sp.la = luma + l.offA;
+        sp.lc = luma + l.offC;
+        sp.w = l.w;
+        sp.h = l.h;
+        sp.bw = l.bw;
+        sp.bh = l.bh;
+        sp.lam = st.searchPenalty;
+        sp.u = u;
+        if (i == n - 1) {
+            sp.parent = nullptr;
+            sp.R = (int)st.searchRadiusTop;
+        } else {
+            const Level& p = lv[i + 1];
+            MedParams mp{};
+            mp.parent = F + p.fM;
+            mp.pw = p.bw;
+            mp.ph = p.bh;
+            mp.bw = l.bw;
+            mp.bh = l.bh;
+            mp.u = u;
+            mp.medIdx = kUMedian + 2 * i;
+            if (launchMedhist(mp, s) != hipSuccess) return false;
+            sp.parent = F + p.fM;
+            sp.pw = p.bw;
+            sp.ph = p.bh;
+            sp.med = u + kUMedian + 2 * i;
+            sp.R = (int)st.searchRadius;
+            sp.extraSeed = (st.seedNet && seedLevel == i && seedReady) ? F + fieldSeed : nullptr;
+        }
+        sp.out = F + l.fS;
+        sp.sadAcc = (i == 0 && !st.subpixel) ? 1 : 0;
+        if (launchSearch(sp, s) != hipSuccess) return false;
+        Median3Params m3{};
+        m3.in = F + l.fS;
+        m3.out = F + l.fM;
+        m3.bw = l.bw;
+        m3.bh = l.bh;
+        if (launchMedian3(m3, s) != hipSuccess) return false;
+        stamp(s, lvNames[std::min(i, 7)]);
+    }
+    // 6. sub-pixel refinement (+ the scene-cut SAD sum)
+    const Level& l0 = lv[0];
+    int bfield = l0.fM;
+    if (st.subpixel) {
+        SubpixParams q{};
+        q.la = luma + l0.offA;
+        q.lc = luma + l0.offC;
+        q.w = l0.w;
+        q.h = l0.h;
+        q.bw = l0.bw;
+        q.bh = l0.bh;
+        q.in = F + l0.fM;
+        q.out = F + fieldSub;
+        q.lam = st.searchPenalty;
+        q.u = u;
+        if (launchSubpix(q, s) != hipSuccess) return false;
+        stamp(s, "subpixel");
+        bfield = fieldSub;
+    }
+    blockFieldOff = bfield;
+    // 8. per-pixel field
+    PixParams pp{};
+    pp.la = luma + l0.offA;
+    pp.lc = luma + l0.offC;
+    pp.w = l0.w;
+    pp.h = l0.h;
+    pp.bf = F + bfield;
+    pp.bw = l0.bw;
+    pp.bh = l0.bh;
+    pp.pf = F + fieldPF;
+    pp.smooth = st.smoothBonus;
+    pp.zero = st.hudBonus;
+    if (launchPixfield(pp, s) != hipSuccess) return false;
+    stamp(s, "pixel field");
+    return true;
+}
+
+void FgrInterpolator::synthParams(SynthParams& p, const FgrFrame& prev, const FgrFrame& curr, float t, int slot) const {
+    const FgrSettings& st = set;
+    p = SynthParams{};
+    p.a = ref(prev);
+    p.c = ref(curr);
+    p.W = (int)W;
+    p.H = (int)H;
+    p.S = (int)st.flowScale;
+    p.fw = fw;
+    p.fh = fh;
+    const float2* F = (const float2*)field;
+    p.field = t != 0.5f ? F + fieldT : F + fieldPF;
+    p.sadsum = u + kUSadSum;
+    p.count = u + kUSynth + slot;
+    p.nblocks = lv[0].bw * lv[0].bh;
+    p.cutSad = st.sceneCutSad;
+    p.gfrac = st.globalFallback;
+    p.trust0 = st.trustLow;
+    p.den = st.trustHigh - st.trustLow;
+    p.rden = 1.0 / (double)p.den;
+    // framegen.cpp:3490-3497: constants folded in double from the float t, then rounded to float
+    const double td = t;
+    p.wA = t != 0.5f ? (float)(1.0 - td) : 0.5f;
+    p.wC = t != 0.5f ? (float)td : 0.5f;
+    p.mid = t == 0.5f;
+    p.nearC = p.wC > 0.5f;
+    p.lut = fmtLut;
+    for (int i = 0; i < 8; i++) p.yuv[i] = yuv[i];
+}
+
+bool FgrInterpolator::enqueueSynth(hipStream_t s, const FgrFrame& prev, const FgrFrame& curr, const FgrFrame& out, float t) {
+    const int slot = synthK & 1;
+    if (t != 0.5f) {
+        LookupParams lk{};
+        lk.pf = (const float2*)field + fieldPF;
+        lk.out = (float2*)field + fieldT;
+        lk.fw = fw;
+        lk.fh = fh;
+        lk.s = (float)(2.0 * (double)t - 1.0);
+        if (launchLookup(lk, s) != hipSuccess) return false;
+        stamp(s, "lookup");
+    }
+    SynthParams sp;
+    synthParams(sp, prev, curr, t, slot);
+    sp.out = (uint8_t*)out.ptr;
+    sp.outc = (uint8_t*)out.chroma;
+    sp.opitch = (int64_t)out.pitch;
+    sp.ocpitch = (int64_t)out.chromaPitch;
+    if (launchSynth(sp, format, s) != hipSuccess) return false;
+    stamp(s, "synth");
+    FallbackParams fp{};
+    fp.a = sp.a;
+    fp.c = sp.c;
+    fp.out = sp.out;
+    fp.outc = format == FGR_FORMAT_P010 ? sp.outc : nullptr;
+    fp.opitch = sp.opitch;
+    fp.ocpitch = sp.ocpitch;
+    fp.W = (int)W;
+    fp.H = (int)H;
+    fp.bpp = bytesPerPixel();
+    fp.count = u + kUSynth + slot;
+    fp.gfrac = set.globalFallback;
+    fp.nearC = sp.nearC;
+    fp.nextCount = u + kUSynth + (slot ^ 1);
+    if (launchFallback(fp, format, s) != hipSuccess) return false;
+    stamp(s, "fallback");
+    lastSlot = slot;
+    lastT = t;
+    synthK++;
+    return true;
+}
+
+int FgrInterpolator::bytesPerPixel() const {
+    switch (format) {
+    case FGR_FORMAT_RGBAF16: return 8;
+    case FGR_FORMAT_P010: return 2;
+    default: return 4;
+    }
+}
+
+extern "C" int fgr_record_estimate(FgrInterpolator* h, void* stream, FgrFrame prev, FgrFrame curr) {
+    if (!h || !h->W || !h->frameOk(prev) || !h->frameOk(curr)) return 0;
+    if (h->ringEst >= FGR_SET_RING) return 0;   // framegen.cpp:2873-2874
+    if (!h->enqueueEstimate((hipStream_t)stream, prev, curr)) return 0;
+    h->ringEst++;
+    h->haveEst = true;
+    h->estPrev = prev.ptr;
+    h->estCurr = curr.ptr;
+    h->estPrevF = prev;
+    h->estCurrF = curr;
+    h->lastT = 0.0f;
+    h->synthK = 0;
+    return 1;
+}
+
+extern "C" int fgr_record_synth(FgrInterpolator* h, void* stream, FgrFrame prev, FgrFrame curr, FgrFrame out, float t) {
+    // framegen.cpp:3467-3472
+    if (!h || !h->W || !(t > 0.0f && t < 1.0f)) return 0;
+    if (!h->haveEst || prev.ptr != h->estPrev || curr.ptr != h->estCurr || !h->frameOk(prev) || !h->frameOk(curr) ||
+        !h->frameOk(out))
+        return 0;
+    if (h->ringSynth >= FGR_SET_RING) return 0;
+    if (!h->enqueueSynth((hipStream_t)stream, prev, curr, out, t)) return 0;
+    h->ringSynth++;
+    return 1;
+}
+
+extern "C" int fgr_record(FgrInterpolator* h, void* stream, FgrFrame prev, FgrFrame curr, FgrFrame out, float t) {
+    // framegen.cpp:3636-3643: both rings must have room, so that a refused record records nothing
+    if (!h || !h->W || !(t > 0.0f && t < 1.0f)) return 0;
+    if (h->ringEst >= FGR_SET_RING || h->ringSynth >= FGR_SET_RING) return 0;
+    if (!h->frameOk(out)) return 0;
+    return fgr_record_estimate(h, stream, prev, curr) && fgr_record_synth(h, stream, prev, curr, out, t);
+}
+
+// ------------------------------------------------------------------------------------------------------ debug
+
+extern "C" int fgr_record_debug_counters(FgrInterpolator* h, void* stream, void* dst) {
+    // framegen.cpp:2073-2084: [untrusted pixels of the latest synth, the estimate's SAD sum x 256]
+    if (!h || !h->W || !dst) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    if (hipMemcpyAsync(dst, h->u + kUSynth + h->lastSlot, 4, hipMemcpyDeviceToDevice, s) != hipSuccess) return 0;
+    if (hipMemcpyAsync((uint8_t*)dst + 4, h->u + kUSadSum, 4, hipMemcpyDeviceToDevice, s) != hipSuccess) return 0;
+    return 1;
+}
+
+extern "C" uint32_t fgr_debug_sad_blocks(const FgrInterpolator* h) {
+    return h && !h->lv.empty() ? (uint32_t)(h->lv[0].bw * h->lv[0].bh) : 0u;
+}
+
+extern "C" int fgr_debug_map_layout(const FgrInterpolator* h, FgrDebugMapLayout* l) {
+    // framegen.cpp:2090-2104
+    if (!h || !h->W || h->lv.empty() || !l) return 0;
+    l->W = h->W;
+    l->H = h->H;
+    l->fw = (uint32_t)h->fw;
+    l->fh = (uint32_t)h->fh;
+    l->bw = (uint32_t)h->lv[0].bw;
+    l->bh = (uint32_t)h->lv[0].bh;
+    l->fscale = h->set.flowScale;
+    l->eStride = (h->W + 3) / 4;
+    l->blockOff = 0;
+    l->pixOff = (uint64_t)l->bw * l->bh * 8;
+    l->eOff = l->pixOff + (uint64_t)l->fw * l->fh * 8;
+    l->size = l->eOff + (uint64_t)l->eStride * l->H * 4;
+    return 1;
+}
+
+extern "C" int fgr_record_debug_maps(FgrInterpolator* h, void* stream, void* dst) {
+    FgrDebugMapLayout l;
+    if (!h || !dst || !h->haveEst || !fgr_debug_map_layout(h, &l)) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    uint8_t* d = (uint8_t*)dst;
+    const float2* F = (const float2*)h->field;
+    if (hipMemcpyAsync(d + l.blockOff, F + h->blockFieldOff, (size_t)l.bw * l.bh * 8, hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return 0;
+    if (hipMemcpyAsync(d + l.pixOff, F + h->fieldPF, (size_t)l.fw * l.fh * 8, hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return 0;
+    if (h->lastT > 0.0f) {
+        // the mismatch of the latest synth, recomputed from its field at t (the fused synth never stores e)
+        SynthParams sp;
+        h->synthParams(sp, h->estPrevF, h->estCurrF, h->lastT, h->lastSlot);
+        if (launchDebugE(sp, (uint32_t*)(d + l.eOff), (int)l.eStride, h->format, s) != hipSuccess) return 0;
+    } else if (hipMemsetAsync(d + l.eOff, 0, (size_t)l.eStride * l.H * 4, s) != hipSuccess) {
+        return 0;
+    }
+    return 1;
+}
+
+// ------------------------------------------------------------------------------------------------------ blur
+
+bool FgrInterpolator::copyFrame(hipStream_t s, const FgrFrame& src, const FgrFrame& dst) {
+    // t <= 0 / t >= 1 of recordSynthBlur: the real frame itself (framegen.h:557-563)
+    const size_t row = (size_t)W * bytesPerPixel();
+    if (hipMemcpy2DAsync(dst.ptr, dst.pitch, src.ptr, src.pitch, row, H, hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return false;
+    if (format == FGR_FORMAT_P010 &&
+        hipMemcpy2DAsync(dst.chroma, dst.chromaPitch, src.chroma, src.chromaPitch, (size_t)((W + 1) / 2) * 4,
+                         (H + 1) / 2, hipMemcpyDeviceToDevice, s) != hipSuccess)
+        return false;
+    return true;
+}
+
+extern "C" int fgr_record_synth_blur(FgrInterpolator* h, void* stream, FgrFrame prev, FgrFrame curr, FgrFrame out,
+                                     float t0, float t1) {
+    // framegen.cpp:3534-3545: 0 <= t0 <= t1 <= 1 (false for NaN); blur off or t0 == t1 -> recordSynth(t1)
+    if (!h || !h->W || !(t0 >= 0.0f && t1 <= 1.0f && t0 <= t1)) return 0;
+    const uint32_t N = h->set.blurSamples;
+    const bool blur = N >= 2 && t0 < t1;
+    if (!blur && t1 > 0.0f && t1 < 1.0f) return fgr_record_synth(h, stream, prev, curr, out, t1);
+    if (!h->haveEst || prev.ptr != h->estPrev || curr.ptr != h->estCurr || !h->frameOk(prev) || !h->frameOk(curr) ||
+        !h->frameOk(out))
+        return 0;
+    if (h->ringSynth >= FGR_SET_RING) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    if (!blur) {
+        if (!h->copyFrame(s, t1 <= 0.0f ? prev : curr, out)) return 0;
+        h->ringSynth++;
+        return 1;
+    }
+    if (!h->enqueueBlur(s, prev, curr, out, t0, t1)) return 0;
+    h->ringSynth++;
+    return 1;
+}
+
+extern "C" int fgr_debug_internal(const FgrInterpolator* h, FgrDebugInternal* o) {
+    if (!h || !h->W || !o) return 0;
+    std::memset(o, 0, sizeof(*o));
+    o->luma = h->luma;
+    o->field = h->field;
+    o->counters = h->u;
+    o->levels = (int32_t)h->lv.size();
+    o->fw = h->fw;
+    o->fh = h->fh;
+    for (size_t i = 0; i < h->lv.size() && i < 16; i++) {
+        const Level& l = h->lv[i];
+        o->w[i] = l.w;
+        o->h[i] = l.h;
+        o->bw[i] = l.bw;
+        o->bh[i] = l.bh;
+        o->lumaOffA[i] = l.offA;
+        o->lumaOffC[i] = l.offC;
+        o->fS[i] = l.fS;
+        o->fM[i] = l.fM;
+    }
+    o->fieldSub = h->fieldSub;
+    o->fieldPF = h->fieldPF;
+    o->fieldT = h->fieldT;
+    o->lumaWords = h->lumaWords;
+    o->fieldEntries = h->fieldEntries;
+    o->lastSynthSlot = h->lastSlot;
+    return 1;
+}

@@ -1,0 +1,69 @@
+// The state behind an FgrInterpolator handle (framegen::Interpolator::Impl, gpu/framegen.cpp:791-994 for V1).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <vector>
+#include "../../include/fgr.h"
+#include "v1_params.hpp"
+
+// One pyramid level (framegen.cpp:658-685 Level): size, block grid, luma offsets of a and c, field offsets of the
+// raw search output (fS) and the median3 output (fM), in float / float2 units of the shared buffers.
+struct Level {
+    int w, h, bw, bh, offA, offC, fS, fM;
+};
+
+struct FgrInterpolator {
+    static constexpr int kMaxQ = 64;
+
+    FgrInitInfo info{};
+    FgrSettings set{};
+    int device = 0;
+    int format = 0;
+    uint32_t W = 0, H = 0;
+    int fw = 0, fh = 0;
+    std::vector<Level> lv;
+    int lumaWords = 0, fieldEntries = 0;
+    int fieldSub = 0, fieldSeed = 0, fieldPF = 0, fieldT = 0, blockFieldOff = 0;
+    float* luma = nullptr;        // luma pyramids of a and c
+    float* field = nullptr;       // float2 fields
+    uint32_t* u = nullptr;        // counters and medians
+
+    // per-format constants (formats.cpp)
+    float* fmtLut = nullptr;      // RgbaF16 linear: 65536 half -> PQ level
+    float yBlack = 0.0f, yGain = 1.0f;
+    float yuv[8] = {};
+    bool formatInit();
+    void formatFree();
+
+    // record state
+    bool haveEst = false;
+    const void* estPrev = nullptr;
+    const void* estCurr = nullptr;
+    FgrFrame estPrevF{}, estCurrF{};
+    uint32_t ringEst = 0, ringSynth = 0;
+    int synthK = 0, lastSlot = 0;
+    float lastT = 0.0f;
+
+    // learned seed proposals (seednet.cpp): level whose search takes them, and whether the buffer holds proposals
+    int seedLevel = -1;
+    bool seedReady = false;
+
+    // profiling
+    bool profiling = false;
+    std::vector<hipEvent_t> events;
+    std::vector<const char*> qnames;
+    int qcount = 0;
+
+    void freeBuffers();
+    bool allocBuffers(uint32_t w, uint32_t h);
+    bool rebuild(uint32_t w, uint32_t h);
+    bool frameOk(const FgrFrame& f) const;
+    fgr::FrameRef ref(const FgrFrame& f) const;
+    int bytesPerPixel() const;
+    void stamp(hipStream_t s, const char* name);
+    void beginProfile(hipStream_t s);
+    bool enqueueEstimate(hipStream_t s, const FgrFrame& prev, const FgrFrame& curr);
+    bool enqueueSynth(hipStream_t s, const FgrFrame& prev, const FgrFrame& curr, const FgrFrame& out, float t);
+    void synthParams(fgr::SynthParams& p, const FgrFrame& prev, const FgrFrame& curr, float t, int slot) const;
+    bool copyFrame(hipStream_t s, const FgrFrame& src, const FgrFrame& dst);
+    bool enqueueBlur(hipStream_t s, const FgrFrame& prev, const FgrFrame& curr, const FgrFrame& out, float t0, float t1);
+};

@@ -1,0 +1,443 @@
+// V1 motion estimate kernels for gfx950 (passes 1-8 of proto/interp.py; gpu/shaders/{luma,down,medhist,search,
+// median3,subpix,pixfield}.comp). One launch per pass and level, in the reference's order (framegen.cpp:2932-3153).
+//
+// Mapping to CDNA4: a 64-lane wavefront owns one 8x8 block in the block passes (search, subpix): the block's 64 pixels
+// are the 64 lanes, reductions run over the wavefront with cross-lane shuffles in the reference's tree order, and the
+// candidate windows are staged in LDS with their search apron. Per-pixel passes use 16x16 tiles with an LDS apron.
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <limits.h>
+#include "v1_device.hpp"
+#include "v1_launch.hpp"
+
+namespace fgr {
+
+// ---------------------------------------------------------------------------------------------------------- luma
+
+// luma.comp:23-26: cv2 BGR2GRAY in 14-bit fixed point on 8-bit R, G, B
+__device__ __forceinline__ int luma8(uint32_t px, int bgra) {
+    int r = px & 255, g = (px >> 8) & 255, b = (px >> 16) & 255;
+    if (bgra) { const int t = r; r = b; b = t; }
+    return (r * 4899 + g * 9617 + b * 1868 + 8192) >> 14;
+}
+
+// luma.comp:58-72 + interp.py:150: level-0 luma = mean of the valid integer lumas of each S x S cell (integer sums,
+// one float32 division). Block (0, 0) also clears the estimate's counters (framegen.cpp:2929 clearCounters(64)).
+__global__ __launch_bounds__(256) void luma_rgba8_kernel(LumaParams P) {
+    const int tid = threadIdx.y * 16 + threadIdx.x;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && tid < kUWords) P.u[tid] = 0u;
+    const int ox = blockIdx.x * 16 + threadIdx.x, oy = blockIdx.y * 16 + threadIdx.y;
+    if (ox >= P.fw || oy >= P.fh) return;
+    int sa = 0, sc = 0, n = 0;
+    for (int j = 0; j < P.S; j++) {
+        const int py = P.S * oy + j;
+        if (py >= P.H) break;
+        const uint32_t* ra = (const uint32_t*)(P.a.p + (int64_t)py * P.a.pitch);
+        const uint32_t* rc = (const uint32_t*)(P.c.p + (int64_t)py * P.c.pitch);
+        for (int i = 0; i < P.S; i++) {
+            const int px = P.S * ox + i;
+            if (px >= P.W) break;
+            sa += luma8(ra[px], P.bgra);
+            sc += luma8(rc[px], P.bgra);
+            n++;
+        }
+    }
+    P.la[oy * P.fw + ox] = (float)sa / (float)n;
+    P.lc[oy * P.fw + ox] = (float)sc / (float)n;
+}
+
+// down.comp:9-26: 2x2 mean of the valid pixels, summed j-outer / i-inner, times 0.25 or float(1/n)
+__global__ __launch_bounds__(256) void down_kernel(DownParams P) {
+    const int ox = blockIdx.x * 16 + threadIdx.x, oy = blockIdx.y * 16 + threadIdx.y;
+    if (ox >= P.w || oy >= P.h) return;
+    float sa = 0.0f, sc = 0.0f;
+    int n = 0;
+    for (int j = 0; j < 2; j++)
+        for (int i = 0; i < 2; i++) {
+            const int px = 2 * ox + i, py = 2 * oy + j;
+            if (px < P.pw && py < P.ph) {
+                sa = sa + P.sa[py * P.pw + px];
+                sc = sc + P.sc[py * P.pw + px];
+                n++;
+            }
+        }
+    const float s = n == 4 ? 0.25f : 1.0f / (float)n;
+    P.da[oy * P.w + ox] = sa * s;
+    P.dc[oy * P.w + ox] = sc * s;
+}
+
+// ---------------------------------------------------------------------------------------------------------- median
+
+// medhist.comp:73-106 (interp.py:142): exact per-component median of the fine level's own-parent vectors. Each
+// parent is counted with the number of fine blocks that map to it (min(ix/2, pw-1), min(iy/2, ph-1)), so the loop runs
+// over the parent grid. Lanes holding the same (bin, weight) add once per wavefront (ballot aggregation), which keeps
+// the LDS atomics off the one hot bin of a global pan. Writes 2 * 0.5 * (k1-th + k2-th) as float bits.
+__device__ __forceinline__ int parentWeight(int p, int np, int nb) {
+    // number of fine indices i in [0, nb) with min(i / 2, np - 1) == p
+    const int lo = 2 * p;
+    const int hi = (p == np - 1) ? nb : min(2 * p + 2, nb);
+    return max(hi - lo, 0);
+}
+
+__device__ __forceinline__ void histAdd(uint32_t* hist, int bin, int weight) {
+    // group the active lanes by key; one LDS atomic per distinct key
+    const int key = bin * 1024 + weight;
+    uint64_t todo = __ballot(1);
+    while (todo) {
+        const int leader = __ffsll((unsigned long long)todo) - 1;
+        const int lk = __shfl(key, leader, 64);
+        const uint64_t same = __ballot(key == lk) & todo;
+        if ((int)(threadIdx.x & 63) == leader) atomicAdd(&hist[lk >> 10], (uint32_t)((lk & 1023) * __popcll(same)));
+        todo &= ~same;
+    }
+}
+
+__global__ __launch_bounds__(256) void medhist_kernel(MedParams P) {
+    __shared__ uint32_t hx[256], hy[256];
+    const int t = threadIdx.x;
+    hx[t] = 0u;
+    hy[t] = 0u;
+    __syncthreads();
+    const int np = P.pw * P.ph;
+    for (int i0 = 0; i0 < np; i0 += 256) {
+        const int i = i0 + t;
+        if (i < np) {
+            const int px = i % P.pw, py = i / P.pw;
+            const int wgt = parentWeight(px, P.pw, P.bw) * parentWeight(py, P.ph, P.bh);
+            if (wgt > 0) {
+                const float2 v = P.parent[i];
+                histAdd(hx, clampi((int)rintf(v.x) + 128, 0, 255), wgt);
+                histAdd(hy, clampi((int)rintf(v.y) + 128, 0, 255), wgt);
+            }
+        }
+    }
+    __syncthreads();
+    if (t < 2) {
+        const uint32_t* hist = t == 0 ? hx : hy;
+        const uint32_t n = (uint32_t)(P.bw * P.bh);
+        const uint32_t k1 = (n - 1u) / 2u, k2 = n / 2u;
+        float v1 = 127.0f, v2 = 127.0f;
+        uint32_t acc = 0u;
+        bool f1 = false;
+        for (int b = 0; b < 256; b++) {
+            acc += hist[b];
+            if (!f1 && acc > k1) { v1 = (float)(b - 128); f1 = true; }
+            if (acc > k2) { v2 = (float)(b - 128); break; }
+        }
+        const float m = 0.5f * (v1 + v2);
+        P.u[P.medIdx + t] = __float_as_uint(2.0f * m);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------- search
+
+// search.comp:23-87 (interp.py:116-144): one wavefront per 8x8 block, four blocks per workgroup.
+//  1. seed (not on the coarsest level): 6 candidates (own parent x2, vertical / horizontal / diagonal neighbour
+//     parents x2, zero, global median) [+ one learned proposal], lane = pixel, SAD by the 64-lane tree;
+//     cost SAD/64 (+ lambda for all but own), strict <;
+//  2. the (8 + 2R)^2 windows of a (at q - seed) and c (at q + seed) into LDS;
+//  3. lane = candidate (dy-major), sequential y-outer / x-inner SAD over LDS, cost SAD/64 + lambda |d|_1;
+//  4. wavefront argmin on (cost, candidate index): the first of the minimal costs, as the sequential strict <.
+template <int MAXR>
+__global__ __launch_bounds__(256) void search_kernel(SearchParams P) {
+    constexpr int MAXT = 8 + 2 * MAXR;
+    __shared__ float ta[4][MAXT * MAXT], tc[4][MAXT * MAXT];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nb = P.bw * P.bh;
+    const int blk = blockIdx.x * 4 + wave;
+    const bool active = blk < nb;
+    const int bx = active ? blk % P.bw : 0, by = active ? blk / P.bw : 0;
+    const int b0x = bx * kBlock, b0y = by * kBlock;
+    int sx = 0, sy = 0;
+    if (P.parent) {
+        const int qx = b0x + (lane & 7), qy = b0y + (lane >> 3);
+        const int px = min(bx / 2, P.pw - 1), py = min(by / 2, P.ph - 1);
+        const int ssy = (by & 1) == 0 ? -1 : 1, ssx = (bx & 1) == 0 ? -1 : 1;
+        const int nc = P.extraSeed ? 7 : 6;
+        float best = 0.0f;
+        for (int k = 0; k < nc; k++) {
+            int cx, cy;
+            if (k < 4) {
+                const int ox = (k == 2 || k == 3) ? ssx : 0, oy = (k == 1 || k == 3) ? ssy : 0;
+                const float2 v = P.parent[clampi(py + oy, 0, P.ph - 1) * P.pw + clampi(px + ox, 0, P.pw - 1)];
+                cx = 2 * (int)rintf(v.x);
+                cy = 2 * (int)rintf(v.y);
+            } else if (k == 4) {
+                cx = 0;
+                cy = 0;
+            } else if (k == 5) {
+                cx = (int)rintf(__uint_as_float(P.med[0]));
+                cy = (int)rintf(__uint_as_float(P.med[1]));
+            } else {
+                const float2 v = P.extraSeed[active ? blk : 0];
+                cx = (int)rintf(v.x);
+                cy = (int)rintf(v.y);
+            }
+            const float d = fabsf(lumaAt(P.la, qx - cx, qy - cy, P.w, P.h) - lumaAt(P.lc, qx + cx, qy + cy, P.w, P.h));
+            float c = treeSum64(d) * (1.0f / 64.0f);
+            if (k == 0) {
+                best = c;
+                sx = cx;
+                sy = cy;
+            } else {
+                c = c + P.lam;
+                if (c < best) {
+                    best = c;
+                    sx = cx;
+                    sy = cy;
+                }
+            }
+        }
+    }
+    const int R = P.R, T = 8 + 2 * R, D = 2 * R + 1;
+    float* A = ta[wave];
+    float* C = tc[wave];
+    for (int i = lane; i < T * T; i += 64) {
+        const int x = i % T, y = i / T;
+        A[i] = lumaAt(P.la, b0x - sx - R + x, b0y - sy - R + y, P.w, P.h);
+        C[i] = lumaAt(P.lc, b0x + sx - R + x, b0y + sy - R + y, P.w, P.h);
+    }
+    __syncthreads();
+    float bestc = FLT_MAX, bests = 0.0f;
+    int besti = INT_MAX;
+    for (int ci = lane; ci < D * D; ci += 64) {
+        const int dx = ci % D - R, dy = ci / D - R;
+        const float* a = A + (R - dy) * T + (R - dx);
+        const float* c = C + (R + dy) * T + (R + dx);
+        float sum = 0.0f;
+        for (int y = 0; y < 8; y++)
+            for (int x = 0; x < 8; x++) sum = sum + fabsf(a[y * T + x] - c[y * T + x]);
+        const float sadc = sum * (1.0f / 64.0f);
+        const float cost = sadc + P.lam * (float)(abs(dx) + abs(dy));
+        if (cost < bestc) {
+            bestc = cost;
+            besti = ci;
+            bests = sadc;
+        }
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const float oc = __shfl_xor(bestc, off, 64);
+        const int oi = __shfl_xor(besti, off, 64);
+        const float os = __shfl_xor(bests, off, 64);
+        if (oc < bestc || (oc == bestc && oi < besti)) {
+            bestc = oc;
+            besti = oi;
+            bests = os;
+        }
+    }
+    if (active && lane == 0) {
+        P.out[blk] = make_float2((float)(sx + besti % D - R), (float)(sy + besti / D - R));
+        if (P.sadAcc) atomicAdd(P.u + kUSadSum, (uint32_t)rintf(bests * 256.0f));
+    }
+}
+
+// median3.comp:8-28: per-component 3x3 median, replicate border (the shader's partial selection sort)
+__device__ __forceinline__ float med9(float* v) {
+    for (int i = 0; i <= 4; i++)
+        for (int j = i + 1; j < 9; j++)
+            if (v[j] < v[i]) {
+                const float t = v[i];
+                v[i] = v[j];
+                v[j] = t;
+            }
+    return v[4];
+}
+
+__global__ __launch_bounds__(256) void median3_kernel(Median3Params P) {
+    const int bx = blockIdx.x * 16 + threadIdx.x, by = blockIdx.y * 16 + threadIdx.y;
+    if (bx >= P.bw || by >= P.bh) return;
+    float xs[9], ys[9];
+    int k = 0;
+    for (int j = -1; j <= 1; j++)
+        for (int i = -1; i <= 1; i++) {
+            const float2 v = fieldAt(P.in, bx + i, by + j, P.bw, P.bh);
+            xs[k] = v.x;
+            ys[k] = v.y;
+            k++;
+        }
+    P.out[by * P.bw + bx] = make_float2(med9(xs), med9(ys));
+}
+
+// ---------------------------------------------------------------------------------------------------------- subpix
+
+// subpix.comp:15-24 tap(): bilinear in tile coordinates (p >= 0), weights exact for quarter steps
+__device__ __forceinline__ float tileTap(const float* t, int T, float px, float py) {
+    const float flx = floorf(px), fly = floorf(py);
+    const float fx = px - flx, fy = py - fly;
+    const int i = (int)fly * T + (int)flx;
+    const float p00 = t[i], p01 = t[i + 1], p10 = t[i + T], p11 = t[i + T + 1];
+    return p00 * ((1.0f - fy) * (1.0f - fx)) + p01 * ((1.0f - fy) * fx) + p10 * (fy * (1.0f - fx)) + p11 * (fy * fx);
+}
+
+// subpix.comp:26-73 (interp.py:162-163): 3x3 at 0.5 px then 3x3 at 0.25 px around the median-filtered integer vector,
+// one wavefront per block: lanes take (candidate, row) items, rows are summed per candidate, strict < argmin.
+// Adds roundEven(256 * winning SAD) to the scene-cut sum.
+__global__ __launch_bounds__(256) void subpix_kernel(SubpixParams P) {
+    constexpr int T = 13, O = 2;
+    __shared__ float ta[4][T * T], tc[4][T * T];
+    __shared__ float rows[4][72];
+    __shared__ float costs[4][9], sads[4][9];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nb = P.bw * P.bh;
+    const int blk = blockIdx.x * 4 + wave;
+    const bool active = blk < nb;
+    const int bx = active ? blk % P.bw : 0, by = active ? blk / P.bw : 0;
+    const int b0x = bx * kBlock, b0y = by * kBlock;
+    const float2 v0 = P.in[active ? blk : 0];
+    const int sx = (int)rintf(v0.x), sy = (int)rintf(v0.y);
+    float* A = ta[wave];
+    float* C = tc[wave];
+    for (int i = lane; i < T * T; i += 64) {
+        const int x = i % T, y = i / T;
+        A[i] = lumaAt(P.la, b0x - sx - O + x, b0y - sy - O + y, P.w, P.h);
+        C[i] = lumaAt(P.lc, b0x + sx - O + x, b0y + sy - O + y, P.w, P.h);
+    }
+    __syncthreads();
+    float basex = 0.0f, basey = 0.0f, bestsad = 0.0f;
+    for (int pass = 0; pass < 2; pass++) {
+        const float step = pass == 0 ? 0.5f : 0.25f;
+        for (int it = lane; it < 72; it += 64) {
+            const int k = it >> 3, y = it & 7;
+            const float ox = basex + (float)(k % 3 - 1) * step, oy = basey + (float)(k / 3 - 1) * step;
+            float sum = 0.0f;
+            for (int x = 0; x < 8; x++)
+                sum = sum + fabsf(tileTap(A, T, (float)(x + O) - ox, (float)(y + O) - oy) -
+                                  tileTap(C, T, (float)(x + O) + ox, (float)(y + O) + oy));
+            rows[wave][it] = sum;
+        }
+        __syncthreads();
+        if (lane < 9) {
+            float sum = 0.0f;
+            for (int y = 0; y < 8; y++) sum = sum + rows[wave][lane * 8 + y];
+            const float sad = sum * (1.0f / 64.0f);
+            const float dx = (float)(lane % 3 - 1), dy = (float)(lane / 3 - 1);
+            sads[wave][lane] = sad;
+            costs[wave][lane] = sad + P.lam * step * (fabsf(dx) + fabsf(dy));
+        }
+        __syncthreads();
+        int bi = 0;
+        float best = costs[wave][0];
+        for (int k = 1; k < 9; k++)
+            if (costs[wave][k] < best) {
+                best = costs[wave][k];
+                bi = k;
+            }
+        basex = basex + (float)(bi % 3 - 1) * step;
+        basey = basey + (float)(bi / 3 - 1) * step;
+        bestsad = sads[wave][bi];
+        __syncthreads();
+    }
+    if (active && lane == 0) {
+        P.out[blk] = make_float2((float)sx + basex, (float)sy + basey);
+        atomicAdd(P.u + kUSadSum, (uint32_t)rintf(bestsad * 256.0f));
+    }
+}
+
+// ---------------------------------------------------------------------------------------------------------- pixfield
+
+// pixfield.comp:19-35 cand(): per-pixel candidate k at flow pixel (qx, qy)
+__device__ __forceinline__ float2 pixCand(int k, int qx, int qy, const float2* bf, int bw, int bh) {
+    const int bx = qx / kBlock, by = qy / kBlock;
+    switch (k) {
+    case 0: {
+        const float sx = ((float)qx + 0.5f) / 8.0f - 0.5f, sy = ((float)qy + 0.5f) / 8.0f - 0.5f;
+        const float flx = floorf(sx), fly = floorf(sy);
+        const float fx = sx - flx, fy = sy - fly;
+        const int ix = (int)flx, iy = (int)fly;
+        const float2 top = mix2(fieldAt(bf, ix, iy, bw, bh), fieldAt(bf, ix + 1, iy, bw, bh), fx);
+        const float2 bot = mix2(fieldAt(bf, ix, iy + 1, bw, bh), fieldAt(bf, ix + 1, iy + 1, bw, bh), fx);
+        return mix2(top, bot, fy);
+    }
+    case 1: return fieldAt(bf, bx, by, bw, bh);
+    case 2: return fieldAt(bf, bx, by - 1, bw, bh);
+    case 3: return fieldAt(bf, bx, by + 1, bw, bh);
+    case 4: return fieldAt(bf, bx - 1, by, bw, bh);
+    case 5: return fieldAt(bf, bx + 1, by, bw, bh);
+    default: return make_float2(0.0f, 0.0f);
+    }
+}
+
+// pixfield.comp:37-63 (interp.py:167-183): 16x16 flow pixels per workgroup, the 7 candidates' errors of the 20x20 tile
+// (+2 apron, replicate) in LDS; cost = float32(exact window sum * (1/25)) with the sum in float64 (cv2.blur's CV_32F
+// path), - smoothBonus for the bilinear candidate, - hudBonus for zero, strict < (first wins).
+__global__ __launch_bounds__(256) void pixfield_kernel(PixParams P) {
+    constexpr int TS = 16, AP = 2, TT = TS + 2 * AP, NC = 7;
+    __shared__ float err[NC][TT * TT];
+    const int tid = threadIdx.y * TS + threadIdx.x;
+    const int t0x = blockIdx.x * TS - AP, t0y = blockIdx.y * TS - AP;
+    for (int i = tid; i < TT * TT; i += TS * TS) {
+        const int qx = clampi(t0x + i % TT, 0, P.w - 1), qy = clampi(t0y + i / TT, 0, P.h - 1);
+        for (int k = 0; k < NC; k++) {
+            const float2 v = pixCand(k, qx, qy, P.bf, P.bw, P.bh);
+            err[k][i] = fabsf(sampleL(P.la, (float)qx - v.x, (float)qy - v.y, P.w, P.h) -
+                              sampleL(P.lc, (float)qx + v.x, (float)qy + v.y, P.w, P.h));
+        }
+    }
+    __syncthreads();
+    const int qx = blockIdx.x * TS + threadIdx.x, qy = blockIdx.y * TS + threadIdx.y;
+    if (qx >= P.w || qy >= P.h) return;
+    int bk = 0;
+    float best = 0.0f;
+    for (int k = 0; k < NC; k++) {
+        double vs = 0.0;
+        for (int y = 0; y < 5; y++) {
+            const float* r = &err[k][(threadIdx.y + y) * TT + threadIdx.x];
+            double hs = 0.0;
+            for (int x = 0; x < 5; x++) hs = hs + (double)r[x];
+            vs = vs + hs;
+        }
+        float c = (float)(vs * (1.0 / 25.0));
+        if (k == 0) c = c - P.smooth;
+        if (k == NC - 1) c = c - P.zero;
+        if (k == 0 || c < best) {
+            best = c;
+            bk = k;
+        }
+    }
+    P.pf[qy * P.w + qx] = pixCand(bk, qx, qy, P.bf, P.bw, P.bh);
+}
+
+// ---------------------------------------------------------------------------------------------------------- launches
+
+static inline dim3 grid2(int w, int h, int t) { return dim3((unsigned)((w + t - 1) / t), (unsigned)((h + t - 1) / t)); }
+
+hipError_t launchLumaRgba8(const LumaParams& p, hipStream_t s) {
+    hipLaunchKernelGGL(luma_rgba8_kernel, grid2(p.fw, p.fh, 16), dim3(16, 16), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launchDown(const DownParams& p, hipStream_t s) {
+    hipLaunchKernelGGL(down_kernel, grid2(p.w, p.h, 16), dim3(16, 16), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launchMedhist(const MedParams& p, hipStream_t s) {
+    hipLaunchKernelGGL(medhist_kernel, dim3(1), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launchSearch(const SearchParams& p, hipStream_t s) {
+    const unsigned g = (unsigned)((p.bw * p.bh + 3) / 4);
+    if (p.R <= 2) hipLaunchKernelGGL(search_kernel<2>, dim3(g), dim3(256), 0, s, p);
+    else if (p.R <= 6) hipLaunchKernelGGL(search_kernel<6>, dim3(g), dim3(256), 0, s, p);
+    else hipLaunchKernelGGL(search_kernel<12>, dim3(g), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launchMedian3(const Median3Params& p, hipStream_t s) {
+    hipLaunchKernelGGL(median3_kernel, grid2(p.bw, p.bh, 16), dim3(16, 16), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launchSubpix(const SubpixParams& p, hipStream_t s) {
+    hipLaunchKernelGGL(subpix_kernel, dim3((unsigned)((p.bw * p.bh + 3) / 4)), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launchPixfield(const PixParams& p, hipStream_t s) {
+    hipLaunchKernelGGL(pixfield_kernel, grid2(p.w, p.h, 16), dim3(16, 16), 0, s, p);
+    return hipGetLastError();
+}
+
+}  // namespace fgr

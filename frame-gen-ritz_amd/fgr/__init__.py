@@ -1,0 +1,271 @@
+"""Python binding of libfgr.so, the MI355X (gfx950) V1 frame interpolator (C ABI: include/fgr.h).
+
+Mirrors framegen::Interpolator (gpu/framegen.h:443-822): ``Interpolator(width, height, ...)`` = init(),
+``record_estimate`` / ``record_synth`` / ``record`` enqueue HIP kernels on the current torch stream (the reference
+records into a VkCommandBuffer), returning True, or False when the call is refused (nothing enqueued). Frames are
+torch uint8 / uint16 / float16 CUDA tensors (device memory); torch supplies memory and streams only.
+
+The library must be built (``python -c "import __graft_entry__ as g; g.build()"``); importing this package without
+it raises ImportError: there is no CPU fallback.
+"""
+import ctypes
+import os
+
+__all__ = ["Settings", "Interpolator", "lib", "default_settings", "validate", "settings_error_for_format",
+           "size_layout", "FORMAT_RGBA8", "FORMAT_RGB10", "FORMAT_RGBAF16", "FORMAT_NV12", "FORMAT_P010",
+           "MIN_SIZE", "SET_RING", "frame_of"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libfgr.so")
+
+FORMAT_RGBA8, FORMAT_RGB10, FORMAT_RGBAF16, FORMAT_NV12, FORMAT_P010 = 0, 1, 2, 3, 4
+TRANSFER_ENCODED, TRANSFER_LINEAR = 0, 1
+YUV_BT601, YUV_BT709, YUV_BT2020 = 0, 1, 2
+MIN_SIZE, SET_RING, BLUR_MAX_SAMPLES = 16, 32, 16
+
+
+class Settings(ctypes.Structure):
+    """framegen.h:167-409 Settings (the V1 fields) == FgrSettings."""
+    _fields_ = [("trustLow", ctypes.c_float), ("trustHigh", ctypes.c_float), ("globalFallback", ctypes.c_float),
+                ("sceneCutSad", ctypes.c_float), ("hudBonus", ctypes.c_float), ("smoothBonus", ctypes.c_float),
+                ("searchPenalty", ctypes.c_float), ("searchRadius", ctypes.c_uint32),
+                ("searchRadiusTop", ctypes.c_uint32), ("subpixel", ctypes.c_uint32),
+                ("motionVersion", ctypes.c_uint32), ("blurSamples", ctypes.c_uint32),
+                ("blurWeights", ctypes.c_uint32), ("flowScale", ctypes.c_uint32),
+                ("pyramidMinSide", ctypes.c_uint32), ("seedNet", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class InitInfo(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("format", ctypes.c_uint32),
+                ("bgra", ctypes.c_uint32), ("transfer", ctypes.c_uint32), ("linearWhiteNits", ctypes.c_float),
+                ("yuvMatrix", ctypes.c_uint32), ("yuvFullRange", ctypes.c_uint32), ("device", ctypes.c_int32),
+                ("settings", Settings)]
+
+
+class Frame(ctypes.Structure):
+    _fields_ = [("ptr", ctypes.c_void_p), ("pitch", ctypes.c_uint64), ("chroma", ctypes.c_void_p),
+                ("chromaPitch", ctypes.c_uint64)]
+
+
+class DebugMapLayout(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in ("W", "H", "fw", "fh", "bw", "bh", "fscale", "eStride")] + \
+               [(n, ctypes.c_uint64) for n in ("blockOff", "pixOff", "eOff", "size")]
+
+
+class SizeLayout(ctypes.Structure):
+    _fields_ = [("fw", ctypes.c_int32), ("fh", ctypes.c_int32), ("levels", ctypes.c_int32),
+                ("w", ctypes.c_int32 * 16), ("h", ctypes.c_int32 * 16), ("bw", ctypes.c_int32 * 16),
+                ("bh", ctypes.c_int32 * 16)]
+
+
+class DebugInternal(ctypes.Structure):
+    _fields_ = [("luma", ctypes.c_void_p), ("field", ctypes.c_void_p), ("counters", ctypes.c_void_p),
+                ("levels", ctypes.c_int32), ("fw", ctypes.c_int32), ("fh", ctypes.c_int32)] + \
+               [(n, ctypes.c_int32 * 16) for n in ("w", "h", "bw", "bh", "lumaOffA", "lumaOffC", "fS", "fM")] + \
+               [(n, ctypes.c_int32) for n in ("fieldSub", "fieldPF", "fieldT", "lumaWords", "fieldEntries",
+                                              "lastSynthSlot")]
+
+
+if not os.path.exists(_LIB_PATH):
+    raise ImportError(f"libfgr.so is not built ({_LIB_PATH}); run __graft_entry__.build()")
+lib = ctypes.CDLL(_LIB_PATH)
+
+_H = ctypes.c_void_p
+_sig = {
+    "fgr_default_settings": (None, [ctypes.POINTER(Settings)]),
+    "fgr_validate": (ctypes.c_int, [ctypes.POINTER(Settings)]),
+    "fgr_settings_error_for_format": (ctypes.c_char_p, [ctypes.POINTER(Settings), ctypes.c_uint32]),
+    "fgr_size_layout": (ctypes.c_int, [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                       ctypes.POINTER(SizeLayout)]),
+    "fgr_version": (ctypes.c_uint32, []),
+    "fgr_create": (_H, [ctypes.POINTER(InitInfo)]),
+    "fgr_last_init_error": (ctypes.c_char_p, []),
+    "fgr_destroy": (None, [_H]),
+    "fgr_format": (ctypes.c_uint32, [_H]),
+    "fgr_resize": (ctypes.c_int, [_H, ctypes.c_uint32, ctypes.c_uint32]),
+    "fgr_settings_error": (ctypes.c_char_p, [_H, ctypes.POINTER(Settings)]),
+    "fgr_size_error": (ctypes.c_char_p, [_H, ctypes.c_uint32, ctypes.c_uint32]),
+    "fgr_set_settings": (ctypes.c_int, [_H, ctypes.POINTER(Settings)]),
+    "fgr_reconfigure": (ctypes.c_int, [_H, ctypes.POINTER(Settings)]),
+    "fgr_get_settings": (None, [_H, ctypes.POINTER(Settings)]),
+    "fgr_begin_frame": (None, [_H]),
+    "fgr_record_estimate": (ctypes.c_int, [_H, ctypes.c_void_p, Frame, Frame]),
+    "fgr_record_synth": (ctypes.c_int, [_H, ctypes.c_void_p, Frame, Frame, Frame, ctypes.c_float]),
+    "fgr_record": (ctypes.c_int, [_H, ctypes.c_void_p, Frame, Frame, Frame, ctypes.c_float]),
+    "fgr_record_synth_blur": (ctypes.c_int, [_H, ctypes.c_void_p, Frame, Frame, Frame, ctypes.c_float,
+                                             ctypes.c_float]),
+    "fgr_set_profiling": (ctypes.c_int, [_H, ctypes.c_int]),
+    "fgr_read_timings": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_uint64),
+                                        ctypes.c_int]),
+    "fgr_record_debug_counters": (ctypes.c_int, [_H, ctypes.c_void_p, ctypes.c_void_p]),
+    "fgr_debug_sad_blocks": (ctypes.c_uint32, [_H]),
+    "fgr_debug_map_layout": (ctypes.c_int, [_H, ctypes.POINTER(DebugMapLayout)]),
+    "fgr_record_debug_maps": (ctypes.c_int, [_H, ctypes.c_void_p, ctypes.c_void_p]),
+    "fgr_debug_internal": (ctypes.c_int, [_H, ctypes.POINTER(DebugInternal)]),
+}
+for _name, (_res, _args) in _sig.items():
+    _f = getattr(lib, _name)
+    _f.restype = _res
+    _f.argtypes = _args
+
+
+def _s(b):
+    return None if b is None else b.decode()
+
+
+def default_settings(**kw):
+    s = Settings()
+    lib.fgr_default_settings(ctypes.byref(s))
+    for k, v in kw.items():
+        if not hasattr(s, k):
+            raise KeyError(k)
+        setattr(s, k, v)
+    return s
+
+
+def validate(s):
+    """framegen.h:464 Interpolator::validate()."""
+    return bool(lib.fgr_validate(ctypes.byref(s)))
+
+
+def settings_error_for_format(s, fmt=FORMAT_RGBA8):
+    return _s(lib.fgr_settings_error_for_format(ctypes.byref(s), fmt))
+
+
+def size_layout(W, H, flow_scale=2, min_side=48):
+    """framegen.cpp:658-685 sizeLayout: (fw, fh, [(w, h, bw, bh), ...] finest first)."""
+    L = SizeLayout()
+    if not lib.fgr_size_layout(W, H, flow_scale, min_side, ctypes.byref(L)):
+        raise ValueError("invalid size / flow scale")
+    return L.fw, L.fh, [(L.w[i], L.h[i], L.bw[i], L.bh[i]) for i in range(L.levels)]
+
+
+def frame_of(t, chroma=None):
+    """A torch CUDA tensor (H, W, C) or (H, W) -> FgrFrame (pitch = bytes per row)."""
+    f = Frame()
+    f.ptr = t.data_ptr()
+    f.pitch = t.stride(0) * t.element_size()
+    if chroma is not None:
+        f.chroma = chroma.data_ptr()
+        f.chromaPitch = chroma.stride(0) * chroma.element_size()
+    return f
+
+
+def _stream(stream):
+    if stream is not None:
+        return ctypes.c_void_p(stream)
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _frame(x):
+    if isinstance(x, Frame):
+        return x
+    if isinstance(x, tuple):
+        return frame_of(x[0], x[1])
+    return frame_of(x)
+
+
+class Interpolator:
+    """framegen::Interpolator over the C ABI. Raises RuntimeError(initError) if init fails."""
+
+    def __init__(self, width, height, fmt=FORMAT_RGBA8, settings=None, bgra=False, transfer=TRANSFER_ENCODED,
+                 linear_white_nits=80.0, yuv_matrix=YUV_BT709, yuv_full_range=False, device=0):
+        info = InitInfo()
+        info.width, info.height, info.format, info.bgra = width, height, fmt, int(bgra)
+        info.transfer, info.linearWhiteNits = transfer, linear_white_nits
+        info.yuvMatrix, info.yuvFullRange, info.device = yuv_matrix, int(yuv_full_range), device
+        info.settings = settings if settings is not None else default_settings()
+        self._h = lib.fgr_create(ctypes.byref(info))
+        if not self._h:
+            raise RuntimeError(_s(lib.fgr_last_init_error()))
+
+    def close(self):
+        if self._h:
+            lib.fgr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def format(self):
+        return lib.fgr_format(self._h)
+
+    def resize(self, w, h):
+        return bool(lib.fgr_resize(self._h, w, h))
+
+    def settings_error(self, s):
+        return _s(lib.fgr_settings_error(self._h, ctypes.byref(s)))
+
+    def size_error(self, w, h):
+        return _s(lib.fgr_size_error(self._h, w, h))
+
+    def set_settings(self, s):
+        return bool(lib.fgr_set_settings(self._h, ctypes.byref(s)))
+
+    def reconfigure(self, s):
+        return bool(lib.fgr_reconfigure(self._h, ctypes.byref(s)))
+
+    def settings(self):
+        s = Settings()
+        lib.fgr_get_settings(self._h, ctypes.byref(s))
+        return s
+
+    def begin_frame(self):
+        lib.fgr_begin_frame(self._h)
+
+    def record_estimate(self, prev, curr, stream=None):
+        return bool(lib.fgr_record_estimate(self._h, _stream(stream), _frame(prev), _frame(curr)))
+
+    def record_synth(self, prev, curr, out, t, stream=None):
+        return bool(lib.fgr_record_synth(self._h, _stream(stream), _frame(prev), _frame(curr), _frame(out), t))
+
+    def record(self, prev, curr, out, t=0.5, stream=None):
+        return bool(lib.fgr_record(self._h, _stream(stream), _frame(prev), _frame(curr), _frame(out), t))
+
+    def record_synth_blur(self, prev, curr, out, t0, t1, stream=None):
+        return bool(lib.fgr_record_synth_blur(self._h, _stream(stream), _frame(prev), _frame(curr), _frame(out),
+                                              t0, t1))
+
+    def set_profiling(self, on):
+        return bool(lib.fgr_set_profiling(self._h, int(on)))
+
+    def read_timings(self):
+        """[(pass group name, ns), ...] of the last profile, or None."""
+        names = (ctypes.c_char_p * 64)()
+        ticks = (ctypes.c_uint64 * 64)()
+        n = lib.fgr_read_timings(self._h, names, ticks, 64)
+        if n < 0:
+            return None
+        return [(names[i].decode(), int(ticks[i])) for i in range(n)]
+
+    def record_debug_counters(self, dst, stream=None):
+        """dst: a CUDA int32/uint32 tensor of >= 2 elements: [untrusted, SAD sum x 256]."""
+        return bool(lib.fgr_record_debug_counters(self._h, _stream(stream), ctypes.c_void_p(dst.data_ptr())))
+
+    def debug_sad_blocks(self):
+        return lib.fgr_debug_sad_blocks(self._h)
+
+    def debug_map_layout(self):
+        l = DebugMapLayout()
+        if not lib.fgr_debug_map_layout(self._h, ctypes.byref(l)):
+            return None
+        return {k: getattr(l, k) for k, _ in l._fields_}
+
+    def debug_internal(self):
+        """Test / debug only: the internal buffers' device pointers and layout (DebugInternal), or None."""
+        d = DebugInternal()
+        return d if lib.fgr_debug_internal(self._h, ctypes.byref(d)) else None
+
+    def record_debug_maps(self, dst, stream=None):
+        return bool(lib.fgr_record_debug_maps(self._h, _stream(stream), ctypes.c_void_p(dst.data_ptr())))

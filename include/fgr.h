@@ -1,0 +1,160 @@
+/* fgr.h -- C ABI of the MI355X (gfx950) implementation of frame-gen-ritz's MotionVersion::V1 path.
+ *
+ * The reference's interface is the C++ class framegen::Interpolator (gpu/framegen.h:443-822), which records Vulkan
+ * compute work into a caller's VkCommandBuffer. Here the same operations enqueue HIP kernels on a caller's
+ * hipStream_t (passed as void*), in stream order, and never synchronise the device inside a record call.
+ * include/framegen.h wraps these entry points 1:1 as the C++ drop-in.
+ *
+ * Conventions (as the reference's):
+ *   - Not thread-safe. All records of one handle go to ONE stream, in recording order: the handle's scratch buffers
+ *     are shared between records and only stream order serialises them (framegen.h:4-6).
+ *   - A record function returns 1 on success, 0 if it was refused; a refused record enqueues nothing and leaves `out`
+ *     untouched (framegen.h:537-548). Reasons are string literals (fgr_init_error, fgr_settings_error, fgr_size_error).
+ *   - Frames are caller-owned device memory: FgrFrame{ptr, pitch} (pitch in bytes), plus the CbCr plane for P010.
+ *     The library allocates its own buffers with hipMalloc only in fgr_create / fgr_resize / fgr_reconfigure.
+ */
+#ifndef FGR_H
+#define FGR_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* framegen.h:138 Format. Rgba8 = R8G8B8A8 / B8G8R8A8 UNORM (FgrInitInfo.bgra picks the byte order), Rgb10 =
+ * A2B10G10R10 UNORM (R in bits 0-9), RgbaF16 = R16G16B16A16 SFLOAT, P010 = 10-bit 4:2:0 in the high bits of 16-bit
+ * words (Y plane + interleaved CbCr plane at half resolution). Nv12 (3) is not implemented (fgr_create refuses it). */
+enum { FGR_FORMAT_RGBA8 = 0, FGR_FORMAT_RGB10 = 1, FGR_FORMAT_RGBAF16 = 2, FGR_FORMAT_NV12 = 3, FGR_FORMAT_P010 = 4 };
+/* framegen.h:144 Transfer (RgbaF16 only): 0 = display-encoded values (level = value * 255), 1 = linear light. */
+enum { FGR_TRANSFER_ENCODED = 0, FGR_TRANSFER_LINEAR = 1 };
+/* framegen.h:150 YuvMatrix (P010 only). */
+enum { FGR_YUV_BT601 = 0, FGR_YUV_BT709 = 1, FGR_YUV_BT2020 = 2 };
+/* framegen.h:28 MotionVersion: only V1 (0) is built. */
+enum { FGR_MOTION_V1 = 0 };
+
+/* framegen.h:167-409 Settings: the fields V1 reads, with the reference's defaults (fgr_default_settings) and ranges
+ * (fgr_validate). seedNet is this library's own addition (no reference counterpart): 1 adds the learned seed
+ * proposal of an attached seed network (fgr_set_seed_net) as a seventh seed candidate on the level it was trained
+ * for; 0 (default) is the plain V1 search, bit-identical to a library without it. */
+typedef struct FgrSettings {
+    float trustLow;           /* [0, 255]; trustHigh >= trustLow + 1                       (framegen.h:173) */
+    float trustHigh;          /* [0, 255]                                                  (framegen.h:174) */
+    float globalFallback;     /* (0, 1]                                                    (framegen.h:177) */
+    float sceneCutSad;        /* (0, 255]                                                  (framegen.h:180) */
+    float hudBonus;           /* [0, 255]                                                  (framegen.h:186) */
+    float smoothBonus;        /* [0, 255]                                                  (framegen.h:189) */
+    float searchPenalty;      /* [0, 255]                                                  (framegen.h:193) */
+    uint32_t searchRadius;    /* [1, 12]                                                   (framegen.h:202) */
+    uint32_t searchRadiusTop; /* [1, 12]                                                   (framegen.h:203) */
+    uint32_t subpixel;        /* 0 / 1                                                     (framegen.h:206) */
+    uint32_t motionVersion;   /* V1 only                                                   (framegen.h:325) */
+    uint32_t blurSamples;     /* 0..16; < 2 = off                                          (framegen.h:315) */
+    uint32_t blurWeights;     /* 0 Even, 1 Gaussian                                        (framegen.h:316) */
+    uint32_t flowScale;       /* structural: 1, 2, 4                                       (framegen.h:403) */
+    uint32_t pyramidMinSide;  /* structural: [16, 256]                                     (framegen.h:408) */
+    uint32_t seedNet;         /* 0 / 1 (no reference counterpart, see above)                             */
+} FgrSettings;
+
+/* framegen.h:411-441 InitInfo, with a HIP device index in place of the Vulkan handles. */
+typedef struct FgrInitInfo {
+    uint32_t width, height;   /* >= FGR_MIN_SIZE each */
+    uint32_t format;          /* FGR_FORMAT_* */
+    uint32_t bgra;            /* Rgba8 only: 1 = B8G8R8A8 byte order (the luma weights follow the channels) */
+    uint32_t transfer;        /* RgbaF16: FGR_TRANSFER_* */
+    float linearWhiteNits;    /* RgbaF16 linear: luminance of 1.0, (0, 10000]; 80 = scRGB */
+    uint32_t yuvMatrix;       /* P010: FGR_YUV_* */
+    uint32_t yuvFullRange;    /* P010: 0 = limited (video) range */
+    int32_t device;           /* HIP device ordinal */
+    FgrSettings settings;
+} FgrInitInfo;
+
+/* framegen.h:154-159 Frame: one caller-owned image. P010: ptr = Y plane (W x H uint16), chroma = CbCr plane
+ * (ceil(W/2) x ceil(H/2) pairs of uint16). Frames are identified by ptr (the reference compares image views). */
+typedef struct FgrFrame {
+    void* ptr;
+    uint64_t pitch;           /* bytes per row of ptr */
+    void* chroma;
+    uint64_t chromaPitch;     /* bytes per row of chroma */
+} FgrFrame;
+
+/* framegen.h:736-739 DebugMapLayout (byte offsets relative to the destination offset). */
+typedef struct FgrDebugMapLayout {
+    uint32_t W, H, fw, fh, bw, bh, fscale, eStride;   /* eStride in uint32 words */
+    uint64_t blockOff, pixOff, eOff, size;
+} FgrDebugMapLayout;
+
+/* Sizes of one frame's motion estimate (framegen.cpp:658-685 sizeLayout). */
+typedef struct FgrSizeLayout {
+    int32_t fw, fh, levels;
+    int32_t w[16], h[16], bw[16], bh[16];
+} FgrSizeLayout;
+
+#define FGR_MIN_SIZE 16        /* framegen.h:817 kMinSize */
+#define FGR_SET_RING 32        /* framegen.h:812 kSetRing: records of each kind per fgr_begin_frame */
+#define FGR_BLUR_MAX_SAMPLES 16 /* framegen.h:814 kBlurMaxSamples */
+
+typedef struct FgrInterpolator FgrInterpolator;
+
+/* ---- settings (host only, no device work) ---- */
+void fgr_default_settings(FgrSettings* out);                                   /* framegen.h:167-409 defaults */
+int fgr_validate(const FgrSettings* s);                                        /* framegen.h:464 validate() */
+const char* fgr_settings_error_for_format(const FgrSettings* s, uint32_t format); /* framegen.h:468 settingsError(s, f) */
+int fgr_size_layout(uint32_t W, uint32_t H, uint32_t flowScale, uint32_t minSide, FgrSizeLayout* out);
+uint32_t fgr_version(void);
+
+/* ---- lifetime (framegen.h:451-461) ---- */
+/* init(): a new handle, or null (fgr_last_init_error() says why: a string literal). */
+FgrInterpolator* fgr_create(const FgrInitInfo* info);
+const char* fgr_last_init_error(void);                                         /* framegen.h:453 initError() */
+void fgr_destroy(FgrInterpolator* h);                                          /* framegen.h:456 destroy() */
+uint32_t fgr_format(const FgrInterpolator* h);                                 /* framegen.h:455 format() */
+/* resize(): re-creates the size-dependent buffers; the caller guarantees earlier records have retired. */
+int fgr_resize(FgrInterpolator* h, uint32_t width, uint32_t height);           /* framegen.h:461 */
+
+/* ---- settings of a handle (framegen.h:472-507) ---- */
+const char* fgr_settings_error(const FgrInterpolator* h, const FgrSettings* s); /* framegen.h:472 */
+const char* fgr_size_error(const FgrInterpolator* h, uint32_t w, uint32_t hgt); /* framegen.h:480 sizeError() */
+int fgr_set_settings(FgrInterpolator* h, const FgrSettings* s);                /* framegen.h:500 setSettings() */
+int fgr_reconfigure(FgrInterpolator* h, const FgrSettings* s);                 /* framegen.h:505 reconfigure() */
+void fgr_get_settings(const FgrInterpolator* h, FgrSettings* out);             /* framegen.h:507 settings() */
+
+/* ---- records (framegen.h:511-573) ---- */
+void fgr_begin_frame(FgrInterpolator* h);                                      /* framegen.h:511 beginFrame() */
+int fgr_record_estimate(FgrInterpolator* h, void* stream, FgrFrame prev, FgrFrame curr);            /* :549 */
+int fgr_record_synth(FgrInterpolator* h, void* stream, FgrFrame prev, FgrFrame curr, FgrFrame out, float t); /* :550 */
+int fgr_record(FgrInterpolator* h, void* stream, FgrFrame prev, FgrFrame curr, FgrFrame out, float t);       /* :551 */
+int fgr_record_synth_blur(FgrInterpolator* h, void* stream, FgrFrame prev, FgrFrame curr, FgrFrame out,
+                          float t0, float t1);                                                       /* :573 */
+
+/* ---- profiling (framegen.h:705-717): HIP events after every pass group; ticks are nanoseconds ---- */
+int fgr_set_profiling(FgrInterpolator* h, int on);
+/* Waits for the last profile. Fills up to cap entries; returns the number of pass groups (-1 if none). */
+int fgr_read_timings(FgrInterpolator* h, const char** names, uint64_t* ticks, int cap);
+
+/* ---- debug (framegen.h:719-741) ---- */
+/* Enqueues a copy of [untrusted pixels, sum of roundEven(256 * blockSAD)] of the latest record into dst (device). */
+int fgr_record_debug_counters(FgrInterpolator* h, void* stream, void* dst);
+uint32_t fgr_debug_sad_blocks(const FgrInterpolator* h);
+int fgr_debug_map_layout(const FgrInterpolator* h, FgrDebugMapLayout* out);
+/* Block field (bw x bh float2, after sub-pixel), per-pixel field (fw x fh float2) and the packed mismatch e of the
+ * latest synth (W x H bytes, eStride words per row, low byte first) into dst (device memory of layout.size bytes). */
+int fgr_record_debug_maps(FgrInterpolator* h, void* stream, void* dst);
+
+/* ---- test / debug only; no counterpart in the reference ----
+ * Device pointers of the handle's internal buffers and their layout (float / float2 offsets), so a test can read every
+ * intermediate of the latest estimate (luma pyramid, per-level raw and median-filtered block fields, medians, the
+ * sub-pixel field, the per-pixel field, the counters). Valid until the next resize / reconfigure / destroy. */
+typedef struct FgrDebugInternal {
+    const void* luma;       /* float */
+    const void* field;      /* float2 */
+    const void* counters;   /* uint32[64]: [1] SAD sum, [4 + 2i] level i median (float bits), [40 + k&1] synth k count */
+    int32_t levels, fw, fh;
+    int32_t w[16], h[16], bw[16], bh[16], lumaOffA[16], lumaOffC[16], fS[16], fM[16];
+    int32_t fieldSub, fieldPF, fieldT, lumaWords, fieldEntries, lastSynthSlot;
+} FgrDebugInternal;
+int fgr_debug_internal(const FgrInterpolator* h, FgrDebugInternal* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
