@@ -83,7 +83,8 @@ def main():
 
     batch = None
     if args.path in ("auto", "batched") and hasattr(fgr, "Batch"):
-        batch = fgr.Batch([(a, c, MULT, o) for (a, c), o in zip(fr, outs)])
+        bouts = [[torch.empty_like(fr[0][0]) for _ in range(MULT - 1)] for _ in fr]
+        batch = fgr.Batch([(a, c, MULT, o) for (a, c), o in zip(fr, bouts)], use_graph=True)
 
     def timed(fn, steps, warmup):
         for _ in range(warmup):

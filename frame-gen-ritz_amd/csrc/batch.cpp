@@ -1,0 +1,216 @@
+// Batched V1: many frame pairs of any sizes (and any multipliers) in one launch per pass, replayed as a hipGraph.
+//
+// Each pair keeps the state of a plain handle (its own pyramid, fields and counters) and produces the same plan of
+// kernel launches as fgr_record_estimate + fgr_record_synth would (FgrInterpolator::planEstimate / planSynth). The
+// batch merges the ops of equal (stage, pass) across pairs into one launch of the pass's batched kernel, whose work
+// list maps each workgroup to (pair's parameter block, block within that pair's grid). Stages keep every pair's own
+// order (coarse-to-fine levels align at each pair's coarsest level), so the results are bit-identical to the
+// per-pair path. The parameter blocks and work lists are built once, at create time, in one device allocation; a
+// record is a fixed sequence of launches, captured once into a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <vector>
+#include "../../include/fgr.h"
+#include "runtime.hpp"
+#include "v1_launch.hpp"
+#include "v1_params.hpp"
+
+using namespace fgr;
+
+namespace {
+thread_local const char* gBatchError = nullptr;
+
+struct Launch {
+    int pass = 0;
+    int n = 0;             // workgroups
+    size_t paramOff = 0;   // byte offsets in the device blob
+    size_t workOff = 0;
+    size_t paramBytes = 0;
+};
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+}  // namespace
+
+struct FgrBatch {
+    FgrBatchInfo info{};
+    std::vector<FgrInterpolator*> pairs;
+    std::vector<FgrBatchPair> desc;
+    std::vector<Launch> launches;
+    std::vector<uint8_t> blob;   // host copy of the device blob
+    uint8_t* dev = nullptr;
+    hipStream_t cap = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+
+    ~FgrBatch() {
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+        if (cap) hipStreamDestroy(cap);
+        if (dev) hipFree(dev);
+        for (FgrInterpolator* p : pairs) fgr_destroy(p);
+    }
+
+    // the plan of every pair, merged by (stage, pass); fills `blob` (params + work lists) and `launches`
+    void build() {
+        std::map<std::pair<int, int>, std::vector<std::pair<int, const PlanOp*>>> groups;
+        std::vector<std::vector<PlanOp>> plans(pairs.size());
+        for (size_t p = 0; p < pairs.size(); p++) {
+            FgrInterpolator& I = *pairs[p];
+            const FgrBatchPair& d = desc[p];
+            I.planEstimate(plans[p], d.prev, d.curr);
+            for (uint32_t k = 1; k < d.multiplier; k++)
+                I.planSynth(plans[p], d.prev, d.curr, d.out[k - 1], (float)((double)k / (double)d.multiplier), (int)k - 1);
+            for (const PlanOp& op : plans[p]) groups[{op.stage, op.pass}].push_back({(int)p, &op});
+        }
+        launches.clear();
+        blob.clear();
+        for (auto& g : groups) {
+            Launch L;
+            L.pass = g.first.second;
+            const size_t psz = g.second[0].second->params.size();
+            L.paramOff = blob.size();
+            L.paramBytes = psz * g.second.size();
+            blob.resize(align256(blob.size() + L.paramBytes));
+            for (size_t i = 0; i < g.second.size(); i++)
+                std::memcpy(blob.data() + L.paramOff + i * psz, g.second[i].second->params.data(), psz);
+            std::vector<uint32_t> work;
+            for (size_t i = 0; i < g.second.size(); i++) {
+                const PlanOp& op = *g.second[i].second;
+                for (int l = 0; l < op.gx * op.gy; l++) work.push_back(((uint32_t)i << kWorkShift) | (uint32_t)l);
+            }
+            L.n = (int)work.size();
+            L.workOff = blob.size();
+            blob.resize(align256(blob.size() + work.size() * 4));
+            std::memcpy(blob.data() + L.workOff, work.data(), work.size() * 4);
+            launches.push_back(L);
+        }
+    }
+
+    bool enqueue(hipStream_t s) {
+        for (const Launch& L : launches)
+            if (launchBatch(L.pass, dev + L.paramOff, (const uint32_t*)(dev + L.workOff), L.n, s) != hipSuccess)
+                return false;
+        return true;
+    }
+
+    void dropGraph() {
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+};
+
+static const char* kBErrNull = "null batch description";
+static const char* kBErrCount = "a batch holds 1 to 1024 pairs";
+static const char* kBErrMult = "a pair's multiplier must be in [2, 16]";
+static const char* kBErrFrames = "a pair has a null frame or pitch";
+static const char* kBErrAlloc = "a device allocation of the batch failed";
+static const char* kBErrStream = "the batch's capture stream could not be created";
+
+extern "C" FgrBatch* fgr_batch_create(const FgrBatchInfo* info, const FgrBatchPair* pairs, uint32_t n) {
+    gBatchError = nullptr;
+    if (!info || !pairs) { gBatchError = kBErrNull; return nullptr; }
+    if (n < 1 || n > 1024) { gBatchError = kBErrCount; return nullptr; }
+    for (uint32_t p = 0; p < n; p++) {
+        if (pairs[p].multiplier < 2 || pairs[p].multiplier > 16) { gBatchError = kBErrMult; return nullptr; }
+        if (!pairs[p].prev.ptr || !pairs[p].curr.ptr || !pairs[p].prev.pitch || !pairs[p].curr.pitch) {
+            gBatchError = kBErrFrames;
+            return nullptr;
+        }
+        for (uint32_t k = 0; k + 1 < pairs[p].multiplier; k++)
+            if (!pairs[p].out[k].ptr || !pairs[p].out[k].pitch) { gBatchError = kBErrFrames; return nullptr; }
+    }
+    FgrBatch* b = new FgrBatch();
+    b->info = *info;
+    b->desc.assign(pairs, pairs + n);
+    for (uint32_t p = 0; p < n; p++) {
+        FgrInitInfo ii{};
+        ii.width = pairs[p].width;
+        ii.height = pairs[p].height;
+        ii.format = pairs[p].format;
+        ii.bgra = info->bgra;
+        ii.transfer = info->transfer;
+        ii.linearWhiteNits = info->linearWhiteNits;
+        ii.yuvMatrix = info->yuvMatrix;
+        ii.yuvFullRange = info->yuvFullRange;
+        ii.device = info->device;
+        ii.settings = info->settings;
+        FgrInterpolator* h = fgr_create(&ii);
+        if (!h) {
+            gBatchError = fgr_last_init_error();
+            delete b;
+            return nullptr;
+        }
+        b->pairs.push_back(h);
+    }
+    b->build();
+    if (hipMalloc(&b->dev, b->blob.size()) != hipSuccess) { gBatchError = kBErrAlloc; delete b; return nullptr; }
+    if (hipMemcpy(b->dev, b->blob.data(), b->blob.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        gBatchError = kBErrAlloc;
+        delete b;
+        return nullptr;
+    }
+    if (hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess) {
+        b->cap = nullptr;
+        gBatchError = kBErrStream;
+        delete b;
+        return nullptr;
+    }
+    return b;
+}
+
+extern "C" const char* fgr_batch_last_error(void) { return gBatchError; }
+
+extern "C" void fgr_batch_destroy(FgrBatch* b) { delete b; }
+
+extern "C" uint32_t fgr_batch_launches(const FgrBatch* b) { return b ? (uint32_t)b->launches.size() : 0u; }
+
+extern "C" int fgr_batch_set_frames(FgrBatch* b, void* stream, const FgrBatchPair* pairs, uint32_t n) {
+    // same sizes, formats and multipliers; new frame pointers: rebuild the parameter blocks, stream-ordered upload
+    if (!b || !pairs || n != b->desc.size()) return 0;
+    for (uint32_t p = 0; p < n; p++)
+        if (pairs[p].width != b->desc[p].width || pairs[p].height != b->desc[p].height ||
+            pairs[p].multiplier != b->desc[p].multiplier || pairs[p].format != b->desc[p].format)
+            return 0;
+    b->desc.assign(pairs, pairs + n);
+    const size_t old = b->blob.size();
+    b->build();
+    if (b->blob.size() != old) return 0;
+    return hipMemcpyAsync(b->dev, b->blob.data(), b->blob.size(), hipMemcpyHostToDevice, (hipStream_t)stream) ==
+           hipSuccess;
+}
+
+extern "C" int fgr_batch_record(FgrBatch* b, void* stream) {
+    if (!b) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    if (!b->info.useGraph) return b->enqueue(s) ? 1 : 0;
+    if (!b->exec) {
+        if (hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal) != hipSuccess) return 0;
+        const bool ok = b->enqueue(b->cap);
+        if (hipStreamEndCapture(b->cap, &b->graph) != hipSuccess || !ok) {
+            b->dropGraph();
+            return 0;
+        }
+        if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess) {
+            b->dropGraph();
+            return 0;
+        }
+    }
+    return hipGraphLaunch(b->exec, s) == hipSuccess ? 1 : 0;
+}
+
+extern "C" int fgr_batch_debug_counters(FgrBatch* b, void* stream, uint32_t pair, uint32_t synth, void* dst) {
+    // [untrusted pixels of the pair's synth `synth` (0-based, t = (synth + 1) / multiplier), SAD sum x 256]
+    if (!b || pair >= b->pairs.size() || synth + 1 >= b->desc[pair].multiplier || !dst) return 0;
+    FgrInterpolator* h = b->pairs[pair];
+    return launchDebugCounters(h->u + kUSynth + kSlots * ((int)synth % kSynthBanks), h->u + kUSad, (uint32_t*)dst,
+                               (hipStream_t)stream) == hipSuccess;
+}
+
+extern "C" int fgr_batch_debug_internal(const FgrBatch* b, uint32_t pair, FgrDebugInternal* out) {
+    if (!b || pair >= b->pairs.size()) return 0;
+    return fgr_debug_internal(b->pairs[pair], out);
+}

@@ -327,27 +327,38 @@ FrameRef FgrInterpolator::ref(const FgrFrame& f) const {
     return r;
 }
 
-bool FgrInterpolator::enqueueEstimate(hipStream_t s, const FgrFrame& prev, const FgrFrame& curr) {
+static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& prev, const FgrFrame& curr) const {
     const FgrSettings& st = set;
-    beginProfile(s);
+    float2* F = (float2*)field;
     // 1. luma + INTER_AREA 1/S (also clears the counters)
-    LumaParams lp{};
-    lp.a = ref(prev);
-    lp.c = ref(curr);
-    lp.W = (int)W;
-    lp.H = (int)H;
-    lp.S = (int)st.flowScale;
-    lp.fw = fw;
-    lp.fh = fh;
-    lp.la = luma + lv[0].offA;
-    lp.lc = luma + lv[0].offC;
-    lp.u = u;
-    lp.bgra = (int)info.bgra;
-    lp.lut = fmtLut;
-    lp.yBlack = yBlack;
-    lp.yGain = yGain;
-    if (launchLuma(lp, format, s) != hipSuccess) return false;
-    stamp(s, "luma");
+    {
+        LumaParams lp{};
+        lp.a = ref(prev);
+        lp.c = ref(curr);
+        lp.W = (int)W;
+        lp.H = (int)H;
+        lp.S = (int)st.flowScale;
+        lp.fw = fw;
+        lp.fh = fh;
+        lp.la = luma + lv[0].offA;
+        lp.lc = luma + lv[0].offC;
+        lp.u = u;
+        lp.bgra = (int)info.bgra;
+        lp.lut = fmtLut;
+        lp.yBlack = yBlack;
+        lp.yGain = yGain;
+        lp.gx = cdiv(fw, 16);
+        PlanOp op;
+        op.pass = kPassLuma;
+        op.stage = 0;
+        op.gx = lp.gx;
+        op.gy = cdiv(fh, 16);
+        op.stamp = "luma";
+        op.set(lp);
+        plan.push_back(op);
+    }
     // 2. pyramid
     for (size_t i = 1; i < lv.size(); i++) {
         DownParams dp{};
@@ -359,14 +370,21 @@ bool FgrInterpolator::enqueueEstimate(hipStream_t s, const FgrFrame& prev, const
         dp.ph = lv[i - 1].h;
         dp.w = lv[i].w;
         dp.h = lv[i].h;
-        if (launchDown(dp, s) != hipSuccess) return false;
+        dp.gx = cdiv(dp.w, 16);
+        PlanOp op;
+        op.pass = kPassDown;
+        op.stage = 100 + (int)i;
+        op.gx = dp.gx;
+        op.gy = cdiv(dp.h, 16);
+        op.stamp = i + 1 == lv.size() ? "pyramid" : nullptr;
+        op.set(dp);
+        plan.push_back(op);
     }
-    stamp(s, "pyramid");
     // 3-5. per level, coarse -> fine: median, seed + search, median3
     static const char* lvNames[] = {"level 0 (finest)", "level 1", "level 2", "level 3", "level 4", "level 5", "level 6", "level 7"};
     const int n = (int)lv.size();
-    float2* F = (float2*)field;
     for (int i = n - 1; i >= 0; i--) {
+        const int step = n - 1 - i;
         const Level& l = lv[i];
         SearchParams sp{};
         sp.la = luma + l.offA;
@@ -381,33 +399,51 @@ bool FgrInterpolator::enqueueEstimate(hipStream_t s, const FgrFrame& prev, const
             sp.parent = nullptr;
             sp.R = (int)st.searchRadiusTop;
         } else {
-            const Level& p = lv[i + 1];
+            const Level& pl = lv[i + 1];
             MedParams mp{};
-            mp.parent = F + p.fM;
-            mp.pw = p.bw;
-            mp.ph = p.bh;
+            mp.parent = F + pl.fM;
+            mp.pw = pl.bw;
+            mp.ph = pl.bh;
             mp.bw = l.bw;
             mp.bh = l.bh;
             mp.u = u;
             mp.medIdx = kUMedian + 2 * i;
-            if (launchMedhist(mp, s) != hipSuccess) return false;
-            sp.parent = F + p.fM;
-            sp.pw = p.bw;
-            sp.ph = p.bh;
+            mp.gx = 1;
+            PlanOp op;
+            op.pass = kPassMedhist;
+            op.stage = 200 + 10 * step;
+            op.set(mp);
+            plan.push_back(op);
+            sp.parent = F + pl.fM;
+            sp.pw = pl.bw;
+            sp.ph = pl.bh;
             sp.med = u + kUMedian + 2 * i;
             sp.R = (int)st.searchRadius;
             sp.extraSeed = (st.seedNet && seedLevel == i && seedReady) ? F + fieldSeed : nullptr;
         }
         sp.out = F + l.fS;
         sp.sadAcc = (i == 0 && !st.subpixel) ? 1 : 0;
-        if (launchSearch(sp, s) != hipSuccess) return false;
+        sp.gx = cdiv(l.bw * l.bh, 4);
+        PlanOp op;
+        op.pass = sp.R <= 2 ? kPassSearch2 : (sp.R <= 6 ? kPassSearch6 : kPassSearch12);
+        op.stage = 201 + 10 * step;
+        op.gx = sp.gx;
+        op.set(sp);
+        plan.push_back(op);
         Median3Params m3{};
         m3.in = F + l.fS;
         m3.out = F + l.fM;
         m3.bw = l.bw;
         m3.bh = l.bh;
-        if (launchMedian3(m3, s) != hipSuccess) return false;
-        stamp(s, lvNames[std::min(i, 7)]);
+        m3.gx = cdiv(l.bw, 16);
+        PlanOp om;
+        om.pass = kPassMedian3;
+        om.stage = 202 + 10 * step;
+        om.gx = m3.gx;
+        om.gy = cdiv(l.bh, 16);
+        om.stamp = lvNames[std::min(i, 7)];
+        om.set(m3);
+        plan.push_back(om);
     }
     // 6. sub-pixel refinement (+ the scene-cut SAD sum)
     const Level& l0 = lv[0];
@@ -424,11 +460,16 @@ bool FgrInterpolator::enqueueEstimate(hipStream_t s, const FgrFrame& prev, const
         q.out = F + fieldSub;
         q.lam = st.searchPenalty;
         q.u = u;
-        if (launchSubpix(q, s) != hipSuccess) return false;
-        stamp(s, "subpixel");
+        q.gx = cdiv(l0.bw * l0.bh, 4);
+        PlanOp op;
+        op.pass = kPassSubpix;
+        op.stage = 400;
+        op.gx = q.gx;
+        op.stamp = "subpixel";
+        op.set(q);
+        plan.push_back(op);
         bfield = fieldSub;
     }
-    blockFieldOff = bfield;
     // 8. per-pixel field
     PixParams pp{};
     pp.la = luma + l0.offA;
@@ -441,12 +482,43 @@ bool FgrInterpolator::enqueueEstimate(hipStream_t s, const FgrFrame& prev, const
     pp.pf = F + fieldPF;
     pp.smooth = st.smoothBonus;
     pp.zero = st.hudBonus;
-    if (launchPixfield(pp, s) != hipSuccess) return false;
-    stamp(s, "pixel field");
+    pp.gx = cdiv(l0.w, 16);
+    PlanOp op;
+    op.pass = kPassPixfield;
+    op.stage = 401;
+    op.gx = pp.gx;
+    op.gy = cdiv(l0.h, 16);
+    op.stamp = "pixel field";
+    op.set(pp);
+    plan.push_back(op);
+}
+
+bool FgrInterpolator::runPlan(hipStream_t s, const std::vector<PlanOp>& plan) {
+    for (const PlanOp& op : plan) {
+        if (launchSingle(op.pass, op.params.data(), op.gx, op.gy, s) != hipSuccess) return false;
+        if (op.stamp) stamp(s, op.stamp);
+    }
     return true;
 }
 
-void FgrInterpolator::synthParams(SynthParams& p, const FgrFrame& prev, const FgrFrame& curr, float t, int slot) const {
+bool FgrInterpolator::enqueueEstimate(hipStream_t s, const FgrFrame& prev, const FgrFrame& curr) {
+    std::vector<PlanOp> plan;
+    planEstimate(plan, prev, curr);
+    beginProfile(s);
+    if (!runPlan(s, plan)) return false;
+    blockFieldOff = set.subpixel ? fieldSub : lv[0].fM;
+    return true;
+}
+
+// Trust as the synth kernel computes it (interp.py:236-237), for the host's tabulation of where it is 0, 1 and > 0.5
+static float trustHost(int S, float lo, double rden) {
+    const float e7 = (float)((double)S * (1.0 / 49.0));
+    const float u = (float)((double)(e7 - lo) * rden);
+    return std::fmin(std::fmax(u, 0.0f), 1.0f);
+}
+
+void FgrInterpolator::synthParams(SynthParams& p, const FgrFrame& prev, const FgrFrame& curr, float t, int slot,
+                                  bool fusedLookup) const {
     const FgrSettings& st = set;
     p = SynthParams{};
     p.a = ref(prev);
@@ -457,45 +529,59 @@ void FgrInterpolator::synthParams(SynthParams& p, const FgrFrame& prev, const Fg
     p.fw = fw;
     p.fh = fh;
     const float2* F = (const float2*)field;
-    p.field = t != 0.5f ? F + fieldT : F + fieldPF;
-    p.sadsum = u + kUSadSum;
-    p.count = u + kUSynth + slot;
+    p.lookup = (t != 0.5f && fusedLookup) ? 1 : 0;
+    p.field = (t != 0.5f && !fusedLookup) ? F + fieldT : F + fieldPF;
+    p.s = (float)(2.0 * (double)t - 1.0);
+    p.sadsum = u + kUSad;
+    p.count = u + kUSynth + kSlots * slot;
     p.nblocks = lv[0].bw * lv[0].bh;
     p.cutSad = st.sceneCutSad;
     p.gfrac = st.globalFallback;
     p.trust0 = st.trustLow;
     p.den = st.trustHigh - st.trustLow;
     p.rden = 1.0 / (double)p.den;
+    // where the (monotone) trust of a 7x7 sum is 0, 1, and past 0.5
+    p.sZero = -1;
+    p.sOne = 49 * 255 + 1;
+    p.sHalf = -1;
+    for (int S = 0; S <= 49 * 255; S++) {
+        const float uS = trustHost(S, p.trust0, p.rden);
+        if (uS == 0.0f) p.sZero = S;
+        if (uS == 1.0f && p.sOne > S) p.sOne = S;
+        if (uS <= 0.5f) p.sHalf = S;
+    }
     // framegen.cpp:3490-3497: constants folded in double from the float t, then rounded to float
     const double td = t;
     p.wA = t != 0.5f ? (float)(1.0 - td) : 0.5f;
     p.wC = t != 0.5f ? (float)td : 0.5f;
     p.mid = t == 0.5f;
     p.nearC = p.wC > 0.5f;
+    const float qa = 4.0f * p.wA, qc = 4.0f * p.wC;
+    p.quarter = (qa == std::rint(qa) && qc == std::rint(qc)) ? 1 : 0;
+    p.kA = (int)qa;
+    p.kC = (int)qc;
     p.lut = fmtLut;
     for (int i = 0; i < 8; i++) p.yuv[i] = yuv[i];
 }
 
-bool FgrInterpolator::enqueueSynth(hipStream_t s, const FgrFrame& prev, const FgrFrame& curr, const FgrFrame& out, float t) {
-    const int slot = synthK & 1;
-    if (t != 0.5f) {
-        LookupParams lk{};
-        lk.pf = (const float2*)field + fieldPF;
-        lk.out = (float2*)field + fieldT;
-        lk.fw = fw;
-        lk.fh = fh;
-        lk.s = (float)(2.0 * (double)t - 1.0);
-        if (launchLookup(lk, s) != hipSuccess) return false;
-        stamp(s, "lookup");
-    }
+void FgrInterpolator::planSynth(std::vector<PlanOp>& plan, const FgrFrame& prev, const FgrFrame& curr,
+                                const FgrFrame& out, float t, int k) const {
+    const int bank = k % kSynthBanks;
     SynthParams sp;
-    synthParams(sp, prev, curr, t, slot);
+    synthParams(sp, prev, curr, t, bank, true);
     sp.out = (uint8_t*)out.ptr;
     sp.outc = (uint8_t*)out.chroma;
     sp.opitch = (int64_t)out.pitch;
     sp.ocpitch = (int64_t)out.chromaPitch;
-    if (launchSynth(sp, format, s) != hipSuccess) return false;
-    stamp(s, "synth");
+    sp.gx = synthTilesX((int)W);
+    PlanOp op;
+    op.pass = set.flowScale == 1 ? kPassSynthRgba8S1 : (set.flowScale == 2 ? kPassSynthRgba8S2 : kPassSynthRgba8S4);
+    op.stage = 500 + 10 * k;
+    op.gx = sp.gx;
+    op.gy = synthTilesY((int)H);
+    op.stamp = "synth";
+    op.set(sp);
+    plan.push_back(op);
     FallbackParams fp{};
     fp.a = sp.a;
     fp.c = sp.c;
@@ -506,13 +592,26 @@ bool FgrInterpolator::enqueueSynth(hipStream_t s, const FgrFrame& prev, const Fg
     fp.W = (int)W;
     fp.H = (int)H;
     fp.bpp = bytesPerPixel();
-    fp.count = u + kUSynth + slot;
+    fp.count = sp.count;
     fp.gfrac = set.globalFallback;
     fp.nearC = sp.nearC;
-    fp.nextCount = u + kUSynth + (slot ^ 1);
-    if (launchFallback(fp, format, s) != hipSuccess) return false;
-    stamp(s, "fallback");
-    lastSlot = slot;
+    fp.nextCount = u + kUSynth + kSlots * ((k + 1) % kSynthBanks);
+    fp.gx = fallbackTilesX((int)W);
+    PlanOp of;
+    of.pass = kPassFallback;
+    of.stage = 501 + 10 * k;
+    of.gx = fp.gx;
+    of.gy = fallbackTilesY((int)H);
+    of.stamp = "fallback";
+    of.set(fp);
+    plan.push_back(of);
+}
+
+bool FgrInterpolator::enqueueSynth(hipStream_t s, const FgrFrame& prev, const FgrFrame& curr, const FgrFrame& out, float t) {
+    std::vector<PlanOp> plan;
+    planSynth(plan, prev, curr, out, t, synthK);
+    if (!runPlan(s, plan)) return false;
+    lastSlot = synthK % kSynthBanks;
     lastT = t;
     synthK++;
     return true;
@@ -566,10 +665,8 @@ extern "C" int fgr_record(FgrInterpolator* h, void* stream, FgrFrame prev, FgrFr
 extern "C" int fgr_record_debug_counters(FgrInterpolator* h, void* stream, void* dst) {
     // framegen.cpp:2073-2084: [untrusted pixels of the latest synth, the estimate's SAD sum x 256]
     if (!h || !h->W || !dst) return 0;
-    hipStream_t s = (hipStream_t)stream;
-    if (hipMemcpyAsync(dst, h->u + kUSynth + h->lastSlot, 4, hipMemcpyDeviceToDevice, s) != hipSuccess) return 0;
-    if (hipMemcpyAsync((uint8_t*)dst + 4, h->u + kUSadSum, 4, hipMemcpyDeviceToDevice, s) != hipSuccess) return 0;
-    return 1;
+    return launchDebugCounters(h->u + kUSynth + kSlots * h->lastSlot, h->u + kUSad, (uint32_t*)dst,
+                               (hipStream_t)stream) == hipSuccess;
 }
 
 extern "C" uint32_t fgr_debug_sad_blocks(const FgrInterpolator* h) {
@@ -607,7 +704,16 @@ extern "C" int fgr_record_debug_maps(FgrInterpolator* h, void* stream, void* dst
     if (h->lastT > 0.0f) {
         // the mismatch of the latest synth, recomputed from its field at t (the fused synth never stores e)
         SynthParams sp;
-        h->synthParams(sp, h->estPrevF, h->estCurrF, h->lastT, h->lastSlot);
+        h->synthParams(sp, h->estPrevF, h->estCurrF, h->lastT, h->lastSlot, false);
+        if (h->lastT != 0.5f) {   // the field at t into the scratch (the fused synth computes it per tile)
+            LookupParams lk{};
+            lk.pf = (const float2*)h->field + h->fieldPF;
+            lk.out = (float2*)h->field + h->fieldT;
+            lk.fw = h->fw;
+            lk.fh = h->fh;
+            lk.s = sp.s;
+            if (launchLookup(lk, s) != hipSuccess) return 0;
+        }
         if (launchDebugE(sp, (uint32_t*)(d + l.eOff), (int)l.eStride, h->format, s) != hipSuccess) return 0;
     } else if (hipMemsetAsync(d + l.eOff, 0, (size_t)l.eStride * l.H * 4, s) != hipSuccess) {
         return 0;

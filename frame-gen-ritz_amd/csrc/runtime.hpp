@@ -1,6 +1,7 @@
 // The state behind an FgrInterpolator handle (framegen::Interpolator::Impl, gpu/framegen.cpp:791-994 for V1).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <cstring>
 #include <vector>
 #include "../../include/fgr.h"
 #include "v1_params.hpp"
@@ -9,6 +10,19 @@
 // raw search output (fS) and the median3 output (fM), in float / float2 units of the shared buffers.
 struct Level {
     int w, h, bw, bh, offA, offC, fS, fM;
+};
+
+// One kernel launch of a record, before it is launched: the pass, its stage in the record (ops of the same stage of
+// different pairs are independent and run as one batched launch), the per-pair grid and the parameter block.
+struct PlanOp {
+    int pass = 0, stage = 0, gx = 1, gy = 1;
+    const char* stamp = nullptr;   // profile group that ends with this op (the single-pair path)
+    std::vector<uint8_t> params;
+    template <class T>
+    void set(const T& p) {
+        params.resize(sizeof(T));
+        std::memcpy(params.data(), &p, sizeof(T));
+    }
 };
 
 struct FgrInterpolator {
@@ -61,9 +75,14 @@ struct FgrInterpolator {
     int bytesPerPixel() const;
     void stamp(hipStream_t s, const char* name);
     void beginProfile(hipStream_t s);
+    void planEstimate(std::vector<PlanOp>& plan, const FgrFrame& prev, const FgrFrame& curr) const;
+    void planSynth(std::vector<PlanOp>& plan, const FgrFrame& prev, const FgrFrame& curr, const FgrFrame& out, float t,
+                   int k) const;
+    bool runPlan(hipStream_t s, const std::vector<PlanOp>& plan);
     bool enqueueEstimate(hipStream_t s, const FgrFrame& prev, const FgrFrame& curr);
     bool enqueueSynth(hipStream_t s, const FgrFrame& prev, const FgrFrame& curr, const FgrFrame& out, float t);
-    void synthParams(fgr::SynthParams& p, const FgrFrame& prev, const FgrFrame& curr, float t, int slot) const;
+    void synthParams(fgr::SynthParams& p, const FgrFrame& prev, const FgrFrame& curr, float t, int slot,
+                     bool fusedLookup) const;
     bool copyFrame(hipStream_t s, const FgrFrame& src, const FgrFrame& dst);
     bool enqueueBlur(hipStream_t s, const FgrFrame& prev, const FgrFrame& curr, const FgrFrame& out, float t0, float t1);
 };

@@ -74,4 +74,14 @@ __device__ __forceinline__ float treeSum64(float d) {
     return __shfl(d, 0, 64);
 }
 
+// Sum of the kSlots partial counters (the whole wavefront must be active): lane i reads slot i, integer wave reduce.
+__device__ __forceinline__ uint32_t slotSum(const uint32_t* slots) {
+    uint32_t v = slots[threadIdx.x & 63];
+#pragma unroll
+    for (int s = 32; s >= 1; s >>= 1) v += __shfl_xor(v, s, 64);
+    return v;
+}
+
+__device__ __forceinline__ int linearBlock() { return (int)(blockIdx.y * gridDim.x + blockIdx.x); }
+
 }  // namespace fgr

@@ -23,10 +23,11 @@ __device__ __forceinline__ int luma8(uint32_t px, int bgra) {
 
 // luma.comp:58-72 + interp.py:150: level-0 luma = mean of the valid integer lumas of each S x S cell (integer sums,
 // one float32 division). Block (0, 0) also clears the estimate's counters (framegen.cpp:2929 clearCounters(64)).
-__global__ __launch_bounds__(256) void luma_rgba8_kernel(LumaParams P) {
-    const int tid = threadIdx.y * 16 + threadIdx.x;
-    if (blockIdx.x == 0 && blockIdx.y == 0 && tid < kUWords) P.u[tid] = 0u;
-    const int ox = blockIdx.x * 16 + threadIdx.x, oy = blockIdx.y * 16 + threadIdx.y;
+__device__ __forceinline__ void lumaRgba8Body(const LumaParams& P, int bx, int by) {
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    if (bx == 0 && by == 0)
+        for (int i = threadIdx.x; i < kUWords; i += 256) P.u[i] = 0u;
+    const int ox = bx * 16 + tx, oy = by * 16 + ty;
     if (ox >= P.fw || oy >= P.fh) return;
     int sa = 0, sc = 0, n = 0;
     for (int j = 0; j < P.S; j++) {
@@ -47,8 +48,8 @@ __global__ __launch_bounds__(256) void luma_rgba8_kernel(LumaParams P) {
 }
 
 // down.comp:9-26: 2x2 mean of the valid pixels, summed j-outer / i-inner, times 0.25 or float(1/n)
-__global__ __launch_bounds__(256) void down_kernel(DownParams P) {
-    const int ox = blockIdx.x * 16 + threadIdx.x, oy = blockIdx.y * 16 + threadIdx.y;
+__device__ __forceinline__ void downBody(const DownParams& P, int bx, int by) {
+    const int ox = bx * 16 + (threadIdx.x & 15), oy = by * 16 + (threadIdx.x >> 4);
     if (ox >= P.w || oy >= P.h) return;
     float sa = 0.0f, sc = 0.0f;
     int n = 0;
@@ -92,14 +93,19 @@ __device__ __forceinline__ void histAdd(uint32_t* hist, int bin, int weight) {
     }
 }
 
-__global__ __launch_bounds__(256) void medhist_kernel(MedParams P) {
+__device__ __forceinline__ void medhistBody(const MedParams& P, int, int) {
     __shared__ uint32_t hx[256], hy[256];
+    __shared__ uint32_t wsum[2][4];
+    __shared__ float kv[4];
     const int t = threadIdx.x;
-    hx[t] = 0u;
-    hy[t] = 0u;
+    if (t < 256) {
+        hx[t] = 0u;
+        hy[t] = 0u;
+    }
+    if (t < 4) kv[t] = 127.0f;
     __syncthreads();
     const int np = P.pw * P.ph;
-    for (int i0 = 0; i0 < np; i0 += 256) {
+    for (int i0 = 0; i0 < np; i0 += 1024) {
         const int i = i0 + t;
         if (i < np) {
             const int px = i % P.pw, py = i / P.pw;
@@ -112,19 +118,42 @@ __global__ __launch_bounds__(256) void medhist_kernel(MedParams P) {
         }
     }
     __syncthreads();
-    if (t < 2) {
-        const uint32_t* hist = t == 0 ? hx : hy;
-        const uint32_t n = (uint32_t)(P.bw * P.bh);
-        const uint32_t k1 = (n - 1u) / 2u, k2 = n / 2u;
-        float v1 = 127.0f, v2 = 127.0f;
-        uint32_t acc = 0u;
-        bool f1 = false;
-        for (int b = 0; b < 256; b++) {
-            acc += hist[b];
-            if (!f1 && acc > k1) { v1 = (float)(b - 128); f1 = true; }
-            if (acc > k2) { v2 = (float)(b - 128); break; }
+    // inclusive prefix over the 256 bins: a scan per wavefront, then the wavefront totals
+    uint32_t cx = 0u, cy = 0u;
+    if (t < 256) {
+        cx = hx[t];
+        cy = hy[t];
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t vx = __shfl_up(cx, o, 64), vy = __shfl_up(cy, o, 64);
+            if ((t & 63) >= o) {
+                cx += vx;
+                cy += vy;
+            }
         }
-        const float m = 0.5f * (v1 + v2);
+        if ((t & 63) == 63) {
+            wsum[0][t >> 6] = cx;
+            wsum[1][t >> 6] = cy;
+        }
+    }
+    __syncthreads();
+    if (t < 256) {
+        for (int j = 0; j < (t >> 6); j++) {
+            cx += wsum[0][j];
+            cy += wsum[1][j];
+        }
+        // the k-th element (0-based) lies in the first bin whose cumulative count exceeds k (medhist.comp:77-84)
+        const uint32_t n = (uint32_t)(P.bw * P.bh), k1 = (n - 1u) / 2u, k2 = n / 2u;
+        const uint32_t ex = cx - hx[t], ey = cy - hy[t];
+        const float b = (float)(t - 128);
+        if (ex <= k1 && k1 < cx) kv[0] = b;
+        if (ex <= k2 && k2 < cx) kv[1] = b;
+        if (ey <= k1 && k1 < cy) kv[2] = b;
+        if (ey <= k2 && k2 < cy) kv[3] = b;
+    }
+    __syncthreads();
+    if (t < 2) {
+        const float m = 0.5f * (kv[2 * t] + kv[2 * t + 1]);
         P.u[P.medIdx + t] = __float_as_uint(2.0f * m);
     }
 }
@@ -139,12 +168,12 @@ __global__ __launch_bounds__(256) void medhist_kernel(MedParams P) {
 //  3. lane = candidate (dy-major), sequential y-outer / x-inner SAD over LDS, cost SAD/64 + lambda |d|_1;
 //  4. wavefront argmin on (cost, candidate index): the first of the minimal costs, as the sequential strict <.
 template <int MAXR>
-__global__ __launch_bounds__(256) void search_kernel(SearchParams P) {
+__device__ __forceinline__ void searchBody(const SearchParams& P, int bx0, int) {
     constexpr int MAXT = 8 + 2 * MAXR;
     __shared__ float ta[4][MAXT * MAXT], tc[4][MAXT * MAXT];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int nb = P.bw * P.bh;
-    const int blk = blockIdx.x * 4 + wave;
+    const int blk = bx0 * 4 + wave;
     const bool active = blk < nb;
     const int bx = active ? blk % P.bw : 0, by = active ? blk / P.bw : 0;
     const int b0x = bx * kBlock, b0y = by * kBlock;
@@ -226,9 +255,12 @@ __global__ __launch_bounds__(256) void search_kernel(SearchParams P) {
             bests = os;
         }
     }
-    if (active && lane == 0) {
-        P.out[blk] = make_float2((float)(sx + besti % D - R), (float)(sy + besti / D - R));
-        if (P.sadAcc) atomicAdd(P.u + kUSadSum, (uint32_t)rintf(bests * 256.0f));
+    if (active && lane == 0) P.out[blk] = make_float2((float)(sx + besti % D - R), (float)(sy + besti / D - R));
+    if (P.sadAcc) {   // search.comp:85, one atomic per workgroup into its slot
+        __shared__ uint32_t wsad[4];
+        if (lane == 0) wsad[wave] = active ? (uint32_t)rintf(bests * 256.0f) : 0u;
+        __syncthreads();
+        if (threadIdx.x == 0) atomicAdd(P.u + kUSad + (bx0 & (kSlots - 1)), wsad[0] + wsad[1] + wsad[2] + wsad[3]);
     }
 }
 
@@ -244,8 +276,8 @@ __device__ __forceinline__ float med9(float* v) {
     return v[4];
 }
 
-__global__ __launch_bounds__(256) void median3_kernel(Median3Params P) {
-    const int bx = blockIdx.x * 16 + threadIdx.x, by = blockIdx.y * 16 + threadIdx.y;
+__device__ __forceinline__ void median3Body(const Median3Params& P, int gbx, int gby) {
+    const int bx = gbx * 16 + (threadIdx.x & 15), by = gby * 16 + (threadIdx.x >> 4);
     if (bx >= P.bw || by >= P.bh) return;
     float xs[9], ys[9];
     int k = 0;
@@ -273,14 +305,14 @@ __device__ __forceinline__ float tileTap(const float* t, int T, float px, float 
 // subpix.comp:26-73 (interp.py:162-163): 3x3 at 0.5 px then 3x3 at 0.25 px around the median-filtered integer vector,
 // one wavefront per block: lanes take (candidate, row) items, rows are summed per candidate, strict < argmin.
 // Adds roundEven(256 * winning SAD) to the scene-cut sum.
-__global__ __launch_bounds__(256) void subpix_kernel(SubpixParams P) {
+__device__ __forceinline__ void subpixBody(const SubpixParams& P, int bx0, int) {
     constexpr int T = 13, O = 2;
     __shared__ float ta[4][T * T], tc[4][T * T];
     __shared__ float rows[4][72];
     __shared__ float costs[4][9], sads[4][9];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int nb = P.bw * P.bh;
-    const int blk = blockIdx.x * 4 + wave;
+    const int blk = bx0 * 4 + wave;
     const bool active = blk < nb;
     const int bx = active ? blk % P.bw : 0, by = active ? blk / P.bw : 0;
     const int b0x = bx * kBlock, b0y = by * kBlock;
@@ -328,10 +360,14 @@ __global__ __launch_bounds__(256) void subpix_kernel(SubpixParams P) {
         bestsad = sads[wave][bi];
         __syncthreads();
     }
-    if (active && lane == 0) {
-        P.out[blk] = make_float2((float)sx + basex, (float)sy + basey);
-        atomicAdd(P.u + kUSadSum, (uint32_t)rintf(bestsad * 256.0f));
+    // subpix.comp:65-69: the block's SAD into the scene-cut sum (one atomic per workgroup into its slot)
+    __shared__ uint32_t wsad[4];
+    if (lane == 0) {
+        if (active) P.out[blk] = make_float2((float)sx + basex, (float)sy + basey);
+        wsad[wave] = active ? (uint32_t)rintf(bestsad * 256.0f) : 0u;
     }
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(P.u + kUSad + (bx0 & (kSlots - 1)), wsad[0] + wsad[1] + wsad[2] + wsad[3]);
 }
 
 // ---------------------------------------------------------------------------------------------------------- pixfield
@@ -361,11 +397,12 @@ __device__ __forceinline__ float2 pixCand(int k, int qx, int qy, const float2* b
 // pixfield.comp:37-63 (interp.py:167-183): 16x16 flow pixels per workgroup, the 7 candidates' errors of the 20x20 tile
 // (+2 apron, replicate) in LDS; cost = float32(exact window sum * (1/25)) with the sum in float64 (cv2.blur's CV_32F
 // path), - smoothBonus for the bilinear candidate, - hudBonus for zero, strict < (first wins).
-__global__ __launch_bounds__(256) void pixfield_kernel(PixParams P) {
+__device__ __forceinline__ void pixfieldBody(const PixParams& P, int gbx, int gby) {
     constexpr int TS = 16, AP = 2, TT = TS + 2 * AP, NC = 7;
     __shared__ float err[NC][TT * TT];
-    const int tid = threadIdx.y * TS + threadIdx.x;
-    const int t0x = blockIdx.x * TS - AP, t0y = blockIdx.y * TS - AP;
+    const int tid = threadIdx.x;
+    const int lx = tid & 15, ly = tid >> 4;
+    const int t0x = gbx * TS - AP, t0y = gby * TS - AP;
     for (int i = tid; i < TT * TT; i += TS * TS) {
         const int qx = clampi(t0x + i % TT, 0, P.w - 1), qy = clampi(t0y + i / TT, 0, P.h - 1);
         for (int k = 0; k < NC; k++) {
@@ -375,14 +412,14 @@ __global__ __launch_bounds__(256) void pixfield_kernel(PixParams P) {
         }
     }
     __syncthreads();
-    const int qx = blockIdx.x * TS + threadIdx.x, qy = blockIdx.y * TS + threadIdx.y;
+    const int qx = gbx * TS + lx, qy = gby * TS + ly;
     if (qx >= P.w || qy >= P.h) return;
     int bk = 0;
     float best = 0.0f;
     for (int k = 0; k < NC; k++) {
         double vs = 0.0;
         for (int y = 0; y < 5; y++) {
-            const float* r = &err[k][(threadIdx.y + y) * TT + threadIdx.x];
+            const float* r = &err[k][(ly + y) * TT + lx];
             double hs = 0.0;
             for (int x = 0; x < 5; x++) hs = hs + (double)r[x];
             vs = vs + hs;
@@ -398,45 +435,120 @@ __global__ __launch_bounds__(256) void pixfield_kernel(PixParams P) {
     P.pf[qy * P.w + qx] = pixCand(bk, qx, qy, P.bf, P.bw, P.bh);
 }
 
+// ---------------------------------------------------------------------------------------------------------- kernels
+
+// Each pass has a per-pair kernel (grid = that pair's blocks) and a batched kernel over a work list spanning many pairs
+// (one launch per pass for a whole batch); both run the same body.
+#define FGR_PASS(name, Params, body, threads)                                                                    \
+    __global__ __launch_bounds__(threads) void name##_kernel(Params P) { body(P, (int)blockIdx.x, (int)blockIdx.y); } \
+    __global__ __launch_bounds__(threads) void name##_batch_kernel(const Params* __restrict__ P,                  \
+                                                                   const uint32_t* __restrict__ work) {          \
+        const uint32_t w = work[blockIdx.x];                                                                     \
+        const Params p = P[w >> kWorkShift];                                                                     \
+        const int l = (int)(w & kWorkMask);                                                                      \
+        body(p, l % p.gx, l / p.gx);                                                                             \
+    }
+
+FGR_PASS(luma_rgba8, LumaParams, lumaRgba8Body, 256)
+FGR_PASS(down, DownParams, downBody, 256)
+FGR_PASS(medhist, MedParams, medhistBody, 1024)
+FGR_PASS(search2, SearchParams, searchBody<2>, 256)
+FGR_PASS(search6, SearchParams, searchBody<6>, 256)
+FGR_PASS(search12, SearchParams, searchBody<12>, 256)
+FGR_PASS(median3, Median3Params, median3Body, 256)
+FGR_PASS(subpix, SubpixParams, subpixBody, 256)
+FGR_PASS(pixfield, PixParams, pixfieldBody, 256)
+
 // ---------------------------------------------------------------------------------------------------------- launches
 
-static inline dim3 grid2(int w, int h, int t) { return dim3((unsigned)((w + t - 1) / t), (unsigned)((h + t - 1) / t)); }
+static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 hipError_t launchLumaRgba8(const LumaParams& p, hipStream_t s) {
-    hipLaunchKernelGGL(luma_rgba8_kernel, grid2(p.fw, p.fh, 16), dim3(16, 16), 0, s, p);
+    LumaParams q = p;
+    q.gx = cdiv(p.fw, 16);
+    hipLaunchKernelGGL(luma_rgba8_kernel, dim3((unsigned)q.gx, (unsigned)cdiv(p.fh, 16)), dim3(256), 0, s, q);
     return hipGetLastError();
 }
 
 hipError_t launchDown(const DownParams& p, hipStream_t s) {
-    hipLaunchKernelGGL(down_kernel, grid2(p.w, p.h, 16), dim3(16, 16), 0, s, p);
+    DownParams q = p;
+    q.gx = cdiv(p.w, 16);
+    hipLaunchKernelGGL(down_kernel, dim3((unsigned)q.gx, (unsigned)cdiv(p.h, 16)), dim3(256), 0, s, q);
     return hipGetLastError();
 }
 
 hipError_t launchMedhist(const MedParams& p, hipStream_t s) {
-    hipLaunchKernelGGL(medhist_kernel, dim3(1), dim3(256), 0, s, p);
+    MedParams q = p;
+    q.gx = 1;
+    hipLaunchKernelGGL(medhist_kernel, dim3(1), dim3(1024), 0, s, q);
     return hipGetLastError();
 }
 
 hipError_t launchSearch(const SearchParams& p, hipStream_t s) {
-    const unsigned g = (unsigned)((p.bw * p.bh + 3) / 4);
-    if (p.R <= 2) hipLaunchKernelGGL(search_kernel<2>, dim3(g), dim3(256), 0, s, p);
-    else if (p.R <= 6) hipLaunchKernelGGL(search_kernel<6>, dim3(g), dim3(256), 0, s, p);
-    else hipLaunchKernelGGL(search_kernel<12>, dim3(g), dim3(256), 0, s, p);
+    SearchParams q = p;
+    q.gx = cdiv(p.bw * p.bh, 4);
+    const dim3 g((unsigned)q.gx);
+    if (p.R <= 2) hipLaunchKernelGGL(search2_kernel, g, dim3(256), 0, s, q);
+    else if (p.R <= 6) hipLaunchKernelGGL(search6_kernel, g, dim3(256), 0, s, q);
+    else hipLaunchKernelGGL(search12_kernel, g, dim3(256), 0, s, q);
     return hipGetLastError();
 }
 
 hipError_t launchMedian3(const Median3Params& p, hipStream_t s) {
-    hipLaunchKernelGGL(median3_kernel, grid2(p.bw, p.bh, 16), dim3(16, 16), 0, s, p);
+    Median3Params q = p;
+    q.gx = cdiv(p.bw, 16);
+    hipLaunchKernelGGL(median3_kernel, dim3((unsigned)q.gx, (unsigned)cdiv(p.bh, 16)), dim3(256), 0, s, q);
     return hipGetLastError();
 }
 
 hipError_t launchSubpix(const SubpixParams& p, hipStream_t s) {
-    hipLaunchKernelGGL(subpix_kernel, dim3((unsigned)((p.bw * p.bh + 3) / 4)), dim3(256), 0, s, p);
+    SubpixParams q = p;
+    q.gx = cdiv(p.bw * p.bh, 4);
+    hipLaunchKernelGGL(subpix_kernel, dim3((unsigned)q.gx), dim3(256), 0, s, q);
     return hipGetLastError();
 }
 
 hipError_t launchPixfield(const PixParams& p, hipStream_t s) {
-    hipLaunchKernelGGL(pixfield_kernel, grid2(p.w, p.h, 16), dim3(16, 16), 0, s, p);
+    PixParams q = p;
+    q.gx = cdiv(p.w, 16);
+    hipLaunchKernelGGL(pixfield_kernel, dim3((unsigned)q.gx, (unsigned)cdiv(p.h, 16)), dim3(256), 0, s, q);
+    return hipGetLastError();
+}
+
+// one plan op of one pair: grid gx x gy of the per-pair kernel
+hipError_t launchSingle(int pass, const void* params, int gx, int gy, hipStream_t s) {
+    const dim3 g((unsigned)gx, (unsigned)gy);
+    switch (pass) {
+    case kPassLuma: hipLaunchKernelGGL(luma_rgba8_kernel, g, dim3(256), 0, s, *(const LumaParams*)params); break;
+    case kPassDown: hipLaunchKernelGGL(down_kernel, g, dim3(256), 0, s, *(const DownParams*)params); break;
+    case kPassMedhist: hipLaunchKernelGGL(medhist_kernel, g, dim3(1024), 0, s, *(const MedParams*)params); break;
+    case kPassSearch2: hipLaunchKernelGGL(search2_kernel, g, dim3(256), 0, s, *(const SearchParams*)params); break;
+    case kPassSearch6: hipLaunchKernelGGL(search6_kernel, g, dim3(256), 0, s, *(const SearchParams*)params); break;
+    case kPassSearch12: hipLaunchKernelGGL(search12_kernel, g, dim3(256), 0, s, *(const SearchParams*)params); break;
+    case kPassMedian3: hipLaunchKernelGGL(median3_kernel, g, dim3(256), 0, s, *(const Median3Params*)params); break;
+    case kPassSubpix: hipLaunchKernelGGL(subpix_kernel, g, dim3(256), 0, s, *(const SubpixParams*)params); break;
+    case kPassPixfield: hipLaunchKernelGGL(pixfield_kernel, g, dim3(256), 0, s, *(const PixParams*)params); break;
+    default: return launchSynthSingle(pass, params, gx, gy, s);
+    }
+    return hipGetLastError();
+}
+
+// batched launches: n work items, one workgroup each
+hipError_t launchBatch(int pass, const void* params, const uint32_t* work, int n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const dim3 g((unsigned)n);
+    switch (pass) {
+    case kPassLuma: hipLaunchKernelGGL(luma_rgba8_batch_kernel, g, dim3(256), 0, s, (const LumaParams*)params, work); break;
+    case kPassDown: hipLaunchKernelGGL(down_batch_kernel, g, dim3(256), 0, s, (const DownParams*)params, work); break;
+    case kPassMedhist: hipLaunchKernelGGL(medhist_batch_kernel, g, dim3(1024), 0, s, (const MedParams*)params, work); break;
+    case kPassSearch2: hipLaunchKernelGGL(search2_batch_kernel, g, dim3(256), 0, s, (const SearchParams*)params, work); break;
+    case kPassSearch6: hipLaunchKernelGGL(search6_batch_kernel, g, dim3(256), 0, s, (const SearchParams*)params, work); break;
+    case kPassSearch12: hipLaunchKernelGGL(search12_batch_kernel, g, dim3(256), 0, s, (const SearchParams*)params, work); break;
+    case kPassMedian3: hipLaunchKernelGGL(median3_batch_kernel, g, dim3(256), 0, s, (const Median3Params*)params, work); break;
+    case kPassSubpix: hipLaunchKernelGGL(subpix_batch_kernel, g, dim3(256), 0, s, (const SubpixParams*)params, work); break;
+    case kPassPixfield: hipLaunchKernelGGL(pixfield_batch_kernel, g, dim3(256), 0, s, (const PixParams*)params, work); break;
+    default: return launchSynthBatch(pass, params, work, n, s);
+    }
     return hipGetLastError();
 }
 

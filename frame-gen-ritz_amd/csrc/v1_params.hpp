@@ -8,11 +8,21 @@ namespace fgr {
 
 constexpr int kBlock = 8;            // interp.py:43 BLOCK
 constexpr int kMaxLevels = 16;
-// u[] counter words (motion_v1.glsl:39 U_UNTRUSTED, U_SADSUM, U_MEDIAN), plus the two per-synth slots of this build.
-constexpr int kUSadSum = 1;
+// u[] counter words. The reference keeps one word per global sum (motion_v1.glsl:39 U_UNTRUSTED, U_SADSUM) and
+// every workgroup adds to it. Device-scope atomics on one address serialise across the chip, so here each sum is
+// spread over kSlots words (a workgroup adds to slot blockId & 63) and its readers add the 64 partial sums (integers:
+// the order does not matter).
+constexpr int kSlots = 64;
 constexpr int kUMedian = 4;          // level i's global median at u[4 + 2i], u[5 + 2i] (float bits)
-constexpr int kUSynth = 40;          // untrusted count of synth k at u[40 + (k & 1)]
-constexpr int kUWords = 64;
+constexpr int kUSad = 64;            // scene-cut SAD partial sums, u[64 .. 127]
+constexpr int kUSynth = 128;         // untrusted partial counts of synth k at u[128 + 64 (k % 16) ..]
+constexpr int kSynthBanks = 16;
+constexpr int kUWords = kUSynth + kSynthBanks * kSlots;
+
+// Batched launches: each workgroup reads its work item, (parameter block index << kWorkShift) | block index within that
+// pass of that pair, and runs the same body as the per-pair kernel.
+constexpr int kWorkShift = 22;
+constexpr uint32_t kWorkMask = (1u << kWorkShift) - 1u;
 
 // Frame formats (fgr.h FGR_FORMAT_*)
 enum Fmt : int { kRgba8 = 0, kRgb10 = 1, kF16 = 2, kP010 = 4 };
@@ -24,6 +34,7 @@ struct FrameRef {
 };
 
 struct LumaParams {
+    int gx;             // blocks per grid row (the batched kernels map a linear block index to (bx, by))
     FrameRef a, c;
     int W, H, S;        // frame size, flow scale
     int fw, fh;         // level-0 size
@@ -37,18 +48,21 @@ struct LumaParams {
 };
 
 struct DownParams {
+    int gx;             // blocks per grid row (the batched kernels map a linear block index to (bx, by))
     const float* sa; const float* sc;   // source level
     float* da; float* dc;               // destination level
     int pw, ph, w, h;
 };
 
 struct MedParams {
+    int gx;             // blocks per grid row (the batched kernels map a linear block index to (bx, by))
     const float2* parent; int pw, ph;   // parent (coarser) block field, median3 output
     int bw, bh;                         // this level's block grid
     uint32_t* u; int medIdx;
 };
 
 struct SearchParams {
+    int gx;             // blocks per grid row (the batched kernels map a linear block index to (bx, by))
     const float* la; const float* lc; int w, h;   // this level's luma
     int bw, bh;
     const float2* parent; int pw, ph;             // null on the coarsest level (seed 0)
@@ -61,10 +75,12 @@ struct SearchParams {
 };
 
 struct Median3Params {
+    int gx;             // blocks per grid row (the batched kernels map a linear block index to (bx, by))
     const float2* in; float2* out; int bw, bh;
 };
 
 struct SubpixParams {
+    int gx;             // blocks per grid row (the batched kernels map a linear block index to (bx, by))
     const float* la; const float* lc; int w, h, bw, bh;
     const float2* in; float2* out;
     float lam;
@@ -72,6 +88,7 @@ struct SubpixParams {
 };
 
 struct PixParams {
+    int gx;             // blocks per grid row (the batched kernels map a linear block index to (bx, by))
     const float* la; const float* lc; int w, h;   // level 0
     const float2* bf; int bw, bh;                 // block field
     float2* pf;                                   // per-pixel field, w x h
@@ -79,17 +96,24 @@ struct PixParams {
 };
 
 struct LookupParams {
+    int gx;             // blocks per grid row (the batched kernels map a linear block index to (bx, by))
     const float2* pf; float2* out; int fw, fh;
     float s;
 };
 
 struct SynthParams {
+    int gx;             // blocks per grid row (the batched kernels map a linear block index to (bx, by))
     FrameRef a, c;
     uint8_t* out; uint8_t* outc; int64_t opitch, ocpitch;
     int W, H, S, fw, fh;
-    const float2* field;      // per-pixel field at time t (flow res)
-    const uint32_t* sadsum;   // u + 1 (the estimate's scene-cut sum)
-    uint32_t* count;          // this synth's untrusted counter
+    const float2* field;      // per-pixel field at time t (flow res): the midpoint field pf, or its lookup at t
+    int lookup;               // 1: field is pf and the kernel looks it up at t itself (lookup.comp), s = 2t - 1
+    float s;
+    int sZero, sOne, sHalf;   // 7x7 sums: trust is 0 for S <= sZero, 1 for S >= sOne, > 0.5 for S > sHalf
+    int quarter;              // 4(1 - t) and 4t are integers: the trusted blend is (kA a + kC c + 2) >> 2 exactly
+    int kA, kC;
+    const uint32_t* sadsum;   // the estimate's scene-cut SAD partial sums (kSlots words)
+    uint32_t* count;          // this synth's untrusted partial counts (kSlots words)
     int nblocks;
     float cutSad, gfrac, trust0, den;
     double rden;              // 1 / (double)den
@@ -101,11 +125,12 @@ struct SynthParams {
 };
 
 struct FallbackParams {
+    int gx;             // blocks per grid row (the batched kernels map a linear block index to (bx, by))
     FrameRef a, c;
     uint8_t* out; uint8_t* outc; int64_t opitch, ocpitch;
     int W, H, bpp;            // bytes per pixel of plane 0
-    const uint32_t* count;
-    uint32_t* nextCount;      // the other synth slot, cleared here for the next synth
+    const uint32_t* count;    // kSlots partial counts
+    uint32_t* nextCount;      // the other synth bank (kSlots words), cleared here for the next synth
     float gfrac;
     int nearC;
 };

@@ -13,69 +13,116 @@
 
 namespace fgr {
 
-constexpr int STW = 32, STH = 16, SAP = 3, SEW = STW + 2 * SAP, SEH = STH + 2 * SAP;
+// Tile of the fused synth kernel: 64 x 32 output pixels per 256-thread workgroup (8 per thread), a 3-pixel apron for
+// the 7x7 box (70 x 38 mismatch values, 1.30 per output pixel).
+constexpr int TW = 64, TH = 32, AP = 3, EW = TW + 2 * AP, EH = TH + 2 * AP, NAPRON = EW * EH - TW * TH;
+
+struct alignas(4) U32x2 {
+    uint32_t x, y;
+};
+
+__device__ __forceinline__ int med3i(int v, int lo, int hi) { return min(max(v, lo), hi); }
+__device__ __forceinline__ int byteOf(uint32_t v, int c) { return (int)((v >> (8 * c)) & 255u); }
 
 __device__ __forceinline__ uint32_t ld32(const FrameRef& f, int x, int y) {
     return *(const uint32_t*)(f.p + (int64_t)y * f.pitch + 4 * (int64_t)x);
 }
 
-__device__ __forceinline__ int byteOf(uint32_t v, int c) { return (int)((v >> (8 * c)) & 255u); }
+// cv2.remap INTER_LINEAR / BORDER_REPLICATE of an 8-bit RGB frame (interp.py:214), at the map position already scaled
+// by 32 (X = 32 px, Y = 32 py, float): map rounded to 1/32 px (half even), integer weights (32-fx)(32-fy), fx(32-fy),
+// (32-fx)fy, fx fy, result (sum + 512) >> 10 -- the exact value rounded half up, as cv2's fixed-point remap. Each tap
+// row is one 8-byte load at the clamped column (W >= 2): the two taps are (x, x+1), or the edge texel twice.
+struct Rgb {
+    uint32_t r, g, b;
+};
 
-// cv2.remap INTER_LINEAR / BORDER_REPLICATE of an 8-bit RGB frame (interp.py:214): map rounded to 1/32 px (half even),
-// integer weights (32-fx)(32-fy), fx(32-fy), (32-fx)fy, fx fy, result (sum + 512) >> 10 -- the exact value rounded half
-// up, as cv2's fixed-point remap. Returns the three warped channels packed as bytes (byte order of the frame).
-__device__ __forceinline__ uint32_t warpRgba8(const FrameRef& f, int W, int H, float px, float py) {
-    const int X = (int)rintf(px * 32.0f), Y = (int)rintf(py * 32.0f);
-    const int ix = X >> 5, iy = Y >> 5, fx = X & 31, fy = Y & 31;
-    const int x0 = clampi(ix, 0, W - 1), x1 = clampi(ix + 1, 0, W - 1);
-    const int y0 = clampi(iy, 0, H - 1), y1 = clampi(iy + 1, 0, H - 1);
-    const uint32_t p00 = ld32(f, x0, y0), p01 = ld32(f, x1, y0), p10 = ld32(f, x0, y1), p11 = ld32(f, x1, y1);
-    const int w00 = (32 - fx) * (32 - fy), w01 = fx * (32 - fy), w10 = (32 - fx) * fy, w11 = fx * fy;
-    uint32_t r = 0u;
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const int v = (w00 * byteOf(p00, c) + w01 * byteOf(p01, c) + w10 * byteOf(p10, c) + w11 * byteOf(p11, c) + 512) >> 10;
-        r |= (uint32_t)v << (8 * c);
-    }
-    return r;
+__device__ __forceinline__ Rgb warpRgba8(const uint8_t* base, uint32_t pitch, int W, int H, float X32, float Y32) {
+    const int X = (int)rintf(X32), Y = (int)rintf(Y32);
+    const int ix = X >> 5, iy = Y >> 5;
+    const uint32_t fx = (uint32_t)(X & 31), fy = (uint32_t)(Y & 31);
+    const uint32_t y0 = (uint32_t)med3i(iy, 0, H - 1), y1 = (uint32_t)med3i(iy + 1, 0, H - 1);
+    const uint32_t xo = 4u * (uint32_t)med3i(ix, 0, W - 2);
+    const U32x2 q0 = *(const U32x2*)(base + (__umul24(y0, pitch) + xo));
+    const U32x2 q1 = *(const U32x2*)(base + (__umul24(y1, pitch) + xo));
+    const bool right = ix >= W - 1, left = ix < 0;
+    const uint32_t p00 = right ? q0.y : q0.x, p01 = left ? q0.x : q0.y;
+    const uint32_t p10 = right ? q1.y : q1.x, p11 = left ? q1.x : q1.y;
+    const uint32_t gx = 32u - fx, gy = 32u - fy;
+    const uint32_t w00 = gx * gy, w01 = fx * gy, w10 = gx * fy, w11 = fx * fy;
+    Rgb o;
+    o.r = (w00 * (p00 & 255u) + w01 * (p01 & 255u) + w10 * (p10 & 255u) + w11 * (p11 & 255u) + 512u) >> 10;
+    o.g = (w00 * ((p00 >> 8) & 255u) + w01 * ((p01 >> 8) & 255u) + w10 * ((p10 >> 8) & 255u) +
+           w11 * ((p11 >> 8) & 255u) + 512u) >> 10;
+    o.b = (w00 * ((p00 >> 16) & 255u) + w01 * ((p01 >> 16) & 255u) + w10 * ((p10 >> 16) & 255u) +
+           w11 * ((p11 >> 16) & 255u) + 512u) >> 10;
+    return o;
 }
 
+__device__ __forceinline__ uint32_t packRgb(const Rgb& v) { return v.r | (v.g << 8) | (v.b << 16); }
+
 // error.comp:9-21 / mismatch() motion_v1.glsl:117-129: e = max-channel |wa - wc| (integers 0..255)
+__device__ __forceinline__ int emaxRgb(const Rgb& a, const Rgb& c) {
+    return (int)max(max(__sad(a.r, c.r, 0u), __sad(a.g, c.g, 0u)), __sad(a.b, c.b, 0u));
+}
+
 __device__ __forceinline__ int emaxRgba8(uint32_t wa, uint32_t wc) {
     const int d0 = abs(byteOf(wa, 0) - byteOf(wc, 0)), d1 = abs(byteOf(wa, 1) - byteOf(wc, 1)),
               d2 = abs(byteOf(wa, 2) - byteOf(wc, 2));
     return max(d0, max(d1, d2));
 }
 
-// the warp pair at full-res pixel (x, y): a at p - 2t h, c at p + 2(1-t) h (float32 product, then the difference)
-__device__ __forceinline__ void warpPairRgba8(const SynthParams& P, int x, int y, uint32_t& wa, uint32_t& wc) {
-    const float2 hv = halfVec(P.field, x, y, P.S, P.fw, P.fh);
+// The warp pair at full-res pixel (x, y) from r, the interpolated field before the x S of halfVec (hv = S r): a at
+// p - 2t hv, c at p + 2(1-t) hv, each the float32 product then the difference (motion_v1.glsl:120-122), then x 32 for
+// the map. Scaling by a power of two commutes with rounding, so 32 (p - ka (S r)) = 32 p - (32 S ka) r exactly; kA32 and
+// kC32 are those folded constants.
+__device__ __forceinline__ void warpPair(const SynthParams& P, float kA32, float kC32, int x, int y, float2 r, Rgb& wa, Rgb& wc) {
+    const float x32 = (float)(32 * x), y32 = (float)(32 * y);
+    wa = warpRgba8(P.a.p, (uint32_t)P.a.pitch, P.W, P.H, x32 - kA32 * r.x, y32 - kA32 * r.y);
+    wc = warpRgba8(P.c.p, (uint32_t)P.c.pitch, P.W, P.H, x32 + kC32 * r.x, y32 + kC32 * r.y);
+}
+
+// the same pair from the plain per-pixel halfVec (debug pass)
+__device__ __forceinline__ void warpPairRgba8(const SynthParams& P, int x, int y, float2 hv, uint32_t& wa, uint32_t& wc) {
     const float ka = 2.0f * P.wC, kc = 2.0f * P.wA;
-    wa = warpRgba8(P.a, P.W, P.H, (float)x - ka * hv.x, (float)y - ka * hv.y);
-    wc = warpRgba8(P.c, P.W, P.H, (float)x + kc * hv.x, (float)y + kc * hv.y);
+    const Rgb a = warpRgba8(P.a.p, (uint32_t)P.a.pitch, P.W, P.H, 32.0f * ((float)x - ka * hv.x), 32.0f * ((float)y - ka * hv.y));
+    const Rgb c = warpRgba8(P.c.p, (uint32_t)P.c.pitch, P.W, P.H, 32.0f * ((float)x + kc * hv.x), 32.0f * ((float)y + kc * hv.y));
+    wa = packRgb(a);
+    wc = packRgb(c);
 }
 
 __device__ __forceinline__ bool sceneCut(const SynthParams& P) {
-    // fallback.comp:10-11: float(sum) * (1/256) / float(blocks) > sceneCutSad
-    const float meanSad = (float)(*P.sadsum) * (1.0f / 256.0f) / (float)P.nblocks;
+    // fallback.comp:10-11: float(sum) * (1/256) / float(blocks) > sceneCutSad (the wavefront adds the partial sums)
+    const float meanSad = (float)slotSum(P.sadsum) * (1.0f / 256.0f) / (float)P.nblocks;
     return meanSad > P.cutSad;
 }
 
 // interp.py:236-237 trust from the exact 7x7 sum S: e7 = float32(S * (1/49)), u = clip((e7 - lo) / (hi - lo), 0, 1).
 // The float32 quotient is taken as float(double(e7 - lo) * (1 / double(hi - lo))): with a 24-bit numerator and
 // denominator the quotient is never a rounding midpoint unless the denominator is a power of two (then both are
-// exact), and the double product is close enough to round the same way.
+// exact), and the double product is close enough to round the same way. u is monotone in S, so the host tabulates
+// where it is 0 and 1 and the kernel evaluates the ramp only in between.
 __device__ __forceinline__ float trustOf(const SynthParams& P, int S) {
+    if (S <= P.sZero) return 0.0f;
+    if (S >= P.sOne) return 1.0f;
     const float e7 = (float)((double)S * (1.0 / 49.0));
     const float u = (float)((double)(e7 - P.trust0) * P.rden);
     return fminf(fmaxf(u, 0.0f), 1.0f);
 }
 
 // resolve.comp:73-82 (interp.py:241-246): m = 0.5 (wa + wc) at t = 0.5, else (1-t) wa + t wc; o = m (1 - u) + near u;
-// out = clamp(floor(o + 0.5), 0, 255); alpha from the nearer frame
+// out = clamp(floor(o + 0.5), 0, 255); alpha from the nearer frame. Two exact shortcuts of the same float32 formula:
+// u = 1 gives o = near; u = 0 gives o = m, and when 4(1-t) and 4t are integers m + 0.5 is exact, so
+// floor(m + 0.5) = (kA a + kC c + 2) >> 2 with kA = 4(1-t), kC = 4t.
 __device__ __forceinline__ uint32_t blendRgba8(const SynthParams& P, uint32_t wa, uint32_t wc, uint32_t nr, float u) {
-    const float omu = 1.0f - u;
+    if (u == 1.0f) return nr;
     uint32_t r = nr & 0xFF000000u;
+    if (u == 0.0f && P.quarter) {
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+            r |= (uint32_t)((P.kA * byteOf(wa, c) + P.kC * byteOf(wc, c) + 2) >> 2) << (8 * c);
+        return r;
+    }
+    const float omu = 1.0f - u;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         const int a8 = byteOf(wa, c), c8 = byteOf(wc, c), n8 = byteOf(nr, c);
@@ -83,88 +130,166 @@ __device__ __forceinline__ uint32_t blendRgba8(const SynthParams& P, uint32_t wa
         if (P.mid) m = 0.5f * (float)(a8 + c8);
         else m = P.wA * (float)a8 + P.wC * (float)c8;
         const float o = m * omu + (float)n8 * u;
-        const int v = clampi((int)floorf(o + 0.5f), 0, 255);
+        const int v = med3i((int)floorf(o + 0.5f), 0, 255);
         r |= (uint32_t)v << (8 * c);
     }
     return r;
 }
 
-__global__ __launch_bounds__(256) void synth_rgba8_kernel(SynthParams P) {
-    __shared__ int es[SEH * SEW];
-    __shared__ int hs[SEH * STW];
+// lookup.comp:10-19 (interp.py:194-205): the field at time t at flow pixel (qx, qy): g <- pf(q - s g) twice
+__device__ __forceinline__ float2 lookupAt(const float2* pf, int fw, int fh, float s, int qx, int qy) {
+    float2 g = pf[qy * fw + qx];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const float px = (float)qx - s * g.x, py = (float)qy - s * g.y;
+        g = sampleF(pf, px, py, fw, fh);
+    }
+    return g;
+}
+
+// Flow column of full-res pixel x: floor((x + 0.5) / S - 0.5) = floor((2x + 1 - S) / 2S), and its weight
+template <int S>
+struct FlowMap {
+    static constexpr int kShift = S == 1 ? 1 : (S == 2 ? 2 : 3);
+    static __device__ __forceinline__ int index(int x) { return (2 * x + 1 - S) >> kShift; }
+    static __device__ __forceinline__ float frac(int x) {
+        return (float)((2 * x + 1 - S) & (2 * S - 1)) * (1.0f / (float)(2 * S));
+    }
+};
+
+template <int S>
+struct SynthTile {
+    // field tile (flow res) covering the tile and its apron, and its rows interpolated to full-res columns
+    static constexpr int FTW = (EW + S - 1) / S + 3, FTH = (EH + S - 1) / S + 3;
+};
+
+// One 64 x 32 tile of the synthesised frame (passes 9-11 fused): field tile (+ lookup at t), mismatch of the tile and
+// its apron, 7x7 box, trust, untrusted count, blend. Scene cut: a copy of the nearer frame.
+//
+// halfVec() motion_v1.glsl:73-84 is split in its two stages: the horizontal mixes of each field row at every full-res
+// column of the tile (row[j][x] = mix(F[j][i0(x)], F[j][i1(x)], fx(x))), then per pixel the vertical mix of two of
+// those rows; the operations and their order are the shader's, so the result is the same float.
+template <int S>
+__device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, int tileY, int nTilesX) {
+    constexpr int FTW = SynthTile<S>::FTW, FTH = SynthTile<S>::FTH;
+    __shared__ float2 ft[FTW * FTH];
+    __shared__ float2 hrow[FTH * EW];
+    __shared__ int es[EH * EW];
+    __shared__ uint16_t hs[EH * TW];
     __shared__ uint32_t cnt;
     const int tid = threadIdx.x;
-    const int x0 = blockIdx.x * STW, y0 = blockIdx.y * STH;
+    const int x0 = tileX * TW, y0 = tileY * TH;
     const FrameRef& nf = P.nearC ? P.c : P.a;
+    const int cx = tid & (TW - 1), ry = tid >> 6;   // this thread's output column and first row (rows ry + 4k)
     if (sceneCut(P)) {   // uniform over the grid: the whole output is the nearer real frame (interp.py:225-226)
-        for (int i = tid; i < STW * STH; i += 256) {
-            const int x = x0 + i % STW, y = y0 + i / STW;
+        for (int k = 0; k < TH / 4; k++) {
+            const int x = x0 + cx, y = y0 + ry + 4 * k;
             if (x < P.W && y < P.H) *(uint32_t*)(P.out + (int64_t)y * P.opitch + 4 * (int64_t)x) = ld32(nf, x, y);
         }
         return;
     }
     if (tid == 0) cnt = 0u;
-    const int ix = tid % STW, iy = tid / STW;
-    uint32_t wa[2], wc[2];
-#pragma unroll
-    for (int r = 0; r < 2; r++) {
-        const int x = min(x0 + ix, P.W - 1), y = min(y0 + iy + 8 * r, P.H - 1);
-        warpPairRgba8(P, x, y, wa[r], wc[r]);
-        es[(iy + 8 * r + SAP) * SEW + ix + SAP] = emaxRgba8(wa[r], wc[r]);
-    }
-    for (int i = tid; i < SEH * SEW; i += 256) {
-        const int ex = i % SEW, ey = i / SEW;
-        if (ex >= SAP && ex < SAP + STW && ey >= SAP && ey < SAP + STH) continue;
-        const int x = clampi(x0 - SAP + ex, 0, P.W - 1), y = clampi(y0 - SAP + ey, 0, P.H - 1);
-        uint32_t a, c;
-        warpPairRgba8(P, x, y, a, c);
-        es[i] = emaxRgba8(a, c);
+    // the field tile covering every pixel of the tile and its apron (clamped to the frame, then to S * fw - 1)
+    const int xlim = S * P.fw - 1, ylim = S * P.fh - 1;
+    const int xmin = min(max(x0 - AP, 0), xlim), ymin = min(max(y0 - AP, 0), ylim);
+    const int tx0 = FlowMap<S>::index(xmin), ty0 = FlowMap<S>::index(ymin);
+    for (int i = tid; i < FTW * FTH; i += 256) {
+        const int qx = med3i(tx0 + i % FTW, 0, P.fw - 1), qy = med3i(ty0 + i / FTW, 0, P.fh - 1);
+        ft[i] = P.lookup ? lookupAt(P.field, P.fw, P.fh, P.s, qx, qy) : P.field[qy * P.fw + qx];
     }
     __syncthreads();
-    for (int i = tid; i < SEH * STW; i += 256) {
-        const int r = i / STW, c = i % STW;
-        const int* e = &es[r * SEW + c];
-        hs[i] = e[0] + e[1] + e[2] + e[3] + e[4] + e[5] + e[6];
+    // horizontal mixes at the full-res columns x0 - AP .. x0 + TW + AP - 1 (clamped as the pixels are)
+    for (int i = tid; i < FTH * EW; i += 256) {
+        const int j = i / EW, ex = i - j * EW;
+        const int x = min(med3i(x0 - AP + ex, 0, P.W - 1), xlim);
+        const float fx = FlowMap<S>::frac(x);
+        const float2* f = ft + j * FTW + (FlowMap<S>::index(x) - tx0);
+        hrow[i] = mix2(f[0], f[1], fx);
     }
     __syncthreads();
+    const float kA32 = (float)(32 * S) * (2.0f * P.wC), kC32 = (float)(32 * S) * (2.0f * P.wA);
+    // the interpolated field r (hv = S r) at region pixel (ex, ey) with frame pixel y
+    auto fieldAtRegion = [&](int ex, int y) {
+        const int yc = min(y, ylim);
+        const float2* h = hrow + (FlowMap<S>::index(yc) - ty0) * EW + ex;
+        return mix2(h[0], h[EW], FlowMap<S>::frac(yc));
+    };
+    // interior pixels: warps kept in registers (packed) for the blend
+    uint32_t wa[TH / 4], wc[TH / 4];
 #pragma unroll
-    for (int r = 0; r < 2; r++) {
-        const int x = x0 + ix, y = y0 + iy + 8 * r;
+    for (int k = 0; k < TH / 4; k++) {
+        const int x = min(x0 + cx, P.W - 1), y = min(y0 + ry + 4 * k, P.H - 1);
+        Rgb a, c;
+        warpPair(P, kA32, kC32, x, y, fieldAtRegion(cx + AP, y), a, c);
+        es[(ry + 4 * k + AP) * EW + cx + AP] = emaxRgb(a, c);
+        wa[k] = packRgb(a);
+        wc[k] = packRgb(c);
+    }
+    // apron: rows 0..2 and EH-3..EH-1 in full, then columns 0..2 and EW-3..EW-1 of the rows between
+    for (int j = tid; j < NAPRON; j += 256) {
+        int ex, ey;
+        if (j < 6 * EW) {
+            const int r = j / EW;
+            ey = r < 3 ? r : r + TH;
+            ex = j - r * EW;
+        } else {
+            const int jj = j - 6 * EW, q = jj / 6, c = jj - 6 * q;
+            ey = AP + q;
+            ex = c < 3 ? c : TW + c;
+        }
+        const int x = med3i(x0 - AP + ex, 0, P.W - 1), y = med3i(y0 - AP + ey, 0, P.H - 1);
+        Rgb a, c;
+        warpPair(P, kA32, kC32, x, y, fieldAtRegion(ex, y), a, c);
+        es[ey * EW + ex] = emaxRgb(a, c);
+    }
+    __syncthreads();
+    for (int i = tid; i < EH * TW; i += 256) {
+        const int r = i >> 6, c = i & (TW - 1);
+        const int* e = &es[r * EW + c];
+        hs[i] = (uint16_t)(e[0] + e[1] + e[2] + e[3] + e[4] + e[5] + e[6]);
+    }
+    __syncthreads();
+    uint32_t mine = 0u;
+#pragma unroll
+    for (int k = 0; k < TH / 4; k++) {
+        const int x = x0 + cx, y = y0 + ry + 4 * k;
         const bool inside = x < P.W && y < P.H;
-        const int* h = &hs[(iy + 8 * r) * STW + ix];
-        const int S = h[0] + h[STW] + h[2 * STW] + h[3 * STW] + h[4 * STW] + h[5 * STW] + h[6 * STW];
-        const float u = trustOf(P, S);
-        const uint64_t b = __ballot(inside && u > 0.5f);
-        if ((tid & 63) == 0 && b) atomicAdd(&cnt, (uint32_t)__popcll(b));
+        const uint16_t* h = &hs[(ry + 4 * k) * TW + cx];
+        const int S49 = (int)h[0] + h[TW] + h[2 * TW] + h[3 * TW] + h[4 * TW] + h[5 * TW] + h[6 * TW];
         if (inside) {
+            mine += S49 > P.sHalf ? 1u : 0u;
             const uint32_t nr = ld32(nf, x, y);
-            *(uint32_t*)(P.out + (int64_t)y * P.opitch + 4 * (int64_t)x) = blendRgba8(P, wa[r], wc[r], nr, u);
+            *(uint32_t*)(P.out + (int64_t)y * P.opitch + 4 * (int64_t)x) = blendRgba8(P, wa[k], wc[k], nr, trustOf(P, S49));
         }
     }
+    // untrusted count: a wavefront sum, one LDS atomic per wavefront, one global atomic per workgroup into its slot
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) mine += __shfl_xor(mine, o, 64);
+    if ((tid & 63) == 0 && mine) atomicAdd(&cnt, mine);
     __syncthreads();
-    if (tid == 0 && cnt) atomicAdd(P.count, cnt);
+    if (tid == 0 && cnt) atomicAdd(P.count + ((tileY * nTilesX + tileX) & (kSlots - 1)), cnt);
+}
+
+template <int S>
+__device__ __forceinline__ void synthRgba8Body(const SynthParams& P, int bx, int by) {
+    synthRgba8Tile<S>(P, bx, by, P.gx);
 }
 
 // lookup.comp:10-19 (interp.py:194-205): the per-pixel field at time t, g <- pf(q - s g) twice (LOOKUP_ITERS = 2)
 __global__ __launch_bounds__(256) void lookup_kernel(LookupParams P) {
-    const int qx = blockIdx.x * 16 + threadIdx.x, qy = blockIdx.y * 16 + threadIdx.y;
+    const int qx = blockIdx.x * 16 + (threadIdx.x & 15), qy = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (qx >= P.fw || qy >= P.fh) return;
-    float2 g = P.pf[qy * P.fw + qx];
-    for (int k = 0; k < 2; k++) {
-        const float px = (float)qx - P.s * g.x, py = (float)qy - P.s * g.y;
-        g = sampleF(P.pf, px, py, P.fw, P.fh);
-    }
-    P.out[qy * P.fw + qx] = g;
+    P.out[qy * P.fw + qx] = lookupAt(P.pf, P.fw, P.fh, P.s, qx, qy);
 }
 
 // fallback.comp:9-24: more than globalFallback x W x H untrusted pixels -> the nearer real frame. Every workgroup
 // exits at once in the normal case; 64x64 pixels per workgroup.
-__global__ __launch_bounds__(256) void fallback_kernel(FallbackParams P) {
-    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *P.nextCount = 0u;
-    const bool flick = (float)(*P.count) > P.gfrac * (float)P.W * (float)P.H;
+__device__ __forceinline__ void fallbackBody(const FallbackParams& P, int gbx, int gby) {
+    if (gbx == 0 && gby == 0 && threadIdx.x < kSlots) P.nextCount[threadIdx.x] = 0u;
+    const bool flick = (float)slotSum(P.count) > P.gfrac * (float)P.W * (float)P.H;
     if (!flick) return;
     const FrameRef& nf = P.nearC ? P.c : P.a;
-    const int bx = blockIdx.x * 64, by = blockIdx.y * 64;
+    const int bx = gbx * 64, by = gby * 64;
     for (int i = threadIdx.x; i < 64 * 64; i += 256) {
         const int x = bx + (i & 63), y = by + (i >> 6);
         if (x >= P.W || y >= P.H) continue;
@@ -183,29 +308,70 @@ __global__ __launch_bounds__(256) void fallback_kernel(FallbackParams P) {
 
 // error.comp:9-21 as a debug pass: the packed mismatch e of the latest synth (4 pixels per uint32, low byte first)
 __global__ __launch_bounds__(256) void debug_e_rgba8_kernel(SynthParams P, uint32_t* dst, int eStride) {
-    const int gx = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+    const int gx = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
     if (gx >= eStride || y >= P.H) return;
     uint32_t packed = 0u;
     for (int i = 0; i < 4; i++) {
         const int x = gx * 4 + i;
         if (x < P.W) {
             uint32_t a, c;
-            warpPairRgba8(P, x, y, a, c);
+            warpPairRgba8(P, x, y, halfVec(P.field, x, y, P.S, P.fw, P.fh), a, c);
             packed |= (uint32_t)emaxRgba8(a, c) << (8 * i);
         }
     }
     dst[(int64_t)y * eStride + gx] = packed;
 }
 
-hipError_t launchLookup(const LookupParams& p, hipStream_t s) {
-    hipLaunchKernelGGL(lookup_kernel, dim3((unsigned)((p.fw + 15) / 16), (unsigned)((p.fh + 15) / 16)), dim3(16, 16), 0, s, p);
+// [untrusted count of a synth, scene-cut SAD sum] from their partial sums (framegen.cpp:2073-2084)
+__global__ __launch_bounds__(64) void debug_counters_kernel(const uint32_t* count, const uint32_t* sad, uint32_t* dst) {
+    const uint32_t c = slotSum(count), s = slotSum(sad);
+    if (threadIdx.x == 0) {
+        dst[0] = c;
+        dst[1] = s;
+    }
+}
+
+hipError_t launchDebugCounters(const uint32_t* count, const uint32_t* sad, uint32_t* dst, hipStream_t s) {
+    hipLaunchKernelGGL(debug_counters_kernel, dim3(1), dim3(64), 0, s, count, sad, dst);
     return hipGetLastError();
 }
 
+#define FGR_PASS(name, Params, body, threads)                                                                    \
+    __global__ __launch_bounds__(threads) void name##_kernel(Params P) { body(P, (int)blockIdx.x, (int)blockIdx.y); } \
+    __global__ __launch_bounds__(threads) void name##_batch_kernel(const Params* __restrict__ P,                  \
+                                                                   const uint32_t* __restrict__ work) {          \
+        const uint32_t w = work[blockIdx.x];                                                                     \
+        const Params p = P[w >> kWorkShift];                                                                     \
+        const int l = (int)(w & kWorkMask);                                                                      \
+        body(p, l % p.gx, l / p.gx);                                                                             \
+    }
+
+FGR_PASS(synth_rgba8_s1, SynthParams, synthRgba8Body<1>, 256)
+FGR_PASS(synth_rgba8_s2, SynthParams, synthRgba8Body<2>, 256)
+FGR_PASS(synth_rgba8_s4, SynthParams, synthRgba8Body<4>, 256)
+FGR_PASS(fallback, FallbackParams, fallbackBody, 256)
+
+static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+hipError_t launchLookup(const LookupParams& p, hipStream_t s) {
+    hipLaunchKernelGGL(lookup_kernel, dim3((unsigned)cdiv(p.fw, 16), (unsigned)cdiv(p.fh, 16)), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+int synthTilesX(int W) { return cdiv(W, TW); }
+int synthTilesY(int H) { return cdiv(H, TH); }
+int fallbackTilesX(int W) { return cdiv(W, 64); }
+int fallbackTilesY(int H) { return cdiv(H, 64); }
+
 hipError_t launchSynth(const SynthParams& p, int format, hipStream_t s) {
-    const dim3 g((unsigned)((p.W + STW - 1) / STW), (unsigned)((p.H + STH - 1) / STH));
-    switch (format) {
-    case kRgba8: hipLaunchKernelGGL(synth_rgba8_kernel, g, dim3(256), 0, s, p); break;
+    SynthParams q = p;
+    q.gx = synthTilesX(p.W);
+    const dim3 g((unsigned)q.gx, (unsigned)synthTilesY(p.H));
+    if (format != kRgba8) return hipErrorInvalidValue;
+    switch (p.S) {
+    case 1: hipLaunchKernelGGL(synth_rgba8_s1_kernel, g, dim3(256), 0, s, q); break;
+    case 2: hipLaunchKernelGGL(synth_rgba8_s2_kernel, g, dim3(256), 0, s, q); break;
+    case 4: hipLaunchKernelGGL(synth_rgba8_s4_kernel, g, dim3(256), 0, s, q); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -213,14 +379,40 @@ hipError_t launchSynth(const SynthParams& p, int format, hipStream_t s) {
 
 hipError_t launchFallback(const FallbackParams& p, int format, hipStream_t s) {
     (void)format;
-    hipLaunchKernelGGL(fallback_kernel, dim3((unsigned)((p.W + 63) / 64), (unsigned)((p.H + 63) / 64)), dim3(256), 0, s, p);
+    FallbackParams q = p;
+    q.gx = fallbackTilesX(p.W);
+    hipLaunchKernelGGL(fallback_kernel, dim3((unsigned)q.gx, (unsigned)fallbackTilesY(p.H)), dim3(256), 0, s, q);
+    return hipGetLastError();
+}
+
+hipError_t launchSynthSingle(int pass, const void* params, int gx, int gy, hipStream_t s) {
+    const dim3 g((unsigned)gx, (unsigned)gy);
+    switch (pass) {
+    case kPassSynthRgba8S1: hipLaunchKernelGGL(synth_rgba8_s1_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
+    case kPassSynthRgba8S2: hipLaunchKernelGGL(synth_rgba8_s2_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
+    case kPassSynthRgba8S4: hipLaunchKernelGGL(synth_rgba8_s4_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
+    case kPassFallback: hipLaunchKernelGGL(fallback_kernel, g, dim3(256), 0, s, *(const FallbackParams*)params); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launchSynthBatch(int pass, const void* params, const uint32_t* work, int n, hipStream_t s) {
+    const dim3 g((unsigned)n);
+    switch (pass) {
+    case kPassSynthRgba8S1: hipLaunchKernelGGL(synth_rgba8_s1_batch_kernel, g, dim3(256), 0, s, (const SynthParams*)params, work); break;
+    case kPassSynthRgba8S2: hipLaunchKernelGGL(synth_rgba8_s2_batch_kernel, g, dim3(256), 0, s, (const SynthParams*)params, work); break;
+    case kPassSynthRgba8S4: hipLaunchKernelGGL(synth_rgba8_s4_batch_kernel, g, dim3(256), 0, s, (const SynthParams*)params, work); break;
+    case kPassFallback: hipLaunchKernelGGL(fallback_batch_kernel, g, dim3(256), 0, s, (const FallbackParams*)params, work); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
 hipError_t launchDebugE(const SynthParams& p, uint32_t* dst, int eStride, int format, hipStream_t s) {
-    const dim3 g((unsigned)((eStride + 15) / 16), (unsigned)((p.H + 15) / 16));
+    const dim3 g((unsigned)cdiv(eStride, 16), (unsigned)cdiv(p.H, 16));
     switch (format) {
-    case kRgba8: hipLaunchKernelGGL(debug_e_rgba8_kernel, g, dim3(16, 16), 0, s, p, dst, eStride); break;
+    case kRgba8: hipLaunchKernelGGL(debug_e_rgba8_kernel, g, dim3(256), 0, s, p, dst, eStride); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

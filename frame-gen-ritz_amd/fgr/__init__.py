@@ -11,7 +11,7 @@ it raises ImportError: there is no CPU fallback.
 import ctypes
 import os
 
-__all__ = ["Settings", "Interpolator", "lib", "default_settings", "validate", "settings_error_for_format",
+__all__ = ["Settings", "Interpolator", "Batch", "DebugInternal", "lib", "default_settings", "validate", "settings_error_for_format",
            "size_layout", "FORMAT_RGBA8", "FORMAT_RGB10", "FORMAT_RGBAF16", "FORMAT_NV12", "FORMAT_P010",
            "MIN_SIZE", "SET_RING", "frame_of"]
 
@@ -69,6 +69,17 @@ class DebugInternal(ctypes.Structure):
                                               "lastSynthSlot")]
 
 
+class BatchPair(ctypes.Structure):
+    _fields_ = [("prev", Frame), ("curr", Frame), ("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("format", ctypes.c_uint32), ("multiplier", ctypes.c_uint32), ("out", Frame * 15)]
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("bgra", ctypes.c_uint32), ("transfer", ctypes.c_uint32),
+                ("linearWhiteNits", ctypes.c_float), ("yuvMatrix", ctypes.c_uint32),
+                ("yuvFullRange", ctypes.c_uint32), ("useGraph", ctypes.c_uint32), ("settings", Settings)]
+
+
 if not os.path.exists(_LIB_PATH):
     raise ImportError(f"libfgr.so is not built ({_LIB_PATH}); run __graft_entry__.build()")
 lib = ctypes.CDLL(_LIB_PATH)
@@ -105,6 +116,15 @@ _sig = {
     "fgr_debug_map_layout": (ctypes.c_int, [_H, ctypes.POINTER(DebugMapLayout)]),
     "fgr_record_debug_maps": (ctypes.c_int, [_H, ctypes.c_void_p, ctypes.c_void_p]),
     "fgr_debug_internal": (ctypes.c_int, [_H, ctypes.POINTER(DebugInternal)]),
+    "fgr_batch_create": (_H, [ctypes.POINTER(BatchInfo), ctypes.POINTER(BatchPair), ctypes.c_uint32]),
+    "fgr_batch_last_error": (ctypes.c_char_p, []),
+    "fgr_batch_destroy": (None, [_H]),
+    "fgr_batch_set_frames": (ctypes.c_int, [_H, ctypes.c_void_p, ctypes.POINTER(BatchPair), ctypes.c_uint32]),
+    "fgr_batch_record": (ctypes.c_int, [_H, ctypes.c_void_p]),
+    "fgr_batch_launches": (ctypes.c_uint32, [_H]),
+    "fgr_batch_debug_counters": (ctypes.c_int, [_H, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
+                                                ctypes.c_void_p]),
+    "fgr_batch_debug_internal": (ctypes.c_int, [_H, ctypes.c_uint32, ctypes.POINTER(DebugInternal)]),
 }
 for _name, (_res, _args) in _sig.items():
     _f = getattr(lib, _name)
@@ -269,3 +289,69 @@ class Interpolator:
 
     def record_debug_maps(self, dst, stream=None):
         return bool(lib.fgr_record_debug_maps(self._h, _stream(stream), ctypes.c_void_p(dst.data_ptr())))
+
+
+def _batch_pairs(items):
+    """items: (prev, curr, multiplier, [out, ...]) with torch tensors (or (tensor, chroma) tuples), optional 5th
+    element: format."""
+    arr = (BatchPair * len(items))()
+    for i, it in enumerate(items):
+        prev, curr, n, outs = it[:4]
+        fmt = it[4] if len(it) > 4 else FORMAT_RGBA8
+        bp = arr[i]
+        bp.prev, bp.curr = _frame(prev), _frame(curr)
+        t = prev[0] if isinstance(prev, tuple) else prev
+        bp.height, bp.width = int(t.shape[0]), int(t.shape[1])
+        bp.format, bp.multiplier = fmt, n
+        if len(outs) != n - 1:
+            raise ValueError("a pair needs multiplier - 1 outputs")
+        for k, o in enumerate(outs):
+            bp.out[k] = _frame(o)
+    return arr
+
+
+class Batch:
+    """Batched V1 over the C ABI (fgr_batch_*): many pairs, one launch per pass, replayed as a hipGraph."""
+
+    def __init__(self, items, settings=None, bgra=False, use_graph=True, device=0, transfer=TRANSFER_ENCODED,
+                 linear_white_nits=80.0, yuv_matrix=YUV_BT709, yuv_full_range=False):
+        info = BatchInfo()
+        info.device, info.bgra, info.useGraph = device, int(bgra), int(use_graph)
+        info.transfer, info.linearWhiteNits = transfer, linear_white_nits
+        info.yuvMatrix, info.yuvFullRange = yuv_matrix, int(yuv_full_range)
+        info.settings = settings if settings is not None else default_settings()
+        self._keep = items
+        arr = _batch_pairs(items)
+        self._h = lib.fgr_batch_create(ctypes.byref(info), arr, len(items))
+        if not self._h:
+            raise RuntimeError(_s(lib.fgr_batch_last_error()))
+
+    def close(self):
+        if self._h:
+            lib.fgr_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, stream=None):
+        if not lib.fgr_batch_record(self._h, _stream(stream)):
+            raise RuntimeError("fgr_batch_record failed")
+
+    def set_frames(self, items, stream=None):
+        self._keep = items
+        return bool(lib.fgr_batch_set_frames(self._h, _stream(stream), _batch_pairs(items), len(items)))
+
+    def launches(self):
+        return lib.fgr_batch_launches(self._h)
+
+    def debug_counters(self, pair, synth, dst, stream=None):
+        return bool(lib.fgr_batch_debug_counters(self._h, _stream(stream), pair, synth,
+                                                 ctypes.c_void_p(dst.data_ptr())))
+
+    def debug_internal(self, pair):
+        d = DebugInternal()
+        return d if lib.fgr_batch_debug_internal(self._h, pair, ctypes.byref(d)) else None

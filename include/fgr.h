@@ -140,6 +140,38 @@ int fgr_debug_map_layout(const FgrInterpolator* h, FgrDebugMapLayout* out);
  * latest synth (W x H bytes, eStride words per row, low byte first) into dst (device memory of layout.size bytes). */
 int fgr_record_debug_maps(FgrInterpolator* h, void* stream, void* dst);
 
+/* ---- batched V1 (this library's addition; no counterpart in the reference) ----
+ * Many pairs of any sizes, each with its own multiplier N (outputs at t = k/N, k = 1..N-1), processed as one launch
+ * per pass for the whole batch: the estimate of every pair, then every pair's k-th synth. The frames are bound at
+ * create time (fgr_batch_set_frames rebinds pointers of the same sizes); a record enqueues the whole batch on a stream,
+ * captured once into a hipGraph and replayed when useGraph is set. Results are bit-identical to fgr_record_estimate +
+ * fgr_record_synth per pair. Not thread-safe; one stream, like a handle. */
+typedef struct FgrBatchPair {
+    FgrFrame prev, curr;
+    uint32_t width, height, format;
+    uint32_t multiplier;       /* N in [2, 16] */
+    FgrFrame out[15];          /* the N - 1 outputs */
+} FgrBatchPair;
+
+typedef struct FgrBatchInfo {
+    int32_t device;
+    uint32_t bgra, transfer;   /* as FgrInitInfo, for every pair */
+    float linearWhiteNits;
+    uint32_t yuvMatrix, yuvFullRange;
+    uint32_t useGraph;         /* 1: capture the batch into a hipGraph on the first record, replay it after */
+    FgrSettings settings;
+} FgrBatchInfo;
+
+typedef struct FgrBatch FgrBatch;
+FgrBatch* fgr_batch_create(const FgrBatchInfo* info, const FgrBatchPair* pairs, uint32_t n);
+const char* fgr_batch_last_error(void);
+void fgr_batch_destroy(FgrBatch* b);
+int fgr_batch_set_frames(FgrBatch* b, void* stream, const FgrBatchPair* pairs, uint32_t n);
+int fgr_batch_record(FgrBatch* b, void* stream);
+uint32_t fgr_batch_launches(const FgrBatch* b);
+/* [untrusted pixels of synth `synth` of pair `pair`, the pair's scene-cut SAD sum x 256] into dst (device) */
+int fgr_batch_debug_counters(FgrBatch* b, void* stream, uint32_t pair, uint32_t synth, void* dst);
+
 /* ---- test / debug only; no counterpart in the reference ----
  * Device pointers of the handle's internal buffers and their layout (float / float2 offsets), so a test can read every
  * intermediate of the latest estimate (luma pyramid, per-level raw and median-filtered block fields, medians, the
@@ -147,12 +179,14 @@ int fgr_record_debug_maps(FgrInterpolator* h, void* stream, void* dst);
 typedef struct FgrDebugInternal {
     const void* luma;       /* float */
     const void* field;      /* float2 */
-    const void* counters;   /* uint32[64]: [1] SAD sum, [4 + 2i] level i median (float bits), [40 + k&1] synth k count */
+    const void* counters;   /* uint32[1152]: [4 + 2i] level i median (float bits), [64..127] partial sums of the scene-cut
+                               SAD, [128 + 64 (k % 16) ..] partial untrusted counts of the k-th synth of the estimate */
     int32_t levels, fw, fh;
     int32_t w[16], h[16], bw[16], bh[16], lumaOffA[16], lumaOffC[16], fS[16], fM[16];
     int32_t fieldSub, fieldPF, fieldT, lumaWords, fieldEntries, lastSynthSlot;
 } FgrDebugInternal;
 int fgr_debug_internal(const FgrInterpolator* h, FgrDebugInternal* out);
+int fgr_batch_debug_internal(const FgrBatch* b, uint32_t pair, FgrDebugInternal* out);
 
 #ifdef __cplusplus
 }

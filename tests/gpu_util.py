@@ -4,6 +4,8 @@ import ctypes
 import numpy as np
 import torch
 
+import fgr
+
 _hip = None
 
 
@@ -33,12 +35,12 @@ class Internals:
     """The latest estimate's intermediates, read through fgr_debug_internal."""
 
     def __init__(self, fi):
-        d = fi.debug_internal()
+        d = fi if isinstance(fi, fgr.DebugInternal) else fi.debug_internal()
         assert d is not None
         self.d = d
         self.luma = read_device(d.luma, d.lumaWords, np.float32)
         self.field = read_device(d.field, 2 * d.fieldEntries, np.float32).reshape(-1, 2)
-        self.u = read_device(d.counters, 64, np.uint32)
+        self.u = read_device(d.counters, 1152, np.uint32)
 
     def luma_level(self, i, which):
         d = self.d
@@ -61,6 +63,9 @@ class Internals:
     def pf(self):
         d = self.d
         return self.field[d.fieldPF:d.fieldPF + d.fw * d.fh].reshape(d.fh, d.fw, 2)
+
+    def sadsum(self):
+        return int(self.u[64:128].astype(np.uint64).sum()) & 0xFFFFFFFF
 
     def median(self, i):
         return self.u[4 + 2 * i:6 + 2 * i].view(np.float32)

@@ -1,0 +1,127 @@
+"""Batched V1 (fgr_batch_*): one launch per pass over many pairs of mixed sizes and multipliers, replayed as a
+hipGraph. Bar: every output bit-identical to the per-pair path (itself bit-exact against the oracle), the estimate's
+intermediates bit-exact against the oracle for pairs of every size, and each graph replay identical to the eager run.
+"""
+import numpy as np
+import pytest
+import torch
+
+import fgr
+from fgr import scenes
+from gpu_util import Internals, dev
+from oracle import v1
+from test_gpu_v1 import check_estimate, check_output
+
+pytestmark = pytest.mark.gpu
+
+
+def per_pair(a, c, n):
+    H, W = a.shape[:2]
+    fi = fgr.Interpolator(W, H)
+    ta, tc = dev(a), dev(c)
+    assert fi.record_estimate(ta, tc)
+    outs = []
+    for k in range(1, n):
+        o = torch.zeros_like(ta)
+        assert fi.record_synth(ta, tc, o, k / n)
+        outs.append(o.cpu().numpy())
+    fi.close()
+    return outs
+
+
+def make_batch(pairs, mults, use_graph=True):
+    items, frames = [], []
+    for (a, c), n in zip(pairs, mults):
+        ta, tc = dev(a), dev(c)
+        outs = [torch.zeros_like(ta) for _ in range(n - 1)]
+        items.append((ta, tc, n, outs))
+    return fgr.Batch(items, use_graph=use_graph), items
+
+
+def mixed_pairs():
+    sizes = [(320, 180), (203, 117), (640, 360), (256, 144), (402, 226)]
+    pairs = []
+    for i, (w, h) in enumerate(sizes):
+        pairs.append(scenes.pair_translate(30 + i, w=w, h=h, sprite=min(48, h // 3), sprite_pos=(w // 3, h // 3),
+                                           shift=(3 + i, -2 + i % 3)))
+    pairs.append(scenes.noise_pair(40, 288, 160))         # a scene cut inside the batch
+    return pairs, [2, 3, 4, 4, 3, 4]
+
+
+@pytest.mark.parametrize("use_graph", [False, True])
+def test_mixed_batch_matches_per_pair_and_oracle(use_graph):
+    pairs, mults = mixed_pairs()
+    b, items = make_batch(pairs, mults, use_graph)
+    b.run()
+    torch.cuda.synchronize()
+    cnt = torch.zeros(2, dtype=torch.int32, device="cuda:0")
+    for p, ((a, c), n) in enumerate(zip(pairs, mults)):
+        got = [o.cpu().numpy() for o in items[p][3]]
+        want = per_pair(a, c, n)
+        for k in range(n - 1):
+            assert np.array_equal(got[k], want[k]), f"pair {p} output {k}"
+        est = v1.estimate(a, c)
+        check_estimate(Internals(b.debug_internal(p)), est)
+        for k in range(n - 1):
+            why, info = check_output(got[k], a, c, est, (k + 1) / n)
+            assert b.debug_counters(p, k, cnt)
+            cv = cnt.cpu().numpy().astype(np.uint32)
+            assert int(cv[1]) == est["sadsum"]
+            if why != "cut":
+                assert int(cv[0]) == info["count"], (p, k)
+    b.close()
+
+
+def test_graph_replays_match_eager():
+    """Replay the captured graph several times with eager work and synchronisations in between; every replay must
+    reproduce the eager outputs exactly (the graph holds no memset nodes: the counters are cleared by a kernel)."""
+    pairs, mults = mixed_pairs()
+    be, items_e = make_batch(pairs, mults, use_graph=False)
+    be.run()
+    torch.cuda.synchronize()
+    ref = [[o.cpu().numpy() for o in it[3]] for it in items_e]
+    bg, items_g = make_batch(pairs, mults, use_graph=True)
+    for rep in range(3):
+        for it in items_g:
+            for o in it[3]:
+                o.zero_()
+        torch.cuda.synchronize()
+        bg.run()
+        x = torch.randn(256, 256, device="cuda:0")   # unrelated eager work between replays
+        (x @ x).sum().item()
+        torch.cuda.synchronize()
+        for p, it in enumerate(items_g):
+            for k, o in enumerate(it[3]):
+                assert np.array_equal(o.cpu().numpy(), ref[p][k]), f"replay {rep} pair {p} output {k}"
+    be.close()
+    bg.close()
+
+
+def test_set_frames_rebinds_pointers():
+    pairs, mults = mixed_pairs()
+    b, items = make_batch(pairs, mults, use_graph=True)
+    b.run()
+    torch.cuda.synchronize()
+    # swap prev / curr of every pair (same sizes): outputs must equal the per-pair path of the swapped pairs
+    swapped = [(it[1], it[0], it[2], it[3]) for it in items]
+    assert b.set_frames(swapped)
+    b.run()
+    torch.cuda.synchronize()
+    for p, ((a, c), n) in enumerate(zip(pairs, mults)):
+        want = per_pair(c, a, n)
+        for k in range(n - 1):
+            assert np.array_equal(swapped[p][3][k].cpu().numpy(), want[k]), f"pair {p} output {k}"
+    b.close()
+
+
+def test_1080p_workload_batch_matches_per_pair():
+    pairs, _ = scenes.pairs_1080p(2, n_pairs=8)
+    b, items = make_batch(pairs, [4] * 8, use_graph=True)
+    b.run()
+    b.run()
+    torch.cuda.synchronize()
+    for p, (a, c) in enumerate(pairs):
+        want = per_pair(a, c, 4)
+        for k in range(3):
+            assert np.array_equal(items[p][3][k].cpu().numpy(), want[k]), f"pair {p} output {k}"
+    b.close()

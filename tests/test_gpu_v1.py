@@ -40,7 +40,7 @@ def check_estimate(I, est):
         errs.append(_mismatch(f"median3 L{i}", I.fm(i), est["fields"][i]))
     if s["subpixel"]:
         errs.append(_mismatch("subpix", I.sub(), est["block_field"]))
-    errs.append(_mismatch("sadsum", int(I.u[1]), est["sadsum"]))
+    errs.append(_mismatch("sadsum", I.sadsum(), est["sadsum"]))
     errs.append(_mismatch("pixfield", I.pf(), est["pf"]))
     errs = [e for e in errs if e]
     assert not errs, "\n".join(errs)
