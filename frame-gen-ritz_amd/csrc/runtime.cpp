@@ -460,7 +460,7 @@ void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& pr
         q.out = F + fieldSub;
         q.lam = st.searchPenalty;
         q.u = u;
-        q.gx = cdiv(l0.bw * l0.bh, 4);
+        q.gx = cdiv(l0.bw * l0.bh, kSubpixBlocks);
         PlanOp op;
         op.pass = kPassSubpix;
         op.stage = 400;
@@ -482,12 +482,12 @@ void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& pr
     pp.pf = F + fieldPF;
     pp.smooth = st.smoothBonus;
     pp.zero = st.hudBonus;
-    pp.gx = cdiv(l0.w, 16);
+    pp.gx = cdiv(l0.w, 32);
     PlanOp op;
     op.pass = kPassPixfield;
     op.stage = 401;
     op.gx = pp.gx;
-    op.gy = cdiv(l0.h, 16);
+    op.gy = cdiv(l0.h, 32);
     op.stamp = "pixel field";
     op.set(pp);
     plan.push_back(op);

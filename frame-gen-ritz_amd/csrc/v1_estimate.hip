@@ -303,71 +303,96 @@ __device__ __forceinline__ float tileTap(const float* t, int T, float px, float 
 }
 
 // subpix.comp:26-73 (interp.py:162-163): 3x3 at 0.5 px then 3x3 at 0.25 px around the median-filtered integer vector,
-// one wavefront per block: lanes take (candidate, row) items, rows are summed per candidate, strict < argmin.
-// Adds roundEven(256 * winning SAD) to the scene-cut sum.
-__device__ __forceinline__ void subpixBody(const SubpixParams& P, int bx0, int) {
-    constexpr int T = 13, O = 2;
-    __shared__ float ta[4][T * T], tc[4][T * T];
-    __shared__ float rows[4][72];
-    __shared__ float costs[4][9], sads[4][9];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+// cost = SAD/64 + (lambda * step) * |d|_1, strict <; adds roundEven(256 * winning SAD) to the scene-cut sum.
+// One 64-lane workgroup takes 7 blocks; lane 9j + k computes candidate k of block j over all 64 pixels, summing each
+// row over x and then the rows (subpix.comp:42-55), so every SAD is the shader's float. The taps of one candidate
+// share their fractional offset, so the four bilinear weights are formed once per candidate (the products tap()
+// forms, subpix.comp:15-24). Both frames' 13x13 windows of the 7 blocks sit in LDS.
+
+
+__device__ __forceinline__ void subpixBody(const SubpixParams& P, int g, int) {
+    constexpr int T = 13, O = 2, NB = kSubpixBlocks;
+    __shared__ float ta[NB][T * T], tc[NB][T * T];
+    __shared__ float cst[NB][9], sads[NB][9];
+    __shared__ int sv[NB][2];
+    __shared__ uint32_t wsad[NB];
+    const int lane = threadIdx.x;
     const int nb = P.bw * P.bh;
-    const int blk = bx0 * 4 + wave;
-    const bool active = blk < nb;
-    const int bx = active ? blk % P.bw : 0, by = active ? blk / P.bw : 0;
-    const int b0x = bx * kBlock, b0y = by * kBlock;
-    const float2 v0 = P.in[active ? blk : 0];
-    const int sx = (int)rintf(v0.x), sy = (int)rintf(v0.y);
-    float* A = ta[wave];
-    float* C = tc[wave];
-    for (int i = lane; i < T * T; i += 64) {
-        const int x = i % T, y = i / T;
-        A[i] = lumaAt(P.la, b0x - sx - O + x, b0y - sy - O + y, P.w, P.h);
-        C[i] = lumaAt(P.lc, b0x + sx - O + x, b0y + sy - O + y, P.w, P.h);
+    if (lane < NB) {
+        const int blk = g * NB + lane;
+        const float2 v0 = P.in[blk < nb ? blk : 0];
+        sv[lane][0] = (int)rintf(v0.x);
+        sv[lane][1] = (int)rintf(v0.y);
     }
     __syncthreads();
+    for (int i = lane; i < NB * T * T; i += 64) {
+        const int j = i / (T * T), r = i - j * (T * T);
+        const int blk = min(g * NB + j, nb - 1);
+        const int b0x = (blk % P.bw) * kBlock, b0y = (blk / P.bw) * kBlock;
+        const int x = r % T, y = r / T;
+        ta[j][r] = lumaAt(P.la, b0x - sv[j][0] - O + x, b0y - sv[j][1] - O + y, P.w, P.h);
+        tc[j][r] = lumaAt(P.lc, b0x + sv[j][0] - O + x, b0y + sv[j][1] - O + y, P.w, P.h);
+    }
+    __syncthreads();
+    const int j = min(lane / 9, NB - 1), k = lane - 9 * (lane / 9);   // lane 63: a duplicate of lane 62's work
+    const float* A = ta[j];
+    const float* C = tc[j];
+    const float dx = (float)(k % 3 - 1), dy = (float)(k / 3 - 1);
     float basex = 0.0f, basey = 0.0f, bestsad = 0.0f;
     for (int pass = 0; pass < 2; pass++) {
         const float step = pass == 0 ? 0.5f : 0.25f;
-        for (int it = lane; it < 72; it += 64) {
-            const int k = it >> 3, y = it & 7;
-            const float ox = basex + (float)(k % 3 - 1) * step, oy = basey + (float)(k / 3 - 1) * step;
-            float sum = 0.0f;
-            for (int x = 0; x < 8; x++)
-                sum = sum + fabsf(tileTap(A, T, (float)(x + O) - ox, (float)(y + O) - oy) -
-                                  tileTap(C, T, (float)(x + O) + ox, (float)(y + O) + oy));
-            rows[wave][it] = sum;
+        const float ox = basex + dx * step, oy = basey + dy * step;
+        // tap coordinates (x + O) -/+ o: integer part x + floor(O -/+ o), one fractional part per candidate
+        const float pax = (float)O - ox, pay = (float)O - oy, pcx = (float)O + ox, pcy = (float)O + oy;
+        const float flax = floorf(pax), flay = floorf(pay), flcx = floorf(pcx), flcy = floorf(pcy);
+        const float fax = pax - flax, fay = pay - flay, fcx = pcx - flcx, fcy = pcy - flcy;
+        const float a00 = (1.0f - fay) * (1.0f - fax), a01 = (1.0f - fay) * fax, a10 = fay * (1.0f - fax), a11 = fay * fax;
+        const float c00 = (1.0f - fcy) * (1.0f - fcx), c01 = (1.0f - fcy) * fcx, c10 = fcy * (1.0f - fcx), c11 = fcy * fcx;
+        const int ia = (int)flay * T + (int)flax, ic = (int)flcy * T + (int)flcx;
+        float total = 0.0f;
+        for (int y = 0; y < 8; y++) {
+            const float* ra = A + ia + y * T;
+            const float* rc = C + ic + y * T;
+            float row = 0.0f;
+#pragma unroll
+            for (int x = 0; x < 8; x++) {
+                const float va = ra[x] * a00 + ra[x + 1] * a01 + ra[x + T] * a10 + ra[x + T + 1] * a11;
+                const float vc = rc[x] * c00 + rc[x + 1] * c01 + rc[x + T] * c10 + rc[x + T + 1] * c11;
+                row = row + fabsf(va - vc);
+            }
+            total = total + row;
         }
-        __syncthreads();
-        if (lane < 9) {
-            float sum = 0.0f;
-            for (int y = 0; y < 8; y++) sum = sum + rows[wave][lane * 8 + y];
-            const float sad = sum * (1.0f / 64.0f);
-            const float dx = (float)(lane % 3 - 1), dy = (float)(lane / 3 - 1);
-            sads[wave][lane] = sad;
-            costs[wave][lane] = sad + P.lam * step * (fabsf(dx) + fabsf(dy));
+        const float sad = total * (1.0f / 64.0f);
+        if (lane < 9 * NB) {
+            sads[j][k] = sad;
+            cst[j][k] = sad + P.lam * step * (fabsf(dx) + fabsf(dy));
         }
         __syncthreads();
         int bi = 0;
-        float best = costs[wave][0];
-        for (int k = 1; k < 9; k++)
-            if (costs[wave][k] < best) {
-                best = costs[wave][k];
-                bi = k;
+        float best = cst[j][0];
+        for (int kk = 1; kk < 9; kk++)
+            if (cst[j][kk] < best) {
+                best = cst[j][kk];
+                bi = kk;
             }
         basex = basex + (float)(bi % 3 - 1) * step;
         basey = basey + (float)(bi / 3 - 1) * step;
-        bestsad = sads[wave][bi];
+        bestsad = sads[j][bi];
         __syncthreads();
     }
-    // subpix.comp:65-69: the block's SAD into the scene-cut sum (one atomic per workgroup into its slot)
-    __shared__ uint32_t wsad[4];
-    if (lane == 0) {
-        if (active) P.out[blk] = make_float2((float)sx + basex, (float)sy + basey);
-        wsad[wave] = active ? (uint32_t)rintf(bestsad * 256.0f) : 0u;
+    const int blk = g * NB + j;
+    const bool active = lane < 9 * NB && blk < nb;
+    if (k == 0 && lane < 9 * NB) {
+        if (active) P.out[blk] = make_float2((float)sv[j][0] + basex, (float)sv[j][1] + basey);
+        wsad[j] = active ? (uint32_t)rintf(bestsad * 256.0f) : 0u;
     }
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(P.u + kUSad + (bx0 & (kSlots - 1)), wsad[0] + wsad[1] + wsad[2] + wsad[3]);
+    // subpix.comp:65-69: the blocks' SADs into the scene-cut sum, one atomic per workgroup into its slot
+    if (lane == 0) {
+        uint32_t sum = 0u;
+        for (int i = 0; i < NB; i++) sum += wsad[i];
+        atomicAdd(P.u + kUSad + (g & (kSlots - 1)), sum);
+    }
 }
 
 // ---------------------------------------------------------------------------------------------------------- pixfield
@@ -394,45 +419,123 @@ __device__ __forceinline__ float2 pixCand(int k, int qx, int qy, const float2* b
     }
 }
 
-// pixfield.comp:37-63 (interp.py:167-183): 16x16 flow pixels per workgroup, the 7 candidates' errors of the 20x20 tile
-// (+2 apron, replicate) in LDS; cost = float32(exact window sum * (1/25)) with the sum in float64 (cv2.blur's CV_32F
-// path), - smoothBonus for the bilinear candidate, - hudBonus for zero, strict < (first wins).
+// Four horizontally adjacent pixels sampled at p - v (sign -1) or p + v (sign +1) with one vector v, each exactly as
+// sampleL() would: when the four 1/32 maps are 32 apart with one fractional part and no tap needs clamping, the four
+// samples share their weights and read 5 consecutive texels per row. Returns false when that does not hold.
+__device__ __forceinline__ bool sampleRun4(const float* img, int w, int h, int qx, int qy, float2 v, float sign, float* out) {
+    const float px0 = (float)qx + sign * v.x, px3 = (float)(qx + 3) + sign * v.x, py = (float)qy + sign * v.y;
+    const int X0 = (int)rintf(px0 * 32.0f), X3 = (int)rintf(px3 * 32.0f), Y = (int)rintf(py * 32.0f);
+    const int ix = X0 >> 5, iy = Y >> 5;
+    if (X3 != X0 + 96 || ix < 0 || ix + 4 > w - 1 || iy < 0 || iy + 1 > h - 1) return false;
+    const float fx = (float)(X0 & 31) * (1.0f / 32.0f), fy = (float)(Y & 31) * (1.0f / 32.0f);
+    const float w00 = (1.0f - fy) * (1.0f - fx), w01 = (1.0f - fy) * fx, w10 = fy * (1.0f - fx), w11 = fy * fx;
+    const float* r0 = img + iy * w + ix;
+    const float* r1 = r0 + w;
+    float a[5], b[5];
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+        a[i] = r0[i];
+        b[i] = r1[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) out[i] = a[i] * w00 + a[i + 1] * w01 + b[i] * w10 + b[i + 1] * w11;
+    return true;
+}
+
+// pixfield.comp:37-63 (interp.py:167-183): per flow pixel, the candidate whose 5x5 window of |a(q - v) - c(q + v)|
+// (each window pixel with its own candidate, replicate border) costs least; cost = float32(exact window sum * (1/25))
+// with the sum in float64 (cv2.blur's CV_32F path), - smoothBonus for the bilinear candidate, - hudBonus for zero,
+// strict < (first wins).
+//
+// 32 x 32 flow pixels per workgroup (4 per thread). The candidates are taken one at a time (unrolled at compile time):
+// the errors of the tile and its apron go to LDS (runs of 4 pixels of one 8x8 block share one vector for candidates
+// 1-5, see sampleRun4; the zero candidate is a texel difference: weights 1, 0, 0, 0), then the horizontal 5-sums
+// (float64), then each pixel's vertical sum and the running minimum in registers.
+constexpr int kPfTile = 32, kPfRW = 40, kPfRH = kPfTile + 4, kPfRuns = (kPfRW / 4) * kPfRH;
+
+// the block offset of candidate k = 1..5 (own, above, below, left, right; pixfield.comp:29-33)
+__device__ __forceinline__ int2 candOffset(int k) {
+    const int dy = k == 2 ? -1 : (k == 3 ? 1 : 0);
+    const int dx = k == 4 ? -1 : (k == 5 ? 1 : 0);
+    return make_int2(dx, dy);
+}
+
+__device__ __forceinline__ void pixErrRun(const PixParams& P, int k, int gx0, int qy, float* e) {
+    if (k == 6) {   // zero vector: sampleL at an integer position is the texel (weights 1, 0, 0, 0)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int qx = clampi(gx0 + i, 0, P.w - 1);
+            e[i] = fabsf(P.la[qy * P.w + qx] - P.lc[qy * P.w + qx]);
+        }
+        return;
+    }
+    if (k > 0 && gx0 >= 0 && gx0 + 3 <= P.w - 1) {
+        const int2 o = candOffset(k);
+        const float2 v = fieldAt(P.bf, gx0 / kBlock + o.x, qy / kBlock + o.y, P.bw, P.bh);   // one block, one vector
+        float sa[4], sc[4];
+        if (sampleRun4(P.la, P.w, P.h, gx0, qy, v, -1.0f, sa) && sampleRun4(P.lc, P.w, P.h, gx0, qy, v, 1.0f, sc)) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) e[i] = fabsf(sa[i] - sc[i]);
+            return;
+        }
+    }
+#pragma unroll 1
+    for (int i = 0; i < 4; i++) {
+        const int qx = clampi(gx0 + i, 0, P.w - 1);
+        const float2 v = pixCand(k, qx, qy, P.bf, P.bw, P.bh);
+        e[i] = fabsf(sampleL(P.la, (float)qx - v.x, (float)qy - v.y, P.w, P.h) -
+                     sampleL(P.lc, (float)qx + v.x, (float)qy + v.y, P.w, P.h));
+    }
+}
+
 __device__ __forceinline__ void pixfieldBody(const PixParams& P, int gbx, int gby) {
-    constexpr int TS = 16, AP = 2, TT = TS + 2 * AP, NC = 7;
-    __shared__ float err[NC][TT * TT];
+    __shared__ float errs[kPfRH * kPfRW];
+    __shared__ double hsum[kPfRH * kPfTile];
     const int tid = threadIdx.x;
-    const int lx = tid & 15, ly = tid >> 4;
-    const int t0x = gbx * TS - AP, t0y = gby * TS - AP;
-    for (int i = tid; i < TT * TT; i += TS * TS) {
-        const int qx = clampi(t0x + i % TT, 0, P.w - 1), qy = clampi(t0y + i / TT, 0, P.h - 1);
-        for (int k = 0; k < NC; k++) {
-            const float2 v = pixCand(k, qx, qy, P.bf, P.bw, P.bh);
-            err[k][i] = fabsf(sampleL(P.la, (float)qx - v.x, (float)qy - v.y, P.w, P.h) -
-                              sampleL(P.lc, (float)qx + v.x, (float)qy + v.y, P.w, P.h));
+    const int x0 = gbx * kPfTile, y0 = gby * kPfTile;
+    const int cx = tid & 31, cy = tid >> 5;
+    float best[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    int bk[4] = {0, 0, 0, 0};
+#pragma unroll 1
+    for (int k = 0; k < 7; k++) {
+        for (int r = tid; r < kPfRuns; r += 256) {
+            const int ry = r / (kPfRW / 4), rc = r - ry * (kPfRW / 4);
+            float e[4];
+            pixErrRun(P, k, x0 - 4 + 4 * rc, clampi(y0 - 2 + ry, 0, P.h - 1), e);
+#pragma unroll
+            for (int i = 0; i < 4; i++) errs[ry * kPfRW + 4 * rc + i] = e[i];
         }
-    }
-    __syncthreads();
-    const int qx = gbx * TS + lx, qy = gby * TS + ly;
-    if (qx >= P.w || qy >= P.h) return;
-    int bk = 0;
-    float best = 0.0f;
-    for (int k = 0; k < NC; k++) {
-        double vs = 0.0;
-        for (int y = 0; y < 5; y++) {
-            const float* r = &err[k][(ly + y) * TT + lx];
+        __syncthreads();
+        // horizontal 5-sums for output columns c = 0..31 (region columns c + 2 .. c + 6)
+        for (int i = tid; i < kPfRH * kPfTile; i += 256) {
+            const int ry = i >> 5, c = i & 31;
+            const float* e = &errs[ry * kPfRW + c + 2];
             double hs = 0.0;
-            for (int x = 0; x < 5; x++) hs = hs + (double)r[x];
-            vs = vs + hs;
+#pragma unroll
+            for (int x = 0; x < 5; x++) hs = hs + (double)e[x];
+            hsum[i] = hs;
         }
-        float c = (float)(vs * (1.0 / 25.0));
-        if (k == 0) c = c - P.smooth;
-        if (k == NC - 1) c = c - P.zero;
-        if (k == 0 || c < best) {
-            best = c;
-            bk = k;
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int ry = cy + 8 * j;
+            double vs = 0.0;
+#pragma unroll
+            for (int y = 0; y < 5; y++) vs = vs + hsum[(ry + y) * kPfTile + cx];
+            float c = (float)(vs * (1.0 / 25.0));
+            if (k == 0) c = c - P.smooth;
+            if (k == 6) c = c - P.zero;
+            if (k == 0 || c < best[j]) {
+                best[j] = c;
+                bk[j] = k;
+            }
         }
     }
-    P.pf[qy * P.w + qx] = pixCand(bk, qx, qy, P.bf, P.bw, P.bh);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int qx = x0 + cx, qy = y0 + cy + 8 * j;
+        if (qx < P.w && qy < P.h) P.pf[qy * P.w + qx] = pixCand(bk[j], qx, qy, P.bf, P.bw, P.bh);
+    }
 }
 
 // ---------------------------------------------------------------------------------------------------------- kernels
@@ -456,7 +559,20 @@ FGR_PASS(search2, SearchParams, searchBody<2>, 256)
 FGR_PASS(search6, SearchParams, searchBody<6>, 256)
 FGR_PASS(search12, SearchParams, searchBody<12>, 256)
 FGR_PASS(median3, Median3Params, median3Body, 256)
-FGR_PASS(subpix, SubpixParams, subpixBody, 256)
+FGR_PASS(subpix, SubpixParams, subpixBody, 64)
+// at least 4 wavefronts per SIMD (<= 128 VGPRs): the unrolled candidates must not hoist their loads across each other
+#define FGR_PASS_W(name, Params, body, threads, waves)                                                           \
+    __global__ __launch_bounds__(threads) __attribute__((amdgpu_waves_per_eu(waves))) void name##_kernel(Params P) { \
+        body(P, (int)blockIdx.x, (int)blockIdx.y);                                                               \
+    }                                                                                                            \
+    __global__ __launch_bounds__(threads) __attribute__((amdgpu_waves_per_eu(waves))) void name##_batch_kernel(   \
+        const Params* __restrict__ P, const uint32_t* __restrict__ work) {                                       \
+        const uint32_t w = work[blockIdx.x];                                                                     \
+        const Params p = P[w >> kWorkShift];                                                                     \
+        const int l = (int)(w & kWorkMask);                                                                      \
+        body(p, l % p.gx, l / p.gx);                                                                             \
+    }
+
 FGR_PASS(pixfield, PixParams, pixfieldBody, 256)
 
 // ---------------------------------------------------------------------------------------------------------- launches
@@ -503,15 +619,15 @@ hipError_t launchMedian3(const Median3Params& p, hipStream_t s) {
 
 hipError_t launchSubpix(const SubpixParams& p, hipStream_t s) {
     SubpixParams q = p;
-    q.gx = cdiv(p.bw * p.bh, 4);
-    hipLaunchKernelGGL(subpix_kernel, dim3((unsigned)q.gx), dim3(256), 0, s, q);
+    q.gx = cdiv(p.bw * p.bh, kSubpixBlocks);
+    hipLaunchKernelGGL(subpix_kernel, dim3((unsigned)q.gx), dim3(64), 0, s, q);
     return hipGetLastError();
 }
 
 hipError_t launchPixfield(const PixParams& p, hipStream_t s) {
     PixParams q = p;
-    q.gx = cdiv(p.w, 16);
-    hipLaunchKernelGGL(pixfield_kernel, dim3((unsigned)q.gx, (unsigned)cdiv(p.h, 16)), dim3(256), 0, s, q);
+    q.gx = cdiv(p.w, 32);
+    hipLaunchKernelGGL(pixfield_kernel, dim3((unsigned)q.gx, (unsigned)cdiv(p.h, 32)), dim3(256), 0, s, q);
     return hipGetLastError();
 }
 
@@ -526,7 +642,7 @@ hipError_t launchSingle(int pass, const void* params, int gx, int gy, hipStream_
     case kPassSearch6: hipLaunchKernelGGL(search6_kernel, g, dim3(256), 0, s, *(const SearchParams*)params); break;
     case kPassSearch12: hipLaunchKernelGGL(search12_kernel, g, dim3(256), 0, s, *(const SearchParams*)params); break;
     case kPassMedian3: hipLaunchKernelGGL(median3_kernel, g, dim3(256), 0, s, *(const Median3Params*)params); break;
-    case kPassSubpix: hipLaunchKernelGGL(subpix_kernel, g, dim3(256), 0, s, *(const SubpixParams*)params); break;
+    case kPassSubpix: hipLaunchKernelGGL(subpix_kernel, g, dim3(64), 0, s, *(const SubpixParams*)params); break;
     case kPassPixfield: hipLaunchKernelGGL(pixfield_kernel, g, dim3(256), 0, s, *(const PixParams*)params); break;
     default: return launchSynthSingle(pass, params, gx, gy, s);
     }
@@ -545,7 +661,7 @@ hipError_t launchBatch(int pass, const void* params, const uint32_t* work, int n
     case kPassSearch6: hipLaunchKernelGGL(search6_batch_kernel, g, dim3(256), 0, s, (const SearchParams*)params, work); break;
     case kPassSearch12: hipLaunchKernelGGL(search12_batch_kernel, g, dim3(256), 0, s, (const SearchParams*)params, work); break;
     case kPassMedian3: hipLaunchKernelGGL(median3_batch_kernel, g, dim3(256), 0, s, (const Median3Params*)params, work); break;
-    case kPassSubpix: hipLaunchKernelGGL(subpix_batch_kernel, g, dim3(256), 0, s, (const SubpixParams*)params, work); break;
+    case kPassSubpix: hipLaunchKernelGGL(subpix_batch_kernel, g, dim3(64), 0, s, (const SubpixParams*)params, work); break;
     case kPassPixfield: hipLaunchKernelGGL(pixfield_batch_kernel, g, dim3(256), 0, s, (const PixParams*)params, work); break;
     default: return launchSynthBatch(pass, params, work, n, s);
     }

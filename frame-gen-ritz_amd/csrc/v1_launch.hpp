@@ -14,6 +14,7 @@ hipError_t launchBatch(int pass, const void* params, const uint32_t* work, int n
 hipError_t launchSingle(int pass, const void* params, int gx, int gy, hipStream_t s);
 hipError_t launchSynthSingle(int pass, const void* params, int gx, int gy, hipStream_t s);
 hipError_t launchSynthBatch(int pass, const void* params, const uint32_t* work, int n, hipStream_t s);
+constexpr int kSubpixBlocks = 7;   // blocks per subpix workgroup
 int synthTilesX(int W);
 int synthTilesY(int H);
 int fallbackTilesX(int W);

@@ -60,6 +60,61 @@ __device__ __forceinline__ Rgb warpRgba8(const uint8_t* base, uint32_t pitch, in
 
 __device__ __forceinline__ uint32_t packRgb(const Rgb& v) { return v.r | (v.g << 8) | (v.b << 16); }
 
+struct alignas(4) U32x4 {
+    uint32_t x, y, z, w;
+};
+
+// Four horizontally adjacent pixels (x .. x+3, one row) warped with one interpolated field value r. When their maps
+// X_i = rint(32 (x + i) - k r.x) are 32 apart (same fractional part) and no tap column needs clamping, the four share
+// Y, the weights and 5 consecutive texels per tap row, and each result equals warpRgba8 of that pixel.
+struct RunMap {
+    int X0, Y;
+    bool ok;
+};
+
+__device__ __forceinline__ RunMap runMap(int W, int x, int y, float k, float2 r) {
+    const float kx = k * r.x;
+    RunMap m;
+    m.X0 = (int)rintf((float)(32 * x) - kx);
+    const int X1 = (int)rintf((float)(32 * (x + 1)) - kx), X2 = (int)rintf((float)(32 * (x + 2)) - kx),
+              X3 = (int)rintf((float)(32 * (x + 3)) - kx);
+    const int ix = m.X0 >> 5;
+    m.ok = X1 == m.X0 + 32 && X2 == m.X0 + 64 && X3 == m.X0 + 96 && ix >= 0 && ix + 4 <= W - 1;
+    m.Y = (int)rintf((float)(32 * y) - k * r.y);
+    return m;
+}
+
+// the four warps of a run whose map passed runMap (ok)
+__device__ __forceinline__ void warpRun4(const uint8_t* base, uint32_t pitch, int H, const RunMap& m, uint32_t* out) {
+    const int ix = m.X0 >> 5, iy = m.Y >> 5;
+    const uint32_t fx = (uint32_t)(m.X0 & 31), fy = (uint32_t)(m.Y & 31);
+    const uint32_t y0 = (uint32_t)med3i(iy, 0, H - 1), y1 = (uint32_t)med3i(iy + 1, 0, H - 1);
+    const uint8_t* r0 = base + (__umul24(y0, pitch) + 4u * (uint32_t)ix);
+    const uint8_t* r1 = base + (__umul24(y1, pitch) + 4u * (uint32_t)ix);
+    const U32x4 a4 = *(const U32x4*)r0;
+    const uint32_t a5 = *(const uint32_t*)(r0 + 16);
+    const U32x4 b4 = *(const U32x4*)r1;
+    const uint32_t b5 = *(const uint32_t*)(r1 + 16);
+    const uint32_t ta[5] = {a4.x, a4.y, a4.z, a4.w, a5}, tb[5] = {b4.x, b4.y, b4.z, b4.w, b5};
+    const uint32_t gx = 32u - fx, gy = 32u - fy;
+    const uint32_t w00 = gx * gy, w01 = fx * gy, w10 = gx * fy, w11 = fx * fy;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const uint32_t p00 = ta[i], p01 = ta[i + 1], p10 = tb[i], p11 = tb[i + 1];
+        const uint32_t vr = (w00 * (p00 & 255u) + w01 * (p01 & 255u) + w10 * (p10 & 255u) + w11 * (p11 & 255u) + 512u) >> 10;
+        const uint32_t vg = (w00 * ((p00 >> 8) & 255u) + w01 * ((p01 >> 8) & 255u) + w10 * ((p10 >> 8) & 255u) +
+                             w11 * ((p11 >> 8) & 255u) + 512u) >> 10;
+        const uint32_t vb = (w00 * ((p00 >> 16) & 255u) + w01 * ((p01 >> 16) & 255u) + w10 * ((p10 >> 16) & 255u) +
+                             w11 * ((p11 >> 16) & 255u) + 512u) >> 10;
+        out[i] = vr | (vg << 8) | (vb << 16);
+    }
+}
+
+__device__ __forceinline__ int emaxPacked(uint32_t a, uint32_t c) {
+    return (int)max(max(__sad(a & 255u, c & 255u, 0u), __sad((a >> 8) & 255u, (c >> 8) & 255u, 0u)),
+                    __sad((a >> 16) & 255u, (c >> 16) & 255u, 0u));
+}
+
 // error.comp:9-21 / mismatch() motion_v1.glsl:117-129: e = max-channel |wa - wc| (integers 0..255)
 __device__ __forceinline__ int emaxRgb(const Rgb& a, const Rgb& c) {
     return (int)max(max(__sad(a.r, c.r, 0u), __sad(a.g, c.g, 0u)), __sad(a.b, c.b, 0u));
@@ -115,6 +170,8 @@ __device__ __forceinline__ float trustOf(const SynthParams& P, int S) {
 // floor(m + 0.5) = (kA a + kC c + 2) >> 2 with kA = 4(1-t), kC = 4t.
 __device__ __forceinline__ uint32_t blendRgba8(const SynthParams& P, uint32_t wa, uint32_t wc, uint32_t nr, float u) {
     if (u == 1.0f) return nr;
+    if (u == 0.0f && P.mid)   // (a + c + 1) >> 1 per byte (v_lerp_u8), alpha from the nearer frame
+        return (__builtin_amdgcn_lerp(wa, wc, 0x01010101u) & 0x00FFFFFFu) | (nr & 0xFF000000u);
     uint32_t r = nr & 0xFF000000u;
     if (u == 0.0f && P.quarter) {
 #pragma unroll
@@ -214,16 +271,43 @@ __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, 
         const float2* h = hrow + (FlowMap<S>::index(yc) - ty0) * EW + ex;
         return mix2(h[0], h[EW], FlowMap<S>::frac(yc));
     };
-    // interior pixels: warps kept in registers (packed) for the blend
-    uint32_t wa[TH / 4], wc[TH / 4];
+    // interior pixels in runs of 4 (columns 4 rc .. 4 rc + 3, rows rr and rr + 16): warps kept in registers (packed)
+    const int rc = tid & 15, rr = tid >> 4;
+    uint32_t wa[8], wc[8];
 #pragma unroll
-    for (int k = 0; k < TH / 4; k++) {
-        const int x = min(x0 + cx, P.W - 1), y = min(y0 + ry + 4 * k, P.H - 1);
-        Rgb a, c;
-        warpPair(P, kA32, kC32, x, y, fieldAtRegion(cx + AP, y), a, c);
-        es[(ry + 4 * k + AP) * EW + cx + AP] = emaxRgb(a, c);
-        wa[k] = packRgb(a);
-        wc[k] = packRgb(c);
+    for (int k = 0; k < 2; k++) {
+        const int y = min(y0 + rr + 16 * k, P.H - 1);
+        const int yc = min(y, ylim);
+        const float2* h = hrow + (FlowMap<S>::index(yc) - ty0) * EW + 4 * rc + AP;
+        const float fy = FlowMap<S>::frac(yc);
+        float2 r[4];
+        int xs[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            xs[i] = min(x0 + 4 * rc + i, P.W - 1);
+            // a clamped column repeats the last pixel: its hrow entry is that pixel's (clamped the same way)
+            r[i] = mix2(h[i], h[i + EW], fy);
+        }
+        const bool same = xs[3] == xs[0] + 3 && r[1].x == r[0].x && r[1].y == r[0].y && r[2].x == r[0].x &&
+                          r[2].y == r[0].y && r[3].x == r[0].x && r[3].y == r[0].y;
+        uint32_t* oa = wa + 4 * k;
+        uint32_t* oc = wc + 4 * k;
+        const RunMap ma = runMap(P.W, xs[0], y, kA32, r[0]), mc = runMap(P.W, xs[0], y, -kC32, r[0]);
+        // the shared-map form only when every lane of the wavefront can take it (no divergent second path)
+        if (__all(same && ma.ok && mc.ok)) {
+            warpRun4(P.a.p, (uint32_t)P.a.pitch, P.H, ma, oa);
+            warpRun4(P.c.p, (uint32_t)P.c.pitch, P.H, mc, oc);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                Rgb a, c;
+                warpPair(P, kA32, kC32, xs[i], y, r[i], a, c);
+                oa[i] = packRgb(a);
+                oc[i] = packRgb(c);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) es[(rr + 16 * k + AP) * EW + 4 * rc + i + AP] = emaxPacked(oa[i], oc[i]);
     }
     // apron: rows 0..2 and EH-3..EH-1 in full, then columns 0..2 and EW-3..EW-1 of the rows between
     for (int j = tid; j < NAPRON; j += 256) {
@@ -251,15 +335,37 @@ __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, 
     __syncthreads();
     uint32_t mine = 0u;
 #pragma unroll
-    for (int k = 0; k < TH / 4; k++) {
-        const int x = x0 + cx, y = y0 + ry + 4 * k;
-        const bool inside = x < P.W && y < P.H;
-        const uint16_t* h = &hs[(ry + 4 * k) * TW + cx];
-        const int S49 = (int)h[0] + h[TW] + h[2 * TW] + h[3 * TW] + h[4 * TW] + h[5 * TW] + h[6 * TW];
-        if (inside) {
-            mine += S49 > P.sHalf ? 1u : 0u;
-            const uint32_t nr = ld32(nf, x, y);
-            *(uint32_t*)(P.out + (int64_t)y * P.opitch + 4 * (int64_t)x) = blendRgba8(P, wa[k], wc[k], nr, trustOf(P, S49));
+    for (int k = 0; k < 2; k++) {
+        const int x = x0 + 4 * rc, y = y0 + rr + 16 * k;
+        if (y >= P.H || x >= P.W) continue;
+        const uint16_t* h = &hs[(rr + 16 * k) * TW + 4 * rc];
+        uint32_t res[4];
+        uint32_t nr[4];
+        uint8_t* dst = P.out + (int64_t)y * P.opitch + 4 * (int64_t)x;
+        const uint8_t* src = nf.p + (int64_t)y * nf.pitch + 4 * (int64_t)x;
+        const bool full = x + 3 < P.W;
+        if (full) {
+            const U32x4 n4 = *(const U32x4*)src;
+            nr[0] = n4.x; nr[1] = n4.y; nr[2] = n4.z; nr[3] = n4.w;
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++) nr[i] = x + i < P.W ? ((const uint32_t*)src)[i] : 0u;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int S49 = (int)h[i] + h[i + TW] + h[i + 2 * TW] + h[i + 3 * TW] + h[i + 4 * TW] + h[i + 5 * TW] + h[i + 6 * TW];
+            const bool inside = x + i < P.W;
+            mine += (inside && S49 > P.sHalf) ? 1u : 0u;
+            res[i] = blendRgba8(P, wa[4 * k + i], wc[4 * k + i], nr[i], trustOf(P, S49));
+        }
+        if (full) {
+            U32x4 o;
+            o.x = res[0]; o.y = res[1]; o.z = res[2]; o.w = res[3];
+            *(U32x4*)dst = o;
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                if (x + i < P.W) ((uint32_t*)dst)[i] = res[i];
         }
     }
     // untrusted count: a wavefront sum, one LDS atomic per wavefront, one global atomic per workgroup into its slot
