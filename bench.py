@@ -9,8 +9,13 @@ three synths at t = 1/4, 1/2, 3/4. Frames are resident in HBM before the timed r
   * batched: one launch per pass for all pairs (fgr_batch_*), replayed as a hipGraph.
 `value` is the batched rate when the batched path is built, else the plain rate (the line says which).
 
-Extra fields: `roofline` of the dominant kernel (the synth kernel: algorithmic bytes per launch over its average
-launch duration, from HIP events on the launch stream), `cpu_baseline` (the numpy oracle on one pair, rank 0 only).
+Extra fields:
+  * `roofline` of the dominant kernel, the batched synth (one launch = the k-th synth of all 8 pairs): algorithmic
+    bytes per launch (3F + P per pair, 2F for the scene-cut pair's copy) over its average launch duration, from HIP
+    events around each launch on the launch stream; `traffic` = the HBM bytes per launch that rocprofv3's FETCH_SIZE
+    and WRITE_SIZE counters read (from the committed summary under profiles/), or null;
+  * `copy_probe`: a device-to-device copy of known size, the bandwidth a plain streaming kernel reaches;
+  * `cpu_baseline`: the numpy oracle on 4 pairs of the workload, one core.
 """
 import argparse
 import json
@@ -45,6 +50,18 @@ def pair_bytes(w, h, n, cut=False):
     return 2 * F + P + (n - 1) * synth_bytes(w, h, cut)
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (each counter
+    scaled by what it reads for a device copy of known size), or None when there is no such summary."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_1080p_4x_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(line.split(":")[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -53,6 +70,7 @@ def main():
     ap.add_argument("--pairs", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--path", choices=["auto", "plain", "batched"], default="auto")
+    ap.add_argument("--copy-mib", type=int, default=512, help="size of the device copy probe")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: every pair is independent, so more GPUs would shard "
@@ -108,41 +126,74 @@ def main():
         value, ms = plain_rate, 1e3 * plain_s / args.steps
         path = "plain"
 
-    # roofline of the synth kernel: HIP events around each synth launch on its stream (the library's profile)
+    # roofline of the dominant kernel, the batched synth (one launch = the k-th synth of every pair): the batch's
+    # launches recorded eagerly between HIP events on the launch stream, `rounds` records
+    rounds = max(2, args.steps // 4)
+    synth_ns, synth_bytes_l = [], []
+    rb = batch if batch is not None else None
+    if rb is not None:
+        rb.set_profiling(True)
+        for _ in range(rounds):
+            rb.run()
+            for stage, pas, ns in rb.read_timings():
+                if 500 <= stage < 700 and stage % 10 == 0:
+                    synth_ns.append(ns)
+                    synth_bytes_l.append(sum(synth_bytes(W, H, cut=(p == cut_index)) for p in range(len(fr))))
+        rb.set_profiling(False)
+        kernel = "synth_rgba8_s2_batch_kernel"
+    # the plain path's per-synth launch, for comparison (the library's profile: events around each synth)
     fi.set_profiling(True)
-    durs, nbytes = [], []
-    for _ in range(max(2, args.steps // 4)):
-        for p, (a, c) in enumerate(fr):
-            fi.begin_frame()
-            assert fi.record_estimate(a, c)
-            for k, t in enumerate(ts):
-                assert fi.record_synth(a, c, outs[p][k], t)
-            tim = fi.read_timings()
-            for name, ns in tim:
-                if name == "synth":
-                    durs.append(ns)
-            nbytes += [synth_bytes(W, H, cut=(p == cut_index))] * (MULT - 1)
+    plain_ns = []
+    for p, (a, c) in enumerate(fr):
+        fi.begin_frame()
+        assert fi.record_estimate(a, c)
+        for k, t in enumerate(ts):
+            assert fi.record_synth(a, c, outs[p][k], t)
+        plain_ns += [ns for name, ns in fi.read_timings() if name == "synth" and p != cut_index]
     fi.set_profiling(False)
-    achieved = sum(nbytes) / (sum(durs) * 1e-9) / 1e9
+    if rb is None:
+        synth_ns = plain_ns
+        synth_bytes_l = [synth_bytes(W, H)] * len(plain_ns)
+        kernel = "synth_rgba8_s2_kernel"
+    avg_ns = sum(synth_ns) / len(synth_ns)
+    bytes_per_launch = sum(synth_bytes_l) / len(synth_bytes_l)
+    achieved = bytes_per_launch / (avg_ns * 1e-9) / 1e9
+    traffic = measured_traffic(kernel)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "synth_rgba8_kernel", "launches": len(durs),
-                "avg_launch_us": round(sum(durs) / len(durs) / 1e3, 3),
-                "bytes_per_launch": int(sum(nbytes) / len(nbytes))}
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": kernel, "launches": len(synth_ns), "avg_launch_us": round(avg_ns / 1e3, 3),
+                "bytes_per_launch": int(bytes_per_launch),
+                "plain_synth_avg_us": round(sum(plain_ns) / len(plain_ns) / 1e3, 3)}
+
+    # a plain device-to-device copy of known size: the bandwidth a streaming kernel reaches on this GPU
+    copy_bytes = args.copy_mib << 20
+    src = torch.empty(copy_bytes, dtype=torch.uint8, device=dev)
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 2 * copy_bytes * 10 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del src, dst
 
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, ROOT)
         from oracle import v1
-        a, c = pairs[0]
+        sample = [0, 1, 2, 3]
         t0 = time.perf_counter()
-        est = v1.estimate(a, c)
-        for t in ts:
-            v1.synth(a, c, est, t)
+        for p in sample:
+            a, c = pairs[p]
+            est = v1.estimate(a, c)
+            for t in ts:
+                v1.synth(a, c, est, t)
         cpu_s = time.perf_counter() - t0
-        cpu = {"value": round(1.0 / cpu_s, 4), "unit": "pairs/s", "cores": 1, "kind": "port",
-               "sample": "pair 0 of the workload (1080p RGBA8, estimate + 3 synths) through the numpy oracle, "
-                         "single-threaded numpy", "seconds": round(cpu_s, 2), "host_cpus": os.cpu_count()}
+        cpu = {"value": round(len(sample) / cpu_s, 4), "unit": "pairs/s", "cores": 1, "kind": "port",
+               "sample": f"pairs {sample} of the workload (1080p RGBA8, estimate + 3 synths each) through the numpy "
+                         "oracle, single-threaded numpy", "seconds": round(cpu_s, 2), "host_cpus": os.cpu_count()}
 
     total_bytes = sum(pair_bytes(W, H, MULT, cut=(p == cut_index)) for p in range(len(fr)))
     line = {
@@ -158,6 +209,7 @@ def main():
         "batched_over_plain": None if batch_rate is None else round(batch_rate / plain_rate, 3),
         "whole_path_hbm_frac": round(total_bytes * args.steps / (ms * 1e-3 * args.steps) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": roofline,
+        "copy_probe": {"bytes": copy_bytes, "gbs": round(copy_gbs, 1), "frac_of_peak": round(copy_gbs / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

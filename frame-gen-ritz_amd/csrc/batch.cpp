@@ -14,6 +14,7 @@
 #include <vector>
 #include "../../include/fgr.h"
 #include "runtime.hpp"
+#include "v1_device.hpp"
 #include "v1_launch.hpp"
 #include "v1_params.hpp"
 
@@ -24,6 +25,7 @@ thread_local const char* gBatchError = nullptr;
 
 struct Launch {
     int pass = 0;
+    int stage = 0;
     int n = 0;             // workgroups
     size_t paramOff = 0;   // byte offsets in the device blob
     size_t workOff = 0;
@@ -43,8 +45,12 @@ struct FgrBatch {
     hipStream_t cap = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    bool profiling = false;
+    std::vector<hipEvent_t> events;   // a (start, end) pair per launch while profiling
+    bool profiled = false;
 
     ~FgrBatch() {
+        for (hipEvent_t e : events) hipEventDestroy(e);
         if (exec) hipGraphExecDestroy(exec);
         if (graph) hipGraphDestroy(graph);
         if (cap) hipStreamDestroy(cap);
@@ -68,6 +74,7 @@ struct FgrBatch {
         blob.clear();
         for (auto& g : groups) {
             Launch L;
+            L.stage = g.first.first;
             L.pass = g.first.second;
             const size_t psz = g.second[0].second->params.size();
             L.paramOff = blob.size();
@@ -80,18 +87,36 @@ struct FgrBatch {
                 const PlanOp& op = *g.second[i].second;
                 for (int l = 0; l < op.gx * op.gy; l++) work.push_back(((uint32_t)i << kWorkShift) | (uint32_t)l);
             }
-            L.n = (int)work.size();
+            // XCD-aware order (xcdChunkTile, as the per-pair kernels use): tiles that read overlapping regions share
+            // an L2
+            const size_t n = work.size();
+            std::vector<uint32_t> order(n);
+            for (size_t i = 0; i < n; i++) order[i] = work[xcdChunkTile((int)i, (int)n)];
+            L.n = (int)n;
             L.workOff = blob.size();
-            blob.resize(align256(blob.size() + work.size() * 4));
-            std::memcpy(blob.data() + L.workOff, work.data(), work.size() * 4);
+            blob.resize(align256(blob.size() + n * 4));
+            std::memcpy(blob.data() + L.workOff, order.data(), n * 4);
             launches.push_back(L);
         }
     }
 
-    bool enqueue(hipStream_t s) {
-        for (const Launch& L : launches)
+    bool enqueue(hipStream_t s, bool timed = false) {
+        for (size_t i = 0; i < launches.size(); i++) {
+            const Launch& L = launches[i];
+            if (timed) hipEventRecord(events[2 * i], s);
             if (launchBatch(L.pass, dev + L.paramOff, (const uint32_t*)(dev + L.workOff), L.n, s) != hipSuccess)
                 return false;
+            if (timed) hipEventRecord(events[2 * i + 1], s);
+        }
+        return true;
+    }
+
+    bool makeEvents() {
+        while (events.size() < 2 * launches.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return false;
+            events.push_back(e);
+        }
         return true;
     }
 
@@ -186,6 +211,10 @@ extern "C" int fgr_batch_set_frames(FgrBatch* b, void* stream, const FgrBatchPai
 extern "C" int fgr_batch_record(FgrBatch* b, void* stream) {
     if (!b) return 0;
     hipStream_t s = (hipStream_t)stream;
+    if (b->profiling) {   // eager launches bracketed by events (a graph replay has no per-launch timestamps)
+        b->profiled = b->enqueue(s, true);
+        return b->profiled ? 1 : 0;
+    }
     if (!b->info.useGraph) return b->enqueue(s) ? 1 : 0;
     if (!b->exec) {
         if (hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal) != hipSuccess) return 0;
@@ -200,6 +229,32 @@ extern "C" int fgr_batch_record(FgrBatch* b, void* stream) {
         }
     }
     return hipGraphLaunch(b->exec, s) == hipSuccess ? 1 : 0;
+}
+
+extern "C" int fgr_batch_set_profiling(FgrBatch* b, int on) {
+    if (!b) return 0;
+    b->profiled = false;
+    b->profiling = false;
+    if (on) {
+        hipSetDevice(b->info.device);
+        if (!b->makeEvents()) return 0;
+    }
+    b->profiling = on != 0;
+    return 1;
+}
+
+extern "C" int fgr_batch_read_timings(FgrBatch* b, int32_t* stage, int32_t* pass, uint64_t* ticks, int cap) {
+    if (!b || !b->profiled || b->launches.empty()) return -1;
+    if (hipEventSynchronize(b->events[2 * b->launches.size() - 1]) != hipSuccess) return -1;
+    const int n = (int)b->launches.size();
+    for (int i = 0; i < n && i < cap; i++) {
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, b->events[2 * i], b->events[2 * i + 1]);
+        if (stage) stage[i] = b->launches[i].stage;
+        if (pass) pass[i] = b->launches[i].pass;
+        if (ticks) ticks[i] = (uint64_t)((double)ms * 1e6 + 0.5);
+    }
+    return n;
 }
 
 extern "C" int fgr_batch_debug_counters(FgrBatch* b, void* stream, uint32_t pair, uint32_t synth, void* dst) {

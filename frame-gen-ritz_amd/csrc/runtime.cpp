@@ -30,6 +30,7 @@ const char* kErrAlloc = "a device allocation failed";
 const char* kErrNull = "null InitInfo";
 const char* kErrWhite = "linearWhiteNits must be in (0, 10000]";
 const char* kErrYuv = "invalid yuvMatrix";
+const char* kErrOdd = "P010 frames need an even width and height (4:2:0 chroma of whole 2x2 blocks)";
 
 bool in(float v, float lo, float hi) { return std::isfinite(v) && v >= lo && v <= hi; }
 
@@ -106,11 +107,12 @@ extern "C" int fgr_size_layout(uint32_t W, uint32_t H, uint32_t flowScale, uint3
 
 extern "C" uint32_t fgr_version(void) { return 0x00010000u; }
 
-static const char* sizeLimitError(uint32_t W, uint32_t H) {
+static const char* sizeLimitError(uint32_t W, uint32_t H, uint32_t format) {
     // framegen.cpp:760-790 (the device limits here: 32-bit indexing and the launch grid)
     if (W < FGR_MIN_SIZE || H < FGR_MIN_SIZE) return kErrMinSize;
     if ((uint64_t)W * H > (1ull << 30)) return kErrPixels;
     if (W > 32768 || H > 32768) return kErrDim;
+    if (format == FGR_FORMAT_P010 && ((W | H) & 1)) return kErrOdd;
     return nullptr;
 }
 
@@ -183,7 +185,7 @@ extern "C" FgrInterpolator* fgr_create(const FgrInitInfo* info) {
     if (!info) { gInitError = kErrNull; return nullptr; }
     if (!formatBuilt(info->format)) { gInitError = kErrFormat; return nullptr; }
     if (const char* e = fgr_settings_error_for_format(&info->settings, info->format)) { gInitError = e; return nullptr; }
-    if (const char* e = sizeLimitError(info->width, info->height)) { gInitError = e; return nullptr; }
+    if (const char* e = sizeLimitError(info->width, info->height, info->format)) { gInitError = e; return nullptr; }
     if (info->format == FGR_FORMAT_RGBAF16 && info->transfer == FGR_TRANSFER_LINEAR &&
         !(info->linearWhiteNits > 0.0f && info->linearWhiteNits <= 10000.0f)) { gInitError = kErrWhite; return nullptr; }
     if (info->format == FGR_FORMAT_P010 && info->yuvMatrix > FGR_YUV_BT2020) { gInitError = kErrYuv; return nullptr; }
@@ -216,7 +218,7 @@ extern "C" uint32_t fgr_format(const FgrInterpolator* h) { return h ? (uint32_t)
 
 extern "C" int fgr_resize(FgrInterpolator* h, uint32_t width, uint32_t height) {
     if (!h) return 0;
-    if (sizeLimitError(width, height)) {
+    if (sizeLimitError(width, height, (uint32_t)h->format)) {
         hipSetDevice(h->device);
         h->freeBuffers();   // framegen.h:457-458: on false the object holds no size until a later resize succeeds
         return 0;
@@ -226,12 +228,12 @@ extern "C" int fgr_resize(FgrInterpolator* h, uint32_t width, uint32_t height) {
 
 extern "C" const char* fgr_settings_error(const FgrInterpolator* h, const FgrSettings* s) {
     if (const char* e = fgr_settings_error_for_format(s, h ? (uint32_t)h->format : FGR_FORMAT_RGBA8)) return e;
-    return h && h->W ? sizeLimitError(h->W, h->H) : nullptr;
+    return h && h->W ? sizeLimitError(h->W, h->H, (uint32_t)h->format) : nullptr;
 }
 
 extern "C" const char* fgr_size_error(const FgrInterpolator* h, uint32_t w, uint32_t hgt) {
     if (!h) return "not initialised";
-    return sizeLimitError(w, hgt);
+    return sizeLimitError(w, hgt, (uint32_t)h->format);
 }
 
 extern "C" int fgr_set_settings(FgrInterpolator* h, const FgrSettings* s) {
@@ -346,12 +348,12 @@ void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& pr
         lp.lc = luma + lv[0].offC;
         lp.u = u;
         lp.bgra = (int)info.bgra;
+        lp.fmt = format;
         lp.lut = fmtLut;
-        lp.yBlack = yBlack;
-        lp.yGain = yGain;
+        lp.yFull = info.yuvFullRange ? 1 : 0;
         lp.gx = cdiv(fw, 16);
         PlanOp op;
-        op.pass = kPassLuma;
+        op.pass = format == FGR_FORMAT_RGBA8 ? kPassLuma : kPassLumaFmt;
         op.stage = 0;
         op.gx = lp.gx;
         op.gy = cdiv(fh, 16);
@@ -560,8 +562,11 @@ void FgrInterpolator::synthParams(SynthParams& p, const FgrFrame& prev, const Fg
     p.quarter = (qa == std::rint(qa) && qc == std::rint(qc)) ? 1 : 0;
     p.kA = (int)qa;
     p.kC = (int)qc;
+    p.fmt = format;
     p.lut = fmtLut;
     for (int i = 0; i < 8; i++) p.yuv[i] = yuv[i];
+    p.cw = ((int)W + 1) / 2;
+    p.ch = ((int)H + 1) / 2;
 }
 
 void FgrInterpolator::planSynth(std::vector<PlanOp>& plan, const FgrFrame& prev, const FgrFrame& curr,
@@ -575,7 +580,10 @@ void FgrInterpolator::planSynth(std::vector<PlanOp>& plan, const FgrFrame& prev,
     sp.ocpitch = (int64_t)out.chromaPitch;
     sp.gx = synthTilesX((int)W);
     PlanOp op;
-    op.pass = set.flowScale == 1 ? kPassSynthRgba8S1 : (set.flowScale == 2 ? kPassSynthRgba8S2 : kPassSynthRgba8S4);
+    if (format == FGR_FORMAT_RGBA8)
+        op.pass = set.flowScale == 1 ? kPassSynthRgba8S1 : (set.flowScale == 2 ? kPassSynthRgba8S2 : kPassSynthRgba8S4);
+    else
+        op.pass = set.flowScale == 1 ? kPassSynthFmtS1 : (set.flowScale == 2 ? kPassSynthFmtS2 : kPassSynthFmtS4);
     op.stage = 500 + 10 * k;
     op.gx = sp.gx;
     op.gy = synthTilesY((int)H);

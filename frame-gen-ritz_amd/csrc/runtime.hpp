@@ -42,9 +42,8 @@ struct FgrInterpolator {
     uint32_t* u = nullptr;        // counters and medians
 
     // per-format constants (formats.cpp)
-    float* fmtLut = nullptr;      // RgbaF16 linear: 65536 half -> PQ level
-    float yBlack = 0.0f, yGain = 1.0f;
-    float yuv[8] = {};
+    int* fmtLut = nullptr;        // RgbaF16: metric level of every half value in 1/16 level (65536 entries)
+    float yuv[8] = {};            // P010: YU_YS, YU_RCR, YU_BCB, YU_GCB, YU_GCR
     bool formatInit();
     void formatFree();
 

@@ -84,4 +84,32 @@ __device__ __forceinline__ uint32_t slotSum(const uint32_t* slots) {
 
 __device__ __forceinline__ int linearBlock() { return (int)(blockIdx.y * gridDim.x + blockIdx.x); }
 
+// XCD-aware tile order. Workgroup i of a launch runs on XCD i mod 8, and each XCD has its own L2. The tiles are
+// dealt to the XCDs in chunks of kXcdChunk consecutive tiles (chunk c to XCD c mod 8), so neighbouring tiles -- which
+// read overlapping aprons and search windows -- mostly share an L2, while every XCD still gets an even share of each
+// pair (whole-pair chunks would leave an XCD idle on a cheap scene-cut pair). The tail beyond the last whole round of
+// 8 chunks keeps the plain order. The batched launches get the same order from their host-built work lists.
+constexpr int kXcds = 8, kXcdChunk = 64;
+__host__ __device__ __forceinline__ int xcdChunkTile(int i, int n) {
+    const int span = kXcds * kXcdChunk;
+    if (i >= n - n % span) return i;
+    const int m = i / kXcds, x = i % kXcds;
+    return ((m / kXcdChunk) * kXcds + x) * kXcdChunk + m % kXcdChunk;
+}
+
+// One device body, two kernels: name_kernel runs one pass of one pair on a (gx, gy) grid; name_batch_kernel runs the
+// same pass for many pairs, workgroup i taking work[i] = (parameter block << kWorkShift) | tile.
+#define FGR_PASS(name, Params, body, threads)                                                                     \
+    __global__ __launch_bounds__(threads) void name##_kernel(Params P) {                                          \
+        const int l = xcdChunkTile(linearBlock(), (int)(gridDim.x * gridDim.y));                                 \
+        body(P, l % (int)gridDim.x, l / (int)gridDim.x);                                                          \
+    }                                                                                                             \
+    __global__ __launch_bounds__(threads) void name##_batch_kernel(const Params* __restrict__ P,                  \
+                                                                   const uint32_t* __restrict__ work) {          \
+        const uint32_t w = work[blockIdx.x];                                                                      \
+        const Params p = P[w >> kWorkShift];                                                                      \
+        const int l = (int)(w & kWorkMask);                                                                       \
+        body(p, l % p.gx, l / p.gx);                                                                              \
+    }
+
 }  // namespace fgr

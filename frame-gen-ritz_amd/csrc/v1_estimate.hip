@@ -542,15 +542,6 @@ __device__ __forceinline__ void pixfieldBody(const PixParams& P, int gbx, int gb
 
 // Each pass has a per-pair kernel (grid = that pair's blocks) and a batched kernel over a work list spanning many pairs
 // (one launch per pass for a whole batch); both run the same body.
-#define FGR_PASS(name, Params, body, threads)                                                                    \
-    __global__ __launch_bounds__(threads) void name##_kernel(Params P) { body(P, (int)blockIdx.x, (int)blockIdx.y); } \
-    __global__ __launch_bounds__(threads) void name##_batch_kernel(const Params* __restrict__ P,                  \
-                                                                   const uint32_t* __restrict__ work) {          \
-        const uint32_t w = work[blockIdx.x];                                                                     \
-        const Params p = P[w >> kWorkShift];                                                                     \
-        const int l = (int)(w & kWorkMask);                                                                      \
-        body(p, l % p.gx, l / p.gx);                                                                             \
-    }
 
 FGR_PASS(luma_rgba8, LumaParams, lumaRgba8Body, 256)
 FGR_PASS(down, DownParams, downBody, 256)
@@ -560,76 +551,11 @@ FGR_PASS(search6, SearchParams, searchBody<6>, 256)
 FGR_PASS(search12, SearchParams, searchBody<12>, 256)
 FGR_PASS(median3, Median3Params, median3Body, 256)
 FGR_PASS(subpix, SubpixParams, subpixBody, 64)
-// at least 4 wavefronts per SIMD (<= 128 VGPRs): the unrolled candidates must not hoist their loads across each other
-#define FGR_PASS_W(name, Params, body, threads, waves)                                                           \
-    __global__ __launch_bounds__(threads) __attribute__((amdgpu_waves_per_eu(waves))) void name##_kernel(Params P) { \
-        body(P, (int)blockIdx.x, (int)blockIdx.y);                                                               \
-    }                                                                                                            \
-    __global__ __launch_bounds__(threads) __attribute__((amdgpu_waves_per_eu(waves))) void name##_batch_kernel(   \
-        const Params* __restrict__ P, const uint32_t* __restrict__ work) {                                       \
-        const uint32_t w = work[blockIdx.x];                                                                     \
-        const Params p = P[w >> kWorkShift];                                                                     \
-        const int l = (int)(w & kWorkMask);                                                                      \
-        body(p, l % p.gx, l / p.gx);                                                                             \
-    }
-
 FGR_PASS(pixfield, PixParams, pixfieldBody, 256)
 
 // ---------------------------------------------------------------------------------------------------------- launches
 
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
-
-hipError_t launchLumaRgba8(const LumaParams& p, hipStream_t s) {
-    LumaParams q = p;
-    q.gx = cdiv(p.fw, 16);
-    hipLaunchKernelGGL(luma_rgba8_kernel, dim3((unsigned)q.gx, (unsigned)cdiv(p.fh, 16)), dim3(256), 0, s, q);
-    return hipGetLastError();
-}
-
-hipError_t launchDown(const DownParams& p, hipStream_t s) {
-    DownParams q = p;
-    q.gx = cdiv(p.w, 16);
-    hipLaunchKernelGGL(down_kernel, dim3((unsigned)q.gx, (unsigned)cdiv(p.h, 16)), dim3(256), 0, s, q);
-    return hipGetLastError();
-}
-
-hipError_t launchMedhist(const MedParams& p, hipStream_t s) {
-    MedParams q = p;
-    q.gx = 1;
-    hipLaunchKernelGGL(medhist_kernel, dim3(1), dim3(1024), 0, s, q);
-    return hipGetLastError();
-}
-
-hipError_t launchSearch(const SearchParams& p, hipStream_t s) {
-    SearchParams q = p;
-    q.gx = cdiv(p.bw * p.bh, 4);
-    const dim3 g((unsigned)q.gx);
-    if (p.R <= 2) hipLaunchKernelGGL(search2_kernel, g, dim3(256), 0, s, q);
-    else if (p.R <= 6) hipLaunchKernelGGL(search6_kernel, g, dim3(256), 0, s, q);
-    else hipLaunchKernelGGL(search12_kernel, g, dim3(256), 0, s, q);
-    return hipGetLastError();
-}
-
-hipError_t launchMedian3(const Median3Params& p, hipStream_t s) {
-    Median3Params q = p;
-    q.gx = cdiv(p.bw, 16);
-    hipLaunchKernelGGL(median3_kernel, dim3((unsigned)q.gx, (unsigned)cdiv(p.bh, 16)), dim3(256), 0, s, q);
-    return hipGetLastError();
-}
-
-hipError_t launchSubpix(const SubpixParams& p, hipStream_t s) {
-    SubpixParams q = p;
-    q.gx = cdiv(p.bw * p.bh, kSubpixBlocks);
-    hipLaunchKernelGGL(subpix_kernel, dim3((unsigned)q.gx), dim3(64), 0, s, q);
-    return hipGetLastError();
-}
-
-hipError_t launchPixfield(const PixParams& p, hipStream_t s) {
-    PixParams q = p;
-    q.gx = cdiv(p.w, 32);
-    hipLaunchKernelGGL(pixfield_kernel, dim3((unsigned)q.gx, (unsigned)cdiv(p.h, 32)), dim3(256), 0, s, q);
-    return hipGetLastError();
-}
 
 // one plan op of one pair: grid gx x gy of the per-pair kernel
 hipError_t launchSingle(int pass, const void* params, int gx, int gy, hipStream_t s) {

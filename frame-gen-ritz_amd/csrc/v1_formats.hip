@@ -1,0 +1,416 @@
+// V1 luma and synth kernels of the formats other than RGBA8 (gpu/shaders/format.glsl:29-106, luma.comp:9-27,
+// motion_v1.glsl:85-144, resolve.comp:48-83): Rgb10 (A2B10G10R10 / A2R10G10B10 UNORM), RgbaF16 (half floats, encoded
+// or linear light) and P010 (Y plane of 16-bit words holding 10-bit codes, half-resolution CbCr plane).
+//
+// The reference reads these formats through the texture filter and leaves the luma unrounded, which no other GPU
+// reproduces bit for bit. Here every step is fixed: the luma is a fixed-point value in 1/16 level; the warp taps are
+// exact integer bilinear sums on the 1/32-px map of the RGBA8 path (the P010 chroma on a 1/64-texel map of its own
+// plane); the mismatch, blend and stores are float32 expressions in the reference's order. The formulas, one per
+// format, are stated at each function. A launch serves pairs of any of the three formats (the format is a parameter,
+// uniform per workgroup), so a mixed batch needs one luma and one synth launch per pass for all of them.
+#include <hip/hip_runtime.h>
+#include "v1_device.hpp"
+#include "v1_launch.hpp"
+#include "v1_synth_common.hpp"
+
+namespace fgr {
+
+__device__ __forceinline__ int floorDiv(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+
+__device__ __forceinline__ float halfBits(uint32_t h) {
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(h & 0xFFFFu));
+}
+
+__device__ __forceinline__ uint32_t toHalfBits(float v) {   // round to nearest even
+    return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)v);
+}
+
+// ---------------------------------------------------------------------------------------------------------- luma
+
+// Luma of pixel (x, y) in units of 1/16 level (luma.comp:9-27 y8() of each format, fixed point):
+//   Rgb10   round((4899 R + 9617 G + 1868 B) * 255 / (1023 * 16384) * 16) on 10-bit codes (halves up)
+//   RgbaF16 (4899 L_r + 9617 L_g + 1868 L_b + 8192) >> 14, L = the 1/16-level metric table of each half value
+//   P010    round((4 code - 256) * 255 / 219) at limited range (16 (code / 4 - 16) * 255 / 219), 4 code at full range
+__device__ __forceinline__ int luma16(const LumaParams& P, const FrameRef& f, int x, int y) {
+    const uint8_t* row = f.p + (int64_t)y * f.pitch;
+    if (P.fmt == kRgb10) {
+        const uint32_t w = ((const uint32_t*)row)[x];
+        uint32_t r = w & 1023u, g = (w >> 10) & 1023u, b = (w >> 20) & 1023u;
+        if (P.bgra) {
+            const uint32_t t = r;
+            r = b;
+            b = t;
+        }
+        const uint32_t n = r * 4899u + g * 9617u + b * 1868u;   // <= 16760832: n * 255 + 523776 < 2^32
+        return (int)((n * 255u + 523776u) / 1047552u);
+    }
+    if (P.fmt == kF16) {
+        const uint2 q = *(const uint2*)(row + 8 * (int64_t)x);
+        const int lr = P.lut[q.x & 0xFFFFu], lg = P.lut[q.x >> 16], lb = P.lut[q.y & 0xFFFFu];
+        return (lr * 4899 + lg * 9617 + lb * 1868 + 8192) >> 14;   // |L| <= 65535: no overflow
+    }
+    const int c = ((const uint16_t*)row)[x] >> 6;
+    return P.yFull ? 4 * c : floorDiv((4 * c - 256) * 510 + 219, 438);
+}
+
+// luma.comp:58-72 on the fixed-point luma: float32(sum of the valid pixels' y16) / float32(16 n). Block (0, 0) also
+// clears the estimate's counters (as the RGBA8 luma pass does).
+__device__ __forceinline__ void lumaFmtBody(const LumaParams& P, int bx, int by) {
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    if (bx == 0 && by == 0)
+        for (int i = threadIdx.x; i < kUWords; i += 256) P.u[i] = 0u;
+    const int ox = bx * 16 + tx, oy = by * 16 + ty;
+    if (ox >= P.fw || oy >= P.fh) return;
+    int sa = 0, sc = 0, n = 0;
+    for (int j = 0; j < P.S; j++) {
+        const int py = P.S * oy + j;
+        if (py >= P.H) break;
+        for (int i = 0; i < P.S; i++) {
+            const int px = P.S * ox + i;
+            if (px >= P.W) break;
+            sa += luma16(P, P.a, px, py);
+            sc += luma16(P, P.c, px, py);
+            n++;
+        }
+    }
+    P.la[oy * P.fw + ox] = (float)sa / (float)(16 * n);
+    P.lc[oy * P.fw + ox] = (float)sc / (float)(16 * n);
+}
+
+// ---------------------------------------------------------------------------------------------------------- taps
+
+// The clamped taps (BORDER_REPLICATE) and integer weights of a bilinear sample at map (X, Y) in 1/2^SH texel units
+template <int SH>
+struct Taps {
+    int x0, x1, y0, y1;
+    int w00, w01, w10, w11;
+    __device__ __forceinline__ Taps(int X, int Y, int W, int H) {
+        constexpr int unit = 1 << SH;
+        const int ix = X >> SH, iy = Y >> SH, fx = X & (unit - 1), fy = Y & (unit - 1);
+        x0 = med3i(ix, 0, W - 1);
+        x1 = med3i(ix + 1, 0, W - 1);
+        y0 = med3i(iy, 0, H - 1);
+        y1 = med3i(iy + 1, 0, H - 1);
+        const int gx = unit - fx, gy = unit - fy;
+        w00 = gx * gy;
+        w01 = fx * gy;
+        w10 = gx * fy;
+        w11 = fx * fy;
+    }
+};
+
+__device__ __forceinline__ int isum(const Taps<5>& t, int p00, int p01, int p10, int p11) {
+    return t.w00 * p00 + t.w01 * p01 + t.w10 * p10 + t.w11 * p11;
+}
+
+// Rgb10: exact bilinear sums of the three 10-bit channels (code * 1024 units)
+__device__ __forceinline__ void rgb10Sums(const FrameRef& f, int W, int H, int X, int Y, int s[3]) {
+    const Taps<5> t(X, Y, W, H);
+    const uint32_t* r0 = (const uint32_t*)(f.p + (int64_t)t.y0 * f.pitch);
+    const uint32_t* r1 = (const uint32_t*)(f.p + (int64_t)t.y1 * f.pitch);
+    const uint32_t p00 = r0[t.x0], p01 = r0[t.x1], p10 = r1[t.x0], p11 = r1[t.x1];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const int sh = 10 * c;
+        s[c] = isum(t, (int)((p00 >> sh) & 1023u), (int)((p01 >> sh) & 1023u), (int)((p10 >> sh) & 1023u),
+                    (int)((p11 >> sh) & 1023u));
+    }
+}
+
+// RgbaF16: the four float32 products w * v (exact) summed in the order p00, p01, p10, p11, times 1/1024
+__device__ __forceinline__ void f16Warp(const FrameRef& f, int W, int H, int X, int Y, float v[3]) {
+    const Taps<5> t(X, Y, W, H);
+    const uint8_t* r0 = f.p + (int64_t)t.y0 * f.pitch;
+    const uint8_t* r1 = f.p + (int64_t)t.y1 * f.pitch;
+    const uint2 q00 = *(const uint2*)(r0 + 8 * t.x0), q01 = *(const uint2*)(r0 + 8 * t.x1);
+    const uint2 q10 = *(const uint2*)(r1 + 8 * t.x0), q11 = *(const uint2*)(r1 + 8 * t.x1);
+    const float w00 = (float)t.w00, w01 = (float)t.w01, w10 = (float)t.w10, w11 = (float)t.w11;
+    const uint32_t a[3] = {q00.x, q00.x >> 16, q00.y}, b[3] = {q01.x, q01.x >> 16, q01.y};
+    const uint32_t c[3] = {q10.x, q10.x >> 16, q10.y}, d[3] = {q11.x, q11.x >> 16, q11.y};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        float s = w00 * halfBits(a[k]) + w01 * halfBits(b[k]);
+        s = s + w10 * halfBits(c[k]);
+        s = s + w11 * halfBits(d[k]);
+        v[k] = s * (1.0f / 1024.0f);
+    }
+}
+
+// P010 luma: exact bilinear sum of the 10-bit Y codes (level = sum / 4096)
+__device__ __forceinline__ int p010YSum(const FrameRef& f, int W, int H, int X, int Y) {
+    const Taps<5> t(X, Y, W, H);
+    const uint16_t* r0 = (const uint16_t*)(f.p + (int64_t)t.y0 * f.pitch);
+    const uint16_t* r1 = (const uint16_t*)(f.p + (int64_t)t.y1 * f.pitch);
+    return isum(t, r0[t.x0] >> 6, r0[t.x1] >> 6, r1[t.x0] >> 6, r1[t.x1] >> 6);
+}
+
+// P010 chroma at luma map (X, Y) (1/32 px): with centre-sited chroma on a half-resolution plane the point is
+// (X - 16) / 64 chroma texels; exact bilinear sums of the 10-bit Cb and Cr codes (level = sum / 16384)
+__device__ __forceinline__ void p010CSums(const FrameRef& f, int cw, int ch, int X, int Y, int& cb, int& cr) {
+    const Taps<6> t(X - 16, Y - 16, cw, ch);
+    const uint32_t* r0 = (const uint32_t*)(f.c + (int64_t)t.y0 * f.cpitch);
+    const uint32_t* r1 = (const uint32_t*)(f.c + (int64_t)t.y1 * f.cpitch);
+    const uint32_t p00 = r0[t.x0], p01 = r0[t.x1], p10 = r1[t.x0], p11 = r1[t.x1];
+    cb = t.w00 * (int)((p00 & 0xFFFFu) >> 6) + t.w01 * (int)((p01 & 0xFFFFu) >> 6) + t.w10 * (int)((p10 & 0xFFFFu) >> 6) +
+         t.w11 * (int)((p11 & 0xFFFFu) >> 6);
+    cr = t.w00 * (int)(p00 >> 22) + t.w01 * (int)(p01 >> 22) + t.w10 * (int)(p10 >> 22) + t.w11 * (int)(p11 >> 22);
+}
+
+// fgYuvDiff format.glsl:98-100: a (Y, Cb, Cr) level difference as the R'G'B' difference, the largest magnitude
+__device__ __forceinline__ float yuvDiffMax(const SynthParams& P, float dY, float dCb, float dCr) {
+    const float ys = P.yuv[0], rcr = P.yuv[1], bcb = P.yuv[2], gcb = P.yuv[3], gcr = P.yuv[4];
+    const float R = ys * dY + rcr * dCr;
+    float G = ys * dY + gcb * dCb;
+    G = G + gcr * dCr;
+    const float B = ys * dY + bcb * dCb;
+    return fmaxf(fabsf(R), fmaxf(fabsf(G), fabsf(B)));
+}
+
+// The 1/32 maps of a and c at pixel (x, y) (as warpPair: 32 p -+ (32 S k) r)
+struct Maps {
+    int XA, YA, XC, YC;
+};
+
+__device__ __forceinline__ Maps mapsAt(float kA32, float kC32, float x32, float y32, float2 r) {
+    Maps m;
+    m.XA = (int)rintf(x32 - kA32 * r.x);
+    m.YA = (int)rintf(y32 - kA32 * r.y);
+    m.XC = (int)rintf(x32 + kC32 * r.x);
+    m.YC = (int)rintf(y32 + kC32 * r.y);
+    return m;
+}
+
+// Mismatch e in 0..255 (error.comp / mismatch() with format.glsl's FG_DIFF and FG_PACKE):
+//   Rgb10   uint(min(max_c |wa - wc|, 255)), levels w = float32(sum) * float32(255 / 1047552)
+//   RgbaF16 min(max_c |L(half(wa)) - L(half(wc))| >> 4, 255)
+//   P010    uint(min(max |fgYuvDiff(Y, Cb, Cr difference)|, 255))
+template <int F>
+__device__ __forceinline__ int mismatchFmt(const SynthParams& P, const Maps& m) {
+    if (F == kRgb10) {
+        int sa[3], sc[3];
+        rgb10Sums(P.a, P.W, P.H, m.XA, m.YA, sa);
+        rgb10Sums(P.c, P.W, P.H, m.XC, m.YC, sc);
+        const float k = (float)(255.0 / 1047552.0);
+        float d = 0.0f;
+#pragma unroll
+        for (int c = 0; c < 3; c++) d = fmaxf(d, fabsf((float)sa[c] * k - (float)sc[c] * k));
+        return (int)fminf(d, 255.0f);
+    }
+    if (F == kF16) {
+        float va[3], vc[3];
+        f16Warp(P.a, P.W, P.H, m.XA, m.YA, va);
+        f16Warp(P.c, P.W, P.H, m.XC, m.YC, vc);
+        int d = 0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) d = max(d, abs(P.lut[toHalfBits(va[c])] - P.lut[toHalfBits(vc[c])]));
+        return min(d >> 4, 255);
+    }
+    const int ya = p010YSum(P.a, P.W, P.H, m.XA, m.YA), yc = p010YSum(P.c, P.W, P.H, m.XC, m.YC);
+    int cba, cra, cbc, crc;
+    p010CSums(P.a, P.cw, P.ch, m.XA, m.YA, cba, cra);
+    p010CSums(P.c, P.cw, P.ch, m.XC, m.YC, cbc, crc);
+    const float dY = (float)(ya - yc) * (1.0f / 4096.0f);
+    const float dCb = (float)(cba - cbc) * (1.0f / 16384.0f), dCr = (float)(cra - crc) * (1.0f / 16384.0f);
+    return (int)fminf(yuvDiffMax(P, dY, dCb, dCr), 255.0f);
+}
+
+// resolve.comp:73-83 blend in float32: m = 0.5 (wa + wc) at t = 1/2, else wA wa + wC wc; o = m (1 - u) + near u
+__device__ __forceinline__ float blendF(const SynthParams& P, float wa, float wc, float nv, float u) {
+    const float m = P.mid ? 0.5f * (wa + wc) : P.wA * wa + P.wC * wc;
+    return m * (1.0f - u) + nv * u;
+}
+
+// One output pixel (x, y) with trust u:
+//   Rgb10   code = floor(clamp(o, 0, 255) * float32(1023 / 255) + 0.5), alpha of the nearer frame
+//   RgbaF16 half(o), alpha of the nearer frame
+//   P010    Y code = clamp(floor(4 o + 0.5), 0, 1023); at an even-even pixel also the CbCr of its 2x2 block, warped at
+//           the block centre p + 1/2 (warpChroma motion_v1.glsl:134-144) and blended with this pixel's trust
+template <int F>
+__device__ __forceinline__ void resolveFmt(const SynthParams& P, float kA32, float kC32, int x, int y, float2 r, float u) {
+    const FrameRef& nf = P.nearC ? P.c : P.a;
+    const Maps m = mapsAt(kA32, kC32, (float)(32 * x), (float)(32 * y), r);
+    if (F == kRgb10) {
+        int sa[3], sc[3];
+        rgb10Sums(P.a, P.W, P.H, m.XA, m.YA, sa);
+        rgb10Sums(P.c, P.W, P.H, m.XC, m.YC, sc);
+        const uint32_t nw = *(const uint32_t*)(nf.p + (int64_t)y * nf.pitch + 4 * (int64_t)x);
+        const float k = (float)(255.0 / 1047552.0), kn = (float)(255.0 / 1023.0), ko = (float)(1023.0 / 255.0);
+        uint32_t o = nw & 0xC0000000u;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const float nv = (float)((nw >> (10 * c)) & 1023u) * kn;
+            const float v = blendF(P, (float)sa[c] * k, (float)sc[c] * k, nv, u);
+            o |= (uint32_t)(int)floorf(fminf(fmaxf(v, 0.0f), 255.0f) * ko + 0.5f) << (10 * c);
+        }
+        *(uint32_t*)(P.out + (int64_t)y * P.opitch + 4 * (int64_t)x) = o;
+        return;
+    }
+    if (F == kF16) {
+        float va[3], vc[3];
+        f16Warp(P.a, P.W, P.H, m.XA, m.YA, va);
+        f16Warp(P.c, P.W, P.H, m.XC, m.YC, vc);
+        const uint2 nq = *(const uint2*)(nf.p + (int64_t)y * nf.pitch + 8 * (int64_t)x);
+        const float n0 = halfBits(nq.x), n1 = halfBits(nq.x >> 16), n2 = halfBits(nq.y);
+        uint2 o;
+        o.x = toHalfBits(blendF(P, va[0], vc[0], n0, u)) | (toHalfBits(blendF(P, va[1], vc[1], n1, u)) << 16);
+        o.y = toHalfBits(blendF(P, va[2], vc[2], n2, u)) | (nq.y & 0xFFFF0000u);
+        *(uint2*)(P.out + (int64_t)y * P.opitch + 8 * (int64_t)x) = o;
+        return;
+    }
+    const int ya = p010YSum(P.a, P.W, P.H, m.XA, m.YA), yc = p010YSum(P.c, P.W, P.H, m.XC, m.YC);
+    const float ny = (float)(((const uint16_t*)(nf.p + (int64_t)y * nf.pitch))[x] >> 6) * 0.25f;
+    const float oy = blendF(P, (float)ya * (1.0f / 4096.0f), (float)yc * (1.0f / 4096.0f), ny, u);
+    const int code = med3i((int)floorf(oy * 4.0f + 0.5f), 0, 1023);
+    ((uint16_t*)(P.out + (int64_t)y * P.opitch))[x] = (uint16_t)(code << 6);
+    if (((x | y) & 1) == 0) {
+        const Maps b = mapsAt(kA32, kC32, (float)(32 * x + 16), (float)(32 * y + 16), r);
+        int cba, cra, cbc, crc;
+        p010CSums(P.a, P.cw, P.ch, b.XA, b.YA, cba, cra);
+        p010CSums(P.c, P.cw, P.ch, b.XC, b.YC, cbc, crc);
+        const int cx = x >> 1, cy = y >> 1;
+        const uint32_t nc = *(const uint32_t*)(nf.c + (int64_t)cy * nf.cpitch + 4 * (int64_t)cx);
+        const float k = 1.0f / 16384.0f;
+        const float ob = blendF(P, (float)cba * k, (float)cbc * k, (float)((nc & 0xFFFFu) >> 6) * 0.25f, u);
+        const float orr = blendF(P, (float)cra * k, (float)crc * k, (float)(nc >> 22) * 0.25f, u);
+        const uint32_t cb = (uint32_t)med3i((int)floorf(ob * 4.0f + 0.5f), 0, 1023);
+        const uint32_t cr = (uint32_t)med3i((int)floorf(orr * 4.0f + 0.5f), 0, 1023);
+        *(uint32_t*)(P.outc + (int64_t)cy * P.ocpitch + 4 * (int64_t)cx) = (cb << 6) | (cr << 22);
+    }
+}
+
+// the nearer frame's pixel (x, y) into the output (scene cut): plane 0, and the chroma of an even-even pixel (P010)
+template <int F>
+__device__ __forceinline__ void copyNear(const SynthParams& P, int x, int y) {
+    const FrameRef& nf = P.nearC ? P.c : P.a;
+    if (F == kRgb10) {
+        *(uint32_t*)(P.out + (int64_t)y * P.opitch + 4 * (int64_t)x) = ld32(nf, x, y);
+    } else if (F == kF16) {
+        *(uint2*)(P.out + (int64_t)y * P.opitch + 8 * (int64_t)x) = *(const uint2*)(nf.p + (int64_t)y * nf.pitch + 8 * (int64_t)x);
+    } else {
+        ((uint16_t*)(P.out + (int64_t)y * P.opitch))[x] = ((const uint16_t*)(nf.p + (int64_t)y * nf.pitch))[x];
+        if (((x | y) & 1) == 0)
+            *(uint32_t*)(P.outc + (int64_t)(y >> 1) * P.ocpitch + 4 * (int64_t)(x >> 1)) =
+                *(const uint32_t*)(nf.c + (int64_t)(y >> 1) * nf.cpitch + 4 * (int64_t)(x >> 1));
+    }
+}
+
+// One 64 x 32 output tile (the RGBA8 kernel's structure): field tile, mismatch of the tile and its 3-pixel apron into
+// LDS, 7x7 box, trust, untrusted count, then per pixel the warps again and the blend (the warped values of these
+// formats are three 20-bit sums or floats per frame, too many to keep in registers for 8 pixels a thread).
+template <int S, int F>
+__device__ __forceinline__ void synthFmtTile(const SynthParams& P, int tileX, int tileY, float2* ft, float2* hrow,
+                                             int* es, uint16_t* hs, uint32_t* cnt) {
+    const int tid = threadIdx.x;
+    const int x0 = tileX * TW, y0 = tileY * TH;
+    const int cx = tid & (TW - 1), ry = tid >> 6;
+    if (sceneCut(P)) {
+        for (int k = 0; k < TH / 4; k++) {
+            const int x = x0 + cx, y = y0 + ry + 4 * k;
+            if (x < P.W && y < P.H) copyNear<F>(P, x, y);
+        }
+        return;
+    }
+    if (tid == 0) *cnt = 0u;
+    const int ty0 = loadFieldTile<S>(P, x0, y0, ft, hrow);
+    const float kA32 = (float)(32 * S) * (2.0f * P.wC), kC32 = (float)(32 * S) * (2.0f * P.wA);
+    for (int i = tid; i < EW * EH; i += 256) {
+        const int ey = i / EW, ex = i - ey * EW;
+        const int x = med3i(x0 - AP + ex, 0, P.W - 1), y = med3i(y0 - AP + ey, 0, P.H - 1);
+        const float2 r = fieldAtRegion<S>(P, hrow, ty0, ex, y);
+        es[i] = mismatchFmt<F>(P, mapsAt(kA32, kC32, (float)(32 * x), (float)(32 * y), r));
+    }
+    __syncthreads();
+    for (int i = tid; i < EH * TW; i += 256) {
+        const int rr = i >> 6, c = i & (TW - 1);
+        const int* e = &es[rr * EW + c];
+        hs[i] = (uint16_t)(e[0] + e[1] + e[2] + e[3] + e[4] + e[5] + e[6]);
+    }
+    __syncthreads();
+    uint32_t mine = 0u;
+    for (int k = 0; k < TH / 4; k++) {
+        const int ly = ry + 4 * k, x = x0 + cx, y = y0 + ly;
+        if (x >= P.W || y >= P.H) continue;
+        const uint16_t* h = &hs[ly * TW + cx];
+        const int S49 = (int)h[0] + h[TW] + h[2 * TW] + h[3 * TW] + h[4 * TW] + h[5 * TW] + h[6 * TW];
+        mine += S49 > P.sHalf ? 1u : 0u;
+        resolveFmt<F>(P, kA32, kC32, x, y, fieldAtRegion<S>(P, hrow, ty0, cx + AP, y), trustOf(P, S49));
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) mine += __shfl_xor(mine, o, 64);
+    if ((tid & 63) == 0 && mine) atomicAdd(cnt, mine);
+    __syncthreads();
+    if (tid == 0 && *cnt) atomicAdd(P.count + ((tileY * P.gx + tileX) & (kSlots - 1)), *cnt);
+}
+
+template <int S>
+__device__ __forceinline__ void synthFmtBody(const SynthParams& P, int bx, int by) {
+    constexpr int FTW = SynthTile<S>::FTW, FTH = SynthTile<S>::FTH;
+    __shared__ float2 ft[FTW * FTH];
+    __shared__ float2 hrow[FTH * EW];
+    __shared__ int es[EH * EW];
+    __shared__ uint16_t hs[EH * TW];
+    __shared__ uint32_t cnt;
+    if (P.fmt == kRgb10) synthFmtTile<S, kRgb10>(P, bx, by, ft, hrow, es, hs, &cnt);
+    else if (P.fmt == kF16) synthFmtTile<S, kF16>(P, bx, by, ft, hrow, es, hs, &cnt);
+    else synthFmtTile<S, kP010>(P, bx, by, ft, hrow, es, hs, &cnt);
+}
+
+FGR_PASS(luma_fmt, LumaParams, lumaFmtBody, 256)
+FGR_PASS(synth_fmt_s1, SynthParams, synthFmtBody<1>, 256)
+FGR_PASS(synth_fmt_s2, SynthParams, synthFmtBody<2>, 256)
+FGR_PASS(synth_fmt_s4, SynthParams, synthFmtBody<4>, 256)
+
+// the packed mismatch e of the latest synth (debug maps), 4 pixels per uint32, low byte first
+__global__ __launch_bounds__(256) void debug_e_fmt_kernel(SynthParams P, uint32_t* dst, int eStride) {
+    const int gx = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (gx >= eStride || y >= P.H) return;
+    const float kA32 = 32.0f * (2.0f * P.wC), kC32 = 32.0f * (2.0f * P.wA);
+    uint32_t packed = 0u;
+    for (int i = 0; i < 4; i++) {
+        const int x = gx * 4 + i;
+        if (x >= P.W) continue;
+        // hv itself (not r): the per-pixel halfVec, whose factor S is folded into the map constants as 1
+        const float2 hv = halfVec(P.field, x, y, P.S, P.fw, P.fh);
+        const Maps m = mapsAt(kA32, kC32, (float)(32 * x), (float)(32 * y), hv);
+        int e;
+        if (P.fmt == kRgb10) e = mismatchFmt<kRgb10>(P, m);
+        else if (P.fmt == kF16) e = mismatchFmt<kF16>(P, m);
+        else e = mismatchFmt<kP010>(P, m);
+        packed |= (uint32_t)e << (8 * i);
+    }
+    dst[(int64_t)y * eStride + gx] = packed;
+}
+
+static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+hipError_t launchFmtSingle(int pass, const void* params, int gx, int gy, hipStream_t s) {
+    const dim3 g((unsigned)gx, (unsigned)gy);
+    switch (pass) {
+    case kPassLumaFmt: hipLaunchKernelGGL(luma_fmt_kernel, g, dim3(256), 0, s, *(const LumaParams*)params); break;
+    case kPassSynthFmtS1: hipLaunchKernelGGL(synth_fmt_s1_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
+    case kPassSynthFmtS2: hipLaunchKernelGGL(synth_fmt_s2_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
+    case kPassSynthFmtS4: hipLaunchKernelGGL(synth_fmt_s4_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launchFmtBatch(int pass, const void* params, const uint32_t* work, int n, hipStream_t s) {
+    const dim3 g((unsigned)n);
+    switch (pass) {
+    case kPassLumaFmt: hipLaunchKernelGGL(luma_fmt_batch_kernel, g, dim3(256), 0, s, (const LumaParams*)params, work); break;
+    case kPassSynthFmtS1: hipLaunchKernelGGL(synth_fmt_s1_batch_kernel, g, dim3(256), 0, s, (const SynthParams*)params, work); break;
+    case kPassSynthFmtS2: hipLaunchKernelGGL(synth_fmt_s2_batch_kernel, g, dim3(256), 0, s, (const SynthParams*)params, work); break;
+    case kPassSynthFmtS4: hipLaunchKernelGGL(synth_fmt_s4_batch_kernel, g, dim3(256), 0, s, (const SynthParams*)params, work); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launchDebugEFmt(const SynthParams& p, uint32_t* dst, int eStride, hipStream_t s) {
+    hipLaunchKernelGGL(debug_e_fmt_kernel, dim3((unsigned)cdiv(eStride, 16), (unsigned)cdiv(p.H, 16)), dim3(256), 0, s,
+                       p, dst, eStride);
+    return hipGetLastError();
+}
+
+}  // namespace fgr

@@ -8,30 +8,24 @@ namespace fgr {
 // passes of the batched launches (launchBatch)
 enum Pass : int {
     kPassLuma = 0, kPassDown, kPassMedhist, kPassSearch2, kPassSearch6, kPassSearch12, kPassMedian3, kPassSubpix,
-    kPassPixfield, kPassSynthRgba8S1, kPassSynthRgba8S2, kPassSynthRgba8S4, kPassFallback
+    kPassPixfield, kPassSynthRgba8S1, kPassSynthRgba8S2, kPassSynthRgba8S4, kPassFallback,
+    kPassLumaFmt, kPassSynthFmtS1, kPassSynthFmtS2, kPassSynthFmtS4   // Rgb10 / RgbaF16 / P010 (v1_formats.hip)
 };
 hipError_t launchBatch(int pass, const void* params, const uint32_t* work, int n, hipStream_t s);
 hipError_t launchSingle(int pass, const void* params, int gx, int gy, hipStream_t s);
 hipError_t launchSynthSingle(int pass, const void* params, int gx, int gy, hipStream_t s);
 hipError_t launchSynthBatch(int pass, const void* params, const uint32_t* work, int n, hipStream_t s);
+hipError_t launchFmtSingle(int pass, const void* params, int gx, int gy, hipStream_t s);
+hipError_t launchFmtBatch(int pass, const void* params, const uint32_t* work, int n, hipStream_t s);
+hipError_t launchDebugEFmt(const SynthParams& p, uint32_t* dst, int eStride, hipStream_t s);
 constexpr int kSubpixBlocks = 7;   // blocks per subpix workgroup
 int synthTilesX(int W);
 int synthTilesY(int H);
 int fallbackTilesX(int W);
 int fallbackTilesY(int H);
 
-hipError_t launchLuma(const LumaParams& p, int format, hipStream_t s);
-hipError_t launchLumaRgba8(const LumaParams& p, hipStream_t s);
-hipError_t launchDown(const DownParams& p, hipStream_t s);
-hipError_t launchMedhist(const MedParams& p, hipStream_t s);
-hipError_t launchSearch(const SearchParams& p, hipStream_t s);
-hipError_t launchMedian3(const Median3Params& p, hipStream_t s);
-hipError_t launchSubpix(const SubpixParams& p, hipStream_t s);
-hipError_t launchPixfield(const PixParams& p, hipStream_t s);
 
 hipError_t launchLookup(const LookupParams& p, hipStream_t s);
-hipError_t launchSynth(const SynthParams& p, int format, hipStream_t s);
-hipError_t launchFallback(const FallbackParams& p, int format, hipStream_t s);
 hipError_t launchDebugCounters(const uint32_t* count, const uint32_t* sad, uint32_t* dst, hipStream_t s);
 hipError_t launchDebugE(const SynthParams& p, uint32_t* dst, int eStride, int format, hipStream_t s);
 

@@ -42,9 +42,9 @@ struct LumaParams {
     float* lc;
     uint32_t* u;        // counters, cleared by block (0, 0)
     int bgra;
-    // format conversion (non-Rgba8): per-format constants, see luma kernels
-    const float* lut;   // RgbaF16 linear: 65536-entry half -> level table (null otherwise)
-    float yBlack, yGain;
+    int fmt;            // Fmt (the Rgba8 kernel ignores it)
+    const int* lut;     // RgbaF16: metric level of every half value, in 1/16 level (65536 entries)
+    int yFull;          // P010: full-range codes (else limited range, black 64, white 940)
 };
 
 struct DownParams {
@@ -120,8 +120,10 @@ struct SynthParams {
     float wA, wC;             // 1 - t, t (float, folded in double)
     int mid;                  // t == 0.5
     int nearC;                // t > 0.5
-    const float* lut;         // RgbaF16 linear: half -> level table
-    float yuv[8];             // P010: ys, rcr, bcb, gcb, gcr, black, scale
+    int fmt;                  // Fmt
+    const int* lut;           // RgbaF16: metric level of every half value, in 1/16 level (65536 entries)
+    float yuv[8];             // P010: ys, rcr, bcb, gcb, gcr (format.glsl YU_YS, YU_RCR, YU_BCB, YU_GCB, YU_GCR)
+    int cw, ch;               // P010: chroma plane size
 };
 
 struct FallbackParams {

@@ -10,23 +10,9 @@
 #include <hip/hip_runtime.h>
 #include "v1_device.hpp"
 #include "v1_launch.hpp"
+#include "v1_synth_common.hpp"
 
 namespace fgr {
-
-// Tile of the fused synth kernel: 64 x 32 output pixels per 256-thread workgroup (8 per thread), a 3-pixel apron for
-// the 7x7 box (70 x 38 mismatch values, 1.30 per output pixel).
-constexpr int TW = 64, TH = 32, AP = 3, EW = TW + 2 * AP, EH = TH + 2 * AP, NAPRON = EW * EH - TW * TH;
-
-struct alignas(4) U32x2 {
-    uint32_t x, y;
-};
-
-__device__ __forceinline__ int med3i(int v, int lo, int hi) { return min(max(v, lo), hi); }
-__device__ __forceinline__ int byteOf(uint32_t v, int c) { return (int)((v >> (8 * c)) & 255u); }
-
-__device__ __forceinline__ uint32_t ld32(const FrameRef& f, int x, int y) {
-    return *(const uint32_t*)(f.p + (int64_t)y * f.pitch + 4 * (int64_t)x);
-}
 
 // cv2.remap INTER_LINEAR / BORDER_REPLICATE of an 8-bit RGB frame (interp.py:214), at the map position already scaled
 // by 32 (X = 32 px, Y = 32 py, float): map rounded to 1/32 px (half even), integer weights (32-fx)(32-fy), fx(32-fy),
@@ -59,10 +45,6 @@ __device__ __forceinline__ Rgb warpRgba8(const uint8_t* base, uint32_t pitch, in
 }
 
 __device__ __forceinline__ uint32_t packRgb(const Rgb& v) { return v.r | (v.g << 8) | (v.b << 16); }
-
-struct alignas(4) U32x4 {
-    uint32_t x, y, z, w;
-};
 
 // Four horizontally adjacent pixels (x .. x+3, one row) warped with one interpolated field value r. When their maps
 // X_i = rint(32 (x + i) - k r.x) are 32 apart (same fractional part) and no tap column needs clamping, the four share
@@ -145,25 +127,6 @@ __device__ __forceinline__ void warpPairRgba8(const SynthParams& P, int x, int y
     wc = packRgb(c);
 }
 
-__device__ __forceinline__ bool sceneCut(const SynthParams& P) {
-    // fallback.comp:10-11: float(sum) * (1/256) / float(blocks) > sceneCutSad (the wavefront adds the partial sums)
-    const float meanSad = (float)slotSum(P.sadsum) * (1.0f / 256.0f) / (float)P.nblocks;
-    return meanSad > P.cutSad;
-}
-
-// interp.py:236-237 trust from the exact 7x7 sum S: e7 = float32(S * (1/49)), u = clip((e7 - lo) / (hi - lo), 0, 1).
-// The float32 quotient is taken as float(double(e7 - lo) * (1 / double(hi - lo))): with a 24-bit numerator and
-// denominator the quotient is never a rounding midpoint unless the denominator is a power of two (then both are
-// exact), and the double product is close enough to round the same way. u is monotone in S, so the host tabulates
-// where it is 0 and 1 and the kernel evaluates the ramp only in between.
-__device__ __forceinline__ float trustOf(const SynthParams& P, int S) {
-    if (S <= P.sZero) return 0.0f;
-    if (S >= P.sOne) return 1.0f;
-    const float e7 = (float)((double)S * (1.0 / 49.0));
-    const float u = (float)((double)(e7 - P.trust0) * P.rden);
-    return fminf(fmaxf(u, 0.0f), 1.0f);
-}
-
 // resolve.comp:73-82 (interp.py:241-246): m = 0.5 (wa + wc) at t = 0.5, else (1-t) wa + t wc; o = m (1 - u) + near u;
 // out = clamp(floor(o + 0.5), 0, 255); alpha from the nearer frame. Two exact shortcuts of the same float32 formula:
 // u = 1 gives o = near; u = 0 gives o = m, and when 4(1-t) and 4t are integers m + 0.5 is exact, so
@@ -193,33 +156,6 @@ __device__ __forceinline__ uint32_t blendRgba8(const SynthParams& P, uint32_t wa
     return r;
 }
 
-// lookup.comp:10-19 (interp.py:194-205): the field at time t at flow pixel (qx, qy): g <- pf(q - s g) twice
-__device__ __forceinline__ float2 lookupAt(const float2* pf, int fw, int fh, float s, int qx, int qy) {
-    float2 g = pf[qy * fw + qx];
-#pragma unroll
-    for (int k = 0; k < 2; k++) {
-        const float px = (float)qx - s * g.x, py = (float)qy - s * g.y;
-        g = sampleF(pf, px, py, fw, fh);
-    }
-    return g;
-}
-
-// Flow column of full-res pixel x: floor((x + 0.5) / S - 0.5) = floor((2x + 1 - S) / 2S), and its weight
-template <int S>
-struct FlowMap {
-    static constexpr int kShift = S == 1 ? 1 : (S == 2 ? 2 : 3);
-    static __device__ __forceinline__ int index(int x) { return (2 * x + 1 - S) >> kShift; }
-    static __device__ __forceinline__ float frac(int x) {
-        return (float)((2 * x + 1 - S) & (2 * S - 1)) * (1.0f / (float)(2 * S));
-    }
-};
-
-template <int S>
-struct SynthTile {
-    // field tile (flow res) covering the tile and its apron, and its rows interpolated to full-res columns
-    static constexpr int FTW = (EW + S - 1) / S + 3, FTH = (EH + S - 1) / S + 3;
-};
-
 // One 64 x 32 tile of the synthesised frame (passes 9-11 fused): field tile (+ lookup at t), mismatch of the tile and
 // its apron, 7x7 box, trust, untrusted count, blend. Scene cut: a copy of the nearer frame.
 //
@@ -246,31 +182,9 @@ __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, 
         return;
     }
     if (tid == 0) cnt = 0u;
-    // the field tile covering every pixel of the tile and its apron (clamped to the frame, then to S * fw - 1)
-    const int xlim = S * P.fw - 1, ylim = S * P.fh - 1;
-    const int xmin = min(max(x0 - AP, 0), xlim), ymin = min(max(y0 - AP, 0), ylim);
-    const int tx0 = FlowMap<S>::index(xmin), ty0 = FlowMap<S>::index(ymin);
-    for (int i = tid; i < FTW * FTH; i += 256) {
-        const int qx = med3i(tx0 + i % FTW, 0, P.fw - 1), qy = med3i(ty0 + i / FTW, 0, P.fh - 1);
-        ft[i] = P.lookup ? lookupAt(P.field, P.fw, P.fh, P.s, qx, qy) : P.field[qy * P.fw + qx];
-    }
-    __syncthreads();
-    // horizontal mixes at the full-res columns x0 - AP .. x0 + TW + AP - 1 (clamped as the pixels are)
-    for (int i = tid; i < FTH * EW; i += 256) {
-        const int j = i / EW, ex = i - j * EW;
-        const int x = min(med3i(x0 - AP + ex, 0, P.W - 1), xlim);
-        const float fx = FlowMap<S>::frac(x);
-        const float2* f = ft + j * FTW + (FlowMap<S>::index(x) - tx0);
-        hrow[i] = mix2(f[0], f[1], fx);
-    }
-    __syncthreads();
+    const int ylim = S * P.fh - 1;
+    const int ty0 = loadFieldTile<S>(P, x0, y0, ft, hrow);
     const float kA32 = (float)(32 * S) * (2.0f * P.wC), kC32 = (float)(32 * S) * (2.0f * P.wA);
-    // the interpolated field r (hv = S r) at region pixel (ex, ey) with frame pixel y
-    auto fieldAtRegion = [&](int ex, int y) {
-        const int yc = min(y, ylim);
-        const float2* h = hrow + (FlowMap<S>::index(yc) - ty0) * EW + ex;
-        return mix2(h[0], h[EW], FlowMap<S>::frac(yc));
-    };
     // interior pixels in runs of 4 (columns 4 rc .. 4 rc + 3, rows rr and rr + 16): warps kept in registers (packed)
     const int rc = tid & 15, rr = tid >> 4;
     uint32_t wa[8], wc[8];
@@ -323,7 +237,7 @@ __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, 
         }
         const int x = med3i(x0 - AP + ex, 0, P.W - 1), y = med3i(y0 - AP + ey, 0, P.H - 1);
         Rgb a, c;
-        warpPair(P, kA32, kC32, x, y, fieldAtRegion(ex, y), a, c);
+        warpPair(P, kA32, kC32, x, y, fieldAtRegion<S>(P, hrow, ty0, ex, y), a, c);
         es[ey * EW + ex] = emaxRgb(a, c);
     }
     __syncthreads();
@@ -442,15 +356,6 @@ hipError_t launchDebugCounters(const uint32_t* count, const uint32_t* sad, uint3
     return hipGetLastError();
 }
 
-#define FGR_PASS(name, Params, body, threads)                                                                    \
-    __global__ __launch_bounds__(threads) void name##_kernel(Params P) { body(P, (int)blockIdx.x, (int)blockIdx.y); } \
-    __global__ __launch_bounds__(threads) void name##_batch_kernel(const Params* __restrict__ P,                  \
-                                                                   const uint32_t* __restrict__ work) {          \
-        const uint32_t w = work[blockIdx.x];                                                                     \
-        const Params p = P[w >> kWorkShift];                                                                     \
-        const int l = (int)(w & kWorkMask);                                                                      \
-        body(p, l % p.gx, l / p.gx);                                                                             \
-    }
 
 FGR_PASS(synth_rgba8_s1, SynthParams, synthRgba8Body<1>, 256)
 FGR_PASS(synth_rgba8_s2, SynthParams, synthRgba8Body<2>, 256)
@@ -469,28 +374,6 @@ int synthTilesY(int H) { return cdiv(H, TH); }
 int fallbackTilesX(int W) { return cdiv(W, 64); }
 int fallbackTilesY(int H) { return cdiv(H, 64); }
 
-hipError_t launchSynth(const SynthParams& p, int format, hipStream_t s) {
-    SynthParams q = p;
-    q.gx = synthTilesX(p.W);
-    const dim3 g((unsigned)q.gx, (unsigned)synthTilesY(p.H));
-    if (format != kRgba8) return hipErrorInvalidValue;
-    switch (p.S) {
-    case 1: hipLaunchKernelGGL(synth_rgba8_s1_kernel, g, dim3(256), 0, s, q); break;
-    case 2: hipLaunchKernelGGL(synth_rgba8_s2_kernel, g, dim3(256), 0, s, q); break;
-    case 4: hipLaunchKernelGGL(synth_rgba8_s4_kernel, g, dim3(256), 0, s, q); break;
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
-}
-
-hipError_t launchFallback(const FallbackParams& p, int format, hipStream_t s) {
-    (void)format;
-    FallbackParams q = p;
-    q.gx = fallbackTilesX(p.W);
-    hipLaunchKernelGGL(fallback_kernel, dim3((unsigned)q.gx, (unsigned)fallbackTilesY(p.H)), dim3(256), 0, s, q);
-    return hipGetLastError();
-}
-
 hipError_t launchSynthSingle(int pass, const void* params, int gx, int gy, hipStream_t s) {
     const dim3 g((unsigned)gx, (unsigned)gy);
     switch (pass) {
@@ -498,7 +381,7 @@ hipError_t launchSynthSingle(int pass, const void* params, int gx, int gy, hipSt
     case kPassSynthRgba8S2: hipLaunchKernelGGL(synth_rgba8_s2_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
     case kPassSynthRgba8S4: hipLaunchKernelGGL(synth_rgba8_s4_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
     case kPassFallback: hipLaunchKernelGGL(fallback_kernel, g, dim3(256), 0, s, *(const FallbackParams*)params); break;
-    default: return hipErrorInvalidValue;
+    default: return launchFmtSingle(pass, params, gx, gy, s);
     }
     return hipGetLastError();
 }
@@ -510,7 +393,7 @@ hipError_t launchSynthBatch(int pass, const void* params, const uint32_t* work, 
     case kPassSynthRgba8S2: hipLaunchKernelGGL(synth_rgba8_s2_batch_kernel, g, dim3(256), 0, s, (const SynthParams*)params, work); break;
     case kPassSynthRgba8S4: hipLaunchKernelGGL(synth_rgba8_s4_batch_kernel, g, dim3(256), 0, s, (const SynthParams*)params, work); break;
     case kPassFallback: hipLaunchKernelGGL(fallback_batch_kernel, g, dim3(256), 0, s, (const FallbackParams*)params, work); break;
-    default: return hipErrorInvalidValue;
+    default: return launchFmtBatch(pass, params, work, n, s);
     }
     return hipGetLastError();
 }
@@ -519,16 +402,9 @@ hipError_t launchDebugE(const SynthParams& p, uint32_t* dst, int eStride, int fo
     const dim3 g((unsigned)cdiv(eStride, 16), (unsigned)cdiv(p.H, 16));
     switch (format) {
     case kRgba8: hipLaunchKernelGGL(debug_e_rgba8_kernel, g, dim3(256), 0, s, p, dst, eStride); break;
-    default: return hipErrorInvalidValue;
+    default: return launchDebugEFmt(p, dst, eStride, s);
     }
     return hipGetLastError();
-}
-
-hipError_t launchLuma(const LumaParams& p, int format, hipStream_t s) {
-    switch (format) {
-    case kRgba8: return launchLumaRgba8(p, s);
-    default: return hipErrorInvalidValue;
-    }
 }
 
 }  // namespace fgr

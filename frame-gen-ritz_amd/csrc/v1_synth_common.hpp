@@ -1,0 +1,109 @@
+// Pieces shared by the synth kernels of every format (v1_synth.hip for RGBA8, v1_formats.hip for the others): the
+// tile geometry, the scene-cut test, the trust ramp, the field lookup and the field tile of a workgroup.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "v1_device.hpp"
+
+namespace fgr {
+
+// Tile of the fused synth kernel: 64 x 32 output pixels per 256-thread workgroup (8 per thread), a 3-pixel apron for
+// the 7x7 box (70 x 38 mismatch values, 1.30 per output pixel).
+constexpr int TW = 64, TH = 32, AP = 3, EW = TW + 2 * AP, EH = TH + 2 * AP, NAPRON = EW * EH - TW * TH;
+
+struct alignas(4) U32x2 {
+    uint32_t x, y;
+};
+
+__device__ __forceinline__ int med3i(int v, int lo, int hi) { return min(max(v, lo), hi); }
+__device__ __forceinline__ int byteOf(uint32_t v, int c) { return (int)((v >> (8 * c)) & 255u); }
+
+__device__ __forceinline__ uint32_t ld32(const FrameRef& f, int x, int y) {
+    return *(const uint32_t*)(f.p + (int64_t)y * f.pitch + 4 * (int64_t)x);
+}
+
+struct alignas(4) U32x4 {
+    uint32_t x, y, z, w;
+};
+
+__device__ __forceinline__ bool sceneCut(const SynthParams& P) {
+    // fallback.comp:10-11: float(sum) * (1/256) / float(blocks) > sceneCutSad (the wavefront adds the partial sums)
+    const float meanSad = (float)slotSum(P.sadsum) * (1.0f / 256.0f) / (float)P.nblocks;
+    return meanSad > P.cutSad;
+}
+
+// interp.py:236-237 trust from the exact 7x7 sum S: e7 = float32(S * (1/49)), u = clip((e7 - lo) / (hi - lo), 0, 1).
+// The float32 quotient is taken as float(double(e7 - lo) * (1 / double(hi - lo))): with a 24-bit numerator and
+// denominator the quotient is never a rounding midpoint unless the denominator is a power of two (then both are
+// exact), and the double product is close enough to round the same way. u is monotone in S, so the host tabulates
+// where it is 0 and 1 and the kernel evaluates the ramp only in between.
+__device__ __forceinline__ float trustOf(const SynthParams& P, int S) {
+    if (S <= P.sZero) return 0.0f;
+    if (S >= P.sOne) return 1.0f;
+    const float e7 = (float)((double)S * (1.0 / 49.0));
+    const float u = (float)((double)(e7 - P.trust0) * P.rden);
+    return fminf(fmaxf(u, 0.0f), 1.0f);
+}
+
+// lookup.comp:10-19 (interp.py:194-205): the field at time t at flow pixel (qx, qy): g <- pf(q - s g) twice
+__device__ __forceinline__ float2 lookupAt(const float2* pf, int fw, int fh, float s, int qx, int qy) {
+    float2 g = pf[qy * fw + qx];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const float px = (float)qx - s * g.x, py = (float)qy - s * g.y;
+        g = sampleF(pf, px, py, fw, fh);
+    }
+    return g;
+}
+
+// Flow column of full-res pixel x: floor((x + 0.5) / S - 0.5) = floor((2x + 1 - S) / 2S), and its weight
+template <int S>
+struct FlowMap {
+    static constexpr int kShift = S == 1 ? 1 : (S == 2 ? 2 : 3);
+    static __device__ __forceinline__ int index(int x) { return (2 * x + 1 - S) >> kShift; }
+    static __device__ __forceinline__ float frac(int x) {
+        return (float)((2 * x + 1 - S) & (2 * S - 1)) * (1.0f / (float)(2 * S));
+    }
+};
+
+template <int S>
+struct SynthTile {
+    // field tile (flow res) covering the tile and its apron, and its rows interpolated to full-res columns
+    static constexpr int FTW = (EW + S - 1) / S + 3, FTH = (EH + S - 1) / S + 3;
+};
+
+// The field tile of an output tile at (x0, y0) and its apron (clamped to the frame, then to S * fw - 1), looked up at t
+// when P.lookup is set, and its rows interpolated at every full-res column x0 - AP .. x0 + TW + AP - 1 of the region
+// (hrow[j * EW + ex]; the first half of halfVec, motion_v1.glsl:73-84). Returns the flow row of ft's first row.
+// Ends with a barrier.
+template <int S>
+__device__ __forceinline__ int loadFieldTile(const SynthParams& P, int x0, int y0, float2* ft, float2* hrow) {
+    constexpr int FTW = SynthTile<S>::FTW, FTH = SynthTile<S>::FTH;
+    const int tid = threadIdx.x;
+    const int xlim = S * P.fw - 1, ylim = S * P.fh - 1;
+    const int xmin = min(max(x0 - AP, 0), xlim), ymin = min(max(y0 - AP, 0), ylim);
+    const int tx0 = FlowMap<S>::index(xmin), ty0 = FlowMap<S>::index(ymin);
+    for (int i = tid; i < FTW * FTH; i += 256) {
+        const int qx = med3i(tx0 + i % FTW, 0, P.fw - 1), qy = med3i(ty0 + i / FTW, 0, P.fh - 1);
+        ft[i] = P.lookup ? lookupAt(P.field, P.fw, P.fh, P.s, qx, qy) : P.field[qy * P.fw + qx];
+    }
+    __syncthreads();
+    for (int i = tid; i < FTH * EW; i += 256) {
+        const int j = i / EW, ex = i - j * EW;
+        const int x = min(med3i(x0 - AP + ex, 0, P.W - 1), xlim);
+        const float fx = FlowMap<S>::frac(x);
+        const float2* f = ft + j * FTW + (FlowMap<S>::index(x) - tx0);
+        hrow[i] = mix2(f[0], f[1], fx);
+    }
+    __syncthreads();
+    return ty0;
+}
+
+// The interpolated field r (hv = S r) at region column ex of frame row y (the second half of halfVec)
+template <int S>
+__device__ __forceinline__ float2 fieldAtRegion(const SynthParams& P, const float2* hrow, int ty0, int ex, int y) {
+    const int yc = min(y, S * P.fh - 1);
+    const float2* h = hrow + (FlowMap<S>::index(yc) - ty0) * EW + ex;
+    return mix2(h[0], h[EW], FlowMap<S>::frac(yc));
+}
+
+}  // namespace fgr

@@ -122,6 +122,9 @@ _sig = {
     "fgr_batch_set_frames": (ctypes.c_int, [_H, ctypes.c_void_p, ctypes.POINTER(BatchPair), ctypes.c_uint32]),
     "fgr_batch_record": (ctypes.c_int, [_H, ctypes.c_void_p]),
     "fgr_batch_launches": (ctypes.c_uint32, [_H]),
+    "fgr_batch_set_profiling": (ctypes.c_int, [_H, ctypes.c_int]),
+    "fgr_batch_read_timings": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                              ctypes.POINTER(ctypes.c_uint64), ctypes.c_int]),
     "fgr_batch_debug_counters": (ctypes.c_int, [_H, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
                                                 ctypes.c_void_p]),
     "fgr_batch_debug_internal": (ctypes.c_int, [_H, ctypes.c_uint32, ctypes.POINTER(DebugInternal)]),
@@ -347,6 +350,20 @@ class Batch:
 
     def launches(self):
         return lib.fgr_batch_launches(self._h)
+
+    def set_profiling(self, on):
+        return bool(lib.fgr_batch_set_profiling(self._h, int(on)))
+
+    def read_timings(self):
+        """[(stage, pass, ns), ...] per launch of the last profiled record, or None."""
+        n = self.launches()
+        st = (ctypes.c_int32 * max(1, n))()
+        ps = (ctypes.c_int32 * max(1, n))()
+        tk = (ctypes.c_uint64 * max(1, n))()
+        got = lib.fgr_batch_read_timings(self._h, st, ps, tk, n)
+        if got < 0:
+            return None
+        return [(int(st[i]), int(ps[i]), int(tk[i])) for i in range(got)]
 
     def debug_counters(self, pair, synth, dst, stream=None):
         return bool(lib.fgr_batch_debug_counters(self._h, _stream(stream), pair, synth,

@@ -169,6 +169,13 @@ void fgr_batch_destroy(FgrBatch* b);
 int fgr_batch_set_frames(FgrBatch* b, void* stream, const FgrBatchPair* pairs, uint32_t n);
 int fgr_batch_record(FgrBatch* b, void* stream);
 uint32_t fgr_batch_launches(const FgrBatch* b);
+/* While profiling is on, a record enqueues the launches eagerly (no graph), each between two HIP events on the stream;
+ * read_timings waits for the last one and fills, per launch, its stage (0 luma, 100 + i down to level i + 1,
+ * 200 + 10 s medhist / 201 + 10 s search / 202 + 10 s median3 of the s-th level from the coarsest, 400 subpix,
+ * 401 pixfield, 500 + 10 k synth k, 501 + 10 k its fallback), its pass and its duration in ns. Returns the number of
+ * launches (-1 if nothing was profiled). */
+int fgr_batch_set_profiling(FgrBatch* b, int on);
+int fgr_batch_read_timings(FgrBatch* b, int32_t* stage, int32_t* pass, uint64_t* ticks, int cap);
 /* [untrusted pixels of synth `synth` of pair `pair`, the pair's scene-cut SAD sum x 256] into dst (device) */
 int fgr_batch_debug_counters(FgrBatch* b, void* stream, uint32_t pair, uint32_t synth, void* dst);
 

@@ -385,15 +385,22 @@ def pixel_field(la, lc, bf, smooth, zero):
 
 # ----------------------------------------------------------------------------------------------- estimate
 
-def estimate(a, c, s=None, bgra=False):
+def estimate(a, c, s=None, bgra=False, luma0=None, size=None):
     """interp.py:186-191 estimate() with the GPU's schedule (framegen.cpp:2932-3153): everything that depends
     only on the pair. The per-pixel field is computed even for a scene cut (LAYOUT.md 3.3); the cut is applied
-    in the synth. Returns a dict with every intermediate."""
+    in the synth. Returns a dict with every intermediate. luma0 = (level-0 luma of a, of c) and size = (W, H)
+    replace the RGBA8 frames for the other formats (oracle/formats.py)."""
     s = s or DEFAULTS
-    H, W = a.shape[:2]
+    W, H = size if size is not None else (a.shape[1], a.shape[0])
     S = s["flowScale"]
     fw, fh, lv = layout(W, H, S, s["pyramidMinSide"])
-    pa, pc = pyramid(a, c, S, lv, bgra)
+    if luma0 is None:
+        pa, pc = pyramid(a, c, S, lv, bgra)
+    else:
+        pa, pc = [luma0[0]], [luma0[1]]
+        for (w, h, _, _) in lv[1:]:
+            pa.append(down(pa[-1], w, h))
+            pc.append(down(pc[-1], w, h))
     lam = s["searchPenalty"]
     n = len(lv)
     fields_raw, fields, meds = [None] * n, [None] * n, [None] * n
