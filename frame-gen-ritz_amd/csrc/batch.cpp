@@ -170,6 +170,11 @@ extern "C" FgrBatch* fgr_batch_create(const FgrBatchInfo* info, const FgrBatchPa
             return nullptr;
         }
         b->pairs.push_back(h);
+        if (info->seedNet && !fgr_set_seed_net(h, info->seedNet)) {
+            gBatchError = kBErrAlloc;
+            delete b;
+            return nullptr;
+        }
     }
     b->build();
     if (hipMalloc(&b->dev, b->blob.size()) != hipSuccess) { gBatchError = kBErrAlloc; delete b; return nullptr; }

@@ -122,9 +122,11 @@ void FgrInterpolator::freeBuffers() {
     if (luma) hipFree(luma);
     if (field) hipFree(field);
     if (u) hipFree(u);
+    if (seedAct) hipFree(seedAct);
     luma = nullptr;
     field = nullptr;
     u = nullptr;
+    seedAct = nullptr;
     W = H = 0;
     haveEst = false;
 }
@@ -173,7 +175,7 @@ bool FgrInterpolator::allocBuffers(uint32_t w, uint32_t h) {
 bool FgrInterpolator::rebuild(uint32_t w, uint32_t h) {
     if (hipSetDevice(device) != hipSuccess) return false;
     freeBuffers();
-    if (!allocBuffers(w, h)) {
+    if (!allocBuffers(w, h) || !allocSeedBuffers()) {
         freeBuffers();
         return false;
     }
@@ -400,6 +402,10 @@ void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& pr
         if (i == n - 1) {
             sp.parent = nullptr;
             sp.R = (int)st.searchRadiusTop;
+            if (st.seedNet && net) {   // the learned proposals: the CNN on this level's luma pair, then a seed candidate
+                planSeeds(plan);
+                sp.extraSeed = F + fieldSeed;
+            }
         } else {
             const Level& pl = lv[i + 1];
             MedParams mp{};
@@ -421,7 +427,6 @@ void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& pr
             sp.ph = pl.bh;
             sp.med = u + kUMedian + 2 * i;
             sp.R = (int)st.searchRadius;
-            sp.extraSeed = (st.seedNet && seedLevel == i && seedReady) ? F + fieldSeed : nullptr;
         }
         sp.out = F + l.fS;
         sp.sadAcc = (i == 0 && !st.subpixel) ? 1 : 0;
@@ -791,5 +796,6 @@ extern "C" int fgr_debug_internal(const FgrInterpolator* h, FgrDebugInternal* o)
     o->lumaWords = h->lumaWords;
     o->fieldEntries = h->fieldEntries;
     o->lastSynthSlot = h->lastSlot;
+    o->fieldSeed = h->fieldSeed;
     return 1;
 }

@@ -56,9 +56,11 @@ struct FgrInterpolator {
     int synthK = 0, lastSlot = 0;
     float lastT = 0.0f;
 
-    // learned seed proposals (seednet.cpp): level whose search takes them, and whether the buffer holds proposals
-    int seedLevel = -1;
-    bool seedReady = false;
+    // learned seed proposals (seednet.hip): the attached network and its activations on the coarsest level
+    FgrSeedNet* net = nullptr;
+    float* seedAct = nullptr;
+    bool allocSeedBuffers();
+    void planSeeds(std::vector<PlanOp>& plan) const;
 
     // profiling
     bool profiling = false;

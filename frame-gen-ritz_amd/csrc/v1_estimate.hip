@@ -163,7 +163,8 @@ __device__ __forceinline__ void medhistBody(const MedParams& P, int, int) {
 // search.comp:23-87 (interp.py:116-144): one wavefront per 8x8 block, four blocks per workgroup.
 //  1. seed (not on the coarsest level): 6 candidates (own parent x2, vertical / horizontal / diagonal neighbour
 //     parents x2, zero, global median) [+ one learned proposal], lane = pixel, SAD by the 64-lane tree;
-//     cost SAD/64 (+ lambda for all but own), strict <;
+//     cost SAD/64 (+ lambda for all but own), strict <. On the coarsest level the seed is zero, or with learned
+//     proposals the better of zero (no lambda) and the proposal (+ lambda);
 //  2. the (8 + 2R)^2 windows of a (at q - seed) and c (at q + seed) into LDS;
 //  3. lane = candidate (dy-major), sequential y-outer / x-inner SAD over LDS, cost SAD/64 + lambda |d|_1;
 //  4. wavefront argmin on (cost, candidate index): the first of the minimal costs, as the sequential strict <.
@@ -178,13 +179,14 @@ __device__ __forceinline__ void searchBody(const SearchParams& P, int bx0, int) 
     const int bx = active ? blk % P.bw : 0, by = active ? blk / P.bw : 0;
     const int b0x = bx * kBlock, b0y = by * kBlock;
     int sx = 0, sy = 0;
-    if (P.parent) {
+    if (P.parent || P.extraSeed) {
         const int qx = b0x + (lane & 7), qy = b0y + (lane >> 3);
-        const int px = min(bx / 2, P.pw - 1), py = min(by / 2, P.ph - 1);
+        const int px = P.parent ? min(bx / 2, P.pw - 1) : 0, py = P.parent ? min(by / 2, P.ph - 1) : 0;
         const int ssy = (by & 1) == 0 ? -1 : 1, ssx = (bx & 1) == 0 ? -1 : 1;
-        const int nc = P.extraSeed ? 7 : 6;
+        const int nc = P.parent ? (P.extraSeed ? 7 : 6) : 2;
         float best = 0.0f;
-        for (int k = 0; k < nc; k++) {
+        for (int kk = 0; kk < nc; kk++) {
+            const int k = P.parent ? kk : (kk == 0 ? 4 : 6);   // coarsest level: zero, then the proposal
             int cx, cy;
             if (k < 4) {
                 const int ox = (k == 2 || k == 3) ? ssx : 0, oy = (k == 1 || k == 3) ? ssy : 0;
@@ -204,7 +206,7 @@ __device__ __forceinline__ void searchBody(const SearchParams& P, int bx0, int) 
             }
             const float d = fabsf(lumaAt(P.la, qx - cx, qy - cy, P.w, P.h) - lumaAt(P.lc, qx + cx, qy + cy, P.w, P.h));
             float c = treeSum64(d) * (1.0f / 64.0f);
-            if (k == 0) {
+            if (kk == 0) {
                 best = c;
                 sx = cx;
                 sy = cy;

@@ -390,7 +390,7 @@ hipError_t launchFmtSingle(int pass, const void* params, int gx, int gy, hipStre
     case kPassSynthFmtS1: hipLaunchKernelGGL(synth_fmt_s1_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
     case kPassSynthFmtS2: hipLaunchKernelGGL(synth_fmt_s2_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
     case kPassSynthFmtS4: hipLaunchKernelGGL(synth_fmt_s4_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
-    default: return hipErrorInvalidValue;
+    default: return launchSeedSingle(pass, params, gx, gy, s);
     }
     return hipGetLastError();
 }
@@ -402,7 +402,7 @@ hipError_t launchFmtBatch(int pass, const void* params, const uint32_t* work, in
     case kPassSynthFmtS1: hipLaunchKernelGGL(synth_fmt_s1_batch_kernel, g, dim3(256), 0, s, (const SynthParams*)params, work); break;
     case kPassSynthFmtS2: hipLaunchKernelGGL(synth_fmt_s2_batch_kernel, g, dim3(256), 0, s, (const SynthParams*)params, work); break;
     case kPassSynthFmtS4: hipLaunchKernelGGL(synth_fmt_s4_batch_kernel, g, dim3(256), 0, s, (const SynthParams*)params, work); break;
-    default: return hipErrorInvalidValue;
+    default: return launchSeedBatch(pass, params, work, n, s);
     }
     return hipGetLastError();
 }

@@ -9,7 +9,8 @@ namespace fgr {
 enum Pass : int {
     kPassLuma = 0, kPassDown, kPassMedhist, kPassSearch2, kPassSearch6, kPassSearch12, kPassMedian3, kPassSubpix,
     kPassPixfield, kPassSynthRgba8S1, kPassSynthRgba8S2, kPassSynthRgba8S4, kPassFallback,
-    kPassLumaFmt, kPassSynthFmtS1, kPassSynthFmtS2, kPassSynthFmtS4   // Rgb10 / RgbaF16 / P010 (v1_formats.hip)
+    kPassLumaFmt, kPassSynthFmtS1, kPassSynthFmtS2, kPassSynthFmtS4,   // Rgb10 / RgbaF16 / P010 (v1_formats.hip)
+    kPassSeedConv                                                      // learned seed network layer (seednet.hip)
 };
 hipError_t launchBatch(int pass, const void* params, const uint32_t* work, int n, hipStream_t s);
 hipError_t launchSingle(int pass, const void* params, int gx, int gy, hipStream_t s);
@@ -18,6 +19,8 @@ hipError_t launchSynthBatch(int pass, const void* params, const uint32_t* work, 
 hipError_t launchFmtSingle(int pass, const void* params, int gx, int gy, hipStream_t s);
 hipError_t launchFmtBatch(int pass, const void* params, const uint32_t* work, int n, hipStream_t s);
 hipError_t launchDebugEFmt(const SynthParams& p, uint32_t* dst, int eStride, hipStream_t s);
+hipError_t launchSeedSingle(int pass, const void* params, int gx, int gy, hipStream_t s);
+hipError_t launchSeedBatch(int pass, const void* params, const uint32_t* work, int n, hipStream_t s);
 constexpr int kSubpixBlocks = 7;   // blocks per subpix workgroup
 int synthTilesX(int W);
 int synthTilesY(int H);

@@ -66,7 +66,7 @@ class DebugInternal(ctypes.Structure):
                 ("levels", ctypes.c_int32), ("fw", ctypes.c_int32), ("fh", ctypes.c_int32)] + \
                [(n, ctypes.c_int32 * 16) for n in ("w", "h", "bw", "bh", "lumaOffA", "lumaOffC", "fS", "fM")] + \
                [(n, ctypes.c_int32) for n in ("fieldSub", "fieldPF", "fieldT", "lumaWords", "fieldEntries",
-                                              "lastSynthSlot")]
+                                              "lastSynthSlot", "fieldSeed")]
 
 
 class BatchPair(ctypes.Structure):
@@ -77,7 +77,8 @@ class BatchPair(ctypes.Structure):
 class BatchInfo(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("bgra", ctypes.c_uint32), ("transfer", ctypes.c_uint32),
                 ("linearWhiteNits", ctypes.c_float), ("yuvMatrix", ctypes.c_uint32),
-                ("yuvFullRange", ctypes.c_uint32), ("useGraph", ctypes.c_uint32), ("settings", Settings)]
+                ("yuvFullRange", ctypes.c_uint32), ("useGraph", ctypes.c_uint32), ("settings", Settings),
+                ("seedNet", ctypes.c_void_p)]
 
 
 if not os.path.exists(_LIB_PATH):
@@ -125,6 +126,23 @@ _sig = {
     "fgr_batch_set_profiling": (ctypes.c_int, [_H, ctypes.c_int]),
     "fgr_batch_read_timings": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                               ctypes.POINTER(ctypes.c_uint64), ctypes.c_int]),
+    "fgr_seednet_create": (ctypes.c_void_p, [ctypes.c_int32, ctypes.c_uint64]),
+    "fgr_seednet_destroy": (None, [_H]),
+    "fgr_seednet_param_count": (ctypes.c_uint32, []),
+    "fgr_seednet_get_weights": (ctypes.c_int, [_H, ctypes.c_void_p, ctypes.c_uint32]),
+    "fgr_seednet_set_weights": (ctypes.c_int, [_H, ctypes.c_void_p, ctypes.c_uint32]),
+    "fgr_seednet_forward": (ctypes.c_int, [_H, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                           ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p]),
+    "fgr_seednet_gradients": (ctypes.c_int, [_H, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                             ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_void_p]),
+    "fgr_seednet_train_step": (ctypes.c_int, [_H, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                              ctypes.c_int32, ctypes.c_int32, ctypes.c_void_p, ctypes.c_float,
+                                              ctypes.c_void_p]),
+    "fgr_seednet_search_field": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                                ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
+                                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "fgr_set_seed_net": (ctypes.c_int, [_H, _H]),
     "fgr_batch_debug_counters": (ctypes.c_int, [_H, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
                                                 ctypes.c_void_p]),
     "fgr_batch_debug_internal": (ctypes.c_int, [_H, ctypes.c_uint32, ctypes.POINTER(DebugInternal)]),
@@ -285,6 +303,11 @@ class Interpolator:
             return None
         return {k: getattr(l, k) for k, _ in l._fields_}
 
+    def set_seed_net(self, net):
+        """Attach (or detach with None) a SeedNet; used when settings.seedNet is 1."""
+        self._net = net
+        return bool(lib.fgr_set_seed_net(self._h, net._h if net is not None else None))
+
     def debug_internal(self):
         """Test / debug only: the internal buffers' device pointers and layout (DebugInternal), or None."""
         d = DebugInternal()
@@ -317,8 +340,10 @@ class Batch:
     """Batched V1 over the C ABI (fgr_batch_*): many pairs, one launch per pass, replayed as a hipGraph."""
 
     def __init__(self, items, settings=None, bgra=False, use_graph=True, device=0, transfer=TRANSFER_ENCODED,
-                 linear_white_nits=80.0, yuv_matrix=YUV_BT709, yuv_full_range=False):
+                 linear_white_nits=80.0, yuv_matrix=YUV_BT709, yuv_full_range=False, seed_net=None):
         info = BatchInfo()
+        info.seedNet = seed_net._h if seed_net is not None else None
+        self._net = seed_net
         info.device, info.bgra, info.useGraph = device, int(bgra), int(use_graph)
         info.transfer, info.linearWhiteNits = transfer, linear_white_nits
         info.yuvMatrix, info.yuvFullRange = yuv_matrix, int(yuv_full_range)
@@ -372,3 +397,64 @@ class Batch:
     def debug_internal(self, pair):
         d = DebugInternal()
         return d if lib.fgr_batch_debug_internal(self._h, pair, ctypes.byref(d)) else None
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+class SeedNet:
+    """The learned seed network (fgr_seednet_*): luma tensors are CUDA float32 (n, h, w) of one level's planes."""
+
+    def __init__(self, seed=0, device=0):
+        self._h = lib.fgr_seednet_create(device, seed)
+        if not self._h:
+            raise RuntimeError("fgr_seednet_create failed")
+
+    def close(self):
+        if self._h:
+            lib.fgr_seednet_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def param_count():
+        return lib.fgr_seednet_param_count()
+
+    def get_weights(self):
+        import numpy as np
+        w = np.empty(self.param_count(), np.float32)
+        assert lib.fgr_seednet_get_weights(self._h, w.ctypes.data, w.size)
+        return w
+
+    def set_weights(self, w):
+        import numpy as np
+        w = np.ascontiguousarray(w, np.float32)
+        assert lib.fgr_seednet_set_weights(self._h, w.ctypes.data, w.size)
+
+    def forward(self, la, lc, out, stream=None):
+        n, h, w = la.shape
+        return bool(lib.fgr_seednet_forward(self._h, _stream(stream), _ptr(la), _ptr(lc), w, h, n, _ptr(out)))
+
+    def gradients(self, la, lc, target, grad=None, loss=None, stream=None):
+        n, h, w = la.shape
+        return bool(lib.fgr_seednet_gradients(self._h, _stream(stream), _ptr(la), _ptr(lc), w, h, n, _ptr(target),
+                                              _ptr(grad), _ptr(loss)))
+
+    def train_step(self, la, lc, target, lr, loss=None, stream=None):
+        n, h, w = la.shape
+        return bool(lib.fgr_seednet_train_step(self._h, _stream(stream), _ptr(la), _ptr(lc), w, h, n, _ptr(target),
+                                               lr, _ptr(loss)))
+
+
+def search_field(la, lc, R, lam, raw, out, seeds=None, stream=None):
+    """The coarsest-level field of n luma pairs: search with radius R around zero (or the better of zero and the
+    proposal in `seeds`), then median3; the plain field (the training target) when seeds is None."""
+    n, h, w = la.shape
+    return bool(lib.fgr_seednet_search_field(_stream(stream), _ptr(la), _ptr(lc), w, h, n, R, lam, _ptr(seeds),
+                                             _ptr(raw), _ptr(out)))

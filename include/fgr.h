@@ -160,6 +160,7 @@ typedef struct FgrBatchInfo {
     uint32_t yuvMatrix, yuvFullRange;
     uint32_t useGraph;         /* 1: capture the batch into a hipGraph on the first record, replay it after */
     FgrSettings settings;
+    struct FgrSeedNet* seedNet; /* attached to every pair (fgr_set_seed_net) when settings.seedNet is 1; may be null */
 } FgrBatchInfo;
 
 typedef struct FgrBatch FgrBatch;
@@ -179,6 +180,35 @@ int fgr_batch_read_timings(FgrBatch* b, int32_t* stage, int32_t* pass, uint64_t*
 /* [untrusted pixels of synth `synth` of pair `pair`, the pair's scene-cut SAD sum x 256] into dst (device) */
 int fgr_batch_debug_counters(FgrBatch* b, void* stream, uint32_t pair, uint32_t synth, void* dst);
 
+/* ---- learned seed proposals (this library's addition; no counterpart in the reference) ----
+ * A 5-layer 3x3 CNN (2 -> 16 -> 32 -> 64 -> 64 -> 2 channels, stride 2 in layers 1-3, ReLU after 1-4) on the coarsest
+ * level's luma pair proposes one vector per 8x8 block of that level; with settings.seedNet = 1 and a network attached
+ * the coarsest search takes the better (by SAD, the proposal paying the seed penalty) of zero and the proposal as its
+ * seed. Forward and both gradients run on the BF16 matrix cores with FP32 accumulation; training (Adam, mean squared
+ * error against the plain search's field) runs on the GPU. Luma arguments are device pointers to n planes of w x h
+ * float32 levels (plane i at + i w h); fields are n x ceil(h/8) x ceil(w/8) float2 (x, y) in that level's pixels. */
+typedef struct FgrSeedNet FgrSeedNet;
+FgrSeedNet* fgr_seednet_create(int32_t device, uint64_t seed);   /* He-normal initial weights from the seed */
+void fgr_seednet_destroy(FgrSeedNet* net);
+uint32_t fgr_seednet_param_count(void);
+int fgr_seednet_get_weights(const FgrSeedNet* net, float* host, uint32_t n);
+int fgr_seednet_set_weights(FgrSeedNet* net, const float* host, uint32_t n);    /* also resets the Adam state */
+int fgr_seednet_forward(FgrSeedNet* net, void* stream, const float* la, const float* lc, int32_t w, int32_t h, int32_t n,
+                        float* out);
+/* gradients of the mean squared error against target into grad (device, param_count floats), the loss into loss
+ * (device float); either may be null */
+int fgr_seednet_gradients(FgrSeedNet* net, void* stream, const float* la, const float* lc, int32_t w, int32_t h,
+                          int32_t n, const float* target, float* grad, float* loss);
+int fgr_seednet_train_step(FgrSeedNet* net, void* stream, const float* la, const float* lc, int32_t w, int32_t h,
+                           int32_t n, const float* target, float lr, float* loss);
+/* the coarsest-level field of n luma pairs: the search with radius R around seed zero -- or, given proposals (n x bh x
+ * bw float2, else null), around the better of zero and the proposal -- then median3; raw and median-filtered fields.
+ * With null proposals this is the plain field, the training target. */
+int fgr_seednet_search_field(void* stream, const float* la, const float* lc, int32_t w, int32_t h, int32_t n, int32_t R,
+                             float lam, const float* seeds, float* raw, float* out);
+/* attach (or detach with null) a network to a handle; it must outlive the handle's records that use it */
+int fgr_set_seed_net(FgrInterpolator* h, FgrSeedNet* net);
+
 /* ---- test / debug only; no counterpart in the reference ----
  * Device pointers of the handle's internal buffers and their layout (float / float2 offsets), so a test can read every
  * intermediate of the latest estimate (luma pyramid, per-level raw and median-filtered block fields, medians, the
@@ -191,6 +221,7 @@ typedef struct FgrDebugInternal {
     int32_t levels, fw, fh;
     int32_t w[16], h[16], bw[16], bh[16], lumaOffA[16], lumaOffC[16], fS[16], fM[16];
     int32_t fieldSub, fieldPF, fieldT, lumaWords, fieldEntries, lastSynthSlot;
+    int32_t fieldSeed;      /* float2 offset of the learned proposals of the coarsest level's blocks */
 } FgrDebugInternal;
 int fgr_debug_internal(const FgrInterpolator* h, FgrDebugInternal* out);
 int fgr_batch_debug_internal(const FgrBatch* b, uint32_t pair, FgrDebugInternal* out);

@@ -195,9 +195,10 @@ def global_median2(parent, bw, bh):
     return np.array(out, f32)
 
 
-def seed_from(la, lc, parent, med2, bw, bh, lam):
+def seed_from(la, lc, parent, med2, bw, bh, lam, extra=None):
     """search.comp:28-59 (interp.py:131-144): 6 candidates (own parent x2, vertical, horizontal and diagonal
-    neighbour parents x2, zero, global median), cost = SAD/64 (+ lambda for all but own), strict <."""
+    neighbour parents x2, zero, global median), cost = SAD/64 (+ lambda for all but own), strict <.
+    extra: a 7th candidate per block (float32 (bh, bw, 2), rounded half-even), the learned proposal."""
     ph, pw = parent.shape[:2]
     by, bx = np.meshgrid(np.arange(bh), np.arange(bw), indexing="ij")
     px, py = np.minimum(bx // 2, pw - 1), np.minimum(by // 2, ph - 1)
@@ -212,6 +213,18 @@ def seed_from(la, lc, parent, med2, bw, bh, lam):
     cands.append(np.zeros((bh, bw, 2), np.int64))
     m = np.rint(med2).astype(np.int64)
     cands.append(np.broadcast_to(m, (bh, bw, 2)).copy())
+    if extra is not None:
+        cands.append(np.rint(extra).astype(np.int64))
+    return _pick_seed(la, lc, cands, lam)
+
+
+def seed_coarse(la, lc, extra, lam):
+    """The coarsest level with a learned proposal: candidates zero (the plain seed, no lambda) and the proposal
+    (rounded half-even, + lambda), chosen as seed_from chooses."""
+    return _pick_seed(la, lc, [np.zeros(extra.shape, np.int64), np.rint(extra).astype(np.int64)], lam)
+
+
+def _pick_seed(la, lc, cands, lam):
     best = bi = None
     for k, cv in enumerate(cands):
         s = block_sad_tree(la, lc, cv[..., 0], cv[..., 1])
@@ -385,11 +398,12 @@ def pixel_field(la, lc, bf, smooth, zero):
 
 # ----------------------------------------------------------------------------------------------- estimate
 
-def estimate(a, c, s=None, bgra=False, luma0=None, size=None):
+def estimate(a, c, s=None, bgra=False, luma0=None, size=None, proposals=None):
     """interp.py:186-191 estimate() with the GPU's schedule (framegen.cpp:2932-3153): everything that depends
     only on the pair. The per-pixel field is computed even for a scene cut (LAYOUT.md 3.3); the cut is applied
     in the synth. Returns a dict with every intermediate. luma0 = (level-0 luma of a, of c) and size = (W, H)
-    replace the RGBA8 frames for the other formats (oracle/formats.py)."""
+    replace the RGBA8 frames for the other formats (oracle/formats.py). proposals: learned seed vectors of the
+    coarsest level's blocks (float32 (bh, bw, 2)), an extra seed candidate of that level's search."""
     s = s or DEFAULTS
     W, H = size if size is not None else (a.shape[1], a.shape[0])
     S = s["flowScale"]
@@ -409,7 +423,10 @@ def estimate(a, c, s=None, bgra=False, luma0=None, size=None):
     for i in range(n - 1, -1, -1):
         w, h, bw, bh = lv[i]
         if i == n - 1:
-            seed = np.zeros((bh, bw, 2), np.int64)
+            if proposals is None:
+                seed = np.zeros((bh, bw, 2), np.int64)
+            else:
+                seed = seed_coarse(pa[i], pc[i], proposals, lam)
             R = s["searchRadiusTop"]
         else:
             meds[i] = global_median2(v, bw, bh)
