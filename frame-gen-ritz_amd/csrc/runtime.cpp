@@ -123,6 +123,9 @@ void FgrInterpolator::freeBuffers() {
     if (field) hipFree(field);
     if (u) hipFree(u);
     if (seedAct) hipFree(seedAct);
+    if (blurScratch) hipFree(blurScratch);
+    blurScratch = nullptr;
+    blurCap = 0;
     luma = nullptr;
     field = nullptr;
     u = nullptr;

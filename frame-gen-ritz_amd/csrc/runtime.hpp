@@ -58,6 +58,8 @@ struct FgrInterpolator {
 
     // learned seed proposals (seednet.hip): the attached network and its activations on the coarsest level
     FgrSeedNet* net = nullptr;
+    uint8_t* blurScratch = nullptr;   // recordSynthBlur's sub-frames (blur.hip), grown by the first blurred record
+    size_t blurCap = 0;
     float* seedAct = nullptr;
     bool allocSeedBuffers();
     void planSeeds(std::vector<PlanOp>& plan) const;

@@ -371,6 +371,36 @@ def synth(a, c, est, t, fmt, exact64=False, lut=None, yuv=None):
     return (yout, cout), None, info
 
 
+def synth_blur(a, c, est, t0, t1, n, fmt, gauss=False, lut=None, yuv=None):
+    """recordSynthBlur for the other formats: the weighted mean of the stored sub-frames (synth() at each sample time,
+    the real frames at t <= 0 / t >= 1) in the format's own codes, rounded once (halves: to nearest even)."""
+    ts, ws = v1.blur_samples(t0, t1, n, gauss)
+    frames = []
+    for t in ts:
+        if t <= 0:
+            frames.append(a)
+        elif t >= 1:
+            frames.append(c)
+        else:
+            frames.append(synth(a, c, est, float(t), fmt, lut=lut, yuv=yuv)[0])
+    if fmt == RGB10:
+        ch = [np.concatenate([rgb10_channels(f), (f >> 30).astype(np.int64)[..., None]], -1) for f in frames]
+        r = v1.average_frames(ch[:1] + ch[1:], ws, hi=1023).astype(np.uint32)
+        r[..., 3] = np.minimum(r[..., 3], 3)
+        return (r[..., 0] | (r[..., 1] << 10) | (r[..., 2] << 20) | (r[..., 3] << 30)).astype(np.uint32)
+    if fmt == F16:
+        wsum = f32(0.0)
+        for w in ws:
+            wsum = f32(wsum + w)
+        acc = np.zeros(frames[0].shape, f32)
+        for f, w in zip(frames, ws):
+            acc = (acc + (w * f.view(np.float16).astype(f32)).astype(f32)).astype(f32)
+        return (acc / wsum).astype(f32).astype(np.float16).view(np.uint16)
+    y = v1.average_frames([(f[0] >> 6) for f in frames], ws, hi=1023).astype(np.uint16) << 6
+    cc = v1.average_frames([(f[1] >> 6) for f in frames], ws, hi=1023).astype(np.uint16) << 6
+    return y.astype(np.uint16), cc.astype(np.uint16)
+
+
 def level_error(fmt, got, want64_info, lut=None):
     """The output error in 8-bit levels against the FP64 oracle's unrounded levels (for the +-1 / +-2 bar)."""
     if fmt == RGB10:

@@ -585,6 +585,45 @@ def synth(a, c, est, t, exact64=False, bgra=False):
     return out, None, info
 
 
+def blur_samples(t0, t1, n, gauss):
+    """framegen.cpp:3549-3555 + blur_resolve.comp:128-135: the midpoints t_k of n equal slices of [t0, t1] (double,
+    then float32, clamped to [t0, t1]) and the weights (1, or exp(-2 x^2), x = (2k + 1 - n) / n, double -> float32)."""
+    t0, t1 = float(f32(t0)), float(f32(t1))
+    ts, ws = [], []
+    for k in range(n):
+        t = f32(t0 + (t1 - t0) * (k + 0.5) / n)
+        ts.append(min(max(t, f32(t0)), f32(t1)))
+        x = (2.0 * k + 1.0 - n) / n
+        ws.append(f32(np.exp(-2.0 * x * x)) if gauss else f32(1.0))
+    return ts, ws
+
+
+def average_frames(frames, ws, hi=255, alpha_hi=255):
+    """The blur's rounding: acc = sum_k w_k v_k (float32, in order), out = clamp(floor(acc / wsum + 0.5), 0, hi)."""
+    wsum = f32(0.0)
+    for w in ws:
+        wsum = f32(wsum + w)
+    acc = np.zeros(frames[0].shape, f32)
+    for v, w in zip(frames, ws):
+        acc = (acc + (w * v.astype(f32)).astype(f32)).astype(f32)
+    return np.clip(np.floor((acc / wsum).astype(f32) + f32(0.5)), 0, hi)
+
+
+def synth_blur(a, c, est, t0, t1, n, gauss=False):
+    """recordSynthBlur (framegen.cpp:3534-3634) with blurSamples = n >= 2, t0 < t1: the weighted mean of the
+    RGBA8 frames synth() gives at the sample times (prev / curr at t <= 0 / t >= 1), rounded once."""
+    ts, ws = blur_samples(t0, t1, n, gauss)
+    frames = []
+    for t in ts:
+        if t <= 0:
+            frames.append(a)
+        elif t >= 1:
+            frames.append(c)
+        else:
+            frames.append(synth(a, c, est, float(t))[0])
+    return average_frames(frames, ws).astype(np.uint8)
+
+
 def interpolate_multi(a, c, n, s=None):
     """interp.py:254-258: one estimate, then the n - 1 frames at t = k / n."""
     est = estimate(a, c, s)
