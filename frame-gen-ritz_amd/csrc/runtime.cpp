@@ -707,6 +707,28 @@ extern "C" int fgr_debug_map_layout(const FgrInterpolator* h, FgrDebugMapLayout*
     return 1;
 }
 
+extern "C" int fgr_fgmaps_header(const FgrInterpolator* h, void* out64) {
+    FgrDebugMapLayout l;
+    if (!h || !out64 || !fgr_debug_map_layout(h, &l)) return 0;
+    uint32_t w[16];
+    w[0] = 0x504D4746u;   // 'FGMP'
+    w[1] = 5;
+    w[2] = l.W;
+    w[3] = l.H;
+    w[4] = l.fw;
+    w[5] = l.fh;
+    w[6] = l.bw;
+    w[7] = l.bh;
+    w[8] = l.fscale;
+    w[9] = l.eStride;
+    const float f[4] = {h->set.trustLow, h->set.trustHigh, h->set.smoothBonus, h->set.hudBonus};
+    std::memcpy(w + 10, f, 16);
+    w[14] = h->set.motionVersion;
+    w[15] = 0;
+    std::memcpy(out64, w, 64);
+    return 1;
+}
+
 extern "C" int fgr_record_debug_maps(FgrInterpolator* h, void* stream, void* dst) {
     FgrDebugMapLayout l;
     if (!h || !dst || !h->haveEst || !fgr_debug_map_layout(h, &l)) return 0;

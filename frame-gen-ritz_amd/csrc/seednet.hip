@@ -58,9 +58,91 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// float32 -> OCP FP8 E4M3 (bias 7, largest finite 448), round to nearest even, saturating; written out so that the CPU
+// oracle can follow it bit for bit
+__device__ __forceinline__ uint32_t fp8e4m3(float v) {
+    const uint32_t s = v < 0.0f ? 0x80u : 0u;
+    const float a = fabsf(v);
+    if (!(a < 448.0f)) return s | 0x7Eu;
+    if (a < 0.015625f) return s | (uint32_t)(int)rintf(a * 512.0f);   // subnormals: multiples of 2^-9 (8 -> 2^-6)
+    int e;
+    frexpf(a, &e);
+    const int eb = e - 1;                                              // a in [2^eb, 2^(eb + 1))
+    int M = (int)rintf(ldexpf(a, 3 - eb)) - 8, E = eb + 7;
+    if (M == 8) {
+        M = 0;
+        E++;
+    }
+    if (E > 15 || (E == 15 && M == 7)) return s | 0x7Eu;
+    return s | (uint32_t)(E << 3) | (uint32_t)M;
+}
+
+// FP8 forward (v_mfma_f32_16x16x32_fp8_fp8): the tile of convFwdBody with 32-deep K steps, 8 E4M3 operands a lane
+__device__ __forceinline__ void convFwdFp8(const ConvParams& P, int bx, int by) {
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int Pn = P.outW * P.outH, K = 9 * P.Cin;
+    const int nPT = (Pn + 15) >> 4, nCT = (P.Cout + 15) >> 4;
+    const int t = bx * 4 + wave;
+    if (t >= nPT * nCT) return;
+    const int ct = t % nCT, pt = t / nCT, img = by;
+    const int row = lane & 15, quad = lane >> 4;
+    const int p = pt * 16 + row;
+    const int ox = p % P.outW, oy = p / P.outW;
+    const int co = ct * 16 + row;
+    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        uint64_t a = 0, b = 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int k = k0 + 8 * quad + j;
+            uint32_t aq = 0, bq = 0;
+            if (k < K) {
+                const int tap = k / P.Cin, ci = k - tap * P.Cin;
+                const int ky = tap / 3, kx = tap - 3 * ky;
+                if (p < Pn) aq = fp8e4m3(inputAt(P, img, ox * P.stride + kx - 1, oy * P.stride + ky - 1, ci) / P.sa[ci]);
+                if (co < P.Cout) bq = P.wq[co * P.kp + k];
+            }
+            a |= (uint64_t)aq << (8 * j);
+            b |= (uint64_t)bq << (8 * j);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)a, (long)b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int pp = pt * 16 + 4 * quad + j, cc = ct * 16 + row;
+        if (pp < Pn && cc < P.Cout) {
+            float v = acc[j] * P.sw[cc] + P.b[cc];
+            if (P.relu) v = fmaxf(v, 0.0f);
+            P.out[((int64_t)img * Pn + pp) * P.Cout + cc] = v;
+        }
+    }
+}
+
+// per-channel largest |input| of a layer over all its pixels and images (one workgroup per channel), for the FP8 scales
+__global__ __launch_bounds__(256) void conv_absmax_kernel(ConvParams P, float* out) {
+    __shared__ float red[256];
+    const int ci = blockIdx.x, Q = P.inW * P.inH;
+    float m = 0.0f;
+    for (int r = threadIdx.x; r < P.n * Q; r += 256) {
+        const int img = r / Q, q = r - img * Q;
+        m = fmaxf(m, fabsf(inputAt(P, img, q % P.inW, q / P.inW, ci)));
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[ci] = red[0];
+}
+
 // Forward: out[img][p][co] = relu?(b[co] + sum_k A[p][k] W[co][k]), A = the 3x3 patch of input pixel
 // (ox s + kx - 1, oy s + ky - 1). Wave tile: 16 output pixels x 16 output channels; workgroup: 4 tiles.
 __device__ __forceinline__ void convFwdBody(const ConvParams& P, int bx, int by) {
+    if (P.fp8) {
+        convFwdFp8(P, bx, by);
+        return;
+    }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int Pn = P.outW * P.outH, K = 9 * P.Cin;
     const int nPT = (Pn + 15) >> 4, nCT = (P.Cout + 15) >> 4;
@@ -315,11 +397,16 @@ struct FgrSeedNet {
     float* part = nullptr;   // weight-gradient partial sums
     float* loss = nullptr;   // one float
     size_t actCap = 0, partCap = 0;
+    // FP8 inference: per layer E4M3 weights (Cout x kp bytes), input scales sa (Cin) and output scales sw (Cout)
+    uint8_t* wq = nullptr;
+    float* sa = nullptr;     // all layers' sa, then all layers' sw
+    bool fp8Ready = false, fp8On = false;
 
     ~FgrSeedNet() {
         hipSetDevice(device);
-        for (float* p : {w, g, m, v, act, grad, part, loss})
+        for (float* p : {w, g, m, v, act, grad, part, loss, sa})
             if (p) hipFree(p);
+        if (wq) hipFree(wq);
     }
 
     bool grow(int w0, int h0, int n) {
@@ -351,7 +438,7 @@ struct FgrSeedNet {
 
     // the ConvParams of layer l over n images of a w0 x h0 luma pair, activations at act (layer outputs in order)
     ConvParams layer(int l, const float* la, const float* lc, int w0, int h0, int n, float* acts, float* grads,
-                     const float* weights) const {
+                     const float* weights, bool fp8 = false) const {
         int lw[kNetLayers + 1], lh[kNetLayers + 1];
         netLayerSizes(w0, h0, lw, lh);
         size_t off[kNetLayers + 1];
@@ -379,10 +466,57 @@ struct FgrSeedNet {
             p.dy = grads + off[l];
             p.dx = l == 0 ? nullptr : grads + off[l - 1];
         }
+        if (fp8 && fp8On && fp8Ready) {
+            p.fp8 = 1;
+            p.wq = wq + wqOffset(l);
+            p.kp = kpOf(l);
+            p.sa = sa + chanOffset(l, true);
+            p.sw = sa + chanOffset(l, false);
+        }
         p.gx = convFwdTiles(p);
         return p;
     }
+
+    static int kpOf(int l) { return (9 * kCinL(l) + 31) / 32 * 32; }
+    static int kCinL(int l);
+    static int kCoutL(int l);
+    static size_t wqOffset(int l) {
+        size_t o = 0;
+        for (int i = 0; i < l; i++) o += (size_t)kCoutL(i) * kpOf(i);
+        return o;
+    }
+    static int chanOffset(int l, bool in) {   // sa: the Cin of every layer, then sw: the Cout of every layer
+        int o = 0, total = 0;
+        for (int i = 0; i < kNetLayers; i++) total += kCinL(i);
+        if (in) {
+            for (int i = 0; i < l; i++) o += kCinL(i);
+            return o;
+        }
+        for (int i = 0; i < l; i++) o += kCoutL(i);
+        return total + o;
+    }
 };
+
+int FgrSeedNet::kCinL(int l) { return kCin[l]; }
+int FgrSeedNet::kCoutL(int l) { return kCout[l]; }
+
+// OCP FP8 E4M3 on the host: the same steps as fp8e4m3 on the device
+static uint8_t fp8e4m3Host(float v) {
+    const uint32_t s = v < 0.0f ? 0x80u : 0u;
+    const float a = std::fabs(v);
+    if (!(a < 448.0f)) return (uint8_t)(s | 0x7Eu);
+    if (a < 0.015625f) return (uint8_t)(s | (uint32_t)(int)std::nearbyint(a * 512.0f));
+    int e;
+    std::frexp(a, &e);
+    const int eb = e - 1;
+    int M = (int)std::nearbyint(std::ldexp(a, 3 - eb)) - 8, E = eb + 7;
+    if (M == 8) {
+        M = 0;
+        E++;
+    }
+    if (E > 15 || (E == 15 && M == 7)) return (uint8_t)(s | 0x7Eu);
+    return (uint8_t)(s | (uint32_t)(E << 3) | (uint32_t)M);
+}
 
 static bool launchFwd(const ConvParams& p, hipStream_t s) {
     hipLaunchKernelGGL(seed_conv_kernel, dim3((unsigned)p.gx, (unsigned)p.n), dim3(256), 0, s, p);
@@ -446,10 +580,89 @@ extern "C" int fgr_seednet_forward(FgrSeedNet* net, void* stream, const float* l
     hipStream_t s = (hipStream_t)stream;
     if (!net->grow(w, h, n)) return 0;
     for (int l = 0; l < kNetLayers; l++) {
-        ConvParams p = net->layer(l, la, lc, w, h, n, net->act, nullptr, net->w);
+        ConvParams p = net->layer(l, la, lc, w, h, n, net->act, nullptr, net->w, true);
         if (l == kNetLayers - 1) p.out = out;
         if (!launchFwd(p, s)) return 0;
     }
+    return 1;
+}
+
+// FP8 calibration: a BF16 forward over the batch, then per layer the largest |input| of each channel c, the scale
+// sa[c] = max / 448 (the largest value maps onto E4M3's largest); the weights with sa folded in, W'[co][k] = W[co][k]
+// sa[ci], get per output channel sw[co] = max_k |W'| / 448 and are stored as e4m3(W' / sw).
+extern "C" int fgr_seednet_calibrate_fp8(FgrSeedNet* net, void* stream, const float* la, const float* lc, int32_t w,
+                                         int32_t h, int32_t n) {
+    if (!net || !la || !lc || w < 8 || h < 8 || n < 1) return 0;
+    hipStream_t s = (hipStream_t)stream;
+    if (!net->grow(w, h, n)) return 0;
+    int nIn = 0, nOut = 0;
+    for (int l = 0; l < kNetLayers; l++) {
+        nIn += kCin[l];
+        nOut += kCout[l];
+    }
+    if (!net->sa && hipMalloc(&net->sa, (size_t)(nIn + nOut) * 4) != hipSuccess) {
+        net->sa = nullptr;
+        return 0;
+    }
+    const size_t wqBytes = FgrSeedNet::wqOffset(kNetLayers);
+    if (!net->wq && hipMalloc(&net->wq, wqBytes) != hipSuccess) {
+        net->wq = nullptr;
+        return 0;
+    }
+    for (int l = 0; l < kNetLayers; l++) {
+        const ConvParams p = net->layer(l, la, lc, w, h, n, net->act, nullptr, net->w);
+        if (!launchFwd(p, s)) return 0;
+        hipLaunchKernelGGL(conv_absmax_kernel, dim3((unsigned)p.Cin), dim3(256), 0, s, p,
+                           net->sa + FgrSeedNet::chanOffset(l, true));
+    }
+    std::vector<float> sa(nIn + nOut), wts(paramCount());
+    if (hipMemcpyAsync(sa.data(), net->sa, (size_t)nIn * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(wts.data(), net->w, wts.size() * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return 0;
+    for (int i = 0; i < nIn; i++) sa[i] = sa[i] > 0.0f ? sa[i] / 448.0f : 1.0f;
+    std::vector<uint8_t> wq(wqBytes, 0);
+    for (int l = 0; l < kNetLayers; l++) {
+        const int K = 9 * kCin[l], kp = FgrSeedNet::kpOf(l);
+        const float* W = wts.data() + layerOffset(l);
+        const float* sal = sa.data() + FgrSeedNet::chanOffset(l, true);
+        float* swl = sa.data() + FgrSeedNet::chanOffset(l, false);
+        uint8_t* q = wq.data() + FgrSeedNet::wqOffset(l);
+        std::vector<float> wp(K);
+        for (int co = 0; co < kCout[l]; co++) {
+            float m = 0.0f;
+            for (int k = 0; k < K; k++) {
+                wp[k] = W[co * K + k] * sal[k % kCin[l]];
+                m = std::max(m, std::fabs(wp[k]));
+            }
+            swl[co] = m > 0.0f ? m / 448.0f : 1.0f;
+            for (int k = 0; k < K; k++) q[co * kp + k] = fp8e4m3Host(wp[k] / swl[co]);
+        }
+    }
+    if (hipMemcpyAsync(net->sa, sa.data(), sa.size() * 4, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(net->wq, wq.data(), wqBytes, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return 0;
+    net->fp8Ready = true;
+    return 1;
+}
+
+extern "C" int fgr_seednet_set_fp8(FgrSeedNet* net, int on) {
+    if (!net || (on && !net->fp8Ready)) return 0;
+    net->fp8On = on != 0;
+    return 1;
+}
+
+extern "C" int fgr_seednet_fp8_scales(const FgrSeedNet* net, float* sa, float* sw) {
+    if (!net || !net->fp8Ready) return 0;
+    int nIn = 0, nOut = 0;
+    for (int l = 0; l < kNetLayers; l++) {
+        nIn += kCin[l];
+        nOut += kCout[l];
+    }
+    hipSetDevice(net->device);
+    if (sa && hipMemcpy(sa, net->sa, (size_t)nIn * 4, hipMemcpyDeviceToHost) != hipSuccess) return 0;
+    if (sw && hipMemcpy(sw, net->sa + nIn, (size_t)nOut * 4, hipMemcpyDeviceToHost) != hipSuccess) return 0;
     return 1;
 }
 
@@ -567,7 +780,7 @@ void FgrInterpolator::planSeeds(std::vector<PlanOp>& plan) const {
     const Level& c = lv.back();
     float2* F = (float2*)field;
     for (int l = 0; l < kNetLayers; l++) {
-        ConvParams p = net->layer(l, luma + c.offA, luma + c.offC, c.w, c.h, 1, seedAct, nullptr, net->w);
+        ConvParams p = net->layer(l, luma + c.offA, luma + c.offC, c.w, c.h, 1, seedAct, nullptr, net->w, true);
         if (l == kNetLayers - 1) p.out = (float*)(F + fieldSeed);
         PlanOp op;
         op.pass = kPassSeedConv;

@@ -22,6 +22,11 @@ struct ConvParams {
     float* dx;                    // gradient of its input
     float* dw; float* db;         // gradients of W and b
     float* dwPart;                // per-split partial sums of dW
+    // FP8 inference (E4M3, per-channel scales): input channel ci is quantised as e4m3(x / sa[ci]); the weights, with
+    // sa folded in, as wq[co][k] = e4m3(W[co][k] sa[ci] / sw[co]) (kp bytes a row); out = sw[co] sum + b[co]
+    int fp8;
+    const uint8_t* wq; int kp;
+    const float* sa; const float* sw;
 };
 
 int convFwdTiles(const ConvParams& p);

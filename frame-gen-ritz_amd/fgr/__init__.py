@@ -143,6 +143,11 @@ _sig = {
                                                 ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
                                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "fgr_set_seed_net": (ctypes.c_int, [_H, _H]),
+    "fgr_fgmaps_header": (ctypes.c_int, [_H, ctypes.c_void_p]),
+    "fgr_seednet_calibrate_fp8": (ctypes.c_int, [_H, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                                 ctypes.c_int32, ctypes.c_int32]),
+    "fgr_seednet_set_fp8": (ctypes.c_int, [_H, ctypes.c_int]),
+    "fgr_seednet_fp8_scales": (ctypes.c_int, [_H, ctypes.c_void_p, ctypes.c_void_p]),
     "fgr_batch_debug_counters": (ctypes.c_int, [_H, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
                                                 ctypes.c_void_p]),
     "fgr_batch_debug_internal": (ctypes.c_int, [_H, ctypes.c_uint32, ctypes.POINTER(DebugInternal)]),
@@ -316,6 +321,25 @@ class Interpolator:
     def record_debug_maps(self, dst, stream=None):
         return bool(lib.fgr_record_debug_maps(self._h, _stream(stream), ctypes.c_void_p(dst.data_ptr())))
 
+    def dump_fgmaps(self, path):
+        """Write the latest record's debug maps as an fgtest --dump file (.fgmaps: 64-byte header, block field,
+        per-pixel field, packed mismatch), readable by the reference's gpu/fgmaps.py."""
+        import torch
+        lay = self.debug_map_layout()
+        if lay is None:
+            return False
+        buf = torch.empty(int(lay["size"]), dtype=torch.uint8, device="cuda")
+        if not self.record_debug_maps(buf):
+            return False
+        head = (ctypes.c_uint8 * 64)()
+        if not lib.fgr_fgmaps_header(self._h, head):
+            return False
+        torch.cuda.synchronize()
+        with open(path, "wb") as f:
+            f.write(bytes(head))
+            f.write(buf.cpu().numpy().tobytes())
+        return True
+
 
 def _batch_pairs(items):
     """items: (prev, curr, multiplier, [out, ...]) with torch tensors (or (tensor, chroma) tuples), optional 5th
@@ -445,6 +469,19 @@ class SeedNet:
         n, h, w = la.shape
         return bool(lib.fgr_seednet_gradients(self._h, _stream(stream), _ptr(la), _ptr(lc), w, h, n, _ptr(target),
                                               _ptr(grad), _ptr(loss)))
+
+    def calibrate_fp8(self, la, lc, stream=None):
+        n, h, w = la.shape
+        return bool(lib.fgr_seednet_calibrate_fp8(self._h, _stream(stream), _ptr(la), _ptr(lc), w, h, n))
+
+    def set_fp8(self, on):
+        return bool(lib.fgr_seednet_set_fp8(self._h, int(on)))
+
+    def fp8_scales(self):
+        import numpy as np
+        sa, sw = np.empty(178, np.float32), np.empty(178, np.float32)
+        assert lib.fgr_seednet_fp8_scales(self._h, sa.ctypes.data, sw.ctypes.data)
+        return sa, sw
 
     def train_step(self, la, lc, target, lr, loss=None, stream=None):
         n, h, w = la.shape

@@ -139,6 +139,11 @@ int fgr_debug_map_layout(const FgrInterpolator* h, FgrDebugMapLayout* out);
 /* Block field (bw x bh float2, after sub-pixel), per-pixel field (fw x fh float2) and the packed mismatch e of the
  * latest synth (W x H bytes, eStride words per row, low byte first) into dst (device memory of layout.size bytes). */
 int fgr_record_debug_maps(FgrInterpolator* h, void* stream, void* dst);
+/* The 64-byte header of an fgtest --dump file (fgtest.cpp:24-28, the .fgmaps format read by gpu/fgmaps.py) for the
+ * maps fgr_record_debug_maps writes: uint32 magic 'FGMP', version 5, W, H, fw, fh, bw, bh, fscale, eStride, float32
+ * trustLow, trustHigh, smoothBonus, hudBonus, uint32 motionVersion, v2Mode (0, 0 for V1). The file is this header
+ * followed by the layout.size bytes of the maps. */
+int fgr_fgmaps_header(const FgrInterpolator* h, void* out64);
 
 /* ---- batched V1 (this library's addition; no counterpart in the reference) ----
  * Many pairs of any sizes, each with its own multiplier N (outputs at t = k/N, k = 1..N-1), processed as one launch
@@ -206,6 +211,15 @@ int fgr_seednet_train_step(FgrSeedNet* net, void* stream, const float* la, const
  * With null proposals this is the plain field, the training target. */
 int fgr_seednet_search_field(void* stream, const float* la, const float* lc, int32_t w, int32_t h, int32_t n, int32_t R,
                              float lam, const float* seeds, float* raw, float* out);
+/* FP8 inference (OCP E4M3 on v_mfma_f32_16x16x32_fp8_fp8, FP32 accumulation): calibrate on a batch of luma pairs
+ * (per-channel input scales sa = max|x| / 448, folded into the weights, per-output-channel weight scales sw), then
+ * switch the network's forward (and the proposals of attached handles, from their next record) between FP8 and BF16.
+ * fp8_scales copies all layers' sa (2 + 16 + 32 + 64 + 64 floats) and sw (16 + 32 + 64 + 64 + 2 floats) to the host.
+ * Training always runs in BF16; calibrate again after training. */
+int fgr_seednet_calibrate_fp8(FgrSeedNet* net, void* stream, const float* la, const float* lc, int32_t w, int32_t h,
+                              int32_t n);
+int fgr_seednet_set_fp8(FgrSeedNet* net, int on);
+int fgr_seednet_fp8_scales(const FgrSeedNet* net, float* sa, float* sw);
 /* attach (or detach with null) a network to a handle; it must outlive the handle's records that use it */
 int fgr_set_seed_net(FgrInterpolator* h, FgrSeedNet* net);
 

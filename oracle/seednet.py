@@ -114,3 +114,47 @@ def backward(la, lc, params, target, emulate=True, dtype=np.float32):
             if emulate:
                 g = g.astype(np.float32)
     return np.concatenate(grads), loss
+
+
+def e4m3(v):
+    """float32 -> OCP FP8 E4M3 value (bias 7, largest 448), round to nearest even, saturating (as float32)."""
+    v = np.asarray(v, np.float32)
+    a = np.abs(v).astype(np.float32)
+    out = np.empty_like(a)
+    sub = a < np.float32(2.0 ** -6)
+    out[sub] = np.rint(a[sub] * np.float32(512.0)) / np.float32(512.0)
+    nrm = ~sub
+    e = np.frexp(a[nrm])[1] - 1
+    out[nrm] = np.ldexp(np.rint(np.ldexp(a[nrm], 3 - e)), e - 3).astype(np.float32)
+    out = np.minimum(out, np.float32(448.0))
+    out[~(a < np.float32(448.0))] = np.float32(448.0)
+    return np.where(v < 0, -out, out).astype(np.float32)
+
+
+def fp8_tables(params, sa_all):
+    """The FP8 weights of every layer as the library builds them from the input scales sa: W' = W sa[ci] (float32),
+    sw[co] = max |W'[co]| / 448, wq = e4m3(W' / sw). Returns [(wq values (Cout, 9 Cin), sw (Cout))]."""
+    out, o = [], 0
+    for (W, b), (ci, co, _, _) in zip(split(params), LAYERS):
+        sa = sa_all[o:o + ci].astype(np.float32)
+        o += ci
+        Wp = (W.astype(np.float32) * np.tile(sa, 9)[None, :]).astype(np.float32)
+        m = np.abs(Wp).max(1)
+        sw = np.where(m > 0, (m / np.float32(448.0)).astype(np.float32), np.float32(1.0)).astype(np.float32)
+        out.append((e4m3((Wp / sw[:, None]).astype(np.float32)), sw))
+    return out
+
+
+def forward_fp8(la, lc, params, sa_all):
+    """FP8 forward: each layer's input x quantised per channel as e4m3(x / sa[ci]), times the FP8 weights, summed
+    (float64 here), times sw[co], plus the bias; ReLU after layers 1-4."""
+    x = net_input(la, lc)
+    o = 0
+    tabs = fp8_tables(params, sa_all)
+    for (W, b), (ci, co, s, relu), (wq, sw) in zip(split(params), LAYERS, tabs):
+        sa = sa_all[o:o + ci].astype(np.float32)
+        o += ci
+        A = e4m3((im2col(x, s) / np.tile(sa, 9)).astype(np.float32))
+        z = (A.astype(np.float64) @ wq.astype(np.float64).T).astype(np.float32) * sw + b.astype(np.float32)
+        x = (np.maximum(z, 0) if relu else z).astype(np.float32)
+    return x, tabs

@@ -46,6 +46,32 @@ def test_forward_matches_oracle():
     net.close()
 
 
+def test_fp8_forward_matches_oracle():
+    """FP8 (E4M3, per-channel scales) forward on v_mfma_f32_16x16x32_fp8_fp8 against the oracle's emulation with the
+    library's calibrated input scales: the weight scales are identical, the outputs equal to a tolerance."""
+    net = fgr.SeedNet(0)
+    p = he_params(5)
+    net.set_weights(p)
+    la, lc, _ = small_batch(4, 48, 32, 8)
+    assert net.calibrate_fp8(la, lc)
+    assert net.set_fp8(True)
+    out = torch.empty((4, 4, 6, 2), device="cuda:0")
+    assert net.forward(la, lc, out)
+    sa, sw = net.fp8_scales()
+    ref, tabs = N.forward_fp8(la.cpu().numpy(), lc.cpu().numpy(), p, sa)
+    assert np.array_equal(np.concatenate([t[1] for t in tabs]), sw)
+    got = out.cpu().numpy()
+    err = np.abs(got - ref).max()
+    assert err <= 2e-2 * (1.0 + np.abs(ref).max()), err
+    # and close to the BF16 forward of the same network
+    assert net.set_fp8(False)
+    out16 = torch.empty_like(out)
+    assert net.forward(la, lc, out16)
+    d = np.abs(out16.cpu().numpy() - got).max()
+    assert d <= 0.1 * (1.0 + np.abs(got).max()), d
+    net.close()
+
+
 def test_gradients_match_oracle():
     net = fgr.SeedNet(0)
     p = he_params(4)
@@ -82,6 +108,12 @@ def test_training_lowers_the_loss_and_beats_the_plain_seeds():
     assert r["epe_learned"] < 0.6 * r["epe_plain"]
     assert r["sad_learned"] < r["sad_plain"]
     assert r["seed_hit_learned"] > r["seed_hit_plain"]
+    # FP8 inference of the trained network, calibrated on training-like pairs: nearly the BF16 gain
+    la, lc, _ = small_batch(64, w, h, 13)
+    assert net.calibrate_fp8(la, lc) and net.set_fp8(True)
+    r8 = seedtrain.evaluate(net, 256, w, h, R=2, seed=12, shift=10.0)
+    print(f"seednet FP8 evaluation {r8}")
+    assert r8["epe_learned"] < 0.7 * r["epe_plain"] and r8["epe_learned"] < 1.25 * r["epe_learned"]
     net.close()
 
 
@@ -89,13 +121,17 @@ def _large_motion_pair():
     return scenes.pair_translate(21, w=640, h=360, sprite=48, sprite_pos=(300, 150), shift=(22, -14))
 
 
-def test_estimate_with_proposals_is_bit_exact():
+@pytest.mark.parametrize("fp8", [False, True])
+def test_estimate_with_proposals_is_bit_exact(fp8):
     """seedNet = 1 with a network attached: the proposals (read back from the library) fed to the oracle reproduce
     every intermediate of the estimate and the output bit for bit; the per-pair and the batched paths agree."""
     a, c = _large_motion_pair()
     s = fgr.default_settings(seedNet=1, pyramidMinSide=24, searchRadiusTop=2)
     net = fgr.SeedNet(1)
     net.set_weights(he_params(9))
+    if fp8:
+        la, lc, _ = small_batch(8, 80, 45, 14)
+        assert net.calibrate_fp8(la, lc) and net.set_fp8(True)
     fi = fgr.Interpolator(640, 360, settings=s)
     assert fi.set_seed_net(net)
     ta, tc = dev(a), dev(c)

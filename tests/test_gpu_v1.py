@@ -185,3 +185,38 @@ def test_1080p_workload_pair(k):
     for t, o in zip(ts, outs):
         check_output(o, a, c, est, t)
     fi.close()
+
+
+def _read_fgmaps(path):
+    """The .fgmaps layout (fgtest.cpp:24-28): 64-byte header, block field, per-pixel field, packed mismatch."""
+    raw = open(path, "rb").read()
+    h = np.frombuffer(raw[:64], np.uint32)
+    W, H, fw, fh, bw, bh, fs, es = (int(x) for x in h[2:10])
+    o = 64
+    block = np.frombuffer(raw, np.float32, bw * bh * 2, o).reshape(bh, bw, 2)
+    o += bw * bh * 8
+    pf = np.frombuffer(raw, np.float32, fw * fh * 2, o).reshape(fh, fw, 2)
+    o += fw * fh * 8
+    ew = np.frombuffer(raw, np.uint32, es * H, o).reshape(H, es)
+    assert o + es * H * 4 == len(raw)
+    e = np.stack([(ew >> (8 * i)) & 255 for i in range(4)], -1).reshape(H, es * 4)[:, :W]
+    return h, np.frombuffer(raw[40:56], np.float32), block, pf, e
+
+
+@pytest.mark.parametrize("t", [0.5, 0.75])
+def test_fgmaps_dump_matches_the_oracle(t, tmp_path):
+    a, c = scenes.pair_translate(41, w=203, h=117, sprite=32, sprite_pos=(70, 40), shift=(3, -2))
+    fi = fgr.Interpolator(203, 117)
+    ta, tc = dev(a), dev(c)   # the debug maps read the record's frames: they must still be alive
+    out = torch.zeros_like(ta)
+    assert fi.record_estimate(ta, tc) and fi.record_synth(ta, tc, out, t)
+    path = str(tmp_path / "pair_0000.fgmaps")
+    assert fi.dump_fgmaps(path)
+    h, hf, block, pf, e = _read_fgmaps(path)
+    est = v1.estimate(a, c)
+    _, _, info = v1.synth(a, c, est, t)
+    assert h[0] == 0x504D4746 and h[1] == 5 and list(h[2:10]) == [203, 117, 102, 58, 13, 8, 2, 51]
+    assert list(hf) == [12.0, 40.0, 0.5, 1.0] and h[14] == 0 and h[15] == 0
+    assert np.array_equal(block, est["block_field"]) and np.array_equal(pf, est["pf"])
+    assert np.array_equal(e, info["e"])
+    fi.close()

@@ -52,3 +52,17 @@ def test_bf16_rounding_is_nearest_even():
     r = N.bf16(x)
     assert r[0] == 1.0 and r[1] == 1.0 and r[2] == 1.0 + 2 ** -6 and r[3] == 1.0 and r[4] == -2.5
     assert abs(r[5] - 3.0e-3) <= 3.0e-3 * 2 ** -8
+
+
+def test_e4m3_rounding_and_range():
+    v = np.array([0.0, 1.0, 1.0625, 1.1875, 448.0, 470.0, 1e6, -3.3, 2.0 ** -9, 3 * 2.0 ** -10, 2.0 ** -11,
+                  0.01708984375], np.float32)
+    q = N.e4m3(v)
+    # 1.0625 is halfway between 1 and 1.125 -> even (1); 1.1875 halfway between 1.125 and 1.25 -> 1.25; saturation at
+    # 448; -3.3 -> -3.25 (step 0.25 in [2, 4)); subnormals are multiples of 2^-9 (3 * 2^-10 is halfway -> 2^-8 even);
+    # 2^-11 -> 0; 0.01708984375 (= 2^-6 * 1.09375) -> 2^-6 * 1.125
+    assert list(q) == [0.0, 1.0, 1.0, 1.25, 448.0, 448.0, 448.0, -3.25, 2.0 ** -9, 2.0 ** -8, 0.0, 2.0 ** -6 * 1.125]
+    # every representable positive value maps to itself
+    vals = np.array([m * 2.0 ** -9 for m in range(8)] + [(1 + m / 8) * 2.0 ** e for e in range(-6, 9) for m in range(8)
+                                                         if (1 + m / 8) * 2.0 ** e <= 448], np.float32)
+    assert np.array_equal(N.e4m3(vals), vals)
