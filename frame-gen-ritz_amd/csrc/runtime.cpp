@@ -336,6 +336,23 @@ FrameRef FgrInterpolator::ref(const FgrFrame& f) const {
 
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
+// framegen.cpp:429-451: the longest integer half-vector (flow pixels) the search can reach at level j of an n-level
+// pyramid -- the top radius Rt, then 2x the parent's + R per level -- and the median histogram of the step whose parent
+// is level `parent`: -1 = bins of [-128, 127] (the default settings always, so their results never change), else the
+// exact two-pass median with first-pass bins of 2^sh pixels, the smallest that hold that bound.
+static int64_t reachBound(int n, int j, int R, int Rt) {
+    const int64_t a = (int64_t)1 << (n - 1 - j);
+    return Rt * a + (int64_t)R * (a - 1);
+}
+
+static int medhistShift(const FgrSettings& s, int n, int parent) {
+    const bool nonDefault = s.searchRadius != 2 || s.searchRadiusTop != 6 || s.pyramidMinSide != 48;
+    const int64_t b = reachBound(n, parent, (int)s.searchRadius, (int)s.searchRadiusTop);
+    int sh = 0;
+    while ((b >> sh) > 127) sh++;
+    return nonDefault && b > 127 ? sh : -1;
+}
+
 void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& prev, const FgrFrame& curr) const {
     const FgrSettings& st = set;
     float2* F = (float2*)field;
@@ -419,6 +436,7 @@ void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& pr
             mp.bh = l.bh;
             mp.u = u;
             mp.medIdx = kUMedian + 2 * i;
+            mp.shift = medhistShift(st, n, i + 1);
             mp.gx = 1;
             PlanOp op;
             op.pass = kPassMedhist;

@@ -93,16 +93,13 @@ __device__ __forceinline__ void histAdd(uint32_t* hist, int bin, int weight) {
     }
 }
 
-__device__ __forceinline__ void medhistBody(const MedParams& P, int, int) {
-    __shared__ uint32_t hx[256], hy[256];
-    __shared__ uint32_t wsum[2][4];
-    __shared__ float kv[4];
-    const int t = threadIdx.x;
-    if (t < 256) {
-        hx[t] = 0u;
-        hy[t] = 0u;
-    }
-    if (t < 4) kv[t] = 127.0f;
+// The median of the wide reach settings (medhist.comp FG_WIDE): first-pass bins of 2^sh pixels, bin = (v >> sh) + 128,
+// hold every vector the settings can reach; each of the four order statistics lies in one bin (tb, with tbase elements
+// before it); a second histogram of the low sh bits of the vectors in those bins gives the exact value.
+__device__ __forceinline__ void medhistFine(const MedParams& P, const float* kvBins, const uint32_t* tbase, float* kv) {
+    __shared__ uint32_t fine[4][256];
+    const int t = threadIdx.x, sh = P.shift, mask = (1 << sh) - 1;
+    fine[t >> 8][t & 255] = 0u;
     __syncthreads();
     const int np = P.pw * P.ph;
     for (int i0 = 0; i0 < np; i0 += 1024) {
@@ -112,8 +109,54 @@ __device__ __forceinline__ void medhistBody(const MedParams& P, int, int) {
             const int wgt = parentWeight(px, P.pw, P.bw) * parentWeight(py, P.ph, P.bh);
             if (wgt > 0) {
                 const float2 v = P.parent[i];
-                histAdd(hx, clampi((int)rintf(v.x) + 128, 0, 255), wgt);
-                histAdd(hy, clampi((int)rintf(v.y) + 128, 0, 255), wgt);
+                const int vx = (int)rintf(v.x), vy = (int)rintf(v.y);
+                for (int j = 0; j < 4; j++) {
+                    const int c = j < 2 ? vx : vy;
+                    const bool in = clampi((c >> sh) + 128, 0, 255) == (int)kvBins[j] + 128;
+                    if (__any(in)) histAdd(fine[j], in ? (c & mask) : 0, in ? wgt : 0);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (t < 4) {
+        const uint32_t n = (uint32_t)(P.bw * P.bh), k = (t & 1) == 0 ? (n - 1u) / 2u : n / 2u;
+        uint32_t acc = tbase[t];
+        int f = 0;
+        for (; f < mask; f++) {
+            acc += fine[t][f];
+            if (acc > k) break;
+        }
+        kv[t] = (float)((int)kvBins[t] * (1 << sh) + f);
+    }
+}
+
+__device__ __forceinline__ void medhistBody(const MedParams& P, int, int) {
+    __shared__ uint32_t hx[256], hy[256];
+    __shared__ uint32_t wsum[2][4];
+    __shared__ float kv[4];
+    __shared__ uint32_t tbase[4];
+    const int t = threadIdx.x;
+    const int sh = P.shift < 0 ? 0 : P.shift;
+    if (t < 256) {
+        hx[t] = 0u;
+        hy[t] = 0u;
+    }
+    if (t < 4) {
+        kv[t] = 127.0f;
+        tbase[t] = 0u;
+    }
+    __syncthreads();
+    const int np = P.pw * P.ph;
+    for (int i0 = 0; i0 < np; i0 += 1024) {
+        const int i = i0 + t;
+        if (i < np) {
+            const int px = i % P.pw, py = i / P.pw;
+            const int wgt = parentWeight(px, P.pw, P.bw) * parentWeight(py, P.ph, P.bh);
+            if (wgt > 0) {
+                const float2 v = P.parent[i];
+                histAdd(hx, clampi(((int)rintf(v.x) >> sh) + 128, 0, 255), wgt);
+                histAdd(hy, clampi(((int)rintf(v.y) >> sh) + 128, 0, 255), wgt);
             }
         }
     }
@@ -146,12 +189,19 @@ __device__ __forceinline__ void medhistBody(const MedParams& P, int, int) {
         const uint32_t n = (uint32_t)(P.bw * P.bh), k1 = (n - 1u) / 2u, k2 = n / 2u;
         const uint32_t ex = cx - hx[t], ey = cy - hy[t];
         const float b = (float)(t - 128);
-        if (ex <= k1 && k1 < cx) kv[0] = b;
-        if (ex <= k2 && k2 < cx) kv[1] = b;
-        if (ey <= k1 && k1 < cy) kv[2] = b;
-        if (ey <= k2 && k2 < cy) kv[3] = b;
+        if (ex <= k1 && k1 < cx) { kv[0] = b; tbase[0] = ex; }
+        if (ex <= k2 && k2 < cx) { kv[1] = b; tbase[1] = ex; }
+        if (ey <= k1 && k1 < cy) { kv[2] = b; tbase[2] = ey; }
+        if (ey <= k2 && k2 < cy) { kv[3] = b; tbase[3] = ey; }
     }
     __syncthreads();
+    if (P.shift >= 0) {
+        __shared__ float kb[4];
+        if (t < 4) kb[t] = kv[t];
+        __syncthreads();
+        medhistFine(P, kb, tbase, kv);
+        __syncthreads();
+    }
     if (t < 2) {
         const float m = 0.5f * (kv[2 * t] + kv[2 * t + 1]);
         P.u[P.medIdx + t] = __float_as_uint(2.0f * m);

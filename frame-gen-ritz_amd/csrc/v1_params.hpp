@@ -59,6 +59,8 @@ struct MedParams {
     const float2* parent; int pw, ph;   // parent (coarser) block field, median3 output
     int bw, bh;                         // this level's block grid
     uint32_t* u; int medIdx;
+    int shift;                          // -1: bins of [-128, 127] parent pixels (clamped); else the exact two-pass median
+                                        // with first-pass bins of 2^shift pixels (medhist.comp FG_WIDE)
 };
 
 struct SearchParams {

@@ -178,10 +178,12 @@ def search(la, lc, seed, R, lam):
     return v, best_sad
 
 
-def global_median2(parent, bw, bh):
+def global_median2(parent, bw, bh, shift=-1):
     """medhist.comp:73-106 (interp.py:142 np.median of the own-parent vectors): exact per-component median of
     roundEven(parent(min(ix/2, pw-1), min(iy/2, ph-1))) over all fine blocks, bins clamped to [-128, 127];
-    returns float32 2 * 0.5 * (k1-th + k2-th), k1 = (n-1)//2, k2 = n//2."""
+    returns float32 2 * 0.5 * (k1-th + k2-th), k1 = (n-1)//2, k2 = n//2.
+    shift >= 0 (medhist.comp:9-71, FG_WIDE): first-pass bins b = clamp((v >> shift) + 128, 0, 255); the k-th element's
+    bin, then the k-th among the low `shift` bits (v & mask) of the vectors in that bin, scanned up to mask - 1."""
     ph, pw = parent.shape[:2]
     iy, ix = np.meshgrid(np.arange(bh), np.arange(bw), indexing="ij")
     v = np.rint(parent[np.minimum(iy // 2, ph - 1), np.minimum(ix // 2, pw - 1)]).astype(np.int64)
@@ -189,10 +191,42 @@ def global_median2(parent, bw, bh):
     k1, k2 = (n - 1) // 2, n // 2
     out = []
     for c in range(2):
-        vals = np.sort(np.clip(v[..., c], -128, 127).ravel())
-        m = f32(0.5) * (f32(vals[k1]) + f32(vals[k2]))
+        if shift < 0:
+            vals = np.sort(np.clip(v[..., c], -128, 127).ravel())
+            m = f32(0.5) * (f32(vals[k1]) + f32(vals[k2]))
+        else:
+            x = v[..., c].ravel()
+            bins = np.clip((x >> shift) + 128, 0, 255)
+            mask = (1 << shift) - 1
+            ks = []
+            for k in (k1, k2):
+                hist = np.bincount(bins, minlength=256)
+                cum = np.cumsum(hist)
+                b = int(np.argmax(cum > k))
+                acc = int(cum[b] - hist[b])
+                fine = np.bincount(x[bins == b] & mask, minlength=max(mask + 1, 1))
+                f = 0
+                while f < mask:
+                    acc += int(fine[f])
+                    if acc > k:
+                        break
+                    f += 1
+                ks.append((b - 128) * (1 << shift) + f)
+            m = f32(0.5) * (f32(ks[0]) + f32(ks[1]))
         out.append(f32(2.0) * m)
     return np.array(out, f32)
+
+
+def medhist_shift(s, n, parent):
+    """framegen.cpp:429-451: -1 (the clamped histogram) unless a reach setting is off its default and the reach
+    bound Rt 2^(n-1-j) + R (2^(n-1-j) - 1) of the parent level j exceeds 127; then the smallest shift holding it."""
+    a = 1 << (n - 1 - parent)
+    b = s["searchRadiusTop"] * a + s["searchRadius"] * (a - 1)
+    non_default = s["searchRadius"] != 2 or s["searchRadiusTop"] != 6 or s["pyramidMinSide"] != 48
+    sh = 0
+    while (b >> sh) > 127:
+        sh += 1
+    return sh if non_default and b > 127 else -1
 
 
 def seed_from(la, lc, parent, med2, bw, bh, lam, extra=None):
@@ -429,7 +463,7 @@ def estimate(a, c, s=None, bgra=False, luma0=None, size=None, proposals=None):
                 seed = seed_coarse(pa[i], pc[i], proposals, lam)
             R = s["searchRadiusTop"]
         else:
-            meds[i] = global_median2(v, bw, bh)
+            meds[i] = global_median2(v, bw, bh, medhist_shift(s, n, i + 1))
             seed = seed_from(pa[i], pc[i], v, meds[i], bw, bh, lam)
             R = s["searchRadius"]
         vi, sadv = search(pa[i], pc[i], seed, R, lam)

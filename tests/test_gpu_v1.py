@@ -220,3 +220,20 @@ def test_fgmaps_dump_matches_the_oracle(t, tmp_path):
     assert np.array_equal(block, est["block_field"]) and np.array_equal(pf, est["pf"])
     assert np.array_equal(e, info["e"])
     fi.close()
+
+
+@pytest.mark.parametrize("kw", [dict(searchRadius=12, searchRadiusTop=12, pyramidMinSide=16),
+                                dict(searchRadius=6, searchRadiusTop=12, pyramidMinSide=16)])
+def test_wide_reach_settings(kw):
+    """Reach settings off their defaults: radius-12 windows (the wide search) and, where the parent level's reach bound
+    exceeds 127 flow pixels, the exact two-pass median (medhist.comp FG_WIDE); a large motion (120, -80) px."""
+    a, c = scenes.pair_translate(43, w=1280, h=720, sprite=64, sprite_pos=(500, 300), shift=(120, -80))
+    s = fgr.default_settings(**kw)
+    fi, I, outs, counts = run(a, c, (0.5, 0.25), settings=s)
+    est = v1.estimate(a, c, oracle_settings(s))
+    n = len(est["levels"])
+    assert any(v1.medhist_shift(est["settings"], n, j) >= 0 for j in range(1, n))
+    check_estimate(I, est)
+    for t, o in zip((0.5, 0.25), outs):
+        check_output(o, a, c, est, t)
+    fi.close()

@@ -273,3 +273,21 @@ def test_interp_selfcheck_thresholds():
         f[..., 3] = 255
     o, _, _ = v1.synth(a2, c2, v1.estimate(a2, c2), 0.5)
     assert psnr(o[hud][..., :3], b2[hud][..., :3]) > 30
+
+
+def test_wide_median_is_the_exact_median():
+    """medhist.comp FG_WIDE: with first-pass bins wide enough for every vector, the two-pass histogram median is the
+    exact median (np.median's order statistics); at shift 0 and small vectors it equals the clamped default."""
+    rng = np.random.default_rng(3)
+    for shift, lim in ((3, 900), (1, 200), (0, 100)):
+        parent = rng.integers(-lim, lim + 1, (9, 13, 2)).astype(np.float32)
+        bw, bh = 25, 17
+        got = v1.global_median2(parent, bw, bh, shift)
+        iy, ix = np.meshgrid(np.arange(bh), np.arange(bw), indexing="ij")
+        v = parent[np.minimum(iy // 2, 8), np.minimum(ix // 2, 12)].reshape(-1, 2)
+        want = np.array([2.0 * np.median(v[:, 0]), 2.0 * np.median(v[:, 1])], np.float32)
+        assert np.array_equal(got, want), (shift, got, want)
+        if lim <= 127:
+            assert np.array_equal(got, v1.global_median2(parent, bw, bh))
+    s = v1.settings(searchRadius=12, searchRadiusTop=12, pyramidMinSide=16)
+    assert v1.medhist_shift(s, 5, 1) == 1 and v1.medhist_shift(s, 5, 4) == -1 and v1.medhist_shift(v1.DEFAULTS, 6, 1) == -1
