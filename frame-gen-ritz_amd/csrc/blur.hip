@@ -84,21 +84,34 @@ __global__ __launch_bounds__(256) void blur_average_kernel(BlurParams P) {
         *(uint2*)(P.out + (int64_t)y * P.opitch + 8 * (int64_t)x) = o;
         return;
     }
-    // P010: the Y code of every pixel; the CbCr codes of each 2x2 block at its even-even pixel
+    // Nv12 / P010: the Y code of every pixel; the CbCr codes of each 2x2 block at its even-even pixel
+    const bool p010 = P.fmt == kP010;
+    const float top = p010 ? 1023.0f : 255.0f;
     float acc = 0.0f;
-    for (int k = 0; k < P.n; k++)
-        acc = acc + P.w[k] * (float)(((const uint16_t*)(P.src[k] + (int64_t)y * P.pitch[k]))[x] >> 6);
-    ((uint16_t*)(P.out + (int64_t)y * P.opitch))[x] = (uint16_t)((uint32_t)rnd(acc, P.wsum, 1023.0f) << 6);
+    for (int k = 0; k < P.n; k++) {
+        const uint8_t* row = P.src[k] + (int64_t)y * P.pitch[k];
+        acc = acc + P.w[k] * (float)(p010 ? ((const uint16_t*)row)[x] >> 6 : row[x]);
+    }
+    const uint32_t yc = (uint32_t)rnd(acc, P.wsum, top);
+    if (p010) ((uint16_t*)(P.out + (int64_t)y * P.opitch))[x] = (uint16_t)(yc << 6);
+    else P.out[(int64_t)y * P.opitch + x] = (uint8_t)yc;
     if (((x | y) & 1) == 0) {
         const int cx = x >> 1, cy = y >> 1;
         float cb = 0.0f, cr = 0.0f;
         for (int k = 0; k < P.n; k++) {
-            const uint32_t v = *(const uint32_t*)(P.srcC[k] + (int64_t)cy * P.cpitch[k] + 4 * (int64_t)cx);
-            cb = cb + P.w[k] * (float)((v & 0xFFFFu) >> 6);
-            cr = cr + P.w[k] * (float)(v >> 22);
+            if (p010) {
+                const uint32_t v = *(const uint32_t*)(P.srcC[k] + (int64_t)cy * P.cpitch[k] + 4 * (int64_t)cx);
+                cb = cb + P.w[k] * (float)((v & 0xFFFFu) >> 6);
+                cr = cr + P.w[k] * (float)(v >> 22);
+            } else {
+                const uint32_t v = *(const uint16_t*)(P.srcC[k] + (int64_t)cy * P.cpitch[k] + 2 * (int64_t)cx);
+                cb = cb + P.w[k] * (float)(v & 255u);
+                cr = cr + P.w[k] * (float)(v >> 8);
+            }
         }
-        *(uint32_t*)(P.outc + (int64_t)cy * P.ocpitch + 4 * (int64_t)cx) =
-            ((uint32_t)rnd(cb, P.wsum, 1023.0f) << 6) | ((uint32_t)rnd(cr, P.wsum, 1023.0f) << 22);
+        const uint32_t b = (uint32_t)rnd(cb, P.wsum, top), r = (uint32_t)rnd(cr, P.wsum, top);
+        if (p010) *(uint32_t*)(P.outc + (int64_t)cy * P.ocpitch + 4 * (int64_t)cx) = (b << 6) | (r << 22);
+        else *(uint16_t*)(P.outc + (int64_t)cy * P.ocpitch + 2 * (int64_t)cx) = (uint16_t)(b | (r << 8));
     }
 }
 
@@ -110,10 +123,12 @@ bool FgrInterpolator::enqueueBlur(hipStream_t s, const FgrFrame& prev, const Fgr
                                   float t0, float t1) {
     const int n = (int)set.blurSamples;
     if (n < 2 || n > FGR_BLUR_MAX_SAMPLES) return false;
-    // scratch: one sub-frame per sample (plane 0, then the P010 chroma plane)
+    // scratch: one sub-frame per sample (plane 0, then the chroma plane of Nv12 / P010)
     const int bpp = bytesPerPixel();
+    const bool yuvFmt = format == FGR_FORMAT_P010 || format == FGR_FORMAT_NV12;
+    const int cbpp = format == FGR_FORMAT_NV12 ? 2 : 4;
     const size_t plane = (size_t)W * H * bpp;
-    const size_t cplane = format == FGR_FORMAT_P010 ? (size_t)((W + 1) / 2) * ((H + 1) / 2) * 4 : 0;
+    const size_t cplane = yuvFmt ? (size_t)((W + 1) / 2) * ((H + 1) / 2) * cbpp : 0;
     const size_t need = (size_t)n * (plane + cplane);
     if (need > blurCap) {
         if (blurScratch) hipFree(blurScratch);
@@ -155,7 +170,7 @@ bool FgrInterpolator::enqueueBlur(hipStream_t s, const FgrFrame& prev, const Fgr
         sub.pitch = (uint64_t)W * bpp;
         if (cplane) {
             sub.chroma = base + plane;
-            sub.chromaPitch = (uint64_t)((W + 1) / 2) * 4;
+            sub.chromaPitch = (uint64_t)((W + 1) / 2) * cbpp;
         }
         planSynth(plan, prev, curr, sub, t, k0++);
         bp.src[k] = (const uint8_t*)sub.ptr;

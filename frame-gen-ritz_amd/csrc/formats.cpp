@@ -3,7 +3,7 @@
 //   RgbaF16: the metric level of every half value (fgMetric format.glsl:70-73) in units of 1/16 level, built in float64:
 //            encoded M = v * 255; linear M = 255 PQ(clamp(v * linearWhiteNits / 10000, 0, 1)) (fgPq :65-69);
 //            L16 = floor(16 M + 0.5), clamped to +-65535 (+-4096 levels), NaN -> 0. One 256 KB device table.
-//   P010:    the YCbCr -> R'G'B' difference constants YU_YS, YU_RCR, YU_BCB, YU_GCB, YU_GCR (framegen.cpp:1519-1533).
+//   Nv12 / P010: the YCbCr -> R'G'B' difference constants YU_YS, YU_RCR, YU_BCB, YU_GCB, YU_GCR (framegen.cpp:1519-1533).
 #include <hip/hip_runtime.h>
 #include <cmath>
 #include <cstring>
@@ -51,7 +51,7 @@ bool FgrInterpolator::formatInit() {
         }
         if (hipMemcpy(fmtLut, lut.data(), 65536 * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) return false;
     }
-    if (format == FGR_FORMAT_P010) {
+    if (format == FGR_FORMAT_P010 || format == FGR_FORMAT_NV12) {
         double kr = 0.2126, kb = 0.0722;   // BT.709
         if (info.yuvMatrix == FGR_YUV_BT601) { kr = 0.299; kb = 0.114; }
         if (info.yuvMatrix == FGR_YUV_BT2020) { kr = 0.2627; kb = 0.0593; }

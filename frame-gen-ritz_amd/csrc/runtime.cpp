@@ -21,7 +21,7 @@ thread_local const char* gInitError = nullptr;
 
 const char* kErrSettings = "invalid settings: a field is out of range or not finite (see fgr_validate)";
 const char* kErrVersion = "only MotionVersion::V1 is built in this library";
-const char* kErrFormat = "unsupported format (Rgba8, Rgb10, RgbaF16 and P010 are built; Nv12 is not)";
+const char* kErrFormat = "unsupported format (Rgba8, Rgb10, RgbaF16, Nv12 and P010 are built)";
 const char* kErrMinSize = "the frame size is below FGR_MIN_SIZE (16 pixels) in width or height";
 const char* kErrPixels = "the frame size exceeds 2^30 pixels (32-bit indexing of the library's buffers)";
 const char* kErrDim = "the frame size exceeds 32768 pixels in width or height";
@@ -30,7 +30,7 @@ const char* kErrAlloc = "a device allocation failed";
 const char* kErrNull = "null InitInfo";
 const char* kErrWhite = "linearWhiteNits must be in (0, 10000]";
 const char* kErrYuv = "invalid yuvMatrix";
-const char* kErrOdd = "P010 frames need an even width and height (4:2:0 chroma of whole 2x2 blocks)";
+const char* kErrOdd = "Nv12 and P010 frames need an even width and height (4:2:0 chroma of whole 2x2 blocks)";
 
 bool in(float v, float lo, float hi) { return std::isfinite(v) && v >= lo && v <= hi; }
 
@@ -74,7 +74,8 @@ extern "C" int fgr_validate(const FgrSettings* s) {
 }
 
 static bool formatBuilt(uint32_t f) {
-    return f == FGR_FORMAT_RGBA8 || f == FGR_FORMAT_RGB10 || f == FGR_FORMAT_RGBAF16 || f == FGR_FORMAT_P010;
+    return f == FGR_FORMAT_RGBA8 || f == FGR_FORMAT_RGB10 || f == FGR_FORMAT_RGBAF16 || f == FGR_FORMAT_NV12 ||
+           f == FGR_FORMAT_P010;
 }
 
 extern "C" const char* fgr_settings_error_for_format(const FgrSettings* s, uint32_t format) {
@@ -112,7 +113,7 @@ static const char* sizeLimitError(uint32_t W, uint32_t H, uint32_t format) {
     if (W < FGR_MIN_SIZE || H < FGR_MIN_SIZE) return kErrMinSize;
     if ((uint64_t)W * H > (1ull << 30)) return kErrPixels;
     if (W > 32768 || H > 32768) return kErrDim;
-    if (format == FGR_FORMAT_P010 && ((W | H) & 1)) return kErrOdd;
+    if ((format == FGR_FORMAT_P010 || format == FGR_FORMAT_NV12) && ((W | H) & 1)) return kErrOdd;
     return nullptr;
 }
 
@@ -193,7 +194,10 @@ extern "C" FgrInterpolator* fgr_create(const FgrInitInfo* info) {
     if (const char* e = sizeLimitError(info->width, info->height, info->format)) { gInitError = e; return nullptr; }
     if (info->format == FGR_FORMAT_RGBAF16 && info->transfer == FGR_TRANSFER_LINEAR &&
         !(info->linearWhiteNits > 0.0f && info->linearWhiteNits <= 10000.0f)) { gInitError = kErrWhite; return nullptr; }
-    if (info->format == FGR_FORMAT_P010 && info->yuvMatrix > FGR_YUV_BT2020) { gInitError = kErrYuv; return nullptr; }
+    if ((info->format == FGR_FORMAT_P010 || info->format == FGR_FORMAT_NV12) && info->yuvMatrix > FGR_YUV_BT2020) {
+        gInitError = kErrYuv;
+        return nullptr;
+    }
     if (hipSetDevice(info->device) != hipSuccess) { gInitError = kErrDevice; return nullptr; }
     FgrInterpolator* h = new FgrInterpolator();
     h->info = *info;
@@ -321,7 +325,7 @@ extern "C" int fgr_read_timings(FgrInterpolator* h, const char** names, uint64_t
 
 bool FgrInterpolator::frameOk(const FgrFrame& f) const {
     if (!f.ptr || f.pitch == 0) return false;
-    if (format == FGR_FORMAT_P010 && !f.chroma) return false;
+    if ((format == FGR_FORMAT_P010 || format == FGR_FORMAT_NV12) && !f.chroma) return false;
     return true;
 }
 
@@ -620,7 +624,8 @@ void FgrInterpolator::planSynth(std::vector<PlanOp>& plan, const FgrFrame& prev,
     fp.a = sp.a;
     fp.c = sp.c;
     fp.out = sp.out;
-    fp.outc = format == FGR_FORMAT_P010 ? sp.outc : nullptr;
+    fp.outc = (format == FGR_FORMAT_P010 || format == FGR_FORMAT_NV12) ? sp.outc : nullptr;
+    fp.cbpp = format == FGR_FORMAT_NV12 ? 2 : 4;
     fp.opitch = sp.opitch;
     fp.ocpitch = sp.ocpitch;
     fp.W = (int)W;
@@ -655,6 +660,7 @@ int FgrInterpolator::bytesPerPixel() const {
     switch (format) {
     case FGR_FORMAT_RGBAF16: return 8;
     case FGR_FORMAT_P010: return 2;
+    case FGR_FORMAT_NV12: return 1;
     default: return 4;
     }
 }
@@ -784,8 +790,9 @@ bool FgrInterpolator::copyFrame(hipStream_t s, const FgrFrame& src, const FgrFra
     const size_t row = (size_t)W * bytesPerPixel();
     if (hipMemcpy2DAsync(dst.ptr, dst.pitch, src.ptr, src.pitch, row, H, hipMemcpyDeviceToDevice, s) != hipSuccess)
         return false;
-    if (format == FGR_FORMAT_P010 &&
-        hipMemcpy2DAsync(dst.chroma, dst.chromaPitch, src.chroma, src.chromaPitch, (size_t)((W + 1) / 2) * 4,
+    if ((format == FGR_FORMAT_P010 || format == FGR_FORMAT_NV12) &&
+        hipMemcpy2DAsync(dst.chroma, dst.chromaPitch, src.chroma, src.chromaPitch,
+                         (size_t)((W + 1) / 2) * (format == FGR_FORMAT_NV12 ? 2 : 4),
                          (H + 1) / 2, hipMemcpyDeviceToDevice, s) != hipSuccess)
         return false;
     return true;

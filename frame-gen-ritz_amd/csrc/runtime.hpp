@@ -43,7 +43,7 @@ struct FgrInterpolator {
 
     // per-format constants (formats.cpp)
     int* fmtLut = nullptr;        // RgbaF16: metric level of every half value in 1/16 level (65536 entries)
-    float yuv[8] = {};            // P010: YU_YS, YU_RCR, YU_BCB, YU_GCB, YU_GCR
+    float yuv[8] = {};            // Nv12 / P010: YU_YS, YU_RCR, YU_BCB, YU_GCB, YU_GCR
     bool formatInit();
     void formatFree();
 

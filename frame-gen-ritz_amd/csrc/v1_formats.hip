@@ -1,6 +1,7 @@
 // V1 luma and synth kernels of the formats other than RGBA8 (gpu/shaders/format.glsl:29-106, luma.comp:9-27,
 // motion_v1.glsl:85-144, resolve.comp:48-83): Rgb10 (A2B10G10R10 / A2R10G10B10 UNORM), RgbaF16 (half floats, encoded
-// or linear light) and P010 (Y plane of 16-bit words holding 10-bit codes, half-resolution CbCr plane).
+// or linear light), Nv12 (8-bit Y plane, half-resolution 8-bit CbCr plane) and P010 (the same with 16-bit words
+// holding 10-bit codes). A level is an 8-bit-equivalent code: Nv12 codes as they are, 10-bit codes / 4.
 //
 // The reference reads these formats through the texture filter and leaves the luma unrounded, which no other GPU
 // reproduces bit for bit. Here every step is fixed: the luma is a fixed-point value in 1/16 level; the warp taps are
@@ -31,6 +32,7 @@ __device__ __forceinline__ uint32_t toHalfBits(float v) {   // round to nearest 
 //   Rgb10   round((4899 R + 9617 G + 1868 B) * 255 / (1023 * 16384) * 16) on 10-bit codes (halves up)
 //   RgbaF16 (4899 L_r + 9617 L_g + 1868 L_b + 8192) >> 14, L = the 1/16-level metric table of each half value
 //   P010    round((4 code - 256) * 255 / 219) at limited range (16 (code / 4 - 16) * 255 / 219), 4 code at full range
+//   Nv12    round((16 code - 256) * 255 / 219) at limited range (16 (code - 16) * 255 / 219), 16 code at full range
 __device__ __forceinline__ int luma16(const LumaParams& P, const FrameRef& f, int x, int y) {
     const uint8_t* row = f.p + (int64_t)y * f.pitch;
     if (P.fmt == kRgb10) {
@@ -48,6 +50,10 @@ __device__ __forceinline__ int luma16(const LumaParams& P, const FrameRef& f, in
         const uint2 q = *(const uint2*)(row + 8 * (int64_t)x);
         const int lr = P.lut[q.x & 0xFFFFu], lg = P.lut[q.x >> 16], lb = P.lut[q.y & 0xFFFFu];
         return (lr * 4899 + lg * 9617 + lb * 1868 + 8192) >> 14;   // |L| <= 65535: no overflow
+    }
+    if (P.fmt == kNv12) {
+        const int c = row[x];
+        return P.yFull ? 16 * c : floorDiv((16 * c - 256) * 510 + 219, 438);
     }
     const int c = ((const uint16_t*)row)[x] >> 6;
     return P.yFull ? 4 * c : floorDiv((4 * c - 256) * 510 + 219, 438);
@@ -136,24 +142,58 @@ __device__ __forceinline__ void f16Warp(const FrameRef& f, int W, int H, int X, 
     }
 }
 
-// P010 luma: exact bilinear sum of the 10-bit Y codes (level = sum / 4096)
-__device__ __forceinline__ int p010YSum(const FrameRef& f, int W, int H, int X, int Y) {
+// Two-plane YCbCr (TEN: P010's 10-bit codes in 16-bit words, else Nv12's bytes). Levels: P010 code / 4, Nv12 code.
+template <bool TEN>
+struct Yuv {
+    static constexpr float kQ = TEN ? 4.0f : 1.0f;          // codes per level
+    static constexpr int kTop = TEN ? 1023 : 255;
+    static __device__ __forceinline__ int y(const uint8_t* row, int x) {
+        return TEN ? ((const uint16_t*)row)[x] >> 6 : row[x];
+    }
+    static __device__ __forceinline__ void c(const uint8_t* row, int x, int& cb, int& cr) {
+        if (TEN) {
+            const uint32_t w = ((const uint32_t*)row)[x];
+            cb = (int)((w & 0xFFFFu) >> 6);
+            cr = (int)(w >> 22);
+        } else {
+            const uint32_t w = ((const uint16_t*)row)[x];
+            cb = (int)(w & 255u);
+            cr = (int)(w >> 8);
+        }
+    }
+    static __device__ __forceinline__ void putY(uint8_t* row, int x, int code) {
+        if (TEN) ((uint16_t*)row)[x] = (uint16_t)(code << 6);
+        else row[x] = (uint8_t)code;
+    }
+    static __device__ __forceinline__ void putC(uint8_t* row, int x, int cb, int cr) {
+        if (TEN) ((uint32_t*)row)[x] = ((uint32_t)cb << 6) | ((uint32_t)cr << 22);
+        else ((uint16_t*)row)[x] = (uint16_t)(cb | (cr << 8));
+    }
+};
+
+// luma: exact bilinear sum of the Y codes (level = sum / (1024 q))
+template <bool TEN>
+__device__ __forceinline__ int yuvYSum(const FrameRef& f, int W, int H, int X, int Y) {
     const Taps<5> t(X, Y, W, H);
-    const uint16_t* r0 = (const uint16_t*)(f.p + (int64_t)t.y0 * f.pitch);
-    const uint16_t* r1 = (const uint16_t*)(f.p + (int64_t)t.y1 * f.pitch);
-    return isum(t, r0[t.x0] >> 6, r0[t.x1] >> 6, r1[t.x0] >> 6, r1[t.x1] >> 6);
+    const uint8_t* r0 = f.p + (int64_t)t.y0 * f.pitch;
+    const uint8_t* r1 = f.p + (int64_t)t.y1 * f.pitch;
+    return isum(t, Yuv<TEN>::y(r0, t.x0), Yuv<TEN>::y(r0, t.x1), Yuv<TEN>::y(r1, t.x0), Yuv<TEN>::y(r1, t.x1));
 }
 
-// P010 chroma at luma map (X, Y) (1/32 px): with centre-sited chroma on a half-resolution plane the point is
-// (X - 16) / 64 chroma texels; exact bilinear sums of the 10-bit Cb and Cr codes (level = sum / 16384)
-__device__ __forceinline__ void p010CSums(const FrameRef& f, int cw, int ch, int X, int Y, int& cb, int& cr) {
+// chroma at luma map (X, Y) (1/32 px): with centre-sited chroma on a half-resolution plane the point is (X - 16) / 64
+// chroma texels; exact bilinear sums of the Cb and Cr codes (level = sum / (4096 q))
+template <bool TEN>
+__device__ __forceinline__ void yuvCSums(const FrameRef& f, int cw, int ch, int X, int Y, int& cb, int& cr) {
     const Taps<6> t(X - 16, Y - 16, cw, ch);
-    const uint32_t* r0 = (const uint32_t*)(f.c + (int64_t)t.y0 * f.cpitch);
-    const uint32_t* r1 = (const uint32_t*)(f.c + (int64_t)t.y1 * f.cpitch);
-    const uint32_t p00 = r0[t.x0], p01 = r0[t.x1], p10 = r1[t.x0], p11 = r1[t.x1];
-    cb = t.w00 * (int)((p00 & 0xFFFFu) >> 6) + t.w01 * (int)((p01 & 0xFFFFu) >> 6) + t.w10 * (int)((p10 & 0xFFFFu) >> 6) +
-         t.w11 * (int)((p11 & 0xFFFFu) >> 6);
-    cr = t.w00 * (int)(p00 >> 22) + t.w01 * (int)(p01 >> 22) + t.w10 * (int)(p10 >> 22) + t.w11 * (int)(p11 >> 22);
+    const uint8_t* r0 = f.c + (int64_t)t.y0 * f.cpitch;
+    const uint8_t* r1 = f.c + (int64_t)t.y1 * f.cpitch;
+    int b00, r00, b01, r01, b10, r10, b11, r11;
+    Yuv<TEN>::c(r0, t.x0, b00, r00);
+    Yuv<TEN>::c(r0, t.x1, b01, r01);
+    Yuv<TEN>::c(r1, t.x0, b10, r10);
+    Yuv<TEN>::c(r1, t.x1, b11, r11);
+    cb = t.w00 * b00 + t.w01 * b01 + t.w10 * b10 + t.w11 * b11;
+    cr = t.w00 * r00 + t.w01 * r01 + t.w10 * r10 + t.w11 * r11;
 }
 
 // fgYuvDiff format.glsl:98-100: a (Y, Cb, Cr) level difference as the R'G'B' difference, the largest magnitude
@@ -183,7 +223,19 @@ __device__ __forceinline__ Maps mapsAt(float kA32, float kC32, float x32, float 
 // Mismatch e in 0..255 (error.comp / mismatch() with format.glsl's FG_DIFF and FG_PACKE):
 //   Rgb10   uint(min(max_c |wa - wc|, 255)), levels w = float32(sum) * float32(255 / 1047552)
 //   RgbaF16 min(max_c |L(half(wa)) - L(half(wc))| >> 4, 255)
-//   P010    uint(min(max |fgYuvDiff(Y, Cb, Cr difference)|, 255))
+//   Nv12, P010 uint(min(max |fgYuvDiff(Y, Cb, Cr difference)|, 255))
+template <bool TEN>
+__device__ __forceinline__ int mismatchYuv(const SynthParams& P, const Maps& m) {
+    const int ya = yuvYSum<TEN>(P.a, P.W, P.H, m.XA, m.YA), yc = yuvYSum<TEN>(P.c, P.W, P.H, m.XC, m.YC);
+    int cba, cra, cbc, crc;
+    yuvCSums<TEN>(P.a, P.cw, P.ch, m.XA, m.YA, cba, cra);
+    yuvCSums<TEN>(P.c, P.cw, P.ch, m.XC, m.YC, cbc, crc);
+    const float ky = 1.0f / (1024.0f * Yuv<TEN>::kQ), kc = 1.0f / (4096.0f * Yuv<TEN>::kQ);
+    const float dY = (float)(ya - yc) * ky;
+    const float dCb = (float)(cba - cbc) * kc, dCr = (float)(cra - crc) * kc;
+    return (int)fminf(yuvDiffMax(P, dY, dCb, dCr), 255.0f);
+}
+
 template <int F>
 __device__ __forceinline__ int mismatchFmt(const SynthParams& P, const Maps& m) {
     if (F == kRgb10) {
@@ -205,13 +257,7 @@ __device__ __forceinline__ int mismatchFmt(const SynthParams& P, const Maps& m) 
         for (int c = 0; c < 3; c++) d = max(d, abs(P.lut[toHalfBits(va[c])] - P.lut[toHalfBits(vc[c])]));
         return min(d >> 4, 255);
     }
-    const int ya = p010YSum(P.a, P.W, P.H, m.XA, m.YA), yc = p010YSum(P.c, P.W, P.H, m.XC, m.YC);
-    int cba, cra, cbc, crc;
-    p010CSums(P.a, P.cw, P.ch, m.XA, m.YA, cba, cra);
-    p010CSums(P.c, P.cw, P.ch, m.XC, m.YC, cbc, crc);
-    const float dY = (float)(ya - yc) * (1.0f / 4096.0f);
-    const float dCb = (float)(cba - cbc) * (1.0f / 16384.0f), dCr = (float)(cra - crc) * (1.0f / 16384.0f);
-    return (int)fminf(yuvDiffMax(P, dY, dCb, dCr), 255.0f);
+    return mismatchYuv<F == kP010>(P, m);
 }
 
 // resolve.comp:73-83 blend in float32: m = 0.5 (wa + wc) at t = 1/2, else wA wa + wC wc; o = m (1 - u) + near u
@@ -223,8 +269,33 @@ __device__ __forceinline__ float blendF(const SynthParams& P, float wa, float wc
 // One output pixel (x, y) with trust u:
 //   Rgb10   code = floor(clamp(o, 0, 255) * float32(1023 / 255) + 0.5), alpha of the nearer frame
 //   RgbaF16 half(o), alpha of the nearer frame
-//   P010    Y code = clamp(floor(4 o + 0.5), 0, 1023); at an even-even pixel also the CbCr of its 2x2 block, warped at
-//           the block centre p + 1/2 (warpChroma motion_v1.glsl:134-144) and blended with this pixel's trust
+//   Nv12 / P010 Y code = clamp(floor(q o + 0.5), 0, top) (q codes per level); at an even-even pixel also the CbCr
+//           of its 2x2 block, warped at the block centre p + 1/2 (warpChroma motion_v1.glsl:134-144) and blended with
+//           this pixel's trust
+template <bool TEN>
+__device__ __forceinline__ void resolveYuv(const SynthParams& P, float kA32, float kC32, int x, int y, float2 r, float u,
+                                           const Maps& m) {
+    using T = Yuv<TEN>;
+    const FrameRef& nf = P.nearC ? P.c : P.a;
+    const int ya = yuvYSum<TEN>(P.a, P.W, P.H, m.XA, m.YA), yc = yuvYSum<TEN>(P.c, P.W, P.H, m.XC, m.YC);
+    const float ky = 1.0f / (1024.0f * T::kQ), kc = 1.0f / (4096.0f * T::kQ), kn = 1.0f / T::kQ;
+    const float ny = (float)T::y(nf.p + (int64_t)y * nf.pitch, x) * kn;
+    const float oy = blendF(P, (float)ya * ky, (float)yc * ky, ny, u);
+    T::putY(P.out + (int64_t)y * P.opitch, x, med3i((int)floorf(oy * T::kQ + 0.5f), 0, T::kTop));
+    if (((x | y) & 1) == 0) {
+        const Maps b = mapsAt(kA32, kC32, (float)(32 * x + 16), (float)(32 * y + 16), r);
+        int cba, cra, cbc, crc, nb, nr;
+        yuvCSums<TEN>(P.a, P.cw, P.ch, b.XA, b.YA, cba, cra);
+        yuvCSums<TEN>(P.c, P.cw, P.ch, b.XC, b.YC, cbc, crc);
+        const int cx = x >> 1, cy = y >> 1;
+        T::c(nf.c + (int64_t)cy * nf.cpitch, cx, nb, nr);
+        const float ob = blendF(P, (float)cba * kc, (float)cbc * kc, (float)nb * kn, u);
+        const float orr = blendF(P, (float)cra * kc, (float)crc * kc, (float)nr * kn, u);
+        T::putC(P.outc + (int64_t)cy * P.ocpitch, cx, med3i((int)floorf(ob * T::kQ + 0.5f), 0, T::kTop),
+                med3i((int)floorf(orr * T::kQ + 0.5f), 0, T::kTop));
+    }
+}
+
 template <int F>
 __device__ __forceinline__ void resolveFmt(const SynthParams& P, float kA32, float kC32, int x, int y, float2 r, float u) {
     const FrameRef& nf = P.nearC ? P.c : P.a;
@@ -257,25 +328,7 @@ __device__ __forceinline__ void resolveFmt(const SynthParams& P, float kA32, flo
         *(uint2*)(P.out + (int64_t)y * P.opitch + 8 * (int64_t)x) = o;
         return;
     }
-    const int ya = p010YSum(P.a, P.W, P.H, m.XA, m.YA), yc = p010YSum(P.c, P.W, P.H, m.XC, m.YC);
-    const float ny = (float)(((const uint16_t*)(nf.p + (int64_t)y * nf.pitch))[x] >> 6) * 0.25f;
-    const float oy = blendF(P, (float)ya * (1.0f / 4096.0f), (float)yc * (1.0f / 4096.0f), ny, u);
-    const int code = med3i((int)floorf(oy * 4.0f + 0.5f), 0, 1023);
-    ((uint16_t*)(P.out + (int64_t)y * P.opitch))[x] = (uint16_t)(code << 6);
-    if (((x | y) & 1) == 0) {
-        const Maps b = mapsAt(kA32, kC32, (float)(32 * x + 16), (float)(32 * y + 16), r);
-        int cba, cra, cbc, crc;
-        p010CSums(P.a, P.cw, P.ch, b.XA, b.YA, cba, cra);
-        p010CSums(P.c, P.cw, P.ch, b.XC, b.YC, cbc, crc);
-        const int cx = x >> 1, cy = y >> 1;
-        const uint32_t nc = *(const uint32_t*)(nf.c + (int64_t)cy * nf.cpitch + 4 * (int64_t)cx);
-        const float k = 1.0f / 16384.0f;
-        const float ob = blendF(P, (float)cba * k, (float)cbc * k, (float)((nc & 0xFFFFu) >> 6) * 0.25f, u);
-        const float orr = blendF(P, (float)cra * k, (float)crc * k, (float)(nc >> 22) * 0.25f, u);
-        const uint32_t cb = (uint32_t)med3i((int)floorf(ob * 4.0f + 0.5f), 0, 1023);
-        const uint32_t cr = (uint32_t)med3i((int)floorf(orr * 4.0f + 0.5f), 0, 1023);
-        *(uint32_t*)(P.outc + (int64_t)cy * P.ocpitch + 4 * (int64_t)cx) = (cb << 6) | (cr << 22);
-    }
+    resolveYuv<F == kP010>(P, kA32, kC32, x, y, r, u, m);
 }
 
 // the nearer frame's pixel (x, y) into the output (scene cut): plane 0, and the chroma of an even-even pixel (P010)
@@ -286,11 +339,16 @@ __device__ __forceinline__ void copyNear(const SynthParams& P, int x, int y) {
         *(uint32_t*)(P.out + (int64_t)y * P.opitch + 4 * (int64_t)x) = ld32(nf, x, y);
     } else if (F == kF16) {
         *(uint2*)(P.out + (int64_t)y * P.opitch + 8 * (int64_t)x) = *(const uint2*)(nf.p + (int64_t)y * nf.pitch + 8 * (int64_t)x);
-    } else {
+    } else if (F == kP010) {
         ((uint16_t*)(P.out + (int64_t)y * P.opitch))[x] = ((const uint16_t*)(nf.p + (int64_t)y * nf.pitch))[x];
         if (((x | y) & 1) == 0)
             *(uint32_t*)(P.outc + (int64_t)(y >> 1) * P.ocpitch + 4 * (int64_t)(x >> 1)) =
                 *(const uint32_t*)(nf.c + (int64_t)(y >> 1) * nf.cpitch + 4 * (int64_t)(x >> 1));
+    } else {
+        P.out[(int64_t)y * P.opitch + x] = nf.p[(int64_t)y * nf.pitch + x];
+        if (((x | y) & 1) == 0)
+            *(uint16_t*)(P.outc + (int64_t)(y >> 1) * P.ocpitch + 2 * (int64_t)(x >> 1)) =
+                *(const uint16_t*)(nf.c + (int64_t)(y >> 1) * nf.cpitch + 2 * (int64_t)(x >> 1));
     }
 }
 
@@ -352,6 +410,7 @@ __device__ __forceinline__ void synthFmtBody(const SynthParams& P, int bx, int b
     __shared__ uint32_t cnt;
     if (P.fmt == kRgb10) synthFmtTile<S, kRgb10>(P, bx, by, ft, hrow, es, hs, &cnt);
     else if (P.fmt == kF16) synthFmtTile<S, kF16>(P, bx, by, ft, hrow, es, hs, &cnt);
+    else if (P.fmt == kNv12) synthFmtTile<S, kNv12>(P, bx, by, ft, hrow, es, hs, &cnt);
     else synthFmtTile<S, kP010>(P, bx, by, ft, hrow, es, hs, &cnt);
 }
 
@@ -375,6 +434,7 @@ __global__ __launch_bounds__(256) void debug_e_fmt_kernel(SynthParams P, uint32_
         int e;
         if (P.fmt == kRgb10) e = mismatchFmt<kRgb10>(P, m);
         else if (P.fmt == kF16) e = mismatchFmt<kF16>(P, m);
+        else if (P.fmt == kNv12) e = mismatchFmt<kNv12>(P, m);
         else e = mismatchFmt<kP010>(P, m);
         packed |= (uint32_t)e << (8 * i);
     }

@@ -25,11 +25,11 @@ constexpr int kWorkShift = 22;
 constexpr uint32_t kWorkMask = (1u << kWorkShift) - 1u;
 
 // Frame formats (fgr.h FGR_FORMAT_*)
-enum Fmt : int { kRgba8 = 0, kRgb10 = 1, kF16 = 2, kP010 = 4 };
+enum Fmt : int { kRgba8 = 0, kRgb10 = 1, kF16 = 2, kNv12 = 3, kP010 = 4 };
 
 struct FrameRef {
     const uint8_t* p;   // plane 0
-    const uint8_t* c;   // CbCr plane (P010)
+    const uint8_t* c;   // CbCr plane (Nv12, P010)
     int64_t pitch, cpitch;
 };
 
@@ -44,7 +44,7 @@ struct LumaParams {
     int bgra;
     int fmt;            // Fmt (the Rgba8 kernel ignores it)
     const int* lut;     // RgbaF16: metric level of every half value, in 1/16 level (65536 entries)
-    int yFull;          // P010: full-range codes (else limited range, black 64, white 940)
+    int yFull;          // Nv12 / P010: full-range codes (else limited range: black 16 / 64, white 235 / 940)
 };
 
 struct DownParams {
@@ -124,8 +124,8 @@ struct SynthParams {
     int nearC;                // t > 0.5
     int fmt;                  // Fmt
     const int* lut;           // RgbaF16: metric level of every half value, in 1/16 level (65536 entries)
-    float yuv[8];             // P010: ys, rcr, bcb, gcb, gcr (format.glsl YU_YS, YU_RCR, YU_BCB, YU_GCB, YU_GCR)
-    int cw, ch;               // P010: chroma plane size
+    float yuv[8];             // Nv12 / P010: ys, rcr, bcb, gcb, gcr (format.glsl YU_YS, YU_RCR, YU_BCB, YU_GCB, YU_GCR)
+    int cw, ch;               // Nv12 / P010: chroma plane size
 };
 
 struct FallbackParams {
@@ -133,6 +133,7 @@ struct FallbackParams {
     FrameRef a, c;
     uint8_t* out; uint8_t* outc; int64_t opitch, ocpitch;
     int W, H, bpp;            // bytes per pixel of plane 0
+    int cbpp;                 // bytes per chroma texel (Nv12 2, P010 4)
     const uint32_t* count;    // kSlots partial counts
     uint32_t* nextCount;      // the other synth bank (kSlots words), cleared here for the next synth
     float gfrac;

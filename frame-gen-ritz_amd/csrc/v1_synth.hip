@@ -318,10 +318,15 @@ __device__ __forceinline__ void fallbackBody(const FallbackParams& P, int gbx, i
         if (P.bpp == 4) *(uint32_t*)dst = *(const uint32_t*)src;
         else if (P.bpp == 8) *(uint2*)dst = *(const uint2*)src;
         else if (P.bpp == 2) *(uint16_t*)dst = *(const uint16_t*)src;
+        else *dst = *src;
         if (P.outc && ((x | y) & 1) == 0) {   // 4:2:0 chroma of the even-even pixel (fallback.comp:19-22)
             const int cx = x >> 1, cy = y >> 1;
-            *(uint32_t*)(P.outc + (int64_t)cy * P.ocpitch + 4 * (int64_t)cx) =
-                *(const uint32_t*)(nf.c + (int64_t)cy * nf.cpitch + 4 * (int64_t)cx);
+            if (P.cbpp == 4)
+                *(uint32_t*)(P.outc + (int64_t)cy * P.ocpitch + 4 * (int64_t)cx) =
+                    *(const uint32_t*)(nf.c + (int64_t)cy * nf.cpitch + 4 * (int64_t)cx);
+            else
+                *(uint16_t*)(P.outc + (int64_t)cy * P.ocpitch + 2 * (int64_t)cx) =
+                    *(const uint16_t*)(nf.c + (int64_t)cy * nf.cpitch + 2 * (int64_t)cx);
         }
     }
 }

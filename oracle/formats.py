@@ -1,5 +1,6 @@
 """CPU oracle of the V1 path for the frame formats other than RGBA8: Rgb10 (A2B10G10R10), RgbaF16 (scRGB / encoded
-float) and P010 (10-bit 4:2:0 YCbCr, two planes). numpy only.
+float), Nv12 (8-bit 4:2:0 YCbCr, two planes) and P010 (10-bit 4:2:0 YCbCr, two planes). numpy only. A level is an
+8-bit-equivalent code: Nv12 codes as they are, 10-bit codes / 4 (q = codes per level: 1, 4).
 
 TEST INFRASTRUCTURE ONLY. Nothing in the product imports this module: only tests/, the smoke check and the
 CPU-baseline leg of bench.py use it, as the checker.
@@ -13,20 +14,21 @@ source's own space) and pins the arithmetic as follows:
   * luma is a fixed-point value in units of 1/16 level (y16, an integer): Rgb10 round((4899 R + 9617 G + 1868 B) *
     255 / (1023 * 16384) * 16) on 10-bit codes; RgbaF16 (4899 L_r + 9617 L_g + 1868 L_b + 8192) >> 14 on the metric
     levels L (1/16 level) of a 65536-entry table built in float64; P010 round(16 (code / 4 - black) * ys), i.e.
-    round((4 code - 256) * 255 / 219) at limited range and 4 code at full range. Halves round up.
+    round((4 code - 256) * 255 / 219) at limited range and 4 code at full range (Nv12: 16 code in place of 4 code).
+    Halves round up.
     The level-0 luma of an S x S cell is float32(sum of y16) / float32(16 n).
   * the warp map is the RGBA8 one (1/32 px, half-even); taps are exact integer bilinear sums with weights
     (32 - fx)(32 - fy) .. fx fy: Rgb10 keeps the sum S (code * 1024 units), the level is float32(S) * float32(255 /
     1047552); RgbaF16 sums the four float32 products w * v in the order p00, p01, p10, p11 and scales by 1/1024; P010
-    luma S_Y / 4096 levels, chroma sampled at the same point of the half-resolution plane, i.e. at (X - 16) / 64
-    chroma texels with weights (64 - fx)(64 - fy) .., S_C / 16384 levels (even frame sizes only).
+    (and Nv12) luma S_Y / (1024 q) levels, chroma sampled at the same point of the half-resolution plane, i.e. at
+    (X - 16) / 64 chroma texels with weights (64 - fx)(64 - fy) .., S_C / (4096 q) levels (even frame sizes only).
   * mismatch e (0..255): Rgb10 uint(min(max |wa - wc|, 255)) in float32 levels (format.glsl FG_DIFF / FG_PACKE);
     RgbaF16 max |L(half(wa)) - L(half(wc))| >> 4, capped at 255 (the warped value rounded to half precision, then the
-    table); P010 uint(min(max |fgYuvDiff(d)|, 255)) with the reference's float32 YCbCr -> R'G'B' constants
+    table); Nv12 / P010 uint(min(max |fgYuvDiff(d)|, 255)) with the reference's float32 YCbCr -> R'G'B' constants
     (framegen.cpp:1519-1533) and its operation order.
   * blend (resolve.comp:73-83): m = 0.5 (wa + wc) at t = 1/2, else wA wa + wC wc; o = m (1 - u) + near u in float32.
     Rgb10 out code = floor(clamp(o, 0, 255) * float32(1023 / 255) + 0.5), alpha of the nearer frame; RgbaF16 out =
-    half(o) (round to nearest even), alpha of the nearer frame; P010 Y code = clamp(floor(4 o + 0.5), 0, 1023) at every
+    half(o) (round to nearest even), alpha of the nearer frame; Nv12 / P010 Y code = clamp(floor(q o + 0.5), 0, top) at every
     pixel, CbCr of each 2x2 block at its even-even pixel, warped at the block centre p + 1/2 (warpChroma
     motion_v1.glsl:134-144) with that pixel's trust.
 """
@@ -39,7 +41,22 @@ from . import v1
 f32 = np.float32
 f64 = np.float64
 
-RGBA8, RGB10, F16, P010 = 0, 1, 2, 4
+RGBA8, RGB10, F16, NV12, P010 = 0, 1, 2, 3, 4
+YUV = (NV12, P010)
+
+
+def yuv_q(fmt):
+    """codes per level and the largest code of a two-plane format"""
+    return (4, 1023) if fmt == P010 else (1, 255)
+
+
+def yuv_codes(plane, fmt):
+    """Y codes (H, W) or CbCr codes (ch, cw, 2) of an Nv12 / P010 plane, int64."""
+    return ((plane >> 6) if fmt == P010 else plane).astype(np.int64)
+
+
+def yuv_store(codes, fmt):
+    return (codes.astype(np.uint16) << 6).astype(np.uint16) if fmt == P010 else codes.astype(np.uint8)
 HDR_PEAK, HDR_GAMMA = 1000.0, 2.4          # fgtest.cpp:203: an 8-bit value v is 1000 (v / 255)^2.4 nits
 K10 = f32(255.0 / 1047552.0)               # Rgb10: level per unit of a bilinear sum (code * 1024)
 K255_1023 = f32(255.0 / 1023.0)
@@ -89,6 +106,27 @@ def encode_f16(rgba, linear=False, scale=1.0):
     out[..., :3] = h
     out[..., 3] = np.float16(1.0)
     return out.view(np.uint16)
+
+
+def encode_nv12(rgba):
+    """fgtest.cpp:240-268 (Nv12: BT.709 limited range, the 8-bit values as R'G'B'): Y plane uint8 (H, W), CbCr plane
+    uint8 (H/2, W/2, 2), chroma of each 2x2 block averaged."""
+    H, W = rgba.shape[:2]
+    kr, kb = 0.2126, 0.0722
+    kg = 1.0 - kr - kb
+    c3 = rgba[..., :3].astype(f64) / 255.0
+    Y = kr * c3[..., 0] + kg * c3[..., 1] + kb * c3[..., 2]
+    cb = ((c3[..., 2] - Y) / (2 * (1 - kb))).astype(np.float32).astype(f64)
+    cr = ((c3[..., 0] - Y) / (2 * (1 - kr))).astype(np.float32).astype(f64)
+    y = np.rint(np.clip(16 + 219 * Y, 0.0, 255.0)).astype(np.uint8)
+    cw, ch = (W + 1) // 2, (H + 1) // 2
+    yi = np.minimum(np.arange(2 * ch), H - 1)
+    xi = np.minimum(np.arange(2 * cw), W - 1)
+    sb = cb[yi][:, xi].reshape(ch, 2, cw, 2).sum((1, 3))
+    sr = cr[yi][:, xi].reshape(ch, 2, cw, 2).sum((1, 3))
+    b = np.rint(np.clip(128 + 224 * sb / 4, 0.0, 255.0)).astype(np.uint8)
+    r = np.rint(np.clip(128 + 224 * sr / 4, 0.0, 255.0)).astype(np.uint8)
+    return y, np.stack([b, r], -1)
 
 
 def encode_p010(rgba):
@@ -156,12 +194,12 @@ def luma16(frame, fmt, bgra=False, lut=None, full_range=False):
         L = lut.astype(np.int64)[frame[..., :3]]
         n = L[..., 0] * 4899 + L[..., 1] * 9617 + L[..., 2] * 1868
         return (n + 8192) >> 14
-    if fmt == P010:
+    if fmt in YUV:
         yp = frame[0] if isinstance(frame, tuple) else frame
-        c = (yp >> 6).astype(np.int64)
+        c = yuv_codes(yp, fmt) * (16 // yuv_q(fmt)[0])   # 16 code / q = 16 x level
         if full_range:
-            return 4 * c
-        return ((4 * c - 256) * 510 + 219) // 438
+            return c
+        return ((c - 256) * 510 + 219) // 438
     raise ValueError(fmt)
 
 
@@ -262,26 +300,25 @@ def _yuv_diff(dY, dCb, dCr, k):
     return np.maximum(np.abs(R), np.maximum(np.abs(G), np.abs(B)))
 
 
-def _chroma_sum(cplane, X, Y):
+def _chroma_sum(cplane, X, Y, fmt=P010):
     """CbCr of the half-resolution plane at luma map (X, Y) (1/32 px): the point (X - 16) / 64 chroma texels."""
-    codes = (cplane >> 6).astype(np.int64)
-    return _isum(codes, X - 16, Y - 16, unit=64, sh=6)
+    return _isum(yuv_codes(cplane, fmt), X - 16, Y - 16, unit=64, sh=6)
 
 
 # ------------------------------------------------------------------------------------------------ synth
 
 def synth(a, c, est, t, fmt, exact64=False, lut=None, yuv=None):
-    """v1.synth for Rgb10 / RgbaF16 / P010 frames (arrays as the encoders return them). Returns (out, reason, info);
-    for P010 out is (Y plane, CbCr plane)."""
+    """v1.synth for Rgb10 / RgbaF16 / Nv12 / P010 frames (arrays as the encoders return them). Returns (out, reason,
+    info); for Nv12 / P010 out is (Y plane, CbCr plane)."""
     s = est["settings"]
-    yp_a = a[0] if fmt == P010 else a
+    yp_a = a[0] if fmt in YUV else a
     H, W = yp_a.shape[:2]
     ts, wA, wC = v1.time_constants(t)
     near_c = wC > f32(0.5)
     near = c if near_c else a
     info = {}
     if est["cut"]:
-        return (tuple(x.copy() for x in near) if fmt == P010 else near.copy()), "cut", info
+        return (tuple(x.copy() for x in near) if fmt in YUV else near.copy()), "cut", info
     (AX, AY), (CX, CY), hv, (ka, kc) = _maps(est, t, W, H)
     if fmt == RGB10:
         sa, sc = _isum(rgb10_channels(a), AX, AY), _isum(rgb10_channels(c), CX, CY)
@@ -292,12 +329,14 @@ def synth(a, c, est, t, fmt, exact64=False, lut=None, yuv=None):
         La = lut.astype(np.int64)[wa.astype(np.float16).view(np.uint16)]
         Lc = lut.astype(np.int64)[wc.astype(np.float16).view(np.uint16)]
         e = np.minimum(np.abs(La - Lc).max(-1) >> 4, 255)
-    elif fmt == P010:
-        ya, yc = (a[0] >> 6).astype(np.int64), (c[0] >> 6).astype(np.int64)
+    elif fmt in YUV:
+        qq, top = yuv_q(fmt)
+        ya, yc = yuv_codes(a[0], fmt), yuv_codes(c[0], fmt)
         sya, syc = _isum(ya, AX, AY), _isum(yc, CX, CY)
-        sca, scc = _chroma_sum(a[1], AX, AY), _chroma_sum(c[1], CX, CY)
-        dY = ((sya - syc).astype(f32) * f32(1.0 / 4096.0)).astype(f32)
-        dC = ((sca - scc).astype(f32) * f32(1.0 / 16384.0)).astype(f32)
+        sca, scc = _chroma_sum(a[1], AX, AY, fmt), _chroma_sum(c[1], CX, CY, fmt)
+        ky, kc_ = f32(1.0 / (1024.0 * qq)), f32(1.0 / (4096.0 * qq))
+        dY = ((sya - syc).astype(f32) * ky).astype(f32)
+        dC = ((sca - scc).astype(f32) * kc_).astype(f32)
         e = np.minimum(_yuv_diff(dY, dC[..., 0], dC[..., 1], yuv), f32(255.0)).astype(np.int64)
     else:
         raise ValueError(fmt)
@@ -310,7 +349,7 @@ def synth(a, c, est, t, fmt, exact64=False, lut=None, yuv=None):
     info.update(e=e, S49=S49, count=count)
     flick = f32(count) > (f32(s["globalFallback"]) * f32(W)) * f32(H)
     if flick:
-        return (tuple(x.copy() for x in near) if fmt == P010 else near.copy()), "untrusted", info
+        return (tuple(x.copy() for x in near) if fmt in YUV else near.copy()), "untrusted", info
     if exact64:
         u = np.clip((S49.astype(f64) / 49.0 - float(lo)) / float(den), 0.0, 1.0)
     else:
@@ -346,12 +385,13 @@ def synth(a, c, est, t, fmt, exact64=False, lut=None, yuv=None):
         out[..., 3] = near[..., 3]
         info.update(values=o)
         return out, None, info
-    # P010: luma at every pixel, chroma at the even-even pixel of each 2x2 block
-    wya = (sya.astype(f32) * f32(1.0 / 4096.0)).astype(f32)
-    wyc = (syc.astype(f32) * f32(1.0 / 4096.0)).astype(f32)
-    ny = ((near[0] >> 6).astype(f32) * f32(0.25)).astype(f32)
+    # Nv12 / P010: luma at every pixel, chroma at the even-even pixel of each 2x2 block
+    kn = f32(1.0 / qq)
+    wya = (sya.astype(f32) * ky).astype(f32)
+    wyc = (syc.astype(f32) * ky).astype(f32)
+    ny = (yuv_codes(near[0], fmt).astype(f32) * kn).astype(f32)
     oy = blend(wya, wyc, ny, u)
-    yout = (np.clip(np.floor(oy * 4 + 0.5), 0, 1023).astype(np.uint16) << 6).astype(np.uint16)
+    yout = yuv_store(np.clip(np.floor(oy * f32(qq) + f32(0.5)), 0, top), fmt)
     # warpChroma: the block centre q = p + 1/2 of every even-even pixel p
     ch, cw = near[1].shape[:2]
     py, px = np.meshgrid(np.arange(ch) * 2, np.arange(cw) * 2, indexing="ij")
@@ -362,11 +402,11 @@ def synth(a, c, est, t, fmt, exact64=False, lut=None, yuv=None):
     BY = q((qy - (ka * hvb[..., 1]).astype(f32)).astype(f32))
     DX = q((qx + (kc * hvb[..., 0]).astype(f32)).astype(f32))
     DY = q((qy + (kc * hvb[..., 1]).astype(f32)).astype(f32))
-    ca = (_chroma_sum(a[1], BX, BY).astype(f32) * f32(1.0 / 16384.0)).astype(f32)
-    cc = (_chroma_sum(c[1], DX, DY).astype(f32) * f32(1.0 / 16384.0)).astype(f32)
-    nc = ((near[1] >> 6).astype(f32) * f32(0.25)).astype(f32)
+    ca = (_chroma_sum(a[1], BX, BY, fmt).astype(f32) * kc_).astype(f32)
+    cc = (_chroma_sum(c[1], DX, DY, fmt).astype(f32) * kc_).astype(f32)
+    nc = (yuv_codes(near[1], fmt).astype(f32) * kn).astype(f32)
     oc = blend(ca, cc, nc, u[py, px][..., None])
-    cout = (np.clip(np.floor(oc * 4 + 0.5), 0, 1023).astype(np.uint16) << 6).astype(np.uint16)
+    cout = yuv_store(np.clip(np.floor(oc * f32(qq) + f32(0.5)), 0, top), fmt)
     info.update(levels_y=oy, levels_c=oc)
     return (yout, cout), None, info
 
@@ -396,9 +436,10 @@ def synth_blur(a, c, est, t0, t1, n, fmt, gauss=False, lut=None, yuv=None):
         for f, w in zip(frames, ws):
             acc = (acc + (w * f.view(np.float16).astype(f32)).astype(f32)).astype(f32)
         return (acc / wsum).astype(f32).astype(np.float16).view(np.uint16)
-    y = v1.average_frames([(f[0] >> 6) for f in frames], ws, hi=1023).astype(np.uint16) << 6
-    cc = v1.average_frames([(f[1] >> 6) for f in frames], ws, hi=1023).astype(np.uint16) << 6
-    return y.astype(np.uint16), cc.astype(np.uint16)
+    top = yuv_q(fmt)[1]
+    y = v1.average_frames([yuv_codes(f[0], fmt) for f in frames], ws, hi=top)
+    cc = v1.average_frames([yuv_codes(f[1], fmt) for f in frames], ws, hi=top)
+    return yuv_store(y, fmt), yuv_store(cc, fmt)
 
 
 def level_error(fmt, got, want64_info, lut=None):
@@ -410,10 +451,11 @@ def level_error(fmt, got, want64_info, lut=None):
         L = lut.astype(f64)[got[..., :3]] / 16.0
         ref = want64_info["values"].astype(np.float16).view(np.uint16)
         return np.abs(L - lut.astype(f64)[ref] / 16.0)
-    if fmt == P010:
-        y = (got[0] >> 6).astype(f64) / 4.0
-        cc = (got[1] >> 6).astype(f64) / 4.0
-        ey = np.abs(y - np.clip(want64_info["levels_y"], 0, 255.75))
-        ec = np.abs(cc - np.clip(want64_info["levels_c"], 0, 255.75))
+    if fmt in YUV:
+        qq, top = yuv_q(fmt)
+        y = yuv_codes(got[0], fmt).astype(f64) / qq
+        cc = yuv_codes(got[1], fmt).astype(f64) / qq
+        ey = np.abs(y - np.clip(want64_info["levels_y"], 0, top / qq))
+        ec = np.abs(cc - np.clip(want64_info["levels_c"], 0, top / qq))
         return np.concatenate([ey.ravel(), ec.ravel()])
     raise ValueError(fmt)

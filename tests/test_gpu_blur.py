@@ -9,7 +9,7 @@ from fgr import scenes
 from gpu_util import dev
 from oracle import formats as F
 from oracle import v1
-from test_gpu_formats import empty_like, encode, lut_of, same, to_dev, to_host
+from test_gpu_formats import empty_like, encode, lut_of, matrix_of, same, to_dev, to_host
 
 pytestmark = pytest.mark.gpu
 
@@ -58,19 +58,19 @@ def test_blur_scene_cut_and_collapses():
         fi.close()
 
 
-@pytest.mark.parametrize("fmt,linear", [(F.RGB10, False), (F.F16, True), (F.P010, False)])
+@pytest.mark.parametrize("fmt,linear", [(F.RGB10, False), (F.F16, True), (F.NV12, False), (F.P010, False)])
 def test_blur_formats(fmt, linear):
     a, c = scenes.pair_translate(34, w=256, h=144, sprite=32, sprite_pos=(90, 40), shift=(3, 2))
     A, C = encode(a, fmt, linear), encode(c, fmt, linear)
     s = fgr.default_settings(blurSamples=4, blurWeights=1)
     fi = fgr.Interpolator(256, 144, fmt=fmt, settings=s, transfer=fgr.TRANSFER_LINEAR if linear else 0,
-                          yuv_matrix=fgr.YUV_BT2020)
+                          yuv_matrix=matrix_of(fmt))
     ta, tc = to_dev(A, fmt), to_dev(C, fmt)
     assert fi.record_estimate(ta, tc)
     out = empty_like(ta, fmt)
     assert fi.record_synth_blur(ta, tc, out, 0.25, 0.75)
     lut = lut_of(fmt, linear)
     est = F.estimate(A, C, fmt, lut=lut)
-    want = F.synth_blur(A, C, est, 0.25, 0.75, 4, fmt, gauss=True, lut=lut, yuv=F.yuv_constants(2))
+    want = F.synth_blur(A, C, est, 0.25, 0.75, 4, fmt, gauss=True, lut=lut, yuv=F.yuv_constants(matrix_of(fmt)))
     assert same(to_host(out, fmt), want, fmt)
     fi.close()

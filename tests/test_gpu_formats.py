@@ -1,4 +1,4 @@
-"""GPU parity of the V1 path for Rgb10, RgbaF16 (encoded and linear light) and P010 frames, through the C ABI.
+"""GPU parity of the V1 path for Rgb10, RgbaF16 (encoded and linear light), Nv12 and P010 frames, through the C ABI.
 
 Bar: every estimate intermediate bit-exact against the oracle (oracle/formats.py pins the fixed-point luma and the
 exact taps of each format), the untrusted count and cut / flick decisions exact, the output bit-identical to the
@@ -30,6 +30,8 @@ def lut_of(fmt, linear, white=80.0):
 
 
 def to_dev(frame, fmt):
+    if fmt == F.NV12:
+        return (torch.from_numpy(frame[0].copy()).cuda(), torch.from_numpy(frame[1].copy()).cuda())
     if fmt == F.P010:
         y, c = frame
         return (torch.from_numpy(y.view(np.int16).copy()).cuda(), torch.from_numpy(c.view(np.int16).copy()).cuda())
@@ -39,6 +41,8 @@ def to_dev(frame, fmt):
 
 
 def to_host(t, fmt):
+    if fmt == F.NV12:
+        return (t[0].cpu().numpy(), t[1].cpu().numpy())
     if fmt == F.P010:
         return (t[0].cpu().numpy().view(np.uint16), t[1].cpu().numpy().view(np.uint16))
     if fmt == F.RGB10:
@@ -47,7 +51,7 @@ def to_host(t, fmt):
 
 
 def empty_like(t, fmt, fill=0):
-    if fmt == F.P010:
+    if fmt in F.YUV:
         return (torch.full_like(t[0], fill), torch.full_like(t[1], fill))
     return torch.full_like(t, fill)
 
@@ -57,19 +61,28 @@ def encode(rgba, fmt, linear=False):
         return F.encode_rgb10(rgba)
     if fmt == F.F16:
         return F.encode_f16(rgba, linear)
+    if fmt == F.NV12:
+        return F.encode_nv12(rgba)
     return F.encode_p010(rgba)
+
+
+def matrix_of(fmt):
+    """the YCbCr matrix of the encoders: Nv12 BT.709 (SDR), P010 BT.2020 (HDR10)"""
+    return fgr.YUV_BT709 if fmt == F.NV12 else fgr.YUV_BT2020
 
 
 CASES = [   # (name, format, linear transfer)
     ("rgb10", F.RGB10, False),
     ("f16", F.F16, False),
     ("f16lin", F.F16, True),
+    ("nv12", F.NV12, False),
     ("p010", F.P010, False),
 ]
 
 
-def run_fmt(A, C, fmt, ts, linear=False, settings=None, yuv_matrix=fgr.YUV_BT2020, full_range=False, bgra=False):
-    H, W = (A[0] if fmt == F.P010 else A).shape[:2]
+def run_fmt(A, C, fmt, ts, linear=False, settings=None, yuv_matrix=None, full_range=False, bgra=False):
+    H, W = (A[0] if fmt in F.YUV else A).shape[:2]
+    yuv_matrix = matrix_of(fmt) if yuv_matrix is None else yuv_matrix
     fi = fgr.Interpolator(W, H, fmt=fmt, settings=settings, bgra=bgra,
                           transfer=fgr.TRANSFER_LINEAR if linear else fgr.TRANSFER_ENCODED,
                           yuv_matrix=yuv_matrix, yuv_full_range=full_range)
@@ -89,13 +102,13 @@ def run_fmt(A, C, fmt, ts, linear=False, settings=None, yuv_matrix=fgr.YUV_BT202
 
 
 def same(got, want, fmt):
-    if fmt == F.P010:
+    if fmt in F.YUV:
         return np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
     return np.array_equal(got, want)
 
 
 def first_diff(got, want, fmt):
-    if fmt == F.P010:
+    if fmt in F.YUV:
         for k in range(2):
             if not np.array_equal(got[k], want[k]):
                 idx = np.argwhere(got[k] != want[k])[0]
@@ -126,7 +139,7 @@ def test_format_pair(name, fmt, linear):
     ts = (0.5, 0.25, 0.75)
     I, outs, counts = run_fmt(A, C, fmt, ts, linear)
     lut = lut_of(fmt, linear)
-    yuv = F.yuv_constants(2)
+    yuv = F.yuv_constants(matrix_of(fmt))
     est = F.estimate(A, C, fmt, lut=lut)
     check_estimate(I, est)
     for t, o, cnt in zip(ts, outs, counts):
@@ -135,7 +148,7 @@ def test_format_pair(name, fmt, linear):
 
 @pytest.mark.parametrize("name,fmt,linear", CASES)
 def test_format_sizes_and_settings(name, fmt, linear):
-    W, H = (202, 118) if fmt == F.P010 else (203, 117)
+    W, H = (202, 118) if fmt in F.YUV else (203, 117)
     a, c = scenes.pair_translate(12, w=W, h=H, sprite=32, sprite_pos=(W // 3, H // 3), shift=(3, -2))
     A, C = encode(a, fmt, linear), encode(c, fmt, linear)
     for kw in (dict(flowScale=1, pyramidMinSide=24), dict(flowScale=4, pyramidMinSide=16, subpixel=0)):
@@ -145,7 +158,7 @@ def test_format_sizes_and_settings(name, fmt, linear):
         est = F.estimate(A, C, fmt, s=oracle_settings(s), lut=lut)
         check_estimate(I, est)
         for t, o, cnt in zip((0.5, 0.3), outs, counts):
-            check_fmt_output(o, A, C, est, t, fmt, lut, F.yuv_constants(2), cnt)
+            check_fmt_output(o, A, C, est, t, fmt, lut, F.yuv_constants(matrix_of(fmt)), cnt)
 
 
 @pytest.mark.parametrize("name,fmt,linear", CASES)
@@ -175,10 +188,10 @@ def test_p010_full_range_bt709_and_rgb10_bgra():
 
 
 def test_mixed_format_batch_matches_per_pair():
-    """One batch of RGBA8, Rgb10, RgbaF16 and P010 pairs of different sizes and multipliers: one launch per pass for
+    """One batch of RGBA8, Rgb10, RgbaF16, Nv12 and P010 pairs of different sizes and multipliers: one launch per pass for
     the three non-RGBA8 formats together; every output equal to the per-pair path (itself checked above)."""
     specs = [(F.RGBA8, 320, 180, 2), (F.RGB10, 256, 144, 3), (F.F16, 288, 160, 4), (F.P010, 320, 176, 4),
-             (F.RGB10, 202, 118, 2), (F.P010, 256, 144, 3)]
+             (F.RGB10, 202, 118, 2), (F.P010, 256, 144, 3), (F.NV12, 224, 128, 3)]
     items, refs = [], []
     for i, (fmt, W, H, n) in enumerate(specs):
         a, c = scenes.pair_translate(20 + i, w=W, h=H, sprite=32, sprite_pos=(W // 3, H // 3), shift=(2 + i, -1))

@@ -80,7 +80,9 @@ def test_size_layout_matches_oracle(W, H, S, m):
 def test_create_refusals_name_the_reason():
     # refused before any device call: unsupported format, invalid settings, sizes
     with pytest.raises(RuntimeError, match="format"):
-        fgr.Interpolator(640, 360, fmt=fgr.FORMAT_NV12)
+        fgr.Interpolator(640, 360, fmt=7)
+    with pytest.raises(RuntimeError, match="even width"):
+        fgr.Interpolator(641, 360, fmt=fgr.FORMAT_NV12)
     with pytest.raises(RuntimeError, match="invalid settings"):
         fgr.Interpolator(640, 360, settings=fgr.default_settings(trustHigh=5.0))
     with pytest.raises(RuntimeError, match="kMinSize|FGR_MIN_SIZE"):

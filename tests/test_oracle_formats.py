@@ -46,6 +46,11 @@ def test_luma16_closed_forms(luts):
     want = [math.floor(Fraction((4 * c - 256) * 255, 219) + Fraction(1, 2)) for c in (64, 940, 502, 0, 1023)]
     assert list(got) == want and got[0] == 0 and got[1] == 4080
     assert list(F.luma16(yp, F.P010, full_range=True)[0]) == [256, 3760, 2008, 0, 4092]
+    # Nv12: code 16 is black, 235 white (255 levels), 16 code at full range
+    y8 = np.array([[16, 235, 126, 0, 255]], np.uint8)
+    want8 = [math.floor(Fraction((16 * c - 256) * 255, 219) + Fraction(1, 2)) for c in (16, 235, 126, 0, 255)]
+    assert list(F.luma16(y8, F.NV12)[0]) == want8 and want8[:2] == [0, 4080]
+    assert list(F.luma16(y8, F.NV12, full_range=True)[0]) == [256, 3760, 2016, 0, 4080]
     # RgbaF16 encoded gray v: luma = 16 * 255 v (weights sum to 16384), rounded
     v = np.array([0.25, 0.5, 1.0], np.float16)
     px = np.stack([v, v, v, np.ones_like(v)], -1).view(np.uint16)[None]
@@ -70,14 +75,14 @@ def _scalar_bilinear(img, X, Y, unit):
     return sum(int(w) * np.asarray(t, np.int64) for w, t in zip(ws, taps))
 
 
-@pytest.mark.parametrize("fmt", [F.RGB10, F.P010, F.F16])
+@pytest.mark.parametrize("fmt", [F.RGB10, F.P010, F.NV12, F.F16])
 def test_mismatch_matches_a_scalar_restatement(fmt, luts):
     """e of every pixel of a tiny pair with a random field, vectorised oracle vs a per-pixel loop."""
     W, H = 24, 20
     rng = np.random.default_rng(5)
     a = rng.integers(0, 256, (H, W, 4), dtype=np.uint8)
     c = rng.integers(0, 256, (H, W, 4), dtype=np.uint8)
-    enc = {F.RGB10: F.encode_rgb10, F.P010: F.encode_p010, F.F16: F.encode_f16}[fmt]
+    enc = {F.RGB10: F.encode_rgb10, F.P010: F.encode_p010, F.NV12: F.encode_nv12, F.F16: F.encode_f16}[fmt]
     A, C = enc(a), enc(c)
     fw, fh, lv = v1.layout(W, H, 2, 16)
     pf = (rng.integers(-40, 41, (fh, fw, 2)) / 8.0).astype(np.float32)
@@ -95,13 +100,15 @@ def test_mismatch_matches_a_scalar_restatement(fmt, luts):
                 sc = _scalar_bilinear(F.rgb10_channels(C), cx, cy, 32)
                 d = max(abs(float(np.float32(sa[k]) * F.K10) - float(np.float32(sc[k]) * F.K10)) for k in range(3))
                 e = int(min(d, 255.0))
-            elif fmt == F.P010:
-                ya = _scalar_bilinear((A[0] >> 6).astype(np.int64), ax, ay, 32)
-                yc = _scalar_bilinear((C[0] >> 6).astype(np.int64), cx, cy, 32)
-                ca = _scalar_bilinear((A[1] >> 6).astype(np.int64), ax - 16, ay - 16, 64)
-                cc = _scalar_bilinear((C[1] >> 6).astype(np.int64), cx - 16, cy - 16, 64)
-                dY = np.float32(int(ya) - int(yc)) * np.float32(1 / 4096)
-                dC = [np.float32(int(ca[k]) - int(cc[k])) * np.float32(1 / 16384) for k in range(2)]
+            elif fmt in F.YUV:
+                q = 4 if fmt == F.P010 else 1
+                code = (lambda p: (p >> 6).astype(np.int64)) if fmt == F.P010 else (lambda p: p.astype(np.int64))
+                ya = _scalar_bilinear(code(A[0]), ax, ay, 32)
+                yc = _scalar_bilinear(code(C[0]), cx, cy, 32)
+                ca = _scalar_bilinear(code(A[1]), ax - 16, ay - 16, 64)
+                cc = _scalar_bilinear(code(C[1]), cx - 16, cy - 16, 64)
+                dY = np.float32(int(ya) - int(yc)) * np.float32(1 / (1024 * q))
+                dC = [np.float32(int(ca[k]) - int(cc[k])) * np.float32(1 / (4096 * q)) for k in range(2)]
                 m = F._yuv_diff(np.array([dY]), np.array([dC[0]]), np.array([dC[1]]), yuv)[0]
                 e = int(min(m, np.float32(255)))
             else:
@@ -113,12 +120,14 @@ def test_mismatch_matches_a_scalar_restatement(fmt, luts):
             assert info["e"][y, x] == e, (x, y)
 
 
-@pytest.mark.parametrize("fmt,linear", [(F.RGB10, False), (F.F16, False), (F.F16, True), (F.P010, False)])
+@pytest.mark.parametrize("fmt,linear", [(F.RGB10, False), (F.F16, False), (F.F16, True), (F.P010, False),
+                                        (F.NV12, False)])
 def test_static_pair_reproduces_the_frame(fmt, linear, luts):
     """Zero motion: every warp tap lands on a texel (the P010 chroma of a block at its centre texel), e = 0, and the
     blend returns the frame's own codes / halves."""
     a, _ = scenes.pair_translate(3, w=64, h=48, sprite=16, sprite_pos=(20, 10))
-    enc = {F.RGB10: F.encode_rgb10, F.P010: F.encode_p010}.get(fmt, lambda x: F.encode_f16(x, linear))
+    enc = {F.RGB10: F.encode_rgb10, F.P010: F.encode_p010, F.NV12: F.encode_nv12}.get(fmt,
+                                                                                   lambda x: F.encode_f16(x, linear))
     A = enc(a)
     lut = luts[linear] if fmt == F.F16 else None
     est = F.estimate(A, A, fmt, lut=lut)
@@ -126,7 +135,7 @@ def test_static_pair_reproduces_the_frame(fmt, linear, luts):
     for t in (0.5, 0.25):
         out, why, info = F.synth(A, A, est, t, fmt, lut=lut, yuv=F.yuv_constants(2))
         assert why is None and info["count"] == 0 and not info["e"].any()
-        if fmt == F.P010:
+        if fmt in F.YUV:
             assert np.array_equal(out[0], A[0]) and np.array_equal(out[1], A[1])
         else:
             assert np.array_equal(out, A)
