@@ -10,7 +10,8 @@
  *     are shared between records and only stream order serialises them (framegen.h:4-6).
  *   - A record function returns 1 on success, 0 if it was refused; a refused record enqueues nothing and leaves `out`
  *     untouched (framegen.h:537-548). Reasons are string literals (fgr_init_error, fgr_settings_error, fgr_size_error).
- *   - Frames are caller-owned device memory: FgrFrame{ptr, pitch} (pitch in bytes), plus the CbCr plane for P010.
+ *   - Frames are caller-owned device memory: FgrFrame{ptr, pitch} (pitch in bytes), plus the CbCr plane for Nv12 /
+ *     P010.
  *     The library allocates its own buffers with hipMalloc only in fgr_create / fgr_resize / fgr_reconfigure.
  */
 #ifndef FGR_H
@@ -22,12 +23,12 @@ extern "C" {
 #endif
 
 /* framegen.h:138 Format. Rgba8 = R8G8B8A8 / B8G8R8A8 UNORM (FgrInitInfo.bgra picks the byte order), Rgb10 =
- * A2B10G10R10 UNORM (R in bits 0-9), RgbaF16 = R16G16B16A16 SFLOAT, P010 = 10-bit 4:2:0 in the high bits of 16-bit
- * words (Y plane + interleaved CbCr plane at half resolution). Nv12 (3) is not implemented (fgr_create refuses it). */
+ * A2B10G10R10 UNORM (R in bits 0-9; bgra: A2R10G10B10), RgbaF16 = R16G16B16A16 SFLOAT, Nv12 = 8-bit 4:2:0 (Y plane +
+ * interleaved CbCr plane at half resolution), P010 = the same with 10-bit codes in the high bits of 16-bit words. */
 enum { FGR_FORMAT_RGBA8 = 0, FGR_FORMAT_RGB10 = 1, FGR_FORMAT_RGBAF16 = 2, FGR_FORMAT_NV12 = 3, FGR_FORMAT_P010 = 4 };
 /* framegen.h:144 Transfer (RgbaF16 only): 0 = display-encoded values (level = value * 255), 1 = linear light. */
 enum { FGR_TRANSFER_ENCODED = 0, FGR_TRANSFER_LINEAR = 1 };
-/* framegen.h:150 YuvMatrix (P010 only). */
+/* framegen.h:150 YuvMatrix (Nv12 and P010). */
 enum { FGR_YUV_BT601 = 0, FGR_YUV_BT709 = 1, FGR_YUV_BT2020 = 2 };
 /* framegen.h:28 MotionVersion: only V1 (0) is built. */
 enum { FGR_MOTION_V1 = 0 };
@@ -62,14 +63,15 @@ typedef struct FgrInitInfo {
     uint32_t bgra;            /* Rgba8 only: 1 = B8G8R8A8 byte order (the luma weights follow the channels) */
     uint32_t transfer;        /* RgbaF16: FGR_TRANSFER_* */
     float linearWhiteNits;    /* RgbaF16 linear: luminance of 1.0, (0, 10000]; 80 = scRGB */
-    uint32_t yuvMatrix;       /* P010: FGR_YUV_* */
-    uint32_t yuvFullRange;    /* P010: 0 = limited (video) range */
+    uint32_t yuvMatrix;       /* Nv12 / P010: FGR_YUV_* */
+    uint32_t yuvFullRange;    /* Nv12 / P010: 0 = limited (video) range */
     int32_t device;           /* HIP device ordinal */
     FgrSettings settings;
 } FgrInitInfo;
 
-/* framegen.h:154-159 Frame: one caller-owned image. P010: ptr = Y plane (W x H uint16), chroma = CbCr plane
- * (ceil(W/2) x ceil(H/2) pairs of uint16). Frames are identified by ptr (the reference compares image views). */
+/* framegen.h:154-159 Frame: one caller-owned image. Nv12 / P010: ptr = Y plane (W x H bytes / uint16 words), chroma =
+ * CbCr plane (W/2 x H/2 pairs of bytes / uint16 words; W and H even). Frames are identified by ptr (the reference
+ * compares image views). */
 typedef struct FgrFrame {
     void* ptr;
     uint64_t pitch;           /* bytes per row of ptr */
