@@ -62,6 +62,185 @@ def measured_traffic(kernel):
     return None
 
 
+def timed_run(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def plain_runner(items, settings=None, net=None, yuv_matrix=2, transfer=1):
+    """The per-pair C ABI path over batch items (prev, curr, n, outs, fmt): one handle per pair."""
+    import fgr
+    his = []
+    for prev, curr, n, outs, fmt in items:
+        t = prev[0] if isinstance(prev, tuple) else prev
+        fi = fgr.Interpolator(int(t.shape[1]), int(t.shape[0]), fmt=fmt, settings=settings, yuv_matrix=yuv_matrix,
+                              transfer=transfer)
+        if net is not None:
+            fi.set_seed_net(net)
+        his.append(fi)
+
+    def step():
+        for fi, (prev, curr, n, outs, fmt) in zip(his, items):
+            fi.begin_frame()
+            assert fi.record_estimate(prev, curr)
+            for k in range(1, n):
+                assert fi.record_synth(prev, curr, outs[k - 1], k / n)
+    return step, his
+
+
+def config_mixed(args):
+    """64 pairs of 1280x720 .. 3840x2160, RGBA8 / Rgb10 / RgbaF16 scRGB / P010, 2x / 3x / 4x: one batch, one launch per
+    pass for all formats, replayed as a hipGraph; the per-pair path timed beside it."""
+    import fgr
+    from fgr import workloads
+    import torch
+    items, total = workloads.mixed_batch()
+    torch.cuda.synchronize()
+    batch = fgr.Batch(items, use_graph=True, yuv_matrix=fgr.YUV_BT2020, transfer=fgr.TRANSFER_LINEAR)
+    bs = timed_run(batch.run, args.steps, args.warmup)
+    step, _ = plain_runner(items)
+    ps = timed_run(step, max(2, args.steps // 4), 1)
+    rate = len(items) * args.steps / bs
+    plain = len(items) * max(2, args.steps // 4) / ps
+    print(json.dumps({
+        "metric": "mixed-size, mixed-format batched V1 pairs/sec on 1 GPU (64 pairs per batch)", "value": round(rate, 2),
+        "unit": "pairs/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(1e3 * bs / args.steps, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u8/u10/f16", "data": "synthetic",
+        "config": {"workload": "64 pairs: 16 each of 1280x720, 1920x1080, 2560x1440, 3840x2160; formats RGBA8 / Rgb10 "
+                               "PQ / RgbaF16 scRGB (0-4) / P010 BT.2020 PQ; multipliers 2x/3x/4x; motion +-24 px",
+                   "launches_per_step": batch.launches()},
+        "plain_pairs_per_s": round(plain, 2), "batched_over_plain": round(rate / plain, 3),
+        "whole_path_hbm_frac": round(total * args.steps / bs / 1e9 / HBM_PEAK_GBS, 4),
+        "algorithmic_bytes_per_step": int(total)}))
+    return 0
+
+
+def config_seednet(args):
+    """The learned seed network: training on the GPU (20k synthetic pairs at the coarsest level of 1080p, flowScale 2,
+    3 pyramid levels: 240x136, batch 32), then the gain of its proposals over the plain seeds for a radius-2 search,
+    in BF16 and FP8, and the forward latency."""
+    import torch
+    import fgr
+    from fgr import seedtrain
+    w, h, batch = 240, 136, 32
+    steps = 20000 // batch
+    net = fgr.SeedNet(3)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = seedtrain.train(net, steps, w, h, batch=batch, lr=2e-3, seed=21, shift=10.0)
+    torch.cuda.synchronize()
+    train_s = time.perf_counter() - t0
+    r16 = seedtrain.evaluate(net, 512, w, h, R=2, seed=22, shift=10.0)
+    gen = torch.Generator(device="cuda:0")
+    gen.manual_seed(23)
+    la, lc, _ = seedtrain.make_pairs(64, w, h, gen, 10.0)
+    assert net.calibrate_fp8(la, lc) and net.set_fp8(True)
+    r8 = seedtrain.evaluate(net, 512, w, h, R=2, seed=22, shift=10.0)
+    out = torch.empty((64, (h + 7) // 8, (w + 7) // 8, 2), device="cuda:0")
+    lat = {}
+    for name, fp8 in (("bf16", False), ("fp8", True)):
+        net.set_fp8(fp8)
+        s = timed_run(lambda: net.forward(la, lc, out), 20, 3)
+        lat[name] = round(1e6 * s / 20 / 64, 3)
+    print(json.dumps({
+        "metric": "learned seeds: training pairs/sec on 1 GPU (BF16 MFMA forward + backward + Adam)",
+        "value": round(steps * batch / train_s, 1), "unit": "pairs/s", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": round(1e3 * train_s / steps, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 (fp32 accumulate); fp8 e4m3 inference", "data": "synthetic",
+        "config": {"workload": "5-layer 3x3 CNN 2-16-32-64-64-2 on 240x136 coarse luma pairs (value noise, affine "
+                               "motion: shift +-10 px, rotation +-3 deg, zoom 0.95-1.05), batch 32, 20k pairs, Adam",
+                   "width": w, "height": h, "batch": batch},
+        "loss_first_last": [round(float(sum(losses[:20]) / 20), 4), round(float(sum(losses[-20:]) / 20), 4)],
+        "eval_radius2_bf16": {k: round(v, 4) for k, v in r16.items()},
+        "eval_radius2_fp8": {k: round(v, 4) for k, v in r8.items()},
+        "forward_us_per_pair": lat}))
+    return 0
+
+
+def _read_dev(ptr, n, dtype):
+    """n elements at a device address as a new CUDA tensor (a device-to-device copy)."""
+    import ctypes
+    import torch
+    t = torch.empty(n, dtype=dtype, device="cuda:0")
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    torch.cuda.synchronize()
+    assert hip.hipMemcpy(t.data_ptr(), ptr, t.numel() * t.element_size(), 3) == 0
+    return t
+
+
+def config_uhd(args):
+    """128 pairs at 3840x2160 (RGBA8 and P010, 4x, 5 % scene cuts, 30 % occluding sprites) in one batch: the plain V1
+    path against the learned FP8 seeds (coarsest level 240x135 of a 4-level pyramid, radius-2 top search), with the
+    proposals' seed-hit rate, the block SAD and the endpoint error of the final field against the plain one."""
+    import torch
+    import fgr
+    from fgr import seedtrain, workloads
+    items, total, cuts = workloads.uhd_batch()
+    torch.cuda.synchronize()
+    net = fgr.SeedNet(3)
+    seedtrain.train(net, 20000 // 32, 240, 136, batch=32, lr=2e-3, seed=21, shift=10.0)
+    gen = torch.Generator(device="cuda:0")
+    gen.manual_seed(23)
+    la, lc, _ = seedtrain.make_pairs(64, 240, 136, gen, 10.0)
+    assert net.calibrate_fp8(la, lc) and net.set_fp8(True)
+    plain_set = fgr.default_settings()
+    seed_set = fgr.default_settings(seedNet=1, pyramidMinSide=72, searchRadiusTop=2)
+    rates, batches = {}, {}
+    for name, s, n in (("plain", plain_set, None), ("learned_fp8", seed_set, net)):
+        b = fgr.Batch(items, settings=s, use_graph=True, yuv_matrix=fgr.YUV_BT2020, seed_net=n)
+        t = timed_run(b.run, args.steps, args.warmup)
+        rates[name] = len(items) * args.steps / t
+        batches[name] = b
+    # quality: per pair, the final (sub-pixel) block fields and the block SAD of both paths; the proposals' hits
+    epe, sad, hits, blocks = [], {"plain": [], "learned_fp8": []}, 0, 0
+    for p, cut in enumerate(cuts):
+        if cut:
+            continue
+        d = {k: b.debug_internal(p) for k, b in batches.items()}
+        f = {k: _read_dev(x.field + 8 * x.fieldSub, 2 * x.bw[0] * x.bh[0], torch.float32).reshape(-1, 2)
+             for k, x in d.items()}
+        epe.append(float((f["learned_fp8"] - f["plain"]).norm(dim=-1).mean()))
+        for k, x in d.items():
+            u = _read_dev(x.counters, 128, torch.int32)[64:128]
+            sad[k].append(float(u.sum()) / 256.0 / (x.bw[0] * x.bh[0]))
+        x = d["learned_fp8"]
+        L = x.levels - 1
+        n = x.w[L] * x.h[L]
+        la_ = _read_dev(x.luma + 4 * x.lumaOffA[L], n, torch.float32).reshape(1, x.h[L], x.w[L])
+        lc_ = _read_dev(x.luma + 4 * x.lumaOffC[L], n, torch.float32).reshape(1, x.h[L], x.w[L])
+        raw = torch.empty((1, x.bh[L], x.bw[L], 2), device="cuda:0")
+        ref = torch.empty_like(raw)
+        assert fgr.search_field(la_, lc_, 6, 0.5, raw, ref)
+        prop = _read_dev(x.field + 8 * x.fieldSeed, 2 * x.bw[L] * x.bh[L], torch.float32).reshape(1, x.bh[L], x.bw[L], 2)
+        hits += int(((prop.round() - ref).abs().amax(-1) <= 2.5).sum())
+        blocks += x.bw[L] * x.bh[L]
+    print(json.dumps({
+        "metric": "4K 4x batched V1 pairs/sec on 1 GPU, FP8 learned seeds vs plain seeds (128 pairs per batch)",
+        "value": round(rates["learned_fp8"], 2), "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * len(items) / rates["learned_fp8"], 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8/u10; fp8 e4m3 seeds",
+        "data": "synthetic",
+        "config": {"workload": "128 pairs 3840x2160, RGBA8 and P010 (BT.2020 PQ), 4x, 5 % scene cuts, 30 % with four "
+                               "384x384 occluding sprites, motion +-24 px",
+                   "learned": "seedNet=1, FP8 E4M3, pyramidMinSide 72, searchRadiusTop 2",
+                   "plain": "default settings (pyramidMinSide 48, searchRadiusTop 6)"},
+        "plain_pairs_per_s": round(rates["plain"], 2), "learned_over_plain": round(rates["learned_fp8"] / rates["plain"], 3),
+        "seed_hit_rate": round(hits / max(blocks, 1), 4),
+        "endpoint_error_vs_plain_field": round(sum(epe) / len(epe), 4),
+        "mean_block_sad": {k: round(sum(v) / len(v), 4) for k, v in sad.items()},
+        "whole_path_hbm_frac": round(total * rates["learned_fp8"] / len(items) / 1e9 / HBM_PEAK_GBS, 4)}))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,11 +250,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--path", choices=["auto", "plain", "batched"], default="auto")
     ap.add_argument("--copy-mib", type=int, default=512, help="size of the device copy probe")
+    ap.add_argument("--config", type=int, choices=[2, 3, 4, 5], default=2,
+                    help="2: the headline 1080p 4x workload (default); 3: 64 mixed sizes / formats / multipliers; "
+                         "4: learned seeds, training and gains; 5: 128 4K pairs, plain vs FP8 learned seeds")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: every pair is independent, so more GPUs would shard "
                                    "the pair list with no communication; multi-GPU runs are not part of this build"}))
         return 2
+    if args.config != 2:
+        return {3: config_mixed, 4: config_seednet, 5: config_uhd}[args.config](args)
 
     import numpy as np
     import torch

@@ -83,6 +83,13 @@ class BatchInfo(ctypes.Structure):
 
 if not os.path.exists(_LIB_PATH):
     raise ImportError(f"libfgr.so is not built ({_LIB_PATH}); run __graft_entry__.build()")
+# One HIP runtime per process: PyTorch-ROCm ships its own libamdhip64 / libhsa-runtime64. Loaded after libfgr, torch
+# would bring a second HSA runtime into the process, and the one that opened the device second cannot use it. With
+# torch loaded first, libfgr's libamdhip64.so.7 resolves to torch's copy. So torch, when installed, is imported here.
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
 lib = ctypes.CDLL(_LIB_PATH)
 
 _H = ctypes.c_void_p
