@@ -104,6 +104,7 @@ def config_mixed(args):
     items, total = workloads.mixed_batch()
     torch.cuda.synchronize()
     batch = fgr.Batch(items, use_graph=True, yuv_matrix=fgr.YUV_BT2020, transfer=fgr.TRANSFER_LINEAR)
+    _, tuning = batch.autotune(reps=2)
     bs = timed_run(batch.run, args.steps, args.warmup)
     step, _ = plain_runner(items)
     ps = timed_run(step, max(2, args.steps // 4), 1)
@@ -116,7 +117,8 @@ def config_mixed(args):
         "vs_baseline": None, "dtype": "u8/u10/f16", "data": "synthetic",
         "config": {"workload": "64 pairs: 16 each of 1280x720, 1920x1080, 2560x1440, 3840x2160; formats RGBA8 / Rgb10 "
                                "PQ / RgbaF16 scRGB (0-4) / P010 BT.2020 PQ; multipliers 2x/3x/4x; motion +-24 px",
-                   "launches_per_step": batch.launches()},
+                   "launches_per_step": batch.launches(), "tuning": {"synth_tile_rows": tuning[0],
+                                                                     "xcd_chunk": tuning[1]}},
         "plain_pairs_per_s": round(plain, 2), "batched_over_plain": round(rate / plain, 3),
         "whole_path_hbm_frac": round(total * args.steps / bs / 1e9 / HBM_PEAK_GBS, 4),
         "algorithmic_bytes_per_step": int(total)}))
@@ -287,6 +289,7 @@ def main():
     if args.path in ("auto", "batched") and hasattr(fgr, "Batch"):
         bouts = [[torch.empty_like(fr[0][0]) for _ in range(MULT - 1)] for _ in fr]
         batch = fgr.Batch([(a, c, MULT, o) for (a, c), o in zip(fr, bouts)], use_graph=True)
+        tune_times, tuning = batch.autotune(reps=3)   # tile rows x XCD chunk, timed on this batch
 
     def timed(fn, steps, warmup):
         for _ in range(warmup):
@@ -324,7 +327,8 @@ def main():
                     synth_ns.append(ns)
                     synth_bytes_l.append(sum(synth_bytes(W, H, cut=(p == cut_index)) for p in range(len(fr))))
         rb.set_profiling(False)
-        kernel = "synth_rgba8_s2_batch_kernel"
+        kernel = {16: "synth_rgba8_s2_t16_batch_kernel", 32: "synth_rgba8_s2_batch_kernel",
+                  64: "synth_rgba8_s2_t64_batch_kernel"}[tuning[0]]
     # the plain path's per-synth launch, for comparison (the library's profile: events around each synth)
     fi.set_profiling(True)
     plain_ns = []
@@ -393,6 +397,9 @@ def main():
         "batched_over_plain": None if batch_rate is None else round(batch_rate / plain_rate, 3),
         "whole_path_hbm_frac": round(total_bytes * args.steps / (ms * 1e-3 * args.steps) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": roofline,
+        "tuning": None if batch is None else {"synth_tile_rows": tuning[0], "xcd_chunk": tuning[1],
+                                               "us_per_step": {f"{r}x{c}": round(t / 1e3, 1)
+                                                               for (r, c), t in tune_times.items()}},
         "copy_probe": {"bytes": copy_bytes, "gbs": round(copy_gbs, 1), "frac_of_peak": round(copy_gbs / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
     }

@@ -10,7 +10,9 @@
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cstring>
+#include <cstdio>
 #include <map>
+#include <string>
 #include <vector>
 #include "../../include/fgr.h"
 #include "runtime.hpp"
@@ -42,12 +44,15 @@ struct FgrBatch {
     std::vector<Launch> launches;
     std::vector<uint8_t> blob;   // host copy of the device blob
     uint8_t* dev = nullptr;
+    size_t devCap = 0;
     hipStream_t cap = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool profiling = false;
     std::vector<hipEvent_t> events;   // a (start, end) pair per launch while profiling
     bool profiled = false;
+    int chunk = kXcdChunk;            // tiles per XCD chunk of the work lists (tuned)
+    int rows = 32;                    // RGBA8 synth tile rows (tuned)
 
     ~FgrBatch() {
         for (hipEvent_t e : events) hipEventDestroy(e);
@@ -91,7 +96,7 @@ struct FgrBatch {
             // an L2
             const size_t n = work.size();
             std::vector<uint32_t> order(n);
-            for (size_t i = 0; i < n; i++) order[i] = work[xcdChunkTile((int)i, (int)n)];
+            for (size_t i = 0; i < n; i++) order[i] = work[xcdChunkTile((int)i, (int)n, chunk)];
             L.n = (int)n;
             L.workOff = blob.size();
             blob.resize(align256(blob.size() + n * 4));
@@ -126,7 +131,39 @@ struct FgrBatch {
         exec = nullptr;
         graph = nullptr;
     }
+
+    // a tile configuration: rebuild the launches and the device blob (its size changes with the grids)
+    bool tune(int r, int c) {
+        rows = r;
+        chunk = c;
+        for (FgrInterpolator* p : pairs) p->synthRows = r;
+        build();
+        dropGraph();
+        if (blob.size() > devCap) {
+            if (dev) hipFree(dev);
+            dev = nullptr;
+            devCap = 0;
+            if (hipMalloc(&dev, blob.size()) != hipSuccess) return false;
+            devCap = blob.size();
+        }
+        return hipMemcpy(dev, blob.data(), blob.size(), hipMemcpyHostToDevice) == hipSuccess;
+    }
+
+    // the tuning cache key: every pair's size, format, multiplier and the settings that shape the grids
+    std::string key() const {
+        std::string k = std::to_string(info.settings.flowScale) + "/" + std::to_string(info.settings.pyramidMinSide);
+        for (const FgrBatchPair& d : desc)
+            k += ";" + std::to_string(d.width) + "x" + std::to_string(d.height) + "f" + std::to_string(d.format) + "n" +
+                 std::to_string(d.multiplier);
+        return k;
+    }
 };
+
+// Tile configurations chosen by fgr_batch_autotune, per batch key (process-wide; fgr_tuning_save / load persist it)
+static std::map<std::string, std::pair<int, int>>& tuningCache() {
+    static std::map<std::string, std::pair<int, int>> m;
+    return m;
+}
 
 static const char* kBErrNull = "null batch description";
 static const char* kBErrCount = "a batch holds 1 to 1024 pairs";
@@ -176,8 +213,15 @@ extern "C" FgrBatch* fgr_batch_create(const FgrBatchInfo* info, const FgrBatchPa
             return nullptr;
         }
     }
+    const auto hit = tuningCache().find(b->key());
+    if (hit != tuningCache().end()) {
+        b->rows = hit->second.first;
+        b->chunk = hit->second.second;
+        for (FgrInterpolator* p : b->pairs) p->synthRows = b->rows;
+    }
     b->build();
     if (hipMalloc(&b->dev, b->blob.size()) != hipSuccess) { gBatchError = kBErrAlloc; delete b; return nullptr; }
+    b->devCap = b->blob.size();
     if (hipMemcpy(b->dev, b->blob.data(), b->blob.size(), hipMemcpyHostToDevice) != hipSuccess) {
         gBatchError = kBErrAlloc;
         delete b;
@@ -206,9 +250,8 @@ extern "C" int fgr_batch_set_frames(FgrBatch* b, void* stream, const FgrBatchPai
             pairs[p].multiplier != b->desc[p].multiplier || pairs[p].format != b->desc[p].format)
             return 0;
     b->desc.assign(pairs, pairs + n);
-    const size_t old = b->blob.size();
     b->build();
-    if (b->blob.size() != old) return 0;
+    if (b->blob.size() > b->devCap) return 0;   // (same sizes: the same layout; a captured graph stays valid)
     return hipMemcpyAsync(b->dev, b->blob.data(), b->blob.size(), hipMemcpyHostToDevice, (hipStream_t)stream) ==
            hipSuccess;
 }
@@ -234,6 +277,79 @@ extern "C" int fgr_batch_record(FgrBatch* b, void* stream) {
         }
     }
     return hipGraphLaunch(b->exec, s) == hipSuccess ? 1 : 0;
+}
+
+extern "C" int fgr_batch_set_tuning(FgrBatch* b, int32_t rows, int32_t chunk) {
+    if (!b || !(rows == 16 || rows == 32 || rows == 64) || chunk < 1 || chunk > 4096) return 0;
+    hipSetDevice(b->info.device);
+    return b->tune(rows, chunk) ? 1 : 0;
+}
+
+extern "C" int fgr_batch_get_tuning(const FgrBatch* b, int32_t* rows, int32_t* chunk) {
+    if (!b) return 0;
+    if (rows) *rows = b->rows;
+    if (chunk) *chunk = b->chunk;
+    return 1;
+}
+
+// Times every tile configuration (RGBA8 synth tiles of 16, 32, 64 rows x work-list XCD chunks of 16, 64, 256 tiles)
+// on this batch's own records -- `reps` eager records each, after one untimed -- keeps the fastest, and caches it for
+// batches of the same key. The records write the batch's outputs. ns[i] receives configuration i's time per record
+// (i = rowsIndex * 3 + chunkIndex; 9 entries) when given.
+extern "C" int fgr_batch_autotune(FgrBatch* b, void* stream, int32_t reps, uint64_t* ns) {
+    if (!b || reps < 1) return 0;
+    hipSetDevice(b->info.device);
+    hipStream_t s = (hipStream_t)stream;
+    static const int kRows[3] = {16, 32, 64}, kChunks[3] = {16, 64, 256};
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return 0;
+    double best = 1e30;
+    int bestR = b->rows, bestC = b->chunk;
+    bool ok = true;
+    for (int ri = 0; ri < 3 && ok; ri++)
+        for (int ci = 0; ci < 3 && ok; ci++) {
+            if (!b->tune(kRows[ri], kChunks[ci]) || !b->enqueue(s)) { ok = false; break; }
+            hipEventRecord(e0, s);
+            for (int r = 0; r < reps && ok; r++) ok = b->enqueue(s);
+            hipEventRecord(e1, s);
+            if (hipEventSynchronize(e1) != hipSuccess) { ok = false; break; }
+            float ms = 0.0f;
+            hipEventElapsedTime(&ms, e0, e1);
+            const double t = (double)ms / reps;
+            if (ns) ns[ri * 3 + ci] = (uint64_t)(t * 1e6 + 0.5);
+            if (t < best) {
+                best = t;
+                bestR = kRows[ri];
+                bestC = kChunks[ci];
+            }
+        }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (!ok || !b->tune(bestR, bestC)) return 0;
+    tuningCache()[b->key()] = {bestR, bestC};
+    return 1;
+}
+
+// the tuning cache as text, one "key rows chunk" line per entry
+extern "C" int fgr_tuning_save(const char* path) {
+    FILE* f = path ? std::fopen(path, "w") : nullptr;
+    if (!f) return 0;
+    for (const auto& e : tuningCache()) std::fprintf(f, "%s %d %d\n", e.first.c_str(), e.second.first, e.second.second);
+    std::fclose(f);
+    return 1;
+}
+
+extern "C" int fgr_tuning_load(const char* path) {
+    FILE* f = path ? std::fopen(path, "r") : nullptr;
+    if (!f) return 0;
+    char key[65536];
+    int r, c, n = 0;
+    while (std::fscanf(f, "%65535s %d %d", key, &r, &c) == 3) {
+        tuningCache()[key] = {r, c};
+        n++;
+    }
+    std::fclose(f);
+    return n;
 }
 
 extern "C" int fgr_batch_set_profiling(FgrBatch* b, int on) {

@@ -610,13 +610,19 @@ void FgrInterpolator::planSynth(std::vector<PlanOp>& plan, const FgrFrame& prev,
     sp.ocpitch = (int64_t)out.chromaPitch;
     sp.gx = synthTilesX((int)W);
     PlanOp op;
-    if (format == FGR_FORMAT_RGBA8)
-        op.pass = set.flowScale == 1 ? kPassSynthRgba8S1 : (set.flowScale == 2 ? kPassSynthRgba8S2 : kPassSynthRgba8S4);
-    else
+    int rows = 32;
+    if (format == FGR_FORMAT_RGBA8) {
+        const int S = (int)set.flowScale;
+        rows = (synthRows == 64 && S == 2) ? 64 : (synthRows == 16 ? 16 : 32);
+        if (rows == 16) op.pass = S == 1 ? kPassSynthRgba8S1T16 : (S == 2 ? kPassSynthRgba8S2T16 : kPassSynthRgba8S4T16);
+        else if (rows == 64) op.pass = kPassSynthRgba8S2T64;
+        else op.pass = S == 1 ? kPassSynthRgba8S1 : (S == 2 ? kPassSynthRgba8S2 : kPassSynthRgba8S4);
+    } else {
         op.pass = set.flowScale == 1 ? kPassSynthFmtS1 : (set.flowScale == 2 ? kPassSynthFmtS2 : kPassSynthFmtS4);
+    }
     op.stage = 500 + 10 * k;
     op.gx = sp.gx;
-    op.gy = synthTilesY((int)H);
+    op.gy = synthTilesY((int)H, rows);
     op.stamp = "synth";
     op.set(sp);
     plan.push_back(op);

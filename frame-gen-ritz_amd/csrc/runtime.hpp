@@ -58,6 +58,7 @@ struct FgrInterpolator {
 
     // learned seed proposals (seednet.hip): the attached network and its activations on the coarsest level
     FgrSeedNet* net = nullptr;
+    int synthRows = 32;               // RGBA8 synth tile height (16, 32, 64 with flowScale 2), chosen by the batch tuner
     uint8_t* blurScratch = nullptr;   // recordSynthBlur's sub-frames (blur.hip), grown by the first blurred record
     size_t blurCap = 0;
     float* seedAct = nullptr;

@@ -90,11 +90,11 @@ __device__ __forceinline__ int linearBlock() { return (int)(blockIdx.y * gridDim
 // pair (whole-pair chunks would leave an XCD idle on a cheap scene-cut pair). The tail beyond the last whole round of
 // 8 chunks keeps the plain order. The batched launches get the same order from their host-built work lists.
 constexpr int kXcds = 8, kXcdChunk = 64;
-__host__ __device__ __forceinline__ int xcdChunkTile(int i, int n) {
-    const int span = kXcds * kXcdChunk;
+__host__ __device__ __forceinline__ int xcdChunkTile(int i, int n, int chunk = kXcdChunk) {
+    const int span = kXcds * chunk;
     if (i >= n - n % span) return i;
     const int m = i / kXcds, x = i % kXcds;
-    return ((m / kXcdChunk) * kXcds + x) * kXcdChunk + m % kXcdChunk;
+    return ((m / chunk) * kXcds + x) * chunk + m % chunk;
 }
 
 // One device body, two kernels: name_kernel runs one pass of one pair on a (gx, gy) grid; name_batch_kernel runs the

@@ -10,7 +10,9 @@ enum Pass : int {
     kPassLuma = 0, kPassDown, kPassMedhist, kPassSearch2, kPassSearch6, kPassSearch12, kPassMedian3, kPassSubpix,
     kPassPixfield, kPassSynthRgba8S1, kPassSynthRgba8S2, kPassSynthRgba8S4, kPassFallback,
     kPassLumaFmt, kPassSynthFmtS1, kPassSynthFmtS2, kPassSynthFmtS4,   // Rgb10 / RgbaF16 / P010 (v1_formats.hip)
-    kPassSeedConv                                                      // learned seed network layer (seednet.hip)
+    kPassSeedConv,                                                     // learned seed network layer (seednet.hip)
+    kPassSynthRgba8S1T16, kPassSynthRgba8S2T16, kPassSynthRgba8S4T16,  // RGBA8 synth, 16-row tiles
+    kPassSynthRgba8S2T64                                               // RGBA8 synth, 64-row tiles (flowScale 2)
 };
 hipError_t launchBatch(int pass, const void* params, const uint32_t* work, int n, hipStream_t s);
 hipError_t launchSingle(int pass, const void* params, int gx, int gy, hipStream_t s);
@@ -23,7 +25,7 @@ hipError_t launchSeedSingle(int pass, const void* params, int gx, int gy, hipStr
 hipError_t launchSeedBatch(int pass, const void* params, const uint32_t* work, int n, hipStream_t s);
 constexpr int kSubpixBlocks = 7;   // blocks per subpix workgroup
 int synthTilesX(int W);
-int synthTilesY(int H);
+int synthTilesY(int H, int rows = 32);
 int fallbackTilesX(int W);
 int fallbackTilesY(int H);
 

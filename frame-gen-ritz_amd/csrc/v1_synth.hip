@@ -156,15 +156,16 @@ __device__ __forceinline__ uint32_t blendRgba8(const SynthParams& P, uint32_t wa
     return r;
 }
 
-// One 64 x 32 tile of the synthesised frame (passes 9-11 fused): field tile (+ lookup at t), mismatch of the tile and
+// One 64 x RT tile (RT = 16, 32 or 64 rows; 4, 8 or 16 pixels a thread) of the synthesised frame (passes 9-11 fused): field tile (+ lookup at t), mismatch of the tile and
 // its apron, 7x7 box, trust, untrusted count, blend. Scene cut: a copy of the nearer frame.
 //
 // halfVec() motion_v1.glsl:73-84 is split in its two stages: the horizontal mixes of each field row at every full-res
 // column of the tile (row[j][x] = mix(F[j][i0(x)], F[j][i1(x)], fx(x))), then per pixel the vertical mix of two of
 // those rows; the operations and their order are the shader's, so the result is the same float.
-template <int S>
+template <int S, int RT>
 __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, int tileY, int nTilesX) {
-    constexpr int FTW = SynthTile<S>::FTW, FTH = SynthTile<S>::FTH;
+    constexpr int FTW = SynthTile<S, RT>::FTW, FTH = SynthTile<S, RT>::FTH;
+    constexpr int TH = RT, EH = TileRows<RT>::EH, NAPRON = TileRows<RT>::NAPRON, RK = RT / 16;
     __shared__ float2 ft[FTW * FTH];
     __shared__ float2 hrow[FTH * EW];
     __shared__ int es[EH * EW];
@@ -183,13 +184,13 @@ __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, 
     }
     if (tid == 0) cnt = 0u;
     const int ylim = S * P.fh - 1;
-    const int ty0 = loadFieldTile<S>(P, x0, y0, ft, hrow);
+    const int ty0 = loadFieldTile<S, RT>(P, x0, y0, ft, hrow);
     const float kA32 = (float)(32 * S) * (2.0f * P.wC), kC32 = (float)(32 * S) * (2.0f * P.wA);
-    // interior pixels in runs of 4 (columns 4 rc .. 4 rc + 3, rows rr and rr + 16): warps kept in registers (packed)
+    // interior pixels in runs of 4 (columns 4 rc .. 4 rc + 3, rows rr + 16 k): warps kept in registers (packed)
     const int rc = tid & 15, rr = tid >> 4;
-    uint32_t wa[8], wc[8];
+    uint32_t wa[4 * RK], wc[4 * RK];
 #pragma unroll
-    for (int k = 0; k < 2; k++) {
+    for (int k = 0; k < RK; k++) {
         const int y = min(y0 + rr + 16 * k, P.H - 1);
         const int yc = min(y, ylim);
         const float2* h = hrow + (FlowMap<S>::index(yc) - ty0) * EW + 4 * rc + AP;
@@ -249,7 +250,7 @@ __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, 
     __syncthreads();
     uint32_t mine = 0u;
 #pragma unroll
-    for (int k = 0; k < 2; k++) {
+    for (int k = 0; k < RK; k++) {
         const int x = x0 + 4 * rc, y = y0 + rr + 16 * k;
         if (y >= P.H || x >= P.W) continue;
         const uint16_t* h = &hs[(rr + 16 * k) * TW + 4 * rc];
@@ -290,9 +291,9 @@ __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, 
     if (tid == 0 && cnt) atomicAdd(P.count + ((tileY * nTilesX + tileX) & (kSlots - 1)), cnt);
 }
 
-template <int S>
+template <int S, int RT>
 __device__ __forceinline__ void synthRgba8Body(const SynthParams& P, int bx, int by) {
-    synthRgba8Tile<S>(P, bx, by, P.gx);
+    synthRgba8Tile<S, RT>(P, bx, by, P.gx);
 }
 
 // lookup.comp:10-19 (interp.py:194-205): the per-pixel field at time t, g <- pf(q - s g) twice (LOOKUP_ITERS = 2)
@@ -362,9 +363,14 @@ hipError_t launchDebugCounters(const uint32_t* count, const uint32_t* sad, uint3
 }
 
 
-FGR_PASS(synth_rgba8_s1, SynthParams, synthRgba8Body<1>, 256)
-FGR_PASS(synth_rgba8_s2, SynthParams, synthRgba8Body<2>, 256)
-FGR_PASS(synth_rgba8_s4, SynthParams, synthRgba8Body<4>, 256)
+#define FGR_SYNTH_BODY(S, RT) synthRgba8Body<S, RT>
+FGR_PASS(synth_rgba8_s1, SynthParams, FGR_SYNTH_BODY(1, 32), 256)
+FGR_PASS(synth_rgba8_s2, SynthParams, FGR_SYNTH_BODY(2, 32), 256)
+FGR_PASS(synth_rgba8_s4, SynthParams, FGR_SYNTH_BODY(4, 32), 256)
+FGR_PASS(synth_rgba8_s1_t16, SynthParams, FGR_SYNTH_BODY(1, 16), 256)
+FGR_PASS(synth_rgba8_s2_t16, SynthParams, FGR_SYNTH_BODY(2, 16), 256)
+FGR_PASS(synth_rgba8_s4_t16, SynthParams, FGR_SYNTH_BODY(4, 16), 256)
+FGR_PASS(synth_rgba8_s2_t64, SynthParams, FGR_SYNTH_BODY(2, 64), 256)
 FGR_PASS(fallback, FallbackParams, fallbackBody, 256)
 
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
@@ -375,7 +381,7 @@ hipError_t launchLookup(const LookupParams& p, hipStream_t s) {
 }
 
 int synthTilesX(int W) { return cdiv(W, TW); }
-int synthTilesY(int H) { return cdiv(H, TH); }
+int synthTilesY(int H, int rows) { return cdiv(H, rows); }
 int fallbackTilesX(int W) { return cdiv(W, 64); }
 int fallbackTilesY(int H) { return cdiv(H, 64); }
 
@@ -385,6 +391,10 @@ hipError_t launchSynthSingle(int pass, const void* params, int gx, int gy, hipSt
     case kPassSynthRgba8S1: hipLaunchKernelGGL(synth_rgba8_s1_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
     case kPassSynthRgba8S2: hipLaunchKernelGGL(synth_rgba8_s2_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
     case kPassSynthRgba8S4: hipLaunchKernelGGL(synth_rgba8_s4_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
+    case kPassSynthRgba8S1T16: hipLaunchKernelGGL(synth_rgba8_s1_t16_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
+    case kPassSynthRgba8S2T16: hipLaunchKernelGGL(synth_rgba8_s2_t16_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
+    case kPassSynthRgba8S4T16: hipLaunchKernelGGL(synth_rgba8_s4_t16_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
+    case kPassSynthRgba8S2T64: hipLaunchKernelGGL(synth_rgba8_s2_t64_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
     case kPassFallback: hipLaunchKernelGGL(fallback_kernel, g, dim3(256), 0, s, *(const FallbackParams*)params); break;
     default: return launchFmtSingle(pass, params, gx, gy, s);
     }
@@ -397,6 +407,10 @@ hipError_t launchSynthBatch(int pass, const void* params, const uint32_t* work, 
     case kPassSynthRgba8S1: hipLaunchKernelGGL(synth_rgba8_s1_batch_kernel, g, dim3(256), 0, s, (const SynthParams*)params, work); break;
     case kPassSynthRgba8S2: hipLaunchKernelGGL(synth_rgba8_s2_batch_kernel, g, dim3(256), 0, s, (const SynthParams*)params, work); break;
     case kPassSynthRgba8S4: hipLaunchKernelGGL(synth_rgba8_s4_batch_kernel, g, dim3(256), 0, s, (const SynthParams*)params, work); break;
+    case kPassSynthRgba8S1T16: hipLaunchKernelGGL(synth_rgba8_s1_t16_batch_kernel, g, dim3(256), 0, s, (const SynthParams*)params, work); break;
+    case kPassSynthRgba8S2T16: hipLaunchKernelGGL(synth_rgba8_s2_t16_batch_kernel, g, dim3(256), 0, s, (const SynthParams*)params, work); break;
+    case kPassSynthRgba8S4T16: hipLaunchKernelGGL(synth_rgba8_s4_t16_batch_kernel, g, dim3(256), 0, s, (const SynthParams*)params, work); break;
+    case kPassSynthRgba8S2T64: hipLaunchKernelGGL(synth_rgba8_s2_t64_batch_kernel, g, dim3(256), 0, s, (const SynthParams*)params, work); break;
     case kPassFallback: hipLaunchKernelGGL(fallback_batch_kernel, g, dim3(256), 0, s, (const FallbackParams*)params, work); break;
     default: return launchFmtBatch(pass, params, work, n, s);
     }

@@ -9,6 +9,11 @@ namespace fgr {
 // Tile of the fused synth kernel: 64 x 32 output pixels per 256-thread workgroup (8 per thread), a 3-pixel apron for
 // the 7x7 box (70 x 38 mismatch values, 1.30 per output pixel).
 constexpr int TW = 64, TH = 32, AP = 3, EW = TW + 2 * AP, EH = TH + 2 * AP, NAPRON = EW * EH - TW * TH;
+// the same geometry for a tile of TH_ rows (the RGBA8 kernel comes in 16, 32 and 64 rows; the tile height is tuned)
+template <int TH_>
+struct TileRows {
+    static constexpr int H = TH_, EH = TH_ + 2 * AP, NAPRON = EW * EH - TW * TH_;
+};
 
 struct alignas(4) U32x2 {
     uint32_t x, y;
@@ -65,19 +70,19 @@ struct FlowMap {
     }
 };
 
-template <int S>
+template <int S, int TH_ = TH>
 struct SynthTile {
     // field tile (flow res) covering the tile and its apron, and its rows interpolated to full-res columns
-    static constexpr int FTW = (EW + S - 1) / S + 3, FTH = (EH + S - 1) / S + 3;
+    static constexpr int FTW = (EW + S - 1) / S + 3, FTH = (TileRows<TH_>::EH + S - 1) / S + 3;
 };
 
 // The field tile of an output tile at (x0, y0) and its apron (clamped to the frame, then to S * fw - 1), looked up at t
 // when P.lookup is set, and its rows interpolated at every full-res column x0 - AP .. x0 + TW + AP - 1 of the region
 // (hrow[j * EW + ex]; the first half of halfVec, motion_v1.glsl:73-84). Returns the flow row of ft's first row.
 // Ends with a barrier.
-template <int S>
+template <int S, int TH_ = TH>
 __device__ __forceinline__ int loadFieldTile(const SynthParams& P, int x0, int y0, float2* ft, float2* hrow) {
-    constexpr int FTW = SynthTile<S>::FTW, FTH = SynthTile<S>::FTH;
+    constexpr int FTW = SynthTile<S, TH_>::FTW, FTH = SynthTile<S, TH_>::FTH;
     const int tid = threadIdx.x;
     const int xlim = S * P.fw - 1, ylim = S * P.fh - 1;
     const int xmin = min(max(x0 - AP, 0), xlim), ymin = min(max(y0 - AP, 0), ylim);

@@ -151,6 +151,11 @@ _sig = {
                                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "fgr_set_seed_net": (ctypes.c_int, [_H, _H]),
     "fgr_fgmaps_header": (ctypes.c_int, [_H, ctypes.c_void_p]),
+    "fgr_batch_autotune": (ctypes.c_int, [_H, ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64)]),
+    "fgr_batch_set_tuning": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.c_int32]),
+    "fgr_batch_get_tuning": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
+    "fgr_tuning_save": (ctypes.c_int, [ctypes.c_char_p]),
+    "fgr_tuning_load": (ctypes.c_int, [ctypes.c_char_p]),
     "fgr_seednet_calibrate_fp8": (ctypes.c_int, [_H, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
                                                  ctypes.c_int32, ctypes.c_int32]),
     "fgr_seednet_set_fp8": (ctypes.c_int, [_H, ctypes.c_int]),
@@ -410,6 +415,24 @@ class Batch:
     def set_profiling(self, on):
         return bool(lib.fgr_batch_set_profiling(self._h, int(on)))
 
+    TUNE_ROWS, TUNE_CHUNKS = (16, 32, 64), (16, 64, 256)
+
+    def autotune(self, reps=3, stream=None):
+        """Time the 9 tile configurations on this batch, keep the fastest; returns ({(rows, chunk): ns}, chosen)."""
+        ns = (ctypes.c_uint64 * 9)()
+        if not lib.fgr_batch_autotune(self._h, _stream(stream), reps, ns):
+            raise RuntimeError("fgr_batch_autotune failed")
+        times = {(r, c): int(ns[i * 3 + j]) for i, r in enumerate(self.TUNE_ROWS) for j, c in enumerate(self.TUNE_CHUNKS)}
+        return times, self.get_tuning()
+
+    def set_tuning(self, rows, chunk):
+        return bool(lib.fgr_batch_set_tuning(self._h, rows, chunk))
+
+    def get_tuning(self):
+        r, c = ctypes.c_int32(), ctypes.c_int32()
+        lib.fgr_batch_get_tuning(self._h, ctypes.byref(r), ctypes.byref(c))
+        return r.value, c.value
+
     def read_timings(self):
         """[(stage, pass, ns), ...] per launch of the last profiled record, or None."""
         n = self.launches()
@@ -502,3 +525,13 @@ def search_field(la, lc, R, lam, raw, out, seeds=None, stream=None):
     n, h, w = la.shape
     return bool(lib.fgr_seednet_search_field(_stream(stream), _ptr(la), _ptr(lc), w, h, n, R, lam, _ptr(seeds),
                                              _ptr(raw), _ptr(out)))
+
+
+def tuning_save(path):
+    """Write the process's batch tuning cache (fgr_tuning_save)."""
+    return bool(lib.fgr_tuning_save(str(path).encode()))
+
+
+def tuning_load(path):
+    """Read a tuning cache file; returns the number of entries."""
+    return int(lib.fgr_tuning_load(str(path).encode()))

@@ -183,6 +183,16 @@ uint32_t fgr_batch_launches(const FgrBatch* b);
  * 401 pixfield, 500 + 10 k synth k, 501 + 10 k its fallback), its pass and its duration in ns. Returns the number of
  * launches (-1 if nothing was profiled). */
 int fgr_batch_set_profiling(FgrBatch* b, int on);
+/* Tile autotuner: the tile configurations are the RGBA8 synth's tile height (16, 32 or 64 rows) and the number of
+ * consecutive tiles each XCD takes from a launch's work list (16, 64 or 256). autotune times all nine on this batch's
+ * own records (`reps` each; the outputs are written), keeps the fastest, and remembers it for batches of the same
+ * pairs (sizes, formats, multipliers, flowScale, pyramidMinSide), which then start with it. Results are bit-identical
+ * under every configuration. ns: 9 times per record (ns, index rows * 3 + chunk), or null. */
+int fgr_batch_autotune(FgrBatch* b, void* stream, int32_t reps, uint64_t* ns);
+int fgr_batch_set_tuning(FgrBatch* b, int32_t rows, int32_t chunk);
+int fgr_batch_get_tuning(const FgrBatch* b, int32_t* rows, int32_t* chunk);
+int fgr_tuning_save(const char* path);
+int fgr_tuning_load(const char* path);   /* returns the number of entries read */
 int fgr_batch_read_timings(FgrBatch* b, int32_t* stage, int32_t* pass, uint64_t* ticks, int cap);
 /* [untrusted pixels of synth `synth` of pair `pair`, the pair's scene-cut SAD sum x 256] into dst (device) */
 int fgr_batch_debug_counters(FgrBatch* b, void* stream, uint32_t pair, uint32_t synth, void* dst);

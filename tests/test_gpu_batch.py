@@ -125,3 +125,38 @@ def test_1080p_workload_batch_matches_per_pair():
         for k in range(3):
             assert np.array_equal(items[p][3][k].cpu().numpy(), want[k]), f"pair {p} output {k}"
     b.close()
+
+
+@pytest.mark.parametrize("rows,chunk", [(16, 16), (32, 256), (64, 64)])
+def test_tile_configurations_are_bit_identical(rows, chunk):
+    """Every tile configuration of the tuner (synth tile rows x XCD chunk) gives the per-pair path's bytes."""
+    pairs, mults = mixed_pairs()
+    b, items = make_batch(pairs, mults, use_graph=True)
+    assert b.set_tuning(rows, chunk) and b.get_tuning() == (rows, chunk)
+    b.run()
+    torch.cuda.synchronize()
+    for p, ((a, c), n) in enumerate(zip(pairs, mults)):
+        want = per_pair(a, c, n)
+        for k in range(n - 1):
+            assert np.array_equal(items[p][3][k].cpu().numpy(), want[k]), f"pair {p} output {k}"
+    b.close()
+
+
+def test_autotune_picks_the_fastest_and_caches_it(tmp_path):
+    pairs, _ = scenes.pairs_1080p(2, n_pairs=4)
+    b, items = make_batch(pairs, [4] * 4, use_graph=True)
+    times, chosen = b.autotune(reps=3)
+    assert len(times) == 9 and all(t > 0 for t in times.values())
+    assert times[chosen] == min(times.values())
+    b.run()
+    torch.cuda.synchronize()
+    want = per_pair(*pairs[1], 4)
+    for k in range(3):
+        assert np.array_equal(items[1][3][k].cpu().numpy(), want[k])
+    # a second batch of the same pairs starts with the cached choice; the cache survives a save / load
+    b2, _ = make_batch(pairs, [4] * 4, use_graph=True)
+    assert b2.get_tuning() == chosen
+    path = tmp_path / "tuning.txt"
+    assert fgr.tuning_save(path) and fgr.tuning_load(path) >= 1
+    b.close()
+    b2.close()
