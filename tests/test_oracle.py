@@ -291,3 +291,38 @@ def test_wide_median_is_the_exact_median():
             assert np.array_equal(got, v1.global_median2(parent, bw, bh))
     s = v1.settings(searchRadius=12, searchRadiusTop=12, pyramidMinSide=16)
     assert v1.medhist_shift(s, 5, 1) == 1 and v1.medhist_shift(s, 5, 4) == -1 and v1.medhist_shift(v1.DEFAULTS, 6, 1) == -1
+
+
+# --------------------------------------------------------------------------------------------- blur (recordSynthBlur)
+
+def test_blur_sample_times_are_slice_midpoints_and_weights_a_gaussian():
+    """framegen.cpp:3549-3555: the midpoints of n equal slices; blur_resolve.comp:128-135: weights of a Gaussian whose
+    window [t0, t1] spans +-2 sigma, sampled at the midpoints (scipy's normal density, normalised to 1 at the centre)."""
+    from scipy.stats import norm
+    ts, ws = v1.blur_samples(0.25, 0.75, 4, False)
+    assert [float(t) for t in ts] == [0.3125, 0.4375, 0.5625, 0.6875]
+    assert [float(w) for w in ws] == [1.0] * 4
+    for n in (2, 3, 4, 7, 16):
+        ts, ws = v1.blur_samples(0.0, 1.0, n, True)
+        mids = (np.arange(n) + 0.5) / n
+        assert np.array_equal(np.array(ts, f32), mids.astype(f32))
+        z = (mids - 0.5) / 0.25                          # sigma = a quarter of the window
+        want = (norm.pdf(z) / norm.pdf(0.0)).astype(f32)
+        assert np.allclose(np.array(ws, f32), want, rtol=2e-7, atol=0)
+        assert ws == ws[::-1]
+
+
+def test_blur_average_rounds_half_up_once():
+    """Two equally weighted frames average to (a + b + 1) >> 1 over every byte pair; any weights of equal frames give
+    the frame back; a static pair's blurred frame is the frame."""
+    a, b = np.meshgrid(np.arange(256), np.arange(256), indexing="ij")
+    got = v1.average_frames([a.astype(np.uint8), b.astype(np.uint8)], [f32(1.0), f32(1.0)])
+    assert np.array_equal(got, (a + b + 1) >> 1)
+    _, ws = v1.blur_samples(0.0, 1.0, 5, True)
+    frame = np.arange(256, dtype=np.uint8).reshape(16, 16)
+    assert np.array_equal(v1.average_frames([frame] * 5, ws), frame)
+    rng = np.random.default_rng(4)
+    img = rng.integers(0, 256, (40, 56, 4), dtype=np.uint8)
+    img[..., 3] = 255
+    est = v1.estimate(img, img)
+    assert np.array_equal(v1.synth_blur(img, img, est, 0.2, 0.8, 4, gauss=True), img)
