@@ -99,6 +99,33 @@ __host__ __device__ __forceinline__ int xcdChunkTile(int i, int n, int chunk = k
 
 // One device body, two kernels: name_kernel runs one pass of one pair on a (gx, gy) grid; name_batch_kernel runs the
 // same pass for many pairs, workgroup i taking work[i] = (parameter block << kWorkShift) | tile.
+// The parameter blocks and work list of a batch are read-only for a whole launch. Read through the constant address
+// space, the pointers inside a block are known to address global memory, so the body's accesses compile to global (not
+// flat) loads and stores; flat accesses would also count against the LDS wait counter. (The host compilation pass only
+// type-checks kernel bodies and has no such address space.)
+#if defined(__HIP_DEVICE_COMPILE__)
+#define FGR_CONST_SPACE __attribute__((address_space(4)))
+#else
+#define FGR_CONST_SPACE
+#endif
+
+#define FGR_PASS_SINGLE(name, Params, body, threads)                                                              \
+    __global__ __launch_bounds__(threads) void name##_kernel(Params P) {                                          \
+        const int l = xcdChunkTile(linearBlock(), (int)(gridDim.x * gridDim.y));                                 \
+        body(P, l % (int)gridDim.x, l / (int)gridDim.x);                                                          \
+    }
+
+#define FGR_PASS_BATCH(name, Params, body, threads)                                                               \
+    __global__ __launch_bounds__(threads) void name##_batch_kernel(const Params* __restrict__ P,                  \
+                                                                   const uint32_t* __restrict__ work) {          \
+        const uint32_t w = ((const FGR_CONST_SPACE uint32_t*)work)[blockIdx.x];                                   \
+        const Params p = ((const FGR_CONST_SPACE Params*)P)[w >> kWorkShift];                                     \
+        const int l = (int)(w & kWorkMask);                                                                       \
+        body(p, l % p.gx, l / p.gx);                                                                              \
+    }
+
+// both kernels of a pass (spelled out rather than through the two macros above, so that a body naming a template with
+// several arguments passes through one macro expansion only)
 #define FGR_PASS(name, Params, body, threads)                                                                     \
     __global__ __launch_bounds__(threads) void name##_kernel(Params P) {                                          \
         const int l = xcdChunkTile(linearBlock(), (int)(gridDim.x * gridDim.y));                                 \
@@ -106,8 +133,8 @@ __host__ __device__ __forceinline__ int xcdChunkTile(int i, int n, int chunk = k
     }                                                                                                             \
     __global__ __launch_bounds__(threads) void name##_batch_kernel(const Params* __restrict__ P,                  \
                                                                    const uint32_t* __restrict__ work) {          \
-        const uint32_t w = work[blockIdx.x];                                                                      \
-        const Params p = P[w >> kWorkShift];                                                                      \
+        const uint32_t w = ((const FGR_CONST_SPACE uint32_t*)work)[blockIdx.x];                                   \
+        const Params p = ((const FGR_CONST_SPACE Params*)P)[w >> kWorkShift];                                     \
         const int l = (int)(w & kWorkMask);                                                                       \
         body(p, l % p.gx, l / p.gx);                                                                              \
     }

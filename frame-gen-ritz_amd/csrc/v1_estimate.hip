@@ -499,11 +499,17 @@ __device__ __forceinline__ bool sampleRun4(const float* img, int w, int h, int q
 // with the sum in float64 (cv2.blur's CV_32F path), - smoothBonus for the bilinear candidate, - hudBonus for zero,
 // strict < (first wins).
 //
-// 32 x 32 flow pixels per workgroup (4 per thread). The candidates are taken one at a time (unrolled at compile time):
-// the errors of the tile and its apron go to LDS (runs of 4 pixels of one 8x8 block share one vector for candidates
-// 1-5, see sampleRun4; the zero candidate is a texel difference: weights 1, 0, 0, 0), then the horizontal 5-sums
-// (float64), then each pixel's vertical sum and the running minimum in registers.
+// 32 x 32 flow pixels per workgroup. The candidates are taken one at a time: the errors of the tile and its apron go to
+// LDS in runs of 4 pixels of one row (inside one 8x8 block, since runs start at multiples of 4), then the horizontal
+// 5-sums (float64, 4 per task), then each pixel's vertical sum (a thread owns a column segment) and the running minimum
+// in registers. Work a run shares:
+//   * candidates 1-5 (a block vector): one vector for the run, one set of bilinear weights (sampleRun4); a run whose
+//     vector equals its own-block vector (candidate 1) reuses those errors -- a smooth field skips most of the work;
+//   * candidate 0 (the bilinear field): the run's pixels share the four block vectors they interpolate (their sample
+//     positions (q + 1/2) / 8 - 1/2 have one integer part within a run of 4 starting at a multiple of 4);
+//   * candidate 6 (zero): a texel difference (weights 1, 0, 0, 0).
 constexpr int kPfTile = 32, kPfRW = 40, kPfRH = kPfTile + 4, kPfRuns = (kPfRW / 4) * kPfRH;
+constexpr int kPfThreads = 512, kPfBatchThreads = 256;
 
 // the block offset of candidate k = 1..5 (own, above, below, left, right; pixfield.comp:29-33)
 __device__ __forceinline__ int2 candOffset(int k) {
@@ -512,7 +518,10 @@ __device__ __forceinline__ int2 candOffset(int k) {
     return make_int2(dx, dy);
 }
 
-__device__ __forceinline__ void pixErrRun(const PixParams& P, int k, int gx0, int qy, float* e) {
+// the errors of the run of 4 pixels (gx0 .. gx0 + 3, row qy) under candidate k. own / ownV: the run's errors and
+// vector under candidate 1, kept from k = 1 for runs inside the frame (ownOk).
+__device__ __forceinline__ void pixErrRun(const PixParams& P, int k, int gx0, int qy, float* e, float* own, float2& ownV,
+                                          bool& ownOk) {
     if (k == 6) {   // zero vector: sampleL at an integer position is the texel (weights 1, 0, 0, 0)
 #pragma unroll
         for (int i = 0; i < 4; i++) {
@@ -521,15 +530,51 @@ __device__ __forceinline__ void pixErrRun(const PixParams& P, int k, int gx0, in
         }
         return;
     }
-    if (k > 0 && gx0 >= 0 && gx0 + 3 <= P.w - 1) {
+    const bool inside = gx0 >= 0 && gx0 + 3 <= P.w - 1;
+    if (k == 0 && inside) {   // pixCand(0, ..) of each pixel with the four block vectors loaded once
+        const float sx0 = ((float)gx0 + 0.5f) / 8.0f - 0.5f, sy = ((float)qy + 0.5f) / 8.0f - 0.5f;
+        const float flx = floorf(sx0), fly = floorf(sy);
+        const float fy = sy - fly;
+        const int ix = (int)flx, iy = (int)fly;
+        const float2 f00 = fieldAt(P.bf, ix, iy, P.bw, P.bh), f10 = fieldAt(P.bf, ix + 1, iy, P.bw, P.bh);
+        const float2 f01 = fieldAt(P.bf, ix, iy + 1, P.bw, P.bh), f11 = fieldAt(P.bf, ix + 1, iy + 1, P.bw, P.bh);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int qx = gx0 + i;
+            const float fx = (((float)qx + 0.5f) / 8.0f - 0.5f) - flx;
+            const float2 v = mix2(mix2(f00, f10, fx), mix2(f01, f11, fx), fy);
+            e[i] = fabsf(sampleL(P.la, (float)qx - v.x, (float)qy - v.y, P.w, P.h) -
+                         sampleL(P.lc, (float)qx + v.x, (float)qy + v.y, P.w, P.h));
+        }
+        return;
+    }
+    if (k > 0 && inside) {
         const int2 o = candOffset(k);
         const float2 v = fieldAt(P.bf, gx0 / kBlock + o.x, qy / kBlock + o.y, P.bw, P.bh);   // one block, one vector
+        if (k > 1 && ownOk && v.x == ownV.x && v.y == ownV.y) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) e[i] = own[i];
+            return;
+        }
         float sa[4], sc[4];
         if (sampleRun4(P.la, P.w, P.h, gx0, qy, v, -1.0f, sa) && sampleRun4(P.lc, P.w, P.h, gx0, qy, v, 1.0f, sc)) {
 #pragma unroll
             for (int i = 0; i < 4; i++) e[i] = fabsf(sa[i] - sc[i]);
-            return;
+        } else {
+#pragma unroll 1
+            for (int i = 0; i < 4; i++) {
+                const int qx = gx0 + i;
+                e[i] = fabsf(sampleL(P.la, (float)qx - v.x, (float)qy - v.y, P.w, P.h) -
+                             sampleL(P.lc, (float)qx + v.x, (float)qy + v.y, P.w, P.h));
+            }
         }
+        if (k == 1) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) own[i] = e[i];
+            ownV = v;
+            ownOk = true;
+        }
+        return;
     }
 #pragma unroll 1
     for (int i = 0; i < 4; i++) {
@@ -540,40 +585,82 @@ __device__ __forceinline__ void pixErrRun(const PixParams& P, int k, int gx0, in
     }
 }
 
+// NT = 512 (the per-pair kernel: few workgroups, so latency decides): one run and 2 pixels a thread. NT = 256 (the
+// batched kernel: many workgroups, so occupancy decides): up to 2 runs and 4 pixels a thread, fewer registers.
+template <int NT>
 __device__ __forceinline__ void pixfieldBody(const PixParams& P, int gbx, int gby) {
-    __shared__ float errs[kPfRH * kPfRW];
-    __shared__ double hsum[kPfRH * kPfTile];
+    static_assert(NT == 256 || NT == 512, "two layouts");
+    constexpr int kRows = kPfTile * kPfTile / NT;      // pixels a thread: column cx, rows r0 .. r0 + kRows - 1
+    __shared__ __align__(16) float errs[kPfRH * kPfRW];
+    __shared__ __align__(16) double hsum[kPfRH * kPfTile];
     const int tid = threadIdx.x;
     const int x0 = gbx * kPfTile, y0 = gby * kPfTile;
-    const int cx = tid & 31, cy = tid >> 5;
-    float best[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    int bk[4] = {0, 0, 0, 0};
+    const int cx = tid & 31, r0 = (tid >> 5) * kRows;
+    float best[kRows];
+    int bk[kRows];
+#pragma unroll
+    for (int j = 0; j < kRows; j++) {
+        best[j] = 0.0f;
+        bk[j] = 0;
+    }
+    // candidate 1's errors of this thread's runs (r = tid and tid + NT), kept for candidates 2-5
+    float own0[4] = {0.0f, 0.0f, 0.0f, 0.0f}, own1[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    float2 ownV0 = make_float2(0.0f, 0.0f), ownV1 = make_float2(0.0f, 0.0f);
+    bool ownOk0 = false, ownOk1 = false;
 #pragma unroll 1
     for (int k = 0; k < 7; k++) {
-        for (int r = tid; r < kPfRuns; r += 256) {
+#pragma unroll 1
+        for (int r = tid; r < kPfRuns; r += NT) {
+            const bool s1 = r >= NT;   // one code path for both runs; the kept values are selected, not indexed
+            float own[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) own[i] = s1 ? own1[i] : own0[i];
+            float2 ownV = s1 ? ownV1 : ownV0;
+            bool ownOk = s1 ? ownOk1 : ownOk0;
             const int ry = r / (kPfRW / 4), rc = r - ry * (kPfRW / 4);
             float e[4];
-            pixErrRun(P, k, x0 - 4 + 4 * rc, clampi(y0 - 2 + ry, 0, P.h - 1), e);
+            pixErrRun(P, k, x0 - 4 + 4 * rc, clampi(y0 - 2 + ry, 0, P.h - 1), e, own, ownV, ownOk);
+            *(float4*)&errs[ry * kPfRW + 4 * rc] = make_float4(e[0], e[1], e[2], e[3]);
+            if (k == 1) {
 #pragma unroll
-            for (int i = 0; i < 4; i++) errs[ry * kPfRW + 4 * rc + i] = e[i];
+                for (int i = 0; i < 4; i++) {
+                    own0[i] = s1 ? own0[i] : own[i];
+                    own1[i] = s1 ? own[i] : own1[i];
+                }
+                ownV0 = s1 ? ownV0 : ownV;
+                ownV1 = s1 ? ownV : ownV1;
+                ownOk0 = s1 ? ownOk0 : ownOk;
+                ownOk1 = s1 ? ownOk : ownOk1;
+            }
         }
         __syncthreads();
-        // horizontal 5-sums for output columns c = 0..31 (region columns c + 2 .. c + 6)
-        for (int i = tid; i < kPfRH * kPfTile; i += 256) {
-            const int ry = i >> 5, c = i & 31;
-            const float* e = &errs[ry * kPfRW + c + 2];
-            double hs = 0.0;
+        // horizontal 5-sums, 4 output columns a task: region columns c0 + 2 .. c0 + 10 of one row
+        for (int t = tid; t < kPfRH * (kPfTile / 4); t += NT) {
+            const int hy = t >> 3, c0 = (t & 7) * 4;
+            const float2* e2 = (const float2*)&errs[hy * kPfRW + c0 + 2];
+            const float2 p0 = e2[0], p1 = e2[1], p2 = e2[2], p3 = e2[3];
+            const float e[8] = {p0.x, p0.y, p1.x, p1.y, p2.x, p2.y, p3.x, p3.y};
+            double h[4];
 #pragma unroll
-            for (int x = 0; x < 5; x++) hs = hs + (double)e[x];
-            hsum[i] = hs;
+            for (int i = 0; i < 4; i++) {
+                double hs = (double)e[i];
+#pragma unroll
+                for (int x = 1; x < 5; x++) hs = hs + (double)e[i + x];
+                h[i] = hs;
+            }
+            double2* h2 = (double2*)&hsum[hy * kPfTile + c0];
+            h2[0] = make_double2(h[0], h[1]);
+            h2[1] = make_double2(h[2], h[3]);
         }
         __syncthreads();
+        double hv[kRows + 4];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int ry = cy + 8 * j;
-            double vs = 0.0;
+        for (int y = 0; y < kRows + 4; y++) hv[y] = hsum[(r0 + y) * kPfTile + cx];
 #pragma unroll
-            for (int y = 0; y < 5; y++) vs = vs + hsum[(ry + y) * kPfTile + cx];
+        for (int j = 0; j < kRows; j++) {
+            double vs = hv[j];
+#pragma unroll
+            for (int y = 1; y < 5; y++) vs = vs + hv[j + y];
             float c = (float)(vs * (1.0 / 25.0));
             if (k == 0) c = c - P.smooth;
             if (k == 6) c = c - P.zero;
@@ -583,10 +670,22 @@ __device__ __forceinline__ void pixfieldBody(const PixParams& P, int gbx, int gb
             }
         }
     }
+    const int qx = x0 + cx;
+    if (qx >= P.w) return;
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int qx = x0 + cx, qy = y0 + cy + 8 * j;
-        if (qx < P.w && qy < P.h) P.pf[qy * P.w + qx] = pixCand(bk[j], qx, qy, P.bf, P.bw, P.bh);
+    for (int j = 0; j < kRows; j++) {
+        const int py = y0 + r0 + j;
+        if (py >= P.h) break;
+        const int k = bk[j];
+        float2 v;
+        if (k == 0) {
+            v = pixCand(0, qx, py, P.bf, P.bw, P.bh);
+        } else {
+            const int2 o = candOffset(k);
+            v = fieldAt(P.bf, qx / kBlock + o.x, py / kBlock + o.y, P.bw, P.bh);
+            if (k == 6) v = make_float2(0.0f, 0.0f);
+        }
+        P.pf[py * P.w + qx] = v;
     }
 }
 
@@ -603,7 +702,8 @@ FGR_PASS(search6, SearchParams, searchBody<6>, 256)
 FGR_PASS(search12, SearchParams, searchBody<12>, 256)
 FGR_PASS(median3, Median3Params, median3Body, 256)
 FGR_PASS(subpix, SubpixParams, subpixBody, 64)
-FGR_PASS(pixfield, PixParams, pixfieldBody, 256)
+FGR_PASS_SINGLE(pixfield, PixParams, pixfieldBody<kPfThreads>, kPfThreads)
+FGR_PASS_BATCH(pixfield, PixParams, pixfieldBody<kPfBatchThreads>, kPfBatchThreads)
 
 // ---------------------------------------------------------------------------------------------------------- launches
 
@@ -621,7 +721,7 @@ hipError_t launchSingle(int pass, const void* params, int gx, int gy, hipStream_
     case kPassSearch12: hipLaunchKernelGGL(search12_kernel, g, dim3(256), 0, s, *(const SearchParams*)params); break;
     case kPassMedian3: hipLaunchKernelGGL(median3_kernel, g, dim3(256), 0, s, *(const Median3Params*)params); break;
     case kPassSubpix: hipLaunchKernelGGL(subpix_kernel, g, dim3(64), 0, s, *(const SubpixParams*)params); break;
-    case kPassPixfield: hipLaunchKernelGGL(pixfield_kernel, g, dim3(256), 0, s, *(const PixParams*)params); break;
+    case kPassPixfield: hipLaunchKernelGGL(pixfield_kernel, g, dim3(kPfThreads), 0, s, *(const PixParams*)params); break;
     default: return launchSynthSingle(pass, params, gx, gy, s);
     }
     return hipGetLastError();
@@ -640,7 +740,7 @@ hipError_t launchBatch(int pass, const void* params, const uint32_t* work, int n
     case kPassSearch12: hipLaunchKernelGGL(search12_batch_kernel, g, dim3(256), 0, s, (const SearchParams*)params, work); break;
     case kPassMedian3: hipLaunchKernelGGL(median3_batch_kernel, g, dim3(256), 0, s, (const Median3Params*)params, work); break;
     case kPassSubpix: hipLaunchKernelGGL(subpix_batch_kernel, g, dim3(64), 0, s, (const SubpixParams*)params, work); break;
-    case kPassPixfield: hipLaunchKernelGGL(pixfield_batch_kernel, g, dim3(256), 0, s, (const PixParams*)params, work); break;
+    case kPassPixfield: hipLaunchKernelGGL(pixfield_batch_kernel, g, dim3(kPfBatchThreads), 0, s, (const PixParams*)params, work); break;
     default: return launchSynthBatch(pass, params, work, n, s);
     }
     return hipGetLastError();
