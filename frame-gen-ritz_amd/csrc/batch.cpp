@@ -24,6 +24,7 @@ using namespace fgr;
 
 namespace {
 thread_local const char* gBatchError = nullptr;
+const char* kBErrUi = "static-UI protection needs the per-frame observes of a handle: batches run with uiProtection Off";
 
 struct Launch {
     int pass = 0;
@@ -176,6 +177,7 @@ extern "C" FgrBatch* fgr_batch_create(const FgrBatchInfo* info, const FgrBatchPa
     gBatchError = nullptr;
     if (!info || !pairs) { gBatchError = kBErrNull; return nullptr; }
     if (n < 1 || n > 1024) { gBatchError = kBErrCount; return nullptr; }
+    if (info->settings.uiProtection != FGR_UI_OFF) { gBatchError = kBErrUi; return nullptr; }
     for (uint32_t p = 0; p < n; p++) {
         if (pairs[p].multiplier < 2 || pairs[p].multiplier > 16) { gBatchError = kBErrMult; return nullptr; }
         if (!pairs[p].prev.ptr || !pairs[p].curr.ptr || !pairs[p].prev.pitch || !pairs[p].curr.pitch) {

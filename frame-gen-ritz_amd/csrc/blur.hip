@@ -172,7 +172,7 @@ bool FgrInterpolator::enqueueBlur(hipStream_t s, const FgrFrame& prev, const Fgr
             sub.chroma = base + plane;
             sub.chromaPitch = (uint64_t)((W + 1) / 2) * cbpp;
         }
-        planSynth(plan, prev, curr, sub, t, k0++);
+        planSynth(plan, prev, curr, sub, t, k0++, false);   // (a protected pair: the patch follows the average)
         bp.src[k] = (const uint8_t*)sub.ptr;
         bp.srcC[k] = (const uint8_t*)sub.chroma;
         bp.pitch[k] = (int64_t)sub.pitch;
@@ -183,6 +183,10 @@ bool FgrInterpolator::enqueueBlur(hipStream_t s, const FgrFrame& prev, const Fgr
     hipLaunchKernelGGL(blur_average_kernel, dim3((W + 15) / 16, (H + 15) / 16), dim3(256), 0, s, bp);
     if (hipGetLastError() != hipSuccess) return false;
     stamp(s, "blur average");
+    if (estUi) {   // framegen.cpp:3622-3628: curr's unchanged UI pixels over the blurred frame
+        if (launchUi(kUiPatch, uiPatchParams(curr, out), s) != hipSuccess) return false;
+        stamp(s, "ui_patch");
+    }
     if (k0 > synthK) {
         lastSlot = (k0 - 1) % kSynthBanks;
         synthK = k0;

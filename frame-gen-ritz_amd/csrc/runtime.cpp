@@ -31,6 +31,9 @@ const char* kErrNull = "null InitInfo";
 const char* kErrWhite = "linearWhiteNits must be in (0, 10000]";
 const char* kErrYuv = "invalid yuvMatrix";
 const char* kErrOdd = "Nv12 and P010 frames need an even width and height (4:2:0 chroma of whole 2x2 blocks)";
+const char* kErrUiMode = "only UiProtection Off and CrosshairV2 are built in this library";
+const char* kErrUiFormat = "CrosshairV2 is built for Rgba8 frames only";
+const char* kErrUiFill = "uiEstimateFill is not built in this library";
 
 bool in(float v, float lo, float hi) { return std::isfinite(v) && v >= lo && v <= hi; }
 
@@ -58,6 +61,23 @@ extern "C" void fgr_default_settings(FgrSettings* s) {
     s->flowScale = 2;
     s->pyramidMinSide = 48;
     s->seedNet = 0;
+    // framegen.h:225-305
+    s->uiProtection = FGR_UI_OFF;
+    s->uiHistory = FGR_UI_FRAMES;
+    s->uiBoxHeightFrac = 0.025f;
+    s->uiBoxWidth = 0;
+    s->uiBoxHeight = 0;
+    s->uiCrosshairArea = 0.03f;
+    s->uiBoxOffsetX = 0;
+    s->uiBoxOffsetY = 0;
+    s->uiStillFrames = 8;
+    s->uiStillThreshold = 3.5f;
+    s->uiInpaintReach = 12;
+    s->uiMaxStillFraction = 0.75f;
+    s->uiContrastThreshold = 20.0f;
+    s->uiLeaveFrames = 4;
+    s->uiReentryFrames = 2;
+    s->uiEstimateFill = 0;
 }
 
 extern "C" int fgr_validate(const FgrSettings* s) {
@@ -70,7 +90,15 @@ extern "C" int fgr_validate(const FgrSettings* s) {
            s->searchRadiusTop >= 1 && s->searchRadiusTop <= 12 && s->subpixel <= 1 &&
            (s->flowScale == 1 || s->flowScale == 2 || s->flowScale == 4) && s->pyramidMinSide >= 16 &&
            s->pyramidMinSide <= 256 && s->motionVersion <= 3 && s->blurSamples <= FGR_BLUR_MAX_SAMPLES &&
-           s->blurWeights <= 1 && s->seedNet <= 1;
+           s->blurWeights <= 1 && s->seedNet <= 1 && s->uiProtection <= FGR_UI_CROSSHAIR_V2 &&
+           in(s->uiCrosshairArea, 0.0f, 1.0f) && s->uiCrosshairArea > 0.0f && in(s->uiBoxHeightFrac, 0.0f, 1.0f) &&
+           s->uiBoxWidth <= 16384 && s->uiBoxHeight <= 16384 && s->uiBoxOffsetX >= -16384 && s->uiBoxOffsetX <= 16384 &&
+           s->uiBoxOffsetY >= -16384 && s->uiBoxOffsetY <= 16384 && s->uiStillFrames >= 1 && s->uiStillFrames <= 255 &&
+           in(s->uiStillThreshold, 0.0f, 255.0f) && s->uiInpaintReach >= 1 && s->uiInpaintReach <= 16 &&
+           in(s->uiMaxStillFraction, 0.0f, 1.0f) && s->uiMaxStillFraction > 0.0f &&
+           in(s->uiContrastThreshold, 0.0f, 255.0f) && s->uiLeaveFrames >= 1 && s->uiLeaveFrames <= 255 &&
+           s->uiReentryFrames <= 255 && s->uiEstimateFill <= 1 && s->uiHistory >= FGR_UI_FRAMES &&
+           s->uiHistory <= FGR_UI_FRAMES_MAX;
 }
 
 static bool formatBuilt(uint32_t f) {
@@ -82,7 +110,9 @@ extern "C" const char* fgr_settings_error_for_format(const FgrSettings* s, uint3
     // framegen.cpp:1702-1707
     if (!fgr_validate(s)) return kErrSettings;
     if (s->motionVersion != FGR_MOTION_V1) return kErrVersion;
-    (void)format;   // no Format rule for V1
+    if (s->uiProtection == FGR_UI_CROSSHAIR || s->uiProtection == FGR_UI_WHOLESCREEN) return kErrUiMode;
+    if (s->uiProtection == FGR_UI_CROSSHAIR_V2 && format != FGR_FORMAT_RGBA8) return kErrUiFormat;
+    if (s->uiProtection == FGR_UI_CROSSHAIR_V2 && s->uiEstimateFill) return kErrUiFill;
     return nullptr;
 }
 
@@ -131,6 +161,7 @@ void FgrInterpolator::freeBuffers() {
     field = nullptr;
     u = nullptr;
     seedAct = nullptr;
+    uiFree();
     W = H = 0;
     haveEst = false;
 }
@@ -204,7 +235,7 @@ extern "C" FgrInterpolator* fgr_create(const FgrInitInfo* info) {
     h->set = info->settings;
     h->device = info->device;
     h->format = (int)info->format;
-    if (!h->formatInit() || !h->rebuild(info->width, info->height)) {
+    if (!h->formatInit() || !h->rebuild(info->width, info->height) || !h->uiEnsure()) {
         gInitError = kErrAlloc;
         fgr_destroy(h);
         return nullptr;
@@ -232,7 +263,12 @@ extern "C" int fgr_resize(FgrInterpolator* h, uint32_t width, uint32_t height) {
         h->freeBuffers();   // framegen.h:457-458: on false the object holds no size until a later resize succeeds
         return 0;
     }
-    return h->rebuild(width, height) ? 1 : 0;
+    if (!h->rebuild(width, height)) return 0;
+    if (!h->uiEnsure()) {   // (as a failed allocation of the size-dependent buffers: no size held)
+        h->freeBuffers();
+        return 0;
+    }
+    return 1;
 }
 
 extern "C" const char* fgr_settings_error(const FgrInterpolator* h, const FgrSettings* s) {
@@ -249,8 +285,16 @@ extern "C" int fgr_set_settings(FgrInterpolator* h, const FgrSettings* s) {
     // framegen.cpp:1724-1740: runtime fields only; structural ones need reconfigure
     if (!h || fgr_settings_error(h, s) || s->flowScale != h->set.flowScale || s->pyramidMinSide != h->set.pyramidMinSide)
         return 0;
-    if (s->motionVersion != h->set.motionVersion) h->haveEst = false;
+    // UI protection may allocate here (never frees); a change of the UI mode or its box drops the estimate
+    const FgrSettings old = h->set;
+    const UiRect oldBox = h->uiBox(), oldWork = h->uiWork();
     h->set = *s;
+    if (!h->uiEnsure()) {
+        h->set = old;
+        return 0;
+    }
+    h->uiAdopted(old);
+    if (s->motionVersion != old.motionVersion || h->uiChanged(old, oldBox, oldWork)) h->haveEst = false;
     return 1;
 }
 
@@ -259,12 +303,23 @@ extern "C" int fgr_reconfigure(FgrInterpolator* h, const FgrSettings* s) {
     if (!h || fgr_settings_error(h, s)) return 0;
     const bool structural = s->flowScale != h->set.flowScale || s->pyramidMinSide != h->set.pyramidMinSide;
     const FgrSettings old = h->set;
+    const UiRect oldBox = h->uiBox(), oldWork = h->uiWork();
     h->set = *s;
     if (structural && h->W) {
-        if (!h->rebuild(h->W, h->H)) return 0;   // (buffers gone: the object holds no size, as a failed resize)
+        // (buffers gone: the object holds no size, as a failed resize)
+        if (!h->rebuild(h->W, h->H)) return 0;
+        if (!h->uiEnsure()) {
+            h->freeBuffers();
+            return 0;
+        }
         return 1;
     }
-    if (s->motionVersion != old.motionVersion) h->haveEst = false;
+    if (!h->uiEnsure()) {
+        h->set = old;
+        return 0;
+    }
+    h->uiAdopted(old);
+    if (s->motionVersion != old.motionVersion || h->uiChanged(old, oldBox, oldWork)) h->haveEst = false;
     return 1;
 }
 
@@ -275,7 +330,7 @@ extern "C" void fgr_get_settings(const FgrInterpolator* h, FgrSettings* out) {
 }
 
 extern "C" void fgr_begin_frame(FgrInterpolator* h) {
-    if (h) h->ringEst = h->ringSynth = 0;
+    if (h) h->ringEst = h->ringSynth = h->uiSets = 0;
 }
 
 // ------------------------------------------------------------------------------------------------------ profiling
@@ -527,7 +582,9 @@ void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& pr
 
 bool FgrInterpolator::runPlan(hipStream_t s, const std::vector<PlanOp>& plan) {
     for (const PlanOp& op : plan) {
-        if (launchSingle(op.pass, op.params.data(), op.gx, op.gy, s) != hipSuccess) return false;
+        const hipError_t e = op.pass == kPassUiPatch ? launchUi(kUiPatch, *(const UiParams*)op.params.data(), s)
+                                                     : launchSingle(op.pass, op.params.data(), op.gx, op.gy, s);
+        if (e != hipSuccess) return false;
         if (op.stamp) stamp(s, op.stamp);
     }
     return true;
@@ -536,7 +593,8 @@ bool FgrInterpolator::runPlan(hipStream_t s, const std::vector<PlanOp>& plan) {
 bool FgrInterpolator::enqueueEstimate(hipStream_t s, const FgrFrame& prev, const FgrFrame& curr) {
     std::vector<PlanOp> plan;
     planEstimate(plan, prev, curr);
-    beginProfile(s);
+    if (!profPending) beginProfile(s);   // else an observe opened this profile
+    profPending = false;
     if (!runPlan(s, plan)) return false;
     blockFieldOff = set.subpixel ? fieldSub : lv[0].fM;
     return true;
@@ -597,10 +655,14 @@ void FgrInterpolator::synthParams(SynthParams& p, const FgrFrame& prev, const Fg
     for (int i = 0; i < 8; i++) p.yuv[i] = yuv[i];
     p.cw = ((int)W + 1) / 2;
     p.ch = ((int)H + 1) / 2;
+    if (estUi) {   // CrosshairV2 pair: the kept masks of prev and curr (ui_reject.glsl)
+        p.uiA = uiMaskRef(estEnt[0]);
+        p.uiC = uiMaskRef(estEnt[1]);
+    }
 }
 
 void FgrInterpolator::planSynth(std::vector<PlanOp>& plan, const FgrFrame& prev, const FgrFrame& curr,
-                                const FgrFrame& out, float t, int k) const {
+                                const FgrFrame& out, float t, int k, bool uiPatch) const {
     const int bank = k % kSynthBanks;
     SynthParams sp;
     synthParams(sp, prev, curr, t, bank, true);
@@ -617,6 +679,10 @@ void FgrInterpolator::planSynth(std::vector<PlanOp>& plan, const FgrFrame& prev,
         if (rows == 16) op.pass = S == 1 ? kPassSynthRgba8S1T16 : (S == 2 ? kPassSynthRgba8S2T16 : kPassSynthRgba8S4T16);
         else if (rows == 64) op.pass = kPassSynthRgba8S2T64;
         else op.pass = S == 1 ? kPassSynthRgba8S1 : (S == 2 ? kPassSynthRgba8S2 : kPassSynthRgba8S4);
+        if (estUi) {   // the tap-rejecting variant (32-row tiles)
+            rows = 32;
+            op.pass = S == 1 ? kPassSynthRgba8UiS1 : (S == 2 ? kPassSynthRgba8UiS2 : kPassSynthRgba8UiS4);
+        }
     } else {
         op.pass = set.flowScale == 1 ? kPassSynthFmtS1 : (set.flowScale == 2 ? kPassSynthFmtS2 : kPassSynthFmtS4);
     }
@@ -650,6 +716,14 @@ void FgrInterpolator::planSynth(std::vector<PlanOp>& plan, const FgrFrame& prev,
     of.stamp = "fallback";
     of.set(fp);
     plan.push_back(of);
+    if (estUi && uiPatch) {   // framegen.cpp:3521-3527: curr's unchanged UI pixels over the finished frame
+        PlanOp pp;
+        pp.pass = kPassUiPatch;
+        pp.stage = 502 + 10 * k;
+        pp.stamp = "ui_patch";
+        pp.set(uiPatchParams(curr, out));
+        plan.push_back(pp);
+    }
 }
 
 bool FgrInterpolator::enqueueSynth(hipStream_t s, const FgrFrame& prev, const FgrFrame& curr, const FgrFrame& out, float t) {
@@ -674,8 +748,42 @@ int FgrInterpolator::bytesPerPixel() const {
 extern "C" int fgr_record_estimate(FgrInterpolator* h, void* stream, FgrFrame prev, FgrFrame curr) {
     if (!h || !h->W || !h->frameOk(prev) || !h->frameOk(curr)) return 0;
     if (h->ringEst >= FGR_SET_RING) return 0;   // framegen.cpp:2873-2874
-    if (!h->enqueueEstimate((hipStream_t)stream, prev, curr)) return 0;
+    // framegen.cpp:2875-2911: a protected estimate needs both frames kept; a host that never observes gets them observed
+    // here (prev if unseen, curr unless it is the newest observed); with explicit observes an unseen frame means the
+    // pair runs unprotected
+    hipStream_t s = (hipStream_t)stream;
+    bool ui = h->uiActive();
+    if (ui) {
+        int needObs = 0;
+        if (!h->uiExplicit) {
+            needObs += h->uiFind(prev.ptr) < 0;
+            needObs += h->uiLast < 0 || h->uiE[h->uiLast].key != curr.ptr;
+        } else if (h->uiFind(prev.ptr) < 0 || h->uiFind(curr.ptr) < 0) {
+            ui = false;
+        }
+        if (ui && h->uiSets + (uint32_t)needObs > FGR_UI_SETS) return 0;
+    }
+    int ep = -1, ec = -1;
+    if (ui) {
+        if (!h->uiExplicit) {
+            if (h->profiling && !h->profPending) h->beginProfile(s);
+            h->profPending = h->profiling;
+            if (h->uiFind(prev.ptr) < 0 && !h->uiObserveRecord(s, prev)) return 0;
+            if ((h->uiLast < 0 || h->uiE[h->uiLast].key != curr.ptr) && !h->uiObserveRecord(s, curr)) return 0;
+        }
+        ep = h->uiFind(prev.ptr);
+        ec = h->uiFind(curr.ptr);
+        if (ep < 0 || ec < 0 || ep == ec) return 0;
+    }
+    if (!h->enqueueEstimate(s, prev, curr)) return 0;
     h->ringEst++;
+    h->estUi = ui;
+    if (ui) {
+        h->estEnt[0] = ep;
+        h->estEnt[1] = ec;
+        h->estSeq[0] = h->uiE[ep].seq;
+        h->estSeq[1] = h->uiE[ec].seq;
+    }
     h->haveEst = true;
     h->estPrev = prev.ptr;
     h->estCurr = curr.ptr;
@@ -693,8 +801,13 @@ extern "C" int fgr_record_synth(FgrInterpolator* h, void* stream, FgrFrame prev,
         !h->frameOk(out))
         return 0;
     if (h->ringSynth >= FGR_SET_RING) return 0;
+    if (h->estUi) {   // framegen.cpp:3473-3478: the entries the estimate used must still hold those frames
+        if (h->uiE[h->estEnt[0]].seq != h->estSeq[0] || h->uiE[h->estEnt[1]].seq != h->estSeq[1]) return 0;
+        if (h->uiSets + 2 > FGR_UI_SETS) return 0;
+    }
     if (!h->enqueueSynth((hipStream_t)stream, prev, curr, out, t)) return 0;
     h->ringSynth++;
+    if (h->estUi) h->uiSets++;
     return 1;
 }
 
@@ -703,6 +816,7 @@ extern "C" int fgr_record(FgrInterpolator* h, void* stream, FgrFrame prev, FgrFr
     if (!h || !h->W || !(t > 0.0f && t < 1.0f)) return 0;
     if (h->ringEst >= FGR_SET_RING || h->ringSynth >= FGR_SET_RING) return 0;
     if (!h->frameOk(out)) return 0;
+    if (h->uiActive() && h->uiSets + 2 + 2 > FGR_UI_SETS) return 0;   // framegen.cpp:3641: 2 implicit observes + the patch
     return fgr_record_estimate(h, stream, prev, curr) && fgr_record_synth(h, stream, prev, curr, out, t);
 }
 
@@ -815,14 +929,19 @@ extern "C" int fgr_record_synth_blur(FgrInterpolator* h, void* stream, FgrFrame 
         !h->frameOk(out))
         return 0;
     if (h->ringSynth >= FGR_SET_RING) return 0;
+    if (h->estUi) {   // framegen.cpp:3543-3548
+        if (h->uiE[h->estEnt[0]].seq != h->estSeq[0] || h->uiE[h->estEnt[1]].seq != h->estSeq[1]) return 0;
+        if (h->uiSets + 2 > FGR_UI_SETS) return 0;
+    }
     hipStream_t s = (hipStream_t)stream;
     if (!blur) {
         if (!h->copyFrame(s, t1 <= 0.0f ? prev : curr, out)) return 0;
-        h->ringSynth++;
-        return 1;
+        if (h->estUi && launchUi(kUiPatch, h->uiPatchParams(curr, out), s) != hipSuccess) return 0;
+    } else if (!h->enqueueBlur(s, prev, curr, out, t0, t1)) {
+        return 0;
     }
-    if (!h->enqueueBlur(s, prev, curr, out, t0, t1)) return 0;
     h->ringSynth++;
+    if (h->estUi) h->uiSets++;
     return 1;
 }
 

@@ -25,6 +25,23 @@ struct PlanOp {
     }
 };
 
+// CrosshairV2 (framegen.cpp:893-950): a rectangle of the frame, and one kept observed frame: its key (the frame
+// pointer), its observe sequence number (LRU; 0 = holds nothing), its mask over the work rectangle it was observed with.
+struct UiRect {
+    int x = 0, y = 0, w = 0, h = 0;
+    bool operator==(const UiRect& o) const { return x == o.x && y == o.y && w == o.w && h == o.h; }
+};
+
+struct UiEntry {
+    const void* key = nullptr;
+    uint64_t seq = 0;
+    bool active = false;   // one of the uiHistory kept frames (an inactive entry keeps its memory)
+    uint8_t* mask = nullptr;
+    size_t cap = 0;
+    UiRect wr;
+    int pitch = 0;
+};
+
 struct FgrInterpolator {
     static constexpr int kMaxQ = 64;
 
@@ -65,8 +82,37 @@ struct FgrInterpolator {
     bool allocSeedBuffers();
     void planSeeds(std::vector<PlanOp>& plan) const;
 
+    // static-UI protection, CrosshairV2 (ui_host.cpp)
+    UiEntry uiE[FGR_UI_FRAMES_MAX];
+    uint64_t uiSeq = 0;
+    int uiLast = -1;                  // the entry observed last
+    bool uiExplicit = false;          // fgr_record_observe has been called: estimates no longer observe implicitly
+    bool uiBoxHist = false;           // the state describes the previous real frame
+    UiRect uiPrevBox;
+    uint32_t* uiState = nullptr;      // 4 words per work-rectangle pixel
+    size_t uiStateCap = 0;
+    uint32_t uiSets = 0;              // UI passes since fgr_begin_frame (FGR_UI_SETS)
+    bool estUi = false;               // the latest estimate ran protected: its entries and their sequence numbers
+    int estEnt[2] = {-1, -1};
+    uint64_t estSeq[2] = {0, 0};
+    bool uiActive() const { return W && set.uiProtection == FGR_UI_CROSSHAIR_V2 && uiState; }
+    UiRect uiBox() const;
+    UiRect uiWork() const;
+    bool uiEnsure();
+    void uiFree();
+    void uiAdopted(const FgrSettings& old);
+    bool uiChanged(const FgrSettings& old, const UiRect& oldBox, const UiRect& oldWork) const;
+    int uiFind(const void* key) const;
+    int uiPick(const void* key) const;
+    void uiForget(const void* key);
+    bool uiObserveRecord(hipStream_t s, const FgrFrame& f);
+    bool uiTrackRecord(hipStream_t s, const FgrFrame& f);
+    fgr::UiMaskRef uiMaskRef(int e) const;
+    fgr::UiParams uiPatchParams(const FgrFrame& curr, const FgrFrame& out) const;
+
     // profiling
     bool profiling = false;
+    bool profPending = false;         // an observe opened a profile that the next estimate continues
     std::vector<hipEvent_t> events;
     std::vector<const char*> qnames;
     int qcount = 0;
@@ -81,7 +127,7 @@ struct FgrInterpolator {
     void beginProfile(hipStream_t s);
     void planEstimate(std::vector<PlanOp>& plan, const FgrFrame& prev, const FgrFrame& curr) const;
     void planSynth(std::vector<PlanOp>& plan, const FgrFrame& prev, const FgrFrame& curr, const FgrFrame& out, float t,
-                   int k) const;
+                   int k, bool uiPatch = true) const;
     bool runPlan(hipStream_t s, const std::vector<PlanOp>& plan);
     bool enqueueEstimate(hipStream_t s, const FgrFrame& prev, const FgrFrame& curr);
     bool enqueueSynth(hipStream_t s, const FgrFrame& prev, const FgrFrame& curr, const FgrFrame& out, float t);

@@ -1,0 +1,149 @@
+// CrosshairV2 static-UI protection for Rgba8 frames (gpu/shaders/ui_v2_update.comp, ui_v2_mask.comp, ui_patch.comp;
+// the host contract is gpu/framegen.h:575-693).
+//
+// Per observed frame, over the work rectangle (the box grown by 1 px, clamped to the frame): the detection state
+// machine of every pixel (one thread a pixel, in place: a thread touches only its own state words; the neighbours of the
+// contrast test are read from the frame), then the frame's mask. After a protected synth, the pixels of mask byte 255
+// (UI, unchanged in the pair) take curr's colour. The box is small (27 x 27 at 1080p by default), so these are latency
+// kernels of one or a few workgroups.
+//
+// State words per pixel (4 a pixel, the fourth unused): w0 = previous original (r | g << 8 | b << 16) | count << 24,
+// w1 = last confirmed colour | flags << 24 (bit 0 in the mask, bit 1 has a stored colour, bit 2 still in this pair),
+// w2 = consecutive changed frames in the mask | consecutive stored-colour matches out of it << 8.
+#include <hip/hip_runtime.h>
+#include "v1_device.hpp"
+#include "v1_launch.hpp"
+
+namespace fgr {
+
+__device__ __forceinline__ uint32_t rgbAt(const UiParams& P, int x, int y) {   // the pixel's three 8-bit levels
+    x = clampi(x, 0, P.W - 1);
+    y = clampi(y, 0, P.H - 1);
+    return *(const uint32_t*)(P.f.p + (int64_t)y * P.f.pitch + 4 * (int64_t)x) & 0x00FFFFFFu;
+}
+
+__device__ __forceinline__ int chan(uint32_t v, int c) { return (int)((v >> (8 * c)) & 255u); }
+
+__device__ __forceinline__ int maxAbs(uint32_t a, uint32_t b) {
+    return max(abs(chan(a, 0) - chan(b, 0)), max(abs(chan(a, 1) - chan(b, 1)), abs(chan(a, 2) - chan(b, 2))));
+}
+
+// max channel |v - median7x7| around (x, y) (frame-clamped window): the median of 49 integer levels is the smallest m
+// with at least 25 of them <= m, found per channel by bisection over 0..255 with the 49 values in registers.
+__device__ __forceinline__ int contrastAt(const UiParams& P, int x, int y, uint32_t v) {
+    uint32_t w[49];
+#pragma unroll
+    for (int j = 0; j < 7; j++)
+#pragma unroll
+        for (int i = 0; i < 7; i++) w[j * 7 + i] = rgbAt(P, x + i - 3, y + j - 3);
+    int d = 0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        int lo = 0, hi = 255;
+        for (int it = 0; it < 8; it++) {
+            const int mid = (lo + hi) >> 1;
+            int cnt = 0;
+#pragma unroll
+            for (int j = 0; j < 49; j++) cnt += chan(w[j], c) <= mid ? 1 : 0;
+            if (cnt >= 25) hi = mid;
+            else lo = mid + 1;
+        }
+        d = max(d, abs(chan(v, c) - lo));
+    }
+    return d;
+}
+
+__global__ __launch_bounds__(256) void ui_update_kernel(UiParams P) {
+    const int px = (int)(blockIdx.x * 16 + (threadIdx.x & 15)), py = (int)(blockIdx.y * 16 + (threadIdx.x >> 4));
+    if (px >= P.ww || py >= P.wh) return;
+    const int x = P.wx + px, y = P.wy + py;
+    const uint32_t now = rgbAt(P, x, y);
+    uint32_t* st = P.state + 4 * ((int64_t)py * P.ww + px);
+    const bool inBox = x >= P.bx && y >= P.by && x < P.bx + P.bw && y < P.by + P.bh;
+    if (!inBox) {   // the margin ring: the pair gate only
+        const uint32_t ring = (!P.reset && (float)maxAbs(now, st[0] & 0x00FFFFFFu) <= P.thr) ? 4u : 0u;
+        st[0] = now;
+        st[1] = ring << 24;
+        st[2] = 0u;
+        return;
+    }
+    uint32_t cnt = 0u, flags = 0u, leave = 0u, match = 0u, stored = 0u;
+    if (!P.reset) {
+        const uint32_t w0 = st[0], w1 = st[1], w2 = st[2];
+        flags = w1 >> 24;
+        stored = w1 & 0x00FFFFFFu;
+        leave = w2 & 255u;
+        match = (w2 >> 8) & 255u;
+        const bool still = (float)maxAbs(now, w0 & 0x00FFFFFFu) <= P.thr;
+        bool inMask = (flags & 1u) != 0u, hasStored = (flags & 2u) != 0u;
+        const bool matchS = !inMask && hasStored && P.reentry > 0 && (float)maxAbs(now, stored) <= P.thr;
+        bool contrasty = P.cthr <= 0.0f;
+        if (!inMask && !contrasty && (still || matchS)) contrasty = (float)contrastAt(P, x, y, now) >= P.cthr;
+        if (inMask) {
+            leave = still ? 0u : min(leave + 1u, 255u);
+            if (still) stored = now;
+            if ((int)leave >= P.leave) {
+                inMask = false;
+                leave = 0u;
+                match = 0u;
+            }
+            cnt = 0u;
+        } else {
+            cnt = (still && contrasty) ? min((w0 >> 24) + 1u, 255u) : 0u;
+            match = (matchS && contrasty) ? min(match + 1u, 255u) : 0u;
+            if ((int)cnt >= P.still || (P.reentry > 0 && hasStored && (int)match >= P.reentry)) {
+                inMask = true;
+                hasStored = true;
+                stored = now;
+                leave = 0u;
+                match = 0u;
+            }
+        }
+        flags = (inMask ? 1u : 0u) | (hasStored ? 2u : 0u) | (still ? 4u : 0u);
+    }
+    st[0] = now | (cnt << 24);
+    st[1] = stored | (flags << 24);
+    st[2] = leave | (match << 8);
+}
+
+// ui_v2_mask.comp: 255 = within 1 px (8-neighbourhood) of a mask pixel and still in this pair, 64 = within 1 px but it
+// changed in this pair, 0 = not UI
+__global__ __launch_bounds__(256) void ui_mask_kernel(UiParams P) {
+    const int px = (int)(blockIdx.x * 16 + (threadIdx.x & 15)), py = (int)(blockIdx.y * 16 + (threadIdx.x >> 4));
+    if (px >= P.ww || py >= P.wh) return;
+    bool near = false;
+    for (int j = -1; j <= 1; j++)
+        for (int i = -1; i <= 1; i++) {
+            const int qx = px + i, qy = py + j;
+            if (qx >= 0 && qy >= 0 && qx < P.ww && qy < P.wh)
+                near = near || ((P.state[4 * ((int64_t)qy * P.ww + qx) + 1] >> 24) & 1u) != 0u;
+        }
+    const uint32_t f = P.state[4 * ((int64_t)py * P.ww + px) + 1] >> 24;
+    P.mask[(int64_t)py * P.mpitch + px] = near ? ((f & 4u) ? 255 : 64) : 0;
+}
+
+// ui_patch.comp with `binary`: a pixel of mask byte 255 takes curr's colour; the generated alpha stays
+__global__ __launch_bounds__(256) void ui_patch_kernel(UiParams P) {
+    const int px = (int)(blockIdx.x * 16 + (threadIdx.x & 15)), py = (int)(blockIdx.y * 16 + (threadIdx.x >> 4));
+    if (px >= P.ww || py >= P.wh) return;
+    if (P.mask[(int64_t)py * P.mpitch + px] != 255) return;
+    const int x = P.wx + px, y = P.wy + py;
+    uint32_t* o = (uint32_t*)(P.out + (int64_t)y * P.opitch + 4 * (int64_t)x);
+    const uint32_t c = *(const uint32_t*)(P.f.p + (int64_t)y * P.f.pitch + 4 * (int64_t)x);
+    *o = (c & 0x00FFFFFFu) | (*o & 0xFF000000u);
+}
+
+static inline unsigned cdivu(int a, int b) { return (unsigned)((a + b - 1) / b); }
+
+hipError_t launchUi(int pass, const UiParams& p, hipStream_t s) {
+    const dim3 g(cdivu(p.ww, 16), cdivu(p.wh, 16));
+    switch (pass) {
+    case kUiUpdate: hipLaunchKernelGGL(ui_update_kernel, g, dim3(256), 0, s, p); break;
+    case kUiMask: hipLaunchKernelGGL(ui_mask_kernel, g, dim3(256), 0, s, p); break;
+    case kUiPatch: hipLaunchKernelGGL(ui_patch_kernel, g, dim3(256), 0, s, p); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace fgr

@@ -103,6 +103,13 @@ struct LookupParams {
     float s;
 };
 
+// A kept CrosshairV2 mask of one frame: bytes over the rectangle (x, y, w, h) of the frame, `pitch` bytes a row;
+// non-zero = a UI pixel. w = 0: no mask (nothing is ever hit).
+struct UiMaskRef {
+    const uint8_t* m;
+    int x, y, w, h, pitch;
+};
+
 struct SynthParams {
     int gx;             // blocks per grid row (the batched kernels map a linear block index to (bx, by))
     FrameRef a, c;
@@ -126,6 +133,21 @@ struct SynthParams {
     const int* lut;           // RgbaF16: metric level of every half value, in 1/16 level (65536 entries)
     float yuv[8];             // Nv12 / P010: ys, rcr, bcb, gcb, gcr (format.glsl YU_YS, YU_RCR, YU_BCB, YU_GCB, YU_GCR)
     int cw, ch;               // Nv12 / P010: chroma plane size
+    UiMaskRef uiA, uiC;       // CrosshairV2 pair: the masks of prev and curr for the tap rejection (the UI kernels)
+};
+
+// CrosshairV2 detection of one observed / tracked frame (ui.hip), the patch of a generated frame
+struct UiParams {
+    FrameRef f;               // the observed frame (update, mask), or curr (patch)
+    uint8_t* out; int64_t opitch;   // patch: the generated frame
+    int W, H;
+    int bx, by, bw, bh;       // the box: detection
+    int wx, wy, ww, wh;       // the work rectangle: state layout and mask
+    int reset;                // the state describes no previous frame
+    float thr, cthr;          // uiStillThreshold, uiContrastThreshold (8-bit levels)
+    int still, leave, reentry;   // uiStillFrames, uiLeaveFrames, uiReentryFrames
+    uint32_t* state;          // 4 words per work-rectangle pixel
+    uint8_t* mask; int mpitch;   // the entry's mask (ww x wh bytes)
 };
 
 struct FallbackParams {

@@ -108,23 +108,49 @@ __device__ __forceinline__ int emaxRgba8(uint32_t wa, uint32_t wc) {
     return max(d0, max(d1, d2));
 }
 
+// CrosshairV2 tap rejection (ui_reject.glsl): does the bilinear footprint of a tap at the 1/32 map (X, Y) -- texels
+// X >> 5 .. + 1 and Y >> 5 .. + 1, those of weight 0 dropped -- touch a UI pixel of the frame's kept mask?
+__device__ __forceinline__ bool uiHit(const UiMaskRef& m, int X, int Y) {
+    const int lx = X >> 5, ly = Y >> 5;
+    const int x0 = max(lx - m.x, 0), y0 = max(ly - m.y, 0);
+    const int x1 = min(lx + ((X & 31) != 0) - m.x, m.w - 1), y1 = min(ly + ((Y & 31) != 0) - m.y, m.h - 1);
+    bool hit = false;
+    for (int y = y0; y <= y1; y++)
+        for (int x = x0; x <= x1; x++) hit = hit || m.m[y * m.pitch + x] != 0;
+    return hit;
+}
+
+// a tap on its own frame's UI (the other not) takes the other frame's tap; both or neither: unchanged
+template <class T>
+__device__ __forceinline__ void uiReject(const SynthParams& P, int Xa, int Ya, int Xc, int Yc, T& wa, T& wc) {
+    const bool ha = uiHit(P.uiA, Xa, Ya), hc = uiHit(P.uiC, Xc, Yc);
+    if (ha != hc) {
+        if (ha) wa = wc;
+        else wc = wa;
+    }
+}
+
 // The warp pair at full-res pixel (x, y) from r, the interpolated field before the x S of halfVec (hv = S r): a at
 // p - 2t hv, c at p + 2(1-t) hv, each the float32 product then the difference (motion_v1.glsl:120-122), then x 32 for
 // the map. Scaling by a power of two commutes with rounding, so 32 (p - ka (S r)) = 32 p - (32 S ka) r exactly; kA32 and
-// kC32 are those folded constants.
+// kC32 are those folded constants. UI: with the CrosshairV2 tap rejection (motion_v1.glsl:124).
+template <bool UI = false>
 __device__ __forceinline__ void warpPair(const SynthParams& P, float kA32, float kC32, int x, int y, float2 r, Rgb& wa, Rgb& wc) {
     const float x32 = (float)(32 * x), y32 = (float)(32 * y);
-    wa = warpRgba8(P.a.p, (uint32_t)P.a.pitch, P.W, P.H, x32 - kA32 * r.x, y32 - kA32 * r.y);
-    wc = warpRgba8(P.c.p, (uint32_t)P.c.pitch, P.W, P.H, x32 + kC32 * r.x, y32 + kC32 * r.y);
+    const float xa = x32 - kA32 * r.x, ya = y32 - kA32 * r.y, xc = x32 + kC32 * r.x, yc = y32 + kC32 * r.y;
+    wa = warpRgba8(P.a.p, (uint32_t)P.a.pitch, P.W, P.H, xa, ya);
+    wc = warpRgba8(P.c.p, (uint32_t)P.c.pitch, P.W, P.H, xc, yc);
+    if (UI) uiReject(P, (int)rintf(xa), (int)rintf(ya), (int)rintf(xc), (int)rintf(yc), wa, wc);
 }
 
 // the same pair from the plain per-pixel halfVec (debug pass)
 __device__ __forceinline__ void warpPairRgba8(const SynthParams& P, int x, int y, float2 hv, uint32_t& wa, uint32_t& wc) {
     const float ka = 2.0f * P.wC, kc = 2.0f * P.wA;
-    const Rgb a = warpRgba8(P.a.p, (uint32_t)P.a.pitch, P.W, P.H, 32.0f * ((float)x - ka * hv.x), 32.0f * ((float)y - ka * hv.y));
-    const Rgb c = warpRgba8(P.c.p, (uint32_t)P.c.pitch, P.W, P.H, 32.0f * ((float)x + kc * hv.x), 32.0f * ((float)y + kc * hv.y));
-    wa = packRgb(a);
-    wc = packRgb(c);
+    const float xa = 32.0f * ((float)x - ka * hv.x), ya = 32.0f * ((float)y - ka * hv.y);
+    const float xc = 32.0f * ((float)x + kc * hv.x), yc = 32.0f * ((float)y + kc * hv.y);
+    wa = packRgb(warpRgba8(P.a.p, (uint32_t)P.a.pitch, P.W, P.H, xa, ya));
+    wc = packRgb(warpRgba8(P.c.p, (uint32_t)P.c.pitch, P.W, P.H, xc, yc));
+    if (P.uiA.w > 0 || P.uiC.w > 0) uiReject(P, (int)rintf(xa), (int)rintf(ya), (int)rintf(xc), (int)rintf(yc), wa, wc);
 }
 
 // resolve.comp:73-82 (interp.py:241-246): m = 0.5 (wa + wc) at t = 0.5, else (1-t) wa + t wc; o = m (1 - u) + near u;
@@ -162,7 +188,7 @@ __device__ __forceinline__ uint32_t blendRgba8(const SynthParams& P, uint32_t wa
 // halfVec() motion_v1.glsl:73-84 is split in its two stages: the horizontal mixes of each field row at every full-res
 // column of the tile (row[j][x] = mix(F[j][i0(x)], F[j][i1(x)], fx(x))), then per pixel the vertical mix of two of
 // those rows; the operations and their order are the shader's, so the result is the same float.
-template <int S, int RT>
+template <int S, int RT, bool UI = false>
 __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, int tileY, int nTilesX) {
     constexpr int FTW = SynthTile<S, RT>::FTW, FTH = SynthTile<S, RT>::FTH;
     constexpr int TH = RT, EH = TileRows<RT>::EH, NAPRON = TileRows<RT>::NAPRON, RK = RT / 16;
@@ -212,11 +238,15 @@ __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, 
         if (__all(same && ma.ok && mc.ok)) {
             warpRun4(P.a.p, (uint32_t)P.a.pitch, P.H, ma, oa);
             warpRun4(P.c.p, (uint32_t)P.c.pitch, P.H, mc, oc);
+            if (UI) {   // the run's maps are X0 + 32 i at one Y
+#pragma unroll
+                for (int i = 0; i < 4; i++) uiReject(P, ma.X0 + 32 * i, ma.Y, mc.X0 + 32 * i, mc.Y, oa[i], oc[i]);
+            }
         } else {
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 Rgb a, c;
-                warpPair(P, kA32, kC32, xs[i], y, r[i], a, c);
+                warpPair<UI>(P, kA32, kC32, xs[i], y, r[i], a, c);
                 oa[i] = packRgb(a);
                 oc[i] = packRgb(c);
             }
@@ -238,7 +268,7 @@ __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, 
         }
         const int x = med3i(x0 - AP + ex, 0, P.W - 1), y = med3i(y0 - AP + ey, 0, P.H - 1);
         Rgb a, c;
-        warpPair(P, kA32, kC32, x, y, fieldAtRegion<S>(P, hrow, ty0, ex, y), a, c);
+        warpPair<UI>(P, kA32, kC32, x, y, fieldAtRegion<S>(P, hrow, ty0, ex, y), a, c);
         es[ey * EW + ex] = emaxRgb(a, c);
     }
     __syncthreads();
@@ -291,9 +321,9 @@ __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, 
     if (tid == 0 && cnt) atomicAdd(P.count + ((tileY * nTilesX + tileX) & (kSlots - 1)), cnt);
 }
 
-template <int S, int RT>
+template <int S, int RT, bool UI = false>
 __device__ __forceinline__ void synthRgba8Body(const SynthParams& P, int bx, int by) {
-    synthRgba8Tile<S, RT>(P, bx, by, P.gx);
+    synthRgba8Tile<S, RT, UI>(P, bx, by, P.gx);
 }
 
 // lookup.comp:10-19 (interp.py:194-205): the per-pixel field at time t, g <- pf(q - s g) twice (LOOKUP_ITERS = 2)
@@ -371,6 +401,10 @@ FGR_PASS(synth_rgba8_s1_t16, SynthParams, FGR_SYNTH_BODY(1, 16), 256)
 FGR_PASS(synth_rgba8_s2_t16, SynthParams, FGR_SYNTH_BODY(2, 16), 256)
 FGR_PASS(synth_rgba8_s4_t16, SynthParams, FGR_SYNTH_BODY(4, 16), 256)
 FGR_PASS(synth_rgba8_s2_t64, SynthParams, FGR_SYNTH_BODY(2, 64), 256)
+#define FGR_SYNTH_UI_BODY(S) synthRgba8Body<S, 32, true>
+FGR_PASS_SINGLE(synth_rgba8_ui_s1, SynthParams, FGR_SYNTH_UI_BODY(1), 256)
+FGR_PASS_SINGLE(synth_rgba8_ui_s2, SynthParams, FGR_SYNTH_UI_BODY(2), 256)
+FGR_PASS_SINGLE(synth_rgba8_ui_s4, SynthParams, FGR_SYNTH_UI_BODY(4), 256)
 FGR_PASS(fallback, FallbackParams, fallbackBody, 256)
 
 static inline int cdiv(int a, int b) { return (a + b - 1) / b; }
@@ -395,6 +429,9 @@ hipError_t launchSynthSingle(int pass, const void* params, int gx, int gy, hipSt
     case kPassSynthRgba8S2T16: hipLaunchKernelGGL(synth_rgba8_s2_t16_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
     case kPassSynthRgba8S4T16: hipLaunchKernelGGL(synth_rgba8_s4_t16_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
     case kPassSynthRgba8S2T64: hipLaunchKernelGGL(synth_rgba8_s2_t64_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
+    case kPassSynthRgba8UiS1: hipLaunchKernelGGL(synth_rgba8_ui_s1_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
+    case kPassSynthRgba8UiS2: hipLaunchKernelGGL(synth_rgba8_ui_s2_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
+    case kPassSynthRgba8UiS4: hipLaunchKernelGGL(synth_rgba8_ui_s4_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
     case kPassFallback: hipLaunchKernelGGL(fallback_kernel, g, dim3(256), 0, s, *(const FallbackParams*)params); break;
     default: return launchFmtSingle(pass, params, gx, gy, s);
     }

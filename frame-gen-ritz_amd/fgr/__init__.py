@@ -22,6 +22,8 @@ FORMAT_RGBA8, FORMAT_RGB10, FORMAT_RGBAF16, FORMAT_NV12, FORMAT_P010 = 0, 1, 2, 
 TRANSFER_ENCODED, TRANSFER_LINEAR = 0, 1
 YUV_BT601, YUV_BT709, YUV_BT2020 = 0, 1, 2
 MIN_SIZE, SET_RING, BLUR_MAX_SAMPLES = 16, 32, 16
+UI_OFF, UI_CROSSHAIR, UI_WHOLESCREEN, UI_CROSSHAIR_V2 = 0, 1, 2, 3
+UI_FRAMES, UI_FRAMES_MAX, UI_SETS = 3, 65, 64
 
 
 class Settings(ctypes.Structure):
@@ -32,7 +34,16 @@ class Settings(ctypes.Structure):
                 ("searchRadiusTop", ctypes.c_uint32), ("subpixel", ctypes.c_uint32),
                 ("motionVersion", ctypes.c_uint32), ("blurSamples", ctypes.c_uint32),
                 ("blurWeights", ctypes.c_uint32), ("flowScale", ctypes.c_uint32),
-                ("pyramidMinSide", ctypes.c_uint32), ("seedNet", ctypes.c_uint32)]
+                ("pyramidMinSide", ctypes.c_uint32), ("seedNet", ctypes.c_uint32),
+                # static-UI protection (framegen.h:225-305)
+                ("uiProtection", ctypes.c_uint32), ("uiHistory", ctypes.c_uint32),
+                ("uiBoxHeightFrac", ctypes.c_float), ("uiBoxWidth", ctypes.c_uint32),
+                ("uiBoxHeight", ctypes.c_uint32), ("uiCrosshairArea", ctypes.c_float),
+                ("uiBoxOffsetX", ctypes.c_int32), ("uiBoxOffsetY", ctypes.c_int32),
+                ("uiStillFrames", ctypes.c_uint32), ("uiStillThreshold", ctypes.c_float),
+                ("uiInpaintReach", ctypes.c_uint32), ("uiMaxStillFraction", ctypes.c_float),
+                ("uiContrastThreshold", ctypes.c_float), ("uiLeaveFrames", ctypes.c_uint32),
+                ("uiReentryFrames", ctypes.c_uint32), ("uiEstimateFill", ctypes.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -163,6 +174,15 @@ _sig = {
     "fgr_batch_debug_counters": (ctypes.c_int, [_H, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
                                                 ctypes.c_void_p]),
     "fgr_batch_debug_internal": (ctypes.c_int, [_H, ctypes.c_uint32, ctypes.POINTER(DebugInternal)]),
+    "fgr_record_observe": (ctypes.c_int, [_H, ctypes.c_void_p, Frame]),
+    "fgr_record_track": (ctypes.c_int, [_H, ctypes.c_void_p, Frame]),
+    "fgr_release_unused_ui": (None, [_H]),
+    "fgr_box_size": (None, [ctypes.POINTER(Settings), ctypes.c_uint32, ctypes.c_uint32,
+                            ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
+    "fgr_box_rect": (None, [ctypes.POINTER(Settings), ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int32)]),
+    "fgr_box_margin": (ctypes.c_uint32, [ctypes.POINTER(Settings)]),
+    "fgr_work_rect": (None, [ctypes.POINTER(Settings), ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int32)]),
+    "fgr_debug_ui_mask": (ctypes.c_int, [_H, ctypes.c_void_p, Frame, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32)]),
 }
 for _name, (_res, _args) in _sig.items():
     _f = getattr(lib, _name)
@@ -182,6 +202,31 @@ def default_settings(**kw):
             raise KeyError(k)
         setattr(s, k, v)
     return s
+
+
+def box_size(s, w, h):
+    """framegen.h:484 boxSize(): (boxW, boxH)"""
+    bw, bh = ctypes.c_uint32(), ctypes.c_uint32()
+    lib.fgr_box_size(ctypes.byref(s), w, h, ctypes.byref(bw), ctypes.byref(bh))
+    return bw.value, bh.value
+
+
+def box_rect(s, w, h):
+    """framegen.h:487 boxRect(): (x, y, w, h)"""
+    r = (ctypes.c_int32 * 4)()
+    lib.fgr_box_rect(ctypes.byref(s), w, h, r)
+    return tuple(r)
+
+
+def box_margin(s):
+    return lib.fgr_box_margin(ctypes.byref(s))
+
+
+def work_rect(s, w, h):
+    """framegen.h:494 workRect(): (x, y, w, h)"""
+    r = (ctypes.c_int32 * 4)()
+    lib.fgr_work_rect(ctypes.byref(s), w, h, r)
+    return tuple(r)
 
 
 def validate(s):
@@ -240,6 +285,7 @@ class Interpolator:
         self._h = lib.fgr_create(ctypes.byref(info))
         if not self._h:
             raise RuntimeError(_s(lib.fgr_last_init_error()))
+        self._size = (width, height)
 
     def close(self):
         if self._h:
@@ -260,7 +306,10 @@ class Interpolator:
         return lib.fgr_format(self._h)
 
     def resize(self, w, h):
-        return bool(lib.fgr_resize(self._h, w, h))
+        ok = bool(lib.fgr_resize(self._h, w, h))
+        if ok:
+            self._size = (w, h)
+        return ok
 
     def settings_error(self, s):
         return _s(lib.fgr_settings_error(self._h, ctypes.byref(s)))
@@ -294,6 +343,35 @@ class Interpolator:
     def record_synth_blur(self, prev, curr, out, t0, t1, stream=None):
         return bool(lib.fgr_record_synth_blur(self._h, _stream(stream), _frame(prev), _frame(curr), _frame(out),
                                               t0, t1))
+
+    def record_observe(self, frame, stream=None):
+        """framegen.h:610 recordObserve(): once per new real frame (static-UI protection)"""
+        return bool(lib.fgr_record_observe(self._h, _stream(stream), _frame(frame)))
+
+    def record_track(self, frame, stream=None):
+        """framegen.h:686 recordTrack(): a real frame passed through without generating"""
+        return bool(lib.fgr_record_track(self._h, _stream(stream), _frame(frame)))
+
+    def release_unused_ui(self):
+        lib.fgr_release_unused_ui(self._h)
+
+    def debug_ui_mask(self, frame=None, stream=None):
+        """Test / debug only: (mask uint8 (h, w) CUDA tensor, (x, y, w, h)) of a kept frame, or the box state
+        (uint32 words (h, w, 4) as int32) with frame None; None if there is none."""
+        import torch
+        r = (ctypes.c_int32 * 4)()
+        if frame is None:
+            x, y, w, h = self.work_rect_now()
+            dst = torch.empty((h, w, 4), dtype=torch.int32, device="cuda")
+            ok = lib.fgr_debug_ui_mask(self._h, _stream(stream), Frame(), ctypes.c_void_p(dst.data_ptr()), r)
+        else:   # the entry's rectangle lies in the frame
+            dst = torch.empty(self._size[0] * self._size[1], dtype=torch.uint8, device="cuda")
+            ok = lib.fgr_debug_ui_mask(self._h, _stream(stream), _frame(frame), ctypes.c_void_p(dst.data_ptr()), r)
+            dst = dst[:r[2] * r[3]].view(r[3], r[2])
+        return (dst, tuple(r)) if ok else None
+
+    def work_rect_now(self):
+        return work_rect(self.settings(), *self._size)
 
     def set_profiling(self, on):
         return bool(lib.fgr_set_profiling(self._h, int(on)))

@@ -54,7 +54,31 @@ typedef struct FgrSettings {
     uint32_t flowScale;       /* structural: 1, 2, 4                                       (framegen.h:403) */
     uint32_t pyramidMinSide;  /* structural: [16, 256]                                     (framegen.h:408) */
     uint32_t seedNet;         /* 0 / 1 (no reference counterpart, see above)                             */
+    /* Static-UI protection (framegen.h:207-305). Built: Off and CrosshairV2 (Rgba8 frames); Crosshair and WholeScreen
+     * are valid values that fgr_settings_error refuses. */
+    uint32_t uiProtection;        /* FGR_UI_*                                                  (framegen.h:225) */
+    uint32_t uiHistory;           /* [FGR_UI_FRAMES, FGR_UI_FRAMES_MAX] kept observed frames   (framegen.h:234) */
+    float uiBoxHeightFrac;        /* [0, 1]: box side = round(H * frac), 0 = uiCrosshairArea   (framegen.h:250) */
+    uint32_t uiBoxWidth;          /* [0, 16384], 0 = from the fraction                         (framegen.h:253) */
+    uint32_t uiBoxHeight;         /* [0, 16384]                                                (framegen.h:253) */
+    float uiCrosshairArea;        /* (0, 1], legacy box area (used when uiBoxHeightFrac = 0)   (framegen.h:257) */
+    int32_t uiBoxOffsetX;         /* [-16384, 16384] from the frame centre                     (framegen.h:260) */
+    int32_t uiBoxOffsetY;         /* [-16384, 16384]                                           (framegen.h:260) */
+    uint32_t uiStillFrames;       /* [1, 255]                                                  (framegen.h:265) */
+    float uiStillThreshold;       /* [0, 255] 8-bit levels                                     (framegen.h:266) */
+    uint32_t uiInpaintReach;      /* [1, 16] (Crosshair / WholeScreen only, validated)         (framegen.h:270) */
+    float uiMaxStillFraction;     /* (0, 1] (WholeScreen only, validated)                      (framegen.h:274) */
+    float uiContrastThreshold;    /* [0, 255] (CrosshairV2)                                    (framegen.h:282) */
+    uint32_t uiLeaveFrames;       /* [1, 255] (CrosshairV2)                                    (framegen.h:286) */
+    uint32_t uiReentryFrames;     /* [0, 255] (CrosshairV2)                                    (framegen.h:291) */
+    uint32_t uiEstimateFill;      /* 0 / 1 (CrosshairV2)                                       (framegen.h:305) */
 } FgrSettings;
+
+/* framegen.h:15-24 UiProtection */
+enum { FGR_UI_OFF = 0, FGR_UI_CROSSHAIR = 1, FGR_UI_WHOLESCREEN = 2, FGR_UI_CROSSHAIR_V2 = 3 };
+#define FGR_UI_FRAMES 3        /* framegen.h:815 kUiFrames: observed frames kept at least */
+#define FGR_UI_FRAMES_MAX 65   /* framegen.h:816 kUiFramesMax */
+#define FGR_UI_SETS 64         /* framegen.h:813 kUiSets: UI-protection passes per fgr_begin_frame */
 
 /* framegen.h:411-441 InitInfo, with a HIP device index in place of the Vulkan handles. */
 typedef struct FgrInitInfo {
@@ -127,6 +151,26 @@ int fgr_record_synth(FgrInterpolator* h, void* stream, FgrFrame prev, FgrFrame c
 int fgr_record(FgrInterpolator* h, void* stream, FgrFrame prev, FgrFrame curr, FgrFrame out, float t);       /* :551 */
 int fgr_record_synth_blur(FgrInterpolator* h, void* stream, FgrFrame prev, FgrFrame curr, FgrFrame out,
                           float t0, float t1);                                                       /* :573 */
+
+/* ---- static-UI protection, CrosshairV2 (framegen.h:575-693) ----
+ * recordObserve: once per new real frame (in presentation order), after its pixels are final: advances the box's
+ * per-pixel state machine and keeps the frame's mask (the frame pointer is the key; uiHistory frames are kept, least
+ * recently observed evicted). recordTrack: a real frame passed through without generating: the state machine only. With
+ * protection on, an estimate of a pair whose frames are both kept runs protected: its synths reject warp taps that land
+ * on a frame's own UI pixels and paste curr's UI pixels that did not change in the pair. A host that never observes gets
+ * the implicit observes of recordEstimate (framegen.h:602-609). Off: both return 1 and enqueue nothing. */
+int fgr_record_observe(FgrInterpolator* h, void* stream, FgrFrame frame);      /* framegen.h:610 recordObserve() */
+int fgr_record_track(FgrInterpolator* h, void* stream, FgrFrame frame);        /* framegen.h:686 recordTrack() */
+void fgr_release_unused_ui(FgrInterpolator* h);                                /* framegen.h:693 releaseUnusedUi() */
+/* box geometry (host only, no validity check): framegen.h:481-494 */
+void fgr_box_size(const FgrSettings* s, uint32_t w, uint32_t h, uint32_t* boxW, uint32_t* boxH);   /* boxSize() */
+void fgr_box_rect(const FgrSettings* s, uint32_t w, uint32_t h, int32_t rect[4]);                  /* boxRect() {x, y, w, h} */
+uint32_t fgr_box_margin(const FgrSettings* s);                                                     /* boxMargin() */
+void fgr_work_rect(const FgrSettings* s, uint32_t w, uint32_t h, int32_t rect[4]);                 /* workRect() */
+/* Debug (this library's addition): the kept mask of `frame` (rect = its work rectangle {x, y, w, h}; rect[2] bytes a
+ * row: 255 = UI unchanged in its pair, 64 = UI that changed, 0 = not UI) copied to dst (device, rect[2] * rect[3] bytes),
+ * or the box state (4 uint32 words per work-rectangle pixel) when frame.ptr is null. Returns 0 if there is none. */
+int fgr_debug_ui_mask(FgrInterpolator* h, void* stream, FgrFrame frame, void* dst, int32_t rect[4]);
 
 /* ---- profiling (framegen.h:705-717): HIP events after every pass group; ticks are nanoseconds ---- */
 int fgr_set_profiling(FgrInterpolator* h, int on);

@@ -22,6 +22,8 @@ enum class BlurWeights : uint32_t { Even = 0, Gaussian = 1 };                   
 enum class Format : uint32_t { Rgba8 = 0, Rgb10 = 1, RgbaF16 = 2, Nv12 = 3, P010 = 4 };    // framegen.h:138
 enum class Transfer : uint32_t { Encoded = 0, Linear = 1 };                                // framegen.h:144
 enum class YuvMatrix : uint32_t { Bt601 = 0, Bt709 = 1, Bt2020 = 2 };                      // framegen.h:150
+// framegen.h:15-24 (Crosshair and WholeScreen are valid values that settingsError refuses: not built)
+enum class UiProtection : uint32_t { Off = 0, Crosshair = 1, WholeScreen = 2, CrosshairV2 = 3 };
 
 // framegen.h:154-159: one caller-owned frame (device memory; Nv12 / P010: the Y plane and the CbCr plane)
 struct Frame {
@@ -58,6 +60,9 @@ public:
     static constexpr uint32_t kSetRing = FGR_SET_RING;           // framegen.h:812
     static constexpr uint32_t kMinSize = FGR_MIN_SIZE;           // framegen.h:817
     static constexpr uint32_t kBlurMaxSamples = FGR_BLUR_MAX_SAMPLES;
+    static constexpr uint32_t kUiSets = FGR_UI_SETS;              // framegen.h:813
+    static constexpr uint32_t kUiFrames = FGR_UI_FRAMES;          // framegen.h:815
+    static constexpr uint32_t kUiFramesMax = FGR_UI_FRAMES_MAX;   // framegen.h:816
 
     Interpolator() = default;
     Interpolator(const Interpolator&) = delete;
@@ -114,6 +119,17 @@ public:
     bool recordSynthBlur(hipStream_t s, Frame prev, Frame curr, Frame out, float t0, float t1) {   // framegen.h:573
         return h_ && fgr_record_synth_blur(h_, s, prev.c(), curr.c(), out.c(), t0, t1);
     }
+
+    // static-UI protection (framegen.h:481-494, 575-693)
+    static void boxSize(const Settings& s, uint32_t w, uint32_t h, uint32_t* boxW, uint32_t* boxH) {
+        fgr_box_size(&s, w, h, boxW, boxH);
+    }
+    static void boxRect(const Settings& s, uint32_t w, uint32_t h, int32_t rect[4]) { fgr_box_rect(&s, w, h, rect); }
+    static uint32_t boxMargin(const Settings& s) { return fgr_box_margin(&s); }
+    static void workRect(const Settings& s, uint32_t w, uint32_t h, int32_t rect[4]) { fgr_work_rect(&s, w, h, rect); }
+    bool recordObserve(hipStream_t s, Frame frame) { return h_ && fgr_record_observe(h_, s, frame.c()); }   // :610
+    bool recordTrack(hipStream_t s, Frame frame) { return h_ && fgr_record_track(h_, s, frame.c()); }       // :686
+    void releaseUnusedUi() { fgr_release_unused_ui(h_); }                                                 // :693
 
     bool setProfiling(bool on) { return h_ && fgr_set_profiling(h_, on ? 1 : 0); }   // framegen.h:705
     bool readTimings(std::vector<const char*>& names, std::vector<uint64_t>& ticks) {   // framegen.h:717

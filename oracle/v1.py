@@ -536,11 +536,17 @@ def half_vec(field, W, H, S):
     return (mix(top, bot, fy) * f32(S)).astype(f32)
 
 
+def maps32(px, py):
+    """the warp's map in 1/32 px: 32 p rounded half even (int64)"""
+    X = np.rint((px * f32(32.0)).astype(f32)).astype(np.int64)
+    Y = np.rint((py * f32(32.0)).astype(f32)).astype(np.int64)
+    return X, Y
+
+
 def warp_rgba8(frame, px, py):
     """cv2.remap INTER_LINEAR / BORDER_REPLICATE on uint8 (interp.py:214): map rounded to 1/32 px (half even),
     integer weights (32 - fx)(32 - fy) .. fx fy, result (sum + 512) >> 10. int64 (H, W, 4)."""
-    X = np.rint((px * f32(32.0)).astype(f32)).astype(np.int64)
-    Y = np.rint((py * f32(32.0)).astype(f32)).astype(np.int64)
+    X, Y = maps32(px, py)
     ix, iy, fx, fy = X >> 5, Y >> 5, X & 31, Y & 31
     img = frame.astype(np.int64)
     p00, p01 = _fetch(img, ix, iy), _fetch(img, ix + 1, iy)
@@ -565,11 +571,13 @@ def box_int(img, r):
     return vs
 
 
-def synth(a, c, est, t, exact64=False, bgra=False):
+def synth(a, c, est, t, exact64=False, bgra=False, ui=None):
     """interp.py:219-246 synth() with the GPU's arithmetic (lookup.comp, error.comp, resolve.comp,
     fallback.comp). Returns (uint8 frame (H, W, 4), reason None / "cut" / "untrusted", dict of intermediates).
     exact64: the blend (trust ramp, time mix, resolve) in float64 -- the FP64 reference of the output; the
-    sampling geometry (vectors, 1/32 maps) and the decisions stay the pinned float32 ones."""
+    sampling geometry (vectors, 1/32 maps) and the decisions stay the pinned float32 ones.
+    ui: (maskA, rectA, maskC, rectC) of a CrosshairV2 pair: the warp taps are rejected as ui.reject does
+    (motion_v1.glsl:124, before the mismatch and the blend); the paste-back is ui.patch, after this."""
     s = est["settings"]
     H, W = a.shape[:2]
     S = s["flowScale"]
@@ -584,8 +592,13 @@ def synth(a, c, est, t, exact64=False, bgra=False):
     py, px = np.meshgrid(np.arange(H), np.arange(W), indexing="ij")
     pxf, pyf = px.astype(f32), py.astype(f32)
     ka, kc = f32(2.0) * wC, f32(2.0) * wA
-    wa = warp_rgba8(a, (pxf - (ka * hv[..., 0]).astype(f32)).astype(f32), (pyf - (ka * hv[..., 1]).astype(f32)).astype(f32))
-    wc = warp_rgba8(c, (pxf + (kc * hv[..., 0]).astype(f32)).astype(f32), (pyf + (kc * hv[..., 1]).astype(f32)).astype(f32))
+    pax, pay = (pxf - (ka * hv[..., 0]).astype(f32)).astype(f32), (pyf - (ka * hv[..., 1]).astype(f32)).astype(f32)
+    pcx, pcy = (pxf + (kc * hv[..., 0]).astype(f32)).astype(f32), (pyf + (kc * hv[..., 1]).astype(f32)).astype(f32)
+    wa = warp_rgba8(a, pax, pay)
+    wc = warp_rgba8(c, pcx, pcy)
+    if ui is not None:
+        from . import ui as uimod
+        wa, wc = uimod.reject(wa, wc, maps32(pax, pay), maps32(pcx, pcy), ui)
     e = np.abs(wa[..., :3] - wc[..., :3]).max(-1)
     S49 = box_int(e, 3)
     lo, hi = f32(s["trustLow"]), f32(s["trustHigh"])
@@ -643,9 +656,9 @@ def average_frames(frames, ws, hi=255, alpha_hi=255):
     return np.clip(np.floor((acc / wsum).astype(f32) + f32(0.5)), 0, hi)
 
 
-def synth_blur(a, c, est, t0, t1, n, gauss=False):
+def synth_blur(a, c, est, t0, t1, n, gauss=False, ui=None):
     """recordSynthBlur (framegen.cpp:3534-3634) with blurSamples = n >= 2, t0 < t1: the weighted mean of the
-    RGBA8 frames synth() gives at the sample times (prev / curr at t <= 0 / t >= 1), rounded once."""
+    RGBA8 frames synth() gives at the sample times (prev / curr at t <= 0 / t >= 1), rounded once. ui: as synth()."""
     ts, ws = blur_samples(t0, t1, n, gauss)
     frames = []
     for t in ts:
@@ -654,7 +667,7 @@ def synth_blur(a, c, est, t0, t1, n, gauss=False):
         elif t >= 1:
             frames.append(c)
         else:
-            frames.append(synth(a, c, est, float(t))[0])
+            frames.append(synth(a, c, est, float(t), ui=ui)[0])
     return average_frames(frames, ws).astype(np.uint8)
 
 
