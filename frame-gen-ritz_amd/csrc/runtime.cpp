@@ -33,7 +33,6 @@ const char* kErrYuv = "invalid yuvMatrix";
 const char* kErrOdd = "Nv12 and P010 frames need an even width and height (4:2:0 chroma of whole 2x2 blocks)";
 const char* kErrUiMode = "only UiProtection Off and CrosshairV2 are built in this library";
 const char* kErrUiFormat = "CrosshairV2 is built for Rgba8 frames only";
-const char* kErrUiFill = "uiEstimateFill is not built in this library";
 
 bool in(float v, float lo, float hi) { return std::isfinite(v) && v >= lo && v <= hi; }
 
@@ -112,7 +111,6 @@ extern "C" const char* fgr_settings_error_for_format(const FgrSettings* s, uint3
     if (s->motionVersion != FGR_MOTION_V1) return kErrVersion;
     if (s->uiProtection == FGR_UI_CROSSHAIR || s->uiProtection == FGR_UI_WHOLESCREEN) return kErrUiMode;
     if (s->uiProtection == FGR_UI_CROSSHAIR_V2 && format != FGR_FORMAT_RGBA8) return kErrUiFormat;
-    if (s->uiProtection == FGR_UI_CROSSHAIR_V2 && s->uiEstimateFill) return kErrUiFill;
     return nullptr;
 }
 
@@ -442,6 +440,34 @@ void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& pr
         op.set(lp);
         plan.push_back(op);
     }
+    // CrosshairV2 uiEstimateFill (framegen.cpp:2959-2963): the UI cells of both frames' level-0 luma, before the pyramid
+    if (estUi && st.uiEstimateFill) {
+        for (int k = 0; k < 2; k++) {
+            const UiEntry& E = uiE[estEnt[k]];
+            UiParams fp{};
+            fp.f = ref(k ? curr : prev);
+            fp.W = (int)W;
+            fp.H = (int)H;
+            fp.wx = E.wr.x;
+            fp.wy = E.wr.y;
+            fp.ww = E.wr.w;
+            fp.wh = E.wr.h;
+            fp.mask = E.mask;
+            fp.mpitch = E.pitch;
+            fp.luma = luma + (k ? lv[0].offC : lv[0].offA);
+            fp.S = (int)st.flowScale;
+            fp.fw = fw;
+            fp.fh = fh;
+            fp.bgra = (int)info.bgra;
+            fp.reach = 8;
+            PlanOp op;
+            op.pass = kPassUiFill;
+            op.stage = 1;
+            op.stamp = k ? "ui_fill" : nullptr;
+            op.set(fp);
+            plan.push_back(op);
+        }
+    }
     // 2. pyramid
     for (size_t i = 1; i < lv.size(); i++) {
         DownParams dp{};
@@ -583,7 +609,8 @@ void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& pr
 bool FgrInterpolator::runPlan(hipStream_t s, const std::vector<PlanOp>& plan) {
     for (const PlanOp& op : plan) {
         const hipError_t e = op.pass == kPassUiPatch ? launchUi(kUiPatch, *(const UiParams*)op.params.data(), s)
-                                                     : launchSingle(op.pass, op.params.data(), op.gx, op.gy, s);
+                             : op.pass == kPassUiFill ? launchUi(kUiFill, *(const UiParams*)op.params.data(), s)
+                                                      : launchSingle(op.pass, op.params.data(), op.gx, op.gy, s);
         if (e != hipSuccess) return false;
         if (op.stamp) stamp(s, op.stamp);
     }
@@ -775,15 +802,19 @@ extern "C" int fgr_record_estimate(FgrInterpolator* h, void* stream, FgrFrame pr
         ec = h->uiFind(curr.ptr);
         if (ep < 0 || ec < 0 || ep == ec) return 0;
     }
-    if (!h->enqueueEstimate(s, prev, curr)) return 0;
-    h->ringEst++;
-    h->estUi = ui;
+    h->estUi = ui;   // (read by the plan: the estimate fill)
     if (ui) {
         h->estEnt[0] = ep;
         h->estEnt[1] = ec;
         h->estSeq[0] = h->uiE[ep].seq;
         h->estSeq[1] = h->uiE[ec].seq;
     }
+    if (!h->enqueueEstimate(s, prev, curr)) {
+        h->estUi = false;
+        h->haveEst = false;
+        return 0;
+    }
+    h->ringEst++;
     h->haveEst = true;
     h->estPrev = prev.ptr;
     h->estCurr = curr.ptr;

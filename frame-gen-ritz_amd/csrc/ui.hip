@@ -133,9 +133,54 @@ __global__ __launch_bounds__(256) void ui_patch_kernel(UiParams P) {
     *o = (c & 0x00FFFFFFu) | (*o & 0xFF000000u);
 }
 
+// ui_v2_lumafill.comp (uiEstimateFill): each flow cell of the work rectangle that holds a UI pixel (mask byte != 0)
+// gets, in the estimate's level-0 luma only, the mean of the integer lumas (luma.comp:23-26) of the not-UI pixels of the
+// cell widened by S px a side per step up to `reach` px while it has none (float32 sum of integers: exact; one
+// division). Cells without UI pixels keep what the luma pass wrote. One thread a cell.
+__device__ __forceinline__ bool uiPx(const UiParams& P, int x, int y) {
+    const int rx = x - P.wx, ry = y - P.wy;
+    return rx >= 0 && ry >= 0 && rx < P.ww && ry < P.wh && P.mask[(int64_t)ry * P.mpitch + rx] != 0;
+}
+
+__global__ __launch_bounds__(256) void ui_fill_kernel(UiParams P) {
+    const int S = P.S;
+    const int cx0 = P.wx / S, cy0 = P.wy / S;
+    const int gx = (int)(blockIdx.x * 16 + (threadIdx.x & 15)), gy = (int)(blockIdx.y * 16 + (threadIdx.x >> 4));
+    const int ox = cx0 + gx, oy = cy0 + gy;
+    if (ox >= P.fw || oy >= P.fh || ox > (P.wx + P.ww - 1) / S || oy > (P.wy + P.wh - 1) / S) return;
+    const int px = ox * S, py = oy * S;
+    bool hasUi = false;
+    for (int j = 0; j < S; j++)
+        for (int i = 0; i < S; i++)
+            if (px + i < P.W && py + j < P.H) hasUi = hasUi || uiPx(P, px + i, py + j);
+    if (!hasUi) return;
+    for (int k = 0; k <= P.reach; k += S) {
+        float sum = 0.0f;
+        int n = 0;
+        for (int y = py - k; y < py + S + k; y++)
+            for (int x = px - k; x < px + S + k; x++) {
+                if (x < 0 || y < 0 || x >= P.W || y >= P.H || uiPx(P, x, y)) continue;
+                const uint32_t v = *(const uint32_t*)(P.f.p + (int64_t)y * P.f.pitch + 4 * (int64_t)x);
+                const uint32_t r = P.bgra ? (v >> 16) & 255u : v & 255u, g = (v >> 8) & 255u,
+                               b = P.bgra ? v & 255u : (v >> 16) & 255u;
+                sum += (float)((r * 4899u + g * 9617u + b * 1868u + 8192u) >> 14);
+                n++;
+            }
+        if (n > 0) {
+            P.luma[(int64_t)oy * P.fw + ox] = sum / (float)n;
+            return;
+        }
+    }
+}
+
 static inline unsigned cdivu(int a, int b) { return (unsigned)((a + b - 1) / b); }
 
 hipError_t launchUi(int pass, const UiParams& p, hipStream_t s) {
+    if (pass == kUiFill) {   // one thread a flow cell of the work rectangle
+        const int ncx = (p.wx + p.ww - 1) / p.S - p.wx / p.S + 1, ncy = (p.wy + p.wh - 1) / p.S - p.wy / p.S + 1;
+        hipLaunchKernelGGL(ui_fill_kernel, dim3(cdivu(ncx, 16), cdivu(ncy, 16)), dim3(256), 0, s, p);
+        return hipGetLastError();
+    }
     const dim3 g(cdivu(p.ww, 16), cdivu(p.wh, 16));
     switch (pass) {
     case kUiUpdate: hipLaunchKernelGGL(ui_update_kernel, g, dim3(256), 0, s, p); break;

@@ -176,7 +176,7 @@ int FgrInterpolator::uiFind(const void* key) const {
 }
 
 int FgrInterpolator::uiPick(const void* key) const {
-    // framegen.cpp:1302-1310: the entry keyed by this frame, else the least recently observed
+    // framegen.cpp:1300-1308: the entry keyed by this frame, else the least recently observed
     const int f = uiFind(key);
     if (f >= 0) return f;
     int best = -1;

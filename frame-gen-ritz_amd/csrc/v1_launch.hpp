@@ -14,10 +14,10 @@ enum Pass : int {
     kPassSynthRgba8S1T16, kPassSynthRgba8S2T16, kPassSynthRgba8S4T16,  // RGBA8 synth, 16-row tiles
     kPassSynthRgba8S2T64,                                              // RGBA8 synth, 64-row tiles (flowScale 2)
     kPassSynthRgba8UiS1, kPassSynthRgba8UiS2, kPassSynthRgba8UiS4,     // RGBA8 synth with CrosshairV2 tap rejection
-    kPassUiPatch                                                       // CrosshairV2 paste-back (ui.hip, UiParams)
+    kPassUiPatch, kPassUiFill                                          // CrosshairV2 paste-back, estimate fill (ui.hip)
 };
 // CrosshairV2 passes (ui.hip), single launches over the work rectangle
-enum UiPass : int { kUiUpdate = 0, kUiMask, kUiPatch };
+enum UiPass : int { kUiUpdate = 0, kUiMask, kUiPatch, kUiFill };
 hipError_t launchUi(int pass, const UiParams& p, hipStream_t s);
 hipError_t launchBatch(int pass, const void* params, const uint32_t* work, int n, hipStream_t s);
 hipError_t launchSingle(int pass, const void* params, int gx, int gy, hipStream_t s);

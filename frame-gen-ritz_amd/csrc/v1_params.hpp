@@ -148,6 +148,7 @@ struct UiParams {
     int still, leave, reentry;   // uiStillFrames, uiLeaveFrames, uiReentryFrames
     uint32_t* state;          // 4 words per work-rectangle pixel
     uint8_t* mask; int mpitch;   // the entry's mask (ww x wh bytes)
+    float* luma; int S, fw, fh, bgra, reach;   // estimate fill: this frame's level-0 luma (fw x fh), flowScale
 };
 
 struct FallbackParams {

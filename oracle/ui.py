@@ -5,8 +5,9 @@ TEST INFRASTRUCTURE ONLY. Nothing in the product imports this module: only tests
 
 Follows gpu/framegen.cpp:1041-1075 (boxSize, boxRect, boxMargin, workRect), gpu/shaders/ui_v2_update.comp (the state
 machine), ui_v2_mask.comp (the mask and its pair gate), ui_reject.glsl (tap rejection: a warp tap whose bilinear
-footprint touches its own frame's UI takes the other frame's tap) and ui_patch.comp with `binary` set (the pixels of
-mask byte 255 become curr's). Levels of an Rgba8 pixel are its three 8-bit channels.
+footprint touches its own frame's UI takes the other frame's tap), ui_patch.comp with `binary` set (the pixels of
+mask byte 255 become curr's) and ui_v2_lumafill.comp (uiEstimateFill: the UI cells of the estimate's level-0 luma
+re-averaged over their not-UI pixels). Levels of an Rgba8 pixel are its three 8-bit channels.
 
 State, per pixel of the work rectangle (the box grown by 1 px, clamped to the frame), three words as the shader keeps
 them: w0 = previous original (r | g << 8 | b << 16) | still-and-contrasty count << 24, w1 = last confirmed colour |
@@ -215,3 +216,31 @@ def patch(out, curr, mask, rect):
     reg = o[y0:y0 + h, x0:x0 + w]
     reg[..., :3] = np.where(sel[..., None], curr[y0:y0 + h, x0:x0 + w, :3], reg[..., :3])
     return o
+
+
+def luma_fill(l0, frame, mask, rect, S, reach=8, bgra=False):
+    """ui_v2_lumafill.comp: every flow cell (S x S pixels) of level-0 luma holding a UI pixel (mask byte != 0) becomes
+    the mean of the integer lumas of the not-UI pixels of its cell, widened by S px a side per step up to `reach` px
+    while it has none; float32 sum (exact: integers), one float32 division. Other cells are unchanged."""
+    from . import v1
+    fh, fw = l0.shape
+    H, W = frame.shape[:2]
+    y8 = v1.luma8(frame, bgra)
+    x0, y0, w, h = rect
+    uipx = np.zeros((H, W), bool)
+    uipx[y0:y0 + h, x0:x0 + w] = mask != 0
+    out = l0.copy()
+    for oy in range(y0 // S, min(fh, (y0 + h - 1) // S + 1)):
+        for ox in range(x0 // S, min(fw, (x0 + w - 1) // S + 1)):
+            px, py = ox * S, oy * S
+            if not uipx[py:min(py + S, H), px:min(px + S, W)].any():
+                continue
+            for k in range(0, reach + 1, S):
+                ya, yb = max(py - k, 0), min(py + S + k, H)
+                xa, xb = max(px - k, 0), min(px + S + k, W)
+                sel = ~uipx[ya:yb, xa:xb]
+                n = int(sel.sum())
+                if n > 0:
+                    out[oy, ox] = np.float32(np.float32(y8[ya:yb, xa:xb][sel].sum()) / np.float32(n))
+                    break
+    return out

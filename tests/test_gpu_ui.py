@@ -11,7 +11,8 @@ import torch
 
 import fgr
 from oracle import ui, v1
-from test_gpu_v1 import oracle_settings
+from gpu_util import Internals
+from test_gpu_v1 import check_estimate, oracle_settings
 
 pytestmark = pytest.mark.gpu
 
@@ -123,13 +124,24 @@ def oracle_masks(frames, s):
     return masks, ui.work_rect(so, W, H)
 
 
+def oracle_estimate(a, c, s, ma, mc, rect):
+    os_ = oracle_settings(s)
+    if not s.uiEstimateFill:
+        return v1.estimate(a, c, os_)
+    S = s.flowScale
+    fw, fh, _ = v1.layout(W, H, S, s.pyramidMinSide)
+    la = ui.luma_fill(v1.luma_level0(a, S, fw, fh), a, ma, rect, S)
+    lc = ui.luma_fill(v1.luma_level0(c, S, fw, fh), c, mc, rect, S)
+    return v1.estimate(a, c, os_, luma0=(la, lc), size=(W, H))
+
+
 def check_protected(frames, s, ts, outs, counts, masks):
     om, rect = oracle_masks(frames, s)
     ma, mc = om[-2], om[-1]
     assert np.array_equal(masks[0][0], ma) and np.array_equal(masks[1][0], mc)
     assert (mc == 255).any(), "the scene must hold a detected crosshair"
     a, c = frames[-2], frames[-1]
-    est = v1.estimate(a, c, oracle_settings(s))
+    est = oracle_estimate(a, c, s, ma, mc, rect)
     for t, out, cnt in zip(ts, outs, counts):
         ref, why, info = v1.synth(a, c, est, t, ui=(ma, rect, mc, rect))
         ref = ui.patch(ref, c, mc, rect)
@@ -156,6 +168,27 @@ def test_protected_synth_matches_the_oracle():
     assert fu.record(a, c, o, 0.5)
     assert (o.cpu().numpy() != outs[0]).any()
     fu.close()
+
+
+def test_estimate_fill_matches_the_oracle():
+    """uiEstimateFill: the UI cells of both frames' level-0 luma re-averaged over their not-UI pixels, every estimate
+    intermediate and the frames bit-exact"""
+    frames = scene(11, seed=13, step=(2, -1))
+    s = v2_settings(uiEstimateFill=1)
+    fi = fgr.Interpolator(W, H, settings=s)
+    devs = [torch.from_numpy(f).cuda() for f in frames]
+    for d in devs:
+        assert fi.record_observe(d)
+    assert fi.record_estimate(devs[-2], devs[-1])
+    I = Internals(fi)
+    om, rect = oracle_masks(frames, s)
+    est = oracle_estimate(frames[-2], frames[-1], s, om[-2], om[-1], rect)
+    check_estimate(I, est)
+    plain = v1.estimate(frames[-2], frames[-1], oracle_settings(s))
+    assert not np.array_equal(est["luma_a"][0], plain["luma_a"][0])   # the fill changed the crosshair's cells
+    fi.close()
+    outs, counts, masks = run_protected(frames, s, (0.5, 0.25))
+    check_protected(frames, s, (0.5, 0.25), outs, counts, masks)
 
 
 def test_implicit_observes_match_explicit_ones():
@@ -208,7 +241,7 @@ def test_refusals_and_settings():
     assert "Off and CrosshairV2" in fgr.settings_error_for_format(fgr.default_settings(uiProtection=1))
     assert "Off and CrosshairV2" in fgr.settings_error_for_format(fgr.default_settings(uiProtection=2))
     assert "Rgba8" in fgr.settings_error_for_format(v2_settings(), fgr.FORMAT_RGBAF16)
-    assert "uiEstimateFill" in fgr.settings_error_for_format(v2_settings(uiEstimateFill=1))
+    assert fgr.settings_error_for_format(v2_settings(uiEstimateFill=1)) is None
     assert not fgr.validate(v2_settings(uiHistory=2)) and not fgr.validate(v2_settings(uiLeaveFrames=0))
     # Off: observe and track are accepted and record nothing
     fo = fgr.Interpolator(W, H)

@@ -218,3 +218,26 @@ def test_rejection_removes_the_doubled_crosshair():
     assert differs_raw.sum() > 20          # the doubled crosshair, without protection
     assert differs_prot.sum() == 0         # gone with the rejection
     assert np.array_equal(prot[cm, :3], C[cm, :3])
+
+
+def test_luma_fill_closed_forms():
+    """ui_v2_lumafill.comp: a cell with a UI pixel = the float32 mean of its other pixels' integer lumas; a cell all UI
+    = the mean over the cell widened by S a side; cells without UI unchanged"""
+    rng = np.random.default_rng(3)
+    f = rng.integers(0, 256, (16, 16, 4), dtype=np.uint8)
+    S = 2
+    l0 = v1.luma_level0(f, S, 8, 8)
+    y8 = v1.luma8(f)
+    mask = np.zeros((6, 6), np.uint8)      # the rectangle (4, 4, 6, 6)
+    mask[0, 0] = 255                         # frame pixel (4, 4): cell (2, 2) has one UI pixel
+    mask[2:4, 2:4] = 64                      # frame pixels (6..7, 6..7): cell (3, 3) is all UI
+    out = ui.luma_fill(l0, f, mask, (4, 4, 6, 6), S)
+    assert out[2, 2] == np.float32(np.float32(y8[4:6, 4:6].sum() - y8[4, 4]) / np.float32(3))
+    win = y8[4:10, 4:10].astype(np.int64)
+    ui_px = np.zeros((6, 6), bool)
+    ui_px[0, 0] = True
+    ui_px[2:4, 2:4] = True
+    assert out[3, 3] == np.float32(np.float32(win[~ui_px].sum()) / np.float32((~ui_px).sum()))
+    keep = np.ones((8, 8), bool)
+    keep[2, 2] = keep[3, 3] = False
+    assert np.array_equal(out[keep], l0[keep])
