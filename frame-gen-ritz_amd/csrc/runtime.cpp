@@ -32,7 +32,6 @@ const char* kErrWhite = "linearWhiteNits must be in (0, 10000]";
 const char* kErrYuv = "invalid yuvMatrix";
 const char* kErrOdd = "Nv12 and P010 frames need an even width and height (4:2:0 chroma of whole 2x2 blocks)";
 const char* kErrUiMode = "only UiProtection Off and CrosshairV2 are built in this library";
-const char* kErrUiFormat = "CrosshairV2 is built for Rgba8 frames only";
 
 bool in(float v, float lo, float hi) { return std::isfinite(v) && v >= lo && v <= hi; }
 
@@ -110,7 +109,6 @@ extern "C" const char* fgr_settings_error_for_format(const FgrSettings* s, uint3
     if (!fgr_validate(s)) return kErrSettings;
     if (s->motionVersion != FGR_MOTION_V1) return kErrVersion;
     if (s->uiProtection == FGR_UI_CROSSHAIR || s->uiProtection == FGR_UI_WHOLESCREEN) return kErrUiMode;
-    if (s->uiProtection == FGR_UI_CROSSHAIR_V2 && format != FGR_FORMAT_RGBA8) return kErrUiFormat;
     return nullptr;
 }
 
@@ -460,6 +458,9 @@ void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& pr
             fp.fh = fh;
             fp.bgra = (int)info.bgra;
             fp.reach = 8;
+            fp.fmt = format;
+            fp.lut = fmtLut;
+            fp.yFull = info.yuvFullRange ? 1 : 0;
             PlanOp op;
             op.pass = kPassUiFill;
             op.stage = 1;
@@ -710,6 +711,8 @@ void FgrInterpolator::planSynth(std::vector<PlanOp>& plan, const FgrFrame& prev,
             rows = 32;
             op.pass = S == 1 ? kPassSynthRgba8UiS1 : (S == 2 ? kPassSynthRgba8UiS2 : kPassSynthRgba8UiS4);
         }
+    } else if (estUi) {   // the tap-rejecting variant
+        op.pass = set.flowScale == 1 ? kPassSynthFmtUiS1 : (set.flowScale == 2 ? kPassSynthFmtUiS2 : kPassSynthFmtUiS4);
     } else {
         op.pass = set.flowScale == 1 ? kPassSynthFmtS1 : (set.flowScale == 2 ? kPassSynthFmtS2 : kPassSynthFmtS4);
     }

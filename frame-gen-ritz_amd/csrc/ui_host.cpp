@@ -197,8 +197,18 @@ void FgrInterpolator::uiForget(const void* key) {
 
 // ------------------------------------------------------------------------------------------------------ passes
 
+// the per-format constants every UI pass reads
+static void uiFormatParams(const FgrInterpolator& I, UiParams& p) {
+    p.fmt = I.format;
+    p.lut = I.fmtLut;
+    for (int i = 0; i < 5; i++) p.yuv[i] = I.yuv[i];
+    p.yFull = I.info.yuvFullRange ? 1 : 0;
+    p.bgra = (int)I.info.bgra;
+}
+
 static UiParams uiBaseParams(const FgrInterpolator& I, const UiRect& bx, const UiRect& wr) {
     UiParams p{};
+    uiFormatParams(I, p);
     p.W = (int)I.W;
     p.H = (int)I.H;
     p.bx = bx.x;
@@ -275,9 +285,12 @@ UiParams FgrInterpolator::uiPatchParams(const FgrFrame& curr, const FgrFrame& ou
     // framegen.cpp:2822-2828: curr's entry, its work rectangle, only the bytes of 255
     const UiEntry& E = uiE[estEnt[1]];
     UiParams p{};
+    uiFormatParams(*this, p);
     p.f = ref(curr);
     p.out = (uint8_t*)out.ptr;
     p.opitch = (int64_t)out.pitch;
+    p.outc = (uint8_t*)out.chroma;
+    p.ocpitch = (int64_t)out.chromaPitch;
     p.W = (int)W;
     p.H = (int)H;
     p.wx = E.wr.x;

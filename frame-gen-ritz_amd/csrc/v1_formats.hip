@@ -13,51 +13,11 @@
 #include "v1_device.hpp"
 #include "v1_launch.hpp"
 #include "v1_synth_common.hpp"
+#include "v1_formats.hpp"
 
 namespace fgr {
 
-__device__ __forceinline__ int floorDiv(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
-
-__device__ __forceinline__ float halfBits(uint32_t h) {
-    return (float)__builtin_bit_cast(_Float16, (uint16_t)(h & 0xFFFFu));
-}
-
-__device__ __forceinline__ uint32_t toHalfBits(float v) {   // round to nearest even
-    return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)v);
-}
-
 // ---------------------------------------------------------------------------------------------------------- luma
-
-// Luma of pixel (x, y) in units of 1/16 level (luma.comp:9-27 y8() of each format, fixed point):
-//   Rgb10   round((4899 R + 9617 G + 1868 B) * 255 / (1023 * 16384) * 16) on 10-bit codes (halves up)
-//   RgbaF16 (4899 L_r + 9617 L_g + 1868 L_b + 8192) >> 14, L = the 1/16-level metric table of each half value
-//   P010    round((4 code - 256) * 255 / 219) at limited range (16 (code / 4 - 16) * 255 / 219), 4 code at full range
-//   Nv12    round((16 code - 256) * 255 / 219) at limited range (16 (code - 16) * 255 / 219), 16 code at full range
-__device__ __forceinline__ int luma16(const LumaParams& P, const FrameRef& f, int x, int y) {
-    const uint8_t* row = f.p + (int64_t)y * f.pitch;
-    if (P.fmt == kRgb10) {
-        const uint32_t w = ((const uint32_t*)row)[x];
-        uint32_t r = w & 1023u, g = (w >> 10) & 1023u, b = (w >> 20) & 1023u;
-        if (P.bgra) {
-            const uint32_t t = r;
-            r = b;
-            b = t;
-        }
-        const uint32_t n = r * 4899u + g * 9617u + b * 1868u;   // <= 16760832: n * 255 + 523776 < 2^32
-        return (int)((n * 255u + 523776u) / 1047552u);
-    }
-    if (P.fmt == kF16) {
-        const uint2 q = *(const uint2*)(row + 8 * (int64_t)x);
-        const int lr = P.lut[q.x & 0xFFFFu], lg = P.lut[q.x >> 16], lb = P.lut[q.y & 0xFFFFu];
-        return (lr * 4899 + lg * 9617 + lb * 1868 + 8192) >> 14;   // |L| <= 65535: no overflow
-    }
-    if (P.fmt == kNv12) {
-        const int c = row[x];
-        return P.yFull ? 16 * c : floorDiv((16 * c - 256) * 510 + 219, 438);
-    }
-    const int c = ((const uint16_t*)row)[x] >> 6;
-    return P.yFull ? 4 * c : floorDiv((4 * c - 256) * 510 + 219, 438);
-}
 
 // luma.comp:58-72 on the fixed-point luma: float32(sum of the valid pixels' y16) / float32(16 n). Block (0, 0) also
 // clears the estimate's counters (as the RGBA8 luma pass does).
@@ -142,35 +102,6 @@ __device__ __forceinline__ void f16Warp(const FrameRef& f, int W, int H, int X, 
     }
 }
 
-// Two-plane YCbCr (TEN: P010's 10-bit codes in 16-bit words, else Nv12's bytes). Levels: P010 code / 4, Nv12 code.
-template <bool TEN>
-struct Yuv {
-    static constexpr float kQ = TEN ? 4.0f : 1.0f;          // codes per level
-    static constexpr int kTop = TEN ? 1023 : 255;
-    static __device__ __forceinline__ int y(const uint8_t* row, int x) {
-        return TEN ? ((const uint16_t*)row)[x] >> 6 : row[x];
-    }
-    static __device__ __forceinline__ void c(const uint8_t* row, int x, int& cb, int& cr) {
-        if (TEN) {
-            const uint32_t w = ((const uint32_t*)row)[x];
-            cb = (int)((w & 0xFFFFu) >> 6);
-            cr = (int)(w >> 22);
-        } else {
-            const uint32_t w = ((const uint16_t*)row)[x];
-            cb = (int)(w & 255u);
-            cr = (int)(w >> 8);
-        }
-    }
-    static __device__ __forceinline__ void putY(uint8_t* row, int x, int code) {
-        if (TEN) ((uint16_t*)row)[x] = (uint16_t)(code << 6);
-        else row[x] = (uint8_t)code;
-    }
-    static __device__ __forceinline__ void putC(uint8_t* row, int x, int cb, int cr) {
-        if (TEN) ((uint32_t*)row)[x] = ((uint32_t)cb << 6) | ((uint32_t)cr << 22);
-        else ((uint16_t*)row)[x] = (uint16_t)(cb | (cr << 8));
-    }
-};
-
 // luma: exact bilinear sum of the Y codes (level = sum / (1024 q))
 template <bool TEN>
 __device__ __forceinline__ int yuvYSum(const FrameRef& f, int W, int H, int X, int Y) {
@@ -224,24 +155,60 @@ __device__ __forceinline__ Maps mapsAt(float kA32, float kC32, float x32, float 
 //   Rgb10   uint(min(max_c |wa - wc|, 255)), levels w = float32(sum) * float32(255 / 1047552)
 //   RgbaF16 min(max_c |L(half(wa)) - L(half(wc))| >> 4, 255)
 //   Nv12, P010 uint(min(max |fgYuvDiff(Y, Cb, Cr difference)|, 255))
-template <bool TEN>
+// (UI: with the CrosshairV2 tap rejection of a protected pair, on the raw sums: motion_v1.glsl:124, 141)
+template <bool TEN, bool UI>
 __device__ __forceinline__ int mismatchYuv(const SynthParams& P, const Maps& m) {
-    const int ya = yuvYSum<TEN>(P.a, P.W, P.H, m.XA, m.YA), yc = yuvYSum<TEN>(P.c, P.W, P.H, m.XC, m.YC);
+    int ya = yuvYSum<TEN>(P.a, P.W, P.H, m.XA, m.YA), yc = yuvYSum<TEN>(P.c, P.W, P.H, m.XC, m.YC);
     int cba, cra, cbc, crc;
     yuvCSums<TEN>(P.a, P.cw, P.ch, m.XA, m.YA, cba, cra);
     yuvCSums<TEN>(P.c, P.cw, P.ch, m.XC, m.YC, cbc, crc);
+    if (UI) {
+        const int w = uiRejectWhich<true>(P, m.XA, m.YA, m.XC, m.YC);
+        if (w == 1) {
+            ya = yc;
+            cba = cbc;
+            cra = crc;
+        } else if (w == 2) {
+            yc = ya;
+            cbc = cba;
+            crc = cra;
+        }
+    }
     const float ky = 1.0f / (1024.0f * Yuv<TEN>::kQ), kc = 1.0f / (4096.0f * Yuv<TEN>::kQ);
     const float dY = (float)(ya - yc) * ky;
     const float dCb = (float)(cba - cbc) * kc, dCr = (float)(cra - crc) * kc;
     return (int)fminf(yuvDiffMax(P, dY, dCb, dCr), 255.0f);
 }
 
-template <int F>
+template <int F, bool UI = false>
+__device__ __forceinline__ void uiRejectArrays(const SynthParams& P, const Maps& m, int* sa, int* sc) {
+    if (!UI) return;
+    const int w = uiRejectWhich(P, m.XA, m.YA, m.XC, m.YC);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        if (w == 1) sa[c] = sc[c];
+        else if (w == 2) sc[c] = sa[c];
+    }
+}
+
+template <bool UI>
+__device__ __forceinline__ void uiRejectArraysF(const SynthParams& P, const Maps& m, float* va, float* vc) {
+    if (!UI) return;
+    const int w = uiRejectWhich(P, m.XA, m.YA, m.XC, m.YC);
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        if (w == 1) va[c] = vc[c];
+        else if (w == 2) vc[c] = va[c];
+    }
+}
+
+template <int F, bool UI = false>
 __device__ __forceinline__ int mismatchFmt(const SynthParams& P, const Maps& m) {
     if (F == kRgb10) {
         int sa[3], sc[3];
         rgb10Sums(P.a, P.W, P.H, m.XA, m.YA, sa);
         rgb10Sums(P.c, P.W, P.H, m.XC, m.YC, sc);
+        uiRejectArrays<F, UI>(P, m, sa, sc);
         const float k = (float)(255.0 / 1047552.0);
         float d = 0.0f;
 #pragma unroll
@@ -252,12 +219,13 @@ __device__ __forceinline__ int mismatchFmt(const SynthParams& P, const Maps& m) 
         float va[3], vc[3];
         f16Warp(P.a, P.W, P.H, m.XA, m.YA, va);
         f16Warp(P.c, P.W, P.H, m.XC, m.YC, vc);
+        uiRejectArraysF<UI>(P, m, va, vc);
         int d = 0;
 #pragma unroll
         for (int c = 0; c < 3; c++) d = max(d, abs(P.lut[toHalfBits(va[c])] - P.lut[toHalfBits(vc[c])]));
         return min(d >> 4, 255);
     }
-    return mismatchYuv<F == kP010>(P, m);
+    return mismatchYuv<F == kP010, UI>(P, m);
 }
 
 // resolve.comp:73-83 blend in float32: m = 0.5 (wa + wc) at t = 1/2, else wA wa + wC wc; o = m (1 - u) + near u
@@ -272,12 +240,17 @@ __device__ __forceinline__ float blendF(const SynthParams& P, float wa, float wc
 //   Nv12 / P010 Y code = clamp(floor(q o + 0.5), 0, top) (q codes per level); at an even-even pixel also the CbCr
 //           of its 2x2 block, warped at the block centre p + 1/2 (warpChroma motion_v1.glsl:134-144) and blended with
 //           this pixel's trust
-template <bool TEN>
+template <bool TEN, bool UI>
 __device__ __forceinline__ void resolveYuv(const SynthParams& P, float kA32, float kC32, int x, int y, float2 r, float u,
                                            const Maps& m) {
     using T = Yuv<TEN>;
     const FrameRef& nf = P.nearC ? P.c : P.a;
-    const int ya = yuvYSum<TEN>(P.a, P.W, P.H, m.XA, m.YA), yc = yuvYSum<TEN>(P.c, P.W, P.H, m.XC, m.YC);
+    int ya = yuvYSum<TEN>(P.a, P.W, P.H, m.XA, m.YA), yc = yuvYSum<TEN>(P.c, P.W, P.H, m.XC, m.YC);
+    if (UI) {
+        const int w = uiRejectWhich<true>(P, m.XA, m.YA, m.XC, m.YC);
+        if (w == 1) ya = yc;
+        else if (w == 2) yc = ya;
+    }
     const float ky = 1.0f / (1024.0f * T::kQ), kc = 1.0f / (4096.0f * T::kQ), kn = 1.0f / T::kQ;
     const float ny = (float)T::y(nf.p + (int64_t)y * nf.pitch, x) * kn;
     const float oy = blendF(P, (float)ya * ky, (float)yc * ky, ny, u);
@@ -287,6 +260,16 @@ __device__ __forceinline__ void resolveYuv(const SynthParams& P, float kA32, flo
         int cba, cra, cbc, crc, nb, nr;
         yuvCSums<TEN>(P.a, P.cw, P.ch, b.XA, b.YA, cba, cra);
         yuvCSums<TEN>(P.c, P.cw, P.ch, b.XC, b.YC, cbc, crc);
+        if (UI) {   // warpChroma's rejection at the block centre (motion_v1.glsl:141)
+            const int w = uiRejectWhich<true>(P, b.XA, b.YA, b.XC, b.YC);
+            if (w == 1) {
+                cba = cbc;
+                cra = crc;
+            } else if (w == 2) {
+                cbc = cba;
+                crc = cra;
+            }
+        }
         const int cx = x >> 1, cy = y >> 1;
         T::c(nf.c + (int64_t)cy * nf.cpitch, cx, nb, nr);
         const float ob = blendF(P, (float)cba * kc, (float)cbc * kc, (float)nb * kn, u);
@@ -296,7 +279,7 @@ __device__ __forceinline__ void resolveYuv(const SynthParams& P, float kA32, flo
     }
 }
 
-template <int F>
+template <int F, bool UI = false>
 __device__ __forceinline__ void resolveFmt(const SynthParams& P, float kA32, float kC32, int x, int y, float2 r, float u) {
     const FrameRef& nf = P.nearC ? P.c : P.a;
     const Maps m = mapsAt(kA32, kC32, (float)(32 * x), (float)(32 * y), r);
@@ -304,6 +287,7 @@ __device__ __forceinline__ void resolveFmt(const SynthParams& P, float kA32, flo
         int sa[3], sc[3];
         rgb10Sums(P.a, P.W, P.H, m.XA, m.YA, sa);
         rgb10Sums(P.c, P.W, P.H, m.XC, m.YC, sc);
+        uiRejectArrays<F, UI>(P, m, sa, sc);
         const uint32_t nw = *(const uint32_t*)(nf.p + (int64_t)y * nf.pitch + 4 * (int64_t)x);
         const float k = (float)(255.0 / 1047552.0), kn = (float)(255.0 / 1023.0), ko = (float)(1023.0 / 255.0);
         uint32_t o = nw & 0xC0000000u;
@@ -320,6 +304,7 @@ __device__ __forceinline__ void resolveFmt(const SynthParams& P, float kA32, flo
         float va[3], vc[3];
         f16Warp(P.a, P.W, P.H, m.XA, m.YA, va);
         f16Warp(P.c, P.W, P.H, m.XC, m.YC, vc);
+        uiRejectArraysF<UI>(P, m, va, vc);
         const uint2 nq = *(const uint2*)(nf.p + (int64_t)y * nf.pitch + 8 * (int64_t)x);
         const float n0 = halfBits(nq.x), n1 = halfBits(nq.x >> 16), n2 = halfBits(nq.y);
         uint2 o;
@@ -328,7 +313,7 @@ __device__ __forceinline__ void resolveFmt(const SynthParams& P, float kA32, flo
         *(uint2*)(P.out + (int64_t)y * P.opitch + 8 * (int64_t)x) = o;
         return;
     }
-    resolveYuv<F == kP010>(P, kA32, kC32, x, y, r, u, m);
+    resolveYuv<F == kP010, UI>(P, kA32, kC32, x, y, r, u, m);
 }
 
 // the nearer frame's pixel (x, y) into the output (scene cut): plane 0, and the chroma of an even-even pixel (P010)
@@ -355,7 +340,7 @@ __device__ __forceinline__ void copyNear(const SynthParams& P, int x, int y) {
 // One 64 x 32 output tile (the RGBA8 kernel's structure): field tile, mismatch of the tile and its 3-pixel apron into
 // LDS, 7x7 box, trust, untrusted count, then per pixel the warps again and the blend (the warped values of these
 // formats are three 20-bit sums or floats per frame, too many to keep in registers for 8 pixels a thread).
-template <int S, int F>
+template <int S, int F, bool UI = false>
 __device__ __forceinline__ void synthFmtTile(const SynthParams& P, int tileX, int tileY, float2* ft, float2* hrow,
                                              int* es, uint16_t* hs, uint32_t* cnt) {
     const int tid = threadIdx.x;
@@ -375,7 +360,7 @@ __device__ __forceinline__ void synthFmtTile(const SynthParams& P, int tileX, in
         const int ey = i / EW, ex = i - ey * EW;
         const int x = med3i(x0 - AP + ex, 0, P.W - 1), y = med3i(y0 - AP + ey, 0, P.H - 1);
         const float2 r = fieldAtRegion<S>(P, hrow, ty0, ex, y);
-        es[i] = mismatchFmt<F>(P, mapsAt(kA32, kC32, (float)(32 * x), (float)(32 * y), r));
+        es[i] = mismatchFmt<F, UI>(P, mapsAt(kA32, kC32, (float)(32 * x), (float)(32 * y), r));
     }
     __syncthreads();
     for (int i = tid; i < EH * TW; i += 256) {
@@ -391,7 +376,7 @@ __device__ __forceinline__ void synthFmtTile(const SynthParams& P, int tileX, in
         const uint16_t* h = &hs[ly * TW + cx];
         const int S49 = (int)h[0] + h[TW] + h[2 * TW] + h[3 * TW] + h[4 * TW] + h[5 * TW] + h[6 * TW];
         mine += S49 > P.sHalf ? 1u : 0u;
-        resolveFmt<F>(P, kA32, kC32, x, y, fieldAtRegion<S>(P, hrow, ty0, cx + AP, y), trustOf(P, S49));
+        resolveFmt<F, UI>(P, kA32, kC32, x, y, fieldAtRegion<S>(P, hrow, ty0, cx + AP, y), trustOf(P, S49));
     }
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) mine += __shfl_xor(mine, o, 64);
@@ -400,7 +385,7 @@ __device__ __forceinline__ void synthFmtTile(const SynthParams& P, int tileX, in
     if (tid == 0 && *cnt) atomicAdd(P.count + ((tileY * P.gx + tileX) & (kSlots - 1)), *cnt);
 }
 
-template <int S>
+template <int S, bool UI = false>
 __device__ __forceinline__ void synthFmtBody(const SynthParams& P, int bx, int by) {
     constexpr int FTW = SynthTile<S>::FTW, FTH = SynthTile<S>::FTH;
     __shared__ float2 ft[FTW * FTH];
@@ -408,16 +393,20 @@ __device__ __forceinline__ void synthFmtBody(const SynthParams& P, int bx, int b
     __shared__ int es[EH * EW];
     __shared__ uint16_t hs[EH * TW];
     __shared__ uint32_t cnt;
-    if (P.fmt == kRgb10) synthFmtTile<S, kRgb10>(P, bx, by, ft, hrow, es, hs, &cnt);
-    else if (P.fmt == kF16) synthFmtTile<S, kF16>(P, bx, by, ft, hrow, es, hs, &cnt);
-    else if (P.fmt == kNv12) synthFmtTile<S, kNv12>(P, bx, by, ft, hrow, es, hs, &cnt);
-    else synthFmtTile<S, kP010>(P, bx, by, ft, hrow, es, hs, &cnt);
+    if (P.fmt == kRgb10) synthFmtTile<S, kRgb10, UI>(P, bx, by, ft, hrow, es, hs, &cnt);
+    else if (P.fmt == kF16) synthFmtTile<S, kF16, UI>(P, bx, by, ft, hrow, es, hs, &cnt);
+    else if (P.fmt == kNv12) synthFmtTile<S, kNv12, UI>(P, bx, by, ft, hrow, es, hs, &cnt);
+    else synthFmtTile<S, kP010, UI>(P, bx, by, ft, hrow, es, hs, &cnt);
 }
 
 FGR_PASS(luma_fmt, LumaParams, lumaFmtBody, 256)
 FGR_PASS(synth_fmt_s1, SynthParams, synthFmtBody<1>, 256)
 FGR_PASS(synth_fmt_s2, SynthParams, synthFmtBody<2>, 256)
 FGR_PASS(synth_fmt_s4, SynthParams, synthFmtBody<4>, 256)
+#define FGR_SYNTH_FMT_UI_BODY(S) synthFmtBody<S, true>
+FGR_PASS_SINGLE(synth_fmt_ui_s1, SynthParams, FGR_SYNTH_FMT_UI_BODY(1), 256)
+FGR_PASS_SINGLE(synth_fmt_ui_s2, SynthParams, FGR_SYNTH_FMT_UI_BODY(2), 256)
+FGR_PASS_SINGLE(synth_fmt_ui_s4, SynthParams, FGR_SYNTH_FMT_UI_BODY(4), 256)
 
 // the packed mismatch e of the latest synth (debug maps), 4 pixels per uint32, low byte first
 __global__ __launch_bounds__(256) void debug_e_fmt_kernel(SynthParams P, uint32_t* dst, int eStride) {
@@ -432,10 +421,17 @@ __global__ __launch_bounds__(256) void debug_e_fmt_kernel(SynthParams P, uint32_
         const float2 hv = halfVec(P.field, x, y, P.S, P.fw, P.fh);
         const Maps m = mapsAt(kA32, kC32, (float)(32 * x), (float)(32 * y), hv);
         int e;
-        if (P.fmt == kRgb10) e = mismatchFmt<kRgb10>(P, m);
-        else if (P.fmt == kF16) e = mismatchFmt<kF16>(P, m);
-        else if (P.fmt == kNv12) e = mismatchFmt<kNv12>(P, m);
-        else e = mismatchFmt<kP010>(P, m);
+        if (P.uiA.w > 0 || P.uiC.w > 0) {   // a protected pair: with the tap rejection
+            if (P.fmt == kRgb10) e = mismatchFmt<kRgb10, true>(P, m);
+            else if (P.fmt == kF16) e = mismatchFmt<kF16, true>(P, m);
+            else if (P.fmt == kNv12) e = mismatchFmt<kNv12, true>(P, m);
+            else e = mismatchFmt<kP010, true>(P, m);
+        } else {
+            if (P.fmt == kRgb10) e = mismatchFmt<kRgb10>(P, m);
+            else if (P.fmt == kF16) e = mismatchFmt<kF16>(P, m);
+            else if (P.fmt == kNv12) e = mismatchFmt<kNv12>(P, m);
+            else e = mismatchFmt<kP010>(P, m);
+        }
         packed |= (uint32_t)e << (8 * i);
     }
     dst[(int64_t)y * eStride + gx] = packed;
@@ -450,6 +446,9 @@ hipError_t launchFmtSingle(int pass, const void* params, int gx, int gy, hipStre
     case kPassSynthFmtS1: hipLaunchKernelGGL(synth_fmt_s1_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
     case kPassSynthFmtS2: hipLaunchKernelGGL(synth_fmt_s2_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
     case kPassSynthFmtS4: hipLaunchKernelGGL(synth_fmt_s4_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
+    case kPassSynthFmtUiS1: hipLaunchKernelGGL(synth_fmt_ui_s1_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
+    case kPassSynthFmtUiS2: hipLaunchKernelGGL(synth_fmt_ui_s2_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
+    case kPassSynthFmtUiS4: hipLaunchKernelGGL(synth_fmt_ui_s4_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
     default: return launchSeedSingle(pass, params, gx, gy, s);
     }
     return hipGetLastError();

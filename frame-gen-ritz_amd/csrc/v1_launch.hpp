@@ -14,7 +14,8 @@ enum Pass : int {
     kPassSynthRgba8S1T16, kPassSynthRgba8S2T16, kPassSynthRgba8S4T16,  // RGBA8 synth, 16-row tiles
     kPassSynthRgba8S2T64,                                              // RGBA8 synth, 64-row tiles (flowScale 2)
     kPassSynthRgba8UiS1, kPassSynthRgba8UiS2, kPassSynthRgba8UiS4,     // RGBA8 synth with CrosshairV2 tap rejection
-    kPassUiPatch, kPassUiFill                                          // CrosshairV2 paste-back, estimate fill (ui.hip)
+    kPassUiPatch, kPassUiFill,                                         // CrosshairV2 paste-back, estimate fill (ui.hip)
+    kPassSynthFmtUiS1, kPassSynthFmtUiS2, kPassSynthFmtUiS4            // the other formats' synth with tap rejection
 };
 // CrosshairV2 passes (ui.hip), single launches over the work rectangle
 enum UiPass : int { kUiUpdate = 0, kUiMask, kUiPatch, kUiFill };

@@ -139,7 +139,7 @@ struct SynthParams {
 // CrosshairV2 detection of one observed / tracked frame (ui.hip), the patch of a generated frame
 struct UiParams {
     FrameRef f;               // the observed frame (update, mask), or curr (patch)
-    uint8_t* out; int64_t opitch;   // patch: the generated frame
+    uint8_t* out; uint8_t* outc; int64_t opitch, ocpitch;   // patch: the generated frame (and its CbCr plane)
     int W, H;
     int bx, by, bw, bh;       // the box: detection
     int wx, wy, ww, wh;       // the work rectangle: state layout and mask
@@ -149,6 +149,10 @@ struct UiParams {
     uint32_t* state;          // 4 words per work-rectangle pixel
     uint8_t* mask; int mpitch;   // the entry's mask (ww x wh bytes)
     float* luma; int S, fw, fh, bgra, reach;   // estimate fill: this frame's level-0 luma (fw x fh), flowScale
+    int fmt;                  // Fmt
+    const int* lut;           // RgbaF16: the 1/16-level metric of every half value
+    float yuv[5];             // Nv12 / P010: ys, rcr, bcb, gcb, gcr (the contrast's R'G'B' difference)
+    int yFull;                // Nv12 / P010 full range (the fill's luma)
 };
 
 struct FallbackParams {

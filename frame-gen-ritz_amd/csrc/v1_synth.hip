@@ -108,28 +108,6 @@ __device__ __forceinline__ int emaxRgba8(uint32_t wa, uint32_t wc) {
     return max(d0, max(d1, d2));
 }
 
-// CrosshairV2 tap rejection (ui_reject.glsl): does the bilinear footprint of a tap at the 1/32 map (X, Y) -- texels
-// X >> 5 .. + 1 and Y >> 5 .. + 1, those of weight 0 dropped -- touch a UI pixel of the frame's kept mask?
-__device__ __forceinline__ bool uiHit(const UiMaskRef& m, int X, int Y) {
-    const int lx = X >> 5, ly = Y >> 5;
-    const int x0 = max(lx - m.x, 0), y0 = max(ly - m.y, 0);
-    const int x1 = min(lx + ((X & 31) != 0) - m.x, m.w - 1), y1 = min(ly + ((Y & 31) != 0) - m.y, m.h - 1);
-    bool hit = false;
-    for (int y = y0; y <= y1; y++)
-        for (int x = x0; x <= x1; x++) hit = hit || m.m[y * m.pitch + x] != 0;
-    return hit;
-}
-
-// a tap on its own frame's UI (the other not) takes the other frame's tap; both or neither: unchanged
-template <class T>
-__device__ __forceinline__ void uiReject(const SynthParams& P, int Xa, int Ya, int Xc, int Yc, T& wa, T& wc) {
-    const bool ha = uiHit(P.uiA, Xa, Ya), hc = uiHit(P.uiC, Xc, Yc);
-    if (ha != hc) {
-        if (ha) wa = wc;
-        else wc = wa;
-    }
-}
-
 // The warp pair at full-res pixel (x, y) from r, the interpolated field before the x S of halfVec (hv = S r): a at
 // p - 2t hv, c at p + 2(1-t) hv, each the float32 product then the difference (motion_v1.glsl:120-122), then x 32 for
 // the map. Scaling by a power of two commutes with rounding, so 32 (p - ka (S r)) = 32 p - (32 S ka) r exactly; kA32 and

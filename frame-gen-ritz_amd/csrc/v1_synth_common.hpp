@@ -111,4 +111,44 @@ __device__ __forceinline__ float2 fieldAtRegion(const SynthParams& P, const floa
     return mix2(h[0], h[EW], FlowMap<S>::frac(yc));
 }
 
+
+// CrosshairV2 tap rejection (ui_reject.glsl). The footprint of a tap at the 1/32 map (X, Y): RGB the texels X >> 5 .. + 1
+// and Y >> 5 .. + 1, those of weight 0 dropped; 4:2:0 the pixels of the 2x2 chroma texels around it (the chroma point
+// is (X - 16) / 64 texels), up to 4x4 pixels, which hold the luma taps too. A tap hits when its footprint touches a UI
+// pixel of its frame's kept mask.
+__device__ __forceinline__ bool uiHitRect(const UiMaskRef& m, int lx, int ly, int hx, int hy) {
+    const int x0 = max(lx - m.x, 0), y0 = max(ly - m.y, 0);
+    const int x1 = min(hx - m.x, m.w - 1), y1 = min(hy - m.y, m.h - 1);
+    bool hit = false;
+    for (int y = y0; y <= y1; y++)
+        for (int x = x0; x <= x1; x++) hit = hit || m.m[y * m.pitch + x] != 0;
+    return hit;
+}
+
+template <bool YUV = false>
+__device__ __forceinline__ bool uiHit(const UiMaskRef& m, int X, int Y) {
+    if (YUV) {
+        const int cx = X - 16, cy = Y - 16;
+        const int ix = cx >> 6, iy = cy >> 6;
+        return uiHitRect(m, 2 * ix, 2 * iy, 2 * (ix + ((cx & 63) != 0)) + 1, 2 * (iy + ((cy & 63) != 0)) + 1);
+    }
+    const int lx = X >> 5, ly = Y >> 5;
+    return uiHitRect(m, lx, ly, lx + ((X & 31) != 0), ly + ((Y & 31) != 0));
+}
+
+// which of a tap pair to replace: 0 none (neither or both on their frame's UI), 1 a's (a takes c's), 2 c's
+template <bool YUV = false>
+__device__ __forceinline__ int uiRejectWhich(const SynthParams& P, int Xa, int Ya, int Xc, int Yc) {
+    const bool ha = uiHit<YUV>(P.uiA, Xa, Ya), hc = uiHit<YUV>(P.uiC, Xc, Yc);
+    return ha == hc ? 0 : (ha ? 1 : 2);
+}
+
+// a tap on its own frame's UI (the other not) takes the other frame's tap; both or neither: unchanged
+template <class T>
+__device__ __forceinline__ void uiReject(const SynthParams& P, int Xa, int Ya, int Xc, int Yc, T& wa, T& wc) {
+    const int w = uiRejectWhich(P, Xa, Ya, Xc, Yc);
+    if (w == 1) wa = wc;
+    else if (w == 2) wc = wa;
+}
+
 }  // namespace fgr

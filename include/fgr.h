@@ -54,7 +54,7 @@ typedef struct FgrSettings {
     uint32_t flowScale;       /* structural: 1, 2, 4                                       (framegen.h:403) */
     uint32_t pyramidMinSide;  /* structural: [16, 256]                                     (framegen.h:408) */
     uint32_t seedNet;         /* 0 / 1 (no reference counterpart, see above)                             */
-    /* Static-UI protection (framegen.h:207-305). Built: Off and CrosshairV2 (Rgba8 frames); Crosshair and WholeScreen
+    /* Static-UI protection (framegen.h:207-305). Built: Off and CrosshairV2 (every format); Crosshair and WholeScreen
      * are valid values that fgr_settings_error refuses. */
     uint32_t uiProtection;        /* FGR_UI_*                                                  (framegen.h:225) */
     uint32_t uiHistory;           /* [FGR_UI_FRAMES, FGR_UI_FRAMES_MAX] kept observed frames   (framegen.h:234) */

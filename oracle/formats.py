@@ -307,9 +307,12 @@ def _chroma_sum(cplane, X, Y, fmt=P010):
 
 # ------------------------------------------------------------------------------------------------ synth
 
-def synth(a, c, est, t, fmt, exact64=False, lut=None, yuv=None):
+def synth(a, c, est, t, fmt, exact64=False, lut=None, yuv=None, ui=None):
     """v1.synth for Rgb10 / RgbaF16 / Nv12 / P010 frames (arrays as the encoders return them). Returns (out, reason,
-    info); for Nv12 / P010 out is (Y plane, CbCr plane)."""
+    info); for Nv12 / P010 out is (Y plane, CbCr plane). ui: (maskA, rectA, maskC, rectC) of a CrosshairV2 pair: the
+    warp taps are rejected as oracle/ui.py reject() does, on the raw sums (motion_v1.glsl:124, 141; 4:2:0 footprints for
+    Nv12 / P010, the luma tap's for Y and CbCr of the mismatch and the blend, the block centre's for the blended CbCr)."""
+    from . import ui as uimod
     s = est["settings"]
     yp_a = a[0] if fmt in YUV else a
     H, W = yp_a.shape[:2]
@@ -322,10 +325,14 @@ def synth(a, c, est, t, fmt, exact64=False, lut=None, yuv=None):
     (AX, AY), (CX, CY), hv, (ka, kc) = _maps(est, t, W, H)
     if fmt == RGB10:
         sa, sc = _isum(rgb10_channels(a), AX, AY), _isum(rgb10_channels(c), CX, CY)
+        if ui is not None:
+            sa, sc = uimod.reject(sa, sc, (AX, AY), (CX, CY), ui)
         wa, wc = (sa.astype(f32) * K10).astype(f32), (sc.astype(f32) * K10).astype(f32)
         e = np.minimum(np.abs(wa - wc).max(-1), f32(255.0)).astype(np.int64)
     elif fmt == F16:
         wa, wc = _f16_warp(f16_values(a)[..., :3], AX, AY), _f16_warp(f16_values(c)[..., :3], CX, CY)
+        if ui is not None:
+            wa, wc = uimod.reject(wa, wc, (AX, AY), (CX, CY), ui)
         La = lut.astype(np.int64)[wa.astype(np.float16).view(np.uint16)]
         Lc = lut.astype(np.int64)[wc.astype(np.float16).view(np.uint16)]
         e = np.minimum(np.abs(La - Lc).max(-1) >> 4, 255)
@@ -334,6 +341,10 @@ def synth(a, c, est, t, fmt, exact64=False, lut=None, yuv=None):
         ya, yc = yuv_codes(a[0], fmt), yuv_codes(c[0], fmt)
         sya, syc = _isum(ya, AX, AY), _isum(yc, CX, CY)
         sca, scc = _chroma_sum(a[1], AX, AY, fmt), _chroma_sum(c[1], CX, CY, fmt)
+        if ui is not None:
+            ra, rc = uimod.reject(np.concatenate([sya[..., None], sca], -1), np.concatenate([syc[..., None], scc], -1),
+                                  (AX, AY), (CX, CY), ui, yuv=True)
+            sya, sca, syc, scc = ra[..., 0], ra[..., 1:], rc[..., 0], rc[..., 1:]
         ky, kc_ = f32(1.0 / (1024.0 * qq)), f32(1.0 / (4096.0 * qq))
         dY = ((sya - syc).astype(f32) * ky).astype(f32)
         dC = ((sca - scc).astype(f32) * kc_).astype(f32)
@@ -402,8 +413,11 @@ def synth(a, c, est, t, fmt, exact64=False, lut=None, yuv=None):
     BY = q((qy - (ka * hvb[..., 1]).astype(f32)).astype(f32))
     DX = q((qx + (kc * hvb[..., 0]).astype(f32)).astype(f32))
     DY = q((qy + (kc * hvb[..., 1]).astype(f32)).astype(f32))
-    ca = (_chroma_sum(a[1], BX, BY, fmt).astype(f32) * kc_).astype(f32)
-    cc = (_chroma_sum(c[1], DX, DY, fmt).astype(f32) * kc_).astype(f32)
+    sba, sbc = _chroma_sum(a[1], BX, BY, fmt), _chroma_sum(c[1], DX, DY, fmt)
+    if ui is not None:
+        sba, sbc = uimod.reject(sba, sbc, (BX, BY), (DX, DY), ui, yuv=True)
+    ca = (sba.astype(f32) * kc_).astype(f32)
+    cc = (sbc.astype(f32) * kc_).astype(f32)
     nc = (yuv_codes(near[1], fmt).astype(f32) * kn).astype(f32)
     oc = blend(ca, cc, nc, u[py, px][..., None])
     cout = yuv_store(np.clip(np.floor(oc * f32(qq) + f32(0.5)), 0, top), fmt)
@@ -411,7 +425,7 @@ def synth(a, c, est, t, fmt, exact64=False, lut=None, yuv=None):
     return (yout, cout), None, info
 
 
-def synth_blur(a, c, est, t0, t1, n, fmt, gauss=False, lut=None, yuv=None):
+def synth_blur(a, c, est, t0, t1, n, fmt, gauss=False, lut=None, yuv=None, ui=None):
     """recordSynthBlur for the other formats: the weighted mean of the stored sub-frames (synth() at each sample time,
     the real frames at t <= 0 / t >= 1) in the format's own codes, rounded once (halves: to nearest even)."""
     ts, ws = v1.blur_samples(t0, t1, n, gauss)
@@ -422,7 +436,7 @@ def synth_blur(a, c, est, t0, t1, n, fmt, gauss=False, lut=None, yuv=None):
         elif t >= 1:
             frames.append(c)
         else:
-            frames.append(synth(a, c, est, float(t), fmt, lut=lut, yuv=yuv)[0])
+            frames.append(synth(a, c, est, float(t), fmt, lut=lut, yuv=yuv, ui=ui)[0])
     if fmt == RGB10:
         ch = [np.concatenate([rgb10_channels(f), (f >> 30).astype(np.int64)[..., None]], -1) for f in frames]
         r = v1.average_frames(ch[:1] + ch[1:], ws, hi=1023).astype(np.uint32)

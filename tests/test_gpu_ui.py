@@ -1,9 +1,9 @@
-"""GPU parity of CrosshairV2 static-UI protection (Rgba8) through the C ABI.
+"""GPU parity of CrosshairV2 static-UI protection (Rgba8; the other formats: test_gpu_ui_formats.py) through the C ABI.
 
 Bar: the box state words and every kept mask bit-exact against oracle/ui.py after every observed / tracked frame; a
 protected synth (tap rejection, scene-cut / flick fallbacks, paste-back) bit-exact against the float32 oracle and within
 the FP64 bar; the implicit observes of a host that never observes give the same frames as explicit ones; the refusals
-of framegen.h:575-693 (evicted entries, modes and formats not built, batches).
+of framegen.h:575-693 (evicted entries, modes not built, batches).
 """
 import numpy as np
 import pytest
@@ -240,7 +240,7 @@ def test_refusals_and_settings():
     # modes and formats that are not built, the estimate fill
     assert "Off and CrosshairV2" in fgr.settings_error_for_format(fgr.default_settings(uiProtection=1))
     assert "Off and CrosshairV2" in fgr.settings_error_for_format(fgr.default_settings(uiProtection=2))
-    assert "Rgba8" in fgr.settings_error_for_format(v2_settings(), fgr.FORMAT_RGBAF16)
+    assert fgr.settings_error_for_format(v2_settings(), fgr.FORMAT_RGBAF16) is None
     assert fgr.settings_error_for_format(v2_settings(uiEstimateFill=1)) is None
     assert not fgr.validate(v2_settings(uiHistory=2)) and not fgr.validate(v2_settings(uiLeaveFrames=0))
     # Off: observe and track are accepted and record nothing

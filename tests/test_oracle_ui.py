@@ -241,3 +241,26 @@ def test_luma_fill_closed_forms():
     keep = np.ones((8, 8), bool)
     keep[2, 2] = keep[3, 3] = False
     assert np.array_equal(out[keep], l0[keep])
+
+
+def test_format_levels_are_the_float_forms():
+    """ui.levels pins FG_STILL / fgStillYuv exactly: Rgb10 (510 code + 1023) // 2046 equals floor(code / 1023 * 255 +
+    1/2) in float32 for every code (no ties); P010 (code + 2) >> 2 equals floor(code / 4 + 1/2); Nv12 codes as they are;
+    RgbaF16 the rounded metric of the table ((L16 + 8) >> 4 = floor(L16 / 16 + 1/2))."""
+    codes = np.arange(1024, dtype=np.int64)
+    rgb10 = (codes | (codes[::-1] << 10) | ((codes * 7 % 1024) << 20)).astype(np.uint32).reshape(1, -1)
+    lv = ui.levels(rgb10, 1)
+    f = np.floor((codes.astype(np.float32) / np.float32(1023.0) * np.float32(255.0)).astype(np.float32) + np.float32(0.5))
+    assert np.array_equal(lv[0, :, 0], f.astype(np.int64))
+    y = (codes << 6).astype(np.uint16).reshape(2, 512)
+    cc = (np.stack([codes[:256], codes[256:512]], -1) << 6).astype(np.uint16).reshape(1, 256, 2)
+    lv = ui.levels((y, cc), 4)
+    assert np.array_equal(lv[..., 0].ravel(), np.minimum(np.floor(codes / 4 + 0.5), 255).astype(np.int64))
+    y8 = np.arange(256, dtype=np.uint8).reshape(2, 128)
+    c8 = np.full((1, 64, 2), 77, np.uint8)
+    assert np.array_equal(ui.levels((y8, c8), 3)[..., 0], y8.astype(np.int64))
+    lut = np.arange(-100, 65436, dtype=np.int64)   # a table of 1/16-level metrics
+    h = np.array([[[0, 50, 100, 0], [108, 116, 5000, 0]]], np.uint16)
+    got = ui.levels(h, 2, lut)
+    want = np.clip(np.floor(lut[h[..., :3]] / 16.0 + 0.5), 0, 255).astype(np.int64)
+    assert np.array_equal(got, want)
