@@ -71,7 +71,7 @@ struct FgrBatch {
         for (size_t p = 0; p < pairs.size(); p++) {
             FgrInterpolator& I = *pairs[p];
             const FgrBatchPair& d = desc[p];
-            I.planEstimate(plans[p], d.prev, d.curr);
+            I.planEstimate(plans[p], d.prev, d.curr, true);
             for (uint32_t k = 1; k < d.multiplier; k++)
                 I.planSynth(plans[p], d.prev, d.curr, d.out[k - 1], (float)((double)k / (double)d.multiplier), (int)k - 1);
             for (const PlanOp& op : plans[p]) groups[{op.stage, op.pass}].push_back({(int)p, &op});

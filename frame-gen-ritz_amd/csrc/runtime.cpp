@@ -408,7 +408,8 @@ static int medhistShift(const FgrSettings& s, int n, int parent) {
     return nonDefault && b > 127 ? sh : -1;
 }
 
-void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& prev, const FgrFrame& curr) const {
+void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& prev, const FgrFrame& curr,
+                                   bool batched) const {
     const FgrSettings& st = set;
     float2* F = (float2*)field;
     // 1. luma + INTER_AREA 1/S (also clears the counters)
@@ -537,9 +538,11 @@ void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& pr
         }
         sp.out = F + l.fS;
         sp.sadAcc = (i == 0 && !st.subpixel) ? 1 : 0;
-        sp.gx = cdiv(l.bw * l.bh, 4);
+        // R <= 2 in a batch: two blocks a wavefront (twice the busy lanes, half the workgroups)
+        const bool wide = batched && sp.R <= 2;
+        sp.gx = cdiv(l.bw * l.bh, wide ? kSearch2Blocks : 4);
         PlanOp op;
-        op.pass = sp.R <= 2 ? kPassSearch2 : (sp.R <= 6 ? kPassSearch6 : kPassSearch12);
+        op.pass = wide ? kPassSearch2x8 : (sp.R <= 2 ? kPassSearch2 : (sp.R <= 6 ? kPassSearch6 : kPassSearch12));
         op.stage = 201 + 10 * step;
         op.gx = sp.gx;
         op.set(sp);

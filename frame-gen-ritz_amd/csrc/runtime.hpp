@@ -125,7 +125,8 @@ struct FgrInterpolator {
     int bytesPerPixel() const;
     void stamp(hipStream_t s, const char* name);
     void beginProfile(hipStream_t s);
-    void planEstimate(std::vector<PlanOp>& plan, const FgrFrame& prev, const FgrFrame& curr) const;
+    // batched: the plan of a batch (fgr_batch_*): kernels laid out for many workgroups rather than for latency
+    void planEstimate(std::vector<PlanOp>& plan, const FgrFrame& prev, const FgrFrame& curr, bool batched = false) const;
     void planSynth(std::vector<PlanOp>& plan, const FgrFrame& prev, const FgrFrame& curr, const FgrFrame& out, float t,
                    int k, bool uiPatch = true) const;
     bool runPlan(hipStream_t s, const std::vector<PlanOp>& plan);

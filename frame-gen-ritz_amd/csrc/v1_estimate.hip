@@ -218,86 +218,82 @@ __device__ __forceinline__ void medhistBody(const MedParams& P, int, int) {
 //  2. the (8 + 2R)^2 windows of a (at q - seed) and c (at q + seed) into LDS;
 //  3. lane = candidate (dy-major), sequential y-outer / x-inner SAD over LDS, cost SAD/64 + lambda |d|_1;
 //  4. wavefront argmin on (cost, candidate index): the first of the minimal costs, as the sequential strict <.
-template <int MAXR>
-__device__ __forceinline__ void searchBody(const SearchParams& P, int bx0, int) {
-    constexpr int MAXT = 8 + 2 * MAXR;
-    __shared__ float ta[4][MAXT * MAXT], tc[4][MAXT * MAXT];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int nb = P.bw * P.bh;
-    const int blk = bx0 * 4 + wave;
-    const bool active = blk < nb;
-    const int bx = active ? blk % P.bw : 0, by = active ? blk / P.bw : 0;
+// The seed of block (bx, by) (step 1 above), wave-uniform: lane = pixel, SAD by the 64-lane tree.
+__device__ __forceinline__ void searchSeed(const SearchParams& P, int blk, bool active, int bx, int by, int lane, int& sx,
+                                           int& sy) {
+    sx = 0;
+    sy = 0;
+    if (!P.parent && !P.extraSeed) return;
     const int b0x = bx * kBlock, b0y = by * kBlock;
-    int sx = 0, sy = 0;
-    if (P.parent || P.extraSeed) {
-        const int qx = b0x + (lane & 7), qy = b0y + (lane >> 3);
-        const int px = P.parent ? min(bx / 2, P.pw - 1) : 0, py = P.parent ? min(by / 2, P.ph - 1) : 0;
-        const int ssy = (by & 1) == 0 ? -1 : 1, ssx = (bx & 1) == 0 ? -1 : 1;
-        const int nc = P.parent ? (P.extraSeed ? 7 : 6) : 2;
-        float best = 0.0f;
-        for (int kk = 0; kk < nc; kk++) {
-            const int k = P.parent ? kk : (kk == 0 ? 4 : 6);   // coarsest level: zero, then the proposal
-            int cx, cy;
-            if (k < 4) {
-                const int ox = (k == 2 || k == 3) ? ssx : 0, oy = (k == 1 || k == 3) ? ssy : 0;
-                const float2 v = P.parent[clampi(py + oy, 0, P.ph - 1) * P.pw + clampi(px + ox, 0, P.pw - 1)];
-                cx = 2 * (int)rintf(v.x);
-                cy = 2 * (int)rintf(v.y);
-            } else if (k == 4) {
-                cx = 0;
-                cy = 0;
-            } else if (k == 5) {
-                cx = (int)rintf(__uint_as_float(P.med[0]));
-                cy = (int)rintf(__uint_as_float(P.med[1]));
-            } else {
-                const float2 v = P.extraSeed[active ? blk : 0];
-                cx = (int)rintf(v.x);
-                cy = (int)rintf(v.y);
-            }
-            const float d = fabsf(lumaAt(P.la, qx - cx, qy - cy, P.w, P.h) - lumaAt(P.lc, qx + cx, qy + cy, P.w, P.h));
-            float c = treeSum64(d) * (1.0f / 64.0f);
-            if (kk == 0) {
+    const int qx = b0x + (lane & 7), qy = b0y + (lane >> 3);
+    const int px = P.parent ? min(bx / 2, P.pw - 1) : 0, py = P.parent ? min(by / 2, P.ph - 1) : 0;
+    const int ssy = (by & 1) == 0 ? -1 : 1, ssx = (bx & 1) == 0 ? -1 : 1;
+    const int nc = P.parent ? (P.extraSeed ? 7 : 6) : 2;
+    float best = 0.0f;
+    for (int kk = 0; kk < nc; kk++) {
+        const int k = P.parent ? kk : (kk == 0 ? 4 : 6);   // coarsest level: zero, then the proposal
+        int cx, cy;
+        if (k < 4) {
+            const int ox = (k == 2 || k == 3) ? ssx : 0, oy = (k == 1 || k == 3) ? ssy : 0;
+            const float2 v = P.parent[clampi(py + oy, 0, P.ph - 1) * P.pw + clampi(px + ox, 0, P.pw - 1)];
+            cx = 2 * (int)rintf(v.x);
+            cy = 2 * (int)rintf(v.y);
+        } else if (k == 4) {
+            cx = 0;
+            cy = 0;
+        } else if (k == 5) {
+            cx = (int)rintf(__uint_as_float(P.med[0]));
+            cy = (int)rintf(__uint_as_float(P.med[1]));
+        } else {
+            const float2 v = P.extraSeed[active ? blk : 0];
+            cx = (int)rintf(v.x);
+            cy = (int)rintf(v.y);
+        }
+        const float d = fabsf(lumaAt(P.la, qx - cx, qy - cy, P.w, P.h) - lumaAt(P.lc, qx + cx, qy + cy, P.w, P.h));
+        float c = treeSum64(d) * (1.0f / 64.0f);
+        if (kk == 0) {
+            best = c;
+            sx = cx;
+            sy = cy;
+        } else {
+            c = c + P.lam;
+            if (c < best) {
                 best = c;
                 sx = cx;
                 sy = cy;
-            } else {
-                c = c + P.lam;
-                if (c < best) {
-                    best = c;
-                    sx = cx;
-                    sy = cy;
-                }
             }
         }
     }
-    const int R = P.R, T = 8 + 2 * R, D = 2 * R + 1;
-    float* A = ta[wave];
-    float* C = tc[wave];
+}
+
+// step 2: the (8 + 2R)^2 windows of a (at q - seed) and c (at q + seed), 64 lanes
+__device__ __forceinline__ void searchWindows(const SearchParams& P, float* A, float* C, int b0x, int b0y, int sx, int sy,
+                                             int R, int lane) {
+    const int T = 8 + 2 * R;
     for (int i = lane; i < T * T; i += 64) {
         const int x = i % T, y = i / T;
         A[i] = lumaAt(P.la, b0x - sx - R + x, b0y - sy - R + y, P.w, P.h);
         C[i] = lumaAt(P.lc, b0x + sx - R + x, b0y + sy - R + y, P.w, P.h);
     }
-    __syncthreads();
-    float bestc = FLT_MAX, bests = 0.0f;
-    int besti = INT_MAX;
-    for (int ci = lane; ci < D * D; ci += 64) {
-        const int dx = ci % D - R, dy = ci / D - R;
-        const float* a = A + (R - dy) * T + (R - dx);
-        const float* c = C + (R + dy) * T + (R + dx);
-        float sum = 0.0f;
-        for (int y = 0; y < 8; y++)
-            for (int x = 0; x < 8; x++) sum = sum + fabsf(a[y * T + x] - c[y * T + x]);
-        const float sadc = sum * (1.0f / 64.0f);
-        const float cost = sadc + P.lam * (float)(abs(dx) + abs(dy));
-        if (cost < bestc) {
-            bestc = cost;
-            besti = ci;
-            bests = sadc;
-        }
-    }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
+}
+
+// step 3 for candidate ci: the SAD over the 64 pixels (y-outer, x-inner) and the cost
+__device__ __forceinline__ void searchCost(const SearchParams& P, const float* A, const float* C, int R, int ci, float& cost,
+                                           float& sadc) {
+    const int T = 8 + 2 * R, D = 2 * R + 1;
+    const int dx = ci % D - R, dy = ci / D - R;
+    const float* a = A + (R - dy) * T + (R - dx);
+    const float* c = C + (R + dy) * T + (R + dx);
+    float sum = 0.0f;
+    for (int y = 0; y < 8; y++)
+        for (int x = 0; x < 8; x++) sum = sum + fabsf(a[y * T + x] - c[y * T + x]);
+    sadc = sum * (1.0f / 64.0f);
+    cost = sadc + P.lam * (float)(abs(dx) + abs(dy));
+}
+
+// step 4 within groups of `width` lanes: the first of the minimal costs
+__device__ __forceinline__ void searchArgmin(float& bestc, int& besti, float& bests, int width) {
+    for (int off = width / 2; off >= 1; off >>= 1) {
         const float oc = __shfl_xor(bestc, off, 64);
         const int oi = __shfl_xor(besti, off, 64);
         const float os = __shfl_xor(bests, off, 64);
@@ -307,12 +303,89 @@ __device__ __forceinline__ void searchBody(const SearchParams& P, int bx0, int) 
             bests = os;
         }
     }
+}
+
+template <int MAXR>
+__device__ __forceinline__ void searchBody(const SearchParams& P, int bx0, int) {
+    constexpr int MAXT = 8 + 2 * MAXR;
+    __shared__ float ta[4][MAXT * MAXT], tc[4][MAXT * MAXT];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nb = P.bw * P.bh;
+    const int blk = bx0 * 4 + wave;
+    const bool active = blk < nb;
+    const int bx = active ? blk % P.bw : 0, by = active ? blk / P.bw : 0;
+    int sx, sy;
+    searchSeed(P, blk, active, bx, by, lane, sx, sy);
+    const int R = P.R, D = 2 * R + 1;
+    float* A = ta[wave];
+    float* C = tc[wave];
+    searchWindows(P, A, C, bx * kBlock, by * kBlock, sx, sy, R, lane);
+    __syncthreads();
+    float bestc = FLT_MAX, bests = 0.0f;
+    int besti = INT_MAX;
+    for (int ci = lane; ci < D * D; ci += 64) {
+        float cost, sadc;
+        searchCost(P, A, C, R, ci, cost, sadc);
+        if (cost < bestc) {
+            bestc = cost;
+            besti = ci;
+            bests = sadc;
+        }
+    }
+    searchArgmin(bestc, besti, bests, 64);
     if (active && lane == 0) P.out[blk] = make_float2((float)(sx + besti % D - R), (float)(sy + besti / D - R));
     if (P.sadAcc) {   // search.comp:85, one atomic per workgroup into its slot
         __shared__ uint32_t wsad[4];
         if (lane == 0) wsad[wave] = active ? (uint32_t)rintf(bests * 256.0f) : 0u;
         __syncthreads();
         if (threadIdx.x == 0) atomicAdd(P.u + kUSad + (bx0 & (kSlots - 1)), wsad[0] + wsad[1] + wsad[2] + wsad[3]);
+    }
+}
+
+// R <= 2 ((2R + 1)^2 <= 25 candidates): two blocks a wavefront, eight a workgroup. The seeds and windows of a wave's two
+// blocks are formed one after the other with all 64 lanes, then lanes 0-31 take the first block's candidates and lanes
+// 32-63 the second's (each lane one candidate, its SAD in the order above), and the argmin runs within each half:
+// the same vectors with twice the lanes busy.
+__device__ __forceinline__ void search2Body(const SearchParams& P, int bx0, int) {
+    constexpr int MAXT = 12;
+    __shared__ float ta[kSearch2Blocks][MAXT * MAXT], tc[kSearch2Blocks][MAXT * MAXT];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int nb = P.bw * P.bh;
+    const int R = P.R, D = 2 * R + 1;
+    int sxs[2], sys[2];
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int blk = bx0 * kSearch2Blocks + 2 * wave + h;
+        const bool active = blk < nb;
+        const int bx = active ? blk % P.bw : 0, by = active ? blk / P.bw : 0;
+        searchSeed(P, blk, active, bx, by, lane, sxs[h], sys[h]);
+        searchWindows(P, ta[2 * wave + h], tc[2 * wave + h], bx * kBlock, by * kBlock, sxs[h], sys[h], R, lane);
+    }
+    __syncthreads();
+    const int half = lane >> 5, ci = lane & 31;
+    const int blk = bx0 * kSearch2Blocks + 2 * wave + half;
+    const bool active = blk < nb;
+    float bestc = FLT_MAX, bests = 0.0f;
+    int besti = INT_MAX;
+    if (ci < D * D) {
+        float cost, sadc;
+        searchCost(P, ta[2 * wave + half], tc[2 * wave + half], R, ci, cost, sadc);
+        bestc = cost;
+        besti = ci;
+        bests = sadc;
+    }
+    searchArgmin(bestc, besti, bests, 32);
+    const int sx = half ? sxs[1] : sxs[0], sy = half ? sys[1] : sys[0];
+    if (active && ci == 0) P.out[blk] = make_float2((float)(sx + besti % D - R), (float)(sy + besti / D - R));
+    if (P.sadAcc) {   // search.comp:85, one atomic per workgroup into its slot
+        __shared__ uint32_t wsad[kSearch2Blocks];
+        if (ci == 0) wsad[2 * wave + half] = active ? (uint32_t)rintf(bests * 256.0f) : 0u;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t sum = 0u;
+            for (int i = 0; i < kSearch2Blocks; i++) sum += wsad[i];
+            atomicAdd(P.u + kUSad + (bx0 & (kSlots - 1)), sum);
+        }
     }
 }
 
@@ -698,6 +771,7 @@ FGR_PASS(luma_rgba8, LumaParams, lumaRgba8Body, 256)
 FGR_PASS(down, DownParams, downBody, 256)
 FGR_PASS(medhist, MedParams, medhistBody, 1024)
 FGR_PASS(search2, SearchParams, searchBody<2>, 256)
+FGR_PASS_BATCH(search2x8, SearchParams, search2Body, 256)
 FGR_PASS(search6, SearchParams, searchBody<6>, 256)
 FGR_PASS(search12, SearchParams, searchBody<12>, 256)
 FGR_PASS(median3, Median3Params, median3Body, 256)
@@ -736,6 +810,7 @@ hipError_t launchBatch(int pass, const void* params, const uint32_t* work, int n
     case kPassDown: hipLaunchKernelGGL(down_batch_kernel, g, dim3(256), 0, s, (const DownParams*)params, work); break;
     case kPassMedhist: hipLaunchKernelGGL(medhist_batch_kernel, g, dim3(1024), 0, s, (const MedParams*)params, work); break;
     case kPassSearch2: hipLaunchKernelGGL(search2_batch_kernel, g, dim3(256), 0, s, (const SearchParams*)params, work); break;
+    case kPassSearch2x8: hipLaunchKernelGGL(search2x8_batch_kernel, g, dim3(256), 0, s, (const SearchParams*)params, work); break;
     case kPassSearch6: hipLaunchKernelGGL(search6_batch_kernel, g, dim3(256), 0, s, (const SearchParams*)params, work); break;
     case kPassSearch12: hipLaunchKernelGGL(search12_batch_kernel, g, dim3(256), 0, s, (const SearchParams*)params, work); break;
     case kPassMedian3: hipLaunchKernelGGL(median3_batch_kernel, g, dim3(256), 0, s, (const Median3Params*)params, work); break;

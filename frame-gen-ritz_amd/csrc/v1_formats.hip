@@ -363,11 +363,7 @@ __device__ __forceinline__ void synthFmtTile(const SynthParams& P, int tileX, in
         es[i] = mismatchFmt<F, UI>(P, mapsAt(kA32, kC32, (float)(32 * x), (float)(32 * y), r));
     }
     __syncthreads();
-    for (int i = tid; i < EH * TW; i += 256) {
-        const int rr = i >> 6, c = i & (TW - 1);
-        const int* e = &es[rr * EW + c];
-        hs[i] = (uint16_t)(e[0] + e[1] + e[2] + e[3] + e[4] + e[5] + e[6]);
-    }
+    boxRows7<EH>(es, hs, tid);
     __syncthreads();
     uint32_t mine = 0u;
     for (int k = 0; k < TH / 4; k++) {
@@ -390,8 +386,8 @@ __device__ __forceinline__ void synthFmtBody(const SynthParams& P, int bx, int b
     constexpr int FTW = SynthTile<S>::FTW, FTH = SynthTile<S>::FTH;
     __shared__ float2 ft[FTW * FTH];
     __shared__ float2 hrow[FTH * EW];
-    __shared__ int es[EH * EW];
-    __shared__ uint16_t hs[EH * TW];
+    __shared__ __align__(16) int es[EH * EW];
+    __shared__ __align__(16) uint16_t hs[EH * TW];
     __shared__ uint32_t cnt;
     if (P.fmt == kRgb10) synthFmtTile<S, kRgb10, UI>(P, bx, by, ft, hrow, es, hs, &cnt);
     else if (P.fmt == kF16) synthFmtTile<S, kF16, UI>(P, bx, by, ft, hrow, es, hs, &cnt);

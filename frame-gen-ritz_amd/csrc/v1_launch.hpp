@@ -15,7 +15,8 @@ enum Pass : int {
     kPassSynthRgba8S2T64,                                              // RGBA8 synth, 64-row tiles (flowScale 2)
     kPassSynthRgba8UiS1, kPassSynthRgba8UiS2, kPassSynthRgba8UiS4,     // RGBA8 synth with CrosshairV2 tap rejection
     kPassUiPatch, kPassUiFill,                                         // CrosshairV2 paste-back, estimate fill (ui.hip)
-    kPassSynthFmtUiS1, kPassSynthFmtUiS2, kPassSynthFmtUiS4            // the other formats' synth with tap rejection
+    kPassSynthFmtUiS1, kPassSynthFmtUiS2, kPassSynthFmtUiS4,           // the other formats' synth with tap rejection
+    kPassSearch2x8                                                     // R <= 2 search, two blocks a wavefront (batches)
 };
 // CrosshairV2 passes (ui.hip), single launches over the work rectangle
 enum UiPass : int { kUiUpdate = 0, kUiMask, kUiPatch, kUiFill };
@@ -30,6 +31,7 @@ hipError_t launchDebugEFmt(const SynthParams& p, uint32_t* dst, int eStride, hip
 hipError_t launchSeedSingle(int pass, const void* params, int gx, int gy, hipStream_t s);
 hipError_t launchSeedBatch(int pass, const void* params, const uint32_t* work, int n, hipStream_t s);
 constexpr int kSubpixBlocks = 7;   // blocks per subpix workgroup
+constexpr int kSearch2Blocks = 8;  // blocks per workgroup of the batched R <= 2 search (4 for the others)
 int synthTilesX(int W);
 int synthTilesY(int H, int rows = 32);
 int fallbackTilesX(int W);

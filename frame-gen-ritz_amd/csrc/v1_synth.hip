@@ -172,8 +172,8 @@ __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, 
     constexpr int TH = RT, EH = TileRows<RT>::EH, NAPRON = TileRows<RT>::NAPRON, RK = RT / 16;
     __shared__ float2 ft[FTW * FTH];
     __shared__ float2 hrow[FTH * EW];
-    __shared__ int es[EH * EW];
-    __shared__ uint16_t hs[EH * TW];
+    __shared__ __align__(16) int es[EH * EW];
+    __shared__ __align__(16) uint16_t hs[EH * TW];
     __shared__ uint32_t cnt;
     const int tid = threadIdx.x;
     const int x0 = tileX * TW, y0 = tileY * TH;
@@ -250,11 +250,7 @@ __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, 
         es[ey * EW + ex] = emaxRgb(a, c);
     }
     __syncthreads();
-    for (int i = tid; i < EH * TW; i += 256) {
-        const int r = i >> 6, c = i & (TW - 1);
-        const int* e = &es[r * EW + c];
-        hs[i] = (uint16_t)(e[0] + e[1] + e[2] + e[3] + e[4] + e[5] + e[6]);
-    }
+    boxRows7<EH>(es, hs, tid);
     __syncthreads();
     uint32_t mine = 0u;
 #pragma unroll

@@ -112,6 +112,32 @@ __device__ __forceinline__ float2 fieldAtRegion(const SynthParams& P, const floa
 }
 
 
+// The 7-tap horizontal sums of a tile's mismatch rows: hs[r][c] = e[r][c] + .. + e[r][c + 6] (exact integers) for the
+// EH_ rows of es (EW ints a row) and the TW output columns. A task is 8 output columns of one row: its 14 values read
+// once (as 7 aligned pairs: EW and the task's first column are even), then a running sum, stored two per word.
+template <int EH_>
+__device__ __forceinline__ void boxRows7(const int* es, uint16_t* hs, int tid) {
+    static_assert(EW % 2 == 0 && TW % 8 == 0, "aligned pairs");
+    for (int t = tid; t < EH_ * (TW / 8); t += 256) {
+        const int r = t / (TW / 8), c0 = (t % (TW / 8)) * 8;
+        const int2* e2 = (const int2*)&es[r * EW + c0];
+        int v[14];
+#pragma unroll
+        for (int i = 0; i < 7; i++) {
+            const int2 q = e2[i];
+            v[2 * i] = q.x;
+            v[2 * i + 1] = q.y;
+        }
+        int h[8];
+        h[0] = v[0] + v[1] + v[2] + v[3] + v[4] + v[5] + v[6];
+#pragma unroll
+        for (int i = 1; i < 8; i++) h[i] = h[i - 1] + v[i + 6] - v[i - 1];
+        uint32_t* o = (uint32_t*)&hs[r * TW + c0];
+#pragma unroll
+        for (int j = 0; j < 4; j++) o[j] = (uint32_t)h[2 * j] | ((uint32_t)h[2 * j + 1] << 16);
+    }
+}
+
 // CrosshairV2 tap rejection (ui_reject.glsl). The footprint of a tap at the 1/32 map (X, Y): RGB the texels X >> 5 .. + 1
 // and Y >> 5 .. + 1, those of weight 0 dropped; 4:2:0 the pixels of the 2x2 chroma texels around it (the chroma point
 // is (X - 16) / 64 texels), up to 4x4 pixels, which hold the luma taps too. A tap hits when its footprint touches a UI
