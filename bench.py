@@ -118,7 +118,7 @@ def config_mixed(args):
         "config": {"workload": "64 pairs: 16 each of 1280x720, 1920x1080, 2560x1440, 3840x2160; formats RGBA8 / Rgb10 "
                                "PQ / RgbaF16 scRGB (0-4) / P010 BT.2020 PQ; multipliers 2x/3x/4x; motion +-24 px",
                    "launches_per_step": batch.launches(), "tuning": {"synth_tile_rows": tuning[0],
-                                                                     "xcd_chunk": tuning[1]}},
+                                                                     "xcd_chunk": tuning[1], "lanes": tuning[2]}},
         "plain_pairs_per_s": round(plain, 2), "batched_over_plain": round(rate / plain, 3),
         "whole_path_hbm_frac": round(total * args.steps / bs / 1e9 / HBM_PEAK_GBS, 4),
         "algorithmic_bytes_per_step": int(total)}))
@@ -319,6 +319,7 @@ def main():
     synth_ns, synth_bytes_l = [], []
     rb = batch if batch is not None else None
     if rb is not None:
+        rb.set_lanes(1)   # one launch per pass covers every pair (the bytes below); the tuned lanes are restored after
         rb.set_profiling(True)
         for _ in range(rounds):
             rb.run()
@@ -327,6 +328,7 @@ def main():
                     synth_ns.append(ns)
                     synth_bytes_l.append(sum(synth_bytes(W, H, cut=(p == cut_index)) for p in range(len(fr))))
         rb.set_profiling(False)
+        rb.set_lanes(tuning[2])
         kernel = {16: "synth_rgba8_s2_t16_batch_kernel", 32: "synth_rgba8_s2_batch_kernel",
                   64: "synth_rgba8_s2_t64_batch_kernel"}[tuning[0]]
     # the plain path's per-synth launch, for comparison (the library's profile: events around each synth)
@@ -397,7 +399,7 @@ def main():
         "batched_over_plain": None if batch_rate is None else round(batch_rate / plain_rate, 3),
         "whole_path_hbm_frac": round(total_bytes * args.steps / (ms * 1e-3 * args.steps) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": roofline,
-        "tuning": None if batch is None else {"synth_tile_rows": tuning[0], "xcd_chunk": tuning[1],
+        "tuning": None if batch is None else {"synth_tile_rows": tuning[0], "xcd_chunk": tuning[1], "lanes": tuning[2],
                                                "us_per_step": {f"{r}x{c}": round(t / 1e3, 1)
                                                                for (r, c), t in tune_times.items()}},
         "copy_probe": {"bytes": copy_bytes, "gbs": round(copy_gbs, 1), "frac_of_peak": round(copy_gbs / HBM_PEAK_GBS, 4)},

@@ -7,11 +7,17 @@
 // order (coarse-to-fine levels align at each pair's coarsest level), so the results are bit-identical to the
 // per-pair path. The parameter blocks and work lists are built once, at create time, in one device allocation; a
 // record is a fixed sequence of launches, captured once into a hipGraph and replayed.
+//
+// Lanes: with two lanes the pairs are split in two halves of about equal cost, each with its own launches, recorded on
+// two streams (a fork and a join around them). Pairs are independent, so the halves' kernels may overlap: the short
+// latency-bound passes of one half (level medians, coarse levels, fallbacks) run beside the long passes of the other.
+// The autotuner picks one or two lanes by timing both.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cstring>
 #include <cstdio>
 #include <map>
+#include <tuple>
 #include <string>
 #include <vector>
 #include "../../include/fgr.h"
@@ -27,6 +33,7 @@ thread_local const char* gBatchError = nullptr;
 const char* kBErrUi = "static-UI protection needs the per-frame observes of a handle: batches run with uiProtection Off";
 
 struct Launch {
+    int lane = 0;
     int pass = 0;
     int stage = 0;
     int n = 0;             // workgroups
@@ -47,6 +54,8 @@ struct FgrBatch {
     uint8_t* dev = nullptr;
     size_t devCap = 0;
     hipStream_t cap = nullptr;
+    hipStream_t side = nullptr;       // the second lane's stream
+    hipEvent_t fork = nullptr, join = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool profiling = false;
@@ -54,9 +63,14 @@ struct FgrBatch {
     bool profiled = false;
     int chunk = kXcdChunk;            // tiles per XCD chunk of the work lists (tuned)
     int rows = 32;                    // RGBA8 synth tile rows (tuned)
+    int lanes = 1;                    // 1 or 2 (tuned)
+    std::vector<int> laneOf;          // each pair's lane
 
     ~FgrBatch() {
         for (hipEvent_t e : events) hipEventDestroy(e);
+        if (fork) hipEventDestroy(fork);
+        if (join) hipEventDestroy(join);
+        if (side) hipStreamDestroy(side);
         if (exec) hipGraphExecDestroy(exec);
         if (graph) hipGraphDestroy(graph);
         if (cap) hipStreamDestroy(cap);
@@ -66,7 +80,21 @@ struct FgrBatch {
 
     // the plan of every pair, merged by (stage, pass); fills `blob` (params + work lists) and `launches`
     void build() {
-        std::map<std::pair<int, int>, std::vector<std::pair<int, const PlanOp*>>> groups;
+        // lanes: the pairs by cost (pixels x multiplier), each to the lighter lane
+        laneOf.assign(pairs.size(), 0);
+        if (lanes == 2) {
+            std::vector<int> idx(pairs.size());
+            for (size_t p = 0; p < idx.size(); p++) idx[p] = (int)p;
+            auto cost = [&](int p) { return (double)desc[p].width * desc[p].height * desc[p].multiplier; };
+            std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost(a) > cost(b); });
+            double load[2] = {0.0, 0.0};
+            for (int p : idx) {
+                const int l = load[1] < load[0] ? 1 : 0;
+                laneOf[p] = l;
+                load[l] += cost(p);
+            }
+        }
+        std::map<std::tuple<int, int, int>, std::vector<std::pair<int, const PlanOp*>>> groups;
         std::vector<std::vector<PlanOp>> plans(pairs.size());
         for (size_t p = 0; p < pairs.size(); p++) {
             FgrInterpolator& I = *pairs[p];
@@ -74,14 +102,15 @@ struct FgrBatch {
             I.planEstimate(plans[p], d.prev, d.curr, true);
             for (uint32_t k = 1; k < d.multiplier; k++)
                 I.planSynth(plans[p], d.prev, d.curr, d.out[k - 1], (float)((double)k / (double)d.multiplier), (int)k - 1);
-            for (const PlanOp& op : plans[p]) groups[{op.stage, op.pass}].push_back({(int)p, &op});
+            for (const PlanOp& op : plans[p]) groups[{op.stage, op.pass, laneOf[p]}].push_back({(int)p, &op});
         }
         launches.clear();
         blob.clear();
         for (auto& g : groups) {
             Launch L;
-            L.stage = g.first.first;
-            L.pass = g.first.second;
+            L.stage = std::get<0>(g.first);
+            L.pass = std::get<1>(g.first);
+            L.lane = std::get<2>(g.first);
             const size_t psz = g.second[0].second->params.size();
             L.paramOff = blob.size();
             L.paramBytes = psz * g.second.size();
@@ -106,14 +135,21 @@ struct FgrBatch {
         }
     }
 
+    // one stream in (stage, pass, lane) order (and timed: each launch between two events), or the two lanes forked
     bool enqueue(hipStream_t s, bool timed = false) {
+        const bool fork2 = lanes == 2 && !timed;
+        if (fork2 && (hipEventRecord(fork, s) != hipSuccess || hipStreamWaitEvent(side, fork, 0) != hipSuccess))
+            return false;
         for (size_t i = 0; i < launches.size(); i++) {
             const Launch& L = launches[i];
-            if (timed) hipEventRecord(events[2 * i], s);
-            if (launchBatch(L.pass, dev + L.paramOff, (const uint32_t*)(dev + L.workOff), L.n, s) != hipSuccess)
+            hipStream_t ls = fork2 && L.lane == 1 ? side : s;
+            if (timed) hipEventRecord(events[2 * i], ls);
+            if (launchBatch(L.pass, dev + L.paramOff, (const uint32_t*)(dev + L.workOff), L.n, ls) != hipSuccess)
                 return false;
-            if (timed) hipEventRecord(events[2 * i + 1], s);
+            if (timed) hipEventRecord(events[2 * i + 1], ls);
         }
+        if (fork2 && (hipEventRecord(join, side) != hipSuccess || hipStreamWaitEvent(s, join, 0) != hipSuccess))
+            return false;
         return true;
     }
 
@@ -134,9 +170,10 @@ struct FgrBatch {
     }
 
     // a tile configuration: rebuild the launches and the device blob (its size changes with the grids)
-    bool tune(int r, int c) {
+    bool tune(int r, int c, int l = -1) {
         rows = r;
         chunk = c;
+        if (l > 0) lanes = l;
         for (FgrInterpolator* p : pairs) p->synthRows = r;
         build();
         dropGraph();
@@ -161,8 +198,11 @@ struct FgrBatch {
 };
 
 // Tile configurations chosen by fgr_batch_autotune, per batch key (process-wide; fgr_tuning_save / load persist it)
-static std::map<std::string, std::pair<int, int>>& tuningCache() {
-    static std::map<std::string, std::pair<int, int>> m;
+struct Tuning {
+    int rows, chunk, lanes;
+};
+static std::map<std::string, Tuning>& tuningCache() {
+    static std::map<std::string, Tuning> m;
     return m;
 }
 
@@ -217,8 +257,9 @@ extern "C" FgrBatch* fgr_batch_create(const FgrBatchInfo* info, const FgrBatchPa
     }
     const auto hit = tuningCache().find(b->key());
     if (hit != tuningCache().end()) {
-        b->rows = hit->second.first;
-        b->chunk = hit->second.second;
+        b->rows = hit->second.rows;
+        b->chunk = hit->second.chunk;
+        b->lanes = hit->second.lanes;
         for (FgrInterpolator* p : b->pairs) p->synthRows = b->rows;
     }
     b->build();
@@ -229,8 +270,10 @@ extern "C" FgrBatch* fgr_batch_create(const FgrBatchInfo* info, const FgrBatchPa
         delete b;
         return nullptr;
     }
-    if (hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess) {
-        b->cap = nullptr;
+    if (hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess) {
         gBatchError = kBErrStream;
         delete b;
         return nullptr;
@@ -294,10 +337,19 @@ extern "C" int fgr_batch_get_tuning(const FgrBatch* b, int32_t* rows, int32_t* c
     return 1;
 }
 
+extern "C" int fgr_batch_set_lanes(FgrBatch* b, int32_t lanes) {
+    if (!b || (lanes != 1 && lanes != 2)) return 0;
+    hipSetDevice(b->info.device);
+    return b->tune(b->rows, b->chunk, lanes) ? 1 : 0;
+}
+
+extern "C" int32_t fgr_batch_get_lanes(const FgrBatch* b) { return b ? b->lanes : 0; }
+
 // Times every tile configuration (RGBA8 synth tiles of 16, 32, 64 rows x work-list XCD chunks of 16, 64, 256 tiles)
 // on this batch's own records -- `reps` eager records each, after one untimed -- keeps the fastest, and caches it for
-// batches of the same key. The records write the batch's outputs. ns[i] receives configuration i's time per record
-// (i = rowsIndex * 3 + chunkIndex; 9 entries) when given.
+// batches of the same key; then one lane and two lanes with those tiles. The records write the batch's outputs. ns[i]
+// receives configuration i's time per record (i = rowsIndex * 3 + chunkIndex; then ns[9], ns[10] one and two lanes;
+// 11 entries) when given.
 extern "C" int fgr_batch_autotune(FgrBatch* b, void* stream, int32_t reps, uint64_t* ns) {
     if (!b || reps < 1) return 0;
     hipSetDevice(b->info.device);
@@ -325,18 +377,34 @@ extern "C" int fgr_batch_autotune(FgrBatch* b, void* stream, int32_t reps, uint6
                 bestC = kChunks[ci];
             }
         }
+    // then one lane or two with the best tiles (ns[9], ns[10] when given)
+    double laneT[3] = {0.0, 0.0, 0.0};
+    for (int l = 1; l <= 2 && ok; l++) {
+        if (!b->tune(bestR, bestC, l) || !b->enqueue(s)) { ok = false; break; }
+        hipEventRecord(e0, s);
+        for (int r = 0; r < reps && ok; r++) ok = b->enqueue(s);
+        hipEventRecord(e1, s);
+        if (hipEventSynchronize(e1) != hipSuccess) { ok = false; break; }
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, e0, e1);
+        const double t = (double)ms / reps;
+        if (ns) ns[8 + l] = (uint64_t)(t * 1e6 + 0.5);
+        laneT[l] = t;
+    }
+    const int bestL = laneT[2] < laneT[1] ? 2 : 1;
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    if (!ok || !b->tune(bestR, bestC)) return 0;
-    tuningCache()[b->key()] = {bestR, bestC};
+    if (!ok || !b->tune(bestR, bestC, bestL)) return 0;
+    tuningCache()[b->key()] = {bestR, bestC, bestL};
     return 1;
 }
 
-// the tuning cache as text, one "key rows chunk" line per entry
+// the tuning cache as text, one "key rows chunk lanes" line per entry
 extern "C" int fgr_tuning_save(const char* path) {
     FILE* f = path ? std::fopen(path, "w") : nullptr;
     if (!f) return 0;
-    for (const auto& e : tuningCache()) std::fprintf(f, "%s %d %d\n", e.first.c_str(), e.second.first, e.second.second);
+    for (const auto& e : tuningCache())
+        std::fprintf(f, "%s %d %d %d\n", e.first.c_str(), e.second.rows, e.second.chunk, e.second.lanes);
     std::fclose(f);
     return 1;
 }
@@ -345,9 +413,9 @@ extern "C" int fgr_tuning_load(const char* path) {
     FILE* f = path ? std::fopen(path, "r") : nullptr;
     if (!f) return 0;
     char key[65536];
-    int r, c, n = 0;
-    while (std::fscanf(f, "%65535s %d %d", key, &r, &c) == 3) {
-        tuningCache()[key] = {r, c};
+    int r, c, l, n = 0;
+    while (std::fscanf(f, "%65535s %d %d %d", key, &r, &c, &l) == 4) {
+        tuningCache()[key] = {r, c, l == 2 ? 2 : 1};
         n++;
     }
     std::fclose(f);

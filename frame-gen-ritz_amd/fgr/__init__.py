@@ -165,6 +165,8 @@ _sig = {
     "fgr_batch_autotune": (ctypes.c_int, [_H, ctypes.c_void_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64)]),
     "fgr_batch_set_tuning": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.c_int32]),
     "fgr_batch_get_tuning": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
+    "fgr_batch_set_lanes": (ctypes.c_int, [_H, ctypes.c_int32]),
+    "fgr_batch_get_lanes": (ctypes.c_int32, [_H]),
     "fgr_tuning_save": (ctypes.c_int, [ctypes.c_char_p]),
     "fgr_tuning_load": (ctypes.c_int, [ctypes.c_char_p]),
     "fgr_seednet_calibrate_fp8": (ctypes.c_int, [_H, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
@@ -496,20 +498,27 @@ class Batch:
     TUNE_ROWS, TUNE_CHUNKS = (16, 32, 64), (16, 64, 256)
 
     def autotune(self, reps=3, stream=None):
-        """Time the 9 tile configurations on this batch, keep the fastest; returns ({(rows, chunk): ns}, chosen)."""
-        ns = (ctypes.c_uint64 * 9)()
+        """Time the 9 tile configurations on this batch, keep the fastest, then one and two lanes with it; returns
+        ({(rows, chunk): ns, ("lanes", 1): ns, ("lanes", 2): ns}, (rows, chunk, lanes))."""
+        ns = (ctypes.c_uint64 * 11)()
         if not lib.fgr_batch_autotune(self._h, _stream(stream), reps, ns):
             raise RuntimeError("fgr_batch_autotune failed")
         times = {(r, c): int(ns[i * 3 + j]) for i, r in enumerate(self.TUNE_ROWS) for j, c in enumerate(self.TUNE_CHUNKS)}
+        times[("lanes", 1)], times[("lanes", 2)] = int(ns[9]), int(ns[10])
         return times, self.get_tuning()
 
-    def set_tuning(self, rows, chunk):
-        return bool(lib.fgr_batch_set_tuning(self._h, rows, chunk))
+    def set_tuning(self, rows, chunk, lanes=None):
+        ok = bool(lib.fgr_batch_set_tuning(self._h, rows, chunk))
+        return ok and (lanes is None or bool(lib.fgr_batch_set_lanes(self._h, lanes)))
 
     def get_tuning(self):
+        """(rows, chunk, lanes)"""
         r, c = ctypes.c_int32(), ctypes.c_int32()
         lib.fgr_batch_get_tuning(self._h, ctypes.byref(r), ctypes.byref(c))
-        return r.value, c.value
+        return r.value, c.value, lib.fgr_batch_get_lanes(self._h)
+
+    def set_lanes(self, lanes):
+        return bool(lib.fgr_batch_set_lanes(self._h, lanes))
 
     def read_timings(self):
         """[(stage, pass, ns), ...] per launch of the last profiled record, or None."""
