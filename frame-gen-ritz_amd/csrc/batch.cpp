@@ -8,10 +8,10 @@
 // per-pair path. The parameter blocks and work lists are built once, at create time, in one device allocation; a
 // record is a fixed sequence of launches, captured once into a hipGraph and replayed.
 //
-// Lanes: with two lanes the pairs are split in two halves of about equal cost, each with its own launches, recorded on
-// two streams (a fork and a join around them). Pairs are independent, so the halves' kernels may overlap: the short
-// latency-bound passes of one half (level medians, coarse levels, fallbacks) run beside the long passes of the other.
-// The autotuner picks one or two lanes by timing both.
+// Lanes: with L lanes (1, 2, 3 or 4) the pairs are split in L groups of about equal cost, each with its own launches,
+// recorded on L streams (a fork and a join around them). Pairs are independent, so the groups' kernels may overlap:
+// the short latency-bound passes of one group (level medians, coarse levels, fallbacks) run beside the long passes of
+// another. The autotuner picks the lane count by timing each.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <cstring>
@@ -54,8 +54,9 @@ struct FgrBatch {
     uint8_t* dev = nullptr;
     size_t devCap = 0;
     hipStream_t cap = nullptr;
-    hipStream_t side = nullptr;       // the second lane's stream
-    hipEvent_t fork = nullptr, join = nullptr;
+    static constexpr int kMaxLanes = 4;
+    hipStream_t side[kMaxLanes - 1] = {};   // lanes 1.. (lane 0 records on the record's stream)
+    hipEvent_t fork = nullptr, join[kMaxLanes - 1] = {};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool profiling = false;
@@ -63,14 +64,16 @@ struct FgrBatch {
     bool profiled = false;
     int chunk = kXcdChunk;            // tiles per XCD chunk of the work lists (tuned)
     int rows = 32;                    // RGBA8 synth tile rows (tuned)
-    int lanes = 1;                    // 1 or 2 (tuned)
+    int lanes = 1;                    // 1 .. kMaxLanes (tuned)
     std::vector<int> laneOf;          // each pair's lane
 
     ~FgrBatch() {
         for (hipEvent_t e : events) hipEventDestroy(e);
         if (fork) hipEventDestroy(fork);
-        if (join) hipEventDestroy(join);
-        if (side) hipStreamDestroy(side);
+        for (int i = 0; i < kMaxLanes - 1; i++) {
+            if (join[i]) hipEventDestroy(join[i]);
+            if (side[i]) hipStreamDestroy(side[i]);
+        }
         if (exec) hipGraphExecDestroy(exec);
         if (graph) hipGraphDestroy(graph);
         if (cap) hipStreamDestroy(cap);
@@ -82,14 +85,16 @@ struct FgrBatch {
     void build() {
         // lanes: the pairs by cost (pixels x multiplier), each to the lighter lane
         laneOf.assign(pairs.size(), 0);
-        if (lanes == 2) {
+        if (lanes > 1) {
             std::vector<int> idx(pairs.size());
             for (size_t p = 0; p < idx.size(); p++) idx[p] = (int)p;
             auto cost = [&](int p) { return (double)desc[p].width * desc[p].height * desc[p].multiplier; };
             std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost(a) > cost(b); });
-            double load[2] = {0.0, 0.0};
+            double load[kMaxLanes] = {};
             for (int p : idx) {
-                const int l = load[1] < load[0] ? 1 : 0;
+                int l = 0;
+                for (int i = 1; i < lanes; i++)
+                    if (load[i] < load[l]) l = i;
                 laneOf[p] = l;
                 load[l] += cost(p);
             }
@@ -137,19 +142,21 @@ struct FgrBatch {
 
     // one stream in (stage, pass, lane) order (and timed: each launch between two events), or the two lanes forked
     bool enqueue(hipStream_t s, bool timed = false) {
-        const bool fork2 = lanes == 2 && !timed;
-        if (fork2 && (hipEventRecord(fork, s) != hipSuccess || hipStreamWaitEvent(side, fork, 0) != hipSuccess))
-            return false;
+        const int nl = timed ? 1 : lanes;
+        if (nl > 1 && hipEventRecord(fork, s) != hipSuccess) return false;
+        for (int i = 0; i + 1 < nl; i++)
+            if (hipStreamWaitEvent(side[i], fork, 0) != hipSuccess) return false;
         for (size_t i = 0; i < launches.size(); i++) {
             const Launch& L = launches[i];
-            hipStream_t ls = fork2 && L.lane == 1 ? side : s;
+            hipStream_t ls = nl > 1 && L.lane > 0 ? side[L.lane - 1] : s;
             if (timed) hipEventRecord(events[2 * i], ls);
             if (launchBatch(L.pass, dev + L.paramOff, (const uint32_t*)(dev + L.workOff), L.n, ls) != hipSuccess)
                 return false;
             if (timed) hipEventRecord(events[2 * i + 1], ls);
         }
-        if (fork2 && (hipEventRecord(join, side) != hipSuccess || hipStreamWaitEvent(s, join, 0) != hipSuccess))
-            return false;
+        for (int i = 0; i + 1 < nl; i++)
+            if (hipEventRecord(join[i], side[i]) != hipSuccess || hipStreamWaitEvent(s, join[i], 0) != hipSuccess)
+                return false;
         return true;
     }
 
@@ -270,10 +277,12 @@ extern "C" FgrBatch* fgr_batch_create(const FgrBatchInfo* info, const FgrBatchPa
         delete b;
         return nullptr;
     }
-    if (hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&b->join, hipEventDisableTiming) != hipSuccess) {
+    bool streams = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) == hipSuccess &&
+                   hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; streams && i < FgrBatch::kMaxLanes - 1; i++)
+        streams = hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking) == hipSuccess &&
+                  hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming) == hipSuccess;
+    if (!streams) {
         gBatchError = kBErrStream;
         delete b;
         return nullptr;
@@ -338,7 +347,7 @@ extern "C" int fgr_batch_get_tuning(const FgrBatch* b, int32_t* rows, int32_t* c
 }
 
 extern "C" int fgr_batch_set_lanes(FgrBatch* b, int32_t lanes) {
-    if (!b || (lanes != 1 && lanes != 2)) return 0;
+    if (!b || lanes < 1 || lanes > FgrBatch::kMaxLanes) return 0;
     hipSetDevice(b->info.device);
     return b->tune(b->rows, b->chunk, lanes) ? 1 : 0;
 }
@@ -347,9 +356,9 @@ extern "C" int32_t fgr_batch_get_lanes(const FgrBatch* b) { return b ? b->lanes 
 
 // Times every tile configuration (RGBA8 synth tiles of 16, 32, 64 rows x work-list XCD chunks of 16, 64, 256 tiles)
 // on this batch's own records -- `reps` eager records each, after one untimed -- keeps the fastest, and caches it for
-// batches of the same key; then one lane and two lanes with those tiles. The records write the batch's outputs. ns[i]
-// receives configuration i's time per record (i = rowsIndex * 3 + chunkIndex; then ns[9], ns[10] one and two lanes;
-// 11 entries) when given.
+// batches of the same key; then 1, 2, 3 and 4 lanes with those tiles. The records write the batch's outputs. ns[i]
+// receives configuration i's time per record (i = rowsIndex * 3 + chunkIndex; then ns[9 .. 12] 1 .. 4 lanes; 13
+// entries) when given.
 extern "C" int fgr_batch_autotune(FgrBatch* b, void* stream, int32_t reps, uint64_t* ns) {
     if (!b || reps < 1) return 0;
     hipSetDevice(b->info.device);
@@ -377,9 +386,9 @@ extern "C" int fgr_batch_autotune(FgrBatch* b, void* stream, int32_t reps, uint6
                 bestC = kChunks[ci];
             }
         }
-    // then one lane or two with the best tiles (ns[9], ns[10] when given)
-    double laneT[3] = {0.0, 0.0, 0.0};
-    for (int l = 1; l <= 2 && ok; l++) {
+    // then 1 .. 4 lanes with the best tiles (ns[9 .. 12] when given)
+    double laneT[FgrBatch::kMaxLanes + 1] = {};
+    for (int l = 1; l <= FgrBatch::kMaxLanes && ok; l++) {
         if (!b->tune(bestR, bestC, l) || !b->enqueue(s)) { ok = false; break; }
         hipEventRecord(e0, s);
         for (int r = 0; r < reps && ok; r++) ok = b->enqueue(s);
@@ -391,7 +400,9 @@ extern "C" int fgr_batch_autotune(FgrBatch* b, void* stream, int32_t reps, uint6
         if (ns) ns[8 + l] = (uint64_t)(t * 1e6 + 0.5);
         laneT[l] = t;
     }
-    const int bestL = laneT[2] < laneT[1] ? 2 : 1;
+    int bestL = 1;
+    for (int l = 2; l <= FgrBatch::kMaxLanes; l++)
+        if (laneT[l] < laneT[bestL]) bestL = l;
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     if (!ok || !b->tune(bestR, bestC, bestL)) return 0;
@@ -415,7 +426,7 @@ extern "C" int fgr_tuning_load(const char* path) {
     char key[65536];
     int r, c, l, n = 0;
     while (std::fscanf(f, "%65535s %d %d %d", key, &r, &c, &l) == 4) {
-        tuningCache()[key] = {r, c, l == 2 ? 2 : 1};
+        tuningCache()[key] = {r, c, std::min(std::max(l, 1), FgrBatch::kMaxLanes)};
         n++;
     }
     std::fclose(f);

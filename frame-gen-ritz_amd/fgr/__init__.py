@@ -498,13 +498,14 @@ class Batch:
     TUNE_ROWS, TUNE_CHUNKS = (16, 32, 64), (16, 64, 256)
 
     def autotune(self, reps=3, stream=None):
-        """Time the 9 tile configurations on this batch, keep the fastest, then one and two lanes with it; returns
-        ({(rows, chunk): ns, ("lanes", 1): ns, ("lanes", 2): ns}, (rows, chunk, lanes))."""
-        ns = (ctypes.c_uint64 * 11)()
+        """Time the 9 tile configurations on this batch, keep the fastest, then 1 to 4 lanes with it; returns
+        ({(rows, chunk): ns, ("lanes", L): ns}, (rows, chunk, lanes))."""
+        ns = (ctypes.c_uint64 * 13)()
         if not lib.fgr_batch_autotune(self._h, _stream(stream), reps, ns):
             raise RuntimeError("fgr_batch_autotune failed")
         times = {(r, c): int(ns[i * 3 + j]) for i, r in enumerate(self.TUNE_ROWS) for j, c in enumerate(self.TUNE_CHUNKS)}
-        times[("lanes", 1)], times[("lanes", 2)] = int(ns[9]), int(ns[10])
+        for lanes in range(1, 5):
+            times[("lanes", lanes)] = int(ns[8 + lanes])
         return times, self.get_tuning()
 
     def set_tuning(self, rows, chunk, lanes=None):

@@ -229,16 +229,16 @@ uint32_t fgr_batch_launches(const FgrBatch* b);
 int fgr_batch_set_profiling(FgrBatch* b, int on);
 /* Tile autotuner: the tile configurations are the RGBA8 synth's tile height (16, 32 or 64 rows) and the number of
  * consecutive tiles each XCD takes from a launch's work list (16, 64 or 256). autotune times all nine on this batch's
- * own records (`reps` each; the outputs are written), keeps the fastest, then times one and two lanes (below) with it,
+ * own records (`reps` each; the outputs are written), keeps the fastest, then times 1 to 4 lanes (below) with it,
  * and remembers the choice for batches of the same pairs (sizes, formats, multipliers, flowScale, pyramidMinSide),
- * which then start with it. Results are bit-identical under every configuration. ns: 11 times per record (ns; index
- * rows * 3 + chunk, then one and two lanes), or null. */
+ * which then start with it. Results are bit-identical under every configuration. ns: 13 times per record (ns; index
+ * rows * 3 + chunk, then 1 .. 4 lanes), or null. */
 int fgr_batch_autotune(FgrBatch* b, void* stream, int32_t reps, uint64_t* ns);
 int fgr_batch_set_tuning(FgrBatch* b, int32_t rows, int32_t chunk);
 int fgr_batch_get_tuning(const FgrBatch* b, int32_t* rows, int32_t* chunk);
-/* Lanes (tuned by fgr_batch_autotune after the tiles: ns[9], ns[10] = one and two lanes): 2 splits the pairs in two
- * halves of about equal cost, each with its own launches, recorded on two streams that fork from and join into the
- * record's stream, so the short passes of one half run beside the long ones of the other. Results are unchanged. */
+/* Lanes (1 to 4; tuned by fgr_batch_autotune after the tiles: ns[9 .. 12] = 1 .. 4 lanes): L lanes split the pairs in
+ * L groups of about equal cost, each with its own launches, recorded on L streams that fork from and join into the
+ * record's stream, so the short passes of one group run beside the long ones of another. Results are unchanged. */
 int fgr_batch_set_lanes(FgrBatch* b, int32_t lanes);
 int32_t fgr_batch_get_lanes(const FgrBatch* b);
 int fgr_tuning_save(const char* path);

@@ -127,7 +127,7 @@ def test_1080p_workload_batch_matches_per_pair():
     b.close()
 
 
-@pytest.mark.parametrize("rows,chunk,lanes", [(16, 16, 1), (32, 256, 2), (64, 64, 2)])
+@pytest.mark.parametrize("rows,chunk,lanes", [(16, 16, 1), (32, 256, 2), (64, 64, 3), (32, 64, 4)])
 def test_tile_configurations_are_bit_identical(rows, chunk, lanes):
     """Every configuration of the tuner (synth tile rows x XCD chunk x lanes) gives the per-pair path's bytes."""
     pairs, mults = mixed_pairs()
@@ -146,10 +146,11 @@ def test_autotune_picks_the_fastest_and_caches_it(tmp_path):
     pairs, _ = scenes.pairs_1080p(2, n_pairs=4)
     b, items = make_batch(pairs, [4] * 4, use_graph=True)
     times, chosen = b.autotune(reps=3)
-    assert len(times) == 11 and all(t > 0 for t in times.values())
+    assert len(times) == 13 and all(t > 0 for t in times.values())
     tiles = {k: v for k, v in times.items() if k[0] != "lanes"}
     assert tiles[chosen[:2]] == min(tiles.values())
-    assert chosen[2] == (2 if times[("lanes", 2)] < times[("lanes", 1)] else 1)
+    lanes = {k[1]: v for k, v in times.items() if k[0] == "lanes"}
+    assert lanes[chosen[2]] == min(lanes.values())
     b.run()
     torch.cuda.synchronize()
     want = per_pair(*pairs[1], 4)
