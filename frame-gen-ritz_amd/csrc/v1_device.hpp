@@ -8,15 +8,25 @@
 
 namespace fgr {
 
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+// v clamped to [lo, hi] (lo <= hi at every use): one v_med3_i32
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
+
+// Row-major element index y w + x of in-range coordinates as a 32-bit unsigned value (y, w < 2^24: a full-rate 24-bit
+// multiply), and the element at it: the byte offset stays 32-bit, so a plane whose base is uniform is read with the
+// base in scalar registers plus one vector offset.
+__device__ __forceinline__ uint32_t idx2(int x, int y, int w) { return __umul24((uint32_t)y, (uint32_t)w) + (uint32_t)x; }
+template <class T>
+__device__ __forceinline__ T ldIdx(const T* base, uint32_t i) {
+    return *(const T*)((const char*)base + i * (uint32_t)sizeof(T));
+}
 
 // L() motion_v1.glsl:42-44: clamp to edge == cv2 BORDER_REPLICATE == interp.py pad_to_block
 __device__ __forceinline__ float lumaAt(const float* img, int x, int y, int w, int h) {
-    return img[clampi(y, 0, h - 1) * w + clampi(x, 0, w - 1)];
+    return ldIdx(img, idx2(clampi(x, 0, w - 1), clampi(y, 0, h - 1), w));
 }
 
 __device__ __forceinline__ float2 fieldAt(const float2* f, int x, int y, int w, int h) {
-    return f[clampi(y, 0, h - 1) * w + clampi(x, 0, w - 1)];
+    return ldIdx(f, idx2(clampi(x, 0, w - 1), clampi(y, 0, h - 1), w));
 }
 
 // mix(x, y, a) as pinned by the oracle: x * (1 - a) + y * a
@@ -31,9 +41,9 @@ __device__ __forceinline__ float sampleL(const float* img, float px, float py, i
     const int X = (int)rintf(px * 32.0f), Y = (int)rintf(py * 32.0f);
     const int ix = X >> 5, iy = Y >> 5;
     const float fx = (float)(X & 31) * (1.0f / 32.0f), fy = (float)(Y & 31) * (1.0f / 32.0f);
-    const int x0 = clampi(ix, 0, w - 1), x1 = clampi(ix + 1, 0, w - 1);
-    const int y0 = clampi(iy, 0, h - 1) * w, y1 = clampi(iy + 1, 0, h - 1) * w;
-    const float p00 = img[y0 + x0], p01 = img[y0 + x1], p10 = img[y1 + x0], p11 = img[y1 + x1];
+    const uint32_t x0 = (uint32_t)clampi(ix, 0, w - 1), x1 = (uint32_t)clampi(ix + 1, 0, w - 1);
+    const uint32_t y0 = idx2(0, clampi(iy, 0, h - 1), w), y1 = idx2(0, clampi(iy + 1, 0, h - 1), w);
+    const float p00 = ldIdx(img, y0 + x0), p01 = ldIdx(img, y0 + x1), p10 = ldIdx(img, y1 + x0), p11 = ldIdx(img, y1 + x1);
     return p00 * ((1.0f - fy) * (1.0f - fx)) + p01 * ((1.0f - fy) * fx) + p10 * (fy * (1.0f - fx)) + p11 * (fy * fx);
 }
 
@@ -42,9 +52,9 @@ __device__ __forceinline__ float2 sampleF(const float2* f, float px, float py, i
     const int X = (int)rintf(px * 32.0f), Y = (int)rintf(py * 32.0f);
     const int ix = X >> 5, iy = Y >> 5;
     const float fx = (float)(X & 31) * (1.0f / 32.0f), fy = (float)(Y & 31) * (1.0f / 32.0f);
-    const int x0 = clampi(ix, 0, w - 1), x1 = clampi(ix + 1, 0, w - 1);
-    const int y0 = clampi(iy, 0, h - 1) * w, y1 = clampi(iy + 1, 0, h - 1) * w;
-    const float2 p00 = f[y0 + x0], p01 = f[y0 + x1], p10 = f[y1 + x0], p11 = f[y1 + x1];
+    const uint32_t x0 = (uint32_t)clampi(ix, 0, w - 1), x1 = (uint32_t)clampi(ix + 1, 0, w - 1);
+    const uint32_t y0 = idx2(0, clampi(iy, 0, h - 1), w), y1 = idx2(0, clampi(iy + 1, 0, h - 1), w);
+    const float2 p00 = ldIdx(f, y0 + x0), p01 = ldIdx(f, y0 + x1), p10 = ldIdx(f, y1 + x0), p11 = ldIdx(f, y1 + x1);
     const float w00 = (1.0f - fy) * (1.0f - fx), w01 = (1.0f - fy) * fx, w10 = fy * (1.0f - fx), w11 = fy * fx;
     return make_float2(p00.x * w00 + p01.x * w01 + p10.x * w10 + p11.x * w11,
                        p00.y * w00 + p01.y * w01 + p10.y * w10 + p11.y * w11);
@@ -58,20 +68,34 @@ __device__ __forceinline__ float2 halfVec(const float2* f, int x, int y, int S, 
     const float sx = ((float)x + 0.5f) * inv - 0.5f, sy = ((float)y + 0.5f) * inv - 0.5f;
     const float flx = floorf(sx), fly = floorf(sy);
     const float fx = sx - flx, fy = sy - fly;
-    const int i0x = clampi((int)flx, 0, fw - 1), i1x = clampi((int)flx + 1, 0, fw - 1);
-    const int i0y = clampi((int)fly, 0, fh - 1) * fw, i1y = clampi((int)fly + 1, 0, fh - 1) * fw;
-    const float2 top = mix2(f[i0y + i0x], f[i0y + i1x], fx);
-    const float2 bot = mix2(f[i1y + i0x], f[i1y + i1x], fx);
+    const uint32_t i0x = (uint32_t)clampi((int)flx, 0, fw - 1), i1x = (uint32_t)clampi((int)flx + 1, 0, fw - 1);
+    const uint32_t i0y = idx2(0, clampi((int)fly, 0, fh - 1), fw), i1y = idx2(0, clampi((int)fly + 1, 0, fh - 1), fw);
+    const float2 top = mix2(ldIdx(f, i0y + i0x), ldIdx(f, i0y + i1x), fx);
+    const float2 bot = mix2(ldIdx(f, i1y + i0x), ldIdx(f, i1y + i1x), fx);
     const float2 r = mix2(top, bot, fy);
     return make_float2(r.x * (float)S, r.y * (float)S);
 }
 
+// v of lane i + s at lane i (what a step of treeSum64 reads; right only for the lanes i < s that the step feeds):
+// DPP row_shl:s inside 16-lane rows for s <= 8, a swizzle (lane ^ 16 within groups of 32 lanes) for s = 16.
+template <int s>
+__device__ __forceinline__ float laneAbove(float v) {
+    static_assert(s == 1 || s == 2 || s == 4 || s == 8 || s == 16, "row shifts and the 16-lane swizzle");
+    if constexpr (s == 16) return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x401F));
+    else return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x100 | s, 0xF, 0xF, true));
+}
+
 // Wave-wide sum in the reference's shared-memory tree order (search.comp:45-49): lane i += lane i + s, s = 32 .. 1.
-// The total ends in lane 0 and is broadcast.
+// Only lanes i < s of a step feed the total, so a step needs lane i + s only there. The whole wavefront must be
+// active. The total ends in lane 0 and is read from there by every lane.
 __device__ __forceinline__ float treeSum64(float d) {
-#pragma unroll
-    for (int s = 32; s >= 1; s >>= 1) d = d + __shfl_down(d, s, 64);
-    return __shfl(d, 0, 64);
+    d = d + __shfl_down(d, 32, 64);
+    d = d + laneAbove<16>(d);
+    d = d + laneAbove<8>(d);
+    d = d + laneAbove<4>(d);
+    d = d + laneAbove<2>(d);
+    d = d + laneAbove<1>(d);
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(d), 0));
 }
 
 // Sum of the kSlots partial counters (the whole wavefront must be active): lane i reads slot i, integer wave reduce.

@@ -270,10 +270,17 @@ __device__ __forceinline__ void searchSeed(const SearchParams& P, int blk, bool 
 __device__ __forceinline__ void searchWindows(const SearchParams& P, float* A, float* C, int b0x, int b0y, int sx, int sy,
                                              int R, int lane) {
     const int T = 8 + 2 * R;
+    const int q = 64 / T, r = 64 - q * T;   // element i + 64 is (x + r, y + q), carried
+    int x = lane % T, y = lane / T;
     for (int i = lane; i < T * T; i += 64) {
-        const int x = i % T, y = i / T;
         A[i] = lumaAt(P.la, b0x - sx - R + x, b0y - sy - R + y, P.w, P.h);
         C[i] = lumaAt(P.lc, b0x + sx - R + x, b0y + sy - R + y, P.w, P.h);
+        x += r;
+        y += q;
+        if (x >= T) {
+            x -= T;
+            y++;
+        }
     }
 }
 
@@ -309,7 +316,7 @@ template <int MAXR>
 __device__ __forceinline__ void searchBody(const SearchParams& P, int bx0, int) {
     constexpr int MAXT = 8 + 2 * MAXR;
     __shared__ float ta[4][MAXT * MAXT], tc[4][MAXT * MAXT];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;   // wave: uniform
     const int nb = P.bw * P.bh;
     const int blk = bx0 * 4 + wave;
     const bool active = blk < nb;
@@ -349,7 +356,7 @@ __device__ __forceinline__ void searchBody(const SearchParams& P, int bx0, int) 
 __device__ __forceinline__ void search2Body(const SearchParams& P, int bx0, int) {
     constexpr int MAXT = 12;
     __shared__ float ta[kSearch2Blocks][MAXT * MAXT], tc[kSearch2Blocks][MAXT * MAXT];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;   // wave: uniform
     const int nb = P.bw * P.bh;
     const int R = P.R, D = 2 * R + 1;
     int sxs[2], sys[2];
@@ -554,7 +561,7 @@ __device__ __forceinline__ bool sampleRun4(const float* img, int w, int h, int q
     if (X3 != X0 + 96 || ix < 0 || ix + 4 > w - 1 || iy < 0 || iy + 1 > h - 1) return false;
     const float fx = (float)(X0 & 31) * (1.0f / 32.0f), fy = (float)(Y & 31) * (1.0f / 32.0f);
     const float w00 = (1.0f - fy) * (1.0f - fx), w01 = (1.0f - fy) * fx, w10 = fy * (1.0f - fx), w11 = fy * fx;
-    const float* r0 = img + iy * w + ix;
+    const float* r0 = img + idx2(ix, iy, w);   // a row pointer, so that the 5 taps of a row load as dwordx4 + dword
     const float* r1 = r0 + w;
     float a[5], b[5];
 #pragma unroll
@@ -598,8 +605,8 @@ __device__ __forceinline__ void pixErrRun(const PixParams& P, int k, int gx0, in
     if (k == 6) {   // zero vector: sampleL at an integer position is the texel (weights 1, 0, 0, 0)
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const int qx = clampi(gx0 + i, 0, P.w - 1);
-            e[i] = fabsf(P.la[qy * P.w + qx] - P.lc[qy * P.w + qx]);
+            const uint32_t q = idx2(clampi(gx0 + i, 0, P.w - 1), qy, P.w);
+            e[i] = fabsf(ldIdx(P.la, q) - ldIdx(P.lc, q));
         }
         return;
     }

@@ -143,7 +143,8 @@ __device__ __forceinline__ uint32_t blendRgba8(const SynthParams& P, uint32_t wa
     if (u == 0.0f && P.quarter) {
 #pragma unroll
         for (int c = 0; c < 3; c++)
-            r |= (uint32_t)((P.kA * byteOf(wa, c) + P.kC * byteOf(wc, c) + 2) >> 2) << (8 * c);
+            r |= ((__umul24((uint32_t)P.kA, (uint32_t)byteOf(wa, c)) + __umul24((uint32_t)P.kC, (uint32_t)byteOf(wc, c)) +
+                   2u) >> 2) << (8 * c);
         return r;
     }
     const float omu = 1.0f - u;
