@@ -148,6 +148,16 @@ __host__ __device__ __forceinline__ int xcdChunkTile(int i, int n, int chunk = k
         body(p, l % p.gx, l / p.gx);                                                                              \
     }
 
+// FGR_PASS_BATCH asking the register allocator for `waves` wavefronts per SIMD
+#define FGR_PASS_BATCH_OCC(name, Params, body, threads, waves)                                                    \
+    __global__ __launch_bounds__(threads) __attribute__((amdgpu_waves_per_eu(waves))) void name##_batch_kernel(   \
+        const Params* __restrict__ P, const uint32_t* __restrict__ work) {                                        \
+        const uint32_t w = ((const FGR_CONST_SPACE uint32_t*)work)[blockIdx.x];                                   \
+        const Params p = ((const FGR_CONST_SPACE Params*)P)[w >> kWorkShift];                                     \
+        const int l = (int)(w & kWorkMask);                                                                       \
+        body(p, l % p.gx, l / p.gx);                                                                              \
+    }
+
 // both kernels of a pass (spelled out rather than through the two macros above, so that a body naming a template with
 // several arguments passes through one macro expansion only)
 #define FGR_PASS(name, Params, body, threads)                                                                     \

@@ -599,8 +599,9 @@ __device__ __forceinline__ int2 candOffset(int k) {
 }
 
 // the errors of the run of 4 pixels (gx0 .. gx0 + 3, row qy) under candidate k. own / ownV: the run's errors and
-// vector under candidate 1, kept from k = 1 for runs inside the frame (ownOk).
-__device__ __forceinline__ void pixErrRun(const PixParams& P, int k, int gx0, int qy, float* e, float* own, float2& ownV,
+// vector under candidate 1, kept from k = 1 (computed first) for runs inside the frame (ownOk). Returns true when the
+// run's errors are own's because every pixel's vector equals ownV (then the errors are the same floats).
+__device__ __forceinline__ bool pixErrRun(const PixParams& P, int k, int gx0, int qy, float* e, float* own, float2& ownV,
                                           bool& ownOk) {
     if (k == 6) {   // zero vector: sampleL at an integer position is the texel (weights 1, 0, 0, 0)
 #pragma unroll
@@ -608,7 +609,7 @@ __device__ __forceinline__ void pixErrRun(const PixParams& P, int k, int gx0, in
             const uint32_t q = idx2(clampi(gx0 + i, 0, P.w - 1), qy, P.w);
             e[i] = fabsf(ldIdx(P.la, q) - ldIdx(P.lc, q));
         }
-        return;
+        return false;
     }
     const bool inside = gx0 >= 0 && gx0 + 3 <= P.w - 1;
     if (k == 0 && inside) {   // pixCand(0, ..) of each pixel with the four block vectors loaded once
@@ -618,15 +619,29 @@ __device__ __forceinline__ void pixErrRun(const PixParams& P, int k, int gx0, in
         const int ix = (int)flx, iy = (int)fly;
         const float2 f00 = fieldAt(P.bf, ix, iy, P.bw, P.bh), f10 = fieldAt(P.bf, ix + 1, iy, P.bw, P.bh);
         const float2 f01 = fieldAt(P.bf, ix, iy + 1, P.bw, P.bh), f11 = fieldAt(P.bf, ix + 1, iy + 1, P.bw, P.bh);
+        auto cand0 = [&](int qx) {
+            const float fx = (((float)qx + 0.5f) / 8.0f - 0.5f) - flx;
+            return mix2(mix2(f00, f10, fx), mix2(f01, f11, fx), fy);
+        };
+        bool same = ownOk;
 #pragma unroll
         for (int i = 0; i < 4; i++) {
+            const float2 v = cand0(gx0 + i);
+            same = same && v.x == ownV.x && v.y == ownV.y;
+        }
+        if (same) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) e[i] = own[i];
+            return true;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {   // the vectors formed again rather than held in registers
             const int qx = gx0 + i;
-            const float fx = (((float)qx + 0.5f) / 8.0f - 0.5f) - flx;
-            const float2 v = mix2(mix2(f00, f10, fx), mix2(f01, f11, fx), fy);
+            const float2 v = cand0(qx);
             e[i] = fabsf(sampleL(P.la, (float)qx - v.x, (float)qy - v.y, P.w, P.h) -
                          sampleL(P.lc, (float)qx + v.x, (float)qy + v.y, P.w, P.h));
         }
-        return;
+        return false;
     }
     if (k > 0 && inside) {
         const int2 o = candOffset(k);
@@ -634,7 +649,7 @@ __device__ __forceinline__ void pixErrRun(const PixParams& P, int k, int gx0, in
         if (k > 1 && ownOk && v.x == ownV.x && v.y == ownV.y) {
 #pragma unroll
             for (int i = 0; i < 4; i++) e[i] = own[i];
-            return;
+            return true;
         }
         float sa[4], sc[4];
         if (sampleRun4(P.la, P.w, P.h, gx0, qy, v, -1.0f, sa) && sampleRun4(P.lc, P.w, P.h, gx0, qy, v, 1.0f, sc)) {
@@ -654,7 +669,7 @@ __device__ __forceinline__ void pixErrRun(const PixParams& P, int k, int gx0, in
             ownV = v;
             ownOk = true;
         }
-        return;
+        return false;
     }
 #pragma unroll 1
     for (int i = 0; i < 4; i++) {
@@ -663,10 +678,16 @@ __device__ __forceinline__ void pixErrRun(const PixParams& P, int k, int gx0, in
         e[i] = fabsf(sampleL(P.la, (float)qx - v.x, (float)qy - v.y, P.w, P.h) -
                      sampleL(P.lc, (float)qx + v.x, (float)qy + v.y, P.w, P.h));
     }
+    return false;
 }
 
 // NT = 512 (the per-pair kernel: few workgroups, so latency decides): one run and 2 pixels a thread. NT = 256 (the
 // batched kernel: many workgroups, so occupancy decides): up to 2 runs and 4 pixels a thread, fewer registers.
+//
+// Candidates are taken in the order 1, 0, 2, 3, 4, 5, 6 (candidate 1's errors are what the others reuse) with the
+// reference's first-minimum rule kept: 0 replaces 1 on a tie (c0 <= c1), every later one needs strict <. When every
+// run of the tile's region reused candidate 1's errors for candidate k (a workgroup vote), every window sum is
+// candidate 1's, so candidate 0 costs c1 - smoothBonus and candidates 2-5 cannot win: the sums are skipped.
 template <int NT>
 __device__ __forceinline__ void pixfieldBody(const PixParams& P, int gbx, int gby) {
     static_assert(NT == 256 || NT == 512, "two layouts");
@@ -676,19 +697,22 @@ __device__ __forceinline__ void pixfieldBody(const PixParams& P, int gbx, int gb
     const int tid = threadIdx.x;
     const int x0 = gbx * kPfTile, y0 = gby * kPfTile;
     const int cx = tid & 31, r0 = (tid >> 5) * kRows;
-    float best[kRows];
+    float best[kRows], c1[kRows];
     int bk[kRows];
 #pragma unroll
     for (int j = 0; j < kRows; j++) {
         best[j] = 0.0f;
+        c1[j] = 0.0f;
         bk[j] = 0;
     }
-    // candidate 1's errors of this thread's runs (r = tid and tid + NT), kept for candidates 2-5
+    // candidate 1's errors of this thread's runs (r = tid and tid + NT), kept for candidates 0 and 2-5
     float own0[4] = {0.0f, 0.0f, 0.0f, 0.0f}, own1[4] = {0.0f, 0.0f, 0.0f, 0.0f};
     float2 ownV0 = make_float2(0.0f, 0.0f), ownV1 = make_float2(0.0f, 0.0f);
     bool ownOk0 = false, ownOk1 = false;
 #pragma unroll 1
-    for (int k = 0; k < 7; k++) {
+    for (int step = 0; step < 7; step++) {
+        const int k = step == 0 ? 1 : (step == 1 ? 0 : step);
+        bool reusedAll = true;
 #pragma unroll 1
         for (int r = tid; r < kPfRuns; r += NT) {
             const bool s1 = r >= NT;   // one code path for both runs; the kept values are selected, not indexed
@@ -699,7 +723,8 @@ __device__ __forceinline__ void pixfieldBody(const PixParams& P, int gbx, int gb
             bool ownOk = s1 ? ownOk1 : ownOk0;
             const int ry = r / (kPfRW / 4), rc = r - ry * (kPfRW / 4);
             float e[4];
-            pixErrRun(P, k, x0 - 4 + 4 * rc, clampi(y0 - 2 + ry, 0, P.h - 1), e, own, ownV, ownOk);
+            reusedAll = pixErrRun(P, k, x0 - 4 + 4 * rc, clampi(y0 - 2 + ry, 0, P.h - 1), e, own, ownV, ownOk) &&
+                        reusedAll;
             *(float4*)&errs[ry * kPfRW + 4 * rc] = make_float4(e[0], e[1], e[2], e[3]);
             if (k == 1) {
 #pragma unroll
@@ -713,7 +738,20 @@ __device__ __forceinline__ void pixfieldBody(const PixParams& P, int gbx, int gb
                 ownOk1 = s1 ? ownOk : ownOk1;
             }
         }
-        __syncthreads();
+        // a barrier (errs complete) and the vote: only candidates 0 and 2-5 ever reuse, so k = 1 and 6 never skip
+        if (__syncthreads_and(reusedAll)) {
+            if (k == 0) {
+#pragma unroll
+                for (int j = 0; j < kRows; j++) {
+                    const float c = c1[j] - P.smooth;
+                    if (c <= best[j]) {
+                        best[j] = c;
+                        bk[j] = 0;
+                    }
+                }
+            }
+            continue;
+        }
         // horizontal 5-sums, 4 output columns a task: region columns c0 + 2 .. c0 + 10 of one row
         for (int t = tid; t < kPfRH * (kPfTile / 4); t += NT) {
             const int hy = t >> 3, c0 = (t & 7) * 4;
@@ -744,7 +782,11 @@ __device__ __forceinline__ void pixfieldBody(const PixParams& P, int gbx, int gb
             float c = (float)(vs * (1.0 / 25.0));
             if (k == 0) c = c - P.smooth;
             if (k == 6) c = c - P.zero;
-            if (k == 0 || c < best[j]) {
+            if (k == 1) {
+                c1[j] = c;
+                best[j] = c;
+                bk[j] = 1;
+            } else if (k == 0 ? c <= best[j] : c < best[j]) {
                 best[j] = c;
                 bk[j] = k;
             }
@@ -784,7 +826,7 @@ FGR_PASS(search12, SearchParams, searchBody<12>, 256)
 FGR_PASS(median3, Median3Params, median3Body, 256)
 FGR_PASS(subpix, SubpixParams, subpixBody, 64)
 FGR_PASS_SINGLE(pixfield, PixParams, pixfieldBody<kPfThreads>, kPfThreads)
-FGR_PASS_BATCH(pixfield, PixParams, pixfieldBody<kPfBatchThreads>, kPfBatchThreads)
+FGR_PASS_BATCH_OCC(pixfield, PixParams, pixfieldBody<kPfBatchThreads>, kPfBatchThreads, 6)
 
 // ---------------------------------------------------------------------------------------------------------- launches
 

@@ -72,6 +72,19 @@ def test_mixed_batch_matches_per_pair_and_oracle(use_graph):
     b.close()
 
 
+def test_uniform_pairs_batch_matches_the_oracle():
+    # the batched pixel field on uniform block fields (a static pair, an integer pan): own-block reuse over whole tiles
+    pairs = [scenes.pair_shift(11, 320, 192, 0, 0), scenes.pair_shift(12, 320, 192, 4, -2)]
+    b, items = make_batch(pairs, [2, 2])
+    b.run()
+    torch.cuda.synchronize()
+    for p, (a, c) in enumerate(pairs):
+        est = v1.estimate(a, c)
+        check_estimate(Internals(b.debug_internal(p)), est)
+        check_output(items[p][3][0].cpu().numpy(), a, c, est, 0.5)
+    b.close()
+
+
 def test_graph_replays_match_eager():
     """Replay the captured graph several times with eager work and synchronisations in between; every replay must
     reproduce the eager outputs exactly (the graph holds no memset nodes: the counters are cleared by a kernel)."""

@@ -142,6 +142,19 @@ def test_static_pair_is_exact():
     fi.close()
 
 
+@pytest.mark.parametrize("shift", [(0, 0), (4, -2)])
+def test_uniform_fields_match_the_oracle(shift):
+    # a static pair and an integer pan: block fields uniform away from the borders, so the interior pixel-field tiles
+    # have every candidate's vector equal to the own block's (the kernel reuses the own-block errors and window sums)
+    a, c = scenes.pair_shift(11, 320, 192, *shift)
+    est = v1.estimate(a, c)
+    fi, I, outs, _ = run(a, c, (0.5, 0.25))
+    check_estimate(I, est)
+    for o, t in zip(outs, (0.5, 0.25)):
+        check_output(o, a, c, est, t)
+    fi.close()
+
+
 def test_bgra_order_is_symmetric():
     """README.md:144: RGBA and BGRA give byte-identical results (channels swapped)."""
     a, c = scenes.pair_translate(6, w=320, h=180)
