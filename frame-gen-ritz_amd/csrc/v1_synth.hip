@@ -161,6 +161,44 @@ __device__ __forceinline__ uint32_t blendRgba8(const SynthParams& P, uint32_t wa
     return r;
 }
 
+// The warp pairs of the 4 region pixels at columns ex0 .. ex0 + 3 (frame columns x0 - AP + ex0 + i, clamped) of frame
+// row y (already clamped), from the interpolated field rows hrow. The shared-map form of warpRun4 when every lane of
+// the wavefront can take it (no divergent second path), else pixel by pixel; the same values either way.
+template <int S, bool UI>
+__device__ __forceinline__ void warpRegionRun(const SynthParams& P, const float2* hrow, int ty0, float kA32, float kC32,
+                                              int x0, int ex0, int y, uint32_t* oa, uint32_t* oc) {
+    const int yc = min(y, S * P.fh - 1);
+    const float2* h = hrow + (FlowMap<S>::index(yc) - ty0) * EW + ex0;
+    const float fy = FlowMap<S>::frac(yc);
+    float2 r[4];
+    int xs[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        xs[i] = med3i(x0 - AP + ex0 + i, 0, P.W - 1);
+        // a clamped column repeats the edge pixel: its hrow entry is that pixel's (clamped the same way)
+        r[i] = mix2(h[i], h[i + EW], fy);
+    }
+    const bool same = xs[3] == xs[0] + 3 && r[1].x == r[0].x && r[1].y == r[0].y && r[2].x == r[0].x &&
+                      r[2].y == r[0].y && r[3].x == r[0].x && r[3].y == r[0].y;
+    const RunMap ma = runMap(P.W, xs[0], y, kA32, r[0]), mc = runMap(P.W, xs[0], y, -kC32, r[0]);
+    if (__all(same && ma.ok && mc.ok)) {
+        warpRun4(P.a.p, (uint32_t)P.a.pitch, P.H, ma, oa);
+        warpRun4(P.c.p, (uint32_t)P.c.pitch, P.H, mc, oc);
+        if (UI) {   // the run's maps are X0 + 32 i at one Y
+#pragma unroll
+            for (int i = 0; i < 4; i++) uiReject(P, ma.X0 + 32 * i, ma.Y, mc.X0 + 32 * i, mc.Y, oa[i], oc[i]);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            Rgb a, c;
+            warpPair<UI>(P, kA32, kC32, xs[i], y, r[i], a, c);
+            oa[i] = packRgb(a);
+            oc[i] = packRgb(c);
+        }
+    }
+}
+
 // One 64 x RT tile (RT = 16, 32 or 64 rows; 4, 8 or 16 pixels a thread) of the synthesised frame (passes 9-11 fused): field tile (+ lookup at t), mismatch of the tile and
 // its apron, 7x7 box, trust, untrusted count, blend. Scene cut: a copy of the nearer frame.
 //
@@ -170,7 +208,7 @@ __device__ __forceinline__ uint32_t blendRgba8(const SynthParams& P, uint32_t wa
 template <int S, int RT, bool UI = false>
 __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, int tileY, int nTilesX) {
     constexpr int FTW = SynthTile<S, RT>::FTW, FTH = SynthTile<S, RT>::FTH;
-    constexpr int TH = RT, EH = TileRows<RT>::EH, NAPRON = TileRows<RT>::NAPRON, RK = RT / 16;
+    constexpr int TH = RT, EH = TileRows<RT>::EH, RK = RT / 16;
     __shared__ float2 ft[FTW * FTH];
     __shared__ float2 hrow[FTH * EW];
     __shared__ __align__(16) int es[EH * EW];
@@ -188,7 +226,6 @@ __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, 
         return;
     }
     if (tid == 0) cnt = 0u;
-    const int ylim = S * P.fh - 1;
     const int ty0 = loadFieldTile<S, RT>(P, x0, y0, ft, hrow);
     const float kA32 = (float)(32 * S) * (2.0f * P.wC), kC32 = (float)(32 * S) * (2.0f * P.wA);
     // interior pixels in runs of 4 (columns 4 rc .. 4 rc + 3, rows rr + 16 k): warps kept in registers (packed)
@@ -197,58 +234,33 @@ __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, 
 #pragma unroll
     for (int k = 0; k < RK; k++) {
         const int y = min(y0 + rr + 16 * k, P.H - 1);
-        const int yc = min(y, ylim);
-        const float2* h = hrow + (FlowMap<S>::index(yc) - ty0) * EW + 4 * rc + AP;
-        const float fy = FlowMap<S>::frac(yc);
-        float2 r[4];
-        int xs[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            xs[i] = min(x0 + 4 * rc + i, P.W - 1);
-            // a clamped column repeats the last pixel: its hrow entry is that pixel's (clamped the same way)
-            r[i] = mix2(h[i], h[i + EW], fy);
-        }
-        const bool same = xs[3] == xs[0] + 3 && r[1].x == r[0].x && r[1].y == r[0].y && r[2].x == r[0].x &&
-                          r[2].y == r[0].y && r[3].x == r[0].x && r[3].y == r[0].y;
         uint32_t* oa = wa + 4 * k;
         uint32_t* oc = wc + 4 * k;
-        const RunMap ma = runMap(P.W, xs[0], y, kA32, r[0]), mc = runMap(P.W, xs[0], y, -kC32, r[0]);
-        // the shared-map form only when every lane of the wavefront can take it (no divergent second path)
-        if (__all(same && ma.ok && mc.ok)) {
-            warpRun4(P.a.p, (uint32_t)P.a.pitch, P.H, ma, oa);
-            warpRun4(P.c.p, (uint32_t)P.c.pitch, P.H, mc, oc);
-            if (UI) {   // the run's maps are X0 + 32 i at one Y
-#pragma unroll
-                for (int i = 0; i < 4; i++) uiReject(P, ma.X0 + 32 * i, ma.Y, mc.X0 + 32 * i, mc.Y, oa[i], oc[i]);
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                Rgb a, c;
-                warpPair<UI>(P, kA32, kC32, xs[i], y, r[i], a, c);
-                oa[i] = packRgb(a);
-                oc[i] = packRgb(c);
-            }
-        }
+        warpRegionRun<S, UI>(P, hrow, ty0, kA32, kC32, x0, 4 * rc + AP, y, oa, oc);
 #pragma unroll
         for (int i = 0; i < 4; i++) es[(rr + 16 * k + AP) * EW + 4 * rc + i + AP] = emaxPacked(oa[i], oc[i]);
     }
-    // apron: rows 0..2 and EH-3..EH-1 in full, then columns 0..2 and EW-3..EW-1 of the rows between
-    for (int j = tid; j < NAPRON; j += 256) {
-        int ex, ey;
-        if (j < 6 * EW) {
-            const int r = j / EW;
-            ey = r < 3 ? r : r + TH;
-            ex = j - r * EW;
+    // apron in runs of 4 as well: rows 0..2 and EH-3..EH-1 as 18 runs a row (the last at column EW - 4, overlapping
+    // its neighbour), then columns 0..3 and EW-4..EW-1 of each row between (overlapping the interior's first and last
+    // column). A pixel taken twice gets the same value twice.
+    constexpr int kRowRuns = (EW + 3) / 4, kApRuns = 2 * AP * kRowRuns + 2 * TH;
+    static_assert(kRowRuns == 18 && 2 * AP * kRowRuns < 256, "one run a thread for the apron rows");
+    for (int j = tid; j < kApRuns; j += 256) {
+        int ex0, ey;
+        if (j < 2 * AP * kRowRuns) {
+            const int r = j / kRowRuns, c = j - r * kRowRuns;
+            ey = r < AP ? r : r + TH;
+            ex0 = min(4 * c, EW - 4);
         } else {
-            const int jj = j - 6 * EW, q = jj / 6, c = jj - 6 * q;
-            ey = AP + q;
-            ex = c < 3 ? c : TW + c;
+            const int jj = j - 2 * AP * kRowRuns;
+            ey = AP + (jj >> 1);
+            ex0 = (jj & 1) ? EW - 4 : 0;
         }
-        const int x = med3i(x0 - AP + ex, 0, P.W - 1), y = med3i(y0 - AP + ey, 0, P.H - 1);
-        Rgb a, c;
-        warpPair<UI>(P, kA32, kC32, x, y, fieldAtRegion<S>(P, hrow, ty0, ex, y), a, c);
-        es[ey * EW + ex] = emaxRgb(a, c);
+        const int y = med3i(y0 - AP + ey, 0, P.H - 1);
+        uint32_t oa[4], oc[4];
+        warpRegionRun<S, UI>(P, hrow, ty0, kA32, kC32, x0, ex0, y, oa, oc);
+#pragma unroll
+        for (int i = 0; i < 4; i++) es[ey * EW + ex0 + i] = emaxPacked(oa[i], oc[i]);
     }
     __syncthreads();
     boxRows7<EH>(es, hs, tid);

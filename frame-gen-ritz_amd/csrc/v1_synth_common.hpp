@@ -51,7 +51,7 @@ __device__ __forceinline__ float trustOf(const SynthParams& P, int S) {
 
 // lookup.comp:10-19 (interp.py:194-205): the field at time t at flow pixel (qx, qy): g <- pf(q - s g) twice
 __device__ __forceinline__ float2 lookupAt(const float2* pf, int fw, int fh, float s, int qx, int qy) {
-    float2 g = pf[qy * fw + qx];
+    float2 g = ldIdx(pf, idx2(qx, qy, fw));
 #pragma unroll
     for (int k = 0; k < 2; k++) {
         const float px = (float)qx - s * g.x, py = (float)qy - s * g.y;
@@ -64,6 +64,7 @@ __device__ __forceinline__ float2 lookupAt(const float2* pf, int fw, int fh, flo
 template <int S>
 struct FlowMap {
     static constexpr int kShift = S == 1 ? 1 : (S == 2 ? 2 : 3);
+    static constexpr int indexC(int x) { return (2 * x + 1 - S) >> kShift; }
     static __device__ __forceinline__ int index(int x) { return (2 * x + 1 - S) >> kShift; }
     static __device__ __forceinline__ float frac(int x) {
         return (float)((2 * x + 1 - S) & (2 * S - 1)) * (1.0f / (float)(2 * S));
@@ -72,8 +73,12 @@ struct FlowMap {
 
 template <int S, int TH_ = TH>
 struct SynthTile {
-    // field tile (flow res) covering the tile and its apron, and its rows interpolated to full-res columns
-    static constexpr int FTW = (EW + S - 1) / S + 3, FTH = (TileRows<TH_>::EH + S - 1) / S + 3;
+    // field tile (flow res) covering the tile and its apron, and its rows interpolated to full-res columns: tiles start
+    // at multiples of TW columns and TH_ rows, so 2 x0 and 2 y0 are multiples of 2S and the flow cells that the region
+    // x0 - AP .. x0 + TW + AP - 1 interpolates are index(x0 - AP) .. index(x0 + TW + AP - 1) + 1 (fewer at a clamped
+    // frame edge); the same for rows
+    static constexpr int FTW = FlowMap<S>::indexC(TW + AP - 1) - FlowMap<S>::indexC(-AP) + 2;
+    static constexpr int FTH = FlowMap<S>::indexC(TH_ + AP - 1) - FlowMap<S>::indexC(-AP) + 2;
 };
 
 // The field tile of an output tile at (x0, y0) and its apron (clamped to the frame, then to S * fw - 1), looked up at t
@@ -87,10 +92,30 @@ __device__ __forceinline__ int loadFieldTile(const SynthParams& P, int x0, int y
     const int xlim = S * P.fw - 1, ylim = S * P.fh - 1;
     const int xmin = min(max(x0 - AP, 0), xlim), ymin = min(max(y0 - AP, 0), ylim);
     const int tx0 = FlowMap<S>::index(xmin), ty0 = FlowMap<S>::index(ymin);
-    for (int i = tid; i < FTW * FTH; i += 256) {
+    // the thread's (at most) IT entries side by side, so that their dependent loads overlap: the index is clamped
+    // rather than branched on (an entry past the end repeats the last one and is not stored)
+    constexpr int N = FTW * FTH, IT = (N + 255) / 256;
+    float2 v[IT];
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int i = min(tid + 256 * it, N - 1);
         const int qx = med3i(tx0 + i % FTW, 0, P.fw - 1), qy = med3i(ty0 + i / FTW, 0, P.fh - 1);
-        ft[i] = P.lookup ? lookupAt(P.field, P.fw, P.fh, P.s, qx, qy) : P.field[qy * P.fw + qx];
+        v[it] = ldIdx(P.field, idx2(qx, qy, P.fw));
     }
+    if (P.lookup) {   // lookup.comp:10-19, g <- pf(q - s g) twice, the IT entries interleaved step by step
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+#pragma unroll
+            for (int it = 0; it < IT; it++) {
+                const int i = min(tid + 256 * it, N - 1);
+                const int qx = med3i(tx0 + i % FTW, 0, P.fw - 1), qy = med3i(ty0 + i / FTW, 0, P.fh - 1);
+                v[it] = sampleF(P.field, (float)qx - P.s * v[it].x, (float)qy - P.s * v[it].y, P.fw, P.fh);
+            }
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < IT; it++)
+        if (tid + 256 * it < N) ft[tid + 256 * it] = v[it];
     __syncthreads();
     for (int i = tid; i < FTH * EW; i += 256) {
         const int j = i / EW, ex = i - j * EW;
