@@ -66,7 +66,25 @@ __device__ __forceinline__ RunMap runMap(int W, int x, int y, float k, float2 r)
     return m;
 }
 
-// the four warps of a run whose map passed runMap (ok)
+// Two 16-bit lanes of one register: a b + c and a b per lane (no carry between lanes; every value here < 2^16), and
+// a.lo b.lo + a.hi b.hi + c in 32 bits
+typedef unsigned short U16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pkMad16(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(U16x2, a) * __builtin_bit_cast(U16x2, b) +
+                                            __builtin_bit_cast(U16x2, c));
+}
+__device__ __forceinline__ uint32_t pkMul16(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(U16x2, a) * __builtin_bit_cast(U16x2, b));
+}
+__device__ __forceinline__ uint32_t dot2u16(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_udot2(__builtin_bit_cast(U16x2, a), __builtin_bit_cast(U16x2, b), c, false);
+}
+
+// the four warps of a run whose map passed runMap (ok). The run shares fy, so each of its 5 texel columns is blended
+// vertically once, two channels a register: V = (32 - fy) top + fy bottom <= 8160 per 16-bit lane ([R, B] and [G, A]).
+// Each pixel then takes one dot product per channel over its two columns with the horizontal weights x 64 and
+// 64 x 512 added: 64 ((32-fx)(32-fy) p00 + fx(32-fy) p01 + (32-fx)fy p10 + fx fy p11 + 512) < 2^24, whose byte 2 is
+// warpRgba8's (sum + 512) >> 10 -- the same integers, regrouped.
 __device__ __forceinline__ void warpRun4(const uint8_t* base, uint32_t pitch, int H, const RunMap& m, uint32_t* out) {
     const int ix = m.X0 >> 5, iy = m.Y >> 5;
     const uint32_t fx = (uint32_t)(m.X0 & 31), fy = (uint32_t)(m.Y & 31);
@@ -78,17 +96,22 @@ __device__ __forceinline__ void warpRun4(const uint8_t* base, uint32_t pitch, in
     const U32x4 b4 = *(const U32x4*)r1;
     const uint32_t b5 = *(const uint32_t*)(r1 + 16);
     const uint32_t ta[5] = {a4.x, a4.y, a4.z, a4.w, a5}, tb[5] = {b4.x, b4.y, b4.z, b4.w, b5};
-    const uint32_t gx = 32u - fx, gy = 32u - fy;
-    const uint32_t w00 = gx * gy, w01 = fx * gy, w10 = gx * fy, w11 = fx * fy;
+    const uint32_t gy2 = (32u - fy) * 0x10001u, fy2 = fy * 0x10001u;
+    const uint32_t hw = ((32u - fx) << 6) | (fx << 22);   // [64 (32 - fx), 64 fx]
+    uint32_t vRB[5], vGA[5];
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+        vRB[j] = pkMad16(ta[j] & 0x00FF00FFu, gy2, pkMul16(tb[j] & 0x00FF00FFu, fy2));
+        vGA[j] = pkMad16(__builtin_amdgcn_perm(0u, ta[j], 0x0C030C01u), gy2,
+                         pkMul16(__builtin_amdgcn_perm(0u, tb[j], 0x0C030C01u), fy2));
+    }
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        const uint32_t p00 = ta[i], p01 = ta[i + 1], p10 = tb[i], p11 = tb[i + 1];
-        const uint32_t vr = (w00 * (p00 & 255u) + w01 * (p01 & 255u) + w10 * (p10 & 255u) + w11 * (p11 & 255u) + 512u) >> 10;
-        const uint32_t vg = (w00 * ((p00 >> 8) & 255u) + w01 * ((p01 >> 8) & 255u) + w10 * ((p10 >> 8) & 255u) +
-                             w11 * ((p11 >> 8) & 255u) + 512u) >> 10;
-        const uint32_t vb = (w00 * ((p00 >> 16) & 255u) + w01 * ((p01 >> 16) & 255u) + w10 * ((p10 >> 16) & 255u) +
-                             w11 * ((p11 >> 16) & 255u) + 512u) >> 10;
-        out[i] = vr | (vg << 8) | (vb << 16);
+        // [V(i), V(i + 1)] of R (low lanes of vRB), B (high lanes) and G (low lanes of vGA)
+        const uint32_t sr = dot2u16(__builtin_amdgcn_perm(vRB[i + 1], vRB[i], 0x05040100u), hw, 32768u);
+        const uint32_t sb = dot2u16(__builtin_amdgcn_perm(vRB[i + 1], vRB[i], 0x07060302u), hw, 32768u);
+        const uint32_t sg = dot2u16(__builtin_amdgcn_perm(vGA[i + 1], vGA[i], 0x05040100u), hw, 32768u);
+        out[i] = __builtin_amdgcn_perm(sg, sr, 0x0C0C0602u) | (sb & 0x00FF0000u);   // bytes R, G, B, alpha 0
     }
 }
 
