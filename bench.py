@@ -13,7 +13,9 @@ Extra fields:
   * `roofline` of the dominant kernel, the batched synth (one launch = the k-th synth of all 8 pairs): algorithmic
     bytes per launch (3F + P per pair, 2F for the scene-cut pair's copy) over its average launch duration, from HIP
     events around each launch on the launch stream; `traffic` = the HBM bytes per launch that rocprofv3's FETCH_SIZE
-    and WRITE_SIZE counters read (from the committed summary under profiles/), or null;
+    and WRITE_SIZE counters read (from the committed summary under profiles/), or null; `valu_issue_frac` = the
+    fraction of the GPU's VALU issue slots the launch's wavefronts filled (SQ_INSTS_VALU over 256 CUs x cycles, from
+    the committed counter summary), the bound this exact integer arithmetic meets before HBM does;
   * `copy_probe`: a device-to-device copy of known size, the bandwidth a plain streaming kernel reaches;
   * `cpu_baseline`: the numpy oracle on 4 pairs of the workload, one core.
 """
@@ -59,6 +61,17 @@ def measured_traffic(kernel):
     for line in open(path):
         if line.startswith("traffic_bytes_per_launch:"):
             return int(line.split(":")[1])
+    return None
+
+
+def measured_valu_issue(kernel):
+    """VALU issue fraction of `kernel`'s launch from the committed rocprofv3 SQ counter summary, or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_1080p_4x_valu.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("valu_issue_frac:"):
+            return float(line.split(":")[1])
     return None
 
 
@@ -351,6 +364,7 @@ def main():
     traffic = measured_traffic(kernel)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "valu_issue_frac": measured_valu_issue(kernel),
                 "kernel": kernel, "launches": len(synth_ns), "avg_launch_us": round(avg_ns / 1e3, 3),
                 "bytes_per_launch": int(bytes_per_launch),
                 "plain_synth_avg_us": round(sum(plain_ns) / len(plain_ns) / 1e3, 3)}
