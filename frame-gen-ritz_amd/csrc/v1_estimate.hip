@@ -229,38 +229,51 @@ __device__ __forceinline__ void searchSeed(const SearchParams& P, int blk, bool 
     const int px = P.parent ? min(bx / 2, P.pw - 1) : 0, py = P.parent ? min(by / 2, P.ph - 1) : 0;
     const int ssy = (by & 1) == 0 ? -1 : 1, ssx = (bx & 1) == 0 ? -1 : 1;
     const int nc = P.parent ? (P.extraSeed ? 7 : 6) : 2;
-    float best = 0.0f;
-    for (int kk = 0; kk < nc; kk++) {
+    // the candidates' vectors first, then all their texel pairs loaded together (one memory latency, not one per
+    // candidate), then the tree sums and the first-minimum choice in candidate order. Slots past nc repeat the zero
+    // vector and are not considered.
+    constexpr int kMaxSeeds = 7;
+    int cxs[kMaxSeeds], cys[kMaxSeeds];
+#pragma unroll
+    for (int kk = 0; kk < kMaxSeeds; kk++) {
         const int k = P.parent ? kk : (kk == 0 ? 4 : 6);   // coarsest level: zero, then the proposal
-        int cx, cy;
-        if (k < 4) {
+        int cx = 0, cy = 0;
+        if (kk >= nc) {
+        } else if (k < 4) {
             const int ox = (k == 2 || k == 3) ? ssx : 0, oy = (k == 1 || k == 3) ? ssy : 0;
             const float2 v = P.parent[clampi(py + oy, 0, P.ph - 1) * P.pw + clampi(px + ox, 0, P.pw - 1)];
             cx = 2 * (int)rintf(v.x);
             cy = 2 * (int)rintf(v.y);
-        } else if (k == 4) {
-            cx = 0;
-            cy = 0;
         } else if (k == 5) {
             cx = (int)rintf(__uint_as_float(P.med[0]));
             cy = (int)rintf(__uint_as_float(P.med[1]));
-        } else {
+        } else if (k == 6) {
             const float2 v = P.extraSeed[active ? blk : 0];
             cx = (int)rintf(v.x);
             cy = (int)rintf(v.y);
         }
-        const float d = fabsf(lumaAt(P.la, qx - cx, qy - cy, P.w, P.h) - lumaAt(P.lc, qx + cx, qy + cy, P.w, P.h));
-        float c = treeSum64(d) * (1.0f / 64.0f);
+        cxs[kk] = cx;
+        cys[kk] = cy;
+    }
+    float d[kMaxSeeds];
+#pragma unroll
+    for (int kk = 0; kk < kMaxSeeds; kk++)
+        d[kk] = fabsf(lumaAt(P.la, qx - cxs[kk], qy - cys[kk], P.w, P.h) - lumaAt(P.lc, qx + cxs[kk], qy + cys[kk], P.w, P.h));
+    float best = 0.0f;
+#pragma unroll
+    for (int kk = 0; kk < kMaxSeeds; kk++) {
+        if (kk >= nc) break;
+        float c = treeSum64(d[kk]) * (1.0f / 64.0f);
         if (kk == 0) {
             best = c;
-            sx = cx;
-            sy = cy;
+            sx = cxs[kk];
+            sy = cys[kk];
         } else {
             c = c + P.lam;
             if (c < best) {
                 best = c;
-                sx = cx;
-                sy = cy;
+                sx = cxs[kk];
+                sy = cys[kk];
             }
         }
     }
