@@ -163,11 +163,13 @@ __device__ __forceinline__ uint32_t blendRgba8(const SynthParams& P, uint32_t wa
     if (u == 0.0f && P.mid)   // (a + c + 1) >> 1 per byte (v_lerp_u8), alpha from the nearer frame
         return (__builtin_amdgcn_lerp(wa, wc, 0x01010101u) & 0x00FFFFFFu) | (nr & 0xFF000000u);
     uint32_t r = nr & 0xFF000000u;
-    if (u == 0.0f && P.quarter) {
+    if (u == 0.0f && P.quarter) {   // per channel the byte pair (a_c, c_c) dotted with (kA, kC), plus 2, >> 2
+        const uint32_t k = (uint32_t)P.kA | ((uint32_t)P.kC << 8);
 #pragma unroll
-        for (int c = 0; c < 3; c++)
-            r |= ((__umul24((uint32_t)P.kA, (uint32_t)byteOf(wa, c)) + __umul24((uint32_t)P.kC, (uint32_t)byteOf(wc, c)) +
-                   2u) >> 2) << (8 * c);
+        for (int c = 0; c < 3; c++) {
+            const uint32_t ac = __builtin_amdgcn_perm(wc, wa, 0x0C0C0400u + 0x0101u * (uint32_t)c);   // [a_c, c_c, 0, 0]
+            r |= (__builtin_amdgcn_udot4(ac, k, 2u, false) >> 2) << (8 * c);
+        }
         return r;
     }
     const float omu = 1.0f - u;
@@ -232,10 +234,16 @@ template <int S, int RT, bool UI = false>
 __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, int tileY, int nTilesX) {
     constexpr int FTW = SynthTile<S, RT>::FTW, FTH = SynthTile<S, RT>::FTH;
     constexpr int TH = RT, EH = TileRows<RT>::EH, RK = RT / 16;
-    __shared__ float2 ft[FTW * FTH];
-    __shared__ float2 hrow[FTH * EW];
-    __shared__ __align__(16) int es[EH * EW];
-    __shared__ __align__(16) uint16_t hs[EH * TW];
+    // LDS in two regions, each holding two arrays whose lifetimes do not overlap: the field tile ft (until the rows
+    // are interpolated) then the mismatch es (from the warps to the box), and the interpolated rows hrow (until the
+    // warps are done) then the box's row sums hs; a barrier separates each pair
+    constexpr int kRegA = ((FTW * FTH * 8 > EH * EW * 4 ? FTW * FTH * 8 : EH * EW * 4) + 15) / 16 * 16;
+    constexpr int kRegB = FTH * EW * 8 > EH * TW * 2 ? FTH * EW * 8 : EH * TW * 2;
+    __shared__ __align__(16) uint8_t pool[kRegA + kRegB];
+    float2* ft = (float2*)pool;
+    int* es = (int*)pool;
+    float2* hrow = (float2*)(pool + kRegA);
+    uint16_t* hs = (uint16_t*)(pool + kRegA);
     __shared__ uint32_t cnt;
     const int tid = threadIdx.x;
     const int x0 = tileX * TW, y0 = tileY * TH;
