@@ -73,6 +73,9 @@ __device__ __forceinline__ uint32_t pkMad16(uint32_t a, uint32_t b, uint32_t c) 
     return __builtin_bit_cast(uint32_t, __builtin_bit_cast(U16x2, a) * __builtin_bit_cast(U16x2, b) +
                                             __builtin_bit_cast(U16x2, c));
 }
+__device__ __forceinline__ uint32_t pkAdd16(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(U16x2, a) + __builtin_bit_cast(U16x2, b));
+}
 __device__ __forceinline__ uint32_t pkMul16(uint32_t a, uint32_t b) {
     return __builtin_bit_cast(uint32_t, __builtin_bit_cast(U16x2, a) * __builtin_bit_cast(U16x2, b));
 }
@@ -314,9 +317,18 @@ __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, 
 #pragma unroll
             for (int i = 0; i < 4; i++) nr[i] = x + i < P.W ? ((const uint32_t*)src)[i] : 0u;
         }
+        // the 7-row sums of the run's 4 columns: 8-byte rows added in 16-bit lanes (every sum <= 49 x 255 < 2^16)
+        uint32_t s01 = 0u, s23 = 0u;
+#pragma unroll
+        for (int m = 0; m < 7; m++) {
+            const uint2 q = *(const uint2*)&h[m * TW];
+            s01 = pkAdd16(s01, q.x);
+            s23 = pkAdd16(s23, q.y);
+        }
+        const int sums[4] = {(int)(s01 & 0xFFFFu), (int)(s01 >> 16), (int)(s23 & 0xFFFFu), (int)(s23 >> 16)};
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const int S49 = (int)h[i] + h[i + TW] + h[i + 2 * TW] + h[i + 3 * TW] + h[i + 4 * TW] + h[i + 5 * TW] + h[i + 6 * TW];
+            const int S49 = sums[i];
             const bool inside = x + i < P.W;
             mine += (inside && S49 > P.sHalf) ? 1u : 0u;
             res[i] = blendRgba8(P, wa[4 * k + i], wc[4 * k + i], nr[i], trustOf(P, S49));

@@ -117,12 +117,15 @@ __device__ __forceinline__ int loadFieldTile(const SynthParams& P, int x0, int y
     for (int it = 0; it < IT; it++)
         if (tid + 256 * it < N) ft[tid + 256 * it] = v[it];
     __syncthreads();
-    for (int i = tid; i < FTH * EW; i += 256) {
-        const int j = i / EW, ex = i - j * EW;
+    // column by column: thread t takes region column t % EW of the rows t / EW, t / EW + G, ..., so the column's
+    // clamped position, weight and flow cell are formed once
+    constexpr int G = 256 / EW;
+    if (tid < G * EW) {
+        const int ex = tid % EW;
         const int x = min(med3i(x0 - AP + ex, 0, P.W - 1), xlim);
         const float fx = FlowMap<S>::frac(x);
-        const float2* f = ft + j * FTW + (FlowMap<S>::index(x) - tx0);
-        hrow[i] = mix2(f[0], f[1], fx);
+        const float2* f = ft + (FlowMap<S>::index(x) - tx0);
+        for (int j = tid / EW; j < FTH; j += G) hrow[j * EW + ex] = mix2(f[j * FTW], f[j * FTW + 1], fx);
     }
     __syncthreads();
     return ty0;
