@@ -495,17 +495,34 @@ __device__ __forceinline__ void subpixBody(const SubpixParams& P, int g, int) {
         const float c00 = (1.0f - fcy) * (1.0f - fcx), c01 = (1.0f - fcy) * fcx, c10 = fcy * (1.0f - fcx), c11 = fcy * fcx;
         const int ia = (int)flay * T + (int)flax, ic = (int)flcy * T + (int)flcx;
         float total = 0.0f;
+        // the 9 taps of window row y and y + 1 per frame; row y + 1 is carried into the next row (one LDS row a step)
+        float ta0[9], tc0[9];
+#pragma unroll
+        for (int x = 0; x < 9; x++) {
+            ta0[x] = A[ia + x];
+            tc0[x] = C[ic + x];
+        }
+#pragma unroll 2
         for (int y = 0; y < 8; y++) {
-            const float* ra = A + ia + y * T;
-            const float* rc = C + ic + y * T;
+            float ta1[9], tc1[9];
+#pragma unroll
+            for (int x = 0; x < 9; x++) {
+                ta1[x] = A[ia + (y + 1) * T + x];
+                tc1[x] = C[ic + (y + 1) * T + x];
+            }
             float row = 0.0f;
 #pragma unroll
             for (int x = 0; x < 8; x++) {
-                const float va = ra[x] * a00 + ra[x + 1] * a01 + ra[x + T] * a10 + ra[x + T + 1] * a11;
-                const float vc = rc[x] * c00 + rc[x + 1] * c01 + rc[x + T] * c10 + rc[x + T + 1] * c11;
+                const float va = ta0[x] * a00 + ta0[x + 1] * a01 + ta1[x] * a10 + ta1[x + 1] * a11;
+                const float vc = tc0[x] * c00 + tc0[x + 1] * c01 + tc1[x] * c10 + tc1[x + 1] * c11;
                 row = row + fabsf(va - vc);
             }
             total = total + row;
+#pragma unroll
+            for (int x = 0; x < 9; x++) {
+                ta0[x] = ta1[x];
+                tc0[x] = tc1[x];
+            }
         }
         const float sad = total * (1.0f / 64.0f);
         if (lane < 9 * NB) {
