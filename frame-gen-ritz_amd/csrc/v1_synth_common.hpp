@@ -8,11 +8,11 @@ namespace fgr {
 
 // Tile of the fused synth kernel: 64 x 32 output pixels per 256-thread workgroup (8 per thread), a 3-pixel apron for
 // the 7x7 box (70 x 38 mismatch values, 1.30 per output pixel).
-constexpr int TW = 64, TH = 32, AP = 3, EW = TW + 2 * AP, EH = TH + 2 * AP, NAPRON = EW * EH - TW * TH;
+constexpr int TW = 64, TH = 32, AP = 3, EW = TW + 2 * AP, EH = TH + 2 * AP;
 // the same geometry for a tile of TH_ rows (the RGBA8 kernel comes in 16, 32 and 64 rows; the tile height is tuned)
 template <int TH_>
 struct TileRows {
-    static constexpr int H = TH_, EH = TH_ + 2 * AP, NAPRON = EW * EH - TW * TH_;
+    static constexpr int H = TH_, EH = TH_ + 2 * AP;
 };
 
 struct alignas(4) U32x2 {
