@@ -151,13 +151,22 @@ __device__ __forceinline__ Maps mapsAt(float kA32, float kC32, float x32, float 
     return m;
 }
 
+// resolve.comp:73-83 blend in float32, in two steps: the mix m = 0.5 (wa + wc) at t = 1/2, else wA wa + wC wc (taken
+// with the mismatch, where the warped values are at hand), then o = m (1 - u) + near u once the trust u is known
+__device__ __forceinline__ float mixF(const SynthParams& P, float wa, float wc) {
+    return P.mid ? 0.5f * (wa + wc) : P.wA * wa + P.wC * wc;
+}
+
+__device__ __forceinline__ float finishF(float m, float nv, float u) { return m * (1.0f - u) + nv * u; }
+
 // Mismatch e in 0..255 (error.comp / mismatch() with format.glsl's FG_DIFF and FG_PACKE):
 //   Rgb10   uint(min(max_c |wa - wc|, 255)), levels w = float32(sum) * float32(255 / 1047552)
 //   RgbaF16 min(max_c |L(half(wa)) - L(half(wc))| >> 4, 255)
 //   Nv12, P010 uint(min(max |fgYuvDiff(Y, Cb, Cr difference)|, 255))
 // (UI: with the CrosshairV2 tap rejection of a protected pair, on the raw sums: motion_v1.glsl:124, 141)
-template <bool TEN, bool UI>
-__device__ __forceinline__ int mismatchYuv(const SynthParams& P, const Maps& m) {
+// MIX: also the blend's mix of the warped values into mix[] (the Y level for Nv12 / P010, the three channels else)
+template <bool TEN, bool UI, bool MIX>
+__device__ __forceinline__ int mismatchYuv(const SynthParams& P, const Maps& m, float* mix) {
     int ya = yuvYSum<TEN>(P.a, P.W, P.H, m.XA, m.YA), yc = yuvYSum<TEN>(P.c, P.W, P.H, m.XC, m.YC);
     int cba, cra, cbc, crc;
     yuvCSums<TEN>(P.a, P.cw, P.ch, m.XA, m.YA, cba, cra);
@@ -177,6 +186,7 @@ __device__ __forceinline__ int mismatchYuv(const SynthParams& P, const Maps& m) 
     const float ky = 1.0f / (1024.0f * Yuv<TEN>::kQ), kc = 1.0f / (4096.0f * Yuv<TEN>::kQ);
     const float dY = (float)(ya - yc) * ky;
     const float dCb = (float)(cba - cbc) * kc, dCr = (float)(cra - crc) * kc;
+    if (MIX) mix[0] = mixF(P, (float)ya * ky, (float)yc * ky);
     return (int)fminf(yuvDiffMax(P, dY, dCb, dCr), 255.0f);
 }
 
@@ -202,8 +212,12 @@ __device__ __forceinline__ void uiRejectArraysF(const SynthParams& P, const Maps
     }
 }
 
-template <int F, bool UI = false>
-__device__ __forceinline__ int mismatchFmt(const SynthParams& P, const Maps& m) {
+// the number of mixes a pixel keeps for its blend: the three channels, or the Y level of Nv12 / P010
+template <int F>
+constexpr int kMixes = (F == kRgb10 || F == kF16) ? 3 : 1;
+
+template <int F, bool UI = false, bool MIX = false>
+__device__ __forceinline__ int mismatchFmt(const SynthParams& P, const Maps& m, float* mix = nullptr) {
     if (F == kRgb10) {
         int sa[3], sc[3];
         rgb10Sums(P.a, P.W, P.H, m.XA, m.YA, sa);
@@ -212,7 +226,11 @@ __device__ __forceinline__ int mismatchFmt(const SynthParams& P, const Maps& m) 
         const float k = (float)(255.0 / 1047552.0);
         float d = 0.0f;
 #pragma unroll
-        for (int c = 0; c < 3; c++) d = fmaxf(d, fabsf((float)sa[c] * k - (float)sc[c] * k));
+        for (int c = 0; c < 3; c++) {
+            const float wa = (float)sa[c] * k, wc = (float)sc[c] * k;
+            d = fmaxf(d, fabsf(wa - wc));
+            if (MIX) mix[c] = mixF(P, wa, wc);
+        }
         return (int)fminf(d, 255.0f);
     }
     if (F == kF16) {
@@ -222,19 +240,21 @@ __device__ __forceinline__ int mismatchFmt(const SynthParams& P, const Maps& m) 
         uiRejectArraysF<UI>(P, m, va, vc);
         int d = 0;
 #pragma unroll
-        for (int c = 0; c < 3; c++) d = max(d, abs(P.lut[toHalfBits(va[c])] - P.lut[toHalfBits(vc[c])]));
+        for (int c = 0; c < 3; c++) {
+            d = max(d, abs(P.lut[toHalfBits(va[c])] - P.lut[toHalfBits(vc[c])]));
+            if (MIX) mix[c] = mixF(P, va[c], vc[c]);
+        }
         return min(d >> 4, 255);
     }
-    return mismatchYuv<F == kP010, UI>(P, m);
+    return mismatchYuv<F == kP010, UI, MIX>(P, m, mix);
 }
 
 // resolve.comp:73-83 blend in float32: m = 0.5 (wa + wc) at t = 1/2, else wA wa + wC wc; o = m (1 - u) + near u
 __device__ __forceinline__ float blendF(const SynthParams& P, float wa, float wc, float nv, float u) {
-    const float m = P.mid ? 0.5f * (wa + wc) : P.wA * wa + P.wC * wc;
-    return m * (1.0f - u) + nv * u;
+    return finishF(mixF(P, wa, wc), nv, u);
 }
 
-// One output pixel (x, y) with trust u:
+// One output pixel (x, y) with trust u and the mixes its mismatch kept:
 //   Rgb10   code = floor(clamp(o, 0, 255) * float32(1023 / 255) + 0.5), alpha of the nearer frame
 //   RgbaF16 half(o), alpha of the nearer frame
 //   Nv12 / P010 Y code = clamp(floor(q o + 0.5), 0, top) (q codes per level); at an even-even pixel also the CbCr
@@ -242,18 +262,12 @@ __device__ __forceinline__ float blendF(const SynthParams& P, float wa, float wc
 //           this pixel's trust
 template <bool TEN, bool UI>
 __device__ __forceinline__ void resolveYuv(const SynthParams& P, float kA32, float kC32, int x, int y, float2 r, float u,
-                                           const Maps& m) {
+                                           float my) {
     using T = Yuv<TEN>;
     const FrameRef& nf = P.nearC ? P.c : P.a;
-    int ya = yuvYSum<TEN>(P.a, P.W, P.H, m.XA, m.YA), yc = yuvYSum<TEN>(P.c, P.W, P.H, m.XC, m.YC);
-    if (UI) {
-        const int w = uiRejectWhich<true>(P, m.XA, m.YA, m.XC, m.YC);
-        if (w == 1) ya = yc;
-        else if (w == 2) yc = ya;
-    }
-    const float ky = 1.0f / (1024.0f * T::kQ), kc = 1.0f / (4096.0f * T::kQ), kn = 1.0f / T::kQ;
+    const float kc = 1.0f / (4096.0f * T::kQ), kn = 1.0f / T::kQ;
     const float ny = (float)T::y(nf.p + (int64_t)y * nf.pitch, x) * kn;
-    const float oy = blendF(P, (float)ya * ky, (float)yc * ky, ny, u);
+    const float oy = finishF(my, ny, u);
     T::putY(P.out + (int64_t)y * P.opitch, x, med3i((int)floorf(oy * T::kQ + 0.5f), 0, T::kTop));
     if (((x | y) & 1) == 0) {
         const Maps b = mapsAt(kA32, kC32, (float)(32 * x + 16), (float)(32 * y + 16), r);
@@ -280,40 +294,32 @@ __device__ __forceinline__ void resolveYuv(const SynthParams& P, float kA32, flo
 }
 
 template <int F, bool UI = false>
-__device__ __forceinline__ void resolveFmt(const SynthParams& P, float kA32, float kC32, int x, int y, float2 r, float u) {
+__device__ __forceinline__ void resolveFmt(const SynthParams& P, float kA32, float kC32, int x, int y, float2 r, float u,
+                                           const float* mix) {
     const FrameRef& nf = P.nearC ? P.c : P.a;
-    const Maps m = mapsAt(kA32, kC32, (float)(32 * x), (float)(32 * y), r);
     if (F == kRgb10) {
-        int sa[3], sc[3];
-        rgb10Sums(P.a, P.W, P.H, m.XA, m.YA, sa);
-        rgb10Sums(P.c, P.W, P.H, m.XC, m.YC, sc);
-        uiRejectArrays<F, UI>(P, m, sa, sc);
         const uint32_t nw = *(const uint32_t*)(nf.p + (int64_t)y * nf.pitch + 4 * (int64_t)x);
-        const float k = (float)(255.0 / 1047552.0), kn = (float)(255.0 / 1023.0), ko = (float)(1023.0 / 255.0);
+        const float kn = (float)(255.0 / 1023.0), ko = (float)(1023.0 / 255.0);
         uint32_t o = nw & 0xC0000000u;
 #pragma unroll
         for (int c = 0; c < 3; c++) {
             const float nv = (float)((nw >> (10 * c)) & 1023u) * kn;
-            const float v = blendF(P, (float)sa[c] * k, (float)sc[c] * k, nv, u);
+            const float v = finishF(mix[c], nv, u);
             o |= (uint32_t)(int)floorf(fminf(fmaxf(v, 0.0f), 255.0f) * ko + 0.5f) << (10 * c);
         }
         *(uint32_t*)(P.out + (int64_t)y * P.opitch + 4 * (int64_t)x) = o;
         return;
     }
     if (F == kF16) {
-        float va[3], vc[3];
-        f16Warp(P.a, P.W, P.H, m.XA, m.YA, va);
-        f16Warp(P.c, P.W, P.H, m.XC, m.YC, vc);
-        uiRejectArraysF<UI>(P, m, va, vc);
         const uint2 nq = *(const uint2*)(nf.p + (int64_t)y * nf.pitch + 8 * (int64_t)x);
         const float n0 = halfBits(nq.x), n1 = halfBits(nq.x >> 16), n2 = halfBits(nq.y);
         uint2 o;
-        o.x = toHalfBits(blendF(P, va[0], vc[0], n0, u)) | (toHalfBits(blendF(P, va[1], vc[1], n1, u)) << 16);
-        o.y = toHalfBits(blendF(P, va[2], vc[2], n2, u)) | (nq.y & 0xFFFF0000u);
+        o.x = toHalfBits(finishF(mix[0], n0, u)) | (toHalfBits(finishF(mix[1], n1, u)) << 16);
+        o.y = toHalfBits(finishF(mix[2], n2, u)) | (nq.y & 0xFFFF0000u);
         *(uint2*)(P.out + (int64_t)y * P.opitch + 8 * (int64_t)x) = o;
         return;
     }
-    resolveYuv<F == kP010, UI>(P, kA32, kC32, x, y, r, u, m);
+    resolveYuv<F == kP010, UI>(P, kA32, kC32, x, y, r, u, mix[0]);
 }
 
 // the nearer frame's pixel (x, y) into the output (scene cut): plane 0, and the chroma of an even-even pixel (P010)
@@ -338,8 +344,9 @@ __device__ __forceinline__ void copyNear(const SynthParams& P, int x, int y) {
 }
 
 // One 64 x 32 output tile (the RGBA8 kernel's structure): field tile, mismatch of the tile and its 3-pixel apron into
-// LDS, 7x7 box, trust, untrusted count, then per pixel the warps again and the blend (the warped values of these
-// formats are three 20-bit sums or floats per frame, too many to keep in registers for 8 pixels a thread).
+// LDS, 7x7 box, trust, untrusted count, blend. A thread warps the 8 pixels it resolves first and keeps their mixes in
+// registers (three floats, or the Y level's one), so the blend reads only the nearer frame; the apron's 612 pixels
+// follow, spread over the workgroup.
 template <int S, int F, bool UI = false>
 __device__ __forceinline__ void synthFmtTile(const SynthParams& P, int tileX, int tileY, float2* ft, float2* hrow,
                                              int* es, uint16_t* hs, uint32_t* cnt) {
@@ -356,23 +363,48 @@ __device__ __forceinline__ void synthFmtTile(const SynthParams& P, int tileX, in
     if (tid == 0) *cnt = 0u;
     const int ty0 = loadFieldTile<S>(P, x0, y0, ft, hrow);
     const float kA32 = (float)(32 * S) * (2.0f * P.wC), kC32 = (float)(32 * S) * (2.0f * P.wA);
-    for (int i = tid; i < EW * EH; i += 256) {
-        const int ey = i / EW, ex = i - ey * EW;
+    // the tile's own pixels (region column cx + AP, row ly + AP; clamped into the image as the apron is)
+    float mix[TH / 4][kMixes<F>];
+#pragma unroll
+    for (int k = 0; k < TH / 4; k++) {
+        const int ly = ry + 4 * k;
+        const int x = min(x0 + cx, P.W - 1), y = min(y0 + ly, P.H - 1);
+        const float2 r = fieldAtRegion<S>(P, hrow, ty0, cx + AP, y);
+        es[(ly + AP) * EW + cx + AP] =
+            mismatchFmt<F, UI, true>(P, mapsAt(kA32, kC32, (float)(32 * x), (float)(32 * y), r), mix[k]);
+    }
+    // the apron: AP rows above and below the tile (EW wide), then AP columns either side of its TH rows
+    constexpr int kRowsAp = 2 * AP * EW, kApron = kRowsAp + 2 * AP * TH;
+    for (int i = tid; i < kApron; i += 256) {
+        int ex, ey;
+        if (i < kRowsAp) {
+            const int j = i / EW;
+            ex = i - j * EW;
+            ey = j < AP ? j : j + TH;
+        } else {
+            const int j = (i - kRowsAp) / (2 * AP), c = i - kRowsAp - j * (2 * AP);
+            ey = j + AP;
+            ex = c < AP ? c : c + TW;
+        }
         const int x = med3i(x0 - AP + ex, 0, P.W - 1), y = med3i(y0 - AP + ey, 0, P.H - 1);
         const float2 r = fieldAtRegion<S>(P, hrow, ty0, ex, y);
-        es[i] = mismatchFmt<F, UI>(P, mapsAt(kA32, kC32, (float)(32 * x), (float)(32 * y), r));
+        es[ey * EW + ex] = mismatchFmt<F, UI>(P, mapsAt(kA32, kC32, (float)(32 * x), (float)(32 * y), r));
     }
     __syncthreads();
     boxRows7<EH>(es, hs, tid);
     __syncthreads();
     uint32_t mine = 0u;
+#pragma unroll
     for (int k = 0; k < TH / 4; k++) {
         const int ly = ry + 4 * k, x = x0 + cx, y = y0 + ly;
         if (x >= P.W || y >= P.H) continue;
         const uint16_t* h = &hs[ly * TW + cx];
         const int S49 = (int)h[0] + h[TW] + h[2 * TW] + h[3 * TW] + h[4 * TW] + h[5 * TW] + h[6 * TW];
         mine += S49 > P.sHalf ? 1u : 0u;
-        resolveFmt<F, UI>(P, kA32, kC32, x, y, fieldAtRegion<S>(P, hrow, ty0, cx + AP, y), trustOf(P, S49));
+        // the field vector only for the chroma of an even-even Nv12 / P010 pixel
+        constexpr bool kYuv = F == kNv12 || F == kP010;
+        const float2 r = kYuv && ((x | y) & 1) == 0 ? fieldAtRegion<S>(P, hrow, ty0, cx + AP, y) : make_float2(0.0f, 0.0f);
+        resolveFmt<F, UI>(P, kA32, kC32, x, y, r, trustOf(P, S49), mix[k]);
     }
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) mine += __shfl_xor(mine, o, 64);
