@@ -711,6 +711,12 @@ __device__ __forceinline__ bool pixErrRun(const PixParams& P, int k, int gx0, in
     return false;
 }
 
+// The horizontal sums' LDS slot of row y, column c: columns with (c & 2) == 0 in one plane, the others in a second,
+// each plane's rows 16 doubles wide, so that a task's two 16-byte stores land in consecutive slots across the lanes
+__device__ __forceinline__ int hsumAt(int y, int c) {
+    return ((c & 2) ? kPfRH * kPfTile / 2 : 0) + y * (kPfTile / 2) + ((c >> 2) << 1) + (c & 1);
+}
+
 // NT = 512 (the per-pair kernel: few workgroups, so latency decides): one run and 2 pixels a thread. NT = 256 (the
 // batched kernel: many workgroups, so occupancy decides): up to 2 runs and 4 pixels a thread, fewer registers.
 //
@@ -782,12 +788,13 @@ __device__ __forceinline__ void pixfieldBody(const PixParams& P, int gbx, int gb
             }
             continue;
         }
-        // horizontal 5-sums, 4 output columns a task: region columns c0 + 2 .. c0 + 10 of one row
+        // horizontal 5-sums, 4 output columns a task: region columns c0 + 2 .. c0 + 10 of one row (read as three
+        // aligned 16-byte quads; a row's 8 tasks cover every LDS bank once)
         for (int t = tid; t < kPfRH * (kPfTile / 4); t += NT) {
             const int hy = t >> 3, c0 = (t & 7) * 4;
-            const float2* e2 = (const float2*)&errs[hy * kPfRW + c0 + 2];
-            const float2 p0 = e2[0], p1 = e2[1], p2 = e2[2], p3 = e2[3];
-            const float e[8] = {p0.x, p0.y, p1.x, p1.y, p2.x, p2.y, p3.x, p3.y};
+            const float4* e4 = (const float4*)&errs[hy * kPfRW + c0];
+            const float4 q0 = e4[0], q1 = e4[1], q2 = e4[2];
+            const float e[8] = {q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x, q2.y};
             double h[4];
 #pragma unroll
             for (int i = 0; i < 4; i++) {
@@ -796,14 +803,13 @@ __device__ __forceinline__ void pixfieldBody(const PixParams& P, int gbx, int gb
                 for (int x = 1; x < 5; x++) hs = hs + (double)e[i + x];
                 h[i] = hs;
             }
-            double2* h2 = (double2*)&hsum[hy * kPfTile + c0];
-            h2[0] = make_double2(h[0], h[1]);
-            h2[1] = make_double2(h[2], h[3]);
+            *(double2*)&hsum[hsumAt(hy, c0)] = make_double2(h[0], h[1]);
+            *(double2*)&hsum[hsumAt(hy, c0 + 2)] = make_double2(h[2], h[3]);
         }
         __syncthreads();
         double hv[kRows + 4];
 #pragma unroll
-        for (int y = 0; y < kRows + 4; y++) hv[y] = hsum[(r0 + y) * kPfTile + cx];
+        for (int y = 0; y < kRows + 4; y++) hv[y] = hsum[hsumAt(r0 + y, cx)];
 #pragma unroll
         for (int j = 0; j < kRows; j++) {
             double vs = hv[j];
