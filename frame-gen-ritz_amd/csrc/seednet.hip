@@ -5,7 +5,7 @@
 // of that level, in that level's pixels. The vector is rounded half-even and offered to the level's search as a second
 // seed candidate beside zero (the plain seed): the exact SAD search picks the better of the two (the proposal pays the
 // seed penalty lambda, as every non-own candidate does) and then searches +-R around it, so a wrong proposal costs
-// nothing but the candidate's SAD, and the plain seeds remain both the fallback and the judge.
+// nothing but the candidate's SAD, and the exact SAD comparison against the plain seed decides.
 //
 // Every convolution -- forward, and both gradients -- is an implicit GEMM on the BF16 matrix cores
 // (v_mfma_f32_16x16x16_bf16, FP32 accumulation): a wavefront owns a 16 x 16 output tile and gathers its A and B operands
