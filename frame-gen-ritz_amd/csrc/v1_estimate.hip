@@ -628,11 +628,20 @@ __device__ __forceinline__ int2 candOffset(int k) {
     return make_int2(dx, dy);
 }
 
+// The 8 x 8 block vectors around a 32 x 32 tile at (x0, y0), blocks x0 / 8 - 2 .. x0 / 8 + 5 and y0 / 8 - 2 .. y0 / 8 + 5
+// (clamped into the block field as fieldAt clamps): every block vector and bilinear corner that a run inside the frame
+// reads, staged in LDS once for the 7 candidates
+struct BlockWin {
+    const float2* v;
+    int bx0, by0;
+    __device__ __forceinline__ float2 at(int bx, int by) const { return v[(by - by0) * 8 + (bx - bx0)]; }
+};
+
 // the errors of the run of 4 pixels (gx0 .. gx0 + 3, row qy) under candidate k. own / ownV: the run's errors and
 // vector under candidate 1, kept from k = 1 (computed first) for runs inside the frame (ownOk). Returns true when the
 // run's errors are own's because every pixel's vector equals ownV (then the errors are the same floats).
-__device__ __forceinline__ bool pixErrRun(const PixParams& P, int k, int gx0, int qy, float* e, float* own, float2& ownV,
-                                          bool& ownOk) {
+__device__ __forceinline__ bool pixErrRun(const PixParams& P, const BlockWin& bw, int k, int gx0, int qy, float* e,
+                                          float* own, float2& ownV, bool& ownOk) {
     if (k == 6) {   // zero vector: sampleL at an integer position is the texel (weights 1, 0, 0, 0)
 #pragma unroll
         for (int i = 0; i < 4; i++) {
@@ -647,8 +656,7 @@ __device__ __forceinline__ bool pixErrRun(const PixParams& P, int k, int gx0, in
         const float flx = floorf(sx0), fly = floorf(sy);
         const float fy = sy - fly;
         const int ix = (int)flx, iy = (int)fly;
-        const float2 f00 = fieldAt(P.bf, ix, iy, P.bw, P.bh), f10 = fieldAt(P.bf, ix + 1, iy, P.bw, P.bh);
-        const float2 f01 = fieldAt(P.bf, ix, iy + 1, P.bw, P.bh), f11 = fieldAt(P.bf, ix + 1, iy + 1, P.bw, P.bh);
+        const float2 f00 = bw.at(ix, iy), f10 = bw.at(ix + 1, iy), f01 = bw.at(ix, iy + 1), f11 = bw.at(ix + 1, iy + 1);
         auto cand0 = [&](int qx) {
             const float fx = (((float)qx + 0.5f) / 8.0f - 0.5f) - flx;
             return mix2(mix2(f00, f10, fx), mix2(f01, f11, fx), fy);
@@ -675,7 +683,7 @@ __device__ __forceinline__ bool pixErrRun(const PixParams& P, int k, int gx0, in
     }
     if (k > 0 && inside) {
         const int2 o = candOffset(k);
-        const float2 v = fieldAt(P.bf, gx0 / kBlock + o.x, qy / kBlock + o.y, P.bw, P.bh);   // one block, one vector
+        const float2 v = bw.at(gx0 / kBlock + o.x, qy / kBlock + o.y);   // one block, one vector
         if (k > 1 && ownOk && v.x == ownV.x && v.y == ownV.y) {
 #pragma unroll
             for (int i = 0; i < 4; i++) e[i] = own[i];
@@ -730,8 +738,12 @@ __device__ __forceinline__ void pixfieldBody(const PixParams& P, int gbx, int gb
     constexpr int kRows = kPfTile * kPfTile / NT;      // pixels a thread: column cx, rows r0 .. r0 + kRows - 1
     __shared__ __align__(16) float errs[kPfRH * kPfRW];
     __shared__ __align__(16) double hsum[kPfRH * kPfTile];
+    __shared__ float2 bwin[64];
     const int tid = threadIdx.x;
     const int x0 = gbx * kPfTile, y0 = gby * kPfTile;
+    const BlockWin bw{bwin, x0 / kBlock - 2, y0 / kBlock - 2};
+    if (tid < 64) bwin[tid] = fieldAt(P.bf, bw.bx0 + (tid & 7), bw.by0 + (tid >> 3), P.bw, P.bh);
+    __syncthreads();
     const int cx = tid & 31, r0 = (tid >> 5) * kRows;
     float best[kRows], c1[kRows];
     int bk[kRows];
@@ -759,7 +771,7 @@ __device__ __forceinline__ void pixfieldBody(const PixParams& P, int gbx, int gb
             bool ownOk = s1 ? ownOk1 : ownOk0;
             const int ry = r / (kPfRW / 4), rc = r - ry * (kPfRW / 4);
             float e[4];
-            reusedAll = pixErrRun(P, k, x0 - 4 + 4 * rc, clampi(y0 - 2 + ry, 0, P.h - 1), e, own, ownV, ownOk) &&
+            reusedAll = pixErrRun(P, bw, k, x0 - 4 + 4 * rc, clampi(y0 - 2 + ry, 0, P.h - 1), e, own, ownV, ownOk) &&
                         reusedAll;
             *(float4*)&errs[ry * kPfRW + 4 * rc] = make_float4(e[0], e[1], e[2], e[3]);
             if (k == 1) {
