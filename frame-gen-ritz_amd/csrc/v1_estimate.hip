@@ -559,8 +559,17 @@ __device__ __forceinline__ void subpixBody(const SubpixParams& P, int g, int) {
 
 // ---------------------------------------------------------------------------------------------------------- pixfield
 
+// The 8 x 8 block vectors around a 32 x 32 tile at (x0, y0), blocks x0 / 8 - 2 .. x0 / 8 + 5 and y0 / 8 - 2 .. y0 / 8 + 5
+// (clamped into the block field as fieldAt clamps): every block vector and bilinear corner that the tile's candidates read
+// (its region's pixels, clamped into the frame, lie in blocks x0 / 8 - 1 .. x0 / 8 + 4), staged in LDS once
+struct BlockWin {
+    const float2* v;
+    int bx0, by0;
+    __device__ __forceinline__ float2 at(int bx, int by) const { return v[(by - by0) * 8 + (bx - bx0)]; }
+};
+
 // pixfield.comp:19-35 cand(): per-pixel candidate k at flow pixel (qx, qy)
-__device__ __forceinline__ float2 pixCand(int k, int qx, int qy, const float2* bf, int bw, int bh) {
+__device__ __forceinline__ float2 pixCand(int k, int qx, int qy, const BlockWin& bw) {
     const int bx = qx / kBlock, by = qy / kBlock;
     switch (k) {
     case 0: {
@@ -568,15 +577,15 @@ __device__ __forceinline__ float2 pixCand(int k, int qx, int qy, const float2* b
         const float flx = floorf(sx), fly = floorf(sy);
         const float fx = sx - flx, fy = sy - fly;
         const int ix = (int)flx, iy = (int)fly;
-        const float2 top = mix2(fieldAt(bf, ix, iy, bw, bh), fieldAt(bf, ix + 1, iy, bw, bh), fx);
-        const float2 bot = mix2(fieldAt(bf, ix, iy + 1, bw, bh), fieldAt(bf, ix + 1, iy + 1, bw, bh), fx);
+        const float2 top = mix2(bw.at(ix, iy), bw.at(ix + 1, iy), fx);
+        const float2 bot = mix2(bw.at(ix, iy + 1), bw.at(ix + 1, iy + 1), fx);
         return mix2(top, bot, fy);
     }
-    case 1: return fieldAt(bf, bx, by, bw, bh);
-    case 2: return fieldAt(bf, bx, by - 1, bw, bh);
-    case 3: return fieldAt(bf, bx, by + 1, bw, bh);
-    case 4: return fieldAt(bf, bx - 1, by, bw, bh);
-    case 5: return fieldAt(bf, bx + 1, by, bw, bh);
+    case 1: return bw.at(bx, by);
+    case 2: return bw.at(bx, by - 1);
+    case 3: return bw.at(bx, by + 1);
+    case 4: return bw.at(bx - 1, by);
+    case 5: return bw.at(bx + 1, by);
     default: return make_float2(0.0f, 0.0f);
     }
 }
@@ -627,15 +636,6 @@ __device__ __forceinline__ int2 candOffset(int k) {
     const int dx = k == 4 ? -1 : (k == 5 ? 1 : 0);
     return make_int2(dx, dy);
 }
-
-// The 8 x 8 block vectors around a 32 x 32 tile at (x0, y0), blocks x0 / 8 - 2 .. x0 / 8 + 5 and y0 / 8 - 2 .. y0 / 8 + 5
-// (clamped into the block field as fieldAt clamps): every block vector and bilinear corner that a run inside the frame
-// reads, staged in LDS once for the 7 candidates
-struct BlockWin {
-    const float2* v;
-    int bx0, by0;
-    __device__ __forceinline__ float2 at(int bx, int by) const { return v[(by - by0) * 8 + (bx - bx0)]; }
-};
 
 // the errors of the run of 4 pixels (gx0 .. gx0 + 3, row qy) under candidate k. own / ownV: the run's errors and
 // vector under candidate 1, kept from k = 1 (computed first) for runs inside the frame (ownOk). Returns true when the
@@ -712,7 +712,7 @@ __device__ __forceinline__ bool pixErrRun(const PixParams& P, const BlockWin& bw
 #pragma unroll 1
     for (int i = 0; i < 4; i++) {
         const int qx = clampi(gx0 + i, 0, P.w - 1);
-        const float2 v = pixCand(k, qx, qy, P.bf, P.bw, P.bh);
+        const float2 v = pixCand(k, qx, qy, bw);
         e[i] = fabsf(sampleL(P.la, (float)qx - v.x, (float)qy - v.y, P.w, P.h) -
                      sampleL(P.lc, (float)qx + v.x, (float)qy + v.y, P.w, P.h));
     }
@@ -849,10 +849,10 @@ __device__ __forceinline__ void pixfieldBody(const PixParams& P, int gbx, int gb
         const int k = bk[j];
         float2 v;
         if (k == 0) {
-            v = pixCand(0, qx, py, P.bf, P.bw, P.bh);
+            v = pixCand(0, qx, py, bw);
         } else {
             const int2 o = candOffset(k);
-            v = fieldAt(P.bf, qx / kBlock + o.x, py / kBlock + o.y, P.bw, P.bh);
+            v = bw.at(qx / kBlock + o.x, py / kBlock + o.y);
             if (k == 6) v = make_float2(0.0f, 0.0f);
         }
         P.pf[py * P.w + qx] = v;
