@@ -218,27 +218,27 @@ __device__ __forceinline__ void medhistBody(const MedParams& P, int, int) {
 //  2. the (8 + 2R)^2 windows of a (at q - seed) and c (at q + seed) into LDS;
 //  3. lane = candidate (dy-major), sequential y-outer / x-inner SAD over LDS, cost SAD/64 + lambda |d|_1;
 //  4. wavefront argmin on (cost, candidate index): the first of the minimal costs, as the sequential strict <.
-// The seed of block (bx, by) (step 1 above), wave-uniform: lane = pixel, SAD by the 64-lane tree.
-__device__ __forceinline__ void searchSeed(const SearchParams& P, int blk, bool active, int bx, int by, int lane, int& sx,
-                                           int& sy) {
-    sx = 0;
-    sy = 0;
-    if (!P.parent && !P.extraSeed) return;
-    const int b0x = bx * kBlock, b0y = by * kBlock;
-    const int qx = b0x + (lane & 7), qy = b0y + (lane >> 3);
+// The seed of block (bx, by) (step 1 above), wave-uniform: lane = pixel, SAD by the 64-lane tree. In three steps, so
+// that a wave can form the seeds of two blocks side by side: the candidates' vectors, then all their texel pairs
+// loaded together (one memory latency, not one per candidate), then the tree sums and the first-minimum choice in
+// candidate order. Slots past nc repeat the zero vector and are not considered.
+constexpr int kMaxSeeds = 7;
+
+struct SeedSet {
+    int nc;
+    int cx[kMaxSeeds], cy[kMaxSeeds];
+    float d[kMaxSeeds];
+};
+
+__device__ __forceinline__ void seedVectors(const SearchParams& P, int blk, bool active, int bx, int by, SeedSet& S) {
     const int px = P.parent ? min(bx / 2, P.pw - 1) : 0, py = P.parent ? min(by / 2, P.ph - 1) : 0;
     const int ssy = (by & 1) == 0 ? -1 : 1, ssx = (bx & 1) == 0 ? -1 : 1;
-    const int nc = P.parent ? (P.extraSeed ? 7 : 6) : 2;
-    // the candidates' vectors first, then all their texel pairs loaded together (one memory latency, not one per
-    // candidate), then the tree sums and the first-minimum choice in candidate order. Slots past nc repeat the zero
-    // vector and are not considered.
-    constexpr int kMaxSeeds = 7;
-    int cxs[kMaxSeeds], cys[kMaxSeeds];
+    S.nc = P.parent ? (P.extraSeed ? 7 : 6) : 2;
 #pragma unroll
     for (int kk = 0; kk < kMaxSeeds; kk++) {
         const int k = P.parent ? kk : (kk == 0 ? 4 : 6);   // coarsest level: zero, then the proposal
         int cx = 0, cy = 0;
-        if (kk >= nc) {
+        if (kk >= S.nc) {
         } else if (k < 4) {
             const int ox = (k == 2 || k == 3) ? ssx : 0, oy = (k == 1 || k == 3) ? ssy : 0;
             const float2 v = P.parent[clampi(py + oy, 0, P.ph - 1) * P.pw + clampi(px + ox, 0, P.pw - 1)];
@@ -252,31 +252,49 @@ __device__ __forceinline__ void searchSeed(const SearchParams& P, int blk, bool 
             cx = (int)rintf(v.x);
             cy = (int)rintf(v.y);
         }
-        cxs[kk] = cx;
-        cys[kk] = cy;
+        S.cx[kk] = cx;
+        S.cy[kk] = cy;
     }
-    float d[kMaxSeeds];
+}
+
+__device__ __forceinline__ void seedTexels(const SearchParams& P, int bx, int by, int lane, SeedSet& S) {
+    const int qx = bx * kBlock + (lane & 7), qy = by * kBlock + (lane >> 3);
 #pragma unroll
     for (int kk = 0; kk < kMaxSeeds; kk++)
-        d[kk] = fabsf(lumaAt(P.la, qx - cxs[kk], qy - cys[kk], P.w, P.h) - lumaAt(P.lc, qx + cxs[kk], qy + cys[kk], P.w, P.h));
+        S.d[kk] = fabsf(lumaAt(P.la, qx - S.cx[kk], qy - S.cy[kk], P.w, P.h) -
+                        lumaAt(P.lc, qx + S.cx[kk], qy + S.cy[kk], P.w, P.h));
+}
+
+__device__ __forceinline__ void seedPick(const SearchParams& P, const SeedSet& S, int& sx, int& sy) {
     float best = 0.0f;
 #pragma unroll
     for (int kk = 0; kk < kMaxSeeds; kk++) {
-        if (kk >= nc) break;
-        float c = treeSum64(d[kk]) * (1.0f / 64.0f);
+        if (kk >= S.nc) break;
+        float c = treeSum64(S.d[kk]) * (1.0f / 64.0f);
         if (kk == 0) {
             best = c;
-            sx = cxs[kk];
-            sy = cys[kk];
+            sx = S.cx[kk];
+            sy = S.cy[kk];
         } else {
             c = c + P.lam;
             if (c < best) {
                 best = c;
-                sx = cxs[kk];
-                sy = cys[kk];
+                sx = S.cx[kk];
+                sy = S.cy[kk];
             }
         }
     }
+}
+
+__device__ __forceinline__ void searchSeed(const SearchParams& P, int blk, bool active, int bx, int by, int lane, int& sx,
+                                           int& sy) {
+    sx = 0;
+    sy = 0;
+    if (!P.parent && !P.extraSeed) return;
+    SeedSet S;
+    seedVectors(P, blk, active, bx, by, S);
+    seedTexels(P, bx, by, lane, S);
+    seedPick(P, S, sx, sy);
 }
 
 // step 2: the (8 + 2R)^2 windows of a (at q - seed) and c (at q + seed), 64 lanes
@@ -372,15 +390,27 @@ __device__ __forceinline__ void search2Body(const SearchParams& P, int bx0, int)
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;   // wave: uniform
     const int nb = P.bw * P.bh;
     const int R = P.R, D = 2 * R + 1;
-    int sxs[2], sys[2];
+    int sxs[2] = {0, 0}, sys[2] = {0, 0}, bxs[2], bys[2];
+    bool acts[2];
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         const int blk = bx0 * kSearch2Blocks + 2 * wave + h;
-        const bool active = blk < nb;
-        const int bx = active ? blk % P.bw : 0, by = active ? blk / P.bw : 0;
-        searchSeed(P, blk, active, bx, by, lane, sxs[h], sys[h]);
-        searchWindows(P, ta[2 * wave + h], tc[2 * wave + h], bx * kBlock, by * kBlock, sxs[h], sys[h], R, lane);
+        acts[h] = blk < nb;
+        bxs[h] = acts[h] ? blk % P.bw : 0;
+        bys[h] = acts[h] ? blk / P.bw : 0;
     }
+    if (P.parent || P.extraSeed) {   // both blocks' seeds side by side, each step for the two together
+        SeedSet S[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) seedVectors(P, bx0 * kSearch2Blocks + 2 * wave + h, acts[h], bxs[h], bys[h], S[h]);
+#pragma unroll
+        for (int h = 0; h < 2; h++) seedTexels(P, bxs[h], bys[h], lane, S[h]);
+#pragma unroll
+        for (int h = 0; h < 2; h++) seedPick(P, S[h], sxs[h], sys[h]);
+    }
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+        searchWindows(P, ta[2 * wave + h], tc[2 * wave + h], bxs[h] * kBlock, bys[h] * kBlock, sxs[h], sys[h], R, lane);
     __syncthreads();
     const int half = lane >> 5, ci = lane & 31;
     const int blk = bx0 * kSearch2Blocks + 2 * wave + half;
