@@ -315,6 +315,44 @@ __device__ __forceinline__ void searchWindows(const SearchParams& P, float* A, f
     }
 }
 
+// step 2 for the two blocks of a wave (T = 8 + 2R <= MAXT): every lane's loads of both windows issued before any store,
+// so the two windows wait on one memory latency (element i of a window as in searchWindows)
+template <int MAXT>
+__device__ __forceinline__ void searchWindowsPair(const SearchParams& P, float* A0, float* C0, float* A1, float* C1,
+                                                  const int* bxs, const int* bys, const int* sxs, const int* sys, int R,
+                                                  int lane) {
+    constexpr int IT = (MAXT * MAXT + 63) / 64;
+    const int T = 8 + 2 * R, N = T * T;
+    const int q = 64 / T, r = 64 - q * T;   // element i + 64 is (x + r, y + q), carried
+    float va[2][IT], vc[2][IT];
+    int x = lane % T, y = lane / T;
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+#pragma unroll
+        for (int h = 0; h < 2; h++) {   // past N: a clamped read that is not stored
+            const int b0x = bxs[h] * kBlock, b0y = bys[h] * kBlock;
+            va[h][it] = lumaAt(P.la, b0x - sxs[h] - R + x, b0y - sys[h] - R + y, P.w, P.h);
+            vc[h][it] = lumaAt(P.lc, b0x + sxs[h] - R + x, b0y + sys[h] - R + y, P.w, P.h);
+        }
+        x += r;
+        y += q;
+        if (x >= T) {
+            x -= T;
+            y++;
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int i = lane + 64 * it;
+        if (i < N) {
+            A0[i] = va[0][it];
+            C0[i] = vc[0][it];
+            A1[i] = va[1][it];
+            C1[i] = vc[1][it];
+        }
+    }
+}
+
 // step 3 for candidate ci: the SAD over the 64 pixels (y-outer, x-inner) and the cost
 __device__ __forceinline__ void searchCost(const SearchParams& P, const float* A, const float* C, int R, int ci, float& cost,
                                            float& sadc) {
@@ -408,9 +446,7 @@ __device__ __forceinline__ void search2Body(const SearchParams& P, int bx0, int)
 #pragma unroll
         for (int h = 0; h < 2; h++) seedPick(P, S[h], sxs[h], sys[h]);
     }
-#pragma unroll
-    for (int h = 0; h < 2; h++)
-        searchWindows(P, ta[2 * wave + h], tc[2 * wave + h], bxs[h] * kBlock, bys[h] * kBlock, sxs[h], sys[h], R, lane);
+    searchWindowsPair<MAXT>(P, ta[2 * wave], tc[2 * wave], ta[2 * wave + 1], tc[2 * wave + 1], bxs, bys, sxs, sys, R, lane);
     __syncthreads();
     const int half = lane >> 5, ci = lane & 31;
     const int blk = bx0 * kSearch2Blocks + 2 * wave + half;
