@@ -525,7 +525,7 @@ __device__ __forceinline__ void subpixBody(const SubpixParams& P, int g, int) {
     constexpr int T = 13, O = 2, NB = kSubpixBlocks;
     __shared__ float ta[NB][T * T], tc[NB][T * T];
     __shared__ float cst[NB][9], sads[NB][9];
-    __shared__ int sv[NB][2];
+    __shared__ int sv[NB][2], sb[NB][2];
     __shared__ uint32_t wsad[NB];
     const int lane = threadIdx.x;
     const int nb = P.bw * P.bh;
@@ -534,15 +534,30 @@ __device__ __forceinline__ void subpixBody(const SubpixParams& P, int g, int) {
         const float2 v0 = P.in[blk < nb ? blk : 0];
         sv[lane][0] = (int)rintf(v0.x);
         sv[lane][1] = (int)rintf(v0.y);
+        const int b = min(blk, nb - 1);
+        sb[lane][0] = (b % P.bw) * kBlock;
+        sb[lane][1] = (b / P.bw) * kBlock;
     }
     __syncthreads();
-    for (int i = lane; i < NB * T * T; i += 64) {
+    // the 7 blocks' windows: every lane's loads issued before any store (one memory latency for the fill); element i
+    // of the flattened [NB][T * T] windows, the lanes past the end reading the last element again without storing it
+    constexpr int N = NB * T * T, IT = (N + 63) / 64;
+    float va[IT], vc[IT];
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int i = min(lane + 64 * it, N - 1);
         const int j = i / (T * T), r = i - j * (T * T);
-        const int blk = min(g * NB + j, nb - 1);
-        const int b0x = (blk % P.bw) * kBlock, b0y = (blk / P.bw) * kBlock;
-        const int x = r % T, y = r / T;
-        ta[j][r] = lumaAt(P.la, b0x - sv[j][0] - O + x, b0y - sv[j][1] - O + y, P.w, P.h);
-        tc[j][r] = lumaAt(P.lc, b0x + sv[j][0] - O + x, b0y + sv[j][1] - O + y, P.w, P.h);
+        const int x = r % T - O, y = r / T - O;
+        va[it] = lumaAt(P.la, sb[j][0] - sv[j][0] + x, sb[j][1] - sv[j][1] + y, P.w, P.h);
+        vc[it] = lumaAt(P.lc, sb[j][0] + sv[j][0] + x, sb[j][1] + sv[j][1] + y, P.w, P.h);
+    }
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int i = lane + 64 * it;
+        if (i < N) {
+            (&ta[0][0])[i] = va[it];
+            (&tc[0][0])[i] = vc[it];
+        }
     }
     __syncthreads();
     const int j = min(lane / 9, NB - 1), k = lane - 9 * (lane / 9);   // lane 63: a duplicate of lane 62's work
