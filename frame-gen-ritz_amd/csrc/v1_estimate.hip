@@ -315,6 +315,36 @@ __device__ __forceinline__ void searchWindows(const SearchParams& P, float* A, f
     }
 }
 
+// step 2 with every lane's loads issued before any store (T = 8 + 2R <= MAXT; one memory latency for the window)
+template <int MAXT>
+__device__ __forceinline__ void searchWindowsIssued(const SearchParams& P, float* A, float* C, int b0x, int b0y, int sx,
+                                                    int sy, int R, int lane) {
+    constexpr int IT = (MAXT * MAXT + 63) / 64;
+    const int T = 8 + 2 * R, N = T * T;
+    const int q = 64 / T, r = 64 - q * T;   // element i + 64 is (x + r, y + q), carried
+    float va[IT], vc[IT];
+    int x = lane % T, y = lane / T;
+#pragma unroll
+    for (int it = 0; it < IT; it++) {   // past N: a clamped read that is not stored
+        va[it] = lumaAt(P.la, b0x - sx - R + x, b0y - sy - R + y, P.w, P.h);
+        vc[it] = lumaAt(P.lc, b0x + sx - R + x, b0y + sy - R + y, P.w, P.h);
+        x += r;
+        y += q;
+        if (x >= T) {
+            x -= T;
+            y++;
+        }
+    }
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int i = lane + 64 * it;
+        if (i < N) {
+            A[i] = va[it];
+            C[i] = vc[it];
+        }
+    }
+}
+
 // step 2 for the two blocks of a wave (T = 8 + 2R <= MAXT): every lane's loads of both windows issued before any store,
 // so the two windows wait on one memory latency (element i of a window as in searchWindows)
 template <int MAXT>
@@ -395,7 +425,8 @@ __device__ __forceinline__ void searchBody(const SearchParams& P, int bx0, int) 
     const int R = P.R, D = 2 * R + 1;
     float* A = ta[wave];
     float* C = tc[wave];
-    searchWindows(P, A, C, bx * kBlock, by * kBlock, sx, sy, R, lane);
+    if constexpr (MAXR <= 6) searchWindowsIssued<MAXT>(P, A, C, bx * kBlock, by * kBlock, sx, sy, R, lane);
+    else searchWindows(P, A, C, bx * kBlock, by * kBlock, sx, sy, R, lane);
     __syncthreads();
     float bestc = FLT_MAX, bests = 0.0f;
     int besti = INT_MAX;
