@@ -364,15 +364,15 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupParams P) {
 }
 
 // fallback.comp:9-24: more than globalFallback x W x H untrusted pixels -> the nearer real frame. Every workgroup
-// exits at once in the normal case; 64x64 pixels per workgroup.
+// exits at once in the normal case (few of them: 128x128 pixels per workgroup).
 __device__ __forceinline__ void fallbackBody(const FallbackParams& P, int gbx, int gby) {
     if (gbx == 0 && gby == 0 && threadIdx.x < kSlots) P.nextCount[threadIdx.x] = 0u;
     const bool flick = (float)slotSum(P.count) > P.gfrac * (float)P.W * (float)P.H;
     if (!flick) return;
     const FrameRef& nf = P.nearC ? P.c : P.a;
-    const int bx = gbx * 64, by = gby * 64;
-    for (int i = threadIdx.x; i < 64 * 64; i += 256) {
-        const int x = bx + (i & 63), y = by + (i >> 6);
+    const int bx = gbx * 128, by = gby * 128;
+    for (int i = threadIdx.x; i < 128 * 128; i += 256) {
+        const int x = bx + (i & 127), y = by + (i >> 7);
         if (x >= P.W || y >= P.H) continue;
         const uint8_t* src = nf.p + (int64_t)y * nf.pitch + (int64_t)x * P.bpp;
         uint8_t* dst = P.out + (int64_t)y * P.opitch + (int64_t)x * P.bpp;
@@ -446,8 +446,8 @@ hipError_t launchLookup(const LookupParams& p, hipStream_t s) {
 
 int synthTilesX(int W) { return cdiv(W, TW); }
 int synthTilesY(int H, int rows) { return cdiv(H, rows); }
-int fallbackTilesX(int W) { return cdiv(W, 64); }
-int fallbackTilesY(int H) { return cdiv(H, 64); }
+int fallbackTilesX(int W) { return cdiv(W, 128); }
+int fallbackTilesY(int H) { return cdiv(H, 128); }
 
 hipError_t launchSynthSingle(int pass, const void* params, int gx, int gy, hipStream_t s) {
     const dim3 g((unsigned)gx, (unsigned)gy);
