@@ -294,9 +294,42 @@ def estimate(a, c, s, lv):
     return res
 
 
+def lookup_t(pf, t):
+    """lookup.comp:10-19 per flow cell: s = float32(2 t - 1) folded in double from the float t (framegen.cpp:3490-3493),
+    then twice g = sampleF(pf, q - s g), starting from g = pf(q). Returns the field at time t."""
+    td = float(f32(t))
+    s = f32(2.0 * td - 1.0)
+    fh, fw = pf.shape[:2]
+    out = np.zeros_like(pf)
+    for qy in range(fh):
+        for qx in range(fw):
+            g = [pf[qy, qx, 0], pf[qy, qx, 1]]
+            for _ in range(2):
+                px = f32(f32(qx) - f32(s * g[0]))
+                py = f32(f32(qy) - f32(s * g[1]))
+                g = [sample(pf[..., 0], px, py), sample(pf[..., 1], px, py)]
+            out[qy, qx] = g
+    return out
+
+
 def synth_t05(a, c, pf, S, s):
     """error.comp + resolve.comp at t = 0.5 (no lookup), scalar. Returns (out, untrusted count)."""
+    return synth_t(a, c, pf, S, s, 0.5)
+
+
+def synth_t(a, c, pf, S, s, t):
+    """lookup.comp (t != 0.5) + error.comp + resolve.comp at any t in (0, 1), scalar, with the constants of
+    framegen.cpp:3490-3497 (wA = float32(1 - t), wC = float32(t) from the double t; warp factors 2 wC, 2 wA) and the
+    blend of resolve.comp:73-82 (0.5 (wa + wc) at t = 0.5, else wA wa + wC wc; near = prev for t <= 0.5, else curr).
+    Returns (out, untrusted count)."""
     H, W = a.shape[:2]
+    td = float(f32(t))
+    wA, wC = f32(1.0 - td), f32(td)
+    mid = wC == f32(0.5)
+    if not mid:
+        pf = lookup_t(pf, t)
+    ka, kc = f32(f32(2) * wC), f32(f32(2) * wA)
+    near = c if wC > f32(0.5) else a
     fh, fw = pf.shape[:2]
 
     def hv(x, y):
@@ -329,10 +362,10 @@ def synth_t05(a, c, pf, S, s):
     for y in range(H):
         for x in range(W):
             h = hv(x, y)
-            wa[y, x] = warp(a, f32(f32(x) - h[0]), f32(f32(y) - h[1]))
-            wc[y, x] = warp(c, f32(f32(x) + h[0]), f32(f32(y) + h[1]))
+            wa[y, x] = warp(a, f32(f32(x) - f32(ka * h[0])), f32(f32(y) - f32(ka * h[1])))
+            wc[y, x] = warp(c, f32(f32(x) + f32(kc * h[0])), f32(f32(y) + f32(kc * h[1])))
             e[y, x] = max(abs(int(wa[y, x, k]) - int(wc[y, x, k])) for k in range(3))
-    out = a.copy()
+    out = near.copy()
     count = 0
     lo, hi = f32(s["trustLow"]), f32(s["trustHigh"])
     for y in range(H):
@@ -344,7 +377,10 @@ def synth_t05(a, c, pf, S, s):
             if u > f32(0.5):
                 count += 1
             for k in range(3):
-                m = f32(f32(0.5) * f32(wa[y, x, k] + wc[y, x, k]))
-                o = f32(f32(m * f32(f32(1) - u)) + f32(f32(a[y, x, k]) * u))
+                if mid:
+                    m = f32(f32(0.5) * f32(wa[y, x, k] + wc[y, x, k]))
+                else:
+                    m = f32(f32(wA * f32(wa[y, x, k])) + f32(wC * f32(wc[y, x, k])))
+                o = f32(f32(m * f32(f32(1) - u)) + f32(f32(near[y, x, k]) * u))
                 out[y, x, k] = clampi(int(np.floor(f32(o + f32(0.5)))), 0, 255)
     return out, count

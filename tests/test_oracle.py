@@ -8,6 +8,7 @@ neither runs here), so each oracle function is pinned by at least one of:
     interp.py self-check thresholds, interp.py:261-291).
 """
 import math
+from fractions import Fraction
 
 import numpy as np
 import pytest
@@ -154,6 +155,117 @@ def test_pix_candidates_closed_forms():
     assert np.array_equal(c[1, 9, 17], bfield[1, 2]) and np.array_equal(c[2, 9, 17], bfield[0, 2])
     assert np.array_equal(c[3, 9, 17], bfield[2, 2]) and np.array_equal(c[4, 9, 17], bfield[1, 1])
     assert np.array_equal(c[5, 9, 17], bfield[1, 3]) and not c[6].any()
+
+
+# --------------------------------------------------------------------------------------------- t != 1/2
+
+def _to_f32(q):
+    """The rational q rounded to the nearest float32 (ties to even; normal range), as a Fraction."""
+    if q == 0:
+        return Fraction(0)
+    sign, a = (-1, -q) if q < 0 else (1, q)
+    e = 0
+    while Fraction(2) ** e > a:
+        e -= 1
+    while Fraction(2) ** (e + 1) <= a:
+        e += 1
+    ulp = Fraction(2) ** (e - 23)
+    return sign * round(a / ulp) * ulp
+
+
+def _affine_lookup(A, b, fw, fh, s, qx, qy, iters=2, sign=1):
+    """lookup.comp:10-19 on the affine field pf(x, y) = A (x, y) + b, in exact rationals: the float32 product s g and
+    difference q - s g are rounded with _to_f32, the map to 1/32 px half even, and the bilinear sample of an affine
+    field at (X / 32, Y / 32) is the field's value there (bilinear interpolation reproduces affine functions), as long
+    as the four taps lie inside the field. Returns the final g, or None when a tap would be clamped."""
+    def pf(x, y):
+        return (A[0][0] * x + A[0][1] * y + b[0], A[1][0] * x + A[1][1] * y + b[1])
+    g = pf(Fraction(qx), Fraction(qy))
+    for _ in range(iters):
+        px = _to_f32(qx - sign * _to_f32(s * g[0]))
+        py = _to_f32(qy - sign * _to_f32(s * g[1]))
+        X, Y = round(px * 32), round(py * 32)
+        if not (0 <= X // 32 <= fw - 2 and 0 <= Y // 32 <= fh - 2):
+            return None
+        g = pf(Fraction(X, 32), Fraction(Y, 32))
+    return g
+
+
+@pytest.mark.parametrize("t", [0.25, 0.75, 0.3])
+def test_field_at_of_an_affine_field_is_the_two_step_iteration(t):
+    """field_at against lookup.comp's g <- pf(q - s g) twice, s = float32(2t - 1), computed in exact rationals on an
+    affine dyadic field (where the bilinear sampler is exact). One iteration, a flipped sign of s or the time
+    constant of the other t give different fields on most cells, so each of those mistakes fails here."""
+    fw, fh = 40, 32
+    A = ((Fraction(3, 32), Fraction(-1, 16)), (Fraction(1, 32), Fraction(5, 32)))
+    b = (Fraction(-3, 4), Fraction(5, 4))
+    qy, qx = np.meshgrid(np.arange(fh), np.arange(fw), indexing="ij")
+    pf = np.stack([float(A[k][0]) * qx + float(A[k][1]) * qy + float(b[k]) for k in range(2)], -1).astype(f32)
+    got = v1.field_at(pf, t)
+    s = Fraction(float(f32(2.0 * float(f32(t)) - 1.0)))
+    s_other = Fraction(float(f32(2.0 * float(f32(1.0 - t)) - 1.0)))
+    checked = differ_one = differ_sign = differ_t = 0
+    for y in range(fh):
+        for x in range(fw):
+            want = _affine_lookup(A, b, fw, fh, s, x, y)
+            if want is None:
+                continue
+            checked += 1
+            assert (Fraction(float(got[y, x, 0])), Fraction(float(got[y, x, 1]))) == want, (x, y)
+            differ_one += _affine_lookup(A, b, fw, fh, s, x, y, iters=1) != want
+            differ_sign += _affine_lookup(A, b, fw, fh, s, x, y, sign=-1) not in (None, want)
+            differ_t += _affine_lookup(A, b, fw, fh, s_other, x, y) not in (None, want)
+    assert checked >= 0.6 * fw * fh
+    assert min(differ_one, differ_sign, differ_t) >= 0.5 * checked
+
+
+@pytest.mark.parametrize("t", [0.25, 0.75, 0.3])
+def test_time_blend_of_a_zero_motion_pair(t):
+    """resolve.comp:73-82 with framegen.cpp:3490-3497's constants: a zero field, frames that differ by at most 5
+    levels (every pixel trusted: u = 0), so each output sample is floor(wA a + wC c + 1/2) with wA = float32(1 - t),
+    wC = float32(t): (3a + c + 2) >> 2 at t = 1/4, (a + 3c + 2) >> 2 at t = 3/4 (swapped weights fail), the exact
+    rational value at t = 0.3 away from rounding ties. Alpha is the nearer frame's."""
+    rng = np.random.default_rng(31)
+    a = rng.integers(0, 256, (24, 40, 4), dtype=np.uint8)
+    c = np.clip(a.astype(np.int64) + rng.integers(-5, 6, a.shape), 0, 255).astype(np.uint8)
+    est = dict(pf=np.zeros((12, 20, 2), f32), cut=False, settings=v1.settings())
+    out, why, info = v1.synth(a, c, est, t)
+    assert why is None and info["count"] == 0
+    ai, ci = a[..., :3].astype(np.int64), c[..., :3].astype(np.int64)
+    if t == 0.25:
+        want = (3 * ai + ci + 2) >> 2
+    elif t == 0.75:
+        want = (ai + 3 * ci + 2) >> 2
+    else:
+        wA, wC = Fraction(float(f32(1.0 - float(f32(t))))), Fraction(float(f32(float(f32(t)))))
+        exact = [[[wA * int(ai[y, x, k]) + wC * int(ci[y, x, k]) + Fraction(1, 2) for k in range(3)]
+                  for x in range(a.shape[1])] for y in range(a.shape[0])]
+        want = np.array([[[math.floor(v) for v in px] for px in row] for row in exact])
+        far = np.array([[[abs(v - round(v)) > Fraction(1, 2 ** 14) for v in px] for px in row] for row in exact])
+        assert far.mean() > 0.7
+        got = out[..., :3].astype(np.int64)
+        assert np.array_equal(got[far], want[far])
+        assert np.abs(got - want).max() <= 1
+        return
+    assert (want != ((ai + 3 * ci + 2) >> 2 if t == 0.25 else (3 * ai + ci + 2) >> 2)).mean() > 0.5
+    assert np.array_equal(out[..., :3], want)
+    assert np.array_equal(out[..., 3], (a if t <= 0.5 else c)[..., 3])
+
+
+@pytest.mark.parametrize("t", [0.25, 0.75, 0.3])
+def test_synth_at_any_t_matches_scalar_restatement(t):
+    """The vectorised synth (lookup, halfVec, both warps, trust, time blend) equals the scalar per-cell / per-pixel
+    restatement at t != 1/2 on a pair with a moving sprite (so trust and the lookup's sampling both matter)."""
+    W, H = 80, 56
+    a, c = scenes.pair_translate(12, w=W, h=H, sprite=16, sprite_pos=(24, 16), sprite_move=(6, 4), shift=(3, -2))
+    s = v1.settings(pyramidMinSide=16)
+    est = v1.estimate(a, c, s)
+    out, why, info = v1.synth(a, c, est, t)
+    rout, rcount = bf.synth_t(a, c, est["pf"], s["flowScale"], s, t)
+    assert info["count"] == rcount and rcount > 0
+    assert why is None
+    assert np.array_equal(out, rout)
+    assert np.array_equal(info["g"], bf.lookup_t(est["pf"], t))
 
 
 # --------------------------------------------------------------------------------------------- arithmetic claims
