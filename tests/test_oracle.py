@@ -292,6 +292,20 @@ def test_trust_quotient_as_double_product_is_the_float_division(lo, hi):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("lo,hi", [(12.0, 40.0), (10.0, 30.5), (0.0, 255.0), (3.3, 7.1), (1.0, 2.5),
+                                   (17.7, 200.9), (20.0, 21.0), (5.0, 60.0), (12.0, 13.0)])
+def test_trust_division_and_resolve_product_take_the_same_untrusted_decision(lo, hi):
+    """The oracle's trust is interp.py's float32 quotient (e7 - lo) / (hi - lo); resolve.comp:39 multiplies by the
+    reciprocal rounded on the CPU. Over every 7x7 sum the two agree on u > 0.5 (the counted, flick-deciding bit)."""
+    S = np.arange(0, 49 * 255 + 1)
+    e7 = (S.astype(np.float64) * (1.0 / 49.0)).astype(f32)
+    lo32, hi32 = f32(lo), f32(hi)
+    x = (e7 - lo32).astype(f32)
+    quot = np.clip((x / f32(hi32 - lo32)).astype(f32), 0, 1)
+    prod = np.clip((x * f32(1.0 / (float(hi32) - float(lo32)))).astype(f32), 0, 1)
+    assert np.array_equal(quot > f32(0.5), prod > f32(0.5))
+
+
 def test_sums_are_exact_at_the_workload_sizes():
     """Level-k luma at flow scale 2 has 2 + 2k fractional bits; a 64-pixel SAD stays below 2^24 units up to level 4
     (the 4K pyramid), so float32 sums of it are exact in any order."""
