@@ -216,7 +216,17 @@ static std::map<std::string, Tuning>& tuningCache() {
 static const char* kBErrNull = "null batch description";
 static const char* kBErrCount = "a batch holds 1 to 1024 pairs";
 static const char* kBErrMult = "a pair's multiplier must be in [2, 16]";
-static const char* kBErrFrames = "a pair has a null frame or pitch";
+static const char* kBErrFrames =
+    "a pair's frame has a null plane or a pitch below a row's bytes, of 2^24 or more, or with a last row past 2^32";
+
+// every frame of a pair (prev, curr and the multiplier - 1 outputs) addressable at the pair's size and format
+static bool pairFramesFit(const FgrBatchPair& d) {
+    if (!frameFits(d.prev, d.width, d.height, (int)d.format) || !frameFits(d.curr, d.width, d.height, (int)d.format))
+        return false;
+    for (uint32_t k = 0; k + 1 < d.multiplier; k++)
+        if (!frameFits(d.out[k], d.width, d.height, (int)d.format)) return false;
+    return true;
+}
 static const char* kBErrAlloc = "a device allocation of the batch failed";
 static const char* kBErrStream = "the batch's capture stream could not be created";
 
@@ -227,12 +237,7 @@ extern "C" FgrBatch* fgr_batch_create(const FgrBatchInfo* info, const FgrBatchPa
     if (info->settings.uiProtection != FGR_UI_OFF) { gBatchError = kBErrUi; return nullptr; }
     for (uint32_t p = 0; p < n; p++) {
         if (pairs[p].multiplier < 2 || pairs[p].multiplier > 16) { gBatchError = kBErrMult; return nullptr; }
-        if (!pairs[p].prev.ptr || !pairs[p].curr.ptr || !pairs[p].prev.pitch || !pairs[p].curr.pitch) {
-            gBatchError = kBErrFrames;
-            return nullptr;
-        }
-        for (uint32_t k = 0; k + 1 < pairs[p].multiplier; k++)
-            if (!pairs[p].out[k].ptr || !pairs[p].out[k].pitch) { gBatchError = kBErrFrames; return nullptr; }
+        if (!pairFramesFit(pairs[p])) { gBatchError = kBErrFrames; return nullptr; }
     }
     FgrBatch* b = new FgrBatch();
     b->info = *info;
@@ -301,7 +306,8 @@ extern "C" int fgr_batch_set_frames(FgrBatch* b, void* stream, const FgrBatchPai
     if (!b || !pairs || n != b->desc.size()) return 0;
     for (uint32_t p = 0; p < n; p++)
         if (pairs[p].width != b->desc[p].width || pairs[p].height != b->desc[p].height ||
-            pairs[p].multiplier != b->desc[p].multiplier || pairs[p].format != b->desc[p].format)
+            pairs[p].multiplier != b->desc[p].multiplier || pairs[p].format != b->desc[p].format ||
+            !pairFramesFit(pairs[p]))
             return 0;
     b->desc.assign(pairs, pairs + n);
     b->build();

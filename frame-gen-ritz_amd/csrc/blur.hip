@@ -119,24 +119,38 @@ __global__ __launch_bounds__(256) void blur_average_kernel(BlurParams P) {
 
 using namespace fgr;
 
+size_t FgrInterpolator::blurBytes(int n) const {
+    const bool yuvFmt = format == FGR_FORMAT_P010 || format == FGR_FORMAT_NV12;
+    const size_t plane = (size_t)W * H * bytesPerPixel();
+    const size_t cplane = yuvFmt ? (size_t)((W + 1) / 2) * ((H + 1) / 2) * (format == FGR_FORMAT_NV12 ? 2 : 4) : 0;
+    return (size_t)n * (plane + cplane);
+}
+
+bool FgrInterpolator::blurEnsure() {
+    const int n = (int)set.blurSamples;
+    if (n < 2 || !W) return true;
+    const size_t need = blurBytes(n);
+    if (need <= blurCap) return true;
+    uint8_t* p = nullptr;
+    if (hipMalloc(&p, need) != hipSuccess) return false;
+    if (blurScratch) blurRetired.push_back(blurScratch);
+    blurScratch = p;
+    blurCap = need;
+    return true;
+}
+
 bool FgrInterpolator::enqueueBlur(hipStream_t s, const FgrFrame& prev, const FgrFrame& curr, const FgrFrame& out,
                                   float t0, float t1) {
     const int n = (int)set.blurSamples;
     if (n < 2 || n > FGR_BLUR_MAX_SAMPLES) return false;
-    // scratch: one sub-frame per sample (plane 0, then the chroma plane of Nv12 / P010)
+    // scratch: one sub-frame per sample (plane 0, then the chroma plane of Nv12 / P010), sized by blurEnsure when the
+    // settings were applied
     const int bpp = bytesPerPixel();
     const bool yuvFmt = format == FGR_FORMAT_P010 || format == FGR_FORMAT_NV12;
     const int cbpp = format == FGR_FORMAT_NV12 ? 2 : 4;
     const size_t plane = (size_t)W * H * bpp;
     const size_t cplane = yuvFmt ? (size_t)((W + 1) / 2) * ((H + 1) / 2) * cbpp : 0;
-    const size_t need = (size_t)n * (plane + cplane);
-    if (need > blurCap) {
-        if (blurScratch) hipFree(blurScratch);
-        blurScratch = nullptr;
-        blurCap = 0;
-        if (hipMalloc(&blurScratch, need) != hipSuccess) return false;
-        blurCap = need;
-    }
+    if (!blurScratch || blurBytes(n) > blurCap) return false;
     BlurParams bp{};
     bp.W = (int)W;
     bp.H = (int)H;

@@ -151,6 +151,8 @@ void FgrInterpolator::freeBuffers() {
     if (u) hipFree(u);
     if (seedAct) hipFree(seedAct);
     if (blurScratch) hipFree(blurScratch);
+    for (uint8_t* p : blurRetired) hipFree(p);
+    blurRetired.clear();
     blurScratch = nullptr;
     blurCap = 0;
     luma = nullptr;
@@ -231,7 +233,7 @@ extern "C" FgrInterpolator* fgr_create(const FgrInitInfo* info) {
     h->set = info->settings;
     h->device = info->device;
     h->format = (int)info->format;
-    if (!h->formatInit() || !h->rebuild(info->width, info->height) || !h->uiEnsure()) {
+    if (!h->formatInit() || !h->rebuild(info->width, info->height) || !h->uiEnsure() || !h->blurEnsure()) {
         gInitError = kErrAlloc;
         fgr_destroy(h);
         return nullptr;
@@ -260,7 +262,7 @@ extern "C" int fgr_resize(FgrInterpolator* h, uint32_t width, uint32_t height) {
         return 0;
     }
     if (!h->rebuild(width, height)) return 0;
-    if (!h->uiEnsure()) {   // (as a failed allocation of the size-dependent buffers: no size held)
+    if (!h->uiEnsure() || !h->blurEnsure()) {   // (as a failed allocation of the size-dependent buffers: no size held)
         h->freeBuffers();
         return 0;
     }
@@ -285,7 +287,7 @@ extern "C" int fgr_set_settings(FgrInterpolator* h, const FgrSettings* s) {
     const FgrSettings old = h->set;
     const UiRect oldBox = h->uiBox(), oldWork = h->uiWork();
     h->set = *s;
-    if (!h->uiEnsure()) {
+    if (!h->uiEnsure() || !h->blurEnsure()) {
         h->set = old;
         return 0;
     }
@@ -304,13 +306,13 @@ extern "C" int fgr_reconfigure(FgrInterpolator* h, const FgrSettings* s) {
     if (structural && h->W) {
         // (buffers gone: the object holds no size, as a failed resize)
         if (!h->rebuild(h->W, h->H)) return 0;
-        if (!h->uiEnsure()) {
+        if (!h->uiEnsure() || !h->blurEnsure()) {
             h->freeBuffers();
             return 0;
         }
         return 1;
     }
-    if (!h->uiEnsure()) {
+    if (!h->uiEnsure() || !h->blurEnsure()) {
         h->set = old;
         return 0;
     }
@@ -374,11 +376,25 @@ extern "C" int fgr_read_timings(FgrInterpolator* h, const char** names, uint64_t
 
 // ------------------------------------------------------------------------------------------------------ records
 
-bool FgrInterpolator::frameOk(const FgrFrame& f) const {
-    if (!f.ptr || f.pitch == 0) return false;
-    if ((format == FGR_FORMAT_P010 || format == FGR_FORMAT_NV12) && !f.chroma) return false;
-    return true;
+static bool planeFits(const void* p, uint64_t pitch, uint64_t rowBytes, uint64_t rows) {
+    return p && pitch >= rowBytes && pitch < (1ull << 24) && (rows - 1) * pitch + rowBytes < (1ull << 32);
 }
+
+bool frameFits(const FgrFrame& f, uint32_t W, uint32_t H, int format) {
+    switch (format) {
+        case FGR_FORMAT_RGBA8:
+        case FGR_FORMAT_RGB10: return planeFits(f.ptr, f.pitch, 4ull * W, H);
+        case FGR_FORMAT_RGBAF16: return planeFits(f.ptr, f.pitch, 8ull * W, H);
+        case FGR_FORMAT_NV12:
+            return planeFits(f.ptr, f.pitch, W, H) && planeFits(f.chroma, f.chromaPitch, 2ull * ((W + 1) / 2), (H + 1) / 2);
+        case FGR_FORMAT_P010:
+            return planeFits(f.ptr, f.pitch, 2ull * W, H) &&
+                   planeFits(f.chroma, f.chromaPitch, 4ull * ((W + 1) / 2), (H + 1) / 2);
+        default: return false;
+    }
+}
+
+bool FgrInterpolator::frameOk(const FgrFrame& f) const { return W && frameFits(f, W, H, format); }
 
 FrameRef FgrInterpolator::ref(const FgrFrame& f) const {
     FrameRef r;

@@ -42,6 +42,11 @@ struct UiEntry {
     int pitch = 0;
 };
 
+// A frame the kernels can address for a W x H pair of `format`: non-null planes (the CbCr plane of Nv12 / P010 too),
+// a pitch of at least one row's bytes, below 2^24 (rows are addressed as 24-bit products), and every row offset plus
+// a row's bytes below 2^32 (32-bit plane offsets).
+bool frameFits(const FgrFrame& f, uint32_t W, uint32_t H, int format);
+
 struct FgrInterpolator {
     static constexpr int kMaxQ = 64;
 
@@ -76,8 +81,14 @@ struct FgrInterpolator {
     // learned seed proposals (seednet.hip): the attached network and its activations on the coarsest level
     FgrSeedNet* net = nullptr;
     int synthRows = 32;               // RGBA8 synth tile height (16, 32, 64 with flowScale 2), chosen by the batch tuner
-    uint8_t* blurScratch = nullptr;   // recordSynthBlur's sub-frames (blur.hip), grown by the first blurred record
+    // recordSynthBlur's sub-frames (blur.hip): sized when settings or sizes are applied with blurSamples >= 2 and only
+    // grown; a replaced buffer is kept (records queued before the settings change may still read it) until the
+    // handle's buffers are freed, so no record allocates
+    uint8_t* blurScratch = nullptr;
     size_t blurCap = 0;
+    std::vector<uint8_t*> blurRetired;
+    size_t blurBytes(int n) const;
+    bool blurEnsure();
     float* seedAct = nullptr;
     bool allocSeedBuffers();
     void planSeeds(std::vector<PlanOp>& plan) const;

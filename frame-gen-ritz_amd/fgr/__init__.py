@@ -486,8 +486,10 @@ class Batch:
             raise RuntimeError("fgr_batch_record failed")
 
     def set_frames(self, items, stream=None):
-        self._keep = items
-        return bool(lib.fgr_batch_set_frames(self._h, _stream(stream), _batch_pairs(items), len(items)))
+        ok = bool(lib.fgr_batch_set_frames(self._h, _stream(stream), _batch_pairs(items), len(items)))
+        if ok:   # a refused call keeps the batch's old frames, so their tensors stay referenced
+            self._keep = items
+        return ok
 
     def launches(self):
         return lib.fgr_batch_launches(self._h)

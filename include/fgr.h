@@ -12,7 +12,11 @@
  *     untouched (framegen.h:537-548). Reasons are string literals (fgr_init_error, fgr_settings_error, fgr_size_error).
  *   - Frames are caller-owned device memory: FgrFrame{ptr, pitch} (pitch in bytes), plus the CbCr plane for Nv12 /
  *     P010.
- *     The library allocates its own buffers with hipMalloc only in fgr_create / fgr_resize / fgr_reconfigure.
+ *     A frame is refused when a plane is null, its pitch is below one row's bytes or 2^24 or more, or its last row
+ *     ends at 2^32 bytes or beyond.
+ *   - The library allocates its own buffers with hipMalloc only in fgr_create / fgr_resize / fgr_reconfigure /
+ *     fgr_set_settings (the last two when a runtime setting needs a larger buffer: the UI box, blurSamples); it never
+ *     allocates or frees inside a record.
  */
 #ifndef FGR_H
 #define FGR_H

@@ -176,3 +176,21 @@ def test_autotune_picks_the_fastest_and_caches_it(tmp_path):
     assert fgr.tuning_save(path) and fgr.tuning_load(path) >= 1
     b.close()
     b2.close()
+
+
+def test_set_frames_refuses_frames_it_cannot_address():
+    """fgr_batch_set_frames checks every new frame as fgr_batch_create does (null planes, pitches), before any
+    upload: the batch keeps its old frames and still matches the per-pair path."""
+    a, c = scenes.pair_translate(41, w=192, h=128)
+    b, items = make_batch([(a, c)], [2])
+    ta, tc, n, outs = items[0]
+    bad = fgr.frame_of(tc)
+    bad.pitch = 192 * 4 - 4
+    assert not b.set_frames([(ta, bad, n, outs)])
+    null = fgr.frame_of(outs[0])
+    null.ptr = None
+    assert not b.set_frames([(ta, tc, n, [null])])
+    b.run()
+    torch.cuda.synchronize()
+    assert np.array_equal(outs[0].cpu().numpy(), per_pair(a, c, 2)[0])
+    b.close()

@@ -74,3 +74,22 @@ def test_blur_formats(fmt, linear):
     want = F.synth_blur(A, C, est, 0.25, 0.75, 4, fmt, gauss=True, lut=lut, yuv=F.yuv_constants(matrix_of(fmt)))
     assert same(to_host(out, fmt), want, fmt)
     fi.close()
+
+
+def test_records_queued_before_a_larger_blur_keep_their_outputs():
+    """A blurred record queued with blurSamples 2, then settings raising it to 16 (a larger sub-frame scratch) and a
+    second blurred record, synchronised only at the end: both outputs equal the oracle's (the first record's scratch
+    outlives the settings change; no record allocates)."""
+    a, c = scenes.pair_translate(35, w=320, h=180, sprite=40, sprite_pos=(100, 50), shift=(5, -3))
+    fi = fgr.Interpolator(320, 180, settings=fgr.default_settings(blurSamples=2))
+    ta, tc = dev(a), dev(c)
+    assert fi.record_estimate(ta, tc)
+    o1, o2 = torch.zeros_like(ta), torch.zeros_like(ta)
+    assert fi.record_synth_blur(ta, tc, o1, 0.1, 0.6)
+    assert fi.set_settings(fgr.default_settings(blurSamples=16))
+    assert fi.record_synth_blur(ta, tc, o2, 0.1, 0.6)
+    torch.cuda.synchronize()
+    est = v1.estimate(a, c)
+    assert np.array_equal(o1.cpu().numpy(), v1.synth_blur(a, c, est, 0.1, 0.6, 2))
+    assert np.array_equal(o2.cpu().numpy(), v1.synth_blur(a, c, est, 0.1, 0.6, 16))
+    fi.close()

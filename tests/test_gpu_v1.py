@@ -186,6 +186,30 @@ def test_refusals_record_nothing():
     fi.close()
 
 
+def test_frames_the_kernels_cannot_address_are_refused():
+    """A pitch below a row's bytes, of 2^24 or more, or a null plane: the record returns 0 and enqueues nothing."""
+    a, c = scenes.pair_translate(8, w=128, h=96)
+    fi = fgr.Interpolator(128, 96)
+    ta, tc = dev(a), dev(c)
+    out = torch.full_like(ta, 7)
+    for pitch in (128 * 4 - 4, 1 << 24):
+        bad = fgr.frame_of(tc)
+        bad.pitch = pitch
+        assert not fi.record_estimate(ta, bad)
+        assert not fi.record(ta, bad, out, 0.5)
+    null = fgr.frame_of(tc)
+    null.ptr = None
+    assert not fi.record_estimate(ta, null)
+    assert fi.record_estimate(ta, tc)
+    o = fgr.frame_of(out)
+    o.pitch = 100
+    assert not fi.record_synth(ta, tc, o, 0.5)
+    assert not fi.record_synth_blur(ta, tc, o, 0.25, 0.75)
+    torch.cuda.synchronize()
+    assert int(out.min()) == 7 and int(out.max()) == 7
+    fi.close()
+
+
 @pytest.mark.parametrize("k", [0, 7])
 def test_1080p_workload_pair(k):
     pairs, _ = scenes.pairs_1080p(2, n_pairs=8)

@@ -91,3 +91,30 @@ def test_create_refusals_name_the_reason():
         fgr.Interpolator(32768, 32769)
     with pytest.raises(RuntimeError, match="linearWhiteNits"):
         fgr.Interpolator(64, 64, fmt=fgr.FORMAT_RGBAF16, transfer=fgr.TRANSFER_LINEAR, linear_white_nits=0.0)
+
+
+def _batch_pair(fmt, W, H, pitch, chroma=0x2000000, cpitch=None, n=2):
+    bp = fgr.BatchPair()
+    bp.width, bp.height, bp.format, bp.multiplier = W, H, fmt, n
+    for f in [bp.prev, bp.curr] + [bp.out[k] for k in range(n - 1)]:
+        f.ptr, f.pitch = 0x1000000, pitch     # never dereferenced: the checks run before any device call
+        f.chroma, f.chromaPitch = chroma, cpitch if cpitch is not None else pitch
+    return bp
+
+
+@pytest.mark.parametrize("fmt,W,H,pitch,chroma,cpitch", [
+    (fgr.FORMAT_P010, 64, 32, 128, 0, 128),               # P010 without its CbCr plane
+    (fgr.FORMAT_NV12, 64, 32, 64, 0, 64),                 # Nv12 without its CbCr plane
+    (fgr.FORMAT_RGBA8, 64, 32, 255, 0, None),             # pitch below a row (64 x 4 bytes)
+    (fgr.FORMAT_RGBAF16, 64, 32, 511, 0, None),           # RgbaF16: 8 bytes a pixel
+    (fgr.FORMAT_RGBA8, 64, 32, 1 << 24, 0, None),         # pitch of 2^24: rows are 24-bit products
+    (fgr.FORMAT_RGBA8, 4096, 32768, 131076 + 4, 0, None), # last row ends past 2^32
+    (fgr.FORMAT_P010, 64, 32, 128, 0x2000000, 127),       # CbCr pitch below 32 x 4 bytes
+])
+def test_batch_refuses_frames_it_cannot_address(fmt, W, H, pitch, chroma, cpitch):
+    bp = _batch_pair(fmt, W, H, pitch, chroma, cpitch)
+    info = fgr.BatchInfo()
+    info.settings = fgr.default_settings()
+    h = fgr.lib.fgr_batch_create(ctypes.byref(info), ctypes.byref(bp), 1)
+    assert not h
+    assert "pitch" in fgr.lib.fgr_batch_last_error().decode()
