@@ -90,6 +90,6 @@ def test_records_queued_before_a_larger_blur_keep_their_outputs():
     assert fi.record_synth_blur(ta, tc, o2, 0.1, 0.6)
     torch.cuda.synchronize()
     est = v1.estimate(a, c)
-    assert np.array_equal(o1.cpu().numpy(), v1.synth_blur(a, c, est, 0.1, 0.6, 2))
-    assert np.array_equal(o2.cpu().numpy(), v1.synth_blur(a, c, est, 0.1, 0.6, 16))
+    assert np.array_equal(o1.cpu().numpy(), v1.synth_blur(a, c, est, 0.1, 0.6, 2, gauss=True))
+    assert np.array_equal(o2.cpu().numpy(), v1.synth_blur(a, c, est, 0.1, 0.6, 16, gauss=True))
     fi.close()
