@@ -217,8 +217,8 @@ def luma_level0(frame, fmt, S, fw, fh, **kw):
     return sums.astype(f32) / (16 * n).astype(f32)
 
 
-def estimate(a, c, fmt, s=None, bgra=False, lut=None, full_range=False):
-    """v1.estimate on the format's luma."""
+def estimate(a, c, fmt, s=None, bgra=False, lut=None, full_range=False, proposals=None):
+    """v1.estimate on the format's luma (proposals: as v1.estimate's, learned seeds of the coarsest level)."""
     s = s or v1.DEFAULTS
     H, W = (a[0] if isinstance(a, tuple) else a).shape[:2]
     S = s["flowScale"]
@@ -226,7 +226,7 @@ def estimate(a, c, fmt, s=None, bgra=False, lut=None, full_range=False):
     kw = dict(bgra=bgra, lut=lut, full_range=full_range)
     la = luma_level0(a, fmt, S, fw, fh, **kw)
     lc = luma_level0(c, fmt, S, fw, fh, **kw)
-    return v1.estimate(None, None, s, luma0=(la, lc), size=(W, H))
+    return v1.estimate(None, None, s, luma0=(la, lc), size=(W, H), proposals=proposals)
 
 
 # ------------------------------------------------------------------------------------------------ warps
