@@ -116,12 +116,12 @@ struct FgrBatch {
             L.stage = std::get<0>(g.first);
             L.pass = std::get<1>(g.first);
             L.lane = std::get<2>(g.first);
-            const size_t psz = g.second[0].second->params.size();
+            const size_t psz = g.second[0].second->size;
             L.paramOff = blob.size();
             L.paramBytes = psz * g.second.size();
             blob.resize(align256(blob.size() + L.paramBytes));
             for (size_t i = 0; i < g.second.size(); i++)
-                std::memcpy(blob.data() + L.paramOff + i * psz, g.second[i].second->params.data(), psz);
+                std::memcpy(blob.data() + L.paramOff + i * psz, g.second[i].second->params, psz);
             std::vector<uint32_t> work;
             for (size_t i = 0; i < g.second.size(); i++) {
                 const PlanOp& op = *g.second[i].second;

@@ -99,7 +99,7 @@ extern "C" int fgr_validate(const FgrSettings* s) {
            s->uiHistory <= FGR_UI_FRAMES_MAX;
 }
 
-static bool formatBuilt(uint32_t f) {
+bool formatBuilt(uint32_t f) {
     return f == FGR_FORMAT_RGBA8 || f == FGR_FORMAT_RGB10 || f == FGR_FORMAT_RGBAF16 || f == FGR_FORMAT_NV12 ||
            f == FGR_FORMAT_P010;
 }
@@ -134,7 +134,7 @@ extern "C" int fgr_size_layout(uint32_t W, uint32_t H, uint32_t flowScale, uint3
 
 extern "C" uint32_t fgr_version(void) { return 0x00010000u; }
 
-static const char* sizeLimitError(uint32_t W, uint32_t H, uint32_t format) {
+const char* sizeLimitError(uint32_t W, uint32_t H, uint32_t format) {
     // framegen.cpp:760-790 (the device limits here: 32-bit indexing and the launch grid)
     if (W < FGR_MIN_SIZE || H < FGR_MIN_SIZE) return kErrMinSize;
     if ((uint64_t)W * H > (1ull << 30)) return kErrPixels;
@@ -164,7 +164,7 @@ void FgrInterpolator::freeBuffers() {
     haveEst = false;
 }
 
-bool FgrInterpolator::allocBuffers(uint32_t w, uint32_t h) {
+void FgrInterpolator::layoutBuffers(uint32_t w, uint32_t h) {
     FgrSizeLayout L;
     fgr_size_layout(w, h, set.flowScale, set.pyramidMinSide, &L);
     fw = L.fw;
@@ -195,6 +195,20 @@ bool FgrInterpolator::allocBuffers(uint32_t w, uint32_t h) {
     fieldOff += fw * fh;
     lumaWords = lumaOff;
     fieldEntries = fieldOff;
+}
+
+static inline size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+size_t FgrInterpolator::bufferBytes() const {
+    return align256((size_t)lumaWords * 4) + align256((size_t)fieldEntries * 8) + (size_t)kUWords * 4;
+}
+
+size_t FgrInterpolator::bufferBytesNoDebug() const {
+    return align256((size_t)lumaWords * 4) + align256((size_t)fieldT * 8) + (size_t)kUWords * 4;
+}
+
+bool FgrInterpolator::allocBuffers(uint32_t w, uint32_t h) {
+    layoutBuffers(w, h);
     if (hipMalloc(&luma, (size_t)lumaWords * 4) != hipSuccess) return false;
     if (hipMalloc(&field, (size_t)fieldEntries * 8) != hipSuccess) return false;
     if (hipMalloc(&u, kUWords * 4) != hipSuccess) return false;
@@ -628,9 +642,9 @@ void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& pr
 
 bool FgrInterpolator::runPlan(hipStream_t s, const std::vector<PlanOp>& plan) {
     for (const PlanOp& op : plan) {
-        const hipError_t e = op.pass == kPassUiPatch ? launchUi(kUiPatch, *(const UiParams*)op.params.data(), s)
-                             : op.pass == kPassUiFill ? launchUi(kUiFill, *(const UiParams*)op.params.data(), s)
-                                                      : launchSingle(op.pass, op.params.data(), op.gx, op.gy, s);
+        const hipError_t e = op.pass == kPassUiPatch ? launchUi(kUiPatch, *(const UiParams*)op.params, s)
+                             : op.pass == kPassUiFill ? launchUi(kUiFill, *(const UiParams*)op.params, s)
+                                                      : launchSingle(op.pass, op.params, op.gx, op.gy, s);
         if (e != hipSuccess) return false;
         if (op.stamp) stamp(s, op.stamp);
     }
@@ -638,11 +652,11 @@ bool FgrInterpolator::runPlan(hipStream_t s, const std::vector<PlanOp>& plan) {
 }
 
 bool FgrInterpolator::enqueueEstimate(hipStream_t s, const FgrFrame& prev, const FgrFrame& curr) {
-    std::vector<PlanOp> plan;
-    planEstimate(plan, prev, curr);
+    planBuf.clear();
+    planEstimate(planBuf, prev, curr);
     if (!profPending) beginProfile(s);   // else an observe opened this profile
     profPending = false;
-    if (!runPlan(s, plan)) return false;
+    if (!runPlan(s, planBuf)) return false;
     blockFieldOff = set.subpixel ? fieldSub : lv[0].fM;
     return true;
 }
@@ -776,9 +790,9 @@ void FgrInterpolator::planSynth(std::vector<PlanOp>& plan, const FgrFrame& prev,
 }
 
 bool FgrInterpolator::enqueueSynth(hipStream_t s, const FgrFrame& prev, const FgrFrame& curr, const FgrFrame& out, float t) {
-    std::vector<PlanOp> plan;
-    planSynth(plan, prev, curr, out, t, synthK);
-    if (!runPlan(s, plan)) return false;
+    planBuf.clear();
+    planSynth(planBuf, prev, curr, out, t, synthK);
+    if (!runPlan(s, planBuf)) return false;
     lastSlot = synthK % kSynthBanks;
     lastT = t;
     synthK++;

@@ -15,13 +15,16 @@ struct Level {
 // One kernel launch of a record, before it is launched: the pass, its stage in the record (ops of the same stage of
 // different pairs are independent and run as one batched launch), the per-pair grid and the parameter block.
 struct PlanOp {
+    static constexpr size_t kMaxParams = 512;
     int pass = 0, stage = 0, gx = 1, gy = 1;
     const char* stamp = nullptr;   // profile group that ends with this op (the single-pair path)
-    std::vector<uint8_t> params;
+    size_t size = 0;
+    alignas(16) uint8_t params[kMaxParams];   // the parameter block, inline: planning a record allocates nothing
     template <class T>
     void set(const T& p) {
-        params.resize(sizeof(T));
-        std::memcpy(params.data(), &p, sizeof(T));
+        static_assert(sizeof(T) <= kMaxParams, "parameter block larger than PlanOp::kMaxParams");
+        size = sizeof(T);
+        std::memcpy(params, &p, sizeof(T));
     }
 };
 
@@ -46,6 +49,9 @@ struct UiEntry {
 // a pitch of at least one row's bytes, below 2^24 (rows are addressed as 24-bit products), and every row offset plus
 // a row's bytes below 2^32 (32-bit plane offsets).
 bool frameFits(const FgrFrame& f, uint32_t W, uint32_t H, int format);
+// fgr_create's checks: a format this library builds, and a size its indexing and launch grids hold (nullptr = fine)
+bool formatBuilt(uint32_t format);
+const char* sizeLimitError(uint32_t W, uint32_t H, uint32_t format);
 
 struct FgrInterpolator {
     static constexpr int kMaxQ = 64;
@@ -128,7 +134,12 @@ struct FgrInterpolator {
     std::vector<const char*> qnames;
     int qcount = 0;
 
+    std::vector<PlanOp> planBuf;      // the launch plan of the latest record, its capacity reused by the next
+    size_t bufferBytes() const;       // luma + fields + counters of the current layout, each 256-byte aligned
+    size_t bufferBytesNoDebug() const;   // the same without the debug maps' field at t (last in the field buffer)
+
     void freeBuffers();
+    void layoutBuffers(uint32_t w, uint32_t h);
     bool allocBuffers(uint32_t w, uint32_t h);
     bool rebuild(uint32_t w, uint32_t h);
     bool frameOk(const FgrFrame& f) const;

@@ -127,6 +127,8 @@ _sig = {
     "fgr_record": (ctypes.c_int, [_H, ctypes.c_void_p, Frame, Frame, Frame, ctypes.c_float]),
     "fgr_record_synth_blur": (ctypes.c_int, [_H, ctypes.c_void_p, Frame, Frame, Frame, ctypes.c_float,
                                              ctypes.c_float]),
+    "fgr_record_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(InitInfo), Frame, Frame, Frame,
+                                          ctypes.c_float, ctypes.POINTER(ctypes.c_char_p)]),
     "fgr_set_profiling": (ctypes.c_int, [_H, ctypes.c_int]),
     "fgr_read_timings": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(ctypes.c_uint64),
                                         ctypes.c_int]),
@@ -272,6 +274,24 @@ def _frame(x):
     if isinstance(x, tuple):
         return frame_of(x[0], x[1])
     return frame_of(x)
+
+
+NOALLOC_SCRATCH_BYTES = 48 << 20
+
+
+def record_noalloc(prev, curr, out, t, width, height, fmt=FORMAT_RGBA8, settings=None, bgra=False,
+                   yuv_matrix=YUV_BT709, yuv_full_range=False, device=0, stream=None):
+    """fgr_record_noalloc: one estimate + one synth at t without a handle and without allocating. Returns
+    (ok, reason): reason is the refusal's string, None on success."""
+    info = InitInfo()
+    info.width, info.height, info.format, info.bgra = width, height, fmt, int(bgra)
+    info.transfer, info.linearWhiteNits = TRANSFER_ENCODED, 80.0
+    info.yuvMatrix, info.yuvFullRange, info.device = yuv_matrix, int(yuv_full_range), device
+    info.settings = settings if settings is not None else default_settings()
+    err = ctypes.c_char_p()
+    ok = bool(lib.fgr_record_noalloc(_stream(stream), ctypes.byref(info), _frame(prev), _frame(curr), _frame(out),
+                                     t, ctypes.byref(err)))
+    return ok, (err.value.decode() if err.value else None)
 
 
 class Interpolator:

@@ -156,6 +156,22 @@ int fgr_record(FgrInterpolator* h, void* stream, FgrFrame prev, FgrFrame curr, F
 int fgr_record_synth_blur(FgrInterpolator* h, void* stream, FgrFrame prev, FgrFrame curr, FgrFrame out,
                           float t0, float t1);                                                       /* :573 */
 
+/* ---- a record that allocates nothing (this library's addition; no counterpart in the reference) ----
+ * For callers that cannot allocate device memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc. The same work and results as fgr_record (one estimate, then one synth at t) without a handle: the
+ * pair's size, format and settings come in `info` (its device and format fields as fgr_create reads them), and every
+ * intermediate (luma pyramid, block and per-pixel fields, counters) lives in one file-scope device array of
+ * FGR_NOALLOC_SCRATCH_BYTES per GPU, held by each GPU the library runs on until the process ends. The call allocates
+ * and frees nothing, on the device or the host. Refused (returns 0, records nothing, *error = a string literal reason
+ * when error is not null): what fgr_create or fgr_record would refuse; a pair whose intermediates exceed the array
+ * (1920x1080 at flowScale 1 or 3840x2160 at flowScale 2 fit); RgbaF16 (its level table is a per-handle device
+ * buffer); uiProtection other than Off and seedNet 1 (per-handle state). The array keeps the last call's contents and
+ * is shared by every call on that GPU: calls in one stream are ordered, calls whose work overlaps on one GPU (two
+ * streams, two threads) race. */
+#define FGR_NOALLOC_SCRATCH_BYTES (48u << 20)
+int fgr_record_noalloc(void* stream, const FgrInitInfo* info, FgrFrame prev, FgrFrame curr, FgrFrame out, float t,
+                       const char** error);
+
 /* ---- static-UI protection, CrosshairV2 (framegen.h:575-693) ----
  * recordObserve: once per new real frame (in presentation order), after its pixels are final: advances the box's
  * per-pixel state machine and keeps the frame's mask (the frame pointer is the key; uiHistory frames are kept, least
