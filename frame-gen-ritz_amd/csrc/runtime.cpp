@@ -629,15 +629,30 @@ void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& pr
     pp.pf = F + fieldPF;
     pp.smooth = st.smoothBonus;
     pp.zero = st.hudBonus;
+    pp.fb = pixExactBits();
     pp.gx = cdiv(l0.w, 32);
     PlanOp op;
-    op.pass = kPassPixfield;
+    // the LDS-window kernel where latency decides (one pair's grid); batches fill the GPU and run the gathering kernel,
+    // whose higher occupancy gives more throughput (the same floats either way)
+    op.pass = pp.fb >= 0 && !batched ? kPassPixfieldWin : kPassPixfield;
     op.stage = 401;
     op.gx = pp.gx;
     op.gy = cdiv(l0.h, 32);
     op.stamp = "pixel field";
     op.set(pp);
     plan.push_back(op);
+}
+
+// The fractional bits fb of the level-0 luma when its values are k 2^-fb with 0 <= k < 2^14 (the pixel field's exact
+// integer sampling), else -1. The cell mean of S x S pixel lumas adds 2 log2(S) bits to the per-pixel luma's: integer
+// levels for Rgba8 (<= 255), 1/16 levels for Rgb10 (<= 4080) and full-range Nv12 / P010 (<= 4092); limited-range YCbCr
+// (below 0 and above 255 levels) and RgbaF16 (up to +-4096 levels) take the float path.
+int FgrInterpolator::pixExactBits() const {
+    const int sb = set.flowScale == 1 ? 0 : (set.flowScale == 2 ? 2 : 4);
+    if (format == FGR_FORMAT_RGBA8) return sb;
+    const bool fixed16 = format == FGR_FORMAT_RGB10 ||
+                         ((format == FGR_FORMAT_NV12 || format == FGR_FORMAT_P010) && info.yuvFullRange);
+    return fixed16 && set.flowScale <= 2 ? 4 + sb : -1;
 }
 
 bool FgrInterpolator::runPlan(hipStream_t s, const std::vector<PlanOp>& plan) {

@@ -145,6 +145,7 @@ struct FgrInterpolator {
     bool frameOk(const FgrFrame& f) const;
     fgr::FrameRef ref(const FgrFrame& f) const;
     int bytesPerPixel() const;
+    int pixExactBits() const;
     void stamp(hipStream_t s, const char* name);
     void beginProfile(hipStream_t s);
     // batched: the plan of a batch (fgr_batch_*): kernels laid out for many workgroups rather than for latency

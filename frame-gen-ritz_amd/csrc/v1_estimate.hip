@@ -844,13 +844,15 @@ __device__ __forceinline__ int hsumAt(int y, int c) {
 // reference's first-minimum rule kept: 0 replaces 1 on a tie (c0 <= c1), every later one needs strict <. When every
 // run of the tile's region reused candidate 1's errors for candidate k (a workgroup vote), every window sum is
 // candidate 1's, so candidate 0 costs c1 - smoothBonus and candidates 2-5 cannot win: the sums are skipped.
+constexpr int kPfFloatPool = kPfRH * kPfRW * 4 + kPfRH * kPfTile * 8 + 64 * 8;   // errs, hsum, block window
+
 template <int NT>
-__device__ __forceinline__ void pixfieldBody(const PixParams& P, int gbx, int gby) {
+__device__ __forceinline__ void pixfieldTile(const PixParams& P, int gbx, int gby, uint8_t* pool) {
     static_assert(NT == 256 || NT == 512, "two layouts");
     constexpr int kRows = kPfTile * kPfTile / NT;      // pixels a thread: column cx, rows r0 .. r0 + kRows - 1
-    __shared__ __align__(16) float errs[kPfRH * kPfRW];
-    __shared__ __align__(16) double hsum[kPfRH * kPfTile];
-    __shared__ float2 bwin[64];
+    double* hsum = (double*)pool;
+    float* errs = (float*)(pool + kPfRH * kPfTile * 8);
+    float2* bwin = (float2*)(pool + kPfRH * kPfTile * 8 + kPfRH * kPfRW * 4);
     const int tid = threadIdx.x;
     const int x0 = gbx * kPfTile, y0 = gby * kPfTile;
     const BlockWin bw{bwin, x0 / kBlock - 2, y0 / kBlock - 2};
@@ -971,6 +973,293 @@ __device__ __forceinline__ void pixfieldBody(const PixParams& P, int gbx, int gb
     }
 }
 
+template <int NT>
+__device__ __forceinline__ void pixfieldBody(const PixParams& P, int gbx, int gby) {
+    __shared__ __align__(16) uint8_t pool[kPfFloatPool];
+    pixfieldTile<NT>(P, gbx, gby, pool);
+}
+
+// ---- the pixel field in exact integers, from LDS windows of the two planes ----
+// For a level-0 luma of values k 2^-fb with 0 <= k < 2^14 (RGBA8 at every flowScale; PixParams.fb) every sampleL is
+// exact in float32: a tap k m / 1024 (m <= 1024) has at most 14 + 11 significant bits, and the sum, a convex combination,
+// at most 14 + 10 bits; so is the error |a - c|, and every window sum of 25 errors fits 32 bits. In integers (units of
+// 2^-(fb + 10)) a sample is the sum of t m over its taps (t = k): one v_dot2_u32_u16 per tap row over the texel pair with
+// the weights (32 - fx, fx), then the vertical blend; the error is one v_sad_u32. The window sum S gives the cost
+// float32(S 2^-(fb + 10) (1/25)), the float32 rounding of cv2.blur's float64 mean, as the float path computes it.
+//
+// The texels come from LDS windows of both planes covering the tile's region plus the reach R of its largest candidate
+// vector: word x of a window row is the texel pair t[x] | t[x + 1] << 16 of the edge-clamped plane, so a window tap is
+// the clamped global tap. A tile whose candidates reach further than kPfWinR flow px runs the float path (pixfieldTile).
+constexpr int kPfWinR = 7;
+constexpr int kPfWinWords = 42 + 2 * kPfWinR, kPfWinRows = 39 + 2 * kPfWinR;   // at R: 42 + 2R words, 39 + 2R rows
+constexpr int kPfHsStride = 36;   // words a row of the horizontal sums (16-byte rows; rows 4 apart in other banks)
+
+struct PixWin {
+    const uint32_t* a;
+    const uint32_t* c;
+    int x0, y0, stride;   // plane coordinates of the windows' word 0, words a row
+};
+
+typedef unsigned short PixU16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pairDot(uint32_t pair, uint32_t w) {
+    return __builtin_amdgcn_udot2(__builtin_bit_cast(PixU16x2, pair), __builtin_bit_cast(PixU16x2, w), 0u, false);
+}
+
+// sampleL at the 1/32 map (X, Y), in units of 2^-(fb + 10)
+__device__ __forceinline__ uint32_t winSample(const uint32_t* w, const PixWin& W, int X, int Y) {
+    const uint32_t fx = (uint32_t)(X & 31), fy = (uint32_t)(Y & 31);
+    const uint32_t* p = w + ((Y >> 5) - W.y0) * W.stride + ((X >> 5) - W.x0);
+    const uint32_t wx = (32u - fx) | (fx << 16);
+    return pairDot(p[0], wx) * (32u - fy) + pairDot(p[W.stride], wx) * fy;
+}
+
+// the four samples at X0 + 32 i, Y (one fractional part, 5 texels a row)
+__device__ __forceinline__ void winSample4(const uint32_t* w, const PixWin& W, int X0, int Y, uint32_t* out) {
+    const uint32_t fx = (uint32_t)(X0 & 31), fy = (uint32_t)(Y & 31);
+    const uint32_t* p = w + ((Y >> 5) - W.y0) * W.stride + ((X0 >> 5) - W.x0);
+    const uint32_t wx = (32u - fx) | (fx << 16);
+#pragma unroll
+    for (int i = 0; i < 4; i++) out[i] = pairDot(p[i], wx) * (32u - fy) + pairDot(p[W.stride + i], wx) * fy;
+}
+
+// |sampleL(a, q - v) - sampleL(c, q + v)| with the maps formed as sampleL forms them
+__device__ __forceinline__ uint32_t winErr(const PixWin& W, int qx, int qy, float2 v) {
+    const int Xa = (int)rintf(((float)qx - v.x) * 32.0f), Ya = (int)rintf(((float)qy - v.y) * 32.0f);
+    const int Xc = (int)rintf(((float)qx + v.x) * 32.0f), Yc = (int)rintf(((float)qy + v.y) * 32.0f);
+    return __sad(winSample(W.a, W, Xa, Ya), winSample(W.c, W, Xc, Yc), 0u);
+}
+
+// pixErrRun's candidates, reuse rules and errors, in integers from the windows
+__device__ __forceinline__ bool winErrRun(const PixParams& P, const PixWin& W, const BlockWin& bw, int k, int gx0, int qy,
+                                          uint32_t* e, uint32_t* own, float2& ownV, bool& ownOk) {
+    if (k == 6) {   // zero vector: the texel difference (weight 1024)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int o = (qy - W.y0) * W.stride + (clampi(gx0 + i, 0, P.w - 1) - W.x0);
+            e[i] = __sad(W.a[o] & 0xFFFFu, W.c[o] & 0xFFFFu, 0u) << 10;
+        }
+        return false;
+    }
+    const bool inside = gx0 >= 0 && gx0 + 3 <= P.w - 1;
+    if (k == 0 && inside) {   // the run's pixels interpolate the same four block vectors
+        const float sx0 = ((float)gx0 + 0.5f) / 8.0f - 0.5f, sy = ((float)qy + 0.5f) / 8.0f - 0.5f;
+        const float flx = floorf(sx0), fly = floorf(sy);
+        const float fy = sy - fly;
+        const int ix = (int)flx, iy = (int)fly;
+        const float2 f00 = bw.at(ix, iy), f10 = bw.at(ix + 1, iy), f01 = bw.at(ix, iy + 1), f11 = bw.at(ix + 1, iy + 1);
+        bool same = ownOk;
+        float2 v[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const float fx = (((float)(gx0 + i) + 0.5f) / 8.0f - 0.5f) - flx;
+            v[i] = mix2(mix2(f00, f10, fx), mix2(f01, f11, fx), fy);
+            same = same && v[i].x == ownV.x && v[i].y == ownV.y;
+        }
+        if (same) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) e[i] = own[i];
+            return true;
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) e[i] = winErr(W, gx0 + i, qy, v[i]);
+        return false;
+    }
+    if (k > 0 && inside) {
+        const int2 o = candOffset(k);
+        const float2 v = bw.at(gx0 / kBlock + o.x, qy / kBlock + o.y);   // one block, one vector
+        if (k > 1 && ownOk && v.x == ownV.x && v.y == ownV.y) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) e[i] = own[i];
+            return true;
+        }
+        const int Xa = (int)rintf(((float)gx0 - v.x) * 32.0f), Xa3 = (int)rintf(((float)(gx0 + 3) - v.x) * 32.0f);
+        const int Xc = (int)rintf(((float)gx0 + v.x) * 32.0f), Xc3 = (int)rintf(((float)(gx0 + 3) + v.x) * 32.0f);
+        if (Xa3 == Xa + 96 && Xc3 == Xc + 96) {   // one map fraction a frame: the run form
+            const int Ya = (int)rintf(((float)qy - v.y) * 32.0f), Yc = (int)rintf(((float)qy + v.y) * 32.0f);
+            uint32_t sa[4], sc[4];
+            winSample4(W.a, W, Xa, Ya, sa);
+            winSample4(W.c, W, Xc, Yc, sc);
+#pragma unroll
+            for (int i = 0; i < 4; i++) e[i] = __sad(sa[i], sc[i], 0u);
+        } else {
+#pragma unroll 1
+            for (int i = 0; i < 4; i++) e[i] = winErr(W, gx0 + i, qy, v);
+        }
+        if (k == 1) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) own[i] = e[i];
+            ownV = v;
+            ownOk = true;
+        }
+        return false;
+    }
+#pragma unroll 1
+    for (int i = 0; i < 4; i++) {
+        const int qx = clampi(gx0 + i, 0, P.w - 1);
+        e[i] = winErr(W, qx, qy, pixCand(k, qx, qy, bw));
+    }
+    return false;
+}
+
+// The pixel field of a 32 x 32 tile from the windows (layout and candidate order as pixfieldTile's). The runs of the
+// tile's 40 x 36 region are r = tid and tid + NT; each candidate's errors go to LDS, then the horizontal 5-sums of the
+// region's rows (exact u32, four a task), then each thread's vertical sums and running minimum.
+template <int NT>
+__device__ __forceinline__ void pixfieldWinBody(const PixParams& P, int gbx, int gby) {
+    static_assert(NT == 256 || NT == 512, "two layouts");
+    constexpr int kRows = kPfTile * kPfTile / NT;
+    constexpr int kRunsPer = (kPfRuns + NT - 1) / NT;
+    constexpr int kWinWords = kPfWinWords * kPfWinRows;
+    constexpr int kIntPool = 2 * kWinWords * 4 + kPfRH * kPfRW * 4 + kPfRH * kPfHsStride * 4;
+    constexpr int kPool = kIntPool > kPfFloatPool ? kIntPool : kPfFloatPool;
+    __shared__ __align__(16) uint8_t pool[kPool];
+    __shared__ float2 bwin[64];
+    __shared__ int winReach;
+    const int tid = threadIdx.x;
+    const int x0 = gbx * kPfTile, y0 = gby * kPfTile;
+    const BlockWin bw{bwin, x0 / kBlock - 2, y0 / kBlock - 2};
+    if (tid < 64) {   // (one wavefront) the block vectors, and R = floor(max |component|) + 1 > every candidate's reach
+        const float2 v = fieldAt(P.bf, bw.bx0 + (tid & 7), bw.by0 + (tid >> 3), P.bw, P.bh);
+        bwin[tid] = v;
+        float m = fmaxf(fabsf(v.x), fabsf(v.y));
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+        if (tid == 0) winReach = (int)fminf(floorf(m) + 1.0f, (float)(kPfWinR + 1));
+    }
+    __syncthreads();
+    const int R = winReach;
+    if (R > kPfWinR) {   // (uniform) a long candidate: the float path, its LDS in the same pool
+        pixfieldTile<NT>(P, gbx, gby, pool);
+        return;
+    }
+    uint32_t* winA = (uint32_t*)pool;
+    uint32_t* errs = winA + 2 * kWinWords;
+    uint32_t* hsum = errs + kPfRH * kPfRW;
+    // the windows: rows y0 - 3 - R .. y0 + 35 + R, word columns x0 - 5 - R .. x0 + 36 + R (a wavefront a row)
+    const PixWin W{winA, winA + kWinWords, x0 - 5 - R, y0 - 3 - R, 42 + 2 * R};
+    {
+        const float up = __int_as_float((127 + P.fb) << 23);
+        const int rows = 39 + 2 * R, lane = tid & 63;
+        if (lane < W.stride) {
+            const uint32_t gx0 = (uint32_t)clampi(W.x0 + lane, 0, P.w - 1), gx1 = (uint32_t)clampi(W.x0 + lane + 1, 0, P.w - 1);
+            for (int wy = tid >> 6; wy < rows; wy += NT / 64) {
+                const uint32_t row = idx2(0, clampi(W.y0 + wy, 0, P.h - 1), P.w);
+                const uint32_t a0 = (uint32_t)(ldIdx(P.la, row + gx0) * up), a1 = (uint32_t)(ldIdx(P.la, row + gx1) * up);
+                const uint32_t c0 = (uint32_t)(ldIdx(P.lc, row + gx0) * up), c1 = (uint32_t)(ldIdx(P.lc, row + gx1) * up);
+                winA[wy * W.stride + lane] = a0 | (a1 << 16);
+                winA[kWinWords + wy * W.stride + lane] = c0 | (c1 << 16);
+            }
+        }
+    }
+    __syncthreads();
+    // cost = float32(double(S) (1/25) 2^-(fb + 10)): the power of two commutes with the rounding of the product
+    const double cscale = (1.0 / 25.0) * (double)__int_as_float((127 - 10 - P.fb) << 23);
+    const int cx = tid & 31, r0 = (tid >> 5) * kRows;
+    float best[kRows], c1[kRows];
+    int bk[kRows];
+#pragma unroll
+    for (int j = 0; j < kRows; j++) {
+        best[j] = 0.0f;
+        c1[j] = 0.0f;
+        bk[j] = 0;
+    }
+    static_assert(kRunsPer <= 2, "at most two runs a thread");
+    // candidate 1's errors of the thread's runs (r = tid and tid + NT), kept for candidates 0 and 2-5; one loop body for
+    // both runs, the kept values selected rather than indexed
+    uint4 own0 = make_uint4(0u, 0u, 0u, 0u), own1 = own0;
+    float2 ownV0 = make_float2(0.0f, 0.0f), ownV1 = ownV0;
+    bool ownOk0 = false, ownOk1 = false;
+#pragma unroll 1
+    for (int step = 0; step < 7; step++) {
+        const int k = step == 0 ? 1 : (step == 1 ? 0 : step);
+        bool reusedAll = true;
+#pragma unroll 1
+        for (int r = tid; r < kPfRuns; r += NT) {
+            const bool s1 = r >= NT;
+            const uint4 o4 = s1 ? own1 : own0;
+            uint32_t own[4] = {o4.x, o4.y, o4.z, o4.w};
+            float2 ownV = s1 ? ownV1 : ownV0;
+            bool ownOk = s1 ? ownOk1 : ownOk0;
+            const int ry = r / (kPfRW / 4), rc = r - ry * (kPfRW / 4);
+            uint32_t e[4];
+            reusedAll = winErrRun(P, W, bw, k, x0 - 4 + 4 * rc, clampi(y0 - 2 + ry, 0, P.h - 1), e, own, ownV, ownOk) &&
+                        reusedAll;
+            *(uint4*)&errs[ry * kPfRW + 4 * rc] = make_uint4(e[0], e[1], e[2], e[3]);
+            if (k == 1) {
+                const uint4 n4 = make_uint4(own[0], own[1], own[2], own[3]);
+                own0 = s1 ? own0 : n4;
+                own1 = s1 ? n4 : own1;
+                ownV0 = s1 ? ownV0 : ownV;
+                ownV1 = s1 ? ownV : ownV1;
+                ownOk0 = s1 ? ownOk0 : ownOk;
+                ownOk1 = s1 ? ownOk : ownOk1;
+            }
+        }
+        if (__syncthreads_and(reusedAll)) {   // every run reused candidate 1's errors: the sums are candidate 1's
+            if (k == 0) {
+#pragma unroll
+                for (int j = 0; j < kRows; j++) {
+                    const float c = c1[j] - P.smooth;
+                    if (c <= best[j]) {
+                        best[j] = c;
+                        bk[j] = 0;
+                    }
+                }
+            }
+            continue;
+        }
+        // horizontal 5-sums of region columns c0 + 2 .. c0 + 9 (three aligned 16-byte quads), four outputs a task
+        for (int t = tid; t < kPfRH * (kPfTile / 4); t += NT) {
+            const int hy = t >> 3, c0 = (t & 7) * 4;
+            const uint4* e4 = (const uint4*)&errs[hy * kPfRW + c0];
+            const uint4 q0 = e4[0], q1 = e4[1], q2 = e4[2];
+            uint4 h;
+            h.x = q0.z + q0.w + q1.x + q1.y + q1.z;
+            h.y = h.x + q1.w - q0.z;
+            h.z = h.y + q2.x - q0.w;
+            h.w = h.z + q2.y - q1.x;
+            *(uint4*)&hsum[hy * kPfHsStride + c0] = h;
+        }
+        __syncthreads();
+        uint32_t hv[kRows + 4];
+#pragma unroll
+        for (int y = 0; y < kRows + 4; y++) hv[y] = hsum[(r0 + y) * kPfHsStride + cx];
+        uint32_t vs = hv[0] + hv[1] + hv[2] + hv[3] + hv[4];
+#pragma unroll
+        for (int j = 0; j < kRows; j++) {
+            if (j > 0) vs = vs + hv[j + 4] - hv[j - 1];
+            float c = (float)((double)vs * cscale);
+            if (k == 0) c = c - P.smooth;
+            if (k == 6) c = c - P.zero;
+            if (k == 1) {
+                c1[j] = c;
+                best[j] = c;
+                bk[j] = 1;
+            } else if (k == 0 ? c <= best[j] : c < best[j]) {
+                best[j] = c;
+                bk[j] = k;
+            }
+        }
+    }
+    const int qx = x0 + cx;
+    if (qx >= P.w) return;
+#pragma unroll
+    for (int j = 0; j < kRows; j++) {
+        const int py = y0 + r0 + j;
+        if (py >= P.h) break;
+        const int k = bk[j];
+        float2 v;
+        if (k == 0) {
+            v = pixCand(0, qx, py, bw);
+        } else {
+            const int2 o = candOffset(k);
+            v = bw.at(qx / kBlock + o.x, py / kBlock + o.y);
+            if (k == 6) v = make_float2(0.0f, 0.0f);
+        }
+        P.pf[py * P.w + qx] = v;
+    }
+}
+
 // ---------------------------------------------------------------------------------------------------------- kernels
 
 // Each pass has a per-pair kernel (grid = that pair's blocks) and a batched kernel over a work list spanning many pairs
@@ -987,6 +1276,8 @@ FGR_PASS(median3, Median3Params, median3Body, 256)
 FGR_PASS(subpix, SubpixParams, subpixBody, 64)
 FGR_PASS_SINGLE(pixfield, PixParams, pixfieldBody<kPfThreads>, kPfThreads)
 FGR_PASS_BATCH_OCC(pixfield, PixParams, pixfieldBody<kPfBatchThreads>, kPfBatchThreads, 6)
+FGR_PASS_SINGLE(pixfield_win, PixParams, pixfieldWinBody<kPfThreads>, kPfThreads)
+FGR_PASS_BATCH(pixfield_win, PixParams, pixfieldWinBody<kPfBatchThreads>, kPfBatchThreads)
 
 // ---------------------------------------------------------------------------------------------------------- launches
 
@@ -1005,6 +1296,7 @@ hipError_t launchSingle(int pass, const void* params, int gx, int gy, hipStream_
     case kPassMedian3: hipLaunchKernelGGL(median3_kernel, g, dim3(256), 0, s, *(const Median3Params*)params); break;
     case kPassSubpix: hipLaunchKernelGGL(subpix_kernel, g, dim3(64), 0, s, *(const SubpixParams*)params); break;
     case kPassPixfield: hipLaunchKernelGGL(pixfield_kernel, g, dim3(kPfThreads), 0, s, *(const PixParams*)params); break;
+    case kPassPixfieldWin: hipLaunchKernelGGL(pixfield_win_kernel, g, dim3(kPfThreads), 0, s, *(const PixParams*)params); break;
     default: return launchSynthSingle(pass, params, gx, gy, s);
     }
     return hipGetLastError();
@@ -1025,6 +1317,7 @@ hipError_t launchBatch(int pass, const void* params, const uint32_t* work, int n
     case kPassMedian3: hipLaunchKernelGGL(median3_batch_kernel, g, dim3(256), 0, s, (const Median3Params*)params, work); break;
     case kPassSubpix: hipLaunchKernelGGL(subpix_batch_kernel, g, dim3(64), 0, s, (const SubpixParams*)params, work); break;
     case kPassPixfield: hipLaunchKernelGGL(pixfield_batch_kernel, g, dim3(kPfBatchThreads), 0, s, (const PixParams*)params, work); break;
+    case kPassPixfieldWin: hipLaunchKernelGGL(pixfield_win_batch_kernel, g, dim3(kPfBatchThreads), 0, s, (const PixParams*)params, work); break;
     default: return launchSynthBatch(pass, params, work, n, s);
     }
     return hipGetLastError();

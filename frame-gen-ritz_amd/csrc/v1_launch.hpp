@@ -16,7 +16,8 @@ enum Pass : int {
     kPassSynthRgba8UiS1, kPassSynthRgba8UiS2, kPassSynthRgba8UiS4,     // RGBA8 synth with CrosshairV2 tap rejection
     kPassUiPatch, kPassUiFill,                                         // CrosshairV2 paste-back, estimate fill (ui.hip)
     kPassSynthFmtUiS1, kPassSynthFmtUiS2, kPassSynthFmtUiS4,           // the other formats' synth with tap rejection
-    kPassSearch2x8                                                     // R <= 2 search, two blocks a wavefront (batches)
+    kPassSearch2x8,                                                    // R <= 2 search, two blocks a wavefront (batches)
+    kPassPixfieldWin                                                   // pixel field from LDS luma windows (exact lumas)
 };
 // CrosshairV2 passes (ui.hip), single launches over the work rectangle
 enum UiPass : int { kUiUpdate = 0, kUiMask, kUiPatch, kUiFill };

@@ -95,6 +95,8 @@ struct PixParams {
     const float2* bf; int bw, bh;                 // block field
     float2* pf;                                   // per-pixel field, w x h
     float smooth, zero;
+    int fb;             // the level-0 luma's fractional bits when it is k 2^-fb with 0 <= k 2^-fb <= 255 and
+                        // k < 2^14 (sampling it is then exact integer arithmetic); -1 otherwise
 };
 
 struct LookupParams {
