@@ -607,9 +607,10 @@ void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& pr
         q.out = F + fieldSub;
         q.lam = st.searchPenalty;
         q.u = u;
+        q.fb = pixExactBits();
         q.gx = cdiv(l0.bw * l0.bh, kSubpixBlocks);
         PlanOp op;
-        op.pass = kPassSubpix;
+        op.pass = q.fb >= 0 ? kPassSubpixInt : kPassSubpix;
         op.stage = 400;
         op.gx = q.gx;
         op.stamp = "subpixel";
@@ -647,7 +648,9 @@ void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& pr
 // integer sampling), else -1. The cell mean of S x S pixel lumas adds 2 log2(S) bits to the per-pixel luma's: integer
 // levels for Rgba8 (<= 255), 1/16 levels for Rgb10 (<= 4080) and full-range Nv12 / P010 (<= 4092); limited-range YCbCr
 // (below 0 and above 255 levels) and RgbaF16 (up to +-4096 levels) take the float path.
+// A protected estimate with uiEstimateFill rewrites the UI cells as float32 means over any number of pixels: not dyadic.
 int FgrInterpolator::pixExactBits() const {
+    if (estUi && set.uiEstimateFill) return -1;
     const int sb = set.flowScale == 1 ? 0 : (set.flowScale == 2 ? 2 : 4);
     if (format == FGR_FORMAT_RGBA8) return sb;
     const bool fixed16 = format == FGR_FORMAT_RGB10 ||

@@ -533,6 +533,12 @@ __device__ __forceinline__ void median3Body(const Median3Params& P, int gbx, int
     P.out[by * P.bw + bx] = make_float2(med9(xs), med9(ys));
 }
 
+// Two u16 lanes dotted with two u16 weights plus c (v_dot2_u32_u16): the exact integer form of a bilinear tap row
+typedef unsigned short U16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pairDot(uint32_t pair, uint32_t w, uint32_t c = 0u) {
+    return __builtin_amdgcn_udot2(__builtin_bit_cast(U16x2, pair), __builtin_bit_cast(U16x2, w), c, false);
+}
+
 // ---------------------------------------------------------------------------------------------------------- subpix
 
 // subpix.comp:15-24 tap(): bilinear in tile coordinates (p >= 0), weights exact for quarter steps
@@ -662,6 +668,160 @@ __device__ __forceinline__ void subpixBody(const SubpixParams& P, int g, int) {
     }
     __syncthreads();
     // subpix.comp:65-69: the blocks' SADs into the scene-cut sum, one atomic per workgroup into its slot
+    if (lane == 0) {
+        uint32_t sum = 0u;
+        for (int i = 0; i < NB; i++) sum += wsad[i];
+        atomicAdd(P.u + kUSad + (g & (kSlots - 1)), sum);
+    }
+}
+
+// The same refinement in exact integers, for a level-0 luma of values k 2^-fb with 0 <= k < 2^14 and fb <= 6
+// (SubpixParams.fb): at quarter-pixel offsets every bilinear weight is a multiple of 1/16, so a tap is exact in float32
+// and, in units of 2^-(fb + 4), the integer sum of t W over its four texels (t = k, W = 16 w in 0..16); the row sums and
+// the total of the 64 errors are exact too (at most 14 + fb + 4 <= 24 bits), so the float SAD is the integer total times
+// 2^-(fb + 4) and every cost, choice and scene-cut term is the float path's. A tap is two v_dot2_u32_u16 over the texel
+// pairs of its rows with the weight pairs (W00, W01) and (W10, W11), an error one v_sad_u32 into the running total.
+//
+// Windows: each frame's 13 x 13 texels of a block as u16 rows of 14 stored twice, E16[i] = t[i] and O16[i] = t[i + 1],
+// so that the texel pair (t[i], t[i + 1]) is the aligned word i / 2 of E16 (i even) or (i - 1) / 2 of O16 (i odd).
+__device__ __forceinline__ void subpixIntBody(const SubpixParams& P, int g, int) {
+    constexpr int T = 13, TS = 14, O = 2, NB = kSubpixBlocks, WB = T * TS;   // WB u16 a block window
+    __shared__ __align__(16) uint16_t ea[NB * WB], oa[NB * WB], ec[NB * WB], oc[NB * WB];
+    __shared__ float cst[NB][9], sads[NB][9];
+    __shared__ int sv[NB][2], sb[NB][2];
+    __shared__ uint32_t wsad[NB];
+    const int lane = threadIdx.x;
+    const int nb = P.bw * P.bh;
+    if (lane < NB) {
+        const int blk = g * NB + lane;
+        const float2 v0 = P.in[blk < nb ? blk : 0];
+        sv[lane][0] = (int)rintf(v0.x);
+        sv[lane][1] = (int)rintf(v0.y);
+        const int b = min(blk, nb - 1);
+        sb[lane][0] = (b % P.bw) * kBlock;
+        sb[lane][1] = (b / P.bw) * kBlock;
+    }
+    __syncthreads();
+    // the 7 blocks' windows, block by block (the block's corner and seed wave-uniform), three 64-texel chunks of its
+    // 13 x 13 texels a block (a lane's row and column fixed per chunk); every load issued before any store
+    constexpr int KC = (T * T + 63) / 64;
+    const float up = __int_as_float((127 + P.fb) << 23);
+    int cr[KC], cc[KC];
+#pragma unroll
+    for (int kc = 0; kc < KC; kc++) {
+        const int r = min(lane + 64 * kc, T * T - 1);   // past the window: the last texel again, not stored
+        cr[kc] = r / T;
+        cc[kc] = r - cr[kc] * T;
+    }
+    float va[NB][KC], vc[NB][KC];
+#pragma unroll
+    for (int jb = 0; jb < NB; jb++) {
+        const int bx = __builtin_amdgcn_readfirstlane(sb[jb][0]), by = __builtin_amdgcn_readfirstlane(sb[jb][1]);
+        const int vx = __builtin_amdgcn_readfirstlane(sv[jb][0]), vy = __builtin_amdgcn_readfirstlane(sv[jb][1]);
+#pragma unroll
+        for (int kc = 0; kc < KC; kc++) {
+            va[jb][kc] = lumaAt(P.la, bx - vx - O + cc[kc], by - vy - O + cr[kc], P.w, P.h);
+            vc[jb][kc] = lumaAt(P.lc, bx + vx - O + cc[kc], by + vy - O + cr[kc], P.w, P.h);
+        }
+    }
+#pragma unroll
+    for (int kc = 0; kc < KC; kc++) {
+        if (lane + 64 * kc < T * T) {
+            const int e0 = cr[kc] * TS + cc[kc];   // the texel's u16 index in its block's window
+#pragma unroll
+            for (int jb = 0; jb < NB; jb++) {
+                const int e = jb * WB + e0;
+                const uint16_t ta = (uint16_t)(uint32_t)(va[jb][kc] * up), tc = (uint16_t)(uint32_t)(vc[jb][kc] * up);
+                ea[e] = ta;
+                ec[e] = tc;
+                if (e > 0) {
+                    oa[e - 1] = ta;
+                    oc[e - 1] = tc;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const int j = min(lane / 9, NB - 1), k = lane - 9 * (lane / 9);   // lane 63: a duplicate of lane 62's work
+    const uint32_t* EA = (const uint32_t*)(ea + j * WB);
+    const uint32_t* OA = (const uint32_t*)(oa + j * WB);
+    const uint32_t* EC = (const uint32_t*)(ec + j * WB);
+    const uint32_t* OC = (const uint32_t*)(oc + j * WB);
+    const float dx = (float)(k % 3 - 1), dy = (float)(k / 3 - 1);
+    const float sadScale = __int_as_float((127 - 4 - P.fb) << 23) * (1.0f / 64.0f);
+    float basex = 0.0f, basey = 0.0f, bestsad = 0.0f;
+    for (int pass = 0; pass < 2; pass++) {
+        const float step = pass == 0 ? 0.5f : 0.25f;
+        const float ox = basex + dx * step, oy = basey + dy * step;
+        const float pax = (float)O - ox, pay = (float)O - oy, pcx = (float)O + ox, pcy = (float)O + oy;
+        const float flax = floorf(pax), flay = floorf(pay), flcx = floorf(pcx), flcy = floorf(pcy);
+        // quarter units of the fractional parts, the 16 w weight pairs of each tap row
+        const uint32_t qax = (uint32_t)((pax - flax) * 4.0f), qay = (uint32_t)((pay - flay) * 4.0f);
+        const uint32_t qcx = (uint32_t)((pcx - flcx) * 4.0f), qcy = (uint32_t)((pcy - flcy) * 4.0f);
+        const uint32_t wa0 = ((4u - qay) * (4u - qax)) | (((4u - qay) * qax) << 16), wa1 = (qay * (4u - qax)) | ((qay * qax) << 16);
+        const uint32_t wc0 = ((4u - qcy) * (4u - qcx)) | (((4u - qcy) * qcx) << 16), wc1 = (qcy * (4u - qcx)) | ((qcy * qcx) << 16);
+        // the texel pair at column c of a row: word (row TS + c) >> 1 of E (c even) or O (c odd); pixel x reads column
+        // col + x, so even x keeps col's parity and odd x flips it
+        const int colA = (int)flax, colC = (int)flcx;
+        const uint32_t* A0 = (colA & 1) ? OA : EA;
+        const uint32_t* A1 = (colA & 1) ? EA : OA;
+        const uint32_t* C0 = (colC & 1) ? OC : EC;
+        const uint32_t* C1 = (colC & 1) ? EC : OC;
+        const int ra = (int)flay * (TS / 2), rc = (int)flcy * (TS / 2);
+        const int a0 = ra + (colA >> 1), a1 = ra + ((colA + 1) >> 1), c0 = rc + (colC >> 1), c1 = rc + ((colC + 1) >> 1);
+        uint32_t pa0[8], pc0[8];
+#pragma unroll
+        for (int x = 0; x < 8; x++) {
+            pa0[x] = (x & 1) ? A1[a1 + (x >> 1)] : A0[a0 + (x >> 1)];
+            pc0[x] = (x & 1) ? C1[c1 + (x >> 1)] : C0[c0 + (x >> 1)];
+        }
+        uint32_t total = 0u;
+#pragma unroll
+        for (int y = 0; y < 8; y++) {
+            const int o = (y + 1) * (TS / 2);
+            uint32_t pa1[8], pc1[8];
+#pragma unroll
+            for (int x = 0; x < 8; x++) {
+                pa1[x] = (x & 1) ? A1[a1 + o + (x >> 1)] : A0[a0 + o + (x >> 1)];
+                pc1[x] = (x & 1) ? C1[c1 + o + (x >> 1)] : C0[c0 + o + (x >> 1)];
+            }
+#pragma unroll
+            for (int x = 0; x < 8; x++) {
+                const uint32_t tA = pairDot(pa1[x], wa1, pairDot(pa0[x], wa0, 0u));
+                const uint32_t tC = pairDot(pc1[x], wc1, pairDot(pc0[x], wc0, 0u));
+                total = __sad(tA, tC, total);
+            }
+#pragma unroll
+            for (int x = 0; x < 8; x++) {
+                pa0[x] = pa1[x];
+                pc0[x] = pc1[x];
+            }
+        }
+        const float sad = (float)total * sadScale;
+        if (lane < 9 * NB) {
+            sads[j][k] = sad;
+            cst[j][k] = sad + P.lam * step * (fabsf(dx) + fabsf(dy));
+        }
+        __syncthreads();
+        int bi = 0;
+        float best = cst[j][0];
+        for (int kk = 1; kk < 9; kk++)
+            if (cst[j][kk] < best) {
+                best = cst[j][kk];
+                bi = kk;
+            }
+        basex = basex + (float)(bi % 3 - 1) * step;
+        basey = basey + (float)(bi / 3 - 1) * step;
+        bestsad = sads[j][bi];
+        __syncthreads();
+    }
+    const int blk = g * NB + j;
+    const bool active = lane < 9 * NB && blk < nb;
+    if (k == 0 && lane < 9 * NB) {
+        if (active) P.out[blk] = make_float2((float)sv[j][0] + basex, (float)sv[j][1] + basey);
+        wsad[j] = active ? (uint32_t)rintf(bestsad * 256.0f) : 0u;
+    }
+    __syncthreads();
     if (lane == 0) {
         uint32_t sum = 0u;
         for (int i = 0; i < NB; i++) sum += wsad[i];
@@ -1000,10 +1160,6 @@ struct PixWin {
     int x0, y0, stride;   // plane coordinates of the windows' word 0, words a row
 };
 
-typedef unsigned short PixU16x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ uint32_t pairDot(uint32_t pair, uint32_t w) {
-    return __builtin_amdgcn_udot2(__builtin_bit_cast(PixU16x2, pair), __builtin_bit_cast(PixU16x2, w), 0u, false);
-}
 
 // sampleL at the 1/32 map (X, Y), in units of 2^-(fb + 10)
 __device__ __forceinline__ uint32_t winSample(const uint32_t* w, const PixWin& W, int X, int Y) {
@@ -1274,6 +1430,7 @@ FGR_PASS(search6, SearchParams, searchBody<6>, 256)
 FGR_PASS(search12, SearchParams, searchBody<12>, 256)
 FGR_PASS(median3, Median3Params, median3Body, 256)
 FGR_PASS(subpix, SubpixParams, subpixBody, 64)
+FGR_PASS(subpix_int, SubpixParams, subpixIntBody, 64)
 FGR_PASS_SINGLE(pixfield, PixParams, pixfieldBody<kPfThreads>, kPfThreads)
 FGR_PASS_BATCH_OCC(pixfield, PixParams, pixfieldBody<kPfBatchThreads>, kPfBatchThreads, 6)
 FGR_PASS_SINGLE(pixfield_win, PixParams, pixfieldWinBody<kPfThreads>, kPfThreads)
@@ -1295,6 +1452,7 @@ hipError_t launchSingle(int pass, const void* params, int gx, int gy, hipStream_
     case kPassSearch12: hipLaunchKernelGGL(search12_kernel, g, dim3(256), 0, s, *(const SearchParams*)params); break;
     case kPassMedian3: hipLaunchKernelGGL(median3_kernel, g, dim3(256), 0, s, *(const Median3Params*)params); break;
     case kPassSubpix: hipLaunchKernelGGL(subpix_kernel, g, dim3(64), 0, s, *(const SubpixParams*)params); break;
+    case kPassSubpixInt: hipLaunchKernelGGL(subpix_int_kernel, g, dim3(64), 0, s, *(const SubpixParams*)params); break;
     case kPassPixfield: hipLaunchKernelGGL(pixfield_kernel, g, dim3(kPfThreads), 0, s, *(const PixParams*)params); break;
     case kPassPixfieldWin: hipLaunchKernelGGL(pixfield_win_kernel, g, dim3(kPfThreads), 0, s, *(const PixParams*)params); break;
     default: return launchSynthSingle(pass, params, gx, gy, s);
@@ -1316,6 +1474,7 @@ hipError_t launchBatch(int pass, const void* params, const uint32_t* work, int n
     case kPassSearch12: hipLaunchKernelGGL(search12_batch_kernel, g, dim3(256), 0, s, (const SearchParams*)params, work); break;
     case kPassMedian3: hipLaunchKernelGGL(median3_batch_kernel, g, dim3(256), 0, s, (const Median3Params*)params, work); break;
     case kPassSubpix: hipLaunchKernelGGL(subpix_batch_kernel, g, dim3(64), 0, s, (const SubpixParams*)params, work); break;
+    case kPassSubpixInt: hipLaunchKernelGGL(subpix_int_batch_kernel, g, dim3(64), 0, s, (const SubpixParams*)params, work); break;
     case kPassPixfield: hipLaunchKernelGGL(pixfield_batch_kernel, g, dim3(kPfBatchThreads), 0, s, (const PixParams*)params, work); break;
     case kPassPixfieldWin: hipLaunchKernelGGL(pixfield_win_batch_kernel, g, dim3(kPfBatchThreads), 0, s, (const PixParams*)params, work); break;
     default: return launchSynthBatch(pass, params, work, n, s);

@@ -17,7 +17,8 @@ enum Pass : int {
     kPassUiPatch, kPassUiFill,                                         // CrosshairV2 paste-back, estimate fill (ui.hip)
     kPassSynthFmtUiS1, kPassSynthFmtUiS2, kPassSynthFmtUiS4,           // the other formats' synth with tap rejection
     kPassSearch2x8,                                                    // R <= 2 search, two blocks a wavefront (batches)
-    kPassPixfieldWin                                                   // pixel field from LDS luma windows (exact lumas)
+    kPassPixfieldWin,                                                  // pixel field from LDS luma windows (exact lumas)
+    kPassSubpixInt                                                     // sub-pixel refinement in integers (exact lumas)
 };
 // CrosshairV2 passes (ui.hip), single launches over the work rectangle
 enum UiPass : int { kUiUpdate = 0, kUiMask, kUiPatch, kUiFill };

@@ -87,6 +87,7 @@ struct SubpixParams {
     const float2* in; float2* out;
     float lam;
     uint32_t* u;
+    int fb;             // as PixParams.fb, for the integer kernel (fb <= 6)
 };
 
 struct PixParams {
