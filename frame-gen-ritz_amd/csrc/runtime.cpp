@@ -442,8 +442,39 @@ void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& pr
                                    bool batched) const {
     const FgrSettings& st = set;
     float2* F = (float2*)field;
-    // 1. luma + INTER_AREA 1/S (also clears the counters)
-    {
+    // 1. luma + INTER_AREA 1/S (also clears the counters), and in the same launch the pyramid's levels up to
+    // kPyrLevels -- unless a CrosshairV2 estimate fill must rewrite level 0 between the two
+    const bool fill = estUi && st.uiEstimateFill;
+    const int fused = fill ? 1 : std::min((int)lv.size(), kPyrLevels);
+    if (!fill) {
+        PyrParams pp{};
+        pp.a = ref(prev);
+        pp.c = ref(curr);
+        pp.W = (int)W;
+        pp.H = (int)H;
+        pp.S = (int)st.flowScale;
+        pp.u = u;
+        pp.bgra = (int)info.bgra;
+        pp.fmt = format;
+        pp.lut = fmtLut;
+        pp.yFull = info.yuvFullRange ? 1 : 0;
+        pp.levels = fused;
+        for (int i = 0; i < fused; i++) {
+            pp.la[i] = luma + lv[i].offA;
+            pp.lc[i] = luma + lv[i].offC;
+            pp.w[i] = lv[i].w;
+            pp.h[i] = lv[i].h;
+        }
+        pp.gx = cdiv(fw, 32);
+        PlanOp op;
+        op.pass = format == FGR_FORMAT_RGBA8 ? kPassPyramid : kPassPyramidFmt;
+        op.stage = 0;
+        op.gx = pp.gx;
+        op.gy = cdiv(fh, 32);
+        op.stamp = "luma";
+        op.set(pp);
+        plan.push_back(op);
+    } else {
         LumaParams lp{};
         lp.a = ref(prev);
         lp.c = ref(curr);
@@ -500,8 +531,8 @@ void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& pr
             plan.push_back(op);
         }
     }
-    // 2. pyramid
-    for (size_t i = 1; i < lv.size(); i++) {
+    // 2. the pyramid's levels past the fused ones
+    for (size_t i = (size_t)fused; i < lv.size(); i++) {
         DownParams dp{};
         dp.sa = luma + lv[i - 1].offA;
         dp.sc = luma + lv[i - 1].offC;

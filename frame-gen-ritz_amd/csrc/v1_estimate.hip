@@ -9,6 +9,7 @@
 #include <limits.h>
 #include "v1_device.hpp"
 #include "v1_launch.hpp"
+#include "v1_pyramid.hpp"
 
 namespace fgr {
 
@@ -45,6 +46,87 @@ __device__ __forceinline__ void lumaRgba8Body(const LumaParams& P, int bx, int b
     }
     P.la[oy * P.fw + ox] = (float)sa / (float)n;
     P.lc[oy * P.fw + ox] = (float)sc / (float)n;
+}
+
+// The fused pyramid's level 0 (RGBA8): the thread's 2 x 2 cells (cx0 .. cx0 + 1, cy0 .. cy0 + 1) as lumaRgba8Body
+// forms them; a pixel row of the cells' 2S-pixel segment is read as S/2 16-byte words (S = 1: one 8-byte word) when the
+// segment lies inside the frame, else pixel by pixel
+template <int S>
+__device__ __forceinline__ void pyrCellsRgba8(const PyrParams& P, int cx0, int cy0, float (&a)[2][2], float (&c)[2][2]) {
+    constexpr int SEG = 2 * S;
+    const int px0 = S * cx0;
+    const bool inside = px0 + SEG <= P.W;
+#pragma unroll
+    for (int r = 0; r < 2; r++) {
+        int sa[2] = {0, 0}, sc[2] = {0, 0};
+#pragma unroll
+        for (int j = 0; j < S; j++) {
+            const int py = S * (cy0 + r) + j;
+            if (py >= P.H) break;
+            const uint8_t* ra = P.a.p + (int64_t)py * P.a.pitch + 4 * (int64_t)px0;
+            const uint8_t* rc = P.c.p + (int64_t)py * P.c.pitch + 4 * (int64_t)px0;
+            uint32_t qa[SEG], qc[SEG];
+            if (inside) {
+                if constexpr (S == 1) {
+                    const uint2 wa = *(const uint2*)ra, wc = *(const uint2*)rc;
+                    qa[0] = wa.x; qa[1] = wa.y;
+                    qc[0] = wc.x; qc[1] = wc.y;
+                } else {
+#pragma unroll
+                    for (int q = 0; q < SEG / 4; q++) {
+                        const uint4 wa = ((const uint4*)ra)[q], wc = ((const uint4*)rc)[q];
+                        qa[4 * q] = wa.x; qa[4 * q + 1] = wa.y; qa[4 * q + 2] = wa.z; qa[4 * q + 3] = wa.w;
+                        qc[4 * q] = wc.x; qc[4 * q + 1] = wc.y; qc[4 * q + 2] = wc.z; qc[4 * q + 3] = wc.w;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < SEG; k++) {
+                    const bool ok = px0 + k < P.W;
+                    qa[k] = ok ? ((const uint32_t*)ra)[k] : 0u;
+                    qc[k] = ok ? ((const uint32_t*)rc)[k] : 0u;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < SEG; k++) {
+                if (px0 + k < P.W) {
+                    sa[k / S] += luma8(qa[k], P.bgra);
+                    sc[k / S] += luma8(qc[k], P.bgra);
+                }
+            }
+        }
+        const int ny = min(max(P.H - S * (cy0 + r), 0), S);
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            const int n = min(max(P.W - S * (cx0 + i), 0), S) * ny;
+            a[r][i] = n ? (float)sa[i] / (float)n : 0.0f;
+            c[r][i] = n ? (float)sc[i] / (float)n : 0.0f;
+        }
+    }
+}
+
+// the thread's level-0 cells into the planes (inside the level), then the coarser levels
+__device__ __forceinline__ void pyramidStore0(const PyrParams& P, int bx, int by, int cx0, int cy0, const float (&a)[2][2],
+                                              const float (&c)[2][2]) {
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            pyrStore(P.la[0], P.w[0], P.h[0], cx0 + i, cy0 + r, a[r][i]);
+            pyrStore(P.lc[0], P.w[0], P.h[0], cx0 + i, cy0 + r, c[r][i]);
+        }
+    pyramidLevels(P, bx, by, a, c);
+}
+
+__device__ __forceinline__ void pyramidRgba8Body(const PyrParams& P, int bx, int by) {
+    if (bx == 0 && by == 0)
+        for (int i = threadIdx.x; i < kUWords; i += 256) P.u[i] = 0u;
+    const int cx0 = bx * 32 + 2 * (threadIdx.x & 15), cy0 = by * 32 + 2 * (threadIdx.x >> 4);
+    float a[2][2], c[2][2];
+    if (P.S == 1) pyrCellsRgba8<1>(P, cx0, cy0, a, c);
+    else if (P.S == 2) pyrCellsRgba8<2>(P, cx0, cy0, a, c);
+    else pyrCellsRgba8<4>(P, cx0, cy0, a, c);
+    pyramidStore0(P, bx, by, cx0, cy0, a, c);
 }
 
 // down.comp:9-26: 2x2 mean of the valid pixels, summed j-outer / i-inner, times 0.25 or float(1/n)
@@ -221,12 +303,15 @@ __device__ __forceinline__ void medhistBody(const MedParams& P, int, int) {
 // The seed of block (bx, by) (step 1 above), wave-uniform: lane = pixel, SAD by the 64-lane tree. In three steps, so
 // that a wave can form the seeds of two blocks side by side: the candidates' vectors, then all their texel pairs
 // loaded together (one memory latency, not one per candidate), then the tree sums and the first-minimum choice in
-// candidate order. Slots past nc repeat the zero vector and are not considered.
+// candidate order. Slots past nc repeat the zero vector and are not considered. A candidate whose vector repeats an
+// earlier one's is skipped (no loads, no sum): its cost is that earlier candidate's SAD plus lambda >= 0, or the same
+// float, so under the strict < it can never be chosen -- the choice is unchanged.
 constexpr int kMaxSeeds = 7;
 
 struct SeedSet {
     int nc;
     int cx[kMaxSeeds], cy[kMaxSeeds];
+    bool dup[kMaxSeeds];
     float d[kMaxSeeds];
 };
 
@@ -254,6 +339,10 @@ __device__ __forceinline__ void seedVectors(const SearchParams& P, int blk, bool
         }
         S.cx[kk] = cx;
         S.cy[kk] = cy;
+        bool dup = kk >= S.nc;
+#pragma unroll
+        for (int j = 0; j < kk; j++) dup = dup || (S.cx[j] == cx && S.cy[j] == cy);
+        S.dup[kk] = dup;
     }
 }
 
@@ -261,8 +350,9 @@ __device__ __forceinline__ void seedTexels(const SearchParams& P, int bx, int by
     const int qx = bx * kBlock + (lane & 7), qy = by * kBlock + (lane >> 3);
 #pragma unroll
     for (int kk = 0; kk < kMaxSeeds; kk++)
-        S.d[kk] = fabsf(lumaAt(P.la, qx - S.cx[kk], qy - S.cy[kk], P.w, P.h) -
-                        lumaAt(P.lc, qx + S.cx[kk], qy + S.cy[kk], P.w, P.h));
+        if (!S.dup[kk])   // (wave-uniform)
+            S.d[kk] = fabsf(lumaAt(P.la, qx - S.cx[kk], qy - S.cy[kk], P.w, P.h) -
+                            lumaAt(P.lc, qx + S.cx[kk], qy + S.cy[kk], P.w, P.h));
 }
 
 __device__ __forceinline__ void seedPick(const SearchParams& P, const SeedSet& S, int& sx, int& sy) {
@@ -270,6 +360,7 @@ __device__ __forceinline__ void seedPick(const SearchParams& P, const SeedSet& S
 #pragma unroll
     for (int kk = 0; kk < kMaxSeeds; kk++) {
         if (kk >= S.nc) break;
+        if (S.dup[kk]) continue;   // (wave-uniform; never kk == 0)
         float c = treeSum64(S.d[kk]) * (1.0f / 64.0f);
         if (kk == 0) {
             best = c;
@@ -1422,6 +1513,7 @@ __device__ __forceinline__ void pixfieldWinBody(const PixParams& P, int gbx, int
 // (one launch per pass for a whole batch); both run the same body.
 
 FGR_PASS(luma_rgba8, LumaParams, lumaRgba8Body, 256)
+FGR_PASS(pyramid_rgba8, PyrParams, pyramidRgba8Body, 256)
 FGR_PASS(down, DownParams, downBody, 256)
 FGR_PASS(medhist, MedParams, medhistBody, 1024)
 FGR_PASS(search2, SearchParams, searchBody<2>, 256)
@@ -1445,6 +1537,7 @@ hipError_t launchSingle(int pass, const void* params, int gx, int gy, hipStream_
     const dim3 g((unsigned)gx, (unsigned)gy);
     switch (pass) {
     case kPassLuma: hipLaunchKernelGGL(luma_rgba8_kernel, g, dim3(256), 0, s, *(const LumaParams*)params); break;
+    case kPassPyramid: hipLaunchKernelGGL(pyramid_rgba8_kernel, g, dim3(256), 0, s, *(const PyrParams*)params); break;
     case kPassDown: hipLaunchKernelGGL(down_kernel, g, dim3(256), 0, s, *(const DownParams*)params); break;
     case kPassMedhist: hipLaunchKernelGGL(medhist_kernel, g, dim3(1024), 0, s, *(const MedParams*)params); break;
     case kPassSearch2: hipLaunchKernelGGL(search2_kernel, g, dim3(256), 0, s, *(const SearchParams*)params); break;
@@ -1466,6 +1559,7 @@ hipError_t launchBatch(int pass, const void* params, const uint32_t* work, int n
     const dim3 g((unsigned)n);
     switch (pass) {
     case kPassLuma: hipLaunchKernelGGL(luma_rgba8_batch_kernel, g, dim3(256), 0, s, (const LumaParams*)params, work); break;
+    case kPassPyramid: hipLaunchKernelGGL(pyramid_rgba8_batch_kernel, g, dim3(256), 0, s, (const PyrParams*)params, work); break;
     case kPassDown: hipLaunchKernelGGL(down_batch_kernel, g, dim3(256), 0, s, (const DownParams*)params, work); break;
     case kPassMedhist: hipLaunchKernelGGL(medhist_batch_kernel, g, dim3(1024), 0, s, (const MedParams*)params, work); break;
     case kPassSearch2: hipLaunchKernelGGL(search2_batch_kernel, g, dim3(256), 0, s, (const SearchParams*)params, work); break;

@@ -14,6 +14,7 @@
 #include "v1_launch.hpp"
 #include "v1_synth_common.hpp"
 #include "v1_formats.hpp"
+#include "v1_pyramid.hpp"
 
 namespace fgr {
 
@@ -41,6 +42,36 @@ __device__ __forceinline__ void lumaFmtBody(const LumaParams& P, int bx, int by)
     }
     P.la[oy * P.fw + ox] = (float)sa / (float)(16 * n);
     P.lc[oy * P.fw + ox] = (float)sc / (float)(16 * n);
+}
+
+// The fused pyramid (v1_pyramid.hpp) of the other formats: level 0 of the thread's 2 x 2 cells as lumaFmtBody forms it
+__device__ __forceinline__ void pyramidFmtBody(const PyrParams& P, int bx, int by) {
+    if (bx == 0 && by == 0)
+        for (int i = threadIdx.x; i < kUWords; i += 256) P.u[i] = 0u;
+    const int cx0 = bx * 32 + 2 * (threadIdx.x & 15), cy0 = by * 32 + 2 * (threadIdx.x >> 4);
+    float a[2][2], c[2][2];
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            int sa = 0, sc = 0, n = 0;
+            for (int j = 0; j < P.S; j++) {
+                const int py = P.S * (cy0 + r) + j;
+                if (py >= P.H) break;
+                for (int q = 0; q < P.S; q++) {
+                    const int px = P.S * (cx0 + i) + q;
+                    if (px >= P.W) break;
+                    sa += luma16(P, P.a, px, py);
+                    sc += luma16(P, P.c, px, py);
+                    n++;
+                }
+            }
+            a[r][i] = n ? (float)sa / (float)(16 * n) : 0.0f;
+            c[r][i] = n ? (float)sc / (float)(16 * n) : 0.0f;
+            pyrStore(P.la[0], P.w[0], P.h[0], cx0 + i, cy0 + r, a[r][i]);
+            pyrStore(P.lc[0], P.w[0], P.h[0], cx0 + i, cy0 + r, c[r][i]);
+        }
+    pyramidLevels(P, bx, by, a, c);
 }
 
 // ---------------------------------------------------------------------------------------------------------- taps
@@ -428,6 +459,7 @@ __device__ __forceinline__ void synthFmtBody(const SynthParams& P, int bx, int b
 }
 
 FGR_PASS(luma_fmt, LumaParams, lumaFmtBody, 256)
+FGR_PASS(pyramid_fmt, PyrParams, pyramidFmtBody, 256)
 FGR_PASS(synth_fmt_s1, SynthParams, synthFmtBody<1>, 256)
 FGR_PASS(synth_fmt_s2, SynthParams, synthFmtBody<2>, 256)
 FGR_PASS(synth_fmt_s4, SynthParams, synthFmtBody<4>, 256)
@@ -471,6 +503,7 @@ hipError_t launchFmtSingle(int pass, const void* params, int gx, int gy, hipStre
     const dim3 g((unsigned)gx, (unsigned)gy);
     switch (pass) {
     case kPassLumaFmt: hipLaunchKernelGGL(luma_fmt_kernel, g, dim3(256), 0, s, *(const LumaParams*)params); break;
+    case kPassPyramidFmt: hipLaunchKernelGGL(pyramid_fmt_kernel, g, dim3(256), 0, s, *(const PyrParams*)params); break;
     case kPassSynthFmtS1: hipLaunchKernelGGL(synth_fmt_s1_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
     case kPassSynthFmtS2: hipLaunchKernelGGL(synth_fmt_s2_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
     case kPassSynthFmtS4: hipLaunchKernelGGL(synth_fmt_s4_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
@@ -486,6 +519,7 @@ hipError_t launchFmtBatch(int pass, const void* params, const uint32_t* work, in
     const dim3 g((unsigned)n);
     switch (pass) {
     case kPassLumaFmt: hipLaunchKernelGGL(luma_fmt_batch_kernel, g, dim3(256), 0, s, (const LumaParams*)params, work); break;
+    case kPassPyramidFmt: hipLaunchKernelGGL(pyramid_fmt_batch_kernel, g, dim3(256), 0, s, (const PyrParams*)params, work); break;
     case kPassSynthFmtS1: hipLaunchKernelGGL(synth_fmt_s1_batch_kernel, g, dim3(256), 0, s, (const SynthParams*)params, work); break;
     case kPassSynthFmtS2: hipLaunchKernelGGL(synth_fmt_s2_batch_kernel, g, dim3(256), 0, s, (const SynthParams*)params, work); break;
     case kPassSynthFmtS4: hipLaunchKernelGGL(synth_fmt_s4_batch_kernel, g, dim3(256), 0, s, (const SynthParams*)params, work); break;

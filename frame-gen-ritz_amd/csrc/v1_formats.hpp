@@ -22,7 +22,8 @@ __device__ __forceinline__ uint32_t toHalfBits(float v) {   // round to nearest 
 //   RgbaF16 (4899 L_r + 9617 L_g + 1868 L_b + 8192) >> 14, L = the 1/16-level metric table of each half value
 //   P010    round((4 code - 256) * 255 / 219) at limited range (16 (code / 4 - 16) * 255 / 219), 4 code at full range
 //   Nv12    round((16 code - 256) * 255 / 219) at limited range (16 (code - 16) * 255 / 219), 16 code at full range
-__device__ __forceinline__ int luma16(const LumaParams& P, const FrameRef& f, int x, int y) {
+template <class LP>   // LumaParams or PyrParams: the fmt, bgra, lut and yFull fields
+__device__ __forceinline__ int luma16(const LP& P, const FrameRef& f, int x, int y) {
     const uint8_t* row = f.p + (int64_t)y * f.pitch;
     if (P.fmt == kRgb10) {
         const uint32_t w = ((const uint32_t*)row)[x];

@@ -18,7 +18,8 @@ enum Pass : int {
     kPassSynthFmtUiS1, kPassSynthFmtUiS2, kPassSynthFmtUiS4,           // the other formats' synth with tap rejection
     kPassSearch2x8,                                                    // R <= 2 search, two blocks a wavefront (batches)
     kPassPixfieldWin,                                                  // pixel field from LDS luma windows (exact lumas)
-    kPassSubpixInt                                                     // sub-pixel refinement in integers (exact lumas)
+    kPassSubpixInt,                                                    // sub-pixel refinement in integers (exact lumas)
+    kPassPyramid, kPassPyramidFmt                                      // level 0 and the coarser levels in one launch
 };
 // CrosshairV2 passes (ui.hip), single launches over the work rectangle
 enum UiPass : int { kUiUpdate = 0, kUiMask, kUiPatch, kUiFill };

@@ -47,6 +47,23 @@ struct LumaParams {
     int yFull;          // Nv12 / P010: full-range codes (else limited range: black 16 / 64, white 235 / 940)
 };
 
+// The fused luma pyramid (v1_pyramid.hpp): level 0 from the frames and the coarser levels, kPyrLevels at most
+constexpr int kPyrLevels = 6;
+struct PyrParams {
+    int gx;             // blocks per grid row (the batched kernels map a linear block index to (bx, by))
+    FrameRef a, c;
+    int W, H, S;        // frame size, flow scale
+    uint32_t* u;        // counters, cleared by block (0, 0)
+    int bgra;
+    int fmt;            // Fmt (the Rgba8 kernel ignores it)
+    const int* lut;     // RgbaF16 level table
+    int yFull;          // Nv12 / P010 full-range codes
+    int levels;         // levels written here (1 .. kPyrLevels)
+    float* la[kPyrLevels];
+    float* lc[kPyrLevels];
+    int w[kPyrLevels], h[kPyrLevels];
+};
+
 struct DownParams {
     int gx;             // blocks per grid row (the batched kernels map a linear block index to (bx, by))
     const float* sa; const float* sc;   // source level

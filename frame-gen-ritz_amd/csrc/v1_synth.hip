@@ -49,6 +49,9 @@ __device__ __forceinline__ uint32_t packRgb(const Rgb& v) { return v.r | (v.g <<
 // Four horizontally adjacent pixels (x .. x+3, one row) warped with one interpolated field value r. When their maps
 // X_i = rint(32 (x + i) - k r.x) are 32 apart (same fractional part) and no tap column needs clamping, the four share
 // Y, the weights and 5 consecutive texels per tap row, and each result equals warpRgba8 of that pixel.
+// The maps are 32 apart whenever d_0 = fl(32 x - k r.x) and d_3 = fl(32 (x + 3) - k r.x) have one sign and one exponent:
+// then the exact differences D_i = D_0 + 32 i lie in one binade, whose float grid (spacing <= 32 below 2^29) holds
+// 32 i, so fl(D_i) = fl(D_0) + 32 i, and rint (ties to even) commutes with adding the even integer 32 i.
 struct RunMap {
     int X0, Y;
     bool ok;
@@ -57,11 +60,10 @@ struct RunMap {
 __device__ __forceinline__ RunMap runMap(int W, int x, int y, float k, float2 r) {
     const float kx = k * r.x;
     RunMap m;
-    m.X0 = (int)rintf((float)(32 * x) - kx);
-    const int X1 = (int)rintf((float)(32 * (x + 1)) - kx), X2 = (int)rintf((float)(32 * (x + 2)) - kx),
-              X3 = (int)rintf((float)(32 * (x + 3)) - kx);
+    const float d0 = (float)(32 * x) - kx, d3 = (float)(32 * (x + 3)) - kx;
+    m.X0 = (int)rintf(d0);
     const int ix = m.X0 >> 5;
-    m.ok = X1 == m.X0 + 32 && X2 == m.X0 + 64 && X3 == m.X0 + 96 && ix >= 0 && ix + 4 <= W - 1;
+    m.ok = (__float_as_uint(d0) ^ __float_as_uint(d3)) < 0x00800000u && ix >= 0 && ix + 4 <= W - 1;
     m.Y = (int)rintf((float)(32 * y) - k * r.y);
     return m;
 }
