@@ -599,6 +599,9 @@ void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& pr
         }
         sp.out = F + l.fS;
         sp.sadAcc = (i == 0 && !st.subpixel) ? 1 : 0;
+        // level i of a dyadic level 0 (fb0 fractional bits) has fb0 + 2i; packed 16-bit SADs while 255.75 2^fb < 2^16
+        const int fb0 = pixExactBits();
+        sp.fb = fb0 >= 0 && fb0 + 2 * i <= 8 ? fb0 + 2 * i : -1;
         // R <= 2 in a batch: two blocks a wavefront (twice the busy lanes, half the workgroups)
         const bool wide = batched && sp.R <= 2;
         sp.gx = cdiv(l.bw * l.bh, wide ? kSearch2Blocks : 4);

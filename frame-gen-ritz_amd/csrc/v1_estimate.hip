@@ -388,15 +388,34 @@ __device__ __forceinline__ void searchSeed(const SearchParams& P, int blk, bool 
     seedPick(P, S, sx, sy);
 }
 
+// step 2 for a dyadic level (P.fb >= 0): the window values as 16-bit integers k (value k 2^-fb), each stored twice,
+// E16[i] = k_i and O16[i - 1] = k_i, so that the texel pair (k_i, k_i+1) is word i / 2 of E16 (i even) or (i - 1) / 2 of
+// O16 (i odd); the windows' rows (T = 8 + 2R) are even
+__device__ __forceinline__ void searchStoreInt(uint16_t* E, int i, int N, float v, float up) {
+    if (i < N) {
+        const uint16_t k = (uint16_t)(uint32_t)(v * up);
+        E[i] = k;
+        if (i > 0) E[N + i - 1] = k;   // O16 follows E16
+    }
+}
+
 // step 2: the (8 + 2R)^2 windows of a (at q - seed) and c (at q + seed), 64 lanes
 __device__ __forceinline__ void searchWindows(const SearchParams& P, float* A, float* C, int b0x, int b0y, int sx, int sy,
-                                             int R, int lane) {
+                                             int R, int lane, bool ints) {
     const int T = 8 + 2 * R;
     const int q = 64 / T, r = 64 - q * T;   // element i + 64 is (x + r, y + q), carried
+    const float up = __int_as_float((127 + max(P.fb, 0)) << 23);
     int x = lane % T, y = lane / T;
     for (int i = lane; i < T * T; i += 64) {
-        A[i] = lumaAt(P.la, b0x - sx - R + x, b0y - sy - R + y, P.w, P.h);
-        C[i] = lumaAt(P.lc, b0x + sx - R + x, b0y + sy - R + y, P.w, P.h);
+        const float va = lumaAt(P.la, b0x - sx - R + x, b0y - sy - R + y, P.w, P.h);
+        const float vc = lumaAt(P.lc, b0x + sx - R + x, b0y + sy - R + y, P.w, P.h);
+        if (ints) {
+            searchStoreInt((uint16_t*)A, i, T * T, va, up);
+            searchStoreInt((uint16_t*)C, i, T * T, vc, up);
+        } else {
+            A[i] = va;
+            C[i] = vc;
+        }
         x += r;
         y += q;
         if (x >= T) {
@@ -409,7 +428,7 @@ __device__ __forceinline__ void searchWindows(const SearchParams& P, float* A, f
 // step 2 with every lane's loads issued before any store (T = 8 + 2R <= MAXT; one memory latency for the window)
 template <int MAXT>
 __device__ __forceinline__ void searchWindowsIssued(const SearchParams& P, float* A, float* C, int b0x, int b0y, int sx,
-                                                    int sy, int R, int lane) {
+                                                    int sy, int R, int lane, bool ints) {
     constexpr int IT = (MAXT * MAXT + 63) / 64;
     const int T = 8 + 2 * R, N = T * T;
     const int q = 64 / T, r = 64 - q * T;   // element i + 64 is (x + r, y + q), carried
@@ -426,6 +445,15 @@ __device__ __forceinline__ void searchWindowsIssued(const SearchParams& P, float
             y++;
         }
     }
+    if (ints) {   // (uniform) the 16-bit pairs of a dyadic level
+        const float up = __int_as_float((127 + P.fb) << 23);
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            searchStoreInt((uint16_t*)A, lane + 64 * it, N, va[it], up);
+            searchStoreInt((uint16_t*)C, lane + 64 * it, N, vc[it], up);
+        }
+        return;
+    }
 #pragma unroll
     for (int it = 0; it < IT; it++) {
         const int i = lane + 64 * it;
@@ -441,7 +469,7 @@ __device__ __forceinline__ void searchWindowsIssued(const SearchParams& P, float
 template <int MAXT>
 __device__ __forceinline__ void searchWindowsPair(const SearchParams& P, float* A0, float* C0, float* A1, float* C1,
                                                   const int* bxs, const int* bys, const int* sxs, const int* sys, int R,
-                                                  int lane) {
+                                                  int lane, bool ints) {
     constexpr int IT = (MAXT * MAXT + 63) / 64;
     const int T = 8 + 2 * R, N = T * T;
     const int q = 64 / T, r = 64 - q * T;   // element i + 64 is (x + r, y + q), carried
@@ -461,6 +489,18 @@ __device__ __forceinline__ void searchWindowsPair(const SearchParams& P, float* 
             x -= T;
             y++;
         }
+    }
+    if (ints) {   // (uniform) the 16-bit pairs of a dyadic level
+        const float up = __int_as_float((127 + P.fb) << 23);
+#pragma unroll
+        for (int it = 0; it < IT; it++) {
+            const int i = lane + 64 * it;
+            searchStoreInt((uint16_t*)A0, i, N, va[0][it], up);
+            searchStoreInt((uint16_t*)C0, i, N, vc[0][it], up);
+            searchStoreInt((uint16_t*)A1, i, N, va[1][it], up);
+            searchStoreInt((uint16_t*)C1, i, N, vc[1][it], up);
+        }
+        return;
     }
 #pragma unroll
     for (int it = 0; it < IT; it++) {
@@ -485,6 +525,25 @@ __device__ __forceinline__ void searchCost(const SearchParams& P, const float* A
     for (int y = 0; y < 8; y++)
         for (int x = 0; x < 8; x++) sum = sum + fabsf(a[y * T + x] - c[y * T + x]);
     sadc = sum * (1.0f / 64.0f);
+    cost = sadc + P.lam * (float)(abs(dx) + abs(dy));
+}
+
+// step 3 for a dyadic level: the 64 |a - c| of candidate ci in 16-bit pairs (v_sad_u16, two pixels an instruction). The
+// candidate's a and c row offsets R - dx and R + dx have one parity, so both rows are four aligned words of E (even) or
+// O (odd). Every level-k SAD below 2^24 units is exact in float32 (fb <= 8), so sum k 2^-fb (1/64) is searchCost's float.
+__device__ __forceinline__ void searchCostInt(const SearchParams& P, const uint16_t* A, const uint16_t* C, int R, int ci,
+                                              float& cost, float& sadc) {
+    const int T = 8 + 2 * R, D = 2 * R + 1, N = T * T;
+    const int dx = ci % D - R, dy = ci / D - R;
+    const int ca = R - dx, cc = R + dx, par = ca & 1;
+    const uint32_t* wa = (const uint32_t*)(A + (par ? N : 0)) + (((R - dy) * T + ca - par) >> 1);
+    const uint32_t* wc = (const uint32_t*)(C + (par ? N : 0)) + (((R + dy) * T + cc - par) >> 1);
+    uint32_t acc = 0u;
+#pragma unroll
+    for (int y = 0; y < 8; y++)
+#pragma unroll
+        for (int m = 0; m < 4; m++) acc = __builtin_amdgcn_sad_u16(wa[y * (T / 2) + m], wc[y * (T / 2) + m], acc);
+    sadc = (float)acc * (__int_as_float((127 - P.fb) << 23) * (1.0f / 64.0f));
     cost = sadc + P.lam * (float)(abs(dx) + abs(dy));
 }
 
@@ -516,14 +575,16 @@ __device__ __forceinline__ void searchBody(const SearchParams& P, int bx0, int) 
     const int R = P.R, D = 2 * R + 1;
     float* A = ta[wave];
     float* C = tc[wave];
-    if constexpr (MAXR <= 6) searchWindowsIssued<MAXT>(P, A, C, bx * kBlock, by * kBlock, sx, sy, R, lane);
-    else searchWindows(P, A, C, bx * kBlock, by * kBlock, sx, sy, R, lane);
+    const bool ints = P.fb >= 0;   // (uniform) packed 16-bit SADs
+    if constexpr (MAXR <= 6) searchWindowsIssued<MAXT>(P, A, C, bx * kBlock, by * kBlock, sx, sy, R, lane, ints);
+    else searchWindows(P, A, C, bx * kBlock, by * kBlock, sx, sy, R, lane, ints);
     __syncthreads();
     float bestc = FLT_MAX, bests = 0.0f;
     int besti = INT_MAX;
     for (int ci = lane; ci < D * D; ci += 64) {
         float cost, sadc;
-        searchCost(P, A, C, R, ci, cost, sadc);
+        if (ints) searchCostInt(P, (const uint16_t*)A, (const uint16_t*)C, R, ci, cost, sadc);
+        else searchCost(P, A, C, R, ci, cost, sadc);
         if (cost < bestc) {
             bestc = cost;
             besti = ci;
@@ -568,7 +629,9 @@ __device__ __forceinline__ void search2Body(const SearchParams& P, int bx0, int)
 #pragma unroll
         for (int h = 0; h < 2; h++) seedPick(P, S[h], sxs[h], sys[h]);
     }
-    searchWindowsPair<MAXT>(P, ta[2 * wave], tc[2 * wave], ta[2 * wave + 1], tc[2 * wave + 1], bxs, bys, sxs, sys, R, lane);
+    const bool ints = P.fb >= 0;   // (uniform) packed 16-bit SADs
+    searchWindowsPair<MAXT>(P, ta[2 * wave], tc[2 * wave], ta[2 * wave + 1], tc[2 * wave + 1], bxs, bys, sxs, sys, R, lane,
+                            ints);
     __syncthreads();
     const int half = lane >> 5, ci = lane & 31;
     const int blk = bx0 * kSearch2Blocks + 2 * wave + half;
@@ -577,7 +640,8 @@ __device__ __forceinline__ void search2Body(const SearchParams& P, int bx0, int)
     int besti = INT_MAX;
     if (ci < D * D) {
         float cost, sadc;
-        searchCost(P, ta[2 * wave + half], tc[2 * wave + half], R, ci, cost, sadc);
+        if (ints) searchCostInt(P, (const uint16_t*)ta[2 * wave + half], (const uint16_t*)tc[2 * wave + half], R, ci, cost, sadc);
+        else searchCost(P, ta[2 * wave + half], tc[2 * wave + half], R, ci, cost, sadc);
         bestc = cost;
         besti = ci;
         bests = sadc;

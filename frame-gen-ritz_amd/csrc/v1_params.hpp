@@ -91,6 +91,7 @@ struct SearchParams {
     float lam;
     float2* out;                                  // raw search field (bw x bh)
     uint32_t* u; int sadAcc;
+    int fb;             // this level's luma is k 2^-fb with 0 <= k < 2^16 (packed 16-bit SADs), or -1 (float SADs)
 };
 
 struct Median3Params {
