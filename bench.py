@@ -413,9 +413,12 @@ def main():
         "batched_over_plain": None if batch_rate is None else round(batch_rate / plain_rate, 3),
         "whole_path_hbm_frac": round(total_bytes * args.steps / (ms * 1e-3 * args.steps) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": roofline,
-        "tuning": None if batch is None else {"synth_tile_rows": tuning[0], "xcd_chunk": tuning[1], "lanes": tuning[2],
-                                               "us_per_step": {f"{r}x{c}": round(t / 1e3, 1)
-                                                               for (r, c), t in tune_times.items()}},
+        "tuning": None if batch is None else {
+            "synth_tile_rows": tuning[0], "xcd_chunk": tuning[1], "lanes": tuning[2],
+            "pixfield": "windows" if batch.get_form() & fgr.Batch.FORM_PIXFIELD_WINDOWS else "gather",
+            "search_r2": "4x1" if batch.get_form() & fgr.Batch.FORM_SEARCH_NARROW else "2x8",
+            "us_per_step": {(f"{r}x{c}" if isinstance(r, int) else f"{r}_{c}"): round(t / 1e3, 1)
+                            for (r, c), t in tune_times.items()}},
         "copy_probe": {"bytes": copy_bytes, "gbs": round(copy_gbs, 1), "frac_of_peak": round(copy_gbs / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
     }

@@ -65,6 +65,7 @@ struct FgrBatch {
     int chunk = kXcdChunk;            // tiles per XCD chunk of the work lists (tuned)
     int rows = 32;                    // RGBA8 synth tile rows (tuned)
     int lanes = 1;                    // 1 .. kMaxLanes (tuned)
+    int form = 0;                     // estimate forms, FGR_FORM_* (tuned)
     std::vector<int> laneOf;          // each pair's lane
 
     ~FgrBatch() {
@@ -177,11 +178,15 @@ struct FgrBatch {
     }
 
     // a tile configuration: rebuild the launches and the device blob (its size changes with the grids)
-    bool tune(int r, int c, int l = -1) {
+    bool tune(int r, int c, int l = -1, int f = -1) {
         rows = r;
         chunk = c;
         if (l > 0) lanes = l;
-        for (FgrInterpolator* p : pairs) p->synthRows = r;
+        if (f >= 0) form = f;
+        for (FgrInterpolator* p : pairs) {
+            p->synthRows = r;
+            p->estForm = form;
+        }
         build();
         dropGraph();
         if (blob.size() > devCap) {
@@ -206,7 +211,7 @@ struct FgrBatch {
 
 // Tile configurations chosen by fgr_batch_autotune, per batch key (process-wide; fgr_tuning_save / load persist it)
 struct Tuning {
-    int rows, chunk, lanes;
+    int rows, chunk, lanes, form;
 };
 static std::map<std::string, Tuning>& tuningCache() {
     static std::map<std::string, Tuning> m;
@@ -272,7 +277,11 @@ extern "C" FgrBatch* fgr_batch_create(const FgrBatchInfo* info, const FgrBatchPa
         b->rows = hit->second.rows;
         b->chunk = hit->second.chunk;
         b->lanes = hit->second.lanes;
-        for (FgrInterpolator* p : b->pairs) p->synthRows = b->rows;
+        b->form = hit->second.form;
+        for (FgrInterpolator* p : b->pairs) {
+            p->synthRows = b->rows;
+            p->estForm = b->form;
+        }
     }
     b->build();
     if (hipMalloc(&b->dev, b->blob.size()) != hipSuccess) { gBatchError = kBErrAlloc; delete b; return nullptr; }
@@ -360,11 +369,12 @@ extern "C" int fgr_batch_set_lanes(FgrBatch* b, int32_t lanes) {
 
 extern "C" int32_t fgr_batch_get_lanes(const FgrBatch* b) { return b ? b->lanes : 0; }
 
-// Times every tile configuration (RGBA8 synth tiles of 16, 32, 64 rows x work-list XCD chunks of 16, 64, 256 tiles)
-// on this batch's own records -- `reps` eager records each, after one untimed -- keeps the fastest, and caches it for
-// batches of the same key; then 1, 2, 3 and 4 lanes with those tiles. The records write the batch's outputs. ns[i]
-// receives configuration i's time per record (i = rowsIndex * 3 + chunkIndex; then ns[9 .. 12] 1 .. 4 lanes; 13
-// entries) when given.
+// Times every tile configuration (RGBA8 synth tiles of 16, 32, 64 rows x work-list XCD chunks of 16, 64, 256 tiles,
+// at one lane and the default forms) on this batch's own records -- `reps` eager records each, after one untimed --
+// keeps the fastest; then each estimate form knob on and off with those tiles; then 1, 2, 3 and 4 lanes with the best
+// of both; and caches the result for batches of the same key. The records write the batch's outputs. ns[i] receives
+// configuration i's time per record (i = rowsIndex * 3 + chunkIndex; then ns[9 .. 12] 1 .. 4 lanes; ns[13 .. 16] the
+// forms; 17 entries) when given.
 extern "C" int fgr_batch_autotune(FgrBatch* b, void* stream, int32_t reps, uint64_t* ns) {
     if (!b || reps < 1) return 0;
     hipSetDevice(b->info.device);
@@ -377,7 +387,7 @@ extern "C" int fgr_batch_autotune(FgrBatch* b, void* stream, int32_t reps, uint6
     bool ok = true;
     for (int ri = 0; ri < 3 && ok; ri++)
         for (int ci = 0; ci < 3 && ok; ci++) {
-            if (!b->tune(kRows[ri], kChunks[ci]) || !b->enqueue(s)) { ok = false; break; }
+            if (!b->tune(kRows[ri], kChunks[ci], 1, 0) || !b->enqueue(s)) { ok = false; break; }
             hipEventRecord(e0, s);
             for (int r = 0; r < reps && ok; r++) ok = b->enqueue(s);
             hipEventRecord(e1, s);
@@ -392,10 +402,31 @@ extern "C" int fgr_batch_autotune(FgrBatch* b, void* stream, int32_t reps, uint6
                 bestC = kChunks[ci];
             }
         }
-    // then 1 .. 4 lanes with the best tiles (ns[9 .. 12] when given)
+    // the estimate forms with the best tiles at one lane, a knob at a time (ns[13 .. 16] when given): the pixel field
+    // gathering / from LDS windows, then the radius <= 2 search two blocks / one block a wavefront
+    int bestF = 0;
+    static const int kKnobs[2] = {FGR_FORM_PIXFIELD_WINDOWS, FGR_FORM_SEARCH_NARROW};
+    for (int kb = 0; kb < 2 && ok; kb++) {
+        double tk[2] = {0.0, 0.0};
+        for (int on = 0; on < 2 && ok; on++) {
+            const int f = on ? (bestF | kKnobs[kb]) : (bestF & ~kKnobs[kb]);
+            if (!b->tune(bestR, bestC, 1, f) || !b->enqueue(s)) { ok = false; break; }
+            hipEventRecord(e0, s);
+            for (int r = 0; r < reps && ok; r++) ok = b->enqueue(s);
+            hipEventRecord(e1, s);
+            if (hipEventSynchronize(e1) != hipSuccess) { ok = false; break; }
+            float ms = 0.0f;
+            hipEventElapsedTime(&ms, e0, e1);
+            tk[on] = (double)ms / reps;
+            if (ns) ns[13 + 2 * kb + on] = (uint64_t)(tk[on] * 1e6 + 0.5);
+        }
+        if (ok && tk[1] < tk[0]) bestF |= kKnobs[kb];
+        else bestF &= ~kKnobs[kb];
+    }
+    // then 1 .. 4 lanes with the best tiles and forms (ns[9 .. 12] when given)
     double laneT[FgrBatch::kMaxLanes + 1] = {};
     for (int l = 1; l <= FgrBatch::kMaxLanes && ok; l++) {
-        if (!b->tune(bestR, bestC, l) || !b->enqueue(s)) { ok = false; break; }
+        if (!b->tune(bestR, bestC, l, bestF) || !b->enqueue(s)) { ok = false; break; }
         hipEventRecord(e0, s);
         for (int r = 0; r < reps && ok; r++) ok = b->enqueue(s);
         hipEventRecord(e1, s);
@@ -411,17 +442,26 @@ extern "C" int fgr_batch_autotune(FgrBatch* b, void* stream, int32_t reps, uint6
         if (laneT[l] < laneT[bestL]) bestL = l;
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    if (!ok || !b->tune(bestR, bestC, bestL)) return 0;
-    tuningCache()[b->key()] = {bestR, bestC, bestL};
+    if (!ok || !b->tune(bestR, bestC, bestL, bestF)) return 0;
+    tuningCache()[b->key()] = {bestR, bestC, bestL, bestF};
     return 1;
 }
 
-// the tuning cache as text, one "key rows chunk lanes" line per entry
+extern "C" int fgr_batch_set_form(FgrBatch* b, int32_t form) {
+    if (!b || form < 0 || form > (FGR_FORM_PIXFIELD_WINDOWS | FGR_FORM_SEARCH_NARROW)) return 0;
+    hipSetDevice(b->info.device);
+    return b->tune(b->rows, b->chunk, b->lanes, form) ? 1 : 0;
+}
+
+extern "C" int32_t fgr_batch_get_form(const FgrBatch* b) { return b ? b->form : 0; }
+
+// the tuning cache as text, one "key rows chunk lanes form" line per entry (a line without the form reads as form 0)
 extern "C" int fgr_tuning_save(const char* path) {
     FILE* f = path ? std::fopen(path, "w") : nullptr;
     if (!f) return 0;
     for (const auto& e : tuningCache())
-        std::fprintf(f, "%s %d %d %d\n", e.first.c_str(), e.second.rows, e.second.chunk, e.second.lanes);
+        std::fprintf(f, "%s %d %d %d %d\n", e.first.c_str(), e.second.rows, e.second.chunk, e.second.lanes,
+                     e.second.form);
     std::fclose(f);
     return 1;
 }
@@ -429,10 +469,13 @@ extern "C" int fgr_tuning_save(const char* path) {
 extern "C" int fgr_tuning_load(const char* path) {
     FILE* f = path ? std::fopen(path, "r") : nullptr;
     if (!f) return 0;
-    char key[65536];
-    int r, c, l, n = 0;
-    while (std::fscanf(f, "%65535s %d %d %d", key, &r, &c, &l) == 4) {
-        tuningCache()[key] = {r, c, std::min(std::max(l, 1), FgrBatch::kMaxLanes)};
+    static thread_local char line[65600], key[65600];
+    int n = 0;
+    while (std::fgets(line, sizeof line, f)) {
+        int r, c, l, fm = 0;
+        if (std::sscanf(line, "%65535s %d %d %d %d", key, &r, &c, &l, &fm) < 4) continue;
+        tuningCache()[key] = {r, c, std::min(std::max(l, 1), FgrBatch::kMaxLanes),
+                              fm & (FGR_FORM_PIXFIELD_WINDOWS | FGR_FORM_SEARCH_NARROW)};
         n++;
     }
     std::fclose(f);

@@ -603,7 +603,7 @@ void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& pr
         const int fb0 = pixExactBits();
         sp.fb = fb0 >= 0 && fb0 + 2 * i <= 8 ? fb0 + 2 * i : -1;
         // R <= 2 in a batch: two blocks a wavefront (twice the busy lanes, half the workgroups)
-        const bool wide = batched && sp.R <= 2;
+        const bool wide = batched && sp.R <= 2 && !(estForm & FGR_FORM_SEARCH_NARROW);
         sp.gx = cdiv(l.bw * l.bh, wide ? kSearch2Blocks : 4);
         PlanOp op;
         op.pass = wide ? kPassSearch2x8 : (sp.R <= 2 ? kPassSearch2 : (sp.R <= 6 ? kPassSearch6 : kPassSearch12));
@@ -667,9 +667,10 @@ void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& pr
     pp.fb = pixExactBits();
     pp.gx = cdiv(l0.w, 32);
     PlanOp op;
-    // the LDS-window kernel where latency decides (one pair's grid); batches fill the GPU and run the gathering kernel,
-    // whose higher occupancy gives more throughput (the same floats either way)
-    op.pass = pp.fb >= 0 && !batched ? kPassPixfieldWin : kPassPixfield;
+    // the LDS-window kernel where latency decides (one pair's grid); in a batch the tuner's choice: by default the
+    // gathering kernel, whose higher occupancy gives more throughput when the grids fill the GPU (the same floats)
+    const bool win = pp.fb >= 0 && (!batched || (estForm & FGR_FORM_PIXFIELD_WINDOWS));
+    op.pass = win ? kPassPixfieldWin : kPassPixfield;
     op.stage = 401;
     op.gx = pp.gx;
     op.gy = cdiv(l0.h, 32);

@@ -87,6 +87,7 @@ struct FgrInterpolator {
     // learned seed proposals (seednet.hip): the attached network and its activations on the coarsest level
     FgrSeedNet* net = nullptr;
     int synthRows = 32;               // RGBA8 synth tile height (16, 32, 64 with flowScale 2), chosen by the batch tuner
+    int estForm = 0;                  // batched estimate kernels, chosen by the batch tuner (FGR_FORM_*)
     // recordSynthBlur's sub-frames (blur.hip): sized when settings or sizes are applied with blurSamples >= 2 and only
     // grown; a replaced buffer is kept (records queued before the settings change may still read it) until the
     // handle's buffers are freed, so no record allocates

@@ -168,6 +168,8 @@ _sig = {
     "fgr_batch_set_tuning": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.c_int32]),
     "fgr_batch_get_tuning": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "fgr_batch_set_lanes": (ctypes.c_int, [_H, ctypes.c_int32]),
+    "fgr_batch_set_form": (ctypes.c_int, [_H, ctypes.c_int32]),
+    "fgr_batch_get_form": (ctypes.c_int32, [_H]),
     "fgr_batch_get_lanes": (ctypes.c_int32, [_H]),
     "fgr_tuning_save": (ctypes.c_int, [ctypes.c_char_p]),
     "fgr_tuning_load": (ctypes.c_int, [ctypes.c_char_p]),
@@ -519,16 +521,27 @@ class Batch:
 
     TUNE_ROWS, TUNE_CHUNKS = (16, 32, 64), (16, 64, 256)
 
+    FORM_PIXFIELD_WINDOWS, FORM_SEARCH_NARROW = 1, 2
+
     def autotune(self, reps=3, stream=None):
-        """Time the 9 tile configurations on this batch, keep the fastest, then 1 to 4 lanes with it; returns
-        ({(rows, chunk): ns, ("lanes", L): ns}, (rows, chunk, lanes))."""
-        ns = (ctypes.c_uint64 * 13)()
+        """Time the 9 tile configurations on this batch, keep the fastest, then each estimate form knob, then 1 to 4
+        lanes; returns ({(rows, chunk): ns, ("lanes", L): ns, ("pixfield", "gather" | "windows"): ns,
+        ("search", "2x8" | "4x1"): ns}, (rows, chunk, lanes))."""
+        ns = (ctypes.c_uint64 * 17)()
         if not lib.fgr_batch_autotune(self._h, _stream(stream), reps, ns):
             raise RuntimeError("fgr_batch_autotune failed")
         times = {(r, c): int(ns[i * 3 + j]) for i, r in enumerate(self.TUNE_ROWS) for j, c in enumerate(self.TUNE_CHUNKS)}
         for lanes in range(1, 5):
             times[("lanes", lanes)] = int(ns[8 + lanes])
+        times[("pixfield", "gather")], times[("pixfield", "windows")] = int(ns[13]), int(ns[14])
+        times[("search", "2x8")], times[("search", "4x1")] = int(ns[15]), int(ns[16])
         return times, self.get_tuning()
+
+    def set_form(self, form):
+        return bool(lib.fgr_batch_set_form(self._h, form))
+
+    def get_form(self):
+        return lib.fgr_batch_get_form(self._h)
 
     def set_tuning(self, rows, chunk, lanes=None):
         ok = bool(lib.fgr_batch_set_tuning(self._h, rows, chunk))

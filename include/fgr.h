@@ -249,10 +249,10 @@ uint32_t fgr_batch_launches(const FgrBatch* b);
 int fgr_batch_set_profiling(FgrBatch* b, int on);
 /* Tile autotuner: the tile configurations are the RGBA8 synth's tile height (16, 32 or 64 rows) and the number of
  * consecutive tiles each XCD takes from a launch's work list (16, 64 or 256). autotune times all nine on this batch's
- * own records (`reps` each; the outputs are written), keeps the fastest, then times 1 to 4 lanes (below) with it,
- * and remembers the choice for batches of the same pairs (sizes, formats, multipliers, flowScale, pyramidMinSide),
- * which then start with it. Results are bit-identical under every configuration. ns: 13 times per record (ns; index
- * rows * 3 + chunk, then 1 .. 4 lanes), or null. */
+ * own records (`reps` each; the outputs are written), keeps the fastest, then the estimate forms (below) one knob at a
+ * time, then 1 to 4 lanes (below), and remembers the choice for batches of the same pairs (sizes, formats,
+ * multipliers, flowScale, pyramidMinSide), which then start with it. Results are bit-identical under every
+ * configuration. ns: 17 times per record (ns; index rows * 3 + chunk, then 1 .. 4 lanes, then the forms), or null. */
 int fgr_batch_autotune(FgrBatch* b, void* stream, int32_t reps, uint64_t* ns);
 int fgr_batch_set_tuning(FgrBatch* b, int32_t rows, int32_t chunk);
 int fgr_batch_get_tuning(const FgrBatch* b, int32_t* rows, int32_t* chunk);
@@ -261,6 +261,13 @@ int fgr_batch_get_tuning(const FgrBatch* b, int32_t* rows, int32_t* chunk);
  * record's stream, so the short passes of one group run beside the long ones of another. Results are unchanged. */
 int fgr_batch_set_lanes(FgrBatch* b, int32_t lanes);
 int32_t fgr_batch_get_lanes(const FgrBatch* b);
+/* Estimate forms of a batch (bits; bit-identical results; fgr_batch_autotune times each, ns[13 .. 16] = pixel field
+ * gathering / LDS windows, then search two blocks a wavefront / one block a wavefront, with the best tiles):
+ *   FGR_FORM_PIXFIELD_WINDOWS: the pixel field from LDS windows of the luma (lumas with exact integer sampling) instead
+ *     of gathering every tap from the planes; FGR_FORM_SEARCH_NARROW: the radius <= 2 searches one block a wavefront. */
+enum { FGR_FORM_PIXFIELD_WINDOWS = 1, FGR_FORM_SEARCH_NARROW = 2 };
+int fgr_batch_set_form(FgrBatch* b, int32_t form);
+int32_t fgr_batch_get_form(const FgrBatch* b);
 int fgr_tuning_save(const char* path);
 int fgr_tuning_load(const char* path);   /* returns the number of entries read */
 int fgr_batch_read_timings(FgrBatch* b, int32_t* stage, int32_t* pass, uint64_t* ticks, int cap);

@@ -140,12 +140,15 @@ def test_1080p_workload_batch_matches_per_pair():
     b.close()
 
 
-@pytest.mark.parametrize("rows,chunk,lanes", [(16, 16, 1), (32, 256, 2), (64, 64, 3), (32, 64, 4)])
-def test_tile_configurations_are_bit_identical(rows, chunk, lanes):
-    """Every configuration of the tuner (synth tile rows x XCD chunk x lanes) gives the per-pair path's bytes."""
+@pytest.mark.parametrize("rows,chunk,lanes,form", [(16, 16, 1, 0), (32, 256, 2, 1), (64, 64, 3, 2), (32, 64, 4, 3),
+                                                    (32, 64, 1, 3)])
+def test_tile_configurations_are_bit_identical(rows, chunk, lanes, form):
+    """Every configuration of the tuner (synth tile rows x XCD chunk x lanes x estimate forms: the pixel field from
+    LDS windows, the radius-2 search one block a wavefront) gives the per-pair path's bytes."""
     pairs, mults = mixed_pairs()
     b, items = make_batch(pairs, mults, use_graph=True)
     assert b.set_tuning(rows, chunk, lanes) and b.get_tuning() == (rows, chunk, lanes)
+    assert b.set_form(form) and b.get_form() == form
     b.run()
     torch.cuda.synchronize()
     for p, ((a, c), n) in enumerate(zip(pairs, mults)):
@@ -159,11 +162,17 @@ def test_autotune_picks_the_fastest_and_caches_it(tmp_path):
     pairs, _ = scenes.pairs_1080p(2, n_pairs=4)
     b, items = make_batch(pairs, [4] * 4, use_graph=True)
     times, chosen = b.autotune(reps=3)
-    assert len(times) == 13 and all(t > 0 for t in times.values())
-    tiles = {k: v for k, v in times.items() if k[0] != "lanes"}
+    assert len(times) == 17 and all(t > 0 for t in times.values())
+    tiles = {k: v for k, v in times.items() if isinstance(k[0], int)}
     assert tiles[chosen[:2]] == min(tiles.values())
     lanes = {k[1]: v for k, v in times.items() if k[0] == "lanes"}
     assert lanes[chosen[2]] == min(lanes.values())
+    form = b.get_form()
+    for knob, key, on in ((fgr.Batch.FORM_PIXFIELD_WINDOWS, "pixfield", "windows"),
+                          (fgr.Batch.FORM_SEARCH_NARROW, "search", "4x1")):
+        t_on, t_off = times[(key, on)], min(v for (k, x), v in times.items() if k == key and x != on)
+        if t_on != t_off:
+            assert bool(form & knob) == (t_on < t_off), (key, t_on, t_off)
     b.run()
     torch.cuda.synchronize()
     want = per_pair(*pairs[1], 4)
@@ -171,7 +180,7 @@ def test_autotune_picks_the_fastest_and_caches_it(tmp_path):
         assert np.array_equal(items[1][3][k].cpu().numpy(), want[k])
     # a second batch of the same pairs starts with the cached choice; the cache survives a save / load
     b2, _ = make_batch(pairs, [4] * 4, use_graph=True)
-    assert b2.get_tuning() == chosen
+    assert b2.get_tuning() == chosen and b2.get_form() == form
     path = tmp_path / "tuning.txt"
     assert fgr.tuning_save(path) and fgr.tuning_load(path) >= 1
     b.close()
