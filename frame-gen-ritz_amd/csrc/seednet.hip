@@ -7,11 +7,13 @@
 // seed penalty lambda, as every non-own candidate does) and then searches +-R around it, so a wrong proposal costs
 // nothing but the candidate's SAD, and the exact SAD comparison against the plain seed decides.
 //
-// Every convolution -- forward, and both gradients -- is an implicit GEMM on the BF16 matrix cores
-// (v_mfma_f32_16x16x16_bf16, FP32 accumulation): a wavefront owns a 16 x 16 output tile and gathers its A and B operands
-// straight from the NHWC activations and the weights, rounding them to bf16 (round to nearest even). Activations,
-// weights, gradients and the Adam state are FP32. Training runs on the GPU with these kernels: forward, mean squared
-// error against the plain search's coarsest-level field, backward, Adam.
+// Every convolution -- forward, and both gradients -- is an implicit GEMM on the matrix cores with FP32 accumulation.
+// Training (forward, mean squared error against the plain search's coarsest-level field, backward, Adam) keeps FP32
+// activations, weights, gradients and Adam state: a wavefront owns a 16 x 16 output tile and gathers its operands
+// straight from the NHWC activations and the weights, rounding them to bf16 (round to nearest even), 32-deep K steps
+// on v_mfma_f32_16x16x32_bf16. Inference (the proposals) stores each layer's activations once, quantised for the next
+// layer -- BF16, or OCP FP8 E4M3 with per-channel power-of-two scales -- so that a lane's operand of one K step is a
+// single 8- or 16-byte load, with the weights staged in LDS (v_mfma_f32_16x16x32_bf16 / _fp8_fp8).
 //
 // Layouts: activations NHWC float32 [n][h][w][C]; layer l's weights W_l [Cout][9 Cin] with k = (ky * 3 + kx) * Cin + ci,
 // then its bias b_l [Cout]; all layers' parameters in one float array in layer order.
@@ -19,6 +21,7 @@
 #include <cmath>
 #include <cstring>
 #include <random>
+#include <type_traits>
 #include <vector>
 #include "../../include/fgr.h"
 #include "runtime.hpp"
@@ -28,14 +31,43 @@
 
 namespace fgr {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
-// float32 -> bf16 bits, round to nearest even (finite inputs)
-__device__ __forceinline__ short bf16Bits(float f) {
-    const uint32_t u = __float_as_uint(f);
-    return (short)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+// two float32 -> two bf16 (round to nearest even; v_cvt_pk_bf16_f32), `lo` in the low half
+__device__ __forceinline__ uint32_t bf16Pair(float lo, float hi) {
+    const f32x2 v = {lo, hi};
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2));
 }
+
+// eight float32 -> the BF16 operand of one 16x16x32 MFMA step (k ascending)
+__device__ __forceinline__ bf16x8 bf16Op(const float (&v)[8]) {
+    const u32x4 u = {bf16Pair(v[0], v[1]), bf16Pair(v[2], v[3]), bf16Pair(v[4], v[5]), bf16Pair(v[6], v[7])};
+    return __builtin_bit_cast(bf16x8, u);
+}
+
+__device__ __forceinline__ f32x4 mfmaBf16(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ f32x4 mfmaFp8(uint64_t a, uint64_t b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)a, (long)b, c, 0, 0, 0);
+}
+
+// float32 -> OCP FP8 E4M3 (bias 7, largest finite 448) on the conversion instruction (v_cvt_pk_fp8_f32, round to
+// nearest even), four values packed a..d from the low byte. The values are clamped to +-448 first, so that larger
+// magnitudes saturate to the largest finite value.
+__device__ __forceinline__ float sat448(float v) { return fminf(fmaxf(v, -448.0f), 448.0f); }
+__device__ __forceinline__ uint32_t fp8Quad(float a, float b, float c, float d) {
+    const int lo = __builtin_amdgcn_cvt_pk_fp8_f32(sat448(a), sat448(b), 0, false);
+    return (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(sat448(c), sat448(d), lo, true);
+}
+
+// ------------------------------------------------------------------------------------ float32 activations (training)
 
 // one operand value of the input of a convolution at input pixel (ix, iy), channel ci (zero outside the image)
 __device__ __forceinline__ float inputAt(const ConvParams& P, int img, int ix, int iy, int ci) {
@@ -52,70 +84,6 @@ __device__ __forceinline__ float gradAt(const ConvParams& P, int img, int p, int
     const int64_t i = ((int64_t)img * P.outW * P.outH + p) * P.Cout + co;
     const float g = P.dy[i];
     return (P.relu && !(P.out[i] > 0.0f)) ? 0.0f : g;
-}
-
-__device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
-}
-
-// float32 -> OCP FP8 E4M3 (bias 7, largest finite 448), round to nearest even, saturating; written out so that the CPU
-// oracle can follow it bit for bit
-__device__ __forceinline__ uint32_t fp8e4m3(float v) {
-    const uint32_t s = v < 0.0f ? 0x80u : 0u;
-    const float a = fabsf(v);
-    if (!(a < 448.0f)) return s | 0x7Eu;
-    if (a < 0.015625f) return s | (uint32_t)(int)rintf(a * 512.0f);   // subnormals: multiples of 2^-9 (8 -> 2^-6)
-    int e;
-    frexpf(a, &e);
-    const int eb = e - 1;                                              // a in [2^eb, 2^(eb + 1))
-    int M = (int)rintf(ldexpf(a, 3 - eb)) - 8, E = eb + 7;
-    if (M == 8) {
-        M = 0;
-        E++;
-    }
-    if (E > 15 || (E == 15 && M == 7)) return s | 0x7Eu;
-    return s | (uint32_t)(E << 3) | (uint32_t)M;
-}
-
-// FP8 forward (v_mfma_f32_16x16x32_fp8_fp8): the tile of convFwdBody with 32-deep K steps, 8 E4M3 operands a lane
-__device__ __forceinline__ void convFwdFp8(const ConvParams& P, int bx, int by) {
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int Pn = P.outW * P.outH, K = 9 * P.Cin;
-    const int nPT = (Pn + 15) >> 4, nCT = (P.Cout + 15) >> 4;
-    const int t = bx * 4 + wave;
-    if (t >= nPT * nCT) return;
-    const int ct = t % nCT, pt = t / nCT, img = by;
-    const int row = lane & 15, quad = lane >> 4;
-    const int p = pt * 16 + row;
-    const int ox = p % P.outW, oy = p / P.outW;
-    const int co = ct * 16 + row;
-    f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int k0 = 0; k0 < K; k0 += 32) {
-        uint64_t a = 0, b = 0;
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int k = k0 + 8 * quad + j;
-            uint32_t aq = 0, bq = 0;
-            if (k < K) {
-                const int tap = k / P.Cin, ci = k - tap * P.Cin;
-                const int ky = tap / 3, kx = tap - 3 * ky;
-                if (p < Pn) aq = fp8e4m3(inputAt(P, img, ox * P.stride + kx - 1, oy * P.stride + ky - 1, ci) / P.sa[ci]);
-                if (co < P.Cout) bq = P.wq[co * P.kp + k];
-            }
-            a |= (uint64_t)aq << (8 * j);
-            b |= (uint64_t)bq << (8 * j);
-        }
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)a, (long)b, acc, 0, 0, 0);
-    }
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const int pp = pt * 16 + 4 * quad + j, cc = ct * 16 + row;
-        if (pp < Pn && cc < P.Cout) {
-            float v = acc[j] * P.sw[cc] + P.b[cc];
-            if (P.relu) v = fmaxf(v, 0.0f);
-            P.out[((int64_t)img * Pn + pp) * P.Cout + cc] = v;
-        }
-    }
 }
 
 // per-channel largest |input| of a layer over all its pixels and images (one workgroup per channel), for the FP8 scales
@@ -136,40 +104,35 @@ __global__ __launch_bounds__(256) void conv_absmax_kernel(ConvParams P, float* o
     if (threadIdx.x == 0) out[ci] = red[0];
 }
 
-// Forward: out[img][p][co] = relu?(b[co] + sum_k A[p][k] W[co][k]), A = the 3x3 patch of input pixel
-// (ox s + kx - 1, oy s + ky - 1). Wave tile: 16 output pixels x 16 output channels; workgroup: 4 tiles.
-__device__ __forceinline__ void convFwdBody(const ConvParams& P, int bx, int by) {
-    if (P.fp8) {
-        convFwdFp8(P, bx, by);
-        return;
-    }
+// Forward on float32 activations (training and FP8 calibration): out[img][p][co] = relu?(b[co] + sum_k A[p][k]
+// W[co][k]), A = the 3x3 patch of input pixel (ox s + kx - 1, oy s + ky - 1), operands rounded to bf16. Wave tile:
+// 16 output pixels x 16 output channels, 32-deep K steps (lane quad q holds k = k0 + 8 q .. + 7); workgroup: 4 tiles.
+__global__ __launch_bounds__(256) void conv_fwd_kernel(ConvParams P) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int Pn = P.outW * P.outH, K = 9 * P.Cin;
     const int nPT = (Pn + 15) >> 4, nCT = (P.Cout + 15) >> 4;
-    const int t = bx * 4 + wave;
+    const int t = (int)blockIdx.x * 4 + wave;
     if (t >= nPT * nCT) return;
-    const int ct = t % nCT, pt = t / nCT, img = by;
+    const int ct = t % nCT, pt = t / nCT, img = (int)blockIdx.y;
     const int row = lane & 15, quad = lane >> 4;
     const int p = pt * 16 + row;
     const int ox = p % P.outW, oy = p / P.outW;
     const int co = ct * 16 + row;
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        bf16x4 a, b;
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        float av[8], bv[8];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int k = k0 + 4 * quad + j;
-            float av = 0.0f, bv = 0.0f;
+        for (int j = 0; j < 8; j++) {
+            const int k = k0 + 8 * quad + j;
+            av[j] = bv[j] = 0.0f;
             if (k < K) {
                 const int tap = k / P.Cin, ci = k - tap * P.Cin;
                 const int ky = tap / 3, kx = tap - 3 * ky;
-                if (p < Pn) av = inputAt(P, img, ox * P.stride + kx - 1, oy * P.stride + ky - 1, ci);
-                if (co < P.Cout) bv = P.w[co * K + k];
+                if (p < Pn) av[j] = inputAt(P, img, ox * P.stride + kx - 1, oy * P.stride + ky - 1, ci);
+                if (co < P.Cout) bv[j] = P.w[co * K + k];
             }
-            a[j] = bf16Bits(av);
-            b[j] = bf16Bits(bv);
         }
-        acc = mfma16(a, b, acc);
+        acc = mfmaBf16(bf16Op(av), bf16Op(bv), acc);
     }
     // D[row 4 quad + j][col row]: pixel pt * 16 + 4 quad + j, channel ct * 16 + row
 #pragma unroll
@@ -185,38 +148,36 @@ __device__ __forceinline__ void convFwdBody(const ConvParams& P, int bx, int by)
 
 // Gradient of the input: dx[img][q][ci] = sum over (tap, co) of W[co][tap Cin + ci] dYm[(q - tap + 1) / s][co] where
 // the output pixel is whole and inside (a transposed convolution), as an implicit GEMM with k = tap Cout + co.
-__device__ __forceinline__ void convDxBody(const ConvParams& P, int bx, int by) {
+__global__ __launch_bounds__(256) void conv_dx_kernel(ConvParams P) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int Qn = P.inW * P.inH, K = 9 * P.Cout, KW = 9 * P.Cin;
     const int nPT = (Qn + 15) >> 4, nCT = (P.Cin + 15) >> 4;
-    const int t = bx * 4 + wave;
+    const int t = (int)blockIdx.x * 4 + wave;
     if (t >= nPT * nCT) return;
-    const int ct = t % nCT, pt = t / nCT, img = by;
+    const int ct = t % nCT, pt = t / nCT, img = (int)blockIdx.y;
     const int row = lane & 15, quad = lane >> 4;
     const int q = pt * 16 + row;
     const int ix = q % P.inW, iy = q / P.inW;
     const int ci = ct * 16 + row;
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        bf16x4 a, b;
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        float av[8], bv[8];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int k = k0 + 4 * quad + j;
-            float av = 0.0f, bv = 0.0f;
+        for (int j = 0; j < 8; j++) {
+            const int k = k0 + 8 * quad + j;
+            av[j] = bv[j] = 0.0f;
             if (k < K) {
                 const int tap = k / P.Cout, co = k - tap * P.Cout;
                 const int ky = tap / 3, kx = tap - 3 * ky;
                 const int sy = iy + 1 - ky, sx = ix + 1 - kx;
                 if (q < Qn && sx >= 0 && sy >= 0 && sx % P.stride == 0 && sy % P.stride == 0) {
                     const int ox = sx / P.stride, oy = sy / P.stride;
-                    if (ox < P.outW && oy < P.outH) av = gradAt(P, img, oy * P.outW + ox, co);
+                    if (ox < P.outW && oy < P.outH) av[j] = gradAt(P, img, oy * P.outW + ox, co);
                 }
-                if (ci < P.Cin) bv = P.w[co * KW + tap * P.Cin + ci];
+                if (ci < P.Cin) bv[j] = P.w[co * KW + tap * P.Cin + ci];
             }
-            a[j] = bf16Bits(av);
-            b[j] = bf16Bits(bv);
         }
-        acc = mfma16(a, b, acc);
+        acc = mfmaBf16(bf16Op(av), bf16Op(bv), acc);
     }
 #pragma unroll
     for (int j = 0; j < 4; j++) {
@@ -227,43 +188,41 @@ __device__ __forceinline__ void convDxBody(const ConvParams& P, int bx, int by) 
 
 // Gradient of the weights, one split of the pixel sum: part[split][co][k] = sum over the split's output pixels r of
 // dYm[r][co] A[r][k] (A the forward patch matrix). Wave tile: 16 channels x 16 k; the pixels of all images are one
-// reduction axis, cut into splits of kDwRows; the splits are added in order by convDwReduce.
-__device__ __forceinline__ void convDwBody(const ConvParams& P, int bx, int by) {
+// reduction axis, cut into splits of kDwRows; the splits are added in order by conv_dw_reduce_kernel.
+__global__ __launch_bounds__(256) void conv_dw_kernel(ConvParams P) {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int Pn = P.outW * P.outH, K = 9 * P.Cin, R = P.n * Pn;
     const int nCT = (P.Cout + 15) >> 4, nKT = (K + 15) >> 4;
-    const int t = bx * 4 + wave;
+    const int t = (int)blockIdx.x * 4 + wave;
     if (t >= nCT * nKT) return;
-    const int ct = t % nCT, kt = t / nCT, split = by;
+    const int ct = t % nCT, kt = t / nCT, split = (int)blockIdx.y;
     const int row = lane & 15, quad = lane >> 4;
     const int co = ct * 16 + row, k = kt * 16 + row;
-    int tap = 0, ci = 0, ky = 0, kx = 0;
+    int ci = 0, ky = 0, kx = 0;
     if (k < K) {
-        tap = k / P.Cin;
+        const int tap = k / P.Cin;
         ci = k - tap * P.Cin;
         ky = tap / 3;
         kx = tap - 3 * ky;
     }
     f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
     const int r0 = split * kDwRows, r1 = min(R, r0 + kDwRows);
-    for (int rb = r0; rb < r1; rb += 16) {
-        bf16x4 a, b;
+    for (int rb = r0; rb < r1; rb += 32) {
+        float av[8], bv[8];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int r = rb + 4 * quad + j;
-            float av = 0.0f, bv = 0.0f;
+        for (int j = 0; j < 8; j++) {
+            const int r = rb + 8 * quad + j;
+            av[j] = bv[j] = 0.0f;
             if (r < r1) {
                 const int img = r / Pn, p = r - img * Pn;
-                if (co < P.Cout) av = gradAt(P, img, p, co);
+                if (co < P.Cout) av[j] = gradAt(P, img, p, co);
                 if (k < K) {
                     const int ox = p % P.outW, oy = p / P.outW;
-                    bv = inputAt(P, img, ox * P.stride + kx - 1, oy * P.stride + ky - 1, ci);
+                    bv[j] = inputAt(P, img, ox * P.stride + kx - 1, oy * P.stride + ky - 1, ci);
                 }
             }
-            a[j] = bf16Bits(av);
-            b[j] = bf16Bits(bv);
         }
-        acc = mfma16(a, b, acc);
+        acc = mfmaBf16(bf16Op(av), bf16Op(bv), acc);
     }
     // D[row 4 quad + j][col row]: channel ct * 16 + 4 quad + j, k kt * 16 + row
     float* part = P.dwPart + (int64_t)split * P.Cout * K;
@@ -331,26 +290,318 @@ __global__ __launch_bounds__(256) void adam_kernel(float* w, const float* g, flo
     w[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
 }
 
-FGR_PASS(seed_conv, ConvParams, convFwdBody, 256)
+// BF16 copy of one layer's weights for the quantised forward: wb[co][k] = bf16(W[co][k]) for k < K, zero up to kp
+__global__ __launch_bounds__(256) void weights_bf16_kernel(const float* w, uint16_t* wb, int cout, int K, int kp) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= cout * kp) return;
+    const int co = i / kp, k = i - co * kp;
+    wb[i] = k < K ? (uint16_t)(bf16Pair(w[co * K + k], 0.0f) & 0xFFFFu) : (uint16_t)0;
+}
 
-__global__ __launch_bounds__(256) void conv_dx_kernel(ConvParams P) { convDxBody(P, (int)blockIdx.x, (int)blockIdx.y); }
-__global__ __launch_bounds__(256) void conv_dw_kernel(ConvParams P) { convDwBody(P, (int)blockIdx.x, (int)blockIdx.y); }
+// the FP8 conversion of the quantised forward on n floats (test / debug)
+__global__ __launch_bounds__(256) void fp8_convert_kernel(const float* in, uint8_t* out, int n) {
+    const int i = (int)(blockIdx.x * 256 + threadIdx.x) * 4;
+    if (i >= n) return;
+    float v[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) v[j] = i + j < n ? in[i + j] : 0.0f;
+    const uint32_t q = fp8Quad(v[0], v[1], v[2], v[3]);
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        if (i + j < n) out[i + j] = (uint8_t)(q >> (8 * j));
+}
+
+// ------------------------------------------------------------------------------ quantised activations (inference)
+//
+// Each layer reads its input activations as stored by the layer before (NHWC, BF16 or FP8 E4M3 quantised with this
+// layer's per-channel scale), and writes its output quantised once for the next layer; the last layer writes float32.
+// Layer 1 reads the luma pair and quantises the normalised values as it gathers them. With k = (ky 3 + kx) Cin + ci
+// and Cin a multiple of 8, the 8 k values a lane feeds one 32-deep MFMA step are 8 consecutive channels of one input
+// pixel: one 8-byte (FP8) or 16-byte (BF16) load. The weights ([Cout][kp], zero past 9 Cin) are staged in LDS once per
+// workgroup. A wavefront computes 16 output pixels x every output channel (Cout / 16 accumulators), the weights as the
+// MFMA's A operand so that a lane ends up holding 4 consecutive channels of one pixel: one packed store.
+
+template <bool F8> struct QOperand;
+template <> struct QOperand<true> { typedef uint64_t T; };   // 8 E4M3
+template <> struct QOperand<false> { typedef bf16x8 T; };    // 8 BF16
+
+// the input operand of the K step at k .. k + 7 (k a multiple of 8) for output pixel (ox, oy) of image img
+template <bool F8, int CIN, int STRIDE>
+__device__ __forceinline__ typename QOperand<F8>::T qInput(const ConvParams& P, int img, int ox, int oy, bool pv,
+                                                           int k) {
+    constexpr int K = 9 * CIN, EB = F8 ? 1 : 2;
+    typedef typename QOperand<F8>::T Op;
+    if constexpr (CIN >= 8) {
+        Op v = {};
+        const int tap = k / CIN, ci = k % CIN;
+        const int ky = tap / 3, kx = tap - 3 * ky;
+        const int ix = ox * STRIDE + kx - 1, iy = oy * STRIDE + ky - 1;
+        if (pv && k < K && ix >= 0 && iy >= 0 && ix < P.inW && iy < P.inH) {
+            const uint8_t* src = (const uint8_t*)P.qin + ((((int64_t)img * P.inH + iy) * P.inW + ix) * CIN + ci) * EB;
+            v = *(const Op*)src;
+        }
+        return v;
+    } else {
+        // the luma pair (CIN 2: k even, so value j is channel j & 1), normalised (v - 128) / 64
+        float x[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int kk = k + j, tap = kk >> 1;
+            const int ky = tap / 3, kx = tap - 3 * ky;
+            const int ix = ox * STRIDE + kx - 1, iy = oy * STRIDE + ky - 1;
+            x[j] = 0.0f;
+            if (pv && kk < K && ix >= 0 && iy >= 0 && ix < P.inW && iy < P.inH) {
+                const float* pl = ((j & 1) ? P.lc : P.la) + (int64_t)img * P.lumaStride;
+                x[j] = (pl[iy * P.inW + ix] - 128.0f) * (1.0f / 64.0f);
+            }
+        }
+        if constexpr (F8) {
+            const float q0 = P.qsIn[0], q1 = P.qsIn[1];
+            const uint32_t lo = fp8Quad(x[0] * q0, x[1] * q1, x[2] * q0, x[3] * q1);
+            const uint32_t hi = fp8Quad(x[4] * q0, x[5] * q1, x[6] * q0, x[7] * q1);
+            return (uint64_t)lo | ((uint64_t)hi << 32);
+        } else {
+            return bf16Op(x);
+        }
+    }
+}
+
+// the layer's weights ([COUT][kp] elements, kp * element bytes = RB bytes a row) into LDS rows RS bytes apart, rows
+// from COUT to ROWS zero
+template <int COUT, int ROWS, int RB, int RS>
+__device__ __forceinline__ void stageWeights(const ConvParams& P, uint8_t* wl) {
+    constexpr int CH = RB / 16, N = ROWS * CH, IT = (N + 255) / 256;   // 16-byte chunks a row, in all, a thread
+    const uint8_t* wg = (const uint8_t*)P.wq;
+    u32x4 v[IT];   // every load in flight before the first LDS store
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int i = it * 256 + (int)threadIdx.x, r = i / CH, c = i - r * CH;
+        v[it] = u32x4{0u, 0u, 0u, 0u};
+        if (i < N && r < COUT) v[it] = *(const u32x4*)(wg + (size_t)r * RB + 16 * c);
+    }
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int i = it * 256 + (int)threadIdx.x, r = i / CH, c = i - r * CH;
+        if (i < N) *(u32x4*)(wl + r * RS + 16 * c) = v[it];
+    }
+}
+
+// The first layer (luma pair -> 16 channels, stride 2, 18-deep K in one 32-deep step). A workgroup owns a block of
+// kQ0W x kQ0H output pixels: it stages the block's input window of both planes in LDS with coalesced loads,
+// normalised and quantised once per value, then every wavefront gathers its operands from LDS for 4 tiles of 16
+// pixels (one output row segment each).
+constexpr int kQ0W = 32, kQ0H = 8;
+constexpr int kQ0InW = 2 * kQ0W + 1, kQ0InH = 2 * kQ0H + 1, kQ0Pitch = 68;
+
+int convQ0Blocks(int outW, int outH) { return ((outW + kQ0W - 1) / kQ0W) * ((outH + kQ0H - 1) / kQ0H); }
+
+template <bool F8>
+__device__ __forceinline__ void convQ0(const ConvParams& P, int bx, int img) {
+    typedef typename std::conditional<F8, uint8_t, uint16_t>::type E;
+    typedef typename QOperand<F8>::T Op;
+    constexpr int EB = F8 ? 1 : 2, RS = 32 * EB + 16;
+    __shared__ __align__(16) uint8_t wl[16 * RS];
+    __shared__ E win[2][kQ0InH][kQ0Pitch];
+    const int nbx = (P.outW + kQ0W - 1) / kQ0W;
+    const int ox0 = (bx % nbx) * kQ0W, oy0 = (bx / nbx) * kQ0H;
+    const int ix0 = 2 * ox0 - 1, iy0 = 2 * oy0 - 1;
+    stageWeights<16, 16, 32 * EB, RS>(P, wl);
+    const float qs0 = F8 ? P.qsIn[0] : 1.0f, qs1 = F8 ? P.qsIn[1] : 1.0f;
+    constexpr int NV = 2 * kQ0InH * kQ0InW, IT = (NV + 255) / 256;
+    float lum[IT];   // every load in flight before the first conversion; 128 (normalised 0) outside the image
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int i = it * 256 + (int)threadIdx.x;
+        const int c = i / (kQ0InH * kQ0InW), r = (i / kQ0InW) % kQ0InH, x = i % kQ0InW;
+        const int ix = ix0 + x, iy = iy0 + r;
+        lum[it] = 128.0f;
+        if (i < NV && ix >= 0 && iy >= 0 && ix < P.inW && iy < P.inH)
+            lum[it] = (c ? P.lc : P.la)[(int64_t)img * P.lumaStride + (int64_t)iy * P.inW + ix];
+    }
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int i = it * 256 + (int)threadIdx.x;
+        if (i >= NV) break;
+        const int c = i / (kQ0InH * kQ0InW), r = (i / kQ0InW) % kQ0InH, x = i % kQ0InW;
+        const float v = (lum[it] - 128.0f) * (1.0f / 64.0f);
+        if constexpr (F8) win[c][r][x] = (uint8_t)fp8Quad(v * (c ? qs1 : qs0), 0.0f, 0.0f, 0.0f);
+        else win[c][r][x] = (uint16_t)bf16Pair(v, 0.0f);
+    }
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, row = lane & 15, quad = lane >> 4;
+    // the lane's 8 operand values k = 8 quad + j (tap k >> 1 of channel k & 1; zero from k = 18): window offsets
+    // relative to the top left input pixel of its output pixel
+    int off[8];
+#pragma unroll
+    for (int j = 0; j < 8; j++) {
+        const int k = 8 * quad + j, tap = k >> 1, ky = tap / 3, kx = tap - 3 * ky;
+        off[j] = k < 18 ? ((k & 1) * kQ0InH + ky) * kQ0Pitch + kx : -1;
+    }
+    const int Pn = P.outW * P.outH, c0 = 4 * quad;
+    // the epilogue's per-channel constants, read once (the output stores could alias them)
+    float bias[4], sw[4], qs[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        bias[r] = P.b[c0 + r];
+        sw[r] = F8 ? P.sw[c0 + r] : 1.0f;
+        qs[r] = F8 ? P.qsOut[c0 + r] : 1.0f;
+    }
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int yl = 2 * wave + (t >> 1), xl = 16 * (t & 1) + row;
+        const E* base = &win[0][0][0] + (2 * yl) * kQ0Pitch + 2 * xl;
+        uint32_t e[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) e[j] = off[j] >= 0 ? (uint32_t)base[off[j]] : 0u;
+        Op in;
+        if constexpr (F8) {
+            in = (uint64_t)(e[0] | e[1] << 8 | e[2] << 16 | e[3] << 24) |
+                 (uint64_t)(e[4] | e[5] << 8 | e[6] << 16 | e[7] << 24) << 32;
+        } else {
+            const u32x4 u = {e[0] | e[1] << 16, e[2] | e[3] << 16, e[4] | e[5] << 16, e[6] | e[7] << 16};
+            in = __builtin_bit_cast(bf16x8, u);
+        }
+        const Op w = *(const Op*)(wl + row * RS + 8 * quad * EB);
+        f32x4 acc = {0.0f, 0.0f, 0.0f, 0.0f};
+        if constexpr (F8) acc = mfmaFp8(w, in, acc);
+        else acc = mfmaBf16(w, in, acc);
+        const int ox = ox0 + xl, oy = oy0 + yl;
+        if (ox >= P.outW || oy >= P.outH) continue;
+        // D[row 4 quad + r][col row]: channel 4 quad + r of this pixel
+        const int64_t pix = (int64_t)img * Pn + (int64_t)oy * P.outW + ox;
+        float v[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) v[r] = fmaxf(acc[r] * sw[r] + bias[r], 0.0f);
+        if constexpr (F8) {
+            *(uint32_t*)((uint8_t*)P.qout + pix * 16 + c0) = fp8Quad(v[0] * qs[0], v[1] * qs[1], v[2] * qs[2], v[3] * qs[3]);
+        } else {
+            const u32x2 u = {bf16Pair(v[0], v[1]), bf16Pair(v[2], v[3])};
+            *(u32x2*)((uint8_t*)P.qout + (pix * 16 + c0) * 2) = u;
+        }
+    }
+}
+
+template <bool F8, int CIN, int COUT, int STRIDE, int TPW>
+__device__ __forceinline__ void convQ(const ConvParams& P, int bx, int img) {
+    constexpr int EB = F8 ? 1 : 2;
+    constexpr int K = 9 * CIN, KS = (K + 31) / 32, KP = 32 * KS, NCT = (COUT + 15) / 16;
+    constexpr int RS = KP * EB + 16;   // LDS bytes a weight row (16 more than the row, against bank conflicts)
+    constexpr bool LAST = COUT < 16;   // the proposals: float32 out, no ReLU
+    typedef typename QOperand<F8>::T Op;
+    __shared__ __align__(16) uint8_t wl[NCT * 16 * RS];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, row = lane & 15, quad = lane >> 4;
+    const int Pn = P.outW * P.outH, nPT = (Pn + 15) >> 4;
+    Op in[KS];
+    const auto loadTile = [&](int pt) {
+        const int p = pt * 16 + row;
+        const bool pv = p < Pn;
+        const int ox = pv ? p % P.outW : 0, oy = pv ? p / P.outW : 0;
+#pragma unroll
+        for (int s = 0; s < KS; s++) in[s] = qInput<F8, CIN, STRIDE>(P, img, ox, oy, pv, 32 * s + 8 * quad);
+    };
+    loadTile(bx * TPW * 4 + wave);   // the first tile's loads in flight with the weight staging
+    stageWeights<COUT, NCT * 16, KP * EB, RS>(P, wl);
+    __syncthreads();
+    for (int i = 0; i < TPW; i++) {
+        const int pt = (bx * TPW + i) * 4 + wave;
+        if (pt >= nPT) break;
+        const int p = pt * 16 + row;
+        const bool pv = p < Pn;
+        if (i > 0) loadTile(pt);
+        f32x4 acc[NCT];
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++) acc[ct] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = 0; s < KS; s++) {
+#pragma unroll
+            for (int ct = 0; ct < NCT; ct++) {
+                const Op w = *(const Op*)(wl + (ct * 16 + row) * RS + (32 * s + 8 * quad) * EB);
+                if constexpr (F8) acc[ct] = mfmaFp8(w, in[s], acc[ct]);
+                else acc[ct] = mfmaBf16(w, in[s], acc[ct]);
+            }
+        }
+        if (!pv) continue;
+        // D[row 4 quad + r][col row]: channel ct * 16 + 4 quad + r of pixel p
+        const int64_t pix = (int64_t)img * Pn + p;
+#pragma unroll
+        for (int ct = 0; ct < NCT; ct++) {
+            const int c0 = ct * 16 + 4 * quad;
+            if constexpr (LAST) {
+                if (quad == 0) {
+#pragma unroll
+                    for (int r = 0; r < COUT; r++) P.out[pix * COUT + r] = acc[ct][r] * (F8 ? P.sw[r] : 1.0f) + P.b[r];
+                }
+            } else {
+                float v[4];
+#pragma unroll
+                for (int r = 0; r < 4; r++) v[r] = fmaxf(acc[ct][r] * (F8 ? P.sw[c0 + r] : 1.0f) + P.b[c0 + r], 0.0f);
+                if constexpr (F8) {
+                    *(uint32_t*)((uint8_t*)P.qout + pix * COUT + c0) =
+                        fp8Quad(v[0] * P.qsOut[c0], v[1] * P.qsOut[c0 + 1], v[2] * P.qsOut[c0 + 2], v[3] * P.qsOut[c0 + 3]);
+                } else {
+                    const u32x2 u = {bf16Pair(v[0], v[1]), bf16Pair(v[2], v[3])};
+                    *(u32x2*)((uint8_t*)P.qout + (pix * COUT + c0) * 2) = u;
+                }
+            }
+        }
+    }
+}
+
+// the ten quantised layer passes: kPassSeedQ + 2 layer + (FP8 ? 1 : 0)
+#define FGR_SEED_Q(name, F8, L)                                                                                     \
+    __device__ __forceinline__ void name##Body(const ConvParams& P, int bx, int by) {                               \
+        convQ<F8, kNetCin[L], kNetCout[L], kNetStride[L], kSeedTpw[L]>(P, bx, by);                                  \
+    }                                                                                                                \
+    FGR_PASS(name, ConvParams, name##Body, 256)
+
+__device__ __forceinline__ void seedQ0Bf16Body(const ConvParams& P, int bx, int by) { convQ0<false>(P, bx, by); }
+__device__ __forceinline__ void seedQ0Fp8Body(const ConvParams& P, int bx, int by) { convQ0<true>(P, bx, by); }
+FGR_PASS(seed_q0_bf16, ConvParams, seedQ0Bf16Body, 256)
+FGR_PASS(seed_q0_fp8, ConvParams, seedQ0Fp8Body, 256)
+FGR_SEED_Q(seed_q1_bf16, false, 1)
+FGR_SEED_Q(seed_q1_fp8, true, 1)
+FGR_SEED_Q(seed_q2_bf16, false, 2)
+FGR_SEED_Q(seed_q2_fp8, true, 2)
+FGR_SEED_Q(seed_q3_bf16, false, 3)
+FGR_SEED_Q(seed_q3_fp8, true, 3)
+FGR_SEED_Q(seed_q4_bf16, false, 4)
+FGR_SEED_Q(seed_q4_fp8, true, 4)
+
+#define FGR_SEED_CASES(launch)                                                                                      \
+    launch(0, seed_q0_bf16) launch(1, seed_q0_fp8) launch(2, seed_q1_bf16) launch(3, seed_q1_fp8)                 \
+    launch(4, seed_q2_bf16) launch(5, seed_q2_fp8) launch(6, seed_q3_bf16) launch(7, seed_q3_fp8)                 \
+    launch(8, seed_q4_bf16) launch(9, seed_q4_fp8)
 
 hipError_t launchSeedSingle(int pass, const void* params, int gx, int gy, hipStream_t s) {
-    if (pass != kPassSeedConv) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(seed_conv_kernel, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, s, *(const ConvParams*)params);
+    const dim3 g((unsigned)gx, (unsigned)gy);
+    const ConvParams& P = *(const ConvParams*)params;
+#define FGR_SEED_SINGLE(i, name)                                                                                    \
+    case i: hipLaunchKernelGGL(name##_kernel, g, dim3(256), 0, s, P); break;
+    switch (pass - kPassSeedQ) {
+        FGR_SEED_CASES(FGR_SEED_SINGLE)
+    default: return hipErrorInvalidValue;
+    }
+#undef FGR_SEED_SINGLE
     return hipGetLastError();
 }
 
 hipError_t launchSeedBatch(int pass, const void* params, const uint32_t* work, int n, hipStream_t s) {
-    if (pass != kPassSeedConv) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(seed_conv_batch_kernel, dim3((unsigned)n), dim3(256), 0, s, (const ConvParams*)params, work);
+    const dim3 g((unsigned)n);
+    const ConvParams* P = (const ConvParams*)params;
+#define FGR_SEED_BATCH(i, name)                                                                                     \
+    case i: hipLaunchKernelGGL(name##_batch_kernel, g, dim3(256), 0, s, P, work); break;
+    switch (pass - kPassSeedQ) {
+        FGR_SEED_CASES(FGR_SEED_BATCH)
+    default: return hipErrorInvalidValue;
+    }
+#undef FGR_SEED_BATCH
     return hipGetLastError();
 }
 
-int convFwdTiles(const ConvParams& p) {
-    const int nPT = (p.outW * p.outH + 15) / 16, nCT = (p.Cout + 15) / 16;
-    return (nPT * nCT + 3) / 4;
+int convQTiles(int layer, const ConvParams& p) {
+    if (layer == 0) return convQ0Blocks(p.outW, p.outH);
+    const int nPT = (p.outW * p.outH + 15) / 16, per = 4 * kSeedTpw[layer];
+    return (nPT + per - 1) / per;
 }
 
 }  // namespace fgr
@@ -359,9 +610,9 @@ using namespace fgr;
 
 // ------------------------------------------------------------------------------------------------------ host side
 
-static const int kCin[kNetLayers] = {2, 16, 32, 64, 64};
-static const int kCout[kNetLayers] = {16, 32, 64, 64, 2};
-static const int kStride[kNetLayers] = {2, 2, 2, 1, 1};
+static const int* const kCin = kNetCin;
+static const int* const kCout = kNetCout;
+static const int* const kStride = kNetStride;
 
 static int layerParams(int l) { return kCout[l] * 9 * kCin[l] + kCout[l]; }
 static int paramCount() {
@@ -384,6 +635,30 @@ void netLayerSizes(int w, int h, int* lw, int* lh) {
     }
 }
 
+// byte offsets of the quantised activations of layers 1-4 for n images of a w0 x h0 pair (room for BF16, each layer
+// 256-byte aligned); returns the total
+static size_t qActLayout(int w0, int h0, int n, size_t* off) {
+    int lw[kNetLayers + 1], lh[kNetLayers + 1];
+    netLayerSizes(w0, h0, lw, lh);
+    size_t o = 0;
+    for (int l = 0; l < kNetLayers - 1; l++) {
+        if (off) off[l] = o;
+        o += ((size_t)n * lw[l + 1] * lh[l + 1] * kCout[l] * 2 + 255) & ~(size_t)255;
+    }
+    return o;
+}
+
+static int nInChannels() {
+    int c = 0;
+    for (int l = 0; l < kNetLayers; l++) c += kCin[l];
+    return c;
+}
+static int nOutChannels() {
+    int c = 0;
+    for (int l = 0; l < kNetLayers; l++) c += kCout[l];
+    return c;
+}
+
 struct FgrSeedNet {
     int device = 0;
     float* w = nullptr;      // parameters (paramCount floats)
@@ -392,14 +667,17 @@ struct FgrSeedNet {
     float* v = nullptr;
     int steps = 0;
     // training buffers, grown on demand
-    float* act = nullptr;    // activations of layers 1..5 (NHWC)
+    float* act = nullptr;    // activations of layers 1..5 (NHWC); the quantised forward's activations use it too
     float* grad = nullptr;   // their gradients
     float* part = nullptr;   // weight-gradient partial sums
     float* loss = nullptr;   // one float
     size_t actCap = 0, partCap = 0;
-    // FP8 inference: per layer E4M3 weights (Cout x kp bytes), input scales sa (Cin) and output scales sw (Cout)
+    // BF16 inference: per layer the weights as bf16 (Cout x kp), refreshed whenever the weights change
+    uint16_t* wb = nullptr;
+    // FP8 inference: per layer E4M3 weights (Cout x kp bytes); scales: the input scales sa (every layer's Cin), the
+    // weight scales sw (every layer's Cout), then qs = 1 / sa (every layer's Cin)
     uint8_t* wq = nullptr;
-    float* sa = nullptr;     // all layers' sa, then all layers' sw
+    float* sa = nullptr;
     bool fp8Ready = false, fp8On = false;
 
     ~FgrSeedNet() {
@@ -407,6 +685,7 @@ struct FgrSeedNet {
         for (float* p : {w, g, m, v, act, grad, part, loss, sa})
             if (p) hipFree(p);
         if (wq) hipFree(wq);
+        if (wb) hipFree(wb);
     }
 
     bool grow(int w0, int h0, int n) {
@@ -418,6 +697,7 @@ struct FgrSeedNet {
             const size_t R = (size_t)n * lw[l + 1] * lh[l + 1];
             partNeed = std::max(partNeed, ((R + kDwRows - 1) / kDwRows) * kCout[l] * 9 * kCin[l]);
         }
+        need = std::max(need, (qActLayout(w0, h0, n, nullptr) + 3) / 4);
         if (need > actCap) {
             if (act) hipFree(act);
             if (grad) hipFree(grad);
@@ -436,9 +716,10 @@ struct FgrSeedNet {
         return true;
     }
 
-    // the ConvParams of layer l over n images of a w0 x h0 luma pair, activations at act (layer outputs in order)
+    // the ConvParams of layer l over n images of a w0 x h0 luma pair on float32 activations (training, calibration),
+    // activations at acts (layer outputs in order)
     ConvParams layer(int l, const float* la, const float* lc, int w0, int h0, int n, float* acts, float* grads,
-                     const float* weights, bool fp8 = false) const {
+                     const float* weights) const {
         int lw[kNetLayers + 1], lh[kNetLayers + 1];
         netLayerSizes(w0, h0, lw, lh);
         size_t off[kNetLayers + 1];
@@ -466,41 +747,89 @@ struct FgrSeedNet {
             p.dy = grads + off[l];
             p.dx = l == 0 ? nullptr : grads + off[l - 1];
         }
-        if (fp8 && fp8On && fp8Ready) {
-            p.fp8 = 1;
-            p.wq = wq + wqOffset(l);
-            p.kp = kpOf(l);
-            p.sa = sa + chanOffset(l, true);
-            p.sw = sa + chanOffset(l, false);
-        }
-        p.gx = convFwdTiles(p);
+        p.gx = (((p.outW * p.outH + 15) / 16) * ((p.Cout + 15) / 16) + 3) / 4;
         return p;
     }
 
-    static int kpOf(int l) { return (9 * kCinL(l) + 31) / 32 * 32; }
-    static int kCinL(int l);
-    static int kCoutL(int l);
-    static size_t wqOffset(int l) {
+    bool fp8Active() const { return fp8On && fp8Ready; }
+
+    // the pass of the quantised forward's layer l (BF16, or FP8 when it is on)
+    int qPass(int l) const { return kPassSeedQ + 2 * l + (fp8Active() ? 1 : 0); }
+
+    // the ConvParams of layer l of the quantised forward over n images: quantised activations at qact (qActLayout),
+    // the last layer's float32 output at out
+    ConvParams qlayer(int l, const float* la, const float* lc, int w0, int h0, int n, uint8_t* qact, float* out) const {
+        int lw[kNetLayers + 1], lh[kNetLayers + 1];
+        netLayerSizes(w0, h0, lw, lh);
+        size_t off[kNetLayers];
+        qActLayout(w0, h0, n, off);
+        const bool f8 = fp8Active();
+        ConvParams p{};
+        p.n = n;
+        p.la = la;
+        p.lc = lc;
+        p.lumaStride = (int64_t)w0 * h0;
+        p.Cin = kCin[l];
+        p.inW = lw[l];
+        p.inH = lh[l];
+        p.Cout = kCout[l];
+        p.outW = lw[l + 1];
+        p.outH = lh[l + 1];
+        p.b = w + layerOffset(l) + kCout[l] * 9 * kCin[l];
+        p.stride = kStride[l];
+        p.relu = l < kNetLayers - 1 ? 1 : 0;
+        p.qin = l == 0 ? nullptr : qact + off[l - 1];
+        p.qout = l == kNetLayers - 1 ? nullptr : qact + off[l];
+        p.out = l == kNetLayers - 1 ? out : nullptr;
+        p.kp = kpOf(l);
+        if (f8) {
+            p.wq = wq + wqOffset(l);
+            p.sw = sa + chanOffset(l, false);
+            p.qsIn = sa + qsOffset(0);
+            p.qsOut = l == kNetLayers - 1 ? nullptr : sa + qsOffset(l + 1);
+        } else {
+            p.wq = wb + wqOffset(l);
+        }
+        p.gx = convQTiles(l, p);
+        return p;
+    }
+
+    static int kpOf(int l) { return (9 * kCin[l] + 31) / 32 * 32; }
+    static size_t wqOffset(int l) {   // elements
         size_t o = 0;
-        for (int i = 0; i < l; i++) o += (size_t)kCoutL(i) * kpOf(i);
+        for (int i = 0; i < l; i++) o += (size_t)kCout[i] * kpOf(i);
         return o;
     }
     static int chanOffset(int l, bool in) {   // sa: the Cin of every layer, then sw: the Cout of every layer
-        int o = 0, total = 0;
-        for (int i = 0; i < kNetLayers; i++) total += kCinL(i);
+        int o = 0;
         if (in) {
-            for (int i = 0; i < l; i++) o += kCinL(i);
+            for (int i = 0; i < l; i++) o += kCin[i];
             return o;
         }
-        for (int i = 0; i < l; i++) o += kCoutL(i);
-        return total + o;
+        for (int i = 0; i < l; i++) o += kCout[i];
+        return nInChannels() + o;
+    }
+    static int qsOffset(int l) { return nInChannels() + nOutChannels() + chanOffset(l, true); }
+
+    // the BF16 weights from the float32 ones (after every change of the weights), on stream s
+    bool refreshBf16(hipStream_t s) const {
+        for (int l = 0; l < kNetLayers; l++) {
+            const int kp = kpOf(l), n = kCout[l] * kp;
+            hipLaunchKernelGGL(weights_bf16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s,
+                               w + layerOffset(l), wb + wqOffset(l), kCout[l], 9 * kCin[l], kp);
+        }
+        return hipGetLastError() == hipSuccess;
     }
 };
 
-int FgrSeedNet::kCinL(int l) { return kCin[l]; }
-int FgrSeedNet::kCoutL(int l) { return kCout[l]; }
+// the smallest power of two at or above v (v > 0, finite)
+static float pow2AtLeast(float v) {
+    int e;
+    const float m = std::frexp(v, &e);   // v = m 2^e, m in [0.5, 1)
+    return m == 0.5f ? std::ldexp(1.0f, e - 1) : std::ldexp(1.0f, e);
+}
 
-// OCP FP8 E4M3 on the host: the same steps as fp8e4m3 on the device
+// OCP FP8 E4M3 (bias 7, largest finite 448), round to nearest even, saturating: the weights' quantisation on the host
 static uint8_t fp8e4m3Host(float v) {
     const uint32_t s = v < 0.0f ? 0x80u : 0u;
     const float a = std::fabs(v);
@@ -519,7 +848,7 @@ static uint8_t fp8e4m3Host(float v) {
 }
 
 static bool launchFwd(const ConvParams& p, hipStream_t s) {
-    hipLaunchKernelGGL(seed_conv_kernel, dim3((unsigned)p.gx, (unsigned)p.n), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(conv_fwd_kernel, dim3((unsigned)p.gx, (unsigned)p.n), dim3(256), 0, s, p);
     return hipGetLastError() == hipSuccess;
 }
 
@@ -536,8 +865,7 @@ extern "C" FgrSeedNet* fgr_seednet_create(int32_t device, uint64_t seed) {
             delete net;
             return nullptr;
         }
-    if (hipMalloc(&net->loss, 4) != hipSuccess) {
-        net->loss = nullptr;
+    if (hipMalloc(&net->loss, 4) != hipSuccess || hipMalloc(&net->wb, FgrSeedNet::wqOffset(kNetLayers) * 2) != hipSuccess) {
         delete net;
         return nullptr;
     }
@@ -554,6 +882,10 @@ extern "C" FgrSeedNet* fgr_seednet_create(int32_t device, uint64_t seed) {
     hipMemset(net->m, 0, (size_t)n * 4);
     hipMemset(net->v, 0, (size_t)n * 4);
     hipMemset(net->g, 0, (size_t)n * 4);
+    if (!net->refreshBf16(nullptr) || hipStreamSynchronize(nullptr) != hipSuccess) {
+        delete net;
+        return nullptr;
+    }
     return net;
 }
 
@@ -571,7 +903,8 @@ extern "C" int fgr_seednet_set_weights(FgrSeedNet* net, const float* host, uint3
     net->steps = 0;
     hipMemset(net->m, 0, (size_t)n * 4);
     hipMemset(net->v, 0, (size_t)n * 4);
-    return hipMemcpy(net->w, host, (size_t)n * 4, hipMemcpyHostToDevice) == hipSuccess;
+    if (hipMemcpy(net->w, host, (size_t)n * 4, hipMemcpyHostToDevice) != hipSuccess) return 0;
+    return net->refreshBf16(nullptr) && hipStreamSynchronize(nullptr) == hipSuccess;
 }
 
 extern "C" int fgr_seednet_forward(FgrSeedNet* net, void* stream, const float* la, const float* lc, int32_t w,
@@ -580,27 +913,23 @@ extern "C" int fgr_seednet_forward(FgrSeedNet* net, void* stream, const float* l
     hipStream_t s = (hipStream_t)stream;
     if (!net->grow(w, h, n)) return 0;
     for (int l = 0; l < kNetLayers; l++) {
-        ConvParams p = net->layer(l, la, lc, w, h, n, net->act, nullptr, net->w, true);
-        if (l == kNetLayers - 1) p.out = out;
-        if (!launchFwd(p, s)) return 0;
+        const ConvParams p = net->qlayer(l, la, lc, w, h, n, (uint8_t*)net->act, out);
+        if (launchSeedSingle(net->qPass(l), &p, p.gx, n, s) != hipSuccess) return 0;
     }
     return 1;
 }
 
-// FP8 calibration: a BF16 forward over the batch, then per layer the largest |input| of each channel c, the scale
-// sa[c] = max / 448 (the largest value maps onto E4M3's largest); the weights with sa folded in, W'[co][k] = W[co][k]
-// sa[ci], get per output channel sw[co] = max_k |W'| / 448 and are stored as e4m3(W' / sw).
+// FP8 calibration: a float32 forward over the batch, then per layer the largest |input| of each channel c, the scale
+// sa[c] = the smallest power of two at or above max / 448 (the largest value maps at most onto E4M3's largest, and
+// x / sa is exact); the weights, with sa folded in, W'[co][k] = W[co][k] sa[ci], get per output channel
+// sw[co] = max_k |W'| / 448 and are stored as e4m3(W' / sw).
 extern "C" int fgr_seednet_calibrate_fp8(FgrSeedNet* net, void* stream, const float* la, const float* lc, int32_t w,
                                          int32_t h, int32_t n) {
     if (!net || !la || !lc || w < 8 || h < 8 || n < 1) return 0;
     hipStream_t s = (hipStream_t)stream;
     if (!net->grow(w, h, n)) return 0;
-    int nIn = 0, nOut = 0;
-    for (int l = 0; l < kNetLayers; l++) {
-        nIn += kCin[l];
-        nOut += kCout[l];
-    }
-    if (!net->sa && hipMalloc(&net->sa, (size_t)(nIn + nOut) * 4) != hipSuccess) {
+    const int nIn = nInChannels(), nOut = nOutChannels();
+    if (!net->sa && hipMalloc(&net->sa, (size_t)(2 * nIn + nOut) * 4) != hipSuccess) {
         net->sa = nullptr;
         return 0;
     }
@@ -615,12 +944,15 @@ extern "C" int fgr_seednet_calibrate_fp8(FgrSeedNet* net, void* stream, const fl
         hipLaunchKernelGGL(conv_absmax_kernel, dim3((unsigned)p.Cin), dim3(256), 0, s, p,
                            net->sa + FgrSeedNet::chanOffset(l, true));
     }
-    std::vector<float> sa(nIn + nOut), wts(paramCount());
+    std::vector<float> sa(2 * nIn + nOut), wts(paramCount());
     if (hipMemcpyAsync(sa.data(), net->sa, (size_t)nIn * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipMemcpyAsync(wts.data(), net->w, wts.size() * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
         hipStreamSynchronize(s) != hipSuccess)
         return 0;
-    for (int i = 0; i < nIn; i++) sa[i] = sa[i] > 0.0f ? sa[i] / 448.0f : 1.0f;
+    for (int i = 0; i < nIn; i++) {
+        sa[i] = sa[i] > 0.0f ? pow2AtLeast(sa[i] / 448.0f) : 1.0f;
+        sa[nIn + nOut + i] = 1.0f / sa[i];
+    }
     std::vector<uint8_t> wq(wqBytes, 0);
     for (int l = 0; l < kNetLayers; l++) {
         const int K = 9 * kCin[l], kp = FgrSeedNet::kpOf(l);
@@ -655,15 +987,18 @@ extern "C" int fgr_seednet_set_fp8(FgrSeedNet* net, int on) {
 
 extern "C" int fgr_seednet_fp8_scales(const FgrSeedNet* net, float* sa, float* sw) {
     if (!net || !net->fp8Ready) return 0;
-    int nIn = 0, nOut = 0;
-    for (int l = 0; l < kNetLayers; l++) {
-        nIn += kCin[l];
-        nOut += kCout[l];
-    }
+    const int nIn = nInChannels(), nOut = nOutChannels();
     hipSetDevice(net->device);
     if (sa && hipMemcpy(sa, net->sa, (size_t)nIn * 4, hipMemcpyDeviceToHost) != hipSuccess) return 0;
     if (sw && hipMemcpy(sw, net->sa + nIn, (size_t)nOut * 4, hipMemcpyDeviceToHost) != hipSuccess) return 0;
     return 1;
+}
+
+extern "C" int fgr_debug_fp8_e4m3(void* stream, const float* in, uint8_t* out, uint32_t n) {
+    if (!in || !out || n == 0 || n > (1u << 30)) return 0;
+    const unsigned groups = (n + 3) / 4;
+    hipLaunchKernelGGL(fp8_convert_kernel, dim3((groups + 255) / 256), dim3(256), 0, (hipStream_t)stream, in, out, (int)n);
+    return hipGetLastError() == hipSuccess;
 }
 
 // forward + loss + backward into net->g; the loss into net->loss (device)
@@ -715,6 +1050,7 @@ extern "C" int fgr_seednet_train_step(FgrSeedNet* net, void* stream, const float
     const int np = paramCount();
     hipLaunchKernelGGL(adam_kernel, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, net->w, net->g, net->m, net->v,
                        np, lr, c1, c2);
+    if (!net->refreshBf16(s)) return 0;
     if (loss && hipMemcpyAsync(loss, net->loss, 4, hipMemcpyDeviceToDevice, s) != hipSuccess) return 0;
     return hipGetLastError() == hipSuccess;
 }
@@ -767,12 +1103,8 @@ bool FgrInterpolator::allocSeedBuffers() {
     seedAct = nullptr;
     if (!net || lv.empty()) return true;
     const Level& c = lv.back();
-    int lw[kNetLayers + 1], lh[kNetLayers + 1];
-    netLayerSizes(c.w, c.h, lw, lh);
-    size_t need = 0;
-    for (int l = 0; l < kNetLayers - 1; l++) need += (size_t)lw[l + 1] * lh[l + 1] * kCout[l];
     hipSetDevice(device);
-    return hipMalloc(&seedAct, need * 4) == hipSuccess;
+    return hipMalloc(&seedAct, qActLayout(c.w, c.h, 1, nullptr)) == hipSuccess;
 }
 
 // the CNN's launches on the coarsest level, before its search (stages 190-194); the proposals into fieldSeed
@@ -780,10 +1112,10 @@ void FgrInterpolator::planSeeds(std::vector<PlanOp>& plan) const {
     const Level& c = lv.back();
     float2* F = (float2*)field;
     for (int l = 0; l < kNetLayers; l++) {
-        ConvParams p = net->layer(l, luma + c.offA, luma + c.offC, c.w, c.h, 1, seedAct, nullptr, net->w, true);
-        if (l == kNetLayers - 1) p.out = (float*)(F + fieldSeed);
+        const ConvParams p = net->qlayer(l, luma + c.offA, luma + c.offC, c.w, c.h, 1, (uint8_t*)seedAct,
+                                         (float*)(F + fieldSeed));
         PlanOp op;
-        op.pass = kPassSeedConv;
+        op.pass = net->qPass(l);
         op.stage = 190 + l;
         op.gx = p.gx;
         op.gy = 1;

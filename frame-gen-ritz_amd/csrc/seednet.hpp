@@ -5,6 +5,11 @@
 namespace fgr {
 
 constexpr int kNetLayers = 5;
+constexpr int kNetCin[kNetLayers] = {2, 16, 32, 64, 64};
+constexpr int kNetCout[kNetLayers] = {16, 32, 64, 64, 2};
+constexpr int kNetStride[kNetLayers] = {2, 2, 2, 1, 1};
+// quantised forward: 16-pixel tiles per wavefront of layers 2-5 (layer 1 has a kernel of its own)
+constexpr int kSeedTpw[kNetLayers] = {0, 2, 1, 1, 1};
 constexpr int kDwRows = 1024;   // output pixels per split of the weight-gradient sum
 
 struct ConvParams {
@@ -22,14 +27,19 @@ struct ConvParams {
     float* dx;                    // gradient of its input
     float* dw; float* db;         // gradients of W and b
     float* dwPart;                // per-split partial sums of dW
-    // FP8 inference (E4M3, per-channel scales): input channel ci is quantised as e4m3(x / sa[ci]); the weights, with
-    // sa folded in, as wq[co][k] = e4m3(W[co][k] sa[ci] / sw[co]) (kp bytes a row); out = sw[co] sum + b[co]
-    int fp8;
-    const uint8_t* wq; int kp;
-    const float* sa; const float* sw;
+    // quantised inference forward (passes kPassSeedQ + 2 layer + FP8): activations NHWC, stored once per layer as
+    // BF16, or as OCP FP8 E4M3 with input channel ci quantised as e4m3(x / sa[ci]), sa a power of two (so x / sa is
+    // the exact product x qs, qs = 1 / sa); FP8 weights with sa folded in, wq[co][k] = e4m3(W[co][k] sa[ci] / sw[co]),
+    // and out = sw[co] sum + b[co]
+    const void* qin;              // the layer's input (layers 2-5)
+    void* qout;                   // its output (layers 1-4; layer 5 writes float32 `out`)
+    const void* wq; int kp;       // weights [Cout][kp] (BF16 W, or FP8 as above), zero from 9 Cin to kp
+    const float* sw;              // FP8: per output channel weight scale
+    const float* qsIn;            // FP8 layer 1: qs of the two luma channels
+    const float* qsOut;           // FP8 layers 1-4: qs of the next layer's input channels
 };
 
-int convFwdTiles(const ConvParams& p);
+int convQTiles(int layer, const ConvParams& p);
 
 }  // namespace fgr
 

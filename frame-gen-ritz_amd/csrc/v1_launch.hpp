@@ -10,7 +10,6 @@ enum Pass : int {
     kPassLuma = 0, kPassDown, kPassMedhist, kPassSearch2, kPassSearch6, kPassSearch12, kPassMedian3, kPassSubpix,
     kPassPixfield, kPassSynthRgba8S1, kPassSynthRgba8S2, kPassSynthRgba8S4, kPassFallback,
     kPassLumaFmt, kPassSynthFmtS1, kPassSynthFmtS2, kPassSynthFmtS4,   // Rgb10 / RgbaF16 / P010 (v1_formats.hip)
-    kPassSeedConv,                                                     // learned seed network layer (seednet.hip)
     kPassSynthRgba8S1T16, kPassSynthRgba8S2T16, kPassSynthRgba8S4T16,  // RGBA8 synth, 16-row tiles
     kPassSynthRgba8S2T64,                                              // RGBA8 synth, 64-row tiles (flowScale 2)
     kPassSynthRgba8UiS1, kPassSynthRgba8UiS2, kPassSynthRgba8UiS4,     // RGBA8 synth with CrosshairV2 tap rejection
@@ -19,7 +18,8 @@ enum Pass : int {
     kPassSearch2x8,                                                    // R <= 2 search, two blocks a wavefront (batches)
     kPassPixfieldWin,                                                  // pixel field from LDS luma windows (exact lumas)
     kPassSubpixInt,                                                    // sub-pixel refinement in integers (exact lumas)
-    kPassPyramid, kPassPyramidFmt                                      // level 0 and the coarser levels in one launch
+    kPassPyramid, kPassPyramidFmt,                                     // level 0 and the coarser levels in one launch
+    kPassSeedQ                     // learned seed network layer l, quantised forward: kPassSeedQ + 2 l (+ 1 FP8)
 };
 // CrosshairV2 passes (ui.hip), single launches over the work rectangle
 enum UiPass : int { kUiUpdate = 0, kUiMask, kUiPatch, kUiFill };

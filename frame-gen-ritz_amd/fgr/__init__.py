@@ -177,6 +177,7 @@ _sig = {
                                                  ctypes.c_int32, ctypes.c_int32]),
     "fgr_seednet_set_fp8": (ctypes.c_int, [_H, ctypes.c_int]),
     "fgr_seednet_fp8_scales": (ctypes.c_int, [_H, ctypes.c_void_p, ctypes.c_void_p]),
+    "fgr_debug_fp8_e4m3": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32]),
     "fgr_batch_debug_counters": (ctypes.c_int, [_H, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32,
                                                 ctypes.c_void_p]),
     "fgr_batch_debug_internal": (ctypes.c_int, [_H, ctypes.c_uint32, ctypes.POINTER(DebugInternal)]),
@@ -640,6 +641,13 @@ class SeedNet:
         n, h, w = la.shape
         return bool(lib.fgr_seednet_train_step(self._h, _stream(stream), _ptr(la), _ptr(lc), w, h, n, _ptr(target),
                                                lr, _ptr(loss)))
+
+
+def debug_fp8_e4m3(x, out, stream=None):
+    """The seed network's FP8 E4M3 conversion of the CUDA float32 tensor x into the CUDA uint8 tensor out."""
+    assert x.numel() == out.numel() and x.is_contiguous() and out.is_contiguous()
+    return bool(lib.fgr_debug_fp8_e4m3(_stream(stream), ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+                                       x.numel()))
 
 
 def search_field(la, lc, R, lam, raw, out, seeds=None, stream=None):

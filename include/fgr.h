@@ -301,10 +301,11 @@ int fgr_seednet_train_step(FgrSeedNet* net, void* stream, const float* la, const
 int fgr_seednet_search_field(void* stream, const float* la, const float* lc, int32_t w, int32_t h, int32_t n, int32_t R,
                              float lam, const float* seeds, float* raw, float* out);
 /* FP8 inference (OCP E4M3 on v_mfma_f32_16x16x32_fp8_fp8, FP32 accumulation): calibrate on a batch of luma pairs
- * (per-channel input scales sa = max|x| / 448, folded into the weights, per-output-channel weight scales sw), then
- * switch the network's forward (and the proposals of attached handles, from their next record) between FP8 and BF16.
- * fp8_scales copies all layers' sa (2 + 16 + 32 + 64 + 64 floats) and sw (16 + 32 + 64 + 64 + 2 floats) to the host.
- * Training always runs in BF16; calibrate again after training. */
+ * (per-channel input scales sa = the smallest power of two at or above max|x| / 448, folded into the weights,
+ * per-output-channel weight scales sw), then switch the network's forward (and the proposals of attached handles,
+ * from their next record) between FP8 and BF16 (v_mfma_f32_16x16x32_bf16). Both store each layer's activations once,
+ * quantised, for the next layer. fp8_scales copies all layers' sa (2 + 16 + 32 + 64 + 64 floats) and sw
+ * (16 + 32 + 64 + 64 + 2 floats) to the host. Training always runs in BF16; calibrate again after training. */
 int fgr_seednet_calibrate_fp8(FgrSeedNet* net, void* stream, const float* la, const float* lc, int32_t w, int32_t h,
                               int32_t n);
 int fgr_seednet_set_fp8(FgrSeedNet* net, int on);
@@ -327,6 +328,9 @@ typedef struct FgrDebugInternal {
     int32_t fieldSeed;      /* float2 offset of the learned proposals of the coarsest level's blocks */
 } FgrDebugInternal;
 int fgr_debug_internal(const FgrInterpolator* h, FgrDebugInternal* out);
+/* the FP8 E4M3 conversion of the seed network's FP8 forward (round to nearest even, saturating at +-448) on n device
+ * floats, into n device bytes */
+int fgr_debug_fp8_e4m3(void* stream, const float* in, uint8_t* out, uint32_t n);
 int fgr_batch_debug_internal(const FgrBatch* b, uint32_t pair, FgrDebugInternal* out);
 
 #ifdef __cplusplus

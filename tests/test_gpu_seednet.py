@@ -46,6 +46,38 @@ def test_forward_matches_oracle():
     net.close()
 
 
+def _e4m3_value(b):
+    """OCP FP8 E4M3 bytes -> float64 values (0x7F / 0xFF NaN)"""
+    b = np.asarray(b, np.int64)
+    s = np.where(b & 0x80, -1.0, 1.0)
+    E, M = (b >> 3) & 15, b & 7
+    v = np.where(E == 0, M * 2.0 ** -9, (1 + M / 8.0) * 2.0 ** (E - 7.0))
+    return np.where((b & 0x7F) == 0x7F, np.nan, s * v)
+
+
+def test_fp8_conversion_matches_oracle():
+    """The FP8 forward's conversion (the conversion instruction after a clamp to +-448) against the oracle's e4m3 on
+    every finite E4M3 value, the midpoints between neighbours (ties to even) and points just off them, subnormals,
+    values past 448 (saturation) and random floats over the whole range."""
+    codes = np.array([c for c in range(256) if (c & 0x7F) != 0x7F])
+    vals = np.unique(_e4m3_value(codes))
+    mids = (vals[:-1] + vals[1:]) / 2
+    near = np.concatenate([np.nextafter(mids.astype(np.float32), np.float32(np.inf)),
+                           np.nextafter(mids.astype(np.float32), np.float32(-np.inf))])
+    rng = np.random.default_rng(9)
+    rand = (rng.standard_normal(20000) * np.exp2(rng.uniform(-14, 10, 20000))).astype(np.float32)
+    big = np.array([448.0, 449.0, 463.9, 464.0, 480.0, 1e4, 3e38, -448.0, -464.0, -1e6], np.float32)
+    tiny = np.array([2.0 ** -10, 3 * 2.0 ** -11, 2.0 ** -11, 1e-30, 0.0], np.float32)
+    x = np.concatenate([vals, mids, near, rand, big, tiny, -tiny]).astype(np.float32)
+    xd = torch.from_numpy(x).cuda()
+    out = torch.empty(x.size, dtype=torch.uint8, device="cuda:0")
+    assert fgr.debug_fp8_e4m3(xd, out)
+    got = _e4m3_value(out.cpu().numpy())
+    want = N.e4m3(x).astype(np.float64)
+    bad = np.nonzero(got != want)[0]
+    assert bad.size == 0, [(float(x[i]), float(got[i]), float(want[i])) for i in bad[:8]]
+
+
 def test_fp8_forward_matches_oracle():
     """FP8 (E4M3, per-channel scales) forward on v_mfma_f32_16x16x32_fp8_fp8 against the oracle's emulation with the
     library's calibrated input scales: the weight scales are identical, the outputs equal to a tolerance."""
@@ -58,6 +90,7 @@ def test_fp8_forward_matches_oracle():
     out = torch.empty((4, 4, 6, 2), device="cuda:0")
     assert net.forward(la, lc, out)
     sa, sw = net.fp8_scales()
+    assert np.all(np.frexp(sa)[0] == 0.5), "the input scales are powers of two"
     ref, tabs = N.forward_fp8(la.cpu().numpy(), lc.cpu().numpy(), p, sa)
     assert np.array_equal(np.concatenate([t[1] for t in tabs]), sw)
     got = out.cpu().numpy()
