@@ -75,6 +75,17 @@ def measured_valu_issue(kernel):
     return None
 
 
+def tuning_block(batch, tuning, tune_times):
+    """The autotuner's choice for a batch and the time per step of every configuration it tried: synth tile rows x XCD
+    chunk (every format's synth), the pixel field's form, the radius-2 search's blocks a wavefront, the lanes."""
+    import fgr
+    return {"synth_tile_rows": tuning[0], "xcd_chunk": tuning[1], "lanes": tuning[2],
+            "pixfield": "windows" if batch.get_form() & fgr.Batch.FORM_PIXFIELD_WINDOWS else "gather",
+            "search_r2": "4x1" if batch.get_form() & fgr.Batch.FORM_SEARCH_NARROW else "2x8",
+            "us_per_step": {(f"{r}x{c}" if isinstance(r, int) else f"{r}_{c}"): round(t / 1e3, 1)
+                            for (r, c), t in tune_times.items()}}
+
+
 def timed_run(fn, steps, warmup):
     import torch
     for _ in range(warmup):
@@ -117,7 +128,7 @@ def config_mixed(args):
     items, total = workloads.mixed_batch()
     torch.cuda.synchronize()
     batch = fgr.Batch(items, use_graph=True, yuv_matrix=fgr.YUV_BT2020, transfer=fgr.TRANSFER_LINEAR)
-    _, tuning = batch.autotune(reps=2)
+    tune_times, tuning = batch.autotune(reps=2)
     bs = timed_run(batch.run, args.steps, args.warmup)
     step, _ = plain_runner(items)
     ps = timed_run(step, max(2, args.steps // 4), 1)
@@ -130,8 +141,8 @@ def config_mixed(args):
         "vs_baseline": None, "dtype": "u8/u10/f16", "data": "synthetic",
         "config": {"workload": "64 pairs: 16 each of 1280x720, 1920x1080, 2560x1440, 3840x2160; formats RGBA8 / Rgb10 "
                                "PQ / RgbaF16 scRGB (0-4) / P010 BT.2020 PQ; multipliers 2x/3x/4x; motion +-24 px",
-                   "launches_per_step": batch.launches(), "tuning": {"synth_tile_rows": tuning[0],
-                                                                     "xcd_chunk": tuning[1], "lanes": tuning[2]}},
+                   "launches_per_step": batch.launches()},
+        "tuning": tuning_block(batch, tuning, tune_times),
         "plain_pairs_per_s": round(plain, 2), "batched_over_plain": round(rate / plain, 3),
         "whole_path_hbm_frac": round(total * args.steps / bs / 1e9 / HBM_PEAK_GBS, 4),
         "algorithmic_bytes_per_step": int(total)}))
@@ -413,12 +424,7 @@ def main():
         "batched_over_plain": None if batch_rate is None else round(batch_rate / plain_rate, 3),
         "whole_path_hbm_frac": round(total_bytes * args.steps / (ms * 1e-3 * args.steps) / 1e9 / HBM_PEAK_GBS, 4),
         "roofline": roofline,
-        "tuning": None if batch is None else {
-            "synth_tile_rows": tuning[0], "xcd_chunk": tuning[1], "lanes": tuning[2],
-            "pixfield": "windows" if batch.get_form() & fgr.Batch.FORM_PIXFIELD_WINDOWS else "gather",
-            "search_r2": "4x1" if batch.get_form() & fgr.Batch.FORM_SEARCH_NARROW else "2x8",
-            "us_per_step": {(f"{r}x{c}" if isinstance(r, int) else f"{r}_{c}"): round(t / 1e3, 1)
-                            for (r, c), t in tune_times.items()}},
+        "tuning": None if batch is None else tuning_block(batch, tuning, tune_times),
         "copy_probe": {"bytes": copy_bytes, "gbs": round(copy_gbs, 1), "frac_of_peak": round(copy_gbs / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
     }

@@ -63,7 +63,7 @@ struct FgrBatch {
     std::vector<hipEvent_t> events;   // a (start, end) pair per launch while profiling
     bool profiled = false;
     int chunk = kXcdChunk;            // tiles per XCD chunk of the work lists (tuned)
-    int rows = 32;                    // RGBA8 synth tile rows (tuned)
+    int rows = 32;                    // synth tile rows (tuned)
     int lanes = 1;                    // 1 .. kMaxLanes (tuned)
     int form = 0;                     // estimate forms, FGR_FORM_* (tuned)
     std::vector<int> laneOf;          // each pair's lane
@@ -369,7 +369,7 @@ extern "C" int fgr_batch_set_lanes(FgrBatch* b, int32_t lanes) {
 
 extern "C" int32_t fgr_batch_get_lanes(const FgrBatch* b) { return b ? b->lanes : 0; }
 
-// Times every tile configuration (RGBA8 synth tiles of 16, 32, 64 rows x work-list XCD chunks of 16, 64, 256 tiles,
+// Times every tile configuration (synth tiles of 16, 32, 64 rows x work-list XCD chunks of 16, 64, 256 tiles,
 // at one lane and the default forms) on this batch's own records -- `reps` eager records each, after one untimed --
 // keeps the fastest; then each estimate form knob on and off with those tiles; then 1, 2, 3 and 4 lanes with the best
 // of both; and caches the result for batches of the same key. The records write the batch's outputs. ns[i] receives

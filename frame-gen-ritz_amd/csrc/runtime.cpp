@@ -799,6 +799,9 @@ void FgrInterpolator::planSynth(std::vector<PlanOp>& plan, const FgrFrame& prev,
         }
     } else if (estUi) {   // the tap-rejecting variant
         op.pass = set.flowScale == 1 ? kPassSynthFmtUiS1 : (set.flowScale == 2 ? kPassSynthFmtUiS2 : kPassSynthFmtUiS4);
+    } else if (set.flowScale == 2 && (synthRows == 16 || synthRows == 64)) {
+        rows = synthRows;
+        op.pass = rows == 16 ? kPassSynthFmtS2T16 : kPassSynthFmtS2T64;
     } else {
         op.pass = set.flowScale == 1 ? kPassSynthFmtS1 : (set.flowScale == 2 ? kPassSynthFmtS2 : kPassSynthFmtS4);
     }

@@ -86,7 +86,8 @@ struct FgrInterpolator {
 
     // learned seed proposals (seednet.hip): the attached network and its activations on the coarsest level
     FgrSeedNet* net = nullptr;
-    int synthRows = 32;               // RGBA8 synth tile height (16, 32, 64 with flowScale 2), chosen by the batch tuner
+    int synthRows = 32;               // synth tile height (16, 32, 64; the other formats: 16 and 64 at flowScale 2 only),
+                                      // chosen by the batch tuner
     int estForm = 0;                  // batched estimate kernels, chosen by the batch tuner (FGR_FORM_*)
     // recordSynthBlur's sub-frames (blur.hip): sized when settings or sizes are applied with blurSamples >= 2 and only
     // grown; a replaced buffer is kept (records queued before the settings change may still read it) until the

@@ -464,15 +464,14 @@ __device__ __forceinline__ void searchWindowsIssued(const SearchParams& P, float
     }
 }
 
-// step 2 for the two blocks of a wave (T = 8 + 2R <= MAXT): every lane's loads of both windows issued before any store,
-// so the two windows wait on one memory latency (element i of a window as in searchWindows)
-template <int MAXT>
+// step 2 for the two blocks of a wave at radius R (T = 8 + 2R): every lane's loads of both windows issued before any
+// store, so the two windows wait on one memory latency (element i of a window as in searchWindows)
+template <int R>
 __device__ __forceinline__ void searchWindowsPair(const SearchParams& P, float* A0, float* C0, float* A1, float* C1,
-                                                  const int* bxs, const int* bys, const int* sxs, const int* sys, int R,
+                                                  const int* bxs, const int* bys, const int* sxs, const int* sys,
                                                   int lane, bool ints) {
-    constexpr int IT = (MAXT * MAXT + 63) / 64;
-    const int T = 8 + 2 * R, N = T * T;
-    const int q = 64 / T, r = 64 - q * T;   // element i + 64 is (x + r, y + q), carried
+    constexpr int T = 8 + 2 * R, N = T * T, IT = (N + 63) / 64;
+    constexpr int q = 64 / T, r = 64 - q * T;   // element i + 64 is (x + r, y + q), carried
     float va[2][IT], vc[2][IT];
     int x = lane % T, y = lane / T;
 #pragma unroll
@@ -547,6 +546,24 @@ __device__ __forceinline__ void searchCostInt(const SearchParams& P, const uint1
     cost = sadc + P.lam * (float)(abs(dx) + abs(dy));
 }
 
+// searchCostInt at a compile-time radius: every LDS offset an immediate
+template <int R>
+__device__ __forceinline__ void searchCostIntR(const SearchParams& P, const uint16_t* A, const uint16_t* C, int ci,
+                                               float& cost, float& sadc) {
+    constexpr int T = 8 + 2 * R, D = 2 * R + 1, N = T * T;
+    const int dx = ci % D - R, dy = ci / D - R;
+    const int ca = R - dx, cc = R + dx, par = ca & 1;
+    const uint32_t* wa = (const uint32_t*)(A + (par ? N : 0)) + (((R - dy) * T + ca - par) >> 1);
+    const uint32_t* wc = (const uint32_t*)(C + (par ? N : 0)) + (((R + dy) * T + cc - par) >> 1);
+    uint32_t acc = 0u;
+#pragma unroll
+    for (int y = 0; y < 8; y++)
+#pragma unroll
+        for (int m = 0; m < 4; m++) acc = __builtin_amdgcn_sad_u16(wa[y * (T / 2) + m], wc[y * (T / 2) + m], acc);
+    sadc = (float)acc * (__int_as_float((127 - P.fb) << 23) * (1.0f / 64.0f));
+    cost = sadc + P.lam * (float)(abs(dx) + abs(dy));
+}
+
 // step 4 within groups of `width` lanes: the first of the minimal costs
 __device__ __forceinline__ void searchArgmin(float& bestc, int& besti, float& bests, int width) {
     for (int off = width / 2; off >= 1; off >>= 1) {
@@ -604,13 +621,14 @@ __device__ __forceinline__ void searchBody(const SearchParams& P, int bx0, int) 
 // R <= 2 ((2R + 1)^2 <= 25 candidates): two blocks a wavefront, eight a workgroup. The seeds and windows of a wave's two
 // blocks are formed one after the other with all 64 lanes, then lanes 0-31 take the first block's candidates and lanes
 // 32-63 the second's (each lane one candidate, its SAD in the order above), and the argmin runs within each half:
-// the same vectors with twice the lanes busy.
-__device__ __forceinline__ void search2Body(const SearchParams& P, int bx0, int) {
-    constexpr int MAXT = 12;
+// the same vectors with twice the lanes busy. R is a template argument (1 or 2), so window and LDS offsets are
+// constants.
+template <int R>
+__device__ __forceinline__ void search2BodyR(const SearchParams& P, int bx0) {
+    constexpr int MAXT = 12, D = 2 * R + 1;
     __shared__ float ta[kSearch2Blocks][MAXT * MAXT], tc[kSearch2Blocks][MAXT * MAXT];
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;   // wave: uniform
     const int nb = P.bw * P.bh;
-    const int R = P.R, D = 2 * R + 1;
     int sxs[2] = {0, 0}, sys[2] = {0, 0}, bxs[2], bys[2];
     bool acts[2];
 #pragma unroll
@@ -630,8 +648,7 @@ __device__ __forceinline__ void search2Body(const SearchParams& P, int bx0, int)
         for (int h = 0; h < 2; h++) seedPick(P, S[h], sxs[h], sys[h]);
     }
     const bool ints = P.fb >= 0;   // (uniform) packed 16-bit SADs
-    searchWindowsPair<MAXT>(P, ta[2 * wave], tc[2 * wave], ta[2 * wave + 1], tc[2 * wave + 1], bxs, bys, sxs, sys, R, lane,
-                            ints);
+    searchWindowsPair<R>(P, ta[2 * wave], tc[2 * wave], ta[2 * wave + 1], tc[2 * wave + 1], bxs, bys, sxs, sys, lane, ints);
     __syncthreads();
     const int half = lane >> 5, ci = lane & 31;
     const int blk = bx0 * kSearch2Blocks + 2 * wave + half;
@@ -640,7 +657,7 @@ __device__ __forceinline__ void search2Body(const SearchParams& P, int bx0, int)
     int besti = INT_MAX;
     if (ci < D * D) {
         float cost, sadc;
-        if (ints) searchCostInt(P, (const uint16_t*)ta[2 * wave + half], (const uint16_t*)tc[2 * wave + half], R, ci, cost, sadc);
+        if (ints) searchCostIntR<R>(P, (const uint16_t*)ta[2 * wave + half], (const uint16_t*)tc[2 * wave + half], ci, cost, sadc);
         else searchCost(P, ta[2 * wave + half], tc[2 * wave + half], R, ci, cost, sadc);
         bestc = cost;
         besti = ci;
@@ -659,6 +676,11 @@ __device__ __forceinline__ void search2Body(const SearchParams& P, int bx0, int)
             atomicAdd(P.u + kUSad + (bx0 & (kSlots - 1)), sum);
         }
     }
+}
+
+__device__ __forceinline__ void search2Body(const SearchParams& P, int bx0, int) {
+    if (P.R == 2) search2BodyR<2>(P, bx0);   // (uniform) the radius of the level
+    else search2BodyR<1>(P, bx0);
 }
 
 // median3.comp:8-28: per-component 3x3 median, replicate border (the shader's partial selection sort)

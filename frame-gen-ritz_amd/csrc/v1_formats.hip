@@ -374,44 +374,44 @@ __device__ __forceinline__ void copyNear(const SynthParams& P, int x, int y) {
     }
 }
 
-// One 64 x 32 output tile (the RGBA8 kernel's structure): field tile, mismatch of the tile and its 3-pixel apron into
-// LDS, 7x7 box, trust, untrusted count, blend. A thread warps the 8 pixels it resolves first and keeps their mixes in
-// registers (three floats, or the Y level's one), so the blend reads only the nearer frame; the apron's 612 pixels
-// follow, spread over the workgroup.
-template <int S, int F, bool UI = false>
+// One 64 x RT output tile (RT = 16, 32 or 64 rows; the RGBA8 kernel's structure): field tile, mismatch of the tile and
+// its 3-pixel apron into LDS, 7x7 box, trust, untrusted count, blend. A thread warps the RT / 4 pixels it resolves
+// first and keeps their mixes in registers (three floats, or the Y level's one), so the blend reads only the nearer
+// frame; the apron's 2 x 3 x 70 + 6 RT pixels follow, spread over the workgroup.
+template <int S, int F, bool UI = false, int RT = TH>
 __device__ __forceinline__ void synthFmtTile(const SynthParams& P, int tileX, int tileY, float2* ft, float2* hrow,
                                              int* es, uint16_t* hs, uint32_t* cnt) {
     const int tid = threadIdx.x;
-    const int x0 = tileX * TW, y0 = tileY * TH;
+    const int x0 = tileX * TW, y0 = tileY * RT;
     const int cx = tid & (TW - 1), ry = tid >> 6;
     if (sceneCut(P)) {
-        for (int k = 0; k < TH / 4; k++) {
+        for (int k = 0; k < RT / 4; k++) {
             const int x = x0 + cx, y = y0 + ry + 4 * k;
             if (x < P.W && y < P.H) copyNear<F>(P, x, y);
         }
         return;
     }
     if (tid == 0) *cnt = 0u;
-    const int ty0 = loadFieldTile<S>(P, x0, y0, ft, hrow);
+    const int ty0 = loadFieldTile<S, RT>(P, x0, y0, ft, hrow);
     const float kA32 = (float)(32 * S) * (2.0f * P.wC), kC32 = (float)(32 * S) * (2.0f * P.wA);
     // the tile's own pixels (region column cx + AP, row ly + AP; clamped into the image as the apron is)
-    float mix[TH / 4][kMixes<F>];
+    float mix[RT / 4][kMixes<F>];
 #pragma unroll
-    for (int k = 0; k < TH / 4; k++) {
+    for (int k = 0; k < RT / 4; k++) {
         const int ly = ry + 4 * k;
         const int x = min(x0 + cx, P.W - 1), y = min(y0 + ly, P.H - 1);
         const float2 r = fieldAtRegion<S>(P, hrow, ty0, cx + AP, y);
         es[(ly + AP) * EW + cx + AP] =
             mismatchFmt<F, UI, true>(P, mapsAt(kA32, kC32, (float)(32 * x), (float)(32 * y), r), mix[k]);
     }
-    // the apron: AP rows above and below the tile (EW wide), then AP columns either side of its TH rows
-    constexpr int kRowsAp = 2 * AP * EW, kApron = kRowsAp + 2 * AP * TH;
+    // the apron: AP rows above and below the tile (EW wide), then AP columns either side of its RT rows
+    constexpr int kRowsAp = 2 * AP * EW, kApron = kRowsAp + 2 * AP * RT;
     for (int i = tid; i < kApron; i += 256) {
         int ex, ey;
         if (i < kRowsAp) {
             const int j = i / EW;
             ex = i - j * EW;
-            ey = j < AP ? j : j + TH;
+            ey = j < AP ? j : j + RT;
         } else {
             const int j = (i - kRowsAp) / (2 * AP), c = i - kRowsAp - j * (2 * AP);
             ey = j + AP;
@@ -422,11 +422,11 @@ __device__ __forceinline__ void synthFmtTile(const SynthParams& P, int tileX, in
         es[ey * EW + ex] = mismatchFmt<F, UI>(P, mapsAt(kA32, kC32, (float)(32 * x), (float)(32 * y), r));
     }
     __syncthreads();
-    boxRows7<EH>(es, hs, tid);
+    boxRows7<RT + 2 * AP>(es, hs, tid);
     __syncthreads();
     uint32_t mine = 0u;
 #pragma unroll
-    for (int k = 0; k < TH / 4; k++) {
+    for (int k = 0; k < RT / 4; k++) {
         const int ly = ry + 4 * k, x = x0 + cx, y = y0 + ly;
         if (x >= P.W || y >= P.H) continue;
         const uint16_t* h = &hs[ly * TW + cx];
@@ -444,18 +444,18 @@ __device__ __forceinline__ void synthFmtTile(const SynthParams& P, int tileX, in
     if (tid == 0 && *cnt) atomicAdd(P.count + ((tileY * P.gx + tileX) & (kSlots - 1)), *cnt);
 }
 
-template <int S, bool UI = false>
+template <int S, bool UI = false, int RT = TH>
 __device__ __forceinline__ void synthFmtBody(const SynthParams& P, int bx, int by) {
-    constexpr int FTW = SynthTile<S>::FTW, FTH = SynthTile<S>::FTH;
+    constexpr int FTW = SynthTile<S, RT>::FTW, FTH = SynthTile<S, RT>::FTH, EHR = RT + 2 * AP;
     __shared__ float2 ft[FTW * FTH];
     __shared__ float2 hrow[FTH * EW];
-    __shared__ __align__(16) int es[EH * EW];
-    __shared__ __align__(16) uint16_t hs[EH * TW];
+    __shared__ __align__(16) int es[EHR * EW];
+    __shared__ __align__(16) uint16_t hs[EHR * TW];
     __shared__ uint32_t cnt;
-    if (P.fmt == kRgb10) synthFmtTile<S, kRgb10, UI>(P, bx, by, ft, hrow, es, hs, &cnt);
-    else if (P.fmt == kF16) synthFmtTile<S, kF16, UI>(P, bx, by, ft, hrow, es, hs, &cnt);
-    else if (P.fmt == kNv12) synthFmtTile<S, kNv12, UI>(P, bx, by, ft, hrow, es, hs, &cnt);
-    else synthFmtTile<S, kP010, UI>(P, bx, by, ft, hrow, es, hs, &cnt);
+    if (P.fmt == kRgb10) synthFmtTile<S, kRgb10, UI, RT>(P, bx, by, ft, hrow, es, hs, &cnt);
+    else if (P.fmt == kF16) synthFmtTile<S, kF16, UI, RT>(P, bx, by, ft, hrow, es, hs, &cnt);
+    else if (P.fmt == kNv12) synthFmtTile<S, kNv12, UI, RT>(P, bx, by, ft, hrow, es, hs, &cnt);
+    else synthFmtTile<S, kP010, UI, RT>(P, bx, by, ft, hrow, es, hs, &cnt);
 }
 
 FGR_PASS(luma_fmt, LumaParams, lumaFmtBody, 256)
@@ -463,6 +463,9 @@ FGR_PASS(pyramid_fmt, PyrParams, pyramidFmtBody, 256)
 FGR_PASS(synth_fmt_s1, SynthParams, synthFmtBody<1>, 256)
 FGR_PASS(synth_fmt_s2, SynthParams, synthFmtBody<2>, 256)
 FGR_PASS(synth_fmt_s4, SynthParams, synthFmtBody<4>, 256)
+#define FGR_SYNTH_FMT_ROWS_BODY(S, RT) synthFmtBody<S, false, RT>
+FGR_PASS(synth_fmt_s2_t16, SynthParams, FGR_SYNTH_FMT_ROWS_BODY(2, 16), 256)
+FGR_PASS(synth_fmt_s2_t64, SynthParams, FGR_SYNTH_FMT_ROWS_BODY(2, 64), 256)
 #define FGR_SYNTH_FMT_UI_BODY(S) synthFmtBody<S, true>
 FGR_PASS_SINGLE(synth_fmt_ui_s1, SynthParams, FGR_SYNTH_FMT_UI_BODY(1), 256)
 FGR_PASS_SINGLE(synth_fmt_ui_s2, SynthParams, FGR_SYNTH_FMT_UI_BODY(2), 256)
@@ -507,6 +510,8 @@ hipError_t launchFmtSingle(int pass, const void* params, int gx, int gy, hipStre
     case kPassSynthFmtS1: hipLaunchKernelGGL(synth_fmt_s1_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
     case kPassSynthFmtS2: hipLaunchKernelGGL(synth_fmt_s2_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
     case kPassSynthFmtS4: hipLaunchKernelGGL(synth_fmt_s4_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
+    case kPassSynthFmtS2T16: hipLaunchKernelGGL(synth_fmt_s2_t16_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
+    case kPassSynthFmtS2T64: hipLaunchKernelGGL(synth_fmt_s2_t64_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
     case kPassSynthFmtUiS1: hipLaunchKernelGGL(synth_fmt_ui_s1_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
     case kPassSynthFmtUiS2: hipLaunchKernelGGL(synth_fmt_ui_s2_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
     case kPassSynthFmtUiS4: hipLaunchKernelGGL(synth_fmt_ui_s4_kernel, g, dim3(256), 0, s, *(const SynthParams*)params); break;
@@ -523,6 +528,8 @@ hipError_t launchFmtBatch(int pass, const void* params, const uint32_t* work, in
     case kPassSynthFmtS1: hipLaunchKernelGGL(synth_fmt_s1_batch_kernel, g, dim3(256), 0, s, (const SynthParams*)params, work); break;
     case kPassSynthFmtS2: hipLaunchKernelGGL(synth_fmt_s2_batch_kernel, g, dim3(256), 0, s, (const SynthParams*)params, work); break;
     case kPassSynthFmtS4: hipLaunchKernelGGL(synth_fmt_s4_batch_kernel, g, dim3(256), 0, s, (const SynthParams*)params, work); break;
+    case kPassSynthFmtS2T16: hipLaunchKernelGGL(synth_fmt_s2_t16_batch_kernel, g, dim3(256), 0, s, (const SynthParams*)params, work); break;
+    case kPassSynthFmtS2T64: hipLaunchKernelGGL(synth_fmt_s2_t64_batch_kernel, g, dim3(256), 0, s, (const SynthParams*)params, work); break;
     default: return launchSeedBatch(pass, params, work, n, s);
     }
     return hipGetLastError();

@@ -19,6 +19,7 @@ enum Pass : int {
     kPassPixfieldWin,                                                  // pixel field from LDS luma windows (exact lumas)
     kPassSubpixInt,                                                    // sub-pixel refinement in integers (exact lumas)
     kPassPyramid, kPassPyramidFmt,                                     // level 0 and the coarser levels in one launch
+    kPassSynthFmtS2T16, kPassSynthFmtS2T64,                            // the other formats' synth, 16- / 64-row tiles
     kPassSeedQ                     // learned seed network layer l, quantised forward: kPassSeedQ + 2 l (+ 1 FP8)
 };
 // CrosshairV2 passes (ui.hip), single launches over the work rectangle

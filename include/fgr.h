@@ -247,7 +247,8 @@ uint32_t fgr_batch_launches(const FgrBatch* b);
  * 401 pixfield, 500 + 10 k synth k, 501 + 10 k its fallback), its pass and its duration in ns. Returns the number of
  * launches (-1 if nothing was profiled). */
 int fgr_batch_set_profiling(FgrBatch* b, int on);
-/* Tile autotuner: the tile configurations are the RGBA8 synth's tile height (16, 32 or 64 rows) and the number of
+/* Tile autotuner: the tile configurations are the synth's tile height (16, 32 or 64 rows: RGBA8 at any flowScale but
+ * 64 rows at 2 only, the other formats at flowScale 2, 32 rows otherwise) and the number of
  * consecutive tiles each XCD takes from a launch's work list (16, 64 or 256). autotune times all nine on this batch's
  * own records (`reps` each; the outputs are written), keeps the fastest, then the estimate forms (below) one knob at a
  * time, then 1 to 4 lanes (below), and remembers the choice for batches of the same pairs (sizes, formats,

@@ -187,9 +187,11 @@ def test_p010_full_range_bt709_and_rgb10_bgra():
     check_fmt_output(outs[0], A, C, est, 0.5, F.RGB10, None, None, counts[0])
 
 
-def test_mixed_format_batch_matches_per_pair():
+@pytest.mark.parametrize("rows", [32, 16, 64])
+def test_mixed_format_batch_matches_per_pair(rows):
     """One batch of RGBA8, Rgb10, RgbaF16, Nv12 and P010 pairs of different sizes and multipliers: one launch per pass for
-    the three non-RGBA8 formats together; every output equal to the per-pair path (itself checked above)."""
+    the three non-RGBA8 formats together; every output equal to the per-pair path (itself checked above), with synth
+    tiles of 32 rows and of the tuner's 16 and 64 rows."""
     specs = [(F.RGBA8, 320, 180, 2), (F.RGB10, 256, 144, 3), (F.F16, 288, 160, 4), (F.P010, 320, 176, 4),
              (F.RGB10, 202, 118, 2), (F.P010, 256, 144, 3), (F.NV12, 224, 128, 3)]
     items, refs = [], []
@@ -211,6 +213,7 @@ def test_mixed_format_batch_matches_per_pair():
         fi.close()
         refs.append(want)
     b = fgr.Batch(items, use_graph=True, yuv_matrix=fgr.YUV_BT2020)
+    assert b.set_tuning(rows, 64) and b.get_tuning()[0] == rows
     b.run()
     b.run()
     torch.cuda.synchronize()
