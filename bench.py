@@ -10,8 +10,8 @@ three synths at t = 1/4, 1/2, 3/4. Frames are resident in HBM before the timed r
 `value` is the batched rate when the batched path is built, else the plain rate (the line says which).
 
 Extra fields:
-  * `roofline` of the dominant kernel, the batched synth (one launch = the k-th synth of all 8 pairs): algorithmic
-    bytes per launch (3F + P per pair, 2F for the scene-cut pair's copy) over its average launch duration, from HIP
+  * `roofline` of the dominant kernel, the batched synth (one launch = the three synths of all 8 pairs): algorithmic
+    bytes per launch (3F + P per synth, 2F for the scene-cut pair's copies) over its average launch duration, from HIP
     events around each launch on the launch stream; `traffic` = the HBM bytes per launch that rocprofv3's FETCH_SIZE
     and WRITE_SIZE counters read (from the committed summary under profiles/), or null; `valu_issue_frac` = the
     fraction of the GPU's VALU issue slots the launch's wavefronts filled (SQ_INSTS_VALU over 256 CUs x cycles, from
@@ -337,7 +337,7 @@ def main():
         value, ms = plain_rate, 1e3 * plain_s / args.steps
         path = "plain"
 
-    # roofline of the dominant kernel, the batched synth (one launch = the k-th synth of every pair): the batch's
+    # roofline of the dominant kernel, the batched synth (one launch = every synth of every pair): the batch's
     # launches recorded eagerly between HIP events on the launch stream, `rounds` records
     rounds = max(2, args.steps // 4)
     synth_ns, synth_bytes_l = [], []
@@ -348,9 +348,10 @@ def main():
         for _ in range(rounds):
             rb.run()
             for stage, pas, ns in rb.read_timings():
-                if 500 <= stage < 700 and stage % 10 == 0:
+                if stage == 500:   # the batch's one synth launch: every synth of every pair
                     synth_ns.append(ns)
-                    synth_bytes_l.append(sum(synth_bytes(W, H, cut=(p == cut_index)) for p in range(len(fr))))
+                    synth_bytes_l.append(sum((MULT - 1) * synth_bytes(W, H, cut=(p == cut_index))
+                                             for p in range(len(fr))))
         rb.set_profiling(False)
         rb.set_lanes(tuning[2])
         kernel = {16: "synth_rgba8_s2_t16_batch_kernel", 32: "synth_rgba8_s2_batch_kernel",

@@ -107,7 +107,8 @@ struct FgrBatch {
             const FgrBatchPair& d = desc[p];
             I.planEstimate(plans[p], d.prev, d.curr, true);
             for (uint32_t k = 1; k < d.multiplier; k++)
-                I.planSynth(plans[p], d.prev, d.curr, d.out[k - 1], (float)((double)k / (double)d.multiplier), (int)k - 1);
+                I.planSynth(plans[p], d.prev, d.curr, d.out[k - 1], (float)((double)k / (double)d.multiplier), (int)k - 1,
+                            true, true);
             for (const PlanOp& op : plans[p]) groups[{op.stage, op.pass, laneOf[p]}].push_back({(int)p, &op});
         }
         launches.clear();
@@ -123,10 +124,19 @@ struct FgrBatch {
             blob.resize(align256(blob.size() + L.paramBytes));
             for (size_t i = 0; i < g.second.size(); i++)
                 std::memcpy(blob.data() + L.paramOff + i * psz, g.second[i].second->params, psz);
+            // a pair's ops in one group (its synths, its fallbacks) have one grid: their tiles interleave, so that the
+            // workgroups that read the same region of the pair's frames run side by side (and share an L2)
             std::vector<uint32_t> work;
-            for (size_t i = 0; i < g.second.size(); i++) {
+            for (size_t i = 0; i < g.second.size();) {
+                size_t j = i + 1;
                 const PlanOp& op = *g.second[i].second;
-                for (int l = 0; l < op.gx * op.gy; l++) work.push_back(((uint32_t)i << kWorkShift) | (uint32_t)l);
+                const int tiles = op.gx * op.gy;
+                while (j < g.second.size() && g.second[j].first == g.second[i].first &&
+                       g.second[j].second->gx * g.second[j].second->gy == tiles)
+                    j++;
+                for (int l = 0; l < tiles; l++)
+                    for (size_t m = i; m < j; m++) work.push_back(((uint32_t)m << kWorkShift) | (uint32_t)l);
+                i = j;
             }
             // XCD-aware order (xcdChunkTile, as the per-pair kernels use): tiles that read overlapping regions share
             // an L2

@@ -776,7 +776,7 @@ void FgrInterpolator::synthParams(SynthParams& p, const FgrFrame& prev, const Fg
 }
 
 void FgrInterpolator::planSynth(std::vector<PlanOp>& plan, const FgrFrame& prev, const FgrFrame& curr,
-                                const FgrFrame& out, float t, int k, bool uiPatch) const {
+                                const FgrFrame& out, float t, int k, bool uiPatch, bool merged) const {
     const int bank = k % kSynthBanks;
     SynthParams sp;
     synthParams(sp, prev, curr, t, bank, true);
@@ -805,7 +805,7 @@ void FgrInterpolator::planSynth(std::vector<PlanOp>& plan, const FgrFrame& prev,
     } else {
         op.pass = set.flowScale == 1 ? kPassSynthFmtS1 : (set.flowScale == 2 ? kPassSynthFmtS2 : kPassSynthFmtS4);
     }
-    op.stage = 500 + 10 * k;
+    op.stage = merged ? 500 : 500 + 10 * k;
     op.gx = sp.gx;
     op.gy = synthTilesY((int)H, rows);
     op.stamp = "synth";
@@ -825,11 +825,11 @@ void FgrInterpolator::planSynth(std::vector<PlanOp>& plan, const FgrFrame& prev,
     fp.count = sp.count;
     fp.gfrac = set.globalFallback;
     fp.nearC = sp.nearC;
-    fp.nextCount = u + kUSynth + kSlots * ((k + 1) % kSynthBanks);
+    fp.nextCount = merged ? nullptr : u + kUSynth + kSlots * ((k + 1) % kSynthBanks);
     fp.gx = fallbackTilesX((int)W);
     PlanOp of;
     of.pass = kPassFallback;
-    of.stage = 501 + 10 * k;
+    of.stage = merged ? 590 : 501 + 10 * k;
     of.gx = fp.gx;
     of.gy = fallbackTilesY((int)H);
     of.stamp = "fallback";
@@ -838,7 +838,7 @@ void FgrInterpolator::planSynth(std::vector<PlanOp>& plan, const FgrFrame& prev,
     if (estUi && uiPatch) {   // framegen.cpp:3521-3527: curr's unchanged UI pixels over the finished frame
         PlanOp pp;
         pp.pass = kPassUiPatch;
-        pp.stage = 502 + 10 * k;
+        pp.stage = merged ? 595 : 502 + 10 * k;
         pp.stamp = "ui_patch";
         pp.set(uiPatchParams(curr, out));
         plan.push_back(pp);

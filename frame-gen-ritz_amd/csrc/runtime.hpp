@@ -152,8 +152,10 @@ struct FgrInterpolator {
     void beginProfile(hipStream_t s);
     // batched: the plan of a batch (fgr_batch_*): kernels laid out for many workgroups rather than for latency
     void planEstimate(std::vector<PlanOp>& plan, const FgrFrame& prev, const FgrFrame& curr, bool batched = false) const;
+    // merged: the batch's form -- every synth of the pair at one stage (one launch for all its synths), every fallback
+    // at one later stage (its count bank was cleared by the estimate, so it clears none)
     void planSynth(std::vector<PlanOp>& plan, const FgrFrame& prev, const FgrFrame& curr, const FgrFrame& out, float t,
-                   int k, bool uiPatch = true) const;
+                   int k, bool uiPatch = true, bool merged = false) const;
     bool runPlan(hipStream_t s, const std::vector<PlanOp>& plan);
     bool enqueueEstimate(hipStream_t s, const FgrFrame& prev, const FgrFrame& curr);
     bool enqueueSynth(hipStream_t s, const FgrFrame& prev, const FgrFrame& curr, const FgrFrame& out, float t);

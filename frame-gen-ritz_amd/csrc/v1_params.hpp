@@ -183,7 +183,7 @@ struct FallbackParams {
     int W, H, bpp;            // bytes per pixel of plane 0
     int cbpp;                 // bytes per chroma texel (Nv12 2, P010 4)
     const uint32_t* count;    // kSlots partial counts
-    uint32_t* nextCount;      // the other synth bank (kSlots words), cleared here for the next synth
+    uint32_t* nextCount;      // the next synth's bank (kSlots words), cleared here for it; null: none (batch)
     float gfrac;
     int nearC;
 };

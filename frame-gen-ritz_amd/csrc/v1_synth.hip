@@ -368,7 +368,7 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupParams P) {
 // fallback.comp:9-24: more than globalFallback x W x H untrusted pixels -> the nearer real frame. Every workgroup
 // exits at once in the normal case (few of them: 128x128 pixels per workgroup).
 __device__ __forceinline__ void fallbackBody(const FallbackParams& P, int gbx, int gby) {
-    if (gbx == 0 && gby == 0 && threadIdx.x < kSlots) P.nextCount[threadIdx.x] = 0u;
+    if (P.nextCount && gbx == 0 && gby == 0 && threadIdx.x < kSlots) P.nextCount[threadIdx.x] = 0u;
     const bool flick = (float)slotSum(P.count) > P.gfrac * (float)P.W * (float)P.H;
     if (!flick) return;
     const FrameRef& nf = P.nearC ? P.c : P.a;
