@@ -552,7 +552,7 @@ void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& pr
         op.set(dp);
         plan.push_back(op);
     }
-    // 3-5. per level, coarse -> fine: median, seed + search, median3
+    // 3-5. per level, coarse -> fine: median (with the coarser level's median3), seed + search; median3 of level 0
     static const char* lvNames[] = {"level 0 (finest)", "level 1", "level 2", "level 3", "level 4", "level 5", "level 6", "level 7"};
     const int n = (int)lv.size();
     for (int i = n - 1; i >= 0; i--) {
@@ -578,6 +578,8 @@ void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& pr
             const Level& pl = lv[i + 1];
             MedParams mp{};
             mp.parent = F + pl.fM;
+            mp.raw = F + pl.fS;   // the parent level's median3 is formed here
+            mp.filtered = F + pl.fM;
             mp.pw = pl.bw;
             mp.ph = pl.bh;
             mp.bw = l.bw;
@@ -610,6 +612,11 @@ void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& pr
         op.stage = 201 + 10 * step;
         op.gx = sp.gx;
         op.set(sp);
+        if (i > 0) {   // this level's median3 runs in the next level's median pass
+            op.stamp = lvNames[std::min(i, 7)];
+            plan.push_back(op);
+            continue;
+        }
         plan.push_back(op);
         Median3Params m3{};
         m3.in = F + l.fS;

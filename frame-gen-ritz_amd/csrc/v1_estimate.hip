@@ -155,6 +155,31 @@ __device__ __forceinline__ void downBody(const DownParams& P, int bx, int by) {
 // parent is counted with the number of fine blocks that map to it (min(ix/2, pw-1), min(iy/2, ph-1)), so the loop runs
 // over the parent grid. Lanes holding the same (bin, weight) add once per wavefront (ballot aggregation), which keeps
 // the LDS atomics off the one hot bin of a global pan. Writes 2 * 0.5 * (k1-th + k2-th) as float bits.
+// median3.comp:8-28: per-component 3x3 median, replicate border (the shader's partial selection sort)
+__device__ __forceinline__ float med9(float* v) {
+    for (int i = 0; i <= 4; i++)
+        for (int j = i + 1; j < 9; j++)
+            if (v[j] < v[i]) {
+                const float t = v[i];
+                v[i] = v[j];
+                v[j] = t;
+            }
+    return v[4];
+}
+
+__device__ __forceinline__ float2 median3At(const float2* in, int bx, int by, int bw, int bh) {
+    float xs[9], ys[9];
+    int k = 0;
+    for (int j = -1; j <= 1; j++)
+        for (int i = -1; i <= 1; i++) {
+            const float2 v = fieldAt(in, bx + i, by + j, bw, bh);
+            xs[k] = v.x;
+            ys[k] = v.y;
+            k++;
+        }
+    return make_float2(med9(xs), med9(ys));
+}
+
 __device__ __forceinline__ int parentWeight(int p, int np, int nb) {
     // number of fine indices i in [0, nb) with min(i / 2, np - 1) == p
     const int lo = 2 * p;
@@ -234,15 +259,20 @@ __device__ __forceinline__ void medhistBody(const MedParams& P, int, int) {
         const int i = i0 + t;
         if (i < np) {
             const int px = i % P.pw, py = i / P.pw;
+            float2 v;
+            if (P.raw) {   // (uniform) the parent level's median3, written for the search and binned here
+                v = median3At(P.raw, px, py, P.pw, P.ph);
+                P.filtered[i] = v;
+            }
             const int wgt = parentWeight(px, P.pw, P.bw) * parentWeight(py, P.ph, P.bh);
             if (wgt > 0) {
-                const float2 v = P.parent[i];
+                if (!P.raw) v = P.parent[i];
                 histAdd(hx, clampi(((int)rintf(v.x) >> sh) + 128, 0, 255), wgt);
                 histAdd(hy, clampi(((int)rintf(v.y) >> sh) + 128, 0, 255), wgt);
             }
         }
     }
-    __syncthreads();
+    __syncthreads();   // (also makes the filtered field the second pass reads visible to the workgroup)
     // inclusive prefix over the 256 bins: a scan per wavefront, then the wavefront totals
     uint32_t cx = 0u, cy = 0u;
     if (t < 256) {
@@ -683,31 +713,10 @@ __device__ __forceinline__ void search2Body(const SearchParams& P, int bx0, int)
     else search2BodyR<1>(P, bx0);
 }
 
-// median3.comp:8-28: per-component 3x3 median, replicate border (the shader's partial selection sort)
-__device__ __forceinline__ float med9(float* v) {
-    for (int i = 0; i <= 4; i++)
-        for (int j = i + 1; j < 9; j++)
-            if (v[j] < v[i]) {
-                const float t = v[i];
-                v[i] = v[j];
-                v[j] = t;
-            }
-    return v[4];
-}
-
 __device__ __forceinline__ void median3Body(const Median3Params& P, int gbx, int gby) {
     const int bx = gbx * 16 + (threadIdx.x & 15), by = gby * 16 + (threadIdx.x >> 4);
     if (bx >= P.bw || by >= P.bh) return;
-    float xs[9], ys[9];
-    int k = 0;
-    for (int j = -1; j <= 1; j++)
-        for (int i = -1; i <= 1; i++) {
-            const float2 v = fieldAt(P.in, bx + i, by + j, P.bw, P.bh);
-            xs[k] = v.x;
-            ys[k] = v.y;
-            k++;
-        }
-    P.out[by * P.bw + bx] = make_float2(med9(xs), med9(ys));
+    P.out[by * P.bw + bx] = median3At(P.in, bx, by, P.bw, P.bh);
 }
 
 // Two u16 lanes dotted with two u16 weights plus c (v_dot2_u32_u16): the exact integer form of a bilinear tap row

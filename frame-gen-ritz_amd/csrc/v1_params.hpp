@@ -74,6 +74,8 @@ struct DownParams {
 struct MedParams {
     int gx;             // blocks per grid row (the batched kernels map a linear block index to (bx, by))
     const float2* parent; int pw, ph;   // parent (coarser) block field, median3 output
+    const float2* raw; float2* filtered; // when raw is set: the parent's search field, whose 3x3 median the kernel first
+                                         // writes to filtered (== parent) -- the parent level's median3 pass
     int bw, bh;                         // this level's block grid
     uint32_t* u; int medIdx;
     int shift;                          // -1: bins of [-128, 127] parent pixels (clamped); else the exact two-pass median
