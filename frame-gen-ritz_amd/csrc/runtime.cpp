@@ -612,7 +612,7 @@ void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& pr
         op.stage = 201 + 10 * step;
         op.gx = sp.gx;
         op.set(sp);
-        if (i > 0) {   // this level's median3 runs in the next level's median pass
+        if (i > 0 || st.subpixel) {   // this level's median3 runs in the next level's median pass (level 0: in subpix)
             op.stamp = lvNames[std::min(i, 7)];
             plan.push_back(op);
             continue;
@@ -645,6 +645,8 @@ void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& pr
         q.bw = l0.bw;
         q.bh = l0.bh;
         q.in = F + l0.fM;
+        q.raw = F + l0.fS;   // level 0's median3 is formed here
+        q.filtered = F + l0.fM;
         q.out = F + fieldSub;
         q.lam = st.searchPenalty;
         q.u = u;

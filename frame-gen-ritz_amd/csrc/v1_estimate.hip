@@ -754,7 +754,14 @@ __device__ __forceinline__ void subpixBody(const SubpixParams& P, int g, int) {
     const int nb = P.bw * P.bh;
     if (lane < NB) {
         const int blk = g * NB + lane;
-        const float2 v0 = P.in[blk < nb ? blk : 0];
+        const int b0 = blk < nb ? blk : 0;
+        float2 v0;
+        if (P.raw) {   // (uniform) level 0's median3, formed here for this workgroup's blocks
+            v0 = median3At(P.raw, b0 % P.bw, b0 / P.bw, P.bw, P.bh);
+            if (blk < nb) P.filtered[blk] = v0;
+        } else {
+            v0 = P.in[b0];
+        }
         sv[lane][0] = (int)rintf(v0.x);
         sv[lane][1] = (int)rintf(v0.y);
         const int b = min(blk, nb - 1);
@@ -880,7 +887,14 @@ __device__ __forceinline__ void subpixIntBody(const SubpixParams& P, int g, int)
     const int nb = P.bw * P.bh;
     if (lane < NB) {
         const int blk = g * NB + lane;
-        const float2 v0 = P.in[blk < nb ? blk : 0];
+        const int b0 = blk < nb ? blk : 0;
+        float2 v0;
+        if (P.raw) {   // (uniform) level 0's median3, formed here for this workgroup's blocks
+            v0 = median3At(P.raw, b0 % P.bw, b0 / P.bw, P.bw, P.bh);
+            if (blk < nb) P.filtered[blk] = v0;
+        } else {
+            v0 = P.in[b0];
+        }
         sv[lane][0] = (int)rintf(v0.x);
         sv[lane][1] = (int)rintf(v0.y);
         const int b = min(blk, nb - 1);

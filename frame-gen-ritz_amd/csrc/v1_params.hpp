@@ -105,6 +105,8 @@ struct SubpixParams {
     int gx;             // blocks per grid row (the batched kernels map a linear block index to (bx, by))
     const float* la; const float* lc; int w, h, bw, bh;
     const float2* in; float2* out;
+    const float2* raw; float2* filtered;   // when raw is set: level 0's search field, whose 3x3 median (the median3
+                                           // pass) the kernel forms for its blocks and writes to filtered (== in)
     float lam;
     uint32_t* u;
     int fb;             // as PixParams.fb, for the integer kernel (fb <= 6)
