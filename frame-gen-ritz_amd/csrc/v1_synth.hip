@@ -242,11 +242,11 @@ __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, 
     // LDS in two regions, each holding two arrays whose lifetimes do not overlap: the field tile ft (until the rows
     // are interpolated) then the mismatch es (from the warps to the box), and the interpolated rows hrow (until the
     // warps are done) then the box's row sums hs; a barrier separates each pair
-    constexpr int kRegA = ((FTW * FTH * 8 > EH * EW * 4 ? FTW * FTH * 8 : EH * EW * 4) + 15) / 16 * 16;
+    constexpr int kRegA = ((FTW * FTH * 8 > EH * EWB ? FTW * FTH * 8 : EH * EWB) + 15) / 16 * 16;
     constexpr int kRegB = FTH * EW * 8 > EH * TW * 2 ? FTH * EW * 8 : EH * TW * 2;
     __shared__ __align__(16) uint8_t pool[kRegA + kRegB];
     float2* ft = (float2*)pool;
-    int* es = (int*)pool;
+    uint8_t* es = pool;   // the mismatch, one byte a pixel, rows EWB bytes apart
     float2* hrow = (float2*)(pool + kRegA);
     uint16_t* hs = (uint16_t*)(pool + kRegA);
     __shared__ uint32_t cnt;
@@ -274,7 +274,7 @@ __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, 
         uint32_t* oc = wc + 4 * k;
         warpRegionRun<S, UI>(P, hrow, ty0, kA32, kC32, x0, 4 * rc + AP, y, oa, oc);
 #pragma unroll
-        for (int i = 0; i < 4; i++) es[(rr + 16 * k + AP) * EW + 4 * rc + i + AP] = emaxPacked(oa[i], oc[i]);
+        for (int i = 0; i < 4; i++) es[(rr + 16 * k + AP) * EWB + 4 * rc + i + AP] = (uint8_t)emaxPacked(oa[i], oc[i]);
     }
     // apron in runs of 4 as well: rows 0..2 and EH-3..EH-1 as 18 runs a row (the last at column EW - 4, overlapping
     // its neighbour), then columns 0..3 and EW-4..EW-1 of each row between (overlapping the interior's first and last
@@ -296,10 +296,10 @@ __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, 
         uint32_t oa[4], oc[4];
         warpRegionRun<S, UI>(P, hrow, ty0, kA32, kC32, x0, ex0, y, oa, oc);
 #pragma unroll
-        for (int i = 0; i < 4; i++) es[ey * EW + ex0 + i] = emaxPacked(oa[i], oc[i]);
+        for (int i = 0; i < 4; i++) es[ey * EWB + ex0 + i] = (uint8_t)emaxPacked(oa[i], oc[i]);
     }
     __syncthreads();
-    boxRows7<EH>(es, hs, tid);
+    boxRows7B<EH>(es, hs, tid);
     __syncthreads();
     uint32_t mine = 0u;
 #pragma unroll

@@ -166,6 +166,30 @@ __device__ __forceinline__ void boxRows7(const int* es, uint16_t* hs, int tid) {
     }
 }
 
+// boxRows7 for a mismatch tile of bytes (every e is 0 .. 255) in rows EWB bytes apart: a task's 14 values are one
+// 16-byte read (two aligned 8-byte halves)
+constexpr int EWB = 72;
+template <int EH_>
+__device__ __forceinline__ void boxRows7B(const uint8_t* es, uint16_t* hs, int tid) {
+    static_assert(EWB % 8 == 0 && EWB >= EW + 2 && TW % 8 == 0, "aligned 16-byte reads inside the row");
+    for (int t = tid; t < EH_ * (TW / 8); t += 256) {
+        const int r = t / (TW / 8), c0 = (t % (TW / 8)) * 8;
+        const uint2* e8 = (const uint2*)&es[r * EWB + c0];
+        const uint2 q0 = e8[0], q1 = e8[1];
+        const uint32_t w[4] = {q0.x, q0.y, q1.x, q1.y};
+        int v[14];
+#pragma unroll
+        for (int i = 0; i < 14; i++) v[i] = (int)((w[i >> 2] >> (8 * (i & 3))) & 255u);
+        int h[8];
+        h[0] = v[0] + v[1] + v[2] + v[3] + v[4] + v[5] + v[6];
+#pragma unroll
+        for (int i = 1; i < 8; i++) h[i] = h[i - 1] + v[i + 6] - v[i - 1];
+        uint32_t* o = (uint32_t*)&hs[r * TW + c0];
+#pragma unroll
+        for (int j = 0; j < 4; j++) o[j] = (uint32_t)h[2 * j] | ((uint32_t)h[2 * j + 1] << 16);
+    }
+}
+
 // CrosshairV2 tap rejection (ui_reject.glsl). The footprint of a tap at the 1/32 map (X, Y): RGB the texels X >> 5 .. + 1
 // and Y >> 5 .. + 1, those of weight 0 dropped; 4:2:0 the pixels of the 2x2 chroma texels around it (the chroma point
 // is (X - 16) / 64 texels), up to 4x4 pixels, which hold the luma taps too. A tap hits when its footprint touches a UI
