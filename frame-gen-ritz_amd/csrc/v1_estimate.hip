@@ -1100,6 +1100,7 @@ __device__ __forceinline__ bool sampleRun4(const float* img, int w, int h, int q
 //     positions (q + 1/2) / 8 - 1/2 have one integer part within a run of 4 starting at a multiple of 4);
 //   * candidate 6 (zero): a texel difference (weights 1, 0, 0, 0).
 constexpr int kPfTile = 32, kPfRW = 40, kPfRH = kPfTile + 4, kPfRuns = (kPfRW / 4) * kPfRH;
+constexpr int kPfHsStride = 36;   // words a row of the horizontal sums (16-byte rows; rows 4 apart in other banks)
 constexpr int kPfThreads = 512, kPfBatchThreads = 256;
 
 // the block offset of candidate k = 1..5 (own, above, below, left, right; pixfield.comp:29-33)
@@ -1204,7 +1205,439 @@ __device__ __forceinline__ int hsumAt(int y, int c) {
 // reference's first-minimum rule kept: 0 replaces 1 on a tie (c0 <= c1), every later one needs strict <. When every
 // run of the tile's region reused candidate 1's errors for candidate k (a workgroup vote), every window sum is
 // candidate 1's, so candidate 0 costs c1 - smoothBonus and candidates 2-5 cannot win: the sums are skipped.
-constexpr int kPfFloatPool = kPfRH * kPfRW * 4 + kPfRH * kPfTile * 8 + 64 * 8;   // errs, hsum, block window
+// ---- uniform tiles ----
+// Every candidate vector of the tile's region (its pixels lie in blocks x0 / 8 - 1 .. x0 / 8 + 4, the same rows, and
+// reach one block further in the plus shape of candidates 2-5) is an entry of the 8 x 8 block window other than its
+// four corners. When those 60 entries are one vector V bit for bit, and the bilinear candidate of V with itself is V at
+// every fraction it uses (odd sixteenths, from (q + 1/2) / 8 - 1/2), candidates 0-5 are V at every pixel and their
+// window sums are one sum: candidate 0 costs c1 - smoothBonus, candidates 2-5 cost c1 and (strict <) cannot win. Only
+// V's and the zero vector's errors are formed, and the winner is V or zero. The values are those of the seven-candidate
+// pass. Integer sums: for exact lumas (fb >= 0) every sampleL is a multiple of 2^-(fb + 10) below 2^24 of them.
+constexpr int kPfUW = 36, kPfUH = kPfTile + 4;   // the windows' region: columns x0 - 2 .. x0 + 33, rows y0 - 2 .. y0 + 33
+constexpr int kPfUPlane = kPfUH * kPfUW;
+constexpr int kPfUniPool = 4 * kPfUPlane * 4;    // errors of V and of zero, then their horizontal 5-sums
+
+// the uniform test, by the wavefront that staged the window (lane i holds entry i): returns V's bits in (vx, vy)
+__device__ __forceinline__ bool pixWindowUniform(float2 v, int lane, uint32_t& vx, uint32_t& vy) {
+    vx = (uint32_t)__shfl((int)__float_as_uint(v.x), 9, 64);
+    vy = (uint32_t)__shfl((int)__float_as_uint(v.y), 9, 64);
+    const bool corner = lane == 0 || lane == 7 || lane == 56 || lane == 63;
+    bool ok = corner || (__float_as_uint(v.x) == vx && __float_as_uint(v.y) == vy);
+    if (lane < 8) {
+        const float t = (float)(2 * lane + 1) * (1.0f / 16.0f), Vx = __uint_as_float(vx), Vy = __uint_as_float(vy);
+        ok = ok && mixf(Vx, Vx, t) == Vx && mixf(Vy, Vy, t) == Vy;
+    }
+    return __all(ok);
+}
+
+template <int NT>
+__device__ __forceinline__ void pixfieldUniform(const PixParams& P, int x0, int y0, float2 V, uint32_t* pool) {
+    constexpr int kRows = kPfTile * kPfTile / NT;
+    uint32_t* eu = pool;                   // [2][36][36]: |errors| of V, of zero, in units of 2^-(fb + 10)
+    uint32_t* hu = pool + 2 * kPfUPlane;   // [2][36][36]: their horizontal 5-sums (columns 0 .. 31 of a row)
+    const int tid = threadIdx.x;
+    const float up = __int_as_float((127 + 10 + P.fb) << 23);
+    for (int r = tid; r < kPfUH * (kPfUW / 4); r += NT) {
+        const int ry = r / (kPfUW / 4), rc = r - ry * (kPfUW / 4);
+        const int qy = clampi(y0 - 2 + ry, 0, P.h - 1), gx0 = x0 - 2 + 4 * rc;
+        float sa[4], sc[4], ev[4];
+        const bool inside = gx0 >= 0 && gx0 + 3 <= P.w - 1;
+        if (inside && sampleRun4(P.la, P.w, P.h, gx0, qy, V, -1.0f, sa) && sampleRun4(P.lc, P.w, P.h, gx0, qy, V, 1.0f, sc)) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) ev[i] = fabsf(sa[i] - sc[i]);
+        } else {
+#pragma unroll 1
+            for (int i = 0; i < 4; i++) {
+                const int qx = clampi(gx0 + i, 0, P.w - 1);
+                ev[i] = fabsf(sampleL(P.la, (float)qx - V.x, (float)qy - V.y, P.w, P.h) -
+                              sampleL(P.lc, (float)qx + V.x, (float)qy + V.y, P.w, P.h));
+            }
+        }
+        uint32_t iv[4], iz[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const uint32_t q = idx2(clampi(gx0 + i, 0, P.w - 1), qy, P.w);
+            iv[i] = (uint32_t)(ev[i] * up);
+            iz[i] = (uint32_t)(fabsf(ldIdx(P.la, q) - ldIdx(P.lc, q)) * up);
+        }
+        *(uint4*)&eu[ry * kPfUW + 4 * rc] = make_uint4(iv[0], iv[1], iv[2], iv[3]);
+        *(uint4*)&eu[kPfUPlane + ry * kPfUW + 4 * rc] = make_uint4(iz[0], iz[1], iz[2], iz[3]);
+    }
+    __syncthreads();
+    // horizontal 5-sums: output column c of a row sums region columns c .. c + 4; four outputs a task (two quads)
+    for (int t = tid; t < 2 * kPfUH * (kPfTile / 4); t += NT) {
+        const int pl = t >= kPfUH * (kPfTile / 4) ? 1 : 0, tt = t - pl * kPfUH * (kPfTile / 4);
+        const int hy = tt >> 3, c0 = (tt & 7) * 4;
+        const uint4* e4 = (const uint4*)&eu[pl * kPfUPlane + hy * kPfUW + c0];
+        const uint4 q0 = e4[0], q1 = e4[1];
+        uint4 h;
+        h.x = q0.x + q0.y + q0.z + q0.w + q1.x;
+        h.y = h.x + q1.y - q0.x;
+        h.z = h.y + q1.z - q0.y;
+        h.w = h.z + q1.w - q0.z;
+        *(uint4*)&hu[pl * kPfUPlane + hy * kPfUW + c0] = h;
+    }
+    __syncthreads();
+    const double cscale = (1.0 / 25.0) * (double)__int_as_float((127 - 10 - P.fb) << 23);
+    const int cx = tid & 31, r0 = (tid >> 5) * kRows;
+    uint32_t hv[kRows + 4], hz[kRows + 4];
+#pragma unroll
+    for (int y = 0; y < kRows + 4; y++) {
+        hv[y] = hu[(r0 + y) * kPfUW + cx];
+        hz[y] = hu[kPfUPlane + (r0 + y) * kPfUW + cx];
+    }
+    uint32_t sv = hv[0] + hv[1] + hv[2] + hv[3] + hv[4], sz = hz[0] + hz[1] + hz[2] + hz[3] + hz[4];
+    const int qx = x0 + cx;
+    if (qx >= P.w) return;
+#pragma unroll
+    for (int j = 0; j < kRows; j++) {
+        if (j > 0) {
+            sv = sv + hv[j + 4] - hv[j - 1];
+            sz = sz + hz[j + 4] - hz[j - 1];
+        }
+        const int py = y0 + r0 + j;
+        if (py >= P.h) break;
+        // the seven-candidate order: 1, then 0 on c0 <= best, then 2-5 (never below c1), then 6 on strict <
+        const float c1 = (float)((double)sv * cscale);
+        float best = c1;
+        const float c0 = c1 - P.smooth;
+        if (c0 <= best) best = c0;
+        const float c6 = (float)((double)sz * cscale) - P.zero;
+        P.pf[py * P.w + qx] = c6 < best ? make_float2(0.0f, 0.0f) : V;
+    }
+}
+
+// sampleRun4 in two steps, so that a caller issues the loads of both frames (and more) before any arithmetic: the plan
+// (the maps, their check, the taps' row and weights), then the sums over the 5 + 5 texels it loads
+struct RunPlan {
+    uint32_t row;   // element index of the first texel of the top tap row
+    float w00, w01, w10, w11;
+    bool ok;
+};
+
+__device__ __forceinline__ RunPlan runPlan(int w, int h, int qx, int qy, float2 v, float sign) {
+    const float px0 = (float)qx + sign * v.x, px3 = (float)(qx + 3) + sign * v.x, py = (float)qy + sign * v.y;
+    const int X0 = (int)rintf(px0 * 32.0f), X3 = (int)rintf(px3 * 32.0f), Y = (int)rintf(py * 32.0f);
+    const int ix = X0 >> 5, iy = Y >> 5;
+    RunPlan p;
+    p.ok = X3 == X0 + 96 && ix >= 0 && ix + 4 <= w - 1 && iy >= 0 && iy + 1 <= h - 1;
+    const float fx = (float)(X0 & 31) * (1.0f / 32.0f), fy = (float)(Y & 31) * (1.0f / 32.0f);
+    p.w00 = (1.0f - fy) * (1.0f - fx);
+    p.w01 = (1.0f - fy) * fx;
+    p.w10 = fy * (1.0f - fx);
+    p.w11 = fy * fx;
+    p.row = p.ok ? idx2(ix, iy, w) : 0u;
+    return p;
+}
+
+__device__ __forceinline__ void runSums(const float* a, const float* b, const RunPlan& p, float* out) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) out[i] = a[i] * p.w00 + a[i + 1] * p.w01 + b[i] * p.w10 + b[i + 1] * p.w11;
+}
+
+// |sampleL(a, q - v_i) - sampleL(c, q + v_i)| at the pixels qx_i of row qy, each with its own vector (the loads of all
+// four pixels are issued together)
+__device__ __forceinline__ void pixErr4(const PixParams& P, const int* qx, int qy, const float2* v, float* ev) {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+        ev[i] = fabsf(sampleL(P.la, (float)qx[i] - v[i].x, (float)qy - v[i].y, P.w, P.h) -
+                      sampleL(P.lc, (float)qx[i] + v[i].x, (float)qy + v[i].y, P.w, P.h));
+}
+
+// |sampleL(a, q - v) - sampleL(c, q + v)| at the 4 pixels gx0 .. gx0 + 3 of row qy (inside the plane), one vector
+__device__ __forceinline__ void pixRunErrs(const PixParams& P, int gx0, int qy, float2 v, float* ev) {
+    const RunPlan pa = runPlan(P.w, P.h, gx0, qy, v, -1.0f), pc = runPlan(P.w, P.h, gx0, qy, v, 1.0f);
+    if (pa.ok && pc.ok) {
+        const float* ra = P.la + pa.row;
+        const float* rc = P.lc + pc.row;
+        float a0[5], a1[5], c0[5], c1[5];
+#pragma unroll
+        for (int i = 0; i < 5; i++) {
+            a0[i] = ra[i];
+            a1[i] = ra[P.w + i];
+            c0[i] = rc[i];
+            c1[i] = rc[P.w + i];
+        }
+        float sa[4], sc[4];
+        runSums(a0, a1, pa, sa);
+        runSums(c0, c1, pc, sc);
+#pragma unroll
+        for (int i = 0; i < 4; i++) ev[i] = fabsf(sa[i] - sc[i]);
+    } else {
+        const int qx[4] = {gx0, gx0 + 1, gx0 + 2, gx0 + 3};
+        const float2 vv[4] = {v, v, v, v};
+        pixErr4(P, qx, qy, vv, ev);
+    }
+}
+
+__device__ __forceinline__ float pixErr1(const PixParams& P, int qx, int qy, float2 v) {
+    return fabsf(sampleL(P.la, (float)qx - v.x, (float)qy - v.y, P.w, P.h) -
+                 sampleL(P.lc, (float)qx + v.x, (float)qy + v.y, P.w, P.h));
+}
+
+__device__ __forceinline__ bool sameVec(float2 a, float2 b) { return a.x == b.x && a.y == b.y; }
+
+// The errors of candidate k (0 or 2-5) on the run of 4 pixels gx0 .. gx0 + 3 of row qy
+__device__ __forceinline__ void pixCandRunErrs(const PixParams& P, const BlockWin& bw, int k, int gx0, int qy, float* ev) {
+    if (gx0 >= 0 && gx0 + 3 <= P.w - 1) {
+        if (k > 0) {
+            const int2 o = candOffset(k);
+            pixRunErrs(P, gx0, qy, bw.at((gx0 >> 3) + o.x, (qy >> 3) + o.y), ev);   // one block, one vector
+            return;
+        }
+        const float sx0 = ((float)gx0 + 0.5f) / 8.0f - 0.5f, sy = ((float)qy + 0.5f) / 8.0f - 0.5f;
+        const float flx = floorf(sx0), fly = floorf(sy);
+        const float fy = sy - fly;
+        const int ix = (int)flx, iy = (int)fly;
+        const float2 f00 = bw.at(ix, iy), f10 = bw.at(ix + 1, iy), f01 = bw.at(ix, iy + 1), f11 = bw.at(ix + 1, iy + 1);
+        float2 v[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const float fx = (((float)(gx0 + i) + 0.5f) / 8.0f - 0.5f) - flx;
+            v[i] = mix2(mix2(f00, f10, fx), mix2(f01, f11, fx), fy);
+        }
+        if (sameVec(v[1], v[0]) && sameVec(v[2], v[0]) && sameVec(v[3], v[0])) {
+            pixRunErrs(P, gx0, qy, v[0], ev);
+        } else {
+            const int qx[4] = {gx0, gx0 + 1, gx0 + 2, gx0 + 3};
+            pixErr4(P, qx, qy, v, ev);
+        }
+        return;
+    }
+    int qx[4];
+    float2 v[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        qx[i] = clampi(gx0 + i, 0, P.w - 1);
+        v[i] = pixCand(k, qx[i], qy, bw);
+    }
+    pixErr4(P, qx, qy, v, ev);
+}
+
+// The seven-candidate pass for exact lumas (fb >= 0): errors as integers in units of 2^-(fb + 10), window sums in 32
+// bits, each cost formed as the float path forms it. Candidates 1 and 6 are always formed, together (their errors, then
+// their sums). A run of 4 pixels whose candidate k (0 or 2-5) is candidate 1's vector at each of its pixels has
+// candidate 1's errors for k (the same floats) and copies them; a candidate that every run of the region copies has
+// candidate 1's window sums, so it is decided without a pass of its own (0 costs c1 - smoothBonus, 2-5 cannot win).
+// The others get a pass each, in the reference's order 0, 2, 3, 4, 5; candidate 6 is compared last (strict <). The runs
+// a pass forms (rather than copies) are listed per candidate in LDS when candidate 1 is formed, and a pass deals its list
+// to the threads densely: a tile with one odd block forms a few runs, and no wavefront runs the copy and the forming
+// paths side by side.
+constexpr int kPfCandLists = 5;   // candidates 0, 2, 3, 4, 5
+constexpr int kPfExactPool = 2 * kPfRH * kPfRW * 4 + 2 * kPfRH * kPfHsStride * 4 + kPfCandLists * kPfRuns * 2;
+
+// list slot of candidate k (0, 2-5)
+__device__ __forceinline__ int candList(int k) { return k == 0 ? 0 : k - 1; }
+
+template <int NT>
+__device__ __forceinline__ void pixfieldExact(const PixParams& P, int x0, int y0, const BlockWin& bw, uint8_t* pool,
+                                              uint32_t* cnt) {
+    constexpr int kRows = kPfTile * kPfTile / NT;
+    constexpr int kRunsRow = kPfRW / 4;
+    uint32_t* e1 = (uint32_t*)pool;              // [36][40] candidate 1's errors (kept)
+    uint32_t* ek = e1 + kPfRH * kPfRW;           // [36][40] candidate 6's errors, then candidate k's
+    uint32_t* h1 = ek + kPfRH * kPfRW;           // horizontal 5-sums of e1, rows kPfHsStride apart
+    uint32_t* hk = h1 + kPfRH * kPfHsStride;     // ... of ek
+    uint16_t* lists = (uint16_t*)(hk + kPfRH * kPfHsStride);   // [5][360] runs a candidate's pass forms
+    const int tid = threadIdx.x;
+    const float up = __int_as_float((127 + 10 + P.fb) << 23);
+    const double cscale = (1.0 / 25.0) * (double)__int_as_float((127 - 10 - P.fb) << 23);
+    static_assert((kPfRuns + NT - 1) / NT <= 2, "at most two runs a thread");
+    uint32_t reuse0 = 0u, reuse1 = 0u;   // bit k: the run's candidate k is candidate 1 at every pixel
+    for (int r = tid; r < kPfRuns; r += NT) {
+        const int ry = r / kRunsRow, rc = r - ry * kRunsRow;
+        const int qy = clampi(y0 - 2 + ry, 0, P.h - 1), gx0 = x0 - 4 + 4 * rc;
+        float ev[4], za[4], zc[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {   // the zero vector's texels, loaded first
+            const uint32_t q = idx2(clampi(gx0 + i, 0, P.w - 1), qy, P.w);
+            za[i] = ldIdx(P.la, q);
+            zc[i] = ldIdx(P.lc, q);
+        }
+        uint32_t bits = 0u;
+        if (gx0 >= 0 && gx0 + 3 <= P.w - 1) {
+            const int bx = gx0 >> 3, by = qy >> 3;
+            const float2 v1 = bw.at(bx, by);
+            pixRunErrs(P, gx0, qy, v1, ev);
+#pragma unroll
+            for (int k = 2; k <= 5; k++) {
+                const int2 o = candOffset(k);
+                bits |= sameVec(bw.at(bx + o.x, by + o.y), v1) ? 1u << k : 0u;
+            }
+            const float sx0 = ((float)gx0 + 0.5f) / 8.0f - 0.5f, sy = ((float)qy + 0.5f) / 8.0f - 0.5f;
+            const float flx = floorf(sx0), fly = floorf(sy);
+            const float fy = sy - fly;
+            const int ix = (int)flx, iy = (int)fly;
+            const float2 f00 = bw.at(ix, iy), f10 = bw.at(ix + 1, iy), f01 = bw.at(ix, iy + 1), f11 = bw.at(ix + 1, iy + 1);
+            bool same0 = true;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const float fx = (((float)(gx0 + i) + 0.5f) / 8.0f - 0.5f) - flx;
+                same0 = same0 && sameVec(mix2(mix2(f00, f10, fx), mix2(f01, f11, fx), fy), v1);
+            }
+            bits |= same0 ? 1u : 0u;
+        } else {   // a run across the frame edge: pixel by pixel at the clamped column
+            uint32_t all = 0x3Du;
+            int qx[4];
+            float2 v1[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                qx[i] = clampi(gx0 + i, 0, P.w - 1);
+                v1[i] = pixCand(1, qx[i], qy, bw);
+                uint32_t b = 0u;
+#pragma unroll
+                for (int k = 0; k <= 5; k++)
+                    if (k != 1) b |= sameVec(pixCand(k, qx[i], qy, bw), v1[i]) ? 1u << k : 0u;
+                all &= b;
+            }
+            pixErr4(P, qx, qy, v1, ev);
+            bits = all;
+        }
+        uint32_t i1[4], i6[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            i1[i] = (uint32_t)(ev[i] * up);
+            i6[i] = (uint32_t)(fabsf(za[i] - zc[i]) * up);
+        }
+        *(uint4*)&e1[ry * kPfRW + 4 * rc] = make_uint4(i1[0], i1[1], i1[2], i1[3]);
+        *(uint4*)&ek[ry * kPfRW + 4 * rc] = make_uint4(i6[0], i6[1], i6[2], i6[3]);
+        if (r >= NT) reuse1 = bits;
+        else reuse0 = bits;
+    }
+    // the lists: per candidate, a wavefront's runs that do not copy, placed by a ballot and one LDS atomic
+    const int lane = tid & 63;
+    const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int slot = 0; slot < (kPfRuns + NT - 1) / NT; slot++) {
+        const int r = tid + slot * NT;
+        const uint32_t bits = slot ? reuse1 : reuse0;
+#pragma unroll
+        for (int l = 0; l < kPfCandLists; l++) {
+            const int k = l == 0 ? 0 : l + 1;
+            const bool forms = r < kPfRuns && !((bits >> k) & 1u);
+            const uint64_t m = __ballot(forms);
+            if (m == 0ull) continue;   // (uniform)
+            uint32_t base = 0u;
+            if (lane == 0) base = atomicAdd(&cnt[l], (uint32_t)__popcll(m));
+            base = (uint32_t)__shfl((int)base, 0, 64);
+            if (forms) lists[l * kPfRuns + base + (uint32_t)__popcll(m & below)] = (uint16_t)r;
+        }
+    }
+    __syncthreads();
+    // horizontal 5-sums of both planes: region columns c0 + 2 .. c0 + 9 of a row (three aligned quads), four a task
+    auto hsums = [&](const uint32_t* e, uint32_t* h, int t) {
+        const int hy = t >> 3, c0 = (t & 7) * 4;
+        const uint4* e4 = (const uint4*)&e[hy * kPfRW + c0];
+        const uint4 q0 = e4[0], q1 = e4[1], q2 = e4[2];
+        uint4 s;
+        s.x = q0.z + q0.w + q1.x + q1.y + q1.z;
+        s.y = s.x + q1.w - q0.z;
+        s.z = s.y + q2.x - q0.w;
+        s.w = s.z + q2.y - q1.x;
+        *(uint4*)&h[hy * kPfHsStride + c0] = s;
+    };
+    constexpr int kTasks = kPfRH * (kPfTile / 4);
+    for (int t = tid; t < 2 * kTasks; t += NT) {
+        if (t < kTasks) hsums(e1, h1, t);
+        else hsums(ek, hk, t - kTasks);
+    }
+    __syncthreads();
+    const int cx = tid & 31, r0 = (tid >> 5) * kRows;
+    float best[kRows], c6[kRows];
+    int bk[kRows];
+    {
+        uint32_t a[kRows + 4], b[kRows + 4];
+#pragma unroll
+        for (int y = 0; y < kRows + 4; y++) {
+            a[y] = h1[(r0 + y) * kPfHsStride + cx];
+            b[y] = hk[(r0 + y) * kPfHsStride + cx];
+        }
+        uint32_t sa = a[0] + a[1] + a[2] + a[3] + a[4], sb = b[0] + b[1] + b[2] + b[3] + b[4];
+#pragma unroll
+        for (int j = 0; j < kRows; j++) {
+            if (j > 0) {
+                sa = sa + a[j + 4] - a[j - 1];
+                sb = sb + b[j + 4] - b[j - 1];
+            }
+            best[j] = (float)((double)sa * cscale);
+            bk[j] = 1;
+            c6[j] = (float)((double)sb * cscale) - P.zero;
+        }
+    }
+#pragma unroll 1
+    for (int step = 0; step < 5; step++) {
+        const int k = step == 0 ? 0 : step + 1;
+        const int nk = (int)cnt[step];
+        if (nk == 0) {   // candidate 1's sums: 0 costs c1 - smoothBonus (best is c1 here), 2-5 cannot win
+            if (k == 0) {
+#pragma unroll
+                for (int j = 0; j < kRows; j++) {
+                    const float c = best[j] - P.smooth;
+                    if (c <= best[j]) {
+                        best[j] = c;
+                        bk[j] = 0;
+                    }
+                }
+            }
+            continue;
+        }
+        // the copies, then the listed runs (disjoint runs: no barrier between)
+        for (int r = tid; r < kPfRuns; r += NT) {
+            const int ry = r / kRunsRow, rc = r - ry * kRunsRow;
+            if ((((r >= NT) ? reuse1 : reuse0) >> k) & 1u)
+                *(uint4*)&ek[ry * kPfRW + 4 * rc] = *(const uint4*)&e1[ry * kPfRW + 4 * rc];
+        }
+        for (int e = tid; e < nk; e += NT) {
+            const int r = lists[step * kPfRuns + e];
+            const int ry = r / kRunsRow, rc = r - ry * kRunsRow;
+            float ev[4];
+            pixCandRunErrs(P, bw, k, x0 - 4 + 4 * rc, clampi(y0 - 2 + ry, 0, P.h - 1), ev);
+            *(uint4*)&ek[ry * kPfRW + 4 * rc] = make_uint4((uint32_t)(ev[0] * up), (uint32_t)(ev[1] * up),
+                                                           (uint32_t)(ev[2] * up), (uint32_t)(ev[3] * up));
+        }
+        __syncthreads();
+        for (int t = tid; t < kTasks; t += NT) hsums(ek, hk, t);
+        __syncthreads();
+        uint32_t a[kRows + 4];
+#pragma unroll
+        for (int y = 0; y < kRows + 4; y++) a[y] = hk[(r0 + y) * kPfHsStride + cx];
+        uint32_t s = a[0] + a[1] + a[2] + a[3] + a[4];
+#pragma unroll
+        for (int j = 0; j < kRows; j++) {
+            if (j > 0) s = s + a[j + 4] - a[j - 1];
+            float c = (float)((double)s * cscale);
+            if (k == 0) c = c - P.smooth;
+            if (k == 0 ? c <= best[j] : c < best[j]) {
+                best[j] = c;
+                bk[j] = k;
+            }
+        }
+    }
+    const int qx = x0 + cx;
+    if (qx >= P.w) return;
+#pragma unroll
+    for (int j = 0; j < kRows; j++) {
+        const int py = y0 + r0 + j;
+        if (py >= P.h) break;
+        int k = bk[j];
+        if (c6[j] < best[j]) k = 6;
+        float2 v;
+        if (k == 0) {
+            v = pixCand(0, qx, py, bw);
+        } else if (k == 6) {
+            v = make_float2(0.0f, 0.0f);
+        } else {
+            const int2 o = candOffset(k);
+            v = bw.at(qx / kBlock + o.x, py / kBlock + o.y);
+        }
+        P.pf[py * P.w + qx] = v;
+    }
+}
+
+constexpr int kPfFloatWork = kPfRH * kPfRW * 4 + kPfRH * kPfTile * 8;   // errs, hsum
+constexpr int kPfWork0 = kPfFloatWork > kPfUniPool ? kPfFloatWork : kPfUniPool;
+constexpr int kPfWork = kPfWork0 > kPfExactPool ? kPfWork0 : kPfExactPool;
+constexpr int kPfFloatPool = kPfWork + 64 * 8 + 32;   // + the block window, the uniform flag and the list lengths
 
 template <int NT>
 __device__ __forceinline__ void pixfieldTile(const PixParams& P, int gbx, int gby, uint8_t* pool) {
@@ -1212,12 +1645,29 @@ __device__ __forceinline__ void pixfieldTile(const PixParams& P, int gbx, int gb
     constexpr int kRows = kPfTile * kPfTile / NT;      // pixels a thread: column cx, rows r0 .. r0 + kRows - 1
     double* hsum = (double*)pool;
     float* errs = (float*)(pool + kPfRH * kPfTile * 8);
-    float2* bwin = (float2*)(pool + kPfRH * kPfTile * 8 + kPfRH * kPfRW * 4);
+    float2* bwin = (float2*)(pool + kPfWork);
+    int* uni = (int*)(pool + kPfWork + 64 * 8);
+    uint32_t* cnt = (uint32_t*)(pool + kPfWork + 64 * 8 + 4);   // [5] list lengths of pixfieldExact
     const int tid = threadIdx.x;
     const int x0 = gbx * kPfTile, y0 = gby * kPfTile;
     const BlockWin bw{bwin, x0 / kBlock - 2, y0 / kBlock - 2};
-    if (tid < 64) bwin[tid] = fieldAt(P.bf, bw.bx0 + (tid & 7), bw.by0 + (tid >> 3), P.bw, P.bh);
+    if (tid < 64) {
+        const float2 v = fieldAt(P.bf, bw.bx0 + (tid & 7), bw.by0 + (tid >> 3), P.bw, P.bh);
+        bwin[tid] = v;
+        uint32_t vx, vy;
+        const bool u = pixWindowUniform(v, tid, vx, vy);
+        if (tid == 0) *uni = u && P.fb >= 0;
+        if (tid < kPfCandLists) cnt[tid] = 0u;
+    }
     __syncthreads();
+    if (*uni) {   // (uniform over the workgroup)
+        pixfieldUniform<NT>(P, x0, y0, bwin[9], (uint32_t*)pool);
+        return;
+    }
+    if (P.fb >= 0) {
+        pixfieldExact<NT>(P, x0, y0, bw, pool, cnt);
+        return;
+    }
     const int cx = tid & 31, r0 = (tid >> 5) * kRows;
     float best[kRows], c1[kRows];
     int bk[kRows];
@@ -1352,7 +1802,6 @@ __device__ __forceinline__ void pixfieldBody(const PixParams& P, int gbx, int gb
 // the clamped global tap. A tile whose candidates reach further than kPfWinR flow px runs the float path (pixfieldTile).
 constexpr int kPfWinR = 7;
 constexpr int kPfWinWords = 42 + 2 * kPfWinR, kPfWinRows = 39 + 2 * kPfWinR;   // at R: 42 + 2R words, 39 + 2R rows
-constexpr int kPfHsStride = 36;   // words a row of the horizontal sums (16-byte rows; rows 4 apart in other banks)
 
 struct PixWin {
     const uint32_t* a;
@@ -1470,7 +1919,7 @@ __device__ __forceinline__ void pixfieldWinBody(const PixParams& P, int gbx, int
     constexpr int kPool = kIntPool > kPfFloatPool ? kIntPool : kPfFloatPool;
     __shared__ __align__(16) uint8_t pool[kPool];
     __shared__ float2 bwin[64];
-    __shared__ int winReach;
+    __shared__ int winReach, winUni;
     const int tid = threadIdx.x;
     const int x0 = gbx * kPfTile, y0 = gby * kPfTile;
     const BlockWin bw{bwin, x0 / kBlock - 2, y0 / kBlock - 2};
@@ -1480,9 +1929,18 @@ __device__ __forceinline__ void pixfieldWinBody(const PixParams& P, int gbx, int
         float m = fmaxf(fabsf(v.x), fabsf(v.y));
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-        if (tid == 0) winReach = (int)fminf(floorf(m) + 1.0f, (float)(kPfWinR + 1));
+        uint32_t vx, vy;
+        const bool u = pixWindowUniform(v, tid, vx, vy);
+        if (tid == 0) {
+            winReach = (int)fminf(floorf(m) + 1.0f, (float)(kPfWinR + 1));
+            winUni = u;
+        }
     }
     __syncthreads();
+    if (winUni) {   // one vector throughout: its errors and zero's, no windows
+        pixfieldUniform<NT>(P, x0, y0, bwin[9], (uint32_t*)pool);
+        return;
+    }
     const int R = winReach;
     if (R > kPfWinR) {   // (uniform) a long candidate: the float path, its LDS in the same pool
         pixfieldTile<NT>(P, gbx, gby, pool);
