@@ -1205,106 +1205,52 @@ __device__ __forceinline__ int hsumAt(int y, int c) {
 // reference's first-minimum rule kept: 0 replaces 1 on a tie (c0 <= c1), every later one needs strict <. When every
 // run of the tile's region reused candidate 1's errors for candidate k (a workgroup vote), every window sum is
 // candidate 1's, so candidate 0 costs c1 - smoothBonus and candidates 2-5 cannot win: the sums are skipped.
-// ---- uniform tiles ----
-// Every candidate vector of the tile's region (its pixels lie in blocks x0 / 8 - 1 .. x0 / 8 + 4, the same rows, and
-// reach one block further in the plus shape of candidates 2-5) is an entry of the 8 x 8 block window other than its
-// four corners. When those 60 entries are one vector V bit for bit, and the bilinear candidate of V with itself is V at
-// every fraction it uses (odd sixteenths, from (q + 1/2) / 8 - 1/2), candidates 0-5 are V at every pixel and their
-// window sums are one sum: candidate 0 costs c1 - smoothBonus, candidates 2-5 cost c1 and (strict <) cannot win. Only
-// V's and the zero vector's errors are formed, and the winner is V or zero. The values are those of the seven-candidate
-// pass. Integer sums: for exact lumas (fb >= 0) every sampleL is a multiple of 2^-(fb + 10) below 2^24 of them.
-constexpr int kPfUW = 36, kPfUH = kPfTile + 4;   // the windows' region: columns x0 - 2 .. x0 + 33, rows y0 - 2 .. y0 + 33
-constexpr int kPfUPlane = kPfUH * kPfUW;
-constexpr int kPfUniPool = 4 * kPfUPlane * 4;    // errors of V and of zero, then their horizontal 5-sums
+// ---- the pixel field in exact integers, from LDS windows of the two planes ----
+// For a level-0 luma of values k 2^-fb with 0 <= k < 2^14 (RGBA8 at every flowScale; PixParams.fb) every sampleL is
+// exact in float32: a tap k m / 1024 (m <= 1024) has at most 14 + 11 significant bits, and the sum, a convex combination,
+// at most 14 + 10 bits; so is the error |a - c|, and every window sum of 25 errors fits 32 bits. In integers (units of
+// 2^-(fb + 10)) a sample is the sum of t m over its taps (t = k): one v_dot2_u32_u16 per tap row over the texel pair with
+// the weights (32 - fx, fx), then the vertical blend; the error is one v_sad_u32. The window sum S gives the cost
+// float32(S 2^-(fb + 10) (1/25)), the float32 rounding of cv2.blur's float64 mean, as the float path computes it.
+//
+// The texels come from LDS windows of both planes covering the tile's region plus the reach R of its largest candidate
+// vector: word x of a window row is the texel pair t[x] | t[x + 1] << 16 of the edge-clamped plane, so a window tap is
+// the clamped global tap. A tile whose candidates reach further than kPfWinR flow px samples the planes in global memory.
+// Bounds: a region pixel q (columns x0 - 4 .. x0 + 35, rows y0 - 2 .. y0 + 33) is sampled at q -+ v with |v| < R, so
+// its taps' words lie in columns x0 - 4 - R .. x0 + 35 + R and rows y0 - 2 - R .. y0 + 34 + R of the plane, inside the
+// windows' columns x0 - 5 - R .. x0 + 36 + R and rows y0 - 3 - R .. y0 + 35 + R.
+constexpr int kPfWinR = 7;
+constexpr int kPfWinWords = 42 + 2 * kPfWinR, kPfWinRows = 39 + 2 * kPfWinR;   // at R: 42 + 2R words, 39 + 2R rows
+constexpr int kPfWinPool = 2 * kPfWinWords * kPfWinRows * 4;
 
-// the uniform test, by the wavefront that staged the window (lane i holds entry i): returns V's bits in (vx, vy)
-__device__ __forceinline__ bool pixWindowUniform(float2 v, int lane, uint32_t& vx, uint32_t& vy) {
-    vx = (uint32_t)__shfl((int)__float_as_uint(v.x), 9, 64);
-    vy = (uint32_t)__shfl((int)__float_as_uint(v.y), 9, 64);
-    const bool corner = lane == 0 || lane == 7 || lane == 56 || lane == 63;
-    bool ok = corner || (__float_as_uint(v.x) == vx && __float_as_uint(v.y) == vy);
-    if (lane < 8) {
-        const float t = (float)(2 * lane + 1) * (1.0f / 16.0f), Vx = __uint_as_float(vx), Vy = __uint_as_float(vy);
-        ok = ok && mixf(Vx, Vx, t) == Vx && mixf(Vy, Vy, t) == Vy;
-    }
-    return __all(ok);
+struct PixWin {
+    const uint32_t* a;
+    const uint32_t* c;
+    int x0, y0, stride;   // plane coordinates of the windows' word 0, words a row
+};
+
+// sampleL at the 1/32 map (X, Y), in units of 2^-(fb + 10)
+__device__ __forceinline__ uint32_t winSample(const uint32_t* w, const PixWin& W, int X, int Y) {
+    const uint32_t fx = (uint32_t)(X & 31), fy = (uint32_t)(Y & 31);
+    const uint32_t* p = w + ((Y >> 5) - W.y0) * W.stride + ((X >> 5) - W.x0);
+    const uint32_t wx = (32u - fx) | (fx << 16);
+    return pairDot(p[0], wx) * (32u - fy) + pairDot(p[W.stride], wx) * fy;
 }
 
-template <int NT>
-__device__ __forceinline__ void pixfieldUniform(const PixParams& P, int x0, int y0, float2 V, uint32_t* pool) {
-    constexpr int kRows = kPfTile * kPfTile / NT;
-    uint32_t* eu = pool;                   // [2][36][36]: |errors| of V, of zero, in units of 2^-(fb + 10)
-    uint32_t* hu = pool + 2 * kPfUPlane;   // [2][36][36]: their horizontal 5-sums (columns 0 .. 31 of a row)
-    const int tid = threadIdx.x;
-    const float up = __int_as_float((127 + 10 + P.fb) << 23);
-    for (int r = tid; r < kPfUH * (kPfUW / 4); r += NT) {
-        const int ry = r / (kPfUW / 4), rc = r - ry * (kPfUW / 4);
-        const int qy = clampi(y0 - 2 + ry, 0, P.h - 1), gx0 = x0 - 2 + 4 * rc;
-        float sa[4], sc[4], ev[4];
-        const bool inside = gx0 >= 0 && gx0 + 3 <= P.w - 1;
-        if (inside && sampleRun4(P.la, P.w, P.h, gx0, qy, V, -1.0f, sa) && sampleRun4(P.lc, P.w, P.h, gx0, qy, V, 1.0f, sc)) {
+// the four samples at X0 + 32 i, Y (one fractional part, 5 texels a row)
+__device__ __forceinline__ void winSample4(const uint32_t* w, const PixWin& W, int X0, int Y, uint32_t* out) {
+    const uint32_t fx = (uint32_t)(X0 & 31), fy = (uint32_t)(Y & 31);
+    const uint32_t* p = w + ((Y >> 5) - W.y0) * W.stride + ((X0 >> 5) - W.x0);
+    const uint32_t wx = (32u - fx) | (fx << 16);
 #pragma unroll
-            for (int i = 0; i < 4; i++) ev[i] = fabsf(sa[i] - sc[i]);
-        } else {
-#pragma unroll 1
-            for (int i = 0; i < 4; i++) {
-                const int qx = clampi(gx0 + i, 0, P.w - 1);
-                ev[i] = fabsf(sampleL(P.la, (float)qx - V.x, (float)qy - V.y, P.w, P.h) -
-                              sampleL(P.lc, (float)qx + V.x, (float)qy + V.y, P.w, P.h));
-            }
-        }
-        uint32_t iv[4], iz[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const uint32_t q = idx2(clampi(gx0 + i, 0, P.w - 1), qy, P.w);
-            iv[i] = (uint32_t)(ev[i] * up);
-            iz[i] = (uint32_t)(fabsf(ldIdx(P.la, q) - ldIdx(P.lc, q)) * up);
-        }
-        *(uint4*)&eu[ry * kPfUW + 4 * rc] = make_uint4(iv[0], iv[1], iv[2], iv[3]);
-        *(uint4*)&eu[kPfUPlane + ry * kPfUW + 4 * rc] = make_uint4(iz[0], iz[1], iz[2], iz[3]);
-    }
-    __syncthreads();
-    // horizontal 5-sums: output column c of a row sums region columns c .. c + 4; four outputs a task (two quads)
-    for (int t = tid; t < 2 * kPfUH * (kPfTile / 4); t += NT) {
-        const int pl = t >= kPfUH * (kPfTile / 4) ? 1 : 0, tt = t - pl * kPfUH * (kPfTile / 4);
-        const int hy = tt >> 3, c0 = (tt & 7) * 4;
-        const uint4* e4 = (const uint4*)&eu[pl * kPfUPlane + hy * kPfUW + c0];
-        const uint4 q0 = e4[0], q1 = e4[1];
-        uint4 h;
-        h.x = q0.x + q0.y + q0.z + q0.w + q1.x;
-        h.y = h.x + q1.y - q0.x;
-        h.z = h.y + q1.z - q0.y;
-        h.w = h.z + q1.w - q0.z;
-        *(uint4*)&hu[pl * kPfUPlane + hy * kPfUW + c0] = h;
-    }
-    __syncthreads();
-    const double cscale = (1.0 / 25.0) * (double)__int_as_float((127 - 10 - P.fb) << 23);
-    const int cx = tid & 31, r0 = (tid >> 5) * kRows;
-    uint32_t hv[kRows + 4], hz[kRows + 4];
-#pragma unroll
-    for (int y = 0; y < kRows + 4; y++) {
-        hv[y] = hu[(r0 + y) * kPfUW + cx];
-        hz[y] = hu[kPfUPlane + (r0 + y) * kPfUW + cx];
-    }
-    uint32_t sv = hv[0] + hv[1] + hv[2] + hv[3] + hv[4], sz = hz[0] + hz[1] + hz[2] + hz[3] + hz[4];
-    const int qx = x0 + cx;
-    if (qx >= P.w) return;
-#pragma unroll
-    for (int j = 0; j < kRows; j++) {
-        if (j > 0) {
-            sv = sv + hv[j + 4] - hv[j - 1];
-            sz = sz + hz[j + 4] - hz[j - 1];
-        }
-        const int py = y0 + r0 + j;
-        if (py >= P.h) break;
-        // the seven-candidate order: 1, then 0 on c0 <= best, then 2-5 (never below c1), then 6 on strict <
-        const float c1 = (float)((double)sv * cscale);
-        float best = c1;
-        const float c0 = c1 - P.smooth;
-        if (c0 <= best) best = c0;
-        const float c6 = (float)((double)sz * cscale) - P.zero;
-        P.pf[py * P.w + qx] = c6 < best ? make_float2(0.0f, 0.0f) : V;
-    }
+    for (int i = 0; i < 4; i++) out[i] = pairDot(p[i], wx) * (32u - fy) + pairDot(p[W.stride + i], wx) * fy;
+}
+
+// |sampleL(a, q - v) - sampleL(c, q + v)| with the maps formed as sampleL forms them
+__device__ __forceinline__ uint32_t winErr(const PixWin& W, int qx, int qy, float2 v) {
+    const int Xa = (int)rintf(((float)qx - v.x) * 32.0f), Ya = (int)rintf(((float)qy - v.y) * 32.0f);
+    const int Xc = (int)rintf(((float)qx + v.x) * 32.0f), Yc = (int)rintf(((float)qy + v.y) * 32.0f);
+    return __sad(winSample(W.a, W, Xa, Ya), winSample(W.c, W, Xc, Yc), 0u);
 }
 
 // sampleRun4 in two steps, so that a caller issues the loads of both frames (and more) before any arithmetic: the plan
@@ -1377,12 +1323,72 @@ __device__ __forceinline__ float pixErr1(const PixParams& P, int qx, int qy, flo
 
 __device__ __forceinline__ bool sameVec(float2 a, float2 b) { return a.x == b.x && a.y == b.y; }
 
+// The exact path's two sources of errors (integers in units of 2^-(fb + 10)): the float planes in global memory, whose
+// samples are exact, or the LDS windows. zero: the texel differences at gx0 .. gx0 + 3 (clamped), run: the errors of
+// one vector on a run inside the plane, pix4: four pixels each with its own vector.
+struct GlobalErrs {
+    const PixParams* P;
+    float up;
+    __device__ __forceinline__ void zero(int gx0, int qy, uint32_t* e) const {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const uint32_t q = idx2(clampi(gx0 + i, 0, P->w - 1), qy, P->w);
+            e[i] = (uint32_t)(fabsf(ldIdx(P->la, q) - ldIdx(P->lc, q)) * up);
+        }
+    }
+    __device__ __forceinline__ void run(int gx0, int qy, float2 v, uint32_t* e) const {
+        float ev[4];
+        pixRunErrs(*P, gx0, qy, v, ev);
+#pragma unroll
+        for (int i = 0; i < 4; i++) e[i] = (uint32_t)(ev[i] * up);
+    }
+    __device__ __forceinline__ void pix4(const int* qx, int qy, const float2* v, uint32_t* e) const {
+        float ev[4];
+        pixErr4(*P, qx, qy, v, ev);
+#pragma unroll
+        for (int i = 0; i < 4; i++) e[i] = (uint32_t)(ev[i] * up);
+    }
+};
+
+struct WinErrs {
+    PixWin W;
+    int w;
+    __device__ __forceinline__ void zero(int gx0, int qy, uint32_t* e) const {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int o = (qy - W.y0) * W.stride + (clampi(gx0 + i, 0, w - 1) - W.x0);
+            e[i] = __sad(W.a[o] & 0xFFFFu, W.c[o] & 0xFFFFu, 0u) << 10;
+        }
+    }
+    __device__ __forceinline__ void run(int gx0, int qy, float2 v, uint32_t* e) const {
+        const int Xa = (int)rintf(((float)gx0 - v.x) * 32.0f), Xa3 = (int)rintf(((float)(gx0 + 3) - v.x) * 32.0f);
+        const int Xc = (int)rintf(((float)gx0 + v.x) * 32.0f), Xc3 = (int)rintf(((float)(gx0 + 3) + v.x) * 32.0f);
+        if (Xa3 == Xa + 96 && Xc3 == Xc + 96) {   // one map fraction a frame: the run form
+            const int Ya = (int)rintf(((float)qy - v.y) * 32.0f), Yc = (int)rintf(((float)qy + v.y) * 32.0f);
+            uint32_t sa[4], sc[4];
+            winSample4(W.a, W, Xa, Ya, sa);
+            winSample4(W.c, W, Xc, Yc, sc);
+#pragma unroll
+            for (int i = 0; i < 4; i++) e[i] = __sad(sa[i], sc[i], 0u);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++) e[i] = winErr(W, gx0 + i, qy, v);
+        }
+    }
+    __device__ __forceinline__ void pix4(const int* qx, int qy, const float2* v, uint32_t* e) const {
+#pragma unroll
+        for (int i = 0; i < 4; i++) e[i] = winErr(W, qx[i], qy, v[i]);
+    }
+};
+
 // The errors of candidate k (0 or 2-5) on the run of 4 pixels gx0 .. gx0 + 3 of row qy
-__device__ __forceinline__ void pixCandRunErrs(const PixParams& P, const BlockWin& bw, int k, int gx0, int qy, float* ev) {
+template <class Errs>
+__device__ __forceinline__ void pixCandRunErrs(const PixParams& P, const BlockWin& bw, const Errs& E, int k, int gx0,
+                                               int qy, uint32_t* e) {
     if (gx0 >= 0 && gx0 + 3 <= P.w - 1) {
         if (k > 0) {
             const int2 o = candOffset(k);
-            pixRunErrs(P, gx0, qy, bw.at((gx0 >> 3) + o.x, (qy >> 3) + o.y), ev);   // one block, one vector
+            E.run(gx0, qy, bw.at((gx0 >> 3) + o.x, (qy >> 3) + o.y), e);   // one block, one vector
             return;
         }
         const float sx0 = ((float)gx0 + 0.5f) / 8.0f - 0.5f, sy = ((float)qy + 0.5f) / 8.0f - 0.5f;
@@ -1397,10 +1403,10 @@ __device__ __forceinline__ void pixCandRunErrs(const PixParams& P, const BlockWi
             v[i] = mix2(mix2(f00, f10, fx), mix2(f01, f11, fx), fy);
         }
         if (sameVec(v[1], v[0]) && sameVec(v[2], v[0]) && sameVec(v[3], v[0])) {
-            pixRunErrs(P, gx0, qy, v[0], ev);
+            E.run(gx0, qy, v[0], e);
         } else {
             const int qx[4] = {gx0, gx0 + 1, gx0 + 2, gx0 + 3};
-            pixErr4(P, qx, qy, v, ev);
+            E.pix4(qx, qy, v, e);
         }
         return;
     }
@@ -1411,7 +1417,98 @@ __device__ __forceinline__ void pixCandRunErrs(const PixParams& P, const BlockWi
         qx[i] = clampi(gx0 + i, 0, P.w - 1);
         v[i] = pixCand(k, qx[i], qy, bw);
     }
-    pixErr4(P, qx, qy, v, ev);
+    E.pix4(qx, qy, v, e);
+}
+
+// ---- uniform tiles ----
+// Every candidate vector of the tile's region (its pixels lie in blocks x0 / 8 - 1 .. x0 / 8 + 4, the same rows, and
+// reach one block further in the plus shape of candidates 2-5) is an entry of the 8 x 8 block window other than its
+// four corners. When those 60 entries are one vector V bit for bit, and the bilinear candidate of V with itself is V at
+// every fraction it uses (odd sixteenths, from (q + 1/2) / 8 - 1/2), candidates 0-5 are V at every pixel and their
+// window sums are one sum: candidate 0 costs c1 - smoothBonus, candidates 2-5 cost c1 and (strict <) cannot win. Only
+// V's and the zero vector's errors are formed, and the winner is V or zero. The values are those of the seven-candidate
+// pass. Integer sums: for exact lumas (fb >= 0) every sampleL is a multiple of 2^-(fb + 10) below 2^24 of them.
+constexpr int kPfUW = 36, kPfUH = kPfTile + 4;   // the windows' region: columns x0 - 2 .. x0 + 33, rows y0 - 2 .. y0 + 33
+constexpr int kPfUPlane = kPfUH * kPfUW;
+constexpr int kPfUniPool = 4 * kPfUPlane * 4;    // errors of V and of zero, then their horizontal 5-sums
+
+// the uniform test, by the wavefront that staged the window (lane i holds entry i): returns V's bits in (vx, vy)
+__device__ __forceinline__ bool pixWindowUniform(float2 v, int lane, uint32_t& vx, uint32_t& vy) {
+    vx = (uint32_t)__shfl((int)__float_as_uint(v.x), 9, 64);
+    vy = (uint32_t)__shfl((int)__float_as_uint(v.y), 9, 64);
+    const bool corner = lane == 0 || lane == 7 || lane == 56 || lane == 63;
+    bool ok = corner || (__float_as_uint(v.x) == vx && __float_as_uint(v.y) == vy);
+    if (lane < 8) {
+        const float t = (float)(2 * lane + 1) * (1.0f / 16.0f), Vx = __uint_as_float(vx), Vy = __uint_as_float(vy);
+        ok = ok && mixf(Vx, Vx, t) == Vx && mixf(Vy, Vy, t) == Vy;
+    }
+    return __all(ok);
+}
+
+template <int NT, class Errs>
+__device__ __forceinline__ void pixfieldUniform(const PixParams& P, int x0, int y0, float2 V, uint32_t* pool, const Errs& E) {
+    constexpr int kRows = kPfTile * kPfTile / NT;
+    uint32_t* eu = pool;                   // [2][36][36]: |errors| of V, of zero, in units of 2^-(fb + 10)
+    uint32_t* hu = pool + 2 * kPfUPlane;   // [2][36][36]: their horizontal 5-sums (columns 0 .. 31 of a row)
+    const int tid = threadIdx.x;
+    for (int r = tid; r < kPfUH * (kPfUW / 4); r += NT) {
+        const int ry = r / (kPfUW / 4), rc = r - ry * (kPfUW / 4);
+        const int qy = clampi(y0 - 2 + ry, 0, P.h - 1), gx0 = x0 - 2 + 4 * rc;
+        uint32_t iv[4], iz[4];
+        E.zero(gx0, qy, iz);
+        if (gx0 >= 0 && gx0 + 3 <= P.w - 1) {
+            E.run(gx0, qy, V, iv);
+        } else {
+            const int qx[4] = {clampi(gx0, 0, P.w - 1), clampi(gx0 + 1, 0, P.w - 1), clampi(gx0 + 2, 0, P.w - 1),
+                               clampi(gx0 + 3, 0, P.w - 1)};
+            const float2 vv[4] = {V, V, V, V};
+            E.pix4(qx, qy, vv, iv);
+        }
+        *(uint4*)&eu[ry * kPfUW + 4 * rc] = make_uint4(iv[0], iv[1], iv[2], iv[3]);
+        *(uint4*)&eu[kPfUPlane + ry * kPfUW + 4 * rc] = make_uint4(iz[0], iz[1], iz[2], iz[3]);
+    }
+    __syncthreads();
+    // horizontal 5-sums: output column c of a row sums region columns c .. c + 4; four outputs a task (two quads)
+    for (int t = tid; t < 2 * kPfUH * (kPfTile / 4); t += NT) {
+        const int pl = t >= kPfUH * (kPfTile / 4) ? 1 : 0, tt = t - pl * kPfUH * (kPfTile / 4);
+        const int hy = tt >> 3, c0 = (tt & 7) * 4;
+        const uint4* e4 = (const uint4*)&eu[pl * kPfUPlane + hy * kPfUW + c0];
+        const uint4 q0 = e4[0], q1 = e4[1];
+        uint4 h;
+        h.x = q0.x + q0.y + q0.z + q0.w + q1.x;
+        h.y = h.x + q1.y - q0.x;
+        h.z = h.y + q1.z - q0.y;
+        h.w = h.z + q1.w - q0.z;
+        *(uint4*)&hu[pl * kPfUPlane + hy * kPfUW + c0] = h;
+    }
+    __syncthreads();
+    const double cscale = (1.0 / 25.0) * (double)__int_as_float((127 - 10 - P.fb) << 23);
+    const int cx = tid & 31, r0 = (tid >> 5) * kRows;
+    uint32_t hv[kRows + 4], hz[kRows + 4];
+#pragma unroll
+    for (int y = 0; y < kRows + 4; y++) {
+        hv[y] = hu[(r0 + y) * kPfUW + cx];
+        hz[y] = hu[kPfUPlane + (r0 + y) * kPfUW + cx];
+    }
+    uint32_t sv = hv[0] + hv[1] + hv[2] + hv[3] + hv[4], sz = hz[0] + hz[1] + hz[2] + hz[3] + hz[4];
+    const int qx = x0 + cx;
+    if (qx >= P.w) return;
+#pragma unroll
+    for (int j = 0; j < kRows; j++) {
+        if (j > 0) {
+            sv = sv + hv[j + 4] - hv[j - 1];
+            sz = sz + hz[j + 4] - hz[j - 1];
+        }
+        const int py = y0 + r0 + j;
+        if (py >= P.h) break;
+        // the seven-candidate order: 1, then 0 on c0 <= best, then 2-5 (never below c1), then 6 on strict <
+        const float c1 = (float)((double)sv * cscale);
+        float best = c1;
+        const float c0 = c1 - P.smooth;
+        if (c0 <= best) best = c0;
+        const float c6 = (float)((double)sz * cscale) - P.zero;
+        P.pf[py * P.w + qx] = c6 < best ? make_float2(0.0f, 0.0f) : V;
+    }
 }
 
 // The seven-candidate pass for exact lumas (fb >= 0): errors as integers in units of 2^-(fb + 10), window sums in 32
@@ -1429,9 +1526,9 @@ constexpr int kPfExactPool = 2 * kPfRH * kPfRW * 4 + 2 * kPfRH * kPfHsStride * 4
 // list slot of candidate k (0, 2-5)
 __device__ __forceinline__ int candList(int k) { return k == 0 ? 0 : k - 1; }
 
-template <int NT>
+template <int NT, class Errs>
 __device__ __forceinline__ void pixfieldExact(const PixParams& P, int x0, int y0, const BlockWin& bw, uint8_t* pool,
-                                              uint32_t* cnt) {
+                                              uint32_t* cnt, const Errs& E) {
     constexpr int kRows = kPfTile * kPfTile / NT;
     constexpr int kRunsRow = kPfRW / 4;
     uint32_t* e1 = (uint32_t*)pool;              // [36][40] candidate 1's errors (kept)
@@ -1440,25 +1537,19 @@ __device__ __forceinline__ void pixfieldExact(const PixParams& P, int x0, int y0
     uint32_t* hk = h1 + kPfRH * kPfHsStride;     // ... of ek
     uint16_t* lists = (uint16_t*)(hk + kPfRH * kPfHsStride);   // [5][360] runs a candidate's pass forms
     const int tid = threadIdx.x;
-    const float up = __int_as_float((127 + 10 + P.fb) << 23);
     const double cscale = (1.0 / 25.0) * (double)__int_as_float((127 - 10 - P.fb) << 23);
     static_assert((kPfRuns + NT - 1) / NT <= 2, "at most two runs a thread");
     uint32_t reuse0 = 0u, reuse1 = 0u;   // bit k: the run's candidate k is candidate 1 at every pixel
     for (int r = tid; r < kPfRuns; r += NT) {
         const int ry = r / kRunsRow, rc = r - ry * kRunsRow;
         const int qy = clampi(y0 - 2 + ry, 0, P.h - 1), gx0 = x0 - 4 + 4 * rc;
-        float ev[4], za[4], zc[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) {   // the zero vector's texels, loaded first
-            const uint32_t q = idx2(clampi(gx0 + i, 0, P.w - 1), qy, P.w);
-            za[i] = ldIdx(P.la, q);
-            zc[i] = ldIdx(P.lc, q);
-        }
+        uint32_t i1[4], i6[4];
+        E.zero(gx0, qy, i6);
         uint32_t bits = 0u;
         if (gx0 >= 0 && gx0 + 3 <= P.w - 1) {
             const int bx = gx0 >> 3, by = qy >> 3;
             const float2 v1 = bw.at(bx, by);
-            pixRunErrs(P, gx0, qy, v1, ev);
+            E.run(gx0, qy, v1, i1);
 #pragma unroll
             for (int k = 2; k <= 5; k++) {
                 const int2 o = candOffset(k);
@@ -1490,14 +1581,8 @@ __device__ __forceinline__ void pixfieldExact(const PixParams& P, int x0, int y0
                     if (k != 1) b |= sameVec(pixCand(k, qx[i], qy, bw), v1[i]) ? 1u << k : 0u;
                 all &= b;
             }
-            pixErr4(P, qx, qy, v1, ev);
+            E.pix4(qx, qy, v1, i1);
             bits = all;
-        }
-        uint32_t i1[4], i6[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            i1[i] = (uint32_t)(ev[i] * up);
-            i6[i] = (uint32_t)(fabsf(za[i] - zc[i]) * up);
         }
         *(uint4*)&e1[ry * kPfRW + 4 * rc] = make_uint4(i1[0], i1[1], i1[2], i1[3]);
         *(uint4*)&ek[ry * kPfRW + 4 * rc] = make_uint4(i6[0], i6[1], i6[2], i6[3]);
@@ -1587,13 +1672,12 @@ __device__ __forceinline__ void pixfieldExact(const PixParams& P, int x0, int y0
             if ((((r >= NT) ? reuse1 : reuse0) >> k) & 1u)
                 *(uint4*)&ek[ry * kPfRW + 4 * rc] = *(const uint4*)&e1[ry * kPfRW + 4 * rc];
         }
-        for (int e = tid; e < nk; e += NT) {
-            const int r = lists[step * kPfRuns + e];
+        for (int li = tid; li < nk; li += NT) {
+            const int r = lists[step * kPfRuns + li];
             const int ry = r / kRunsRow, rc = r - ry * kRunsRow;
-            float ev[4];
-            pixCandRunErrs(P, bw, k, x0 - 4 + 4 * rc, clampi(y0 - 2 + ry, 0, P.h - 1), ev);
-            *(uint4*)&ek[ry * kPfRW + 4 * rc] = make_uint4((uint32_t)(ev[0] * up), (uint32_t)(ev[1] * up),
-                                                           (uint32_t)(ev[2] * up), (uint32_t)(ev[3] * up));
+            uint32_t e[4];
+            pixCandRunErrs(P, bw, E, k, x0 - 4 + 4 * rc, clampi(y0 - 2 + ry, 0, P.h - 1), e);
+            *(uint4*)&ek[ry * kPfRW + 4 * rc] = make_uint4(e[0], e[1], e[2], e[3]);
         }
         __syncthreads();
         for (int t = tid; t < kTasks; t += NT) hsums(ek, hk, t);
@@ -1639,33 +1723,80 @@ constexpr int kPfWork0 = kPfFloatWork > kPfUniPool ? kPfFloatWork : kPfUniPool;
 constexpr int kPfWork = kPfWork0 > kPfExactPool ? kPfWork0 : kPfExactPool;
 constexpr int kPfFloatPool = kPfWork + 64 * 8 + 32;   // + the block window, the uniform flag and the list lengths
 
+// The LDS windows of a tile at reach kPfWinR, NT threads: a wavefront a row (one texel load a lane, the pair's second
+// texel from the next lane), every load of a thread issued before its stores
 template <int NT>
-__device__ __forceinline__ void pixfieldTile(const PixParams& P, int gbx, int gby, uint8_t* pool) {
+__device__ __forceinline__ PixWin pixFillWindows(const PixParams& P, int x0, int y0, uint32_t* win) {
+    constexpr int R = kPfWinR, kWaves = NT / 64, kIt = (kPfWinRows + kWaves - 1) / kWaves;
+    const PixWin W{win, win + kPfWinWords * kPfWinRows, x0 - 5 - R, y0 - 3 - R, kPfWinWords};
+    const float up = __int_as_float((127 + P.fb) << 23);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t gx = (uint32_t)clampi(W.x0 + lane, 0, P.w - 1);
+    float a[kIt], c[kIt];
+#pragma unroll
+    for (int j = 0; j < kIt; j++) {
+        const uint32_t row = idx2(0, clampi(W.y0 + min(wave + kWaves * j, kPfWinRows - 1), 0, P.h - 1), P.w);
+        a[j] = ldIdx(P.la, row + gx);
+        c[j] = ldIdx(P.lc, row + gx);
+    }
+#pragma unroll
+    for (int j = 0; j < kIt; j++) {
+        const int wy = wave + kWaves * j;
+        const uint32_t ta = (uint32_t)(a[j] * up), tc = (uint32_t)(c[j] * up);
+        const uint32_t ta1 = (uint32_t)__shfl_down((int)ta, 1, 64), tc1 = (uint32_t)__shfl_down((int)tc, 1, 64);
+        if (wy < kPfWinRows && lane < kPfWinWords) {
+            win[wy * kPfWinWords + lane] = ta | (ta1 << 16);
+            win[kPfWinWords * kPfWinRows + wy * kPfWinWords + lane] = tc | (tc1 << 16);
+        }
+    }
+    return W;
+}
+
+// A 32 x 32 tile of the pixel field. Win: the exact path samples LDS windows of the planes when the tile's candidates
+// reach at most kPfWinR flow px (win: kPfWinPool bytes of LDS), else the planes in global memory.
+template <int NT, bool Win>
+__device__ __forceinline__ void pixfieldTile(const PixParams& P, int gbx, int gby, uint8_t* pool, uint32_t* win) {
     static_assert(NT == 256 || NT == 512, "two layouts");
+    static_assert(kPfWinWords + 1 <= 64, "a window row is one wavefront's texels");
     constexpr int kRows = kPfTile * kPfTile / NT;      // pixels a thread: column cx, rows r0 .. r0 + kRows - 1
     double* hsum = (double*)pool;
     float* errs = (float*)(pool + kPfRH * kPfTile * 8);
     float2* bwin = (float2*)(pool + kPfWork);
     int* uni = (int*)(pool + kPfWork + 64 * 8);
     uint32_t* cnt = (uint32_t*)(pool + kPfWork + 64 * 8 + 4);   // [5] list lengths of pixfieldExact
+    int* reach = (int*)(pool + kPfWork + 64 * 8 + 24);
     const int tid = threadIdx.x;
     const int x0 = gbx * kPfTile, y0 = gby * kPfTile;
     const BlockWin bw{bwin, x0 / kBlock - 2, y0 / kBlock - 2};
-    if (tid < 64) {
+    // the windows (exact lumas) at the fixed reach, their loads issued with the block vectors' (a tile whose candidates
+    // reach further samples global memory instead)
+    PixWin W{};
+    if (Win && P.fb >= 0) W = pixFillWindows<NT>(P, x0, y0, win);
+    if (tid < 64) {   // (one wavefront) the block vectors, the uniform test, R = floor(max |component|) + 1
         const float2 v = fieldAt(P.bf, bw.bx0 + (tid & 7), bw.by0 + (tid >> 3), P.bw, P.bh);
         bwin[tid] = v;
         uint32_t vx, vy;
         const bool u = pixWindowUniform(v, tid, vx, vy);
-        if (tid == 0) *uni = u && P.fb >= 0;
+        float m = fmaxf(fabsf(v.x), fabsf(v.y));
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+        if (tid == 0) {
+            *uni = u && P.fb >= 0;
+            *reach = (int)fminf(floorf(m) + 1.0f, (float)(kPfWinR + 1));
+        }
         if (tid < kPfCandLists) cnt[tid] = 0u;
     }
     __syncthreads();
-    if (*uni) {   // (uniform over the workgroup)
-        pixfieldUniform<NT>(P, x0, y0, bwin[9], (uint32_t*)pool);
-        return;
-    }
     if (P.fb >= 0) {
-        pixfieldExact<NT>(P, x0, y0, bw, pool, cnt);
+        const bool inWin = Win && *reach <= kPfWinR;
+        const GlobalErrs G{&P, __int_as_float((127 + 10 + P.fb) << 23)};
+        if (*uni) {   // (uniform over the workgroup)
+            if (inWin) pixfieldUniform<NT>(P, x0, y0, bwin[9], (uint32_t*)pool, WinErrs{W, P.w});
+            else pixfieldUniform<NT>(P, x0, y0, bwin[9], (uint32_t*)pool, G);
+        } else {
+            if (inWin) pixfieldExact<NT>(P, x0, y0, bw, pool, cnt, WinErrs{W, P.w});
+            else pixfieldExact<NT>(P, x0, y0, bw, pool, cnt, G);
+        }
         return;
     }
     const int cx = tid & 31, r0 = (tid >> 5) * kRows;
@@ -1783,295 +1914,11 @@ __device__ __forceinline__ void pixfieldTile(const PixParams& P, int gbx, int gb
     }
 }
 
-template <int NT>
+template <int NT, bool Win>
 __device__ __forceinline__ void pixfieldBody(const PixParams& P, int gbx, int gby) {
     __shared__ __align__(16) uint8_t pool[kPfFloatPool];
-    pixfieldTile<NT>(P, gbx, gby, pool);
-}
-
-// ---- the pixel field in exact integers, from LDS windows of the two planes ----
-// For a level-0 luma of values k 2^-fb with 0 <= k < 2^14 (RGBA8 at every flowScale; PixParams.fb) every sampleL is
-// exact in float32: a tap k m / 1024 (m <= 1024) has at most 14 + 11 significant bits, and the sum, a convex combination,
-// at most 14 + 10 bits; so is the error |a - c|, and every window sum of 25 errors fits 32 bits. In integers (units of
-// 2^-(fb + 10)) a sample is the sum of t m over its taps (t = k): one v_dot2_u32_u16 per tap row over the texel pair with
-// the weights (32 - fx, fx), then the vertical blend; the error is one v_sad_u32. The window sum S gives the cost
-// float32(S 2^-(fb + 10) (1/25)), the float32 rounding of cv2.blur's float64 mean, as the float path computes it.
-//
-// The texels come from LDS windows of both planes covering the tile's region plus the reach R of its largest candidate
-// vector: word x of a window row is the texel pair t[x] | t[x + 1] << 16 of the edge-clamped plane, so a window tap is
-// the clamped global tap. A tile whose candidates reach further than kPfWinR flow px runs the float path (pixfieldTile).
-constexpr int kPfWinR = 7;
-constexpr int kPfWinWords = 42 + 2 * kPfWinR, kPfWinRows = 39 + 2 * kPfWinR;   // at R: 42 + 2R words, 39 + 2R rows
-
-struct PixWin {
-    const uint32_t* a;
-    const uint32_t* c;
-    int x0, y0, stride;   // plane coordinates of the windows' word 0, words a row
-};
-
-
-// sampleL at the 1/32 map (X, Y), in units of 2^-(fb + 10)
-__device__ __forceinline__ uint32_t winSample(const uint32_t* w, const PixWin& W, int X, int Y) {
-    const uint32_t fx = (uint32_t)(X & 31), fy = (uint32_t)(Y & 31);
-    const uint32_t* p = w + ((Y >> 5) - W.y0) * W.stride + ((X >> 5) - W.x0);
-    const uint32_t wx = (32u - fx) | (fx << 16);
-    return pairDot(p[0], wx) * (32u - fy) + pairDot(p[W.stride], wx) * fy;
-}
-
-// the four samples at X0 + 32 i, Y (one fractional part, 5 texels a row)
-__device__ __forceinline__ void winSample4(const uint32_t* w, const PixWin& W, int X0, int Y, uint32_t* out) {
-    const uint32_t fx = (uint32_t)(X0 & 31), fy = (uint32_t)(Y & 31);
-    const uint32_t* p = w + ((Y >> 5) - W.y0) * W.stride + ((X0 >> 5) - W.x0);
-    const uint32_t wx = (32u - fx) | (fx << 16);
-#pragma unroll
-    for (int i = 0; i < 4; i++) out[i] = pairDot(p[i], wx) * (32u - fy) + pairDot(p[W.stride + i], wx) * fy;
-}
-
-// |sampleL(a, q - v) - sampleL(c, q + v)| with the maps formed as sampleL forms them
-__device__ __forceinline__ uint32_t winErr(const PixWin& W, int qx, int qy, float2 v) {
-    const int Xa = (int)rintf(((float)qx - v.x) * 32.0f), Ya = (int)rintf(((float)qy - v.y) * 32.0f);
-    const int Xc = (int)rintf(((float)qx + v.x) * 32.0f), Yc = (int)rintf(((float)qy + v.y) * 32.0f);
-    return __sad(winSample(W.a, W, Xa, Ya), winSample(W.c, W, Xc, Yc), 0u);
-}
-
-// pixErrRun's candidates, reuse rules and errors, in integers from the windows
-__device__ __forceinline__ bool winErrRun(const PixParams& P, const PixWin& W, const BlockWin& bw, int k, int gx0, int qy,
-                                          uint32_t* e, uint32_t* own, float2& ownV, bool& ownOk) {
-    if (k == 6) {   // zero vector: the texel difference (weight 1024)
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int o = (qy - W.y0) * W.stride + (clampi(gx0 + i, 0, P.w - 1) - W.x0);
-            e[i] = __sad(W.a[o] & 0xFFFFu, W.c[o] & 0xFFFFu, 0u) << 10;
-        }
-        return false;
-    }
-    const bool inside = gx0 >= 0 && gx0 + 3 <= P.w - 1;
-    if (k == 0 && inside) {   // the run's pixels interpolate the same four block vectors
-        const float sx0 = ((float)gx0 + 0.5f) / 8.0f - 0.5f, sy = ((float)qy + 0.5f) / 8.0f - 0.5f;
-        const float flx = floorf(sx0), fly = floorf(sy);
-        const float fy = sy - fly;
-        const int ix = (int)flx, iy = (int)fly;
-        const float2 f00 = bw.at(ix, iy), f10 = bw.at(ix + 1, iy), f01 = bw.at(ix, iy + 1), f11 = bw.at(ix + 1, iy + 1);
-        bool same = ownOk;
-        float2 v[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const float fx = (((float)(gx0 + i) + 0.5f) / 8.0f - 0.5f) - flx;
-            v[i] = mix2(mix2(f00, f10, fx), mix2(f01, f11, fx), fy);
-            same = same && v[i].x == ownV.x && v[i].y == ownV.y;
-        }
-        if (same) {
-#pragma unroll
-            for (int i = 0; i < 4; i++) e[i] = own[i];
-            return true;
-        }
-#pragma unroll
-        for (int i = 0; i < 4; i++) e[i] = winErr(W, gx0 + i, qy, v[i]);
-        return false;
-    }
-    if (k > 0 && inside) {
-        const int2 o = candOffset(k);
-        const float2 v = bw.at(gx0 / kBlock + o.x, qy / kBlock + o.y);   // one block, one vector
-        if (k > 1 && ownOk && v.x == ownV.x && v.y == ownV.y) {
-#pragma unroll
-            for (int i = 0; i < 4; i++) e[i] = own[i];
-            return true;
-        }
-        const int Xa = (int)rintf(((float)gx0 - v.x) * 32.0f), Xa3 = (int)rintf(((float)(gx0 + 3) - v.x) * 32.0f);
-        const int Xc = (int)rintf(((float)gx0 + v.x) * 32.0f), Xc3 = (int)rintf(((float)(gx0 + 3) + v.x) * 32.0f);
-        if (Xa3 == Xa + 96 && Xc3 == Xc + 96) {   // one map fraction a frame: the run form
-            const int Ya = (int)rintf(((float)qy - v.y) * 32.0f), Yc = (int)rintf(((float)qy + v.y) * 32.0f);
-            uint32_t sa[4], sc[4];
-            winSample4(W.a, W, Xa, Ya, sa);
-            winSample4(W.c, W, Xc, Yc, sc);
-#pragma unroll
-            for (int i = 0; i < 4; i++) e[i] = __sad(sa[i], sc[i], 0u);
-        } else {
-#pragma unroll 1
-            for (int i = 0; i < 4; i++) e[i] = winErr(W, gx0 + i, qy, v);
-        }
-        if (k == 1) {
-#pragma unroll
-            for (int i = 0; i < 4; i++) own[i] = e[i];
-            ownV = v;
-            ownOk = true;
-        }
-        return false;
-    }
-#pragma unroll 1
-    for (int i = 0; i < 4; i++) {
-        const int qx = clampi(gx0 + i, 0, P.w - 1);
-        e[i] = winErr(W, qx, qy, pixCand(k, qx, qy, bw));
-    }
-    return false;
-}
-
-// The pixel field of a 32 x 32 tile from the windows (layout and candidate order as pixfieldTile's). The runs of the
-// tile's 40 x 36 region are r = tid and tid + NT; each candidate's errors go to LDS, then the horizontal 5-sums of the
-// region's rows (exact u32, four a task), then each thread's vertical sums and running minimum.
-template <int NT>
-__device__ __forceinline__ void pixfieldWinBody(const PixParams& P, int gbx, int gby) {
-    static_assert(NT == 256 || NT == 512, "two layouts");
-    constexpr int kRows = kPfTile * kPfTile / NT;
-    constexpr int kRunsPer = (kPfRuns + NT - 1) / NT;
-    constexpr int kWinWords = kPfWinWords * kPfWinRows;
-    constexpr int kIntPool = 2 * kWinWords * 4 + kPfRH * kPfRW * 4 + kPfRH * kPfHsStride * 4;
-    constexpr int kPool = kIntPool > kPfFloatPool ? kIntPool : kPfFloatPool;
-    __shared__ __align__(16) uint8_t pool[kPool];
-    __shared__ float2 bwin[64];
-    __shared__ int winReach, winUni;
-    const int tid = threadIdx.x;
-    const int x0 = gbx * kPfTile, y0 = gby * kPfTile;
-    const BlockWin bw{bwin, x0 / kBlock - 2, y0 / kBlock - 2};
-    if (tid < 64) {   // (one wavefront) the block vectors, and R = floor(max |component|) + 1 > every candidate's reach
-        const float2 v = fieldAt(P.bf, bw.bx0 + (tid & 7), bw.by0 + (tid >> 3), P.bw, P.bh);
-        bwin[tid] = v;
-        float m = fmaxf(fabsf(v.x), fabsf(v.y));
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-        uint32_t vx, vy;
-        const bool u = pixWindowUniform(v, tid, vx, vy);
-        if (tid == 0) {
-            winReach = (int)fminf(floorf(m) + 1.0f, (float)(kPfWinR + 1));
-            winUni = u;
-        }
-    }
-    __syncthreads();
-    if (winUni) {   // one vector throughout: its errors and zero's, no windows
-        pixfieldUniform<NT>(P, x0, y0, bwin[9], (uint32_t*)pool);
-        return;
-    }
-    const int R = winReach;
-    if (R > kPfWinR) {   // (uniform) a long candidate: the float path, its LDS in the same pool
-        pixfieldTile<NT>(P, gbx, gby, pool);
-        return;
-    }
-    uint32_t* winA = (uint32_t*)pool;
-    uint32_t* errs = winA + 2 * kWinWords;
-    uint32_t* hsum = errs + kPfRH * kPfRW;
-    // the windows: rows y0 - 3 - R .. y0 + 35 + R, word columns x0 - 5 - R .. x0 + 36 + R (a wavefront a row)
-    const PixWin W{winA, winA + kWinWords, x0 - 5 - R, y0 - 3 - R, 42 + 2 * R};
-    {
-        const float up = __int_as_float((127 + P.fb) << 23);
-        const int rows = 39 + 2 * R, lane = tid & 63;
-        if (lane < W.stride) {
-            const uint32_t gx0 = (uint32_t)clampi(W.x0 + lane, 0, P.w - 1), gx1 = (uint32_t)clampi(W.x0 + lane + 1, 0, P.w - 1);
-            for (int wy = tid >> 6; wy < rows; wy += NT / 64) {
-                const uint32_t row = idx2(0, clampi(W.y0 + wy, 0, P.h - 1), P.w);
-                const uint32_t a0 = (uint32_t)(ldIdx(P.la, row + gx0) * up), a1 = (uint32_t)(ldIdx(P.la, row + gx1) * up);
-                const uint32_t c0 = (uint32_t)(ldIdx(P.lc, row + gx0) * up), c1 = (uint32_t)(ldIdx(P.lc, row + gx1) * up);
-                winA[wy * W.stride + lane] = a0 | (a1 << 16);
-                winA[kWinWords + wy * W.stride + lane] = c0 | (c1 << 16);
-            }
-        }
-    }
-    __syncthreads();
-    // cost = float32(double(S) (1/25) 2^-(fb + 10)): the power of two commutes with the rounding of the product
-    const double cscale = (1.0 / 25.0) * (double)__int_as_float((127 - 10 - P.fb) << 23);
-    const int cx = tid & 31, r0 = (tid >> 5) * kRows;
-    float best[kRows], c1[kRows];
-    int bk[kRows];
-#pragma unroll
-    for (int j = 0; j < kRows; j++) {
-        best[j] = 0.0f;
-        c1[j] = 0.0f;
-        bk[j] = 0;
-    }
-    static_assert(kRunsPer <= 2, "at most two runs a thread");
-    // candidate 1's errors of the thread's runs (r = tid and tid + NT), kept for candidates 0 and 2-5; one loop body for
-    // both runs, the kept values selected rather than indexed
-    uint4 own0 = make_uint4(0u, 0u, 0u, 0u), own1 = own0;
-    float2 ownV0 = make_float2(0.0f, 0.0f), ownV1 = ownV0;
-    bool ownOk0 = false, ownOk1 = false;
-#pragma unroll 1
-    for (int step = 0; step < 7; step++) {
-        const int k = step == 0 ? 1 : (step == 1 ? 0 : step);
-        bool reusedAll = true;
-#pragma unroll 1
-        for (int r = tid; r < kPfRuns; r += NT) {
-            const bool s1 = r >= NT;
-            const uint4 o4 = s1 ? own1 : own0;
-            uint32_t own[4] = {o4.x, o4.y, o4.z, o4.w};
-            float2 ownV = s1 ? ownV1 : ownV0;
-            bool ownOk = s1 ? ownOk1 : ownOk0;
-            const int ry = r / (kPfRW / 4), rc = r - ry * (kPfRW / 4);
-            uint32_t e[4];
-            reusedAll = winErrRun(P, W, bw, k, x0 - 4 + 4 * rc, clampi(y0 - 2 + ry, 0, P.h - 1), e, own, ownV, ownOk) &&
-                        reusedAll;
-            *(uint4*)&errs[ry * kPfRW + 4 * rc] = make_uint4(e[0], e[1], e[2], e[3]);
-            if (k == 1) {
-                const uint4 n4 = make_uint4(own[0], own[1], own[2], own[3]);
-                own0 = s1 ? own0 : n4;
-                own1 = s1 ? n4 : own1;
-                ownV0 = s1 ? ownV0 : ownV;
-                ownV1 = s1 ? ownV : ownV1;
-                ownOk0 = s1 ? ownOk0 : ownOk;
-                ownOk1 = s1 ? ownOk : ownOk1;
-            }
-        }
-        if (__syncthreads_and(reusedAll)) {   // every run reused candidate 1's errors: the sums are candidate 1's
-            if (k == 0) {
-#pragma unroll
-                for (int j = 0; j < kRows; j++) {
-                    const float c = c1[j] - P.smooth;
-                    if (c <= best[j]) {
-                        best[j] = c;
-                        bk[j] = 0;
-                    }
-                }
-            }
-            continue;
-        }
-        // horizontal 5-sums of region columns c0 + 2 .. c0 + 9 (three aligned 16-byte quads), four outputs a task
-        for (int t = tid; t < kPfRH * (kPfTile / 4); t += NT) {
-            const int hy = t >> 3, c0 = (t & 7) * 4;
-            const uint4* e4 = (const uint4*)&errs[hy * kPfRW + c0];
-            const uint4 q0 = e4[0], q1 = e4[1], q2 = e4[2];
-            uint4 h;
-            h.x = q0.z + q0.w + q1.x + q1.y + q1.z;
-            h.y = h.x + q1.w - q0.z;
-            h.z = h.y + q2.x - q0.w;
-            h.w = h.z + q2.y - q1.x;
-            *(uint4*)&hsum[hy * kPfHsStride + c0] = h;
-        }
-        __syncthreads();
-        uint32_t hv[kRows + 4];
-#pragma unroll
-        for (int y = 0; y < kRows + 4; y++) hv[y] = hsum[(r0 + y) * kPfHsStride + cx];
-        uint32_t vs = hv[0] + hv[1] + hv[2] + hv[3] + hv[4];
-#pragma unroll
-        for (int j = 0; j < kRows; j++) {
-            if (j > 0) vs = vs + hv[j + 4] - hv[j - 1];
-            float c = (float)((double)vs * cscale);
-            if (k == 0) c = c - P.smooth;
-            if (k == 6) c = c - P.zero;
-            if (k == 1) {
-                c1[j] = c;
-                best[j] = c;
-                bk[j] = 1;
-            } else if (k == 0 ? c <= best[j] : c < best[j]) {
-                best[j] = c;
-                bk[j] = k;
-            }
-        }
-    }
-    const int qx = x0 + cx;
-    if (qx >= P.w) return;
-#pragma unroll
-    for (int j = 0; j < kRows; j++) {
-        const int py = y0 + r0 + j;
-        if (py >= P.h) break;
-        const int k = bk[j];
-        float2 v;
-        if (k == 0) {
-            v = pixCand(0, qx, py, bw);
-        } else {
-            const int2 o = candOffset(k);
-            v = bw.at(qx / kBlock + o.x, py / kBlock + o.y);
-            if (k == 6) v = make_float2(0.0f, 0.0f);
-        }
-        P.pf[py * P.w + qx] = v;
-    }
+    __shared__ __align__(16) uint32_t win[Win ? kPfWinPool / 4 : 1];
+    pixfieldTile<NT, Win>(P, gbx, gby, pool, win);
 }
 
 // ---------------------------------------------------------------------------------------------------------- kernels
@@ -2090,10 +1937,11 @@ FGR_PASS(search12, SearchParams, searchBody<12>, 256)
 FGR_PASS(median3, Median3Params, median3Body, 256)
 FGR_PASS(subpix, SubpixParams, subpixBody, 64)
 FGR_PASS(subpix_int, SubpixParams, subpixIntBody, 64)
-FGR_PASS_SINGLE(pixfield, PixParams, pixfieldBody<kPfThreads>, kPfThreads)
-FGR_PASS_BATCH_OCC(pixfield, PixParams, pixfieldBody<kPfBatchThreads>, kPfBatchThreads, 6)
-FGR_PASS_SINGLE(pixfield_win, PixParams, pixfieldWinBody<kPfThreads>, kPfThreads)
-FGR_PASS_BATCH(pixfield_win, PixParams, pixfieldWinBody<kPfBatchThreads>, kPfBatchThreads)
+#define FGR_PIX_BODY(NT, WIN) pixfieldBody<NT, WIN>
+FGR_PASS_SINGLE(pixfield, PixParams, FGR_PIX_BODY(kPfThreads, false), kPfThreads)
+FGR_PASS_BATCH_OCC(pixfield, PixParams, FGR_PIX_BODY(kPfBatchThreads, false), kPfBatchThreads, 6)
+FGR_PASS_SINGLE(pixfield_win, PixParams, FGR_PIX_BODY(kPfThreads, true), kPfThreads)
+FGR_PASS_BATCH_OCC(pixfield_win, PixParams, FGR_PIX_BODY(kPfThreads, true), kPfThreads, 6)
 
 // ---------------------------------------------------------------------------------------------------------- launches
 
@@ -2137,7 +1985,7 @@ hipError_t launchBatch(int pass, const void* params, const uint32_t* work, int n
     case kPassSubpix: hipLaunchKernelGGL(subpix_batch_kernel, g, dim3(64), 0, s, (const SubpixParams*)params, work); break;
     case kPassSubpixInt: hipLaunchKernelGGL(subpix_int_batch_kernel, g, dim3(64), 0, s, (const SubpixParams*)params, work); break;
     case kPassPixfield: hipLaunchKernelGGL(pixfield_batch_kernel, g, dim3(kPfBatchThreads), 0, s, (const PixParams*)params, work); break;
-    case kPassPixfieldWin: hipLaunchKernelGGL(pixfield_win_batch_kernel, g, dim3(kPfBatchThreads), 0, s, (const PixParams*)params, work); break;
+    case kPassPixfieldWin: hipLaunchKernelGGL(pixfield_win_batch_kernel, g, dim3(kPfThreads), 0, s, (const PixParams*)params, work); break;
     default: return launchSynthBatch(pass, params, work, n, s);
     }
     return hipGetLastError();
