@@ -653,10 +653,11 @@ __device__ __forceinline__ void searchBody(const SearchParams& P, int bx0, int) 
 // 32-63 the second's (each lane one candidate, its SAD in the order above), and the argmin runs within each half:
 // the same vectors with twice the lanes busy. R is a template argument (1 or 2), so window and LDS offsets are
 // constants.
+constexpr int kSearch2MaxT = 12;
 template <int R>
-__device__ __forceinline__ void search2BodyR(const SearchParams& P, int bx0) {
-    constexpr int MAXT = 12, D = 2 * R + 1;
-    __shared__ float ta[kSearch2Blocks][MAXT * MAXT], tc[kSearch2Blocks][MAXT * MAXT];
+__device__ __forceinline__ void search2BodyR(const SearchParams& P, int bx0, float (*ta)[kSearch2MaxT * kSearch2MaxT],
+                                             float (*tc)[kSearch2MaxT * kSearch2MaxT]) {
+    constexpr int D = 2 * R + 1;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;   // wave: uniform
     const int nb = P.bw * P.bh;
     int sxs[2] = {0, 0}, sys[2] = {0, 0}, bxs[2], bys[2];
@@ -709,8 +710,10 @@ __device__ __forceinline__ void search2BodyR(const SearchParams& P, int bx0) {
 }
 
 __device__ __forceinline__ void search2Body(const SearchParams& P, int bx0, int) {
-    if (P.R == 2) search2BodyR<2>(P, bx0);   // (uniform) the radius of the level
-    else search2BodyR<1>(P, bx0);
+    // one LDS allocation for both radii (each instantiation's own arrays would both be allocated)
+    __shared__ float ta[kSearch2Blocks][kSearch2MaxT * kSearch2MaxT], tc[kSearch2Blocks][kSearch2MaxT * kSearch2MaxT];
+    if (P.R == 2) search2BodyR<2>(P, bx0, ta, tc);   // (uniform) the radius of the level
+    else search2BodyR<1>(P, bx0, ta, tc);
 }
 
 __device__ __forceinline__ void median3Body(const Median3Params& P, int gbx, int gby) {
