@@ -202,14 +202,31 @@ __device__ __forceinline__ void warpRegionRun(const SynthParams& P, const float2
     const float fy = FlowMap<S>::frac(yc);
     float2 r[4];
     int xs[4];
+    float2 ht[4], hb[4];
+    bool hsame = true;   // the run's two interpolated rows hold one value each, bit for bit: one field value
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         xs[i] = med3i(x0 - AP + ex0 + i, 0, P.W - 1);
         // a clamped column repeats the edge pixel: its hrow entry is that pixel's (clamped the same way)
-        r[i] = mix2(h[i], h[i + EW], fy);
+        ht[i] = h[i];
+        hb[i] = h[i + EW];
+        if (i > 0)
+            hsame = hsame && __float_as_uint(ht[i].x) == __float_as_uint(ht[0].x) &&
+                    __float_as_uint(ht[i].y) == __float_as_uint(ht[0].y) &&
+                    __float_as_uint(hb[i].x) == __float_as_uint(hb[0].x) &&
+                    __float_as_uint(hb[i].y) == __float_as_uint(hb[0].y);
     }
-    const bool same = xs[3] == xs[0] + 3 && r[1].x == r[0].x && r[1].y == r[0].y && r[2].x == r[0].x &&
-                      r[2].y == r[0].y && r[3].x == r[0].x && r[3].y == r[0].y;
+    bool same;
+    if (hsame) {
+        r[0] = mix2(ht[0], hb[0], fy);
+        r[1] = r[2] = r[3] = r[0];
+        same = xs[3] == xs[0] + 3;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; i++) r[i] = mix2(ht[i], hb[i], fy);
+        same = xs[3] == xs[0] + 3 && r[1].x == r[0].x && r[1].y == r[0].y && r[2].x == r[0].x && r[2].y == r[0].y &&
+               r[3].x == r[0].x && r[3].y == r[0].y;
+    }
     const RunMap ma = runMap(P.W, xs[0], y, kA32, r[0]), mc = runMap(P.W, xs[0], y, -kC32, r[0]);
     if (__all(same && ma.ok && mc.ok)) {
         warpRun4(P.a.p, (uint32_t)P.a.pitch, P.H, ma, oa);

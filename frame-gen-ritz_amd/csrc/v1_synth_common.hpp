@@ -95,6 +95,8 @@ __device__ __forceinline__ int loadFieldTile(const SynthParams& P, int x0, int y
     // the thread's (at most) IT entries side by side, so that their dependent loads overlap: the index is clamped
     // rather than branched on (an entry past the end repeats the last one and is not stored)
     constexpr int N = FTW * FTH, IT = (N + 255) / 256;
+    // the last step's entries past N are not stored; a wavefront with none left there skips that step's lookup
+    const bool lastIt = 256 * (IT - 1) + (tid & ~63) < N;
     float2 v[IT];
 #pragma unroll
     for (int it = 0; it < IT; it++) {
@@ -107,6 +109,7 @@ __device__ __forceinline__ int loadFieldTile(const SynthParams& P, int x0, int y
         for (int k = 0; k < 2; k++) {
 #pragma unroll
             for (int it = 0; it < IT; it++) {
+                if (it == IT - 1 && !lastIt) continue;   // (wave-uniform)
                 const int i = min(tid + 256 * it, N - 1);
                 const int qx = med3i(tx0 + i % FTW, 0, P.fw - 1), qy = med3i(ty0 + i / FTW, 0, P.fh - 1);
                 v[it] = sampleF(P.field, (float)qx - P.s * v[it].x, (float)qy - P.s * v[it].y, P.fw, P.fh);
