@@ -494,24 +494,26 @@ __device__ __forceinline__ void searchWindowsIssued(const SearchParams& P, float
     }
 }
 
-// step 2 for the two blocks of a wave at radius R (T = 8 + 2R): every lane's loads of both windows issued before any
-// store, so the two windows wait on one memory latency (element i of a window as in searchWindows)
+// step 2 at radius R (T = 8 + 2R) in two halves, so that a wave can issue its loads early: a lane's IT values of the
+// window of a (at q - seed) and c (at q + seed) of one block (element i of a window as in searchWindows; past N a
+// clamped read that is not stored), then their stores
 template <int R>
-__device__ __forceinline__ void searchWindowsPair(const SearchParams& P, float* A0, float* C0, float* A1, float* C1,
-                                                  const int* bxs, const int* bys, const int* sxs, const int* sys,
-                                                  int lane, bool ints) {
-    constexpr int T = 8 + 2 * R, N = T * T, IT = (N + 63) / 64;
+struct SearchWin {
+    static constexpr int T = 8 + 2 * R, N = T * T, IT = (N + 63) / 64;
+    float va[IT], vc[IT];
+};
+
+template <int R>
+__device__ __forceinline__ void searchWinLoad(const SearchParams& P, int bx, int by, int sx, int sy, int lane,
+                                              SearchWin<R>& w) {
+    constexpr int T = SearchWin<R>::T, IT = SearchWin<R>::IT;
     constexpr int q = 64 / T, r = 64 - q * T;   // element i + 64 is (x + r, y + q), carried
-    float va[2][IT], vc[2][IT];
     int x = lane % T, y = lane / T;
+    const int b0x = bx * kBlock, b0y = by * kBlock;
 #pragma unroll
     for (int it = 0; it < IT; it++) {
-#pragma unroll
-        for (int h = 0; h < 2; h++) {   // past N: a clamped read that is not stored
-            const int b0x = bxs[h] * kBlock, b0y = bys[h] * kBlock;
-            va[h][it] = lumaAt(P.la, b0x - sxs[h] - R + x, b0y - sys[h] - R + y, P.w, P.h);
-            vc[h][it] = lumaAt(P.lc, b0x + sxs[h] - R + x, b0y + sys[h] - R + y, P.w, P.h);
-        }
+        w.va[it] = lumaAt(P.la, b0x - sx - R + x, b0y - sy - R + y, P.w, P.h);
+        w.vc[it] = lumaAt(P.lc, b0x + sx - R + x, b0y + sy - R + y, P.w, P.h);
         x += r;
         y += q;
         if (x >= T) {
@@ -519,15 +521,18 @@ __device__ __forceinline__ void searchWindowsPair(const SearchParams& P, float* 
             y++;
         }
     }
+}
+
+template <int R>
+__device__ __forceinline__ void searchWinStore(const SearchParams& P, const SearchWin<R>& w, float* A, float* C, int lane,
+                                               bool ints) {
+    constexpr int N = SearchWin<R>::N, IT = SearchWin<R>::IT;
     if (ints) {   // (uniform) the 16-bit pairs of a dyadic level
         const float up = __int_as_float((127 + P.fb) << 23);
 #pragma unroll
         for (int it = 0; it < IT; it++) {
-            const int i = lane + 64 * it;
-            searchStoreInt((uint16_t*)A0, i, N, va[0][it], up);
-            searchStoreInt((uint16_t*)C0, i, N, vc[0][it], up);
-            searchStoreInt((uint16_t*)A1, i, N, va[1][it], up);
-            searchStoreInt((uint16_t*)C1, i, N, vc[1][it], up);
+            searchStoreInt((uint16_t*)A, lane + 64 * it, N, w.va[it], up);
+            searchStoreInt((uint16_t*)C, lane + 64 * it, N, w.vc[it], up);
         }
         return;
     }
@@ -535,10 +540,8 @@ __device__ __forceinline__ void searchWindowsPair(const SearchParams& P, float* 
     for (int it = 0; it < IT; it++) {
         const int i = lane + 64 * it;
         if (i < N) {
-            A0[i] = va[0][it];
-            C0[i] = vc[0][it];
-            A1[i] = va[1][it];
-            C1[i] = vc[1][it];
+            A[i] = w.va[it];
+            C[i] = w.vc[it];
         }
     }
 }
@@ -669,17 +672,30 @@ __device__ __forceinline__ void search2BodyR(const SearchParams& P, int bx0, flo
         bxs[h] = acts[h] ? blk % P.bw : 0;
         bys[h] = acts[h] ? blk / P.bw : 0;
     }
+    SearchWin<R> win[2];
     if (P.parent || P.extraSeed) {   // both blocks' seeds side by side, each step for the two together
         SeedSet S[2];
 #pragma unroll
         for (int h = 0; h < 2; h++) seedVectors(P, bx0 * kSearch2Blocks + 2 * wave + h, acts[h], bxs[h], bys[h], S[h]);
+        // the windows at the first candidate's vector (the own parent's, or zero on the coarsest level), which is
+        // usually the seed, loaded with the candidates' texels: one memory latency instead of two
+#pragma unroll
+        for (int h = 0; h < 2; h++) searchWinLoad<R>(P, bxs[h], bys[h], S[h].cx[0], S[h].cy[0], lane, win[h]);
 #pragma unroll
         for (int h = 0; h < 2; h++) seedTexels(P, bxs[h], bys[h], lane, S[h]);
 #pragma unroll
         for (int h = 0; h < 2; h++) seedPick(P, S[h], sxs[h], sys[h]);
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+            if (sxs[h] != S[h].cx[0] || sys[h] != S[h].cy[0])   // (wave-uniform) another seed: its windows
+                searchWinLoad<R>(P, bxs[h], bys[h], sxs[h], sys[h], lane, win[h]);
+    } else {
+#pragma unroll
+        for (int h = 0; h < 2; h++) searchWinLoad<R>(P, bxs[h], bys[h], 0, 0, lane, win[h]);
     }
     const bool ints = P.fb >= 0;   // (uniform) packed 16-bit SADs
-    searchWindowsPair<R>(P, ta[2 * wave], tc[2 * wave], ta[2 * wave + 1], tc[2 * wave + 1], bxs, bys, sxs, sys, lane, ints);
+#pragma unroll
+    for (int h = 0; h < 2; h++) searchWinStore<R>(P, win[h], ta[2 * wave + h], tc[2 * wave + h], lane, ints);
     __syncthreads();
     const int half = lane >> 5, ci = lane & 31;
     const int blk = bx0 * kSearch2Blocks + 2 * wave + half;
