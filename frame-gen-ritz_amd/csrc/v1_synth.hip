@@ -345,12 +345,39 @@ __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, 
             s23 = pkAdd16(s23, q.y);
         }
         const int sums[4] = {(int)(s01 & 0xFFFFu), (int)(s01 >> 16), (int)(s23 & 0xFFFFu), (int)(s23 >> 16)};
+        // every pixel of the wavefront's runs trusted (u = 0 at S <= sZero <= sHalf: none counted): the blend of
+        // u = 0 for all four without a branch a pixel; the same values as blendRgba8(.., 0)
+        const bool trusted = max(max(sums[0], sums[1]), max(sums[2], sums[3])) <= P.sZero;
+        if (__all(trusted)) {
+            if (P.mid) {
 #pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const int S49 = sums[i];
-            const bool inside = x + i < P.W;
-            mine += (inside && S49 > P.sHalf) ? 1u : 0u;
-            res[i] = blendRgba8(P, wa[4 * k + i], wc[4 * k + i], nr[i], trustOf(P, S49));
+                for (int i = 0; i < 4; i++)
+                    res[i] = (__builtin_amdgcn_lerp(wa[4 * k + i], wc[4 * k + i], 0x01010101u) & 0x00FFFFFFu) |
+                             (nr[i] & 0xFF000000u);
+            } else if (P.quarter) {
+                const uint32_t kk = (uint32_t)P.kA | ((uint32_t)P.kC << 8);
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    uint32_t r = nr[i] & 0xFF000000u;
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        const uint32_t ac = __builtin_amdgcn_perm(wc[4 * k + i], wa[4 * k + i], 0x0C0C0400u + 0x0101u * (uint32_t)c);
+                        r |= (__builtin_amdgcn_udot4(ac, kk, 2u, false) >> 2) << (8 * c);
+                    }
+                    res[i] = r;
+                }
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; i++) res[i] = blendRgba8(P, wa[4 * k + i], wc[4 * k + i], nr[i], 0.0f);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int S49 = sums[i];
+                const bool inside = x + i < P.W;
+                mine += (inside && S49 > P.sHalf) ? 1u : 0u;
+                res[i] = blendRgba8(P, wa[4 * k + i], wc[4 * k + i], nr[i], trustOf(P, S49));
+            }
         }
         if (full) {
             U32x4 o;
