@@ -435,7 +435,9 @@ __device__ __forceinline__ void synthFmtTile(const SynthParams& P, int tileX, in
         // the field vector only for the chroma of an even-even Nv12 / P010 pixel
         constexpr bool kYuv = F == kNv12 || F == kP010;
         const float2 r = kYuv && ((x | y) & 1) == 0 ? fieldAtRegion<S>(P, hrow, ty0, cx + AP, y) : make_float2(0.0f, 0.0f);
-        resolveFmt<F, UI>(P, kA32, kC32, x, y, r, trustOf(P, S49), mix[k]);
+        // a wavefront whose pixels are all trusted (S <= sZero: u = 0) skips the trust ramp's per-pixel branches
+        const float u = __all(S49 <= P.sZero) ? 0.0f : trustOf(P, S49);
+        resolveFmt<F, UI>(P, kA32, kC32, x, y, r, u, mix[k]);
     }
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) mine += __shfl_xor(mine, o, 64);
