@@ -68,26 +68,20 @@ __device__ __forceinline__ RunMap runMap(int W, int x, int y, float k, float2 r)
     return m;
 }
 
-// Two 16-bit lanes of one register: a b + c and a b per lane (no carry between lanes; every value here < 2^16), and
+// Two 16-bit lanes of one register added lane by lane (no carry between lanes; every value here < 2^16), and
 // a.lo b.lo + a.hi b.hi + c in 32 bits
 typedef unsigned short U16x2 __attribute__((ext_vector_type(2)));
-__device__ __forceinline__ uint32_t pkMad16(uint32_t a, uint32_t b, uint32_t c) {
-    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(U16x2, a) * __builtin_bit_cast(U16x2, b) +
-                                            __builtin_bit_cast(U16x2, c));
-}
 __device__ __forceinline__ uint32_t pkAdd16(uint32_t a, uint32_t b) {
     return __builtin_bit_cast(uint32_t, __builtin_bit_cast(U16x2, a) + __builtin_bit_cast(U16x2, b));
-}
-__device__ __forceinline__ uint32_t pkMul16(uint32_t a, uint32_t b) {
-    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(U16x2, a) * __builtin_bit_cast(U16x2, b));
 }
 __device__ __forceinline__ uint32_t dot2u16(uint32_t a, uint32_t b, uint32_t c) {
     return __builtin_amdgcn_udot2(__builtin_bit_cast(U16x2, a), __builtin_bit_cast(U16x2, b), c, false);
 }
 
 // the four warps of a run whose map passed runMap (ok). The run shares fy, so each of its 5 texel columns is blended
-// vertically once, two channels a register: V = (32 - fy) top + fy bottom <= 8160 per 16-bit lane ([R, B] and [G, A]).
-// Each pixel then takes one dot product per channel over its two columns with the horizontal weights x 64 and
+// vertically once per channel: the column's top and bottom bytes interleaved by one byte permute ([R, R', G, G'] and
+// [B, B', 0, 0]) and dotted with the bytes [32 - fy, fy] (v_dot4_u32_u8), V = (32 - fy) top + fy bottom <= 8160.
+// Each pixel then takes one dot product per channel over its two columns' V with the horizontal weights x 64 and
 // 64 x 512 added: 64 ((32-fx)(32-fy) p00 + fx(32-fy) p01 + (32-fx)fy p10 + fx fy p11 + 512) < 2^24, whose byte 2 is
 // warpRgba8's (sum + 512) >> 10 -- the same integers, regrouped.
 __device__ __forceinline__ void warpRun4(const uint8_t* base, uint32_t pitch, int H, const RunMap& m, uint32_t* out) {
@@ -101,21 +95,23 @@ __device__ __forceinline__ void warpRun4(const uint8_t* base, uint32_t pitch, in
     const U32x4 b4 = *(const U32x4*)r1;
     const uint32_t b5 = *(const uint32_t*)(r1 + 16);
     const uint32_t ta[5] = {a4.x, a4.y, a4.z, a4.w, a5}, tb[5] = {b4.x, b4.y, b4.z, b4.w, b5};
-    const uint32_t gy2 = (32u - fy) * 0x10001u, fy2 = fy * 0x10001u;
+    const uint32_t wv = (32u - fy) | (fy << 8);           // bytes [32 - fy, fy, 0, 0]
     const uint32_t hw = ((32u - fx) << 6) | (fx << 22);   // [64 (32 - fx), 64 fx]
-    uint32_t vRB[5], vGA[5];
+    uint32_t vR[5], vG[5], vB[5];
 #pragma unroll
     for (int j = 0; j < 5; j++) {
-        vRB[j] = pkMad16(ta[j] & 0x00FF00FFu, gy2, pkMul16(tb[j] & 0x00FF00FFu, fy2));
-        vGA[j] = pkMad16(__builtin_amdgcn_perm(0u, ta[j], 0x0C030C01u), gy2,
-                         pkMul16(__builtin_amdgcn_perm(0u, tb[j], 0x0C030C01u), fy2));
+        const uint32_t rg = __builtin_amdgcn_perm(tb[j], ta[j], 0x05010400u);   // [top R, bottom R, top G, bottom G]
+        const uint32_t bb = __builtin_amdgcn_perm(tb[j], ta[j], 0x0C0C0602u);   // [top B, bottom B, 0, 0]
+        vR[j] = __builtin_amdgcn_udot4(rg, wv, 0u, false);
+        vG[j] = __builtin_amdgcn_udot4(rg, wv << 16, 0u, false);
+        vB[j] = __builtin_amdgcn_udot4(bb, wv, 0u, false);
     }
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        // [V(i), V(i + 1)] of R (low lanes of vRB), B (high lanes) and G (low lanes of vGA)
-        const uint32_t sr = dot2u16(__builtin_amdgcn_perm(vRB[i + 1], vRB[i], 0x05040100u), hw, 32768u);
-        const uint32_t sb = dot2u16(__builtin_amdgcn_perm(vRB[i + 1], vRB[i], 0x07060302u), hw, 32768u);
-        const uint32_t sg = dot2u16(__builtin_amdgcn_perm(vGA[i + 1], vGA[i], 0x05040100u), hw, 32768u);
+        // [V(i), V(i + 1)] of each channel as two 16-bit lanes
+        const uint32_t sr = dot2u16(__builtin_amdgcn_perm(vR[i + 1], vR[i], 0x05040100u), hw, 32768u);
+        const uint32_t sg = dot2u16(__builtin_amdgcn_perm(vG[i + 1], vG[i], 0x05040100u), hw, 32768u);
+        const uint32_t sb = dot2u16(__builtin_amdgcn_perm(vB[i + 1], vB[i], 0x05040100u), hw, 32768u);
         out[i] = __builtin_amdgcn_perm(sg, sr, 0x0C0C0602u) | (sb & 0x00FF0000u);   // bytes R, G, B, alpha 0
     }
 }
@@ -200,7 +196,6 @@ __device__ __forceinline__ void warpRegionRun(const SynthParams& P, const float2
     const int yc = min(y, S * P.fh - 1);
     const float2* h = hrow + (FlowMap<S>::index(yc) - ty0) * EW + ex0;
     const float fy = FlowMap<S>::frac(yc);
-    float2 r[4];
     int xs[4];
     float2 ht[4], hb[4];
     bool hsame = true;   // the run's two interpolated rows hold one value each, bit for bit: one field value
@@ -210,24 +205,21 @@ __device__ __forceinline__ void warpRegionRun(const SynthParams& P, const float2
         // a clamped column repeats the edge pixel: its hrow entry is that pixel's (clamped the same way)
         ht[i] = h[i];
         hb[i] = h[i + EW];
-        if (i > 0)
-            hsame = hsame && __float_as_uint(ht[i].x) == __float_as_uint(ht[0].x) &&
-                    __float_as_uint(ht[i].y) == __float_as_uint(ht[0].y) &&
-                    __float_as_uint(hb[i].x) == __float_as_uint(hb[0].x) &&
-                    __float_as_uint(hb[i].y) == __float_as_uint(hb[0].y);
+        if (i > 0)   // (no short-circuit: every load issued before the first compare)
+            hsame = hsame & (__builtin_bit_cast(uint64_t, ht[i]) == __builtin_bit_cast(uint64_t, ht[0])) &
+                    (__builtin_bit_cast(uint64_t, hb[i]) == __builtin_bit_cast(uint64_t, hb[0]));
     }
-    bool same;
-    if (hsame) {
-        r[0] = mix2(ht[0], hb[0], fy);
-        r[1] = r[2] = r[3] = r[0];
-        same = xs[3] == xs[0] + 3;
-    } else {
+    // the run's field value; the others are formed (and compared) only when the rows are not one value each
+    const float2 r0 = mix2(ht[0], hb[0], fy);
+    bool same = xs[3] == xs[0] + 3;
+    if (!hsame) {
 #pragma unroll
-        for (int i = 0; i < 4; i++) r[i] = mix2(ht[i], hb[i], fy);
-        same = xs[3] == xs[0] + 3 && r[1].x == r[0].x && r[1].y == r[0].y && r[2].x == r[0].x && r[2].y == r[0].y &&
-               r[3].x == r[0].x && r[3].y == r[0].y;
+        for (int i = 1; i < 4; i++) {
+            const float2 ri = mix2(ht[i], hb[i], fy);
+            same = same && ri.x == r0.x && ri.y == r0.y;
+        }
     }
-    const RunMap ma = runMap(P.W, xs[0], y, kA32, r[0]), mc = runMap(P.W, xs[0], y, -kC32, r[0]);
+    const RunMap ma = runMap(P.W, xs[0], y, kA32, r0), mc = runMap(P.W, xs[0], y, -kC32, r0);
     if (__all(same && ma.ok && mc.ok)) {
         warpRun4(P.a.p, (uint32_t)P.a.pitch, P.H, ma, oa);
         warpRun4(P.c.p, (uint32_t)P.c.pitch, P.H, mc, oc);
@@ -239,7 +231,7 @@ __device__ __forceinline__ void warpRegionRun(const SynthParams& P, const float2
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             Rgb a, c;
-            warpPair<UI>(P, kA32, kC32, xs[i], y, r[i], a, c);
+            warpPair<UI>(P, kA32, kC32, xs[i], y, mix2(ht[i], hb[i], fy), a, c);
             oa[i] = packRgb(a);
             oc[i] = packRgb(c);
         }
