@@ -116,9 +116,11 @@ __device__ __forceinline__ void warpRun4(const uint8_t* base, uint32_t pitch, in
     }
 }
 
+// the same from the packed bytes: max |d| over the three channel differences d = max(max d, -min d)
 __device__ __forceinline__ int emaxPacked(uint32_t a, uint32_t c) {
-    return (int)max(max(__sad(a & 255u, c & 255u, 0u), __sad((a >> 8) & 255u, (c >> 8) & 255u, 0u)),
-                    __sad((a >> 16) & 255u, (c >> 16) & 255u, 0u));
+    const int dr = (int)(a & 255u) - (int)(c & 255u), dg = (int)((a >> 8) & 255u) - (int)((c >> 8) & 255u),
+              db = (int)((a >> 16) & 255u) - (int)((c >> 16) & 255u);
+    return max(max(max(dr, dg), db), -min(min(dr, dg), db));
 }
 
 // error.comp:9-21 / mismatch() motion_v1.glsl:117-129: e = max-channel |wa - wc| (integers 0..255)
@@ -155,6 +157,25 @@ __device__ __forceinline__ void warpPairRgba8(const SynthParams& P, int x, int y
     if (P.uiA.w > 0 || P.uiC.w > 0) uiReject(P, (int)rintf(xa), (int)rintf(ya), (int)rintf(xc), (int)rintf(yc), wa, wc);
 }
 
+// The trusted quarter-t blend (kA a + kC c + 2) >> 2 of every byte, (kA, kC) = (3, 1) or (1, 3): with the per-byte
+// floor mean u = (a + c) >> 1 (v_lerp_u8, rounding bits 0), 3a + c + 2 = 2 (a + u + 1) + ((a + c) & 1), whose quotient
+// by 4 rounded down is (a + u + 1) >> 1: a second v_lerp_u8 with rounding bits 1 (a + 3c the same with c). Alpha
+// bytes of wa and wc are 0, so the result's alpha byte is 0; other (kA, kC): per channel dot products.
+__device__ __forceinline__ uint32_t quarterRgba8(const SynthParams& P, uint32_t wa, uint32_t wc) {
+    if (P.kA == 3 || P.kA == 1) {
+        const uint32_t u = __builtin_amdgcn_lerp(wa, wc, 0u);
+        return __builtin_amdgcn_lerp(P.kA == 3 ? wa : wc, u, 0x01010101u);
+    }
+    const uint32_t k = (uint32_t)P.kA | ((uint32_t)P.kC << 8);
+    uint32_t r = 0u;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const uint32_t ac = __builtin_amdgcn_perm(wc, wa, 0x0C0C0400u + 0x0101u * (uint32_t)c);   // [a_c, c_c, 0, 0]
+        r |= (__builtin_amdgcn_udot4(ac, k, 2u, false) >> 2) << (8 * c);
+    }
+    return r;
+}
+
 // resolve.comp:73-82 (interp.py:241-246): m = 0.5 (wa + wc) at t = 0.5, else (1-t) wa + t wc; o = m (1 - u) + near u;
 // out = clamp(floor(o + 0.5), 0, 255); alpha from the nearer frame. Two exact shortcuts of the same float32 formula:
 // u = 1 gives o = near; u = 0 gives o = m, and when 4(1-t) and 4t are integers m + 0.5 is exact, so
@@ -163,16 +184,8 @@ __device__ __forceinline__ uint32_t blendRgba8(const SynthParams& P, uint32_t wa
     if (u == 1.0f) return nr;
     if (u == 0.0f && P.mid)   // (a + c + 1) >> 1 per byte (v_lerp_u8), alpha from the nearer frame
         return (__builtin_amdgcn_lerp(wa, wc, 0x01010101u) & 0x00FFFFFFu) | (nr & 0xFF000000u);
+    if (u == 0.0f && P.quarter) return quarterRgba8(P, wa, wc) | (nr & 0xFF000000u);
     uint32_t r = nr & 0xFF000000u;
-    if (u == 0.0f && P.quarter) {   // per channel the byte pair (a_c, c_c) dotted with (kA, kC), plus 2, >> 2
-        const uint32_t k = (uint32_t)P.kA | ((uint32_t)P.kC << 8);
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const uint32_t ac = __builtin_amdgcn_perm(wc, wa, 0x0C0C0400u + 0x0101u * (uint32_t)c);   // [a_c, c_c, 0, 0]
-            r |= (__builtin_amdgcn_udot4(ac, k, 2u, false) >> 2) << (8 * c);
-        }
-        return r;
-    }
     const float omu = 1.0f - u;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
@@ -347,17 +360,8 @@ __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, 
                     res[i] = (__builtin_amdgcn_lerp(wa[4 * k + i], wc[4 * k + i], 0x01010101u) & 0x00FFFFFFu) |
                              (nr[i] & 0xFF000000u);
             } else if (P.quarter) {
-                const uint32_t kk = (uint32_t)P.kA | ((uint32_t)P.kC << 8);
 #pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    uint32_t r = nr[i] & 0xFF000000u;
-#pragma unroll
-                    for (int c = 0; c < 3; c++) {
-                        const uint32_t ac = __builtin_amdgcn_perm(wc[4 * k + i], wa[4 * k + i], 0x0C0C0400u + 0x0101u * (uint32_t)c);
-                        r |= (__builtin_amdgcn_udot4(ac, kk, 2u, false) >> 2) << (8 * c);
-                    }
-                    res[i] = r;
-                }
+                for (int i = 0; i < 4; i++) res[i] = quarterRgba8(P, wa[4 * k + i], wc[4 * k + i]) | (nr[i] & 0xFF000000u);
             } else {
 #pragma unroll
                 for (int i = 0; i < 4; i++) res[i] = blendRgba8(P, wa[4 * k + i], wc[4 * k + i], nr[i], 0.0f);
