@@ -11,6 +11,15 @@ namespace fgr {
 // v clamped to [lo, hi] (lo <= hi at every use): one v_med3_i32
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return min(max(v, lo), hi); }
 
+// |a - b| + c of unsigned values as one v_sad_u32. Written out: HIP's __sad forms the signed difference, its negation,
+// their maximum and the sum (four instructions; the same value for operands below 2^31), and umax - umin + c inside a
+// running sum is reassociated by the compiler into separate maxima, minima and adds (about three an element).
+__device__ __forceinline__ uint32_t sadU32(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t r;
+    asm("v_sad_u32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
 // Row-major element index y w + x of in-range coordinates as a 32-bit unsigned value (y, w < 2^24: a full-rate 24-bit
 // multiply), and the element at it: the byte offset stays 32-bit, so a plane whose base is uniform is read with the
 // base in scalar registers plus one vector offset.

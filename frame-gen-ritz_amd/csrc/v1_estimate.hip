@@ -1008,7 +1008,7 @@ __device__ __forceinline__ void subpixIntBody(const SubpixParams& P, int g, int)
             for (int x = 0; x < 8; x++) {
                 const uint32_t tA = pairDot(pa1[x], wa1, pairDot(pa0[x], wa0, 0u));
                 const uint32_t tC = pairDot(pc1[x], wc1, pairDot(pc0[x], wc0, 0u));
-                total = __sad(tA, tC, total);
+                total = sadU32(tA, tC, total);
             }
 #pragma unroll
             for (int x = 0; x < 8; x++) {
@@ -1269,7 +1269,7 @@ __device__ __forceinline__ void winSample4(const uint32_t* w, const PixWin& W, i
 __device__ __forceinline__ uint32_t winErr(const PixWin& W, int qx, int qy, float2 v) {
     const int Xa = (int)rintf(((float)qx - v.x) * 32.0f), Ya = (int)rintf(((float)qy - v.y) * 32.0f);
     const int Xc = (int)rintf(((float)qx + v.x) * 32.0f), Yc = (int)rintf(((float)qy + v.y) * 32.0f);
-    return __sad(winSample(W.a, W, Xa, Ya), winSample(W.c, W, Xc, Yc), 0u);
+    return sadU32(winSample(W.a, W, Xa, Ya), winSample(W.c, W, Xc, Yc), 0u);
 }
 
 // sampleRun4 in two steps, so that a caller issues the loads of both frames (and more) before any arithmetic: the plan
@@ -1376,7 +1376,7 @@ struct WinErrs {
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const int o = (qy - W.y0) * W.stride + (clampi(gx0 + i, 0, w - 1) - W.x0);
-            e[i] = __sad(W.a[o] & 0xFFFFu, W.c[o] & 0xFFFFu, 0u) << 10;
+            e[i] = sadU32(W.a[o] & 0xFFFFu, W.c[o] & 0xFFFFu, 0u) << 10;
         }
     }
     __device__ __forceinline__ void run(int gx0, int qy, float2 v, uint32_t* e) const {
@@ -1388,7 +1388,7 @@ struct WinErrs {
             winSample4(W.a, W, Xa, Ya, sa);
             winSample4(W.c, W, Xc, Yc, sc);
 #pragma unroll
-            for (int i = 0; i < 4; i++) e[i] = __sad(sa[i], sc[i], 0u);
+            for (int i = 0; i < 4; i++) e[i] = sadU32(sa[i], sc[i], 0u);
         } else {
 #pragma unroll
             for (int i = 0; i < 4; i++) e[i] = winErr(W, gx0 + i, qy, v);

@@ -125,7 +125,7 @@ __device__ __forceinline__ int emaxPacked(uint32_t a, uint32_t c) {
 
 // error.comp:9-21 / mismatch() motion_v1.glsl:117-129: e = max-channel |wa - wc| (integers 0..255)
 __device__ __forceinline__ int emaxRgb(const Rgb& a, const Rgb& c) {
-    return (int)max(max(__sad(a.r, c.r, 0u), __sad(a.g, c.g, 0u)), __sad(a.b, c.b, 0u));
+    return (int)max(max(sadU32(a.r, c.r, 0u), sadU32(a.g, c.g, 0u)), sadU32(a.b, c.b, 0u));
 }
 
 __device__ __forceinline__ int emaxRgba8(uint32_t wa, uint32_t wc) {
