@@ -894,10 +894,14 @@ __device__ __forceinline__ void subpixBody(const SubpixParams& P, int g, int) {
 // 2^-(fb + 4) and every cost, choice and scene-cut term is the float path's. A tap is two v_dot2_u32_u16 over the texel
 // pairs of its rows with the weight pairs (W00, W01) and (W10, W11), an error one v_sad_u32 into the running total.
 //
-// Windows: each frame's 13 x 13 texels of a block as u16 rows of 14 stored twice, E16[i] = t[i] and O16[i] = t[i + 1],
-// so that the texel pair (t[i], t[i + 1]) is the aligned word i / 2 of E16 (i even) or (i - 1) / 2 of O16 (i odd).
+// Windows: the offsets stay within 0.75 px of the integer vector, so a pixel x + O of the float path's 13 x 13 tile taps
+// columns floor(x + O -/+ o) .. + 1 within 1 .. 10 (rows alike): each frame's 10 x 10 texels of a block (the tile's
+// rows and columns 1 .. 10) as u16 rows of 10 stored twice, E16[i] = t[i] and O16[i] = t[i + 1], so that the texel pair
+// (t[i], t[i + 1]) is the aligned word i / 2 of E16 (i even) or (i - 1) / 2 of O16 (i odd). The windows' 5.6 KB of LDS
+// a workgroup leave room for 6 wavefronts a SIMD.
 __device__ __forceinline__ void subpixIntBody(const SubpixParams& P, int g, int) {
-    constexpr int T = 13, TS = 14, O = 2, NB = kSubpixBlocks, WB = T * TS;   // WB u16 a block window
+    constexpr int T = 10, TS = 10, O = 2, NB = kSubpixBlocks, WB = T * TS;   // WB u16 a block window
+    static_assert(TS % 2 == 0 && TS >= T, "rows of whole texel-pair words");
     __shared__ __align__(16) uint16_t ea[NB * WB], oa[NB * WB], ec[NB * WB], oc[NB * WB];
     __shared__ float cst[NB][9], sads[NB][9];
     __shared__ int sv[NB][2], sb[NB][2];
@@ -921,8 +925,8 @@ __device__ __forceinline__ void subpixIntBody(const SubpixParams& P, int g, int)
         sb[lane][1] = (b / P.bw) * kBlock;
     }
     __syncthreads();
-    // the 7 blocks' windows, block by block (the block's corner and seed wave-uniform), three 64-texel chunks of its
-    // 13 x 13 texels a block (a lane's row and column fixed per chunk); every load issued before any store
+    // the 7 blocks' windows, block by block (the block's corner and seed wave-uniform), two 64-texel chunks of its
+    // 10 x 10 texels a block (a lane's row and column fixed per chunk); every load issued before any store
     constexpr int KC = (T * T + 63) / 64;
     const float up = __int_as_float((127 + P.fb) << 23);
     int cr[KC], cc[KC];
@@ -939,8 +943,8 @@ __device__ __forceinline__ void subpixIntBody(const SubpixParams& P, int g, int)
         const int vx = __builtin_amdgcn_readfirstlane(sv[jb][0]), vy = __builtin_amdgcn_readfirstlane(sv[jb][1]);
 #pragma unroll
         for (int kc = 0; kc < KC; kc++) {
-            va[jb][kc] = lumaAt(P.la, bx - vx - O + cc[kc], by - vy - O + cr[kc], P.w, P.h);
-            vc[jb][kc] = lumaAt(P.lc, bx + vx - O + cc[kc], by + vy - O + cr[kc], P.w, P.h);
+            va[jb][kc] = lumaAt(P.la, bx - vx - O + 1 + cc[kc], by - vy - O + 1 + cr[kc], P.w, P.h);
+            vc[jb][kc] = lumaAt(P.lc, bx + vx - O + 1 + cc[kc], by + vy - O + 1 + cr[kc], P.w, P.h);
         }
     }
 #pragma unroll
@@ -979,14 +983,14 @@ __device__ __forceinline__ void subpixIntBody(const SubpixParams& P, int g, int)
         const uint32_t qcx = (uint32_t)((pcx - flcx) * 4.0f), qcy = (uint32_t)((pcy - flcy) * 4.0f);
         const uint32_t wa0 = ((4u - qay) * (4u - qax)) | (((4u - qay) * qax) << 16), wa1 = (qay * (4u - qax)) | ((qay * qax) << 16);
         const uint32_t wc0 = ((4u - qcy) * (4u - qcx)) | (((4u - qcy) * qcx) << 16), wc1 = (qcy * (4u - qcx)) | ((qcy * qcx) << 16);
-        // the texel pair at column c of a row: word (row TS + c) >> 1 of E (c even) or O (c odd); pixel x reads column
-        // col + x, so even x keeps col's parity and odd x flips it
-        const int colA = (int)flax, colC = (int)flcx;
+        // the texel pair at window column c of a row: word (row TS + c) >> 1 of E (c even) or O (c odd); pixel x reads
+        // window column col + x (the tile's column col + 1 + x), so even x keeps col's parity and odd x flips it
+        const int colA = (int)flax - 1, colC = (int)flcx - 1;   // 0 or 1
         const uint32_t* A0 = (colA & 1) ? OA : EA;
         const uint32_t* A1 = (colA & 1) ? EA : OA;
         const uint32_t* C0 = (colC & 1) ? OC : EC;
         const uint32_t* C1 = (colC & 1) ? EC : OC;
-        const int ra = (int)flay * (TS / 2), rc = (int)flcy * (TS / 2);
+        const int ra = ((int)flay - 1) * (TS / 2), rc = ((int)flcy - 1) * (TS / 2);
         const int a0 = ra + (colA >> 1), a1 = ra + ((colA + 1) >> 1), c0 = rc + (colC >> 1), c1 = rc + ((colC + 1) >> 1);
         uint32_t pa0[8], pc0[8];
 #pragma unroll
