@@ -56,9 +56,9 @@ __device__ __forceinline__ float sampleL(const float* img, float px, float py, i
     return p00 * ((1.0f - fy) * (1.0f - fx)) + p01 * ((1.0f - fy) * fx) + p10 * (fy * (1.0f - fx)) + p11 * (fy * fx);
 }
 
-// sampleF() motion_v1.glsl:62-70: the same on the two-channel field
-__device__ __forceinline__ float2 sampleF(const float2* f, float px, float py, int w, int h) {
-    const int X = (int)rintf(px * 32.0f), Y = (int)rintf(py * 32.0f);
+// sampleF() motion_v1.glsl:62-70: the same on the two-channel field, at the map already scaled by 32 (X32 = 32 px)
+__device__ __forceinline__ float2 sampleF32(const float2* f, float X32, float Y32, int w, int h) {
+    const int X = (int)rintf(X32), Y = (int)rintf(Y32);
     const int ix = X >> 5, iy = Y >> 5;
     const float fx = (float)(X & 31) * (1.0f / 32.0f), fy = (float)(Y & 31) * (1.0f / 32.0f);
     const uint32_t x0 = (uint32_t)clampi(ix, 0, w - 1), x1 = (uint32_t)clampi(ix + 1, 0, w - 1);
@@ -67,6 +67,9 @@ __device__ __forceinline__ float2 sampleF(const float2* f, float px, float py, i
     const float w00 = (1.0f - fy) * (1.0f - fx), w01 = (1.0f - fy) * fx, w10 = fy * (1.0f - fx), w11 = fy * fx;
     return make_float2(p00.x * w00 + p01.x * w01 + p10.x * w10 + p11.x * w11,
                        p00.y * w00 + p01.y * w01 + p10.y * w10 + p11.y * w11);
+}
+__device__ __forceinline__ float2 sampleF(const float2* f, float px, float py, int w, int h) {
+    return sampleF32(f, px * 32.0f, py * 32.0f, w, h);
 }
 
 // halfVec() motion_v1.glsl:73-84: cv2.resize(field, x S, INTER_LINEAR) + edge pad, times S, at full-res pixel (x, y)

@@ -105,6 +105,9 @@ __device__ __forceinline__ int loadFieldTile(const SynthParams& P, int x0, int y
         v[it] = ldIdx(P.field, idx2(qx, qy, P.fw));
     }
     if (P.lookup) {   // lookup.comp:10-19, g <- pf(q - s g) twice, the IT entries interleaved step by step
+        // the map 32 (q - s g) as 32 q - (32 s) g: scaling by 32 commutes with the rounding of s g and of the difference
+        // (field values are never subnormal), so it is the same float with one multiply fewer a coordinate
+        const float s32 = 32.0f * P.s;
 #pragma unroll
         for (int k = 0; k < 2; k++) {
 #pragma unroll
@@ -112,7 +115,8 @@ __device__ __forceinline__ int loadFieldTile(const SynthParams& P, int x0, int y
                 if (it == IT - 1 && !lastIt) continue;   // (wave-uniform)
                 const int i = min(tid + 256 * it, N - 1);
                 const int qx = med3i(tx0 + i % FTW, 0, P.fw - 1), qy = med3i(ty0 + i / FTW, 0, P.fh - 1);
-                v[it] = sampleF(P.field, (float)qx - P.s * v[it].x, (float)qy - P.s * v[it].y, P.fw, P.fh);
+                v[it] = sampleF32(P.field, (float)(32 * qx) - s32 * v[it].x, (float)(32 * qy) - s32 * v[it].y, P.fw,
+                                  P.fh);
             }
         }
     }
