@@ -185,4 +185,18 @@ __host__ __device__ __forceinline__ int xcdChunkTile(int i, int n, int chunk = k
         body(p, l % p.gx, l / p.gx);                                                                              \
     }
 
+// FGR_PASS asking the register allocator for `waves` wavefronts per SIMD in both kernels
+#define FGR_PASS_OCC(name, Params, body, threads, waves)                                                           \
+    __global__ __launch_bounds__(threads) __attribute__((amdgpu_waves_per_eu(waves))) void name##_kernel(Params P) { \
+        const int l = xcdChunkTile(linearBlock(), (int)(gridDim.x * gridDim.y));                                 \
+        body(P, l % (int)gridDim.x, l / (int)gridDim.x);                                                          \
+    }                                                                                                             \
+    __global__ __launch_bounds__(threads) __attribute__((amdgpu_waves_per_eu(waves))) void name##_batch_kernel(   \
+        const Params* __restrict__ P, const uint32_t* __restrict__ work) {                                        \
+        const uint32_t w = ((const FGR_CONST_SPACE uint32_t*)work)[blockIdx.x];                                   \
+        const Params p = ((const FGR_CONST_SPACE Params*)P)[w >> kWorkShift];                                     \
+        const int l = (int)(w & kWorkMask);                                                                       \
+        body(p, l % p.gx, l / p.gx);                                                                              \
+    }
+
 }  // namespace fgr

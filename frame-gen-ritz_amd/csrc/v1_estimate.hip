@@ -369,9 +369,16 @@ __device__ __forceinline__ void seedVectors(const SearchParams& P, int blk, bool
         }
         S.cx[kk] = cx;
         S.cy[kk] = cy;
+    }
+    // repeats, each vector as one 32-bit key (|cx|, |cy| < 2^15), compared without short-circuit
+    uint32_t key[kMaxSeeds];
+#pragma unroll
+    for (int kk = 0; kk < kMaxSeeds; kk++) key[kk] = ((uint32_t)S.cx[kk] & 0xFFFFu) | ((uint32_t)S.cy[kk] << 16);
+#pragma unroll
+    for (int kk = 0; kk < kMaxSeeds; kk++) {
         bool dup = kk >= S.nc;
 #pragma unroll
-        for (int j = 0; j < kk; j++) dup = dup || (S.cx[j] == cx && S.cy[j] == cy);
+        for (int j = 0; j < kk; j++) dup = dup | (key[j] == key[kk]);
         S.dup[kk] = dup;
     }
 }

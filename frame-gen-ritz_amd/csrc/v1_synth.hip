@@ -78,15 +78,18 @@ __device__ __forceinline__ uint32_t dot2u16(uint32_t a, uint32_t b, uint32_t c) 
     return __builtin_amdgcn_udot2(__builtin_bit_cast(U16x2, a), __builtin_bit_cast(U16x2, b), c, false);
 }
 
-// the four warps of a run whose map passed runMap (ok). The run shares fy, so each of its 5 texel columns is blended
+// The four warps of a run whose pixels share the map row Y and take the tap columns X_i >> 5 = ix + i (in the frame:
+// ix >= 0, ix + 4 <= W - 1), each with its own column weight fx_i = X_i & 31 -- a run whose map passed runMap (ok) is
+// the case X_i = X_0 + 32 i. The run shares fy and 5 consecutive texels a row, so each texel column is blended
 // vertically once per channel: the column's top and bottom bytes interleaved by one byte permute ([R, R', G, G'] and
 // [B, B', 0, 0]) and dotted with the bytes [32 - fy, fy] (v_dot4_u32_u8), V = (32 - fy) top + fy bottom <= 8160.
-// Each pixel then takes one dot product per channel over its two columns' V with the horizontal weights x 64 and
+// Each pixel then takes one dot product per channel over its two columns' V with its horizontal weights x 64 and
 // 64 x 512 added: 64 ((32-fx)(32-fy) p00 + fx(32-fy) p01 + (32-fx)fy p10 + fx fy p11 + 512) < 2^24, whose byte 2 is
 // warpRgba8's (sum + 512) >> 10 -- the same integers, regrouped.
-__device__ __forceinline__ void warpRun4(const uint8_t* base, uint32_t pitch, int H, const RunMap& m, uint32_t* out) {
-    const int ix = m.X0 >> 5, iy = m.Y >> 5;
-    const uint32_t fx = (uint32_t)(m.X0 & 31), fy = (uint32_t)(m.Y & 31);
+__device__ __forceinline__ void warpRow4(const uint8_t* base, uint32_t pitch, int H, int ix, int Y, const uint32_t* fx,
+                                         uint32_t* out) {
+    const int iy = Y >> 5;
+    const uint32_t fy = (uint32_t)(Y & 31);
     const uint32_t y0 = (uint32_t)med3i(iy, 0, H - 1), y1 = (uint32_t)med3i(iy + 1, 0, H - 1);
     const uint8_t* r0 = base + (__umul24(y0, pitch) + 4u * (uint32_t)ix);
     const uint8_t* r1 = base + (__umul24(y1, pitch) + 4u * (uint32_t)ix);
@@ -95,8 +98,7 @@ __device__ __forceinline__ void warpRun4(const uint8_t* base, uint32_t pitch, in
     const U32x4 b4 = *(const U32x4*)r1;
     const uint32_t b5 = *(const uint32_t*)(r1 + 16);
     const uint32_t ta[5] = {a4.x, a4.y, a4.z, a4.w, a5}, tb[5] = {b4.x, b4.y, b4.z, b4.w, b5};
-    const uint32_t wv = (32u - fy) | (fy << 8);           // bytes [32 - fy, fy, 0, 0]
-    const uint32_t hw = ((32u - fx) << 6) | (fx << 22);   // [64 (32 - fx), 64 fx]
+    const uint32_t wv = (32u - fy) | (fy << 8);   // bytes [32 - fy, fy, 0, 0]
     uint32_t vR[5], vG[5], vB[5];
 #pragma unroll
     for (int j = 0; j < 5; j++) {
@@ -108,12 +110,20 @@ __device__ __forceinline__ void warpRun4(const uint8_t* base, uint32_t pitch, in
     }
 #pragma unroll
     for (int i = 0; i < 4; i++) {
+        const uint32_t hw = ((32u - fx[i]) << 6) | (fx[i] << 22);   // [64 (32 - fx), 64 fx]
         // [V(i), V(i + 1)] of each channel as two 16-bit lanes
         const uint32_t sr = dot2u16(__builtin_amdgcn_perm(vR[i + 1], vR[i], 0x05040100u), hw, 32768u);
         const uint32_t sg = dot2u16(__builtin_amdgcn_perm(vG[i + 1], vG[i], 0x05040100u), hw, 32768u);
         const uint32_t sb = dot2u16(__builtin_amdgcn_perm(vB[i + 1], vB[i], 0x05040100u), hw, 32768u);
         out[i] = __builtin_amdgcn_perm(sg, sr, 0x0C0C0602u) | (sb & 0x00FF0000u);   // bytes R, G, B, alpha 0
     }
+}
+
+// a run whose map passed runMap: one column weight for the four
+__device__ __forceinline__ void warpRun4(const uint8_t* base, uint32_t pitch, int H, const RunMap& m, uint32_t* out) {
+    const uint32_t f = (uint32_t)(m.X0 & 31);
+    const uint32_t fx[4] = {f, f, f, f};
+    warpRow4(base, pitch, H, m.X0 >> 5, m.Y, fx, out);
 }
 
 // the same from the packed bytes: max |d| over the three channel differences d = max(max d, -min d)
@@ -233,21 +243,56 @@ __device__ __forceinline__ void warpRegionRun(const SynthParams& P, const float2
         }
     }
     const RunMap ma = runMap(P.W, xs[0], y, kA32, r0), mc = runMap(P.W, xs[0], y, -kC32, r0);
-    if (__all(same && ma.ok && mc.ok)) {
+    const bool fast = same && ma.ok && mc.ok;
+    if (__all(fast)) {
         warpRun4(P.a.p, (uint32_t)P.a.pitch, P.H, ma, oa);
         warpRun4(P.c.p, (uint32_t)P.c.pitch, P.H, mc, oc);
         if (UI) {   // the run's maps are X0 + 32 i at one Y
 #pragma unroll
             for (int i = 0; i < 4; i++) uiReject(P, ma.X0 + 32 * i, ma.Y, mc.X0 + 32 * i, mc.Y, oa[i], oc[i]);
         }
-    } else {
+        return;
+    }
+    // each pixel's maps as warpPair forms them; a run whose pixels share the map row and take consecutive tap columns
+    // (a field that varies by less than a pixel along the run) shares its texels and vertical blends (warpRow4)
+    int Xa[4], Xc[4], Ya = 0, Yc = 0;
+    bool row = xs[3] == xs[0] + 3;
+    const float y32 = (float)(32 * y);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const float2 ri = mix2(ht[i], hb[i], fy);
+        const float x32 = (float)(32 * xs[i]);
+        Xa[i] = (int)rintf(x32 - kA32 * ri.x);
+        Xc[i] = (int)rintf(x32 + kC32 * ri.x);
+        const int ya = (int)rintf(y32 - kA32 * ri.y), yc = (int)rintf(y32 + kC32 * ri.y);
+        if (i == 0) {
+            Ya = ya;
+            Yc = yc;
+        }
+        row = row & (ya == Ya) & (yc == Yc) & ((Xa[i] >> 5) == (Xa[0] >> 5) + i) & ((Xc[i] >> 5) == (Xc[0] >> 5) + i);
+    }
+    row = row & (Xa[0] >= 0) & ((Xa[0] >> 5) + 4 <= P.W - 1) & (Xc[0] >= 0) & ((Xc[0] >> 5) + 4 <= P.W - 1);
+    if (__all(row)) {
+        uint32_t fa[4], fc[4];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            Rgb a, c;
-            warpPair<UI>(P, kA32, kC32, xs[i], y, mix2(ht[i], hb[i], fy), a, c);
-            oa[i] = packRgb(a);
-            oc[i] = packRgb(c);
+            fa[i] = (uint32_t)(Xa[i] & 31);
+            fc[i] = (uint32_t)(Xc[i] & 31);
         }
+        warpRow4(P.a.p, (uint32_t)P.a.pitch, P.H, Xa[0] >> 5, Ya, fa, oa);
+        warpRow4(P.c.p, (uint32_t)P.c.pitch, P.H, Xc[0] >> 5, Yc, fc, oc);
+        if (UI) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) uiReject(P, Xa[i], Ya, Xc[i], Yc, oa[i], oc[i]);
+        }
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        Rgb a, c;
+        warpPair<UI>(P, kA32, kC32, xs[i], y, mix2(ht[i], hb[i], fy), a, c);
+        oa[i] = packRgb(a);
+        oc[i] = packRgb(c);
     }
 }
 
@@ -472,7 +517,7 @@ FGR_PASS(synth_rgba8_s4, SynthParams, FGR_SYNTH_BODY(4, 32), 256)
 FGR_PASS(synth_rgba8_s1_t16, SynthParams, FGR_SYNTH_BODY(1, 16), 256)
 FGR_PASS(synth_rgba8_s2_t16, SynthParams, FGR_SYNTH_BODY(2, 16), 256)
 FGR_PASS(synth_rgba8_s4_t16, SynthParams, FGR_SYNTH_BODY(4, 16), 256)
-FGR_PASS(synth_rgba8_s2_t64, SynthParams, FGR_SYNTH_BODY(2, 64), 256)
+FGR_PASS_OCC(synth_rgba8_s2_t64, SynthParams, FGR_SYNTH_BODY(2, 64), 256, 5)   // 5 workgroups a CU (30.5 KB of LDS each)
 #define FGR_SYNTH_UI_BODY(S) synthRgba8Body<S, 32, true>
 FGR_PASS_SINGLE(synth_rgba8_ui_s1, SynthParams, FGR_SYNTH_UI_BODY(1), 256)
 FGR_PASS_SINGLE(synth_rgba8_ui_s2, SynthParams, FGR_SYNTH_UI_BODY(2), 256)
