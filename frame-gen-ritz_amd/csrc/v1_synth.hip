@@ -119,6 +119,27 @@ __device__ __forceinline__ void warpRow4(const uint8_t* base, uint32_t pitch, in
     }
 }
 
+// warpRgba8 of one pixel whose tap columns ix, ix + 1 lie in the frame (0 <= ix <= W - 2), as warpRow4 forms it: each
+// tap column blended vertically per channel by byte dot products, then one 16-bit pair dot product per channel
+__device__ __forceinline__ uint32_t warpPix(const uint8_t* base, uint32_t pitch, int H, int X, int Y) {
+    const int ix = X >> 5, iy = Y >> 5;
+    const uint32_t fx = (uint32_t)(X & 31), fy = (uint32_t)(Y & 31);
+    const uint32_t y0 = (uint32_t)med3i(iy, 0, H - 1), y1 = (uint32_t)med3i(iy + 1, 0, H - 1);
+    const U32x2 q0 = *(const U32x2*)(base + (__umul24(y0, pitch) + 4u * (uint32_t)ix));
+    const U32x2 q1 = *(const U32x2*)(base + (__umul24(y1, pitch) + 4u * (uint32_t)ix));
+    const uint32_t wv = (32u - fy) | (fy << 8);
+    const uint32_t hw = ((32u - fx) << 6) | (fx << 22);
+    const uint32_t rg0 = __builtin_amdgcn_perm(q1.x, q0.x, 0x05010400u), bb0 = __builtin_amdgcn_perm(q1.x, q0.x, 0x0C0C0602u);
+    const uint32_t rg1 = __builtin_amdgcn_perm(q1.y, q0.y, 0x05010400u), bb1 = __builtin_amdgcn_perm(q1.y, q0.y, 0x0C0C0602u);
+    const uint32_t r0 = __builtin_amdgcn_udot4(rg0, wv, 0u, false), r1 = __builtin_amdgcn_udot4(rg1, wv, 0u, false);
+    const uint32_t g0 = __builtin_amdgcn_udot4(rg0, wv << 16, 0u, false), g1 = __builtin_amdgcn_udot4(rg1, wv << 16, 0u, false);
+    const uint32_t b0 = __builtin_amdgcn_udot4(bb0, wv, 0u, false), b1 = __builtin_amdgcn_udot4(bb1, wv, 0u, false);
+    const uint32_t sr = dot2u16(__builtin_amdgcn_perm(r1, r0, 0x05040100u), hw, 32768u);
+    const uint32_t sg = dot2u16(__builtin_amdgcn_perm(g1, g0, 0x05040100u), hw, 32768u);
+    const uint32_t sb = dot2u16(__builtin_amdgcn_perm(b1, b0, 0x05040100u), hw, 32768u);
+    return __builtin_amdgcn_perm(sg, sr, 0x0C0C0602u) | (sb & 0x00FF0000u);
+}
+
 // a run whose map passed runMap: one column weight for the four
 __device__ __forceinline__ void warpRun4(const uint8_t* base, uint32_t pitch, int H, const RunMap& m, uint32_t* out) {
     const uint32_t f = (uint32_t)(m.X0 & 31);
@@ -213,7 +234,7 @@ __device__ __forceinline__ uint32_t blendRgba8(const SynthParams& P, uint32_t wa
 // The warp pairs of the 4 region pixels at columns ex0 .. ex0 + 3 (frame columns x0 - AP + ex0 + i, clamped) of frame
 // row y (already clamped), from the interpolated field rows hrow. The shared-map form of warpRun4 when every lane of
 // the wavefront can take it (no divergent second path), else pixel by pixel; the same values either way.
-template <int S, bool UI>
+template <int S, bool UI, bool PixDot>
 __device__ __forceinline__ void warpRegionRun(const SynthParams& P, const float2* hrow, int ty0, float kA32, float kC32,
                                               int x0, int ex0, int y, uint32_t* oa, uint32_t* oc) {
     const int yc = min(y, S * P.fh - 1);
@@ -287,6 +308,28 @@ __device__ __forceinline__ void warpRegionRun(const SynthParams& P, const float2
         }
         return;
     }
+    // pixel by pixel: the byte-dot-product form when every tap column of the wavefront lies in the frame (PixDot: in
+    // the 16- and 32-row tiles; the 64-row tile keeps twice the warps in registers and would spill)
+    bool inside = PixDot;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+        inside = inside & ((uint32_t)(Xa[i] >> 5) <= (uint32_t)(P.W - 2)) & ((uint32_t)(Xc[i] >> 5) <= (uint32_t)(P.W - 2));
+    if (PixDot && __all(inside)) {
+        int ya[4], yc[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const float2 ri = mix2(ht[i], hb[i], fy);
+            ya[i] = (int)rintf(y32 - kA32 * ri.y);
+            yc[i] = (int)rintf(y32 + kC32 * ri.y);
+            oa[i] = warpPix(P.a.p, (uint32_t)P.a.pitch, P.H, Xa[i], ya[i]);
+            oc[i] = warpPix(P.c.p, (uint32_t)P.c.pitch, P.H, Xc[i], yc[i]);
+        }
+        if (UI) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) uiReject(P, Xa[i], ya[i], Xc[i], yc[i], oa[i], oc[i]);
+        }
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         Rgb a, c;
@@ -339,7 +382,7 @@ __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, 
         const int y = min(y0 + rr + 16 * k, P.H - 1);
         uint32_t* oa = wa + 4 * k;
         uint32_t* oc = wc + 4 * k;
-        warpRegionRun<S, UI>(P, hrow, ty0, kA32, kC32, x0, 4 * rc + AP, y, oa, oc);
+        warpRegionRun<S, UI, RT <= 32>(P, hrow, ty0, kA32, kC32, x0, 4 * rc + AP, y, oa, oc);
 #pragma unroll
         for (int i = 0; i < 4; i++) es[(rr + 16 * k + AP) * EWB + 4 * rc + i + AP] = (uint8_t)emaxPacked(oa[i], oc[i]);
     }
@@ -361,7 +404,7 @@ __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, 
         }
         const int y = med3i(y0 - AP + ey, 0, P.H - 1);
         uint32_t oa[4], oc[4];
-        warpRegionRun<S, UI>(P, hrow, ty0, kA32, kC32, x0, ex0, y, oa, oc);
+        warpRegionRun<S, UI, RT <= 32>(P, hrow, ty0, kA32, kC32, x0, ex0, y, oa, oc);
 #pragma unroll
         for (int i = 0; i < 4; i++) es[ey * EWB + ex0 + i] = (uint8_t)emaxPacked(oa[i], oc[i]);
     }
