@@ -168,7 +168,7 @@ __device__ __forceinline__ float yuvDiffMax(const SynthParams& P, float dY, floa
     return fmaxf(fabsf(R), fmaxf(fabsf(G), fabsf(B)));
 }
 
-// The 1/32 maps of a and c at pixel (x, y) (as warpPair: 32 p -+ (32 S k) r)
+// The 1/32 maps of a and c at pixel (x, y) (as the RGBA8 synth forms them: 32 p -+ (32 S k) r)
 struct Maps {
     int XA, YA, XC, YC;
 };

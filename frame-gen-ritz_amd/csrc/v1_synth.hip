@@ -22,8 +22,7 @@ struct Rgb {
     uint32_t r, g, b;
 };
 
-__device__ __forceinline__ Rgb warpRgba8(const uint8_t* base, uint32_t pitch, int W, int H, float X32, float Y32) {
-    const int X = (int)rintf(X32), Y = (int)rintf(Y32);
+__device__ __forceinline__ Rgb warpRgba8At(const uint8_t* base, uint32_t pitch, int W, int H, int X, int Y) {
     const int ix = X >> 5, iy = Y >> 5;
     const uint32_t fx = (uint32_t)(X & 31), fy = (uint32_t)(Y & 31);
     const uint32_t y0 = (uint32_t)med3i(iy, 0, H - 1), y1 = (uint32_t)med3i(iy + 1, 0, H - 1);
@@ -42,6 +41,10 @@ __device__ __forceinline__ Rgb warpRgba8(const uint8_t* base, uint32_t pitch, in
     o.b = (w00 * ((p00 >> 16) & 255u) + w01 * ((p01 >> 16) & 255u) + w10 * ((p10 >> 16) & 255u) +
            w11 * ((p11 >> 16) & 255u) + 512u) >> 10;
     return o;
+}
+
+__device__ __forceinline__ Rgb warpRgba8(const uint8_t* base, uint32_t pitch, int W, int H, float X32, float Y32) {
+    return warpRgba8At(base, pitch, W, H, (int)rintf(X32), (int)rintf(Y32));
 }
 
 __device__ __forceinline__ uint32_t packRgb(const Rgb& v) { return v.r | (v.g << 8) | (v.b << 16); }
@@ -165,19 +168,6 @@ __device__ __forceinline__ int emaxRgba8(uint32_t wa, uint32_t wc) {
     return max(d0, max(d1, d2));
 }
 
-// The warp pair at full-res pixel (x, y) from r, the interpolated field before the x S of halfVec (hv = S r): a at
-// p - 2t hv, c at p + 2(1-t) hv, each the float32 product then the difference (motion_v1.glsl:120-122), then x 32 for
-// the map. Scaling by a power of two commutes with rounding, so 32 (p - ka (S r)) = 32 p - (32 S ka) r exactly; kA32 and
-// kC32 are those folded constants. UI: with the CrosshairV2 tap rejection (motion_v1.glsl:124).
-template <bool UI = false>
-__device__ __forceinline__ void warpPair(const SynthParams& P, float kA32, float kC32, int x, int y, float2 r, Rgb& wa, Rgb& wc) {
-    const float x32 = (float)(32 * x), y32 = (float)(32 * y);
-    const float xa = x32 - kA32 * r.x, ya = y32 - kA32 * r.y, xc = x32 + kC32 * r.x, yc = y32 + kC32 * r.y;
-    wa = warpRgba8(P.a.p, (uint32_t)P.a.pitch, P.W, P.H, xa, ya);
-    wc = warpRgba8(P.c.p, (uint32_t)P.c.pitch, P.W, P.H, xc, yc);
-    if (UI) uiReject(P, (int)rintf(xa), (int)rintf(ya), (int)rintf(xc), (int)rintf(yc), wa, wc);
-}
-
 // the same pair from the plain per-pixel halfVec (debug pass)
 __device__ __forceinline__ void warpPairRgba8(const SynthParams& P, int x, int y, float2 hv, uint32_t& wa, uint32_t& wc) {
     const float ka = 2.0f * P.wC, kc = 2.0f * P.wA;
@@ -274,9 +264,13 @@ __device__ __forceinline__ void warpRegionRun(const SynthParams& P, const float2
         }
         return;
     }
-    // each pixel's maps as warpPair forms them; a run whose pixels share the map row and take consecutive tap columns
-    // (a field that varies by less than a pixel along the run) shares its texels and vertical blends (warpRow4)
-    int Xa[4], Xc[4], Ya = 0, Yc = 0;
+    // Each pixel's maps: from r, the interpolated field before the x S of halfVec (hv = S r), a at p - 2t hv and c at
+    // p + 2(1-t) hv, each the float32 product then the difference (motion_v1.glsl:120-122), then x 32 for the map;
+    // scaling by a power of two commutes with rounding, so 32 (p - ka (S r)) = 32 p - (32 S ka) r exactly (kA32 and
+    // kC32 are those folded constants). UI: with the CrosshairV2 tap rejection (motion_v1.glsl:124). A run whose pixels
+    // share the map row and take consecutive tap columns (a field that varies by less than a pixel along the run)
+    // shares its texels and vertical blends (warpRow4)
+    int Xa[4], Xc[4], Ya[4], Yc[4];
     bool row = xs[3] == xs[0] + 3;
     const float y32 = (float)(32 * y);
 #pragma unroll
@@ -285,12 +279,10 @@ __device__ __forceinline__ void warpRegionRun(const SynthParams& P, const float2
         const float x32 = (float)(32 * xs[i]);
         Xa[i] = (int)rintf(x32 - kA32 * ri.x);
         Xc[i] = (int)rintf(x32 + kC32 * ri.x);
-        const int ya = (int)rintf(y32 - kA32 * ri.y), yc = (int)rintf(y32 + kC32 * ri.y);
-        if (i == 0) {
-            Ya = ya;
-            Yc = yc;
-        }
-        row = row & (ya == Ya) & (yc == Yc) & ((Xa[i] >> 5) == (Xa[0] >> 5) + i) & ((Xc[i] >> 5) == (Xc[0] >> 5) + i);
+        Ya[i] = (int)rintf(y32 - kA32 * ri.y);
+        Yc[i] = (int)rintf(y32 + kC32 * ri.y);
+        row = row & (Ya[i] == Ya[0]) & (Yc[i] == Yc[0]) & ((Xa[i] >> 5) == (Xa[0] >> 5) + i) &
+              ((Xc[i] >> 5) == (Xc[0] >> 5) + i);
     }
     row = row & (Xa[0] >= 0) & ((Xa[0] >> 5) + 4 <= P.W - 1) & (Xc[0] >= 0) & ((Xc[0] >> 5) + 4 <= P.W - 1);
     if (__all(row)) {
@@ -300,42 +292,32 @@ __device__ __forceinline__ void warpRegionRun(const SynthParams& P, const float2
             fa[i] = (uint32_t)(Xa[i] & 31);
             fc[i] = (uint32_t)(Xc[i] & 31);
         }
-        warpRow4(P.a.p, (uint32_t)P.a.pitch, P.H, Xa[0] >> 5, Ya, fa, oa);
-        warpRow4(P.c.p, (uint32_t)P.c.pitch, P.H, Xc[0] >> 5, Yc, fc, oc);
-        if (UI) {
+        warpRow4(P.a.p, (uint32_t)P.a.pitch, P.H, Xa[0] >> 5, Ya[0], fa, oa);
+        warpRow4(P.c.p, (uint32_t)P.c.pitch, P.H, Xc[0] >> 5, Yc[0], fc, oc);
+    } else {
+        // pixel by pixel: the byte-dot-product form when every tap column of the wavefront lies in the frame (PixDot: in
+        // the 16- and 32-row tiles; the 64-row tile keeps twice the warps in registers and would spill)
+        bool inside = PixDot;
 #pragma unroll
-            for (int i = 0; i < 4; i++) uiReject(P, Xa[i], Ya, Xc[i], Yc, oa[i], oc[i]);
+        for (int i = 0; i < 4; i++)
+            inside = inside & ((uint32_t)(Xa[i] >> 5) <= (uint32_t)(P.W - 2)) & ((uint32_t)(Xc[i] >> 5) <= (uint32_t)(P.W - 2));
+        if (PixDot && __all(inside)) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                oa[i] = warpPix(P.a.p, (uint32_t)P.a.pitch, P.H, Xa[i], Ya[i]);
+                oc[i] = warpPix(P.c.p, (uint32_t)P.c.pitch, P.H, Xc[i], Yc[i]);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                oa[i] = packRgb(warpRgba8At(P.a.p, (uint32_t)P.a.pitch, P.W, P.H, Xa[i], Ya[i]));
+                oc[i] = packRgb(warpRgba8At(P.c.p, (uint32_t)P.c.pitch, P.W, P.H, Xc[i], Yc[i]));
+            }
         }
-        return;
     }
-    // pixel by pixel: the byte-dot-product form when every tap column of the wavefront lies in the frame (PixDot: in
-    // the 16- and 32-row tiles; the 64-row tile keeps twice the warps in registers and would spill)
-    bool inside = PixDot;
+    if (UI) {
 #pragma unroll
-    for (int i = 0; i < 4; i++)
-        inside = inside & ((uint32_t)(Xa[i] >> 5) <= (uint32_t)(P.W - 2)) & ((uint32_t)(Xc[i] >> 5) <= (uint32_t)(P.W - 2));
-    if (PixDot && __all(inside)) {
-        int ya[4], yc[4];
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            const float2 ri = mix2(ht[i], hb[i], fy);
-            ya[i] = (int)rintf(y32 - kA32 * ri.y);
-            yc[i] = (int)rintf(y32 + kC32 * ri.y);
-            oa[i] = warpPix(P.a.p, (uint32_t)P.a.pitch, P.H, Xa[i], ya[i]);
-            oc[i] = warpPix(P.c.p, (uint32_t)P.c.pitch, P.H, Xc[i], yc[i]);
-        }
-        if (UI) {
-#pragma unroll
-            for (int i = 0; i < 4; i++) uiReject(P, Xa[i], ya[i], Xc[i], yc[i], oa[i], oc[i]);
-        }
-        return;
-    }
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        Rgb a, c;
-        warpPair<UI>(P, kA32, kC32, xs[i], y, mix2(ht[i], hb[i], fy), a, c);
-        oa[i] = packRgb(a);
-        oc[i] = packRgb(c);
+        for (int i = 0; i < 4; i++) uiReject(P, Xa[i], Ya[i], Xc[i], Yc[i], oa[i], oc[i]);
     }
 }
 
