@@ -295,8 +295,9 @@ __device__ __forceinline__ void warpRegionRun(const SynthParams& P, const float2
         warpRow4(P.a.p, (uint32_t)P.a.pitch, P.H, Xa[0] >> 5, Ya[0], fa, oa);
         warpRow4(P.c.p, (uint32_t)P.c.pitch, P.H, Xc[0] >> 5, Yc[0], fc, oc);
     } else {
-        // pixel by pixel: the byte-dot-product form when every tap column of the wavefront lies in the frame (PixDot: in
-        // the 16- and 32-row tiles; the 64-row tile keeps twice the warps in registers and would spill)
+        // pixel by pixel: the byte-dot-product form when every tap column of the wavefront lies in the frame (PixDot:
+        // every run of the 16- and 32-row tiles and the interior runs of the 64-row tile, whose apron runs, with its 16
+        // interior warp pairs held in registers, would spill)
         bool inside = PixDot;
 #pragma unroll
         for (int i = 0; i < 4; i++)
@@ -364,7 +365,7 @@ __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, 
         const int y = min(y0 + rr + 16 * k, P.H - 1);
         uint32_t* oa = wa + 4 * k;
         uint32_t* oc = wc + 4 * k;
-        warpRegionRun<S, UI, RT <= 32>(P, hrow, ty0, kA32, kC32, x0, 4 * rc + AP, y, oa, oc);
+        warpRegionRun<S, UI, true>(P, hrow, ty0, kA32, kC32, x0, 4 * rc + AP, y, oa, oc);
 #pragma unroll
         for (int i = 0; i < 4; i++) es[(rr + 16 * k + AP) * EWB + 4 * rc + i + AP] = (uint8_t)emaxPacked(oa[i], oc[i]);
     }
