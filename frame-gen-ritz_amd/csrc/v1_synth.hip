@@ -224,7 +224,7 @@ __device__ __forceinline__ uint32_t blendRgba8(const SynthParams& P, uint32_t wa
 // The warp pairs of the 4 region pixels at columns ex0 .. ex0 + 3 (frame columns x0 - AP + ex0 + i, clamped) of frame
 // row y (already clamped), from the interpolated field rows hrow. The shared-map form of warpRun4 when every lane of
 // the wavefront can take it (no divergent second path), else pixel by pixel; the same values either way.
-template <int S, bool UI, bool PixDot>
+template <int S, bool UI>
 __device__ __forceinline__ void warpRegionRun(const SynthParams& P, const float2* hrow, int ty0, float kA32, float kC32,
                                               int x0, int ex0, int y, uint32_t* oa, uint32_t* oc) {
     const int yc = min(y, S * P.fh - 1);
@@ -295,14 +295,12 @@ __device__ __forceinline__ void warpRegionRun(const SynthParams& P, const float2
         warpRow4(P.a.p, (uint32_t)P.a.pitch, P.H, Xa[0] >> 5, Ya[0], fa, oa);
         warpRow4(P.c.p, (uint32_t)P.c.pitch, P.H, Xc[0] >> 5, Yc[0], fc, oc);
     } else {
-        // pixel by pixel: the byte-dot-product form when every tap column of the wavefront lies in the frame (PixDot:
-        // every run of the 16- and 32-row tiles and the interior runs of the 64-row tile, whose apron runs, with its 16
-        // interior warp pairs held in registers, would spill)
-        bool inside = PixDot;
+        // pixel by pixel: the byte-dot-product form when every tap column of the wavefront lies in the frame
+        bool inside = true;
 #pragma unroll
         for (int i = 0; i < 4; i++)
             inside = inside & ((uint32_t)(Xa[i] >> 5) <= (uint32_t)(P.W - 2)) & ((uint32_t)(Xc[i] >> 5) <= (uint32_t)(P.W - 2));
-        if (PixDot && __all(inside)) {
+        if (__all(inside)) {
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 oa[i] = warpPix(P.a.p, (uint32_t)P.a.pitch, P.H, Xa[i], Ya[i]);
@@ -357,21 +355,9 @@ __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, 
     if (tid == 0) cnt = 0u;
     const int ty0 = loadFieldTile<S, RT>(P, x0, y0, ft, hrow);
     const float kA32 = (float)(32 * S) * (2.0f * P.wC), kC32 = (float)(32 * S) * (2.0f * P.wA);
-    // interior pixels in runs of 4 (columns 4 rc .. 4 rc + 3, rows rr + 16 k): warps kept in registers (packed)
-    const int rc = tid & 15, rr = tid >> 4;
-    uint32_t wa[4 * RK], wc[4 * RK];
-#pragma unroll
-    for (int k = 0; k < RK; k++) {
-        const int y = min(y0 + rr + 16 * k, P.H - 1);
-        uint32_t* oa = wa + 4 * k;
-        uint32_t* oc = wc + 4 * k;
-        warpRegionRun<S, UI, true>(P, hrow, ty0, kA32, kC32, x0, 4 * rc + AP, y, oa, oc);
-#pragma unroll
-        for (int i = 0; i < 4; i++) es[(rr + 16 * k + AP) * EWB + 4 * rc + i + AP] = (uint8_t)emaxPacked(oa[i], oc[i]);
-    }
-    // apron in runs of 4 as well: rows 0..2 and EH-3..EH-1 as 18 runs a row (the last at column EW - 4, overlapping
-    // its neighbour), then columns 0..3 and EW-4..EW-1 of each row between (overlapping the interior's first and last
-    // column). A pixel taken twice gets the same value twice.
+    // the apron first (no interior warps held in registers yet), in runs of 4 as well: rows 0..2 and EH-3..EH-1 as 18
+    // runs a row (the last at column EW - 4, overlapping its neighbour), then columns 0..3 and EW-4..EW-1 of each row
+    // between (overlapping the interior's first and last column). A pixel taken twice gets the same value twice.
     constexpr int kRowRuns = (EW + 3) / 4, kApRuns = 2 * AP * kRowRuns + 2 * TH;
     static_assert(kRowRuns == 18 && 2 * AP * kRowRuns < 256, "one run a thread for the apron rows");
     for (int j = tid; j < kApRuns; j += 256) {
@@ -387,9 +373,21 @@ __device__ __forceinline__ void synthRgba8Tile(const SynthParams& P, int tileX, 
         }
         const int y = med3i(y0 - AP + ey, 0, P.H - 1);
         uint32_t oa[4], oc[4];
-        warpRegionRun<S, UI, RT <= 32>(P, hrow, ty0, kA32, kC32, x0, ex0, y, oa, oc);
+        warpRegionRun<S, UI>(P, hrow, ty0, kA32, kC32, x0, ex0, y, oa, oc);
 #pragma unroll
         for (int i = 0; i < 4; i++) es[ey * EWB + ex0 + i] = (uint8_t)emaxPacked(oa[i], oc[i]);
+    }
+    // interior pixels in runs of 4 (columns 4 rc .. 4 rc + 3, rows rr + 16 k): warps kept in registers (packed)
+    const int rc = tid & 15, rr = tid >> 4;
+    uint32_t wa[4 * RK], wc[4 * RK];
+#pragma unroll
+    for (int k = 0; k < RK; k++) {
+        const int y = min(y0 + rr + 16 * k, P.H - 1);
+        uint32_t* oa = wa + 4 * k;
+        uint32_t* oc = wc + 4 * k;
+        warpRegionRun<S, UI>(P, hrow, ty0, kA32, kC32, x0, 4 * rc + AP, y, oa, oc);
+#pragma unroll
+        for (int i = 0; i < 4; i++) es[(rr + 16 * k + AP) * EWB + 4 * rc + i + AP] = (uint8_t)emaxPacked(oa[i], oc[i]);
     }
     __syncthreads();
     boxRows7B<EH>(es, hs, tid);
