@@ -1,7 +1,8 @@
-"""The integer regroupings the RGBA8 synth kernel uses, checked exhaustively (or over every weight) on the CPU against
-the formulas they replace: the cv2 fixed-point remap tap of interp.py:214, the trusted blends of resolve.comp:73-82 at
-t = 1/4, 3/4, and the max-channel mismatch of error.comp:9-21. The GPU tests compare the kernels' outputs with the
-oracle; these pin the arithmetic identities over all inputs, including values the test images may never produce."""
+"""The integer regroupings and the median network that the kernels use, checked exhaustively (or over every weight) on
+the CPU against the formulas they replace: the cv2 fixed-point remap tap of interp.py:214, the trusted blends of
+resolve.comp:73-82 at t = 1/4, 3/4, the max-channel mismatch of error.comp:9-21, and the median of median3.comp:8-28.
+The GPU tests compare the kernels' outputs with the oracle; these pin the identities over all inputs, including values
+the test images may never produce."""
 import numpy as np
 
 
@@ -42,3 +43,28 @@ def test_run_warp_regrouping_is_the_cv2_tap():
     s = 64 * gx * v0 + 64 * fx * v1 + 32768
     assert s.max() < 1 << 24
     assert np.array_equal((s >> 16) & 255, cv2)
+
+
+# med9 in csrc/v1_estimate.hip: the exchanges (min to the first slot, max to the second) in order
+MED9_NET = [(1, 2), (4, 5), (7, 8), (0, 1), (3, 4), (6, 7), (1, 2), (4, 5), (7, 8), (0, 3),
+            (5, 8), (4, 7), (3, 6), (1, 4), (2, 5), (4, 7), (4, 2), (6, 4), (4, 2)]
+
+
+def med9_network(v):
+    v = np.array(v, copy=True)
+    for a, b in MED9_NET:
+        lo, hi = np.minimum(v[..., a], v[..., b]), np.maximum(v[..., a], v[..., b])
+        v[..., a], v[..., b] = lo, hi
+    return v[..., 4]
+
+
+def test_median_network_is_the_fifth_smallest():
+    """median3.comp's partial selection sort returns the 5th smallest value; the network must too. All 2^9 inputs of
+    0s and 1s (the 0-1 principle: a comparator network right on those is right on every input), then random values
+    with many ties and the float values of quarter-pel vectors."""
+    bits = ((np.arange(512)[:, None] >> np.arange(9)) & 1).astype(np.float32)
+    assert np.array_equal(med9_network(bits), np.sort(bits, axis=1)[:, 4])
+    rng = np.random.default_rng(3)
+    for v in (rng.integers(0, 4, size=(100000, 9)).astype(np.float32),
+              (rng.integers(-64, 65, size=(100000, 9)) / 4.0).astype(np.float32)):
+        assert np.array_equal(med9_network(v), np.sort(v, axis=1)[:, 4])
