@@ -662,6 +662,90 @@ __device__ __forceinline__ void searchBody(const SearchParams& P, int bx0, int) 
     }
 }
 
+// The seeds of a wave's two blocks on a dyadic level (values k 2^-fb: every SAD below 2^24 units is exact in float32,
+// so it does not depend on the order of its sum): lane 32 h + 4 kk + rp takes candidate slot kk (0..7) of block h and
+// its rows 2 rp, 2 rp + 1, two 8-texel rows of each frame (four 16-byte loads when the rows lie inside the plane),
+// then the sums of the slot's four lanes, the cost SAD/64 (+ lambda for all but slot 0; slots past nc never chosen)
+// and the first minimum over the slots in order, as seedPick's strict < chooses. A slot whose vector repeats an earlier
+// one's costs that one's SAD plus lambda >= 0 and is never chosen, so repeats need no test. In two steps, so that a
+// caller can issue other loads (the windows at slot 0's vector) between them: the slot's vector (seedVectors'
+// candidate kk), then the sums and the choice, the chosen vector of block lane >> 5 at every lane.
+struct alignas(4) F4 {
+    float x, y, z, w;
+};
+
+__device__ __forceinline__ void seedSlotVector(const SearchParams& P, const int* bxs, const int* bys, const bool* acts,
+                                               int blk0, int lane, int& cx, int& cy) {
+    const int h = lane >> 5, kk = (lane >> 2) & 7;
+    const int bx = h ? bxs[1] : bxs[0], by = h ? bys[1] : bys[0];
+    const int nc = P.parent ? (P.extraSeed ? 7 : 6) : 2;
+    const int k = P.parent ? kk : (kk == 0 ? 4 : 6);
+    cx = 0;
+    cy = 0;
+    if (kk >= nc) return;
+    if (k < 4) {
+        const int px = min(bx / 2, P.pw - 1), py = min(by / 2, P.ph - 1);
+        const int ssy = (by & 1) == 0 ? -1 : 1, ssx = (bx & 1) == 0 ? -1 : 1;
+        const int ox = (k == 2 || k == 3) ? ssx : 0, oy = (k == 1 || k == 3) ? ssy : 0;
+        const float2 v = P.parent[clampi(py + oy, 0, P.ph - 1) * P.pw + clampi(px + ox, 0, P.pw - 1)];
+        cx = 2 * (int)rintf(v.x);
+        cy = 2 * (int)rintf(v.y);
+    } else if (k == 5) {
+        cx = (int)rintf(__uint_as_float(P.med[0]));
+        cy = (int)rintf(__uint_as_float(P.med[1]));
+    } else if (k == 6) {
+        const bool active = h ? acts[1] : acts[0];
+        const float2 v = P.extraSeed[active ? blk0 + h : 0];
+        cx = (int)rintf(v.x);
+        cy = (int)rintf(v.y);
+    }
+}
+
+__device__ __forceinline__ void seedPickRows(const SearchParams& P, const int* bxs, const int* bys, int lane, int cx,
+                                             int cy, int& sx, int& sy) {
+    const int h = lane >> 5, kk = (lane >> 2) & 7, rp = lane & 3;
+    const int bx = h ? bxs[1] : bxs[0], by = h ? bys[1] : bys[0];
+    const int nc = P.parent ? (P.extraSeed ? 7 : 6) : 2;
+    const int xa = bx * kBlock - cx, xc = bx * kBlock + cx;
+    float sum = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const int y = by * kBlock + 2 * rp + j, ya = y - cy, yc = y + cy;
+        if (xa >= 0 && xa + 7 <= P.w - 1 && xc >= 0 && xc + 7 <= P.w - 1 && ya >= 0 && ya <= P.h - 1 && yc >= 0 &&
+            yc <= P.h - 1) {
+            const F4* ra = (const F4*)(P.la + idx2(xa, ya, P.w));
+            const F4* rc = (const F4*)(P.lc + idx2(xc, yc, P.w));
+            const F4 a0 = ra[0], a1 = ra[1], c0 = rc[0], c1 = rc[1];
+            sum = sum + fabsf(a0.x - c0.x) + fabsf(a0.y - c0.y) + fabsf(a0.z - c0.z) + fabsf(a0.w - c0.w) +
+                  fabsf(a1.x - c1.x) + fabsf(a1.y - c1.y) + fabsf(a1.z - c1.z) + fabsf(a1.w - c1.w);
+        } else {   // a row that leaves the plane: the clamped texels
+#pragma unroll
+            for (int x = 0; x < 8; x++)
+                sum = sum + fabsf(lumaAt(P.la, xa + x, ya, P.w, P.h) - lumaAt(P.lc, xc + x, yc, P.w, P.h));
+        }
+    }
+    sum = sum + __shfl_xor(sum, 1, 64);
+    sum = sum + __shfl_xor(sum, 2, 64);
+    float c = sum * (1.0f / 64.0f);
+    if (kk > 0) c = c + P.lam;
+    if (kk >= nc) c = FLT_MAX;
+    // the first minimum over the block's 8 slots (lanes 4 kk of its half)
+    int bk = kk;
+    sx = cx;
+    sy = cy;
+#pragma unroll
+    for (int o = 4; o <= 16; o <<= 1) {
+        const float oc = __shfl_xor(c, o, 64);
+        const int ok = __shfl_xor(bk, o, 64), ox = __shfl_xor(sx, o, 64), oy = __shfl_xor(sy, o, 64);
+        if (oc < c || (oc == c && ok < bk)) {
+            c = oc;
+            bk = ok;
+            sx = ox;
+            sy = oy;
+        }
+    }
+}
+
 // R <= 2 ((2R + 1)^2 <= 25 candidates): two blocks a wavefront, eight a workgroup. The seeds and windows of a wave's two
 // blocks are formed one after the other with all 64 lanes, then lanes 0-31 take the first block's candidates and lanes
 // 32-63 the second's (each lane one candidate, its SAD in the order above), and the argmin runs within each half:
@@ -684,7 +768,27 @@ __device__ __forceinline__ void search2BodyR(const SearchParams& P, int bx0, flo
         bys[h] = acts[h] ? blk / P.bw : 0;
     }
     SearchWin<R> win[2];
-    if (P.parent || P.extraSeed) {   // both blocks' seeds side by side, each step for the two together
+    const bool ints = P.fb >= 0;   // (uniform) packed 16-bit SADs
+    if ((P.parent || P.extraSeed) && ints) {   // the seeds by slot and row pairs (exact sums)
+        int cx, cy, sx, sy, c0x[2], c0y[2];
+        seedSlotVector(P, bxs, bys, acts, bx0 * kSearch2Blocks + 2 * wave, lane, cx, cy);
+        // the windows at slot 0's vector (the own parent's, or zero on the coarsest level), usually the seed, loaded
+        // with the slots' rows
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            c0x[h] = __builtin_amdgcn_readlane(cx, 32 * h);
+            c0y[h] = __builtin_amdgcn_readlane(cy, 32 * h);
+            searchWinLoad<R>(P, bxs[h], bys[h], c0x[h], c0y[h], lane, win[h]);
+        }
+        seedPickRows(P, bxs, bys, lane, cx, cy, sx, sy);
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            sxs[h] = __builtin_amdgcn_readlane(sx, 32 * h);
+            sys[h] = __builtin_amdgcn_readlane(sy, 32 * h);
+            if (sxs[h] != c0x[h] || sys[h] != c0y[h])   // (wave-uniform) another seed: its windows
+                searchWinLoad<R>(P, bxs[h], bys[h], sxs[h], sys[h], lane, win[h]);
+        }
+    } else if (P.parent || P.extraSeed) {   // both blocks' seeds side by side, each step for the two together
         SeedSet S[2];
 #pragma unroll
         for (int h = 0; h < 2; h++) seedVectors(P, bx0 * kSearch2Blocks + 2 * wave + h, acts[h], bxs[h], bys[h], S[h]);
@@ -704,7 +808,6 @@ __device__ __forceinline__ void search2BodyR(const SearchParams& P, int bx0, flo
 #pragma unroll
         for (int h = 0; h < 2; h++) searchWinLoad<R>(P, bxs[h], bys[h], 0, 0, lane, win[h]);
     }
-    const bool ints = P.fb >= 0;   // (uniform) packed 16-bit SADs
 #pragma unroll
     for (int h = 0; h < 2; h++) searchWinStore<R>(P, win[h], ta[2 * wave + h], tc[2 * wave + h], lane, ints);
     __syncthreads();
