@@ -604,8 +604,8 @@ void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& pr
         // level i of a dyadic level 0 (fb0 fractional bits) has fb0 + 2i; packed 16-bit SADs while 255.75 2^fb < 2^16
         const int fb0 = pixExactBits();
         sp.fb = fb0 >= 0 && fb0 + 2 * i <= 8 ? fb0 + 2 * i : -1;
-        // R <= 2 in a batch: two blocks a wavefront (twice the busy lanes, half the workgroups)
-        const bool wide = batched && sp.R <= 2 && !(estForm & FGR_FORM_SEARCH_NARROW);
+        // R <= 2: two blocks a wavefront (twice the busy lanes, half the workgroups); a batch may ask for one
+        const bool wide = sp.R <= 2 && !(batched && (estForm & FGR_FORM_SEARCH_NARROW));
         sp.gx = cdiv(l.bw * l.bh, wide ? kSearch2Blocks : 4);
         PlanOp op;
         op.pass = wide ? kPassSearch2x8 : (sp.R <= 2 ? kPassSearch2 : (sp.R <= 6 ? kPassSearch6 : kPassSearch12));

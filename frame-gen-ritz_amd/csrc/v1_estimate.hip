@@ -2068,7 +2068,7 @@ FGR_PASS(pyramid_rgba8, PyrParams, pyramidRgba8Body, 256)
 FGR_PASS(down, DownParams, downBody, 256)
 FGR_PASS(medhist, MedParams, medhistBody, 1024)
 FGR_PASS(search2, SearchParams, searchBody<2>, 256)
-FGR_PASS_BATCH(search2x8, SearchParams, search2Body, 256)
+FGR_PASS(search2x8, SearchParams, search2Body, 256)
 FGR_PASS(search6, SearchParams, searchBody<6>, 256)
 FGR_PASS(search12, SearchParams, searchBody<12>, 256)
 FGR_PASS(median3, Median3Params, median3Body, 256)
@@ -2093,6 +2093,7 @@ hipError_t launchSingle(int pass, const void* params, int gx, int gy, hipStream_
     case kPassDown: hipLaunchKernelGGL(down_kernel, g, dim3(256), 0, s, *(const DownParams*)params); break;
     case kPassMedhist: hipLaunchKernelGGL(medhist_kernel, g, dim3(1024), 0, s, *(const MedParams*)params); break;
     case kPassSearch2: hipLaunchKernelGGL(search2_kernel, g, dim3(256), 0, s, *(const SearchParams*)params); break;
+    case kPassSearch2x8: hipLaunchKernelGGL(search2x8_kernel, g, dim3(256), 0, s, *(const SearchParams*)params); break;
     case kPassSearch6: hipLaunchKernelGGL(search6_kernel, g, dim3(256), 0, s, *(const SearchParams*)params); break;
     case kPassSearch12: hipLaunchKernelGGL(search12_kernel, g, dim3(256), 0, s, *(const SearchParams*)params); break;
     case kPassMedian3: hipLaunchKernelGGL(median3_kernel, g, dim3(256), 0, s, *(const Median3Params*)params); break;
