@@ -265,7 +265,8 @@ int32_t fgr_batch_get_lanes(const FgrBatch* b);
 /* Estimate forms of a batch (bits; bit-identical results; fgr_batch_autotune times each, ns[13 .. 16] = pixel field
  * gathering / LDS windows, then search two blocks a wavefront / one block a wavefront, with the best tiles):
  *   FGR_FORM_PIXFIELD_WINDOWS: the pixel field from LDS windows of the luma (lumas with exact integer sampling) instead
- *     of gathering every tap from the planes; FGR_FORM_SEARCH_NARROW: the radius <= 2 searches one block a wavefront. */
+ *     of gathering every tap from the planes; FGR_FORM_SEARCH_NARROW: the radius <= 2 searches one block a wavefront.
+ * (The per-pair records always take the LDS windows and two blocks a wavefront.) */
 enum { FGR_FORM_PIXFIELD_WINDOWS = 1, FGR_FORM_SEARCH_NARROW = 2 };
 int fgr_batch_set_form(FgrBatch* b, int32_t form);
 int32_t fgr_batch_get_form(const FgrBatch* b);
