@@ -222,8 +222,10 @@ __device__ __forceinline__ uint32_t blendRgba8(const SynthParams& P, uint32_t wa
 }
 
 // The warp pairs of the 4 region pixels at columns ex0 .. ex0 + 3 (frame columns x0 - AP + ex0 + i, clamped) of frame
-// row y (already clamped), from the interpolated field rows hrow. The shared-map form of warpRun4 when every lane of
-// the wavefront can take it (no divergent second path), else pixel by pixel; the same values either way.
+// row y (already clamped), from the interpolated field rows hrow. Each form is taken when every lane of the wavefront
+// can take it (no divergent second path): one map for the run (warpRun4), else one map row and consecutive tap columns
+// (warpRow4), else pixel by pixel (warpPix with every tap column inside the frame, else warpRgba8At with its edge
+// clamps); every form gives the same values.
 template <int S, bool UI>
 __device__ __forceinline__ void warpRegionRun(const SynthParams& P, const float2* hrow, int ty0, float kA32, float kC32,
                                               int x0, int ex0, int y, uint32_t* oa, uint32_t* oc) {
@@ -320,8 +322,9 @@ __device__ __forceinline__ void warpRegionRun(const SynthParams& P, const float2
     }
 }
 
-// One 64 x RT tile (RT = 16, 32 or 64 rows; 4, 8 or 16 pixels a thread) of the synthesised frame (passes 9-11 fused): field tile (+ lookup at t), mismatch of the tile and
-// its apron, 7x7 box, trust, untrusted count, blend. Scene cut: a copy of the nearer frame.
+// One 64 x RT tile (RT = 16, 32 or 64 rows; 4, 8 or 16 pixels a thread) of the synthesised frame (passes 9-11 fused):
+// field tile (+ lookup at t), mismatch of the tile and its apron, 7x7 box, trust, untrusted count, blend. Scene cut: a
+// copy of the nearer frame.
 //
 // halfVec() motion_v1.glsl:73-84 is split in its two stages: the horizontal mixes of each field row at every full-res
 // column of the tile (row[j][x] = mix(F[j][i0(x)], F[j][i1(x)], fx(x))), then per pixel the vertical mix of two of
