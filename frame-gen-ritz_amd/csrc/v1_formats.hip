@@ -380,7 +380,7 @@ __device__ __forceinline__ void copyNear(const SynthParams& P, int x, int y) {
 // frame; the apron's 2 x 3 x 70 + 6 RT pixels follow, spread over the workgroup.
 template <int S, int F, bool UI = false, int RT = TH>
 __device__ __forceinline__ void synthFmtTile(const SynthParams& P, int tileX, int tileY, float2* ft, float2* hrow,
-                                             int* es, uint16_t* hs, uint32_t* cnt) {
+                                             uint8_t* es, uint16_t* hs, uint32_t* cnt) {
     const int tid = threadIdx.x;
     const int x0 = tileX * TW, y0 = tileY * RT;
     const int cx = tid & (TW - 1), ry = tid >> 6;
@@ -401,8 +401,8 @@ __device__ __forceinline__ void synthFmtTile(const SynthParams& P, int tileX, in
         const int ly = ry + 4 * k;
         const int x = min(x0 + cx, P.W - 1), y = min(y0 + ly, P.H - 1);
         const float2 r = fieldAtRegion<S>(P, hrow, ty0, cx + AP, y);
-        es[(ly + AP) * EW + cx + AP] =
-            mismatchFmt<F, UI, true>(P, mapsAt(kA32, kC32, (float)(32 * x), (float)(32 * y), r), mix[k]);
+        es[(ly + AP) * EWB + cx + AP] =
+            (uint8_t)mismatchFmt<F, UI, true>(P, mapsAt(kA32, kC32, (float)(32 * x), (float)(32 * y), r), mix[k]);
     }
     // the apron: AP rows above and below the tile (EW wide), then AP columns either side of its RT rows
     constexpr int kRowsAp = 2 * AP * EW, kApron = kRowsAp + 2 * AP * RT;
@@ -419,10 +419,10 @@ __device__ __forceinline__ void synthFmtTile(const SynthParams& P, int tileX, in
         }
         const int x = med3i(x0 - AP + ex, 0, P.W - 1), y = med3i(y0 - AP + ey, 0, P.H - 1);
         const float2 r = fieldAtRegion<S>(P, hrow, ty0, ex, y);
-        es[ey * EW + ex] = mismatchFmt<F, UI>(P, mapsAt(kA32, kC32, (float)(32 * x), (float)(32 * y), r));
+        es[ey * EWB + ex] = (uint8_t)mismatchFmt<F, UI>(P, mapsAt(kA32, kC32, (float)(32 * x), (float)(32 * y), r));
     }
     __syncthreads();
-    boxRows7<RT + 2 * AP>(es, hs, tid);
+    boxRows7B<RT + 2 * AP>(es, hs, tid);
     __syncthreads();
     uint32_t mine = 0u;
 #pragma unroll
@@ -451,7 +451,7 @@ __device__ __forceinline__ void synthFmtBody(const SynthParams& P, int bx, int b
     constexpr int FTW = SynthTile<S, RT>::FTW, FTH = SynthTile<S, RT>::FTH, EHR = RT + 2 * AP;
     __shared__ float2 ft[FTW * FTH];
     __shared__ float2 hrow[FTH * EW];
-    __shared__ __align__(16) int es[EHR * EW];
+    __shared__ __align__(16) uint8_t es[EHR * EWB];   // the mismatch (0 .. 255), one byte a pixel
     __shared__ __align__(16) uint16_t hs[EHR * TW];
     __shared__ uint32_t cnt;
     if (P.fmt == kRgb10) synthFmtTile<S, kRgb10, UI, RT>(P, bx, by, ft, hrow, es, hs, &cnt);
@@ -463,8 +463,8 @@ __device__ __forceinline__ void synthFmtBody(const SynthParams& P, int bx, int b
 FGR_PASS(luma_fmt, LumaParams, lumaFmtBody, 256)
 FGR_PASS(pyramid_fmt, PyrParams, pyramidFmtBody, 256)
 FGR_PASS(synth_fmt_s1, SynthParams, synthFmtBody<1>, 256)
-FGR_PASS(synth_fmt_s2, SynthParams, synthFmtBody<2>, 256)
-FGR_PASS(synth_fmt_s4, SynthParams, synthFmtBody<4>, 256)
+FGR_PASS_OCC(synth_fmt_s2, SynthParams, synthFmtBody<2>, 256, 5)   // 5 workgroups a CU (24.6 KB of LDS each)
+FGR_PASS_OCC(synth_fmt_s4, SynthParams, synthFmtBody<4>, 256, 5)
 #define FGR_SYNTH_FMT_ROWS_BODY(S, RT) synthFmtBody<S, false, RT>
 FGR_PASS(synth_fmt_s2_t16, SynthParams, FGR_SYNTH_FMT_ROWS_BODY(2, 16), 256)
 FGR_PASS(synth_fmt_s2_t64, SynthParams, FGR_SYNTH_FMT_ROWS_BODY(2, 64), 256)
