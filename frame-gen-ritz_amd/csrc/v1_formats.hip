@@ -50,6 +50,26 @@ __device__ __forceinline__ void pyramidFmtBody(const PyrParams& P, int bx, int b
         for (int i = threadIdx.x; i < kUWords; i += 256) P.u[i] = 0u;
     const int cx0 = bx * 32 + 2 * (threadIdx.x & 15), cy0 = by * 32 + 2 * (threadIdx.x >> 4);
     float a[2][2], c[2][2];
+    if (P.S == 2 && 2 * cx0 + 3 <= P.W - 1 && 2 * cy0 + 3 <= P.H - 1) {
+        // flowScale 2 inside the frame: the thread's 4 x 4 pixels as four row loads a frame, the same integer sums
+        int la[4][4], lc[4][4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            luma16Row4(P, P.a, 2 * cx0, 2 * cy0 + j, la[j]);
+            luma16Row4(P, P.c, 2 * cx0, 2 * cy0 + j, lc[j]);
+        }
+#pragma unroll
+        for (int r = 0; r < 2; r++)
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const int sa = la[2 * r][2 * i] + la[2 * r][2 * i + 1] + la[2 * r + 1][2 * i] + la[2 * r + 1][2 * i + 1];
+                const int sc = lc[2 * r][2 * i] + lc[2 * r][2 * i + 1] + lc[2 * r + 1][2 * i] + lc[2 * r + 1][2 * i + 1];
+                a[r][i] = (float)sa / (float)(16 * 4);
+                c[r][i] = (float)sc / (float)(16 * 4);
+                pyrStore(P.la[0], P.w[0], P.h[0], cx0 + i, cy0 + r, a[r][i]);
+                pyrStore(P.lc[0], P.w[0], P.h[0], cx0 + i, cy0 + r, c[r][i]);
+            }
+    } else {
 #pragma unroll
     for (int r = 0; r < 2; r++)
 #pragma unroll
@@ -71,7 +91,8 @@ __device__ __forceinline__ void pyramidFmtBody(const PyrParams& P, int bx, int b
             pyrStore(P.la[0], P.w[0], P.h[0], cx0 + i, cy0 + r, a[r][i]);
             pyrStore(P.lc[0], P.w[0], P.h[0], cx0 + i, cy0 + r, c[r][i]);
         }
-    pyramidLevels(P, bx, by, a, c);
+    }
+    pyramidLevels(P, bx, by, a, c);   // (its barriers reached by every thread, after either form)
 }
 
 // ---------------------------------------------------------------------------------------------------------- taps
