@@ -651,7 +651,7 @@ void FgrInterpolator::planEstimate(std::vector<PlanOp>& plan, const FgrFrame& pr
         q.lam = st.searchPenalty;
         q.u = u;
         q.fb = pixExactBits();
-        q.gx = cdiv(l0.bw * l0.bh, kSubpixBlocks);
+        q.gx = cdiv(l0.bw * l0.bh, kSubpixBlocks * (q.fb >= 0 ? kSubpixIntWaves : 1));
         PlanOp op;
         op.pass = q.fb >= 0 ? kPassSubpixInt : kPassSubpix;
         op.stage = 400;

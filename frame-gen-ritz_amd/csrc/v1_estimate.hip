@@ -1011,16 +1011,29 @@ __device__ __forceinline__ void subpixBody(const SubpixParams& P, int g, int) {
 // Windows: the offsets stay within 0.75 px of the integer vector, so a pixel x + O of the float path's 13 x 13 tile taps
 // columns floor(x + O -/+ o) .. + 1 within 1 .. 10 (rows alike): each frame's 10 x 10 texels of a block (the tile's
 // rows and columns 1 .. 10) as u16 rows of 10 stored twice, E16[i] = t[i] and O16[i] = t[i + 1], so that the texel pair
-// (t[i], t[i + 1]) is the aligned word i / 2 of E16 (i even) or (i - 1) / 2 of O16 (i odd). The windows' 5.6 KB of LDS
-// a workgroup leave room for 6 wavefronts a SIMD.
-__device__ __forceinline__ void subpixIntBody(const SubpixParams& P, int g, int) {
-    constexpr int T = 10, TS = 10, O = 2, NB = kSubpixBlocks, WB = T * TS;   // WB u16 a block window
+// (t[i], t[i + 1]) is the aligned word i / 2 of E16 (i even) or (i - 1) / 2 of O16 (i odd).
+//
+// A workgroup is kSubpixIntWaves wavefronts, each with its own 7 blocks (group gw = g kSubpixIntWaves + wave) and its
+// own 6.2 KB of LDS: single-wavefront workgroups would be held to the CU's workgroup slots rather than its registers and
+// LDS. Every wavefront runs the same barriers.
+__device__ __forceinline__ void subpixIntBody(const SubpixParams& P, int gwg, int) {
+    constexpr int T = 10, TS = 10, O = 2, NB = kSubpixBlocks, WB = T * TS, NW = kSubpixIntWaves;   // WB u16 a block window
     static_assert(TS % 2 == 0 && TS >= T, "rows of whole texel-pair words");
-    __shared__ __align__(16) uint16_t ea[NB * WB], oa[NB * WB], ec[NB * WB], oc[NB * WB];
-    __shared__ float cst[NB][9], sads[NB][9];
-    __shared__ int sv[NB][2], sb[NB][2];
-    __shared__ uint32_t wsad[NB];
-    const int lane = threadIdx.x;
+    __shared__ __align__(16) uint16_t eaW[NW][NB * WB], oaW[NW][NB * WB], ecW[NW][NB * WB], ocW[NW][NB * WB];
+    __shared__ float cstW[NW][NB][9], sadsW[NW][NB][9];
+    __shared__ int svW[NW][NB][2], sbW[NW][NB][2];
+    __shared__ uint32_t wsadW[NW][NB];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int g = gwg * NW + wave;
+    uint16_t* ea = eaW[wave];
+    uint16_t* oa = oaW[wave];
+    uint16_t* ec = ecW[wave];
+    uint16_t* oc = ocW[wave];
+    float(*cst)[9] = cstW[wave];
+    float(*sads)[9] = sadsW[wave];
+    int(*sv)[2] = svW[wave];
+    int(*sb)[2] = sbW[wave];
+    uint32_t* wsad = wsadW[wave];
     const int nb = P.bw * P.bh;
     if (lane < NB) {
         const int blk = g * NB + lane;
@@ -2073,7 +2086,7 @@ FGR_PASS(search6, SearchParams, searchBody<6>, 256)
 FGR_PASS(search12, SearchParams, searchBody<12>, 256)
 FGR_PASS(median3, Median3Params, median3Body, 256)
 FGR_PASS(subpix, SubpixParams, subpixBody, 64)
-FGR_PASS(subpix_int, SubpixParams, subpixIntBody, 64)
+FGR_PASS(subpix_int, SubpixParams, subpixIntBody, 64 * kSubpixIntWaves)
 #define FGR_PIX_BODY(NT, WIN) pixfieldBody<NT, WIN>
 FGR_PASS_SINGLE(pixfield, PixParams, FGR_PIX_BODY(kPfThreads, false), kPfThreads)
 FGR_PASS_BATCH_OCC(pixfield, PixParams, FGR_PIX_BODY(kPfBatchThreads, false), kPfBatchThreads, 6)
@@ -2098,7 +2111,7 @@ hipError_t launchSingle(int pass, const void* params, int gx, int gy, hipStream_
     case kPassSearch12: hipLaunchKernelGGL(search12_kernel, g, dim3(256), 0, s, *(const SearchParams*)params); break;
     case kPassMedian3: hipLaunchKernelGGL(median3_kernel, g, dim3(256), 0, s, *(const Median3Params*)params); break;
     case kPassSubpix: hipLaunchKernelGGL(subpix_kernel, g, dim3(64), 0, s, *(const SubpixParams*)params); break;
-    case kPassSubpixInt: hipLaunchKernelGGL(subpix_int_kernel, g, dim3(64), 0, s, *(const SubpixParams*)params); break;
+    case kPassSubpixInt: hipLaunchKernelGGL(subpix_int_kernel, g, dim3(64 * kSubpixIntWaves), 0, s, *(const SubpixParams*)params); break;
     case kPassPixfield: hipLaunchKernelGGL(pixfield_kernel, g, dim3(kPfThreads), 0, s, *(const PixParams*)params); break;
     case kPassPixfieldWin: hipLaunchKernelGGL(pixfield_win_kernel, g, dim3(kPfThreads), 0, s, *(const PixParams*)params); break;
     default: return launchSynthSingle(pass, params, gx, gy, s);
@@ -2121,7 +2134,7 @@ hipError_t launchBatch(int pass, const void* params, const uint32_t* work, int n
     case kPassSearch12: hipLaunchKernelGGL(search12_batch_kernel, g, dim3(256), 0, s, (const SearchParams*)params, work); break;
     case kPassMedian3: hipLaunchKernelGGL(median3_batch_kernel, g, dim3(256), 0, s, (const Median3Params*)params, work); break;
     case kPassSubpix: hipLaunchKernelGGL(subpix_batch_kernel, g, dim3(64), 0, s, (const SubpixParams*)params, work); break;
-    case kPassSubpixInt: hipLaunchKernelGGL(subpix_int_batch_kernel, g, dim3(64), 0, s, (const SubpixParams*)params, work); break;
+    case kPassSubpixInt: hipLaunchKernelGGL(subpix_int_batch_kernel, g, dim3(64 * kSubpixIntWaves), 0, s, (const SubpixParams*)params, work); break;
     case kPassPixfield: hipLaunchKernelGGL(pixfield_batch_kernel, g, dim3(kPfBatchThreads), 0, s, (const PixParams*)params, work); break;
     case kPassPixfieldWin: hipLaunchKernelGGL(pixfield_win_batch_kernel, g, dim3(kPfThreads), 0, s, (const PixParams*)params, work); break;
     default: return launchSynthBatch(pass, params, work, n, s);

@@ -34,7 +34,8 @@ hipError_t launchFmtBatch(int pass, const void* params, const uint32_t* work, in
 hipError_t launchDebugEFmt(const SynthParams& p, uint32_t* dst, int eStride, hipStream_t s);
 hipError_t launchSeedSingle(int pass, const void* params, int gx, int gy, hipStream_t s);
 hipError_t launchSeedBatch(int pass, const void* params, const uint32_t* work, int n, hipStream_t s);
-constexpr int kSubpixBlocks = 7;   // blocks per subpix workgroup
+constexpr int kSubpixBlocks = 7;   // blocks per subpix wavefront
+constexpr int kSubpixIntWaves = 4; // wavefronts per workgroup of the integer subpix pass (each with its own blocks)
 constexpr int kSearch2Blocks = 8;  // blocks per workgroup of the batched R <= 2 search (4 for the others)
 int synthTilesX(int W);
 int synthTilesY(int H, int rows = 32);
