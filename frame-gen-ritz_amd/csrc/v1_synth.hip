@@ -478,15 +478,33 @@ __global__ __launch_bounds__(256) void lookup_kernel(LookupParams P) {
 }
 
 // fallback.comp:9-24: more than globalFallback x W x H untrusted pixels -> the nearer real frame. Every workgroup
-// exits at once in the normal case (few of them: 128x128 pixels per workgroup).
+// exits at once in the normal case; kFbTile x kFbTile pixels a workgroup keep those workgroups few (40 a 1080p frame).
+constexpr int kFbTile = 256;
 __device__ __forceinline__ void fallbackBody(const FallbackParams& P, int gbx, int gby) {
     if (P.nextCount && gbx == 0 && gby == 0 && threadIdx.x < kSlots) P.nextCount[threadIdx.x] = 0u;
     const bool flick = (float)slotSum(P.count) > P.gfrac * (float)P.W * (float)P.H;
     if (!flick) return;
     const FrameRef& nf = P.nearC ? P.c : P.a;
-    const int bx = gbx * 128, by = gby * 128;
-    for (int i = threadIdx.x; i < 128 * 128; i += 256) {
-        const int x = bx + (i & 127), y = by + (i >> 7);
+    const int bx = gbx * kFbTile, by = gby * kFbTile;
+    if (P.bpp >= 4 && !P.outc) {   // one plane of 4- or 8-byte pixels: the tile's rows as 16-byte chunks
+        const int w = min(kFbTile, P.W - bx), h = min(kFbTile, P.H - by);
+        const int words = w * P.bpp / 4, chunks = (words + 3) >> 2;
+#pragma unroll 4
+        for (int i = threadIdx.x; i < chunks * h; i += 256) {
+            const int r = i / chunks, c = i - r * chunks;
+            const uint32_t* src = (const uint32_t*)(nf.p + (int64_t)(by + r) * nf.pitch + (int64_t)bx * P.bpp) + 4 * c;
+            uint32_t* dst = (uint32_t*)(P.out + (int64_t)(by + r) * P.opitch + (int64_t)bx * P.bpp) + 4 * c;
+            if (4 * c + 4 <= words) {
+                *(U32x4*)dst = *(const U32x4*)src;
+            } else {
+                for (int k = 0; k < words - 4 * c; k++) dst[k] = src[k];
+            }
+        }
+        return;
+    }
+#pragma unroll 4
+    for (int i = threadIdx.x; i < kFbTile * kFbTile; i += 256) {
+        const int x = bx + (i % kFbTile), y = by + (i / kFbTile);
         if (x >= P.W || y >= P.H) continue;
         const uint8_t* src = nf.p + (int64_t)y * nf.pitch + (int64_t)x * P.bpp;
         uint8_t* dst = P.out + (int64_t)y * P.opitch + (int64_t)x * P.bpp;
@@ -560,8 +578,8 @@ hipError_t launchLookup(const LookupParams& p, hipStream_t s) {
 
 int synthTilesX(int W) { return cdiv(W, TW); }
 int synthTilesY(int H, int rows) { return cdiv(H, rows); }
-int fallbackTilesX(int W) { return cdiv(W, 128); }
-int fallbackTilesY(int H) { return cdiv(H, 128); }
+int fallbackTilesX(int W) { return cdiv(W, kFbTile); }
+int fallbackTilesY(int H) { return cdiv(H, kFbTile); }
 
 hipError_t launchSynthSingle(int pass, const void* params, int gx, int gy, hipStream_t s) {
     const dim3 g((unsigned)gx, (unsigned)gy);

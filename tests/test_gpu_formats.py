@@ -172,6 +172,22 @@ def test_format_scene_cut(name, fmt, linear):
     assert same(outs[0], A, fmt) and same(outs[1], C, fmt)
 
 
+@pytest.mark.parametrize("name,fmt,linear", CASES)
+def test_format_flick(name, fmt, linear):
+    # more untrusted pixels than globalFallback allows: the fallback pass copies the nearer frame over the synth's output
+    # (4- and 8-byte pixels as 16-byte chunks of a tile's rows with a word tail at an odd width; 4:2:0 pixel by pixel)
+    W, H = (202, 118) if fmt in F.YUV else (203, 117)
+    a, c = scenes.pair_translate(14, w=W, h=H, sprite=32, sprite_pos=(W // 3, H // 3), shift=(3, -2))
+    A, C = encode(a, fmt, linear), encode(c, fmt, linear)
+    s = fgr.default_settings(globalFallback=1e-5)
+    I, outs, counts = run_fmt(A, C, fmt, (0.5, 0.75), linear, settings=s)
+    lut = lut_of(fmt, linear)
+    est = F.estimate(A, C, fmt, s=oracle_settings(s), lut=lut)
+    for t, o, cnt in zip((0.5, 0.75), outs, counts):
+        assert check_fmt_output(o, A, C, est, t, fmt, lut, F.yuv_constants(matrix_of(fmt)), cnt) == "untrusted"
+        assert same(o, A if t <= 0.5 else C, fmt)
+
+
 def test_p010_full_range_bt709_and_rgb10_bgra():
     a, c = scenes.pair_translate(14, w=256, h=144, sprite=32, sprite_pos=(90, 40), shift=(4, 2))
     A, C = F.encode_p010(a), F.encode_p010(c)

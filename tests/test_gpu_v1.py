@@ -122,8 +122,10 @@ def test_scene_cut_gives_near_frame():
     fi.close()
 
 
-def test_flick_gives_near_frame():
-    a, c = scenes.pair_translate(4, w=320, h=200, sprite=48, sprite_pos=(100, 60))
+@pytest.mark.parametrize("size", [(320, 200), (203, 117)])
+def test_flick_gives_near_frame(size):
+    w, h = size
+    a, c = scenes.pair_translate(4, w=w, h=h, sprite=48, sprite_pos=(w // 3, h // 3))
     s = fgr.default_settings(globalFallback=1e-5)
     fi, I, outs, counts = run(a, c, (0.5, 0.75), settings=s)
     est = v1.estimate(a, c, oracle_settings(s))
